@@ -1,0 +1,169 @@
+/*
+ * libtsm: tall-skinny C = A^T B for AMD Instinct MI355X (gfx950).
+ *
+ * Plain C ABI: device pointers, sizes, status codes and an opaque stream
+ * handle (a hipStream_t; NULL = the default stream).  No C++ or torch types
+ * cross this boundary and no call throws.
+ *
+ * Shapes follow the reference (tsmttsm.hh:5-10, sycl_codegen_new.py:56-67):
+ *   A is n x k, B is n x m, both row-major with leading dimensions
+ *   lda >= k and ldb >= m (elements); C is k x m row-major, ldc >= m.
+ *   Reference names: K = n, M = k, N = m.  1 <= k, m <= 64, 0 <= n < 2^62.
+ *
+ * Each entry point below names the reference interface it replaces.
+ */
+#ifndef TSM_TSM_H
+#define TSM_TSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The library builds with hidden visibility; everything declared here is exported. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
+#define TSM_VERSION_MAJOR 0
+#define TSM_VERSION_MINOR 1
+#define TSM_MAX_WIDTH 64
+
+typedef void* tsm_stream_t; /* hipStream_t */
+
+typedef enum {
+  TSM_F64 = 0,  /* IEEE binary64 */
+  TSM_F32 = 1,  /* IEEE binary32 */
+  TSM_BF16 = 2  /* bfloat16 (upper half of a binary32) */
+} tsm_dtype;
+
+typedef enum {
+  TSM_SUCCESS = 0,
+  TSM_ERR_INVALID_ARGUMENT = 1, /* null pointer, n < 0, k or m outside [1,64], ld too small */
+  TSM_ERR_UNSUPPORTED = 2,      /* dtype combination or kernel request not provided */
+  TSM_ERR_MISALIGNED = 3,       /* pointer not aligned to its element size */
+  TSM_ERR_WORKSPACE = 4,        /* caller workspace smaller than tsm_atb_workspace_size() */
+  TSM_ERR_HIP = 5,              /* a HIP runtime call failed: see tsm_last_hip_error() */
+  TSM_ERR_NO_KERNEL = 6         /* no generated kernel matches the requested config */
+} tsm_status;
+
+/* Distributions of tsm_fill_random (the synthetic inputs):
+ *   TSM_DIST_UNIFORM: (w >> 11) * 2^-52 - 1, uniform on [-1, 1)
+ *   TSM_DIST_POW2:    sign(bit 63) * 2^e, e = ((w >> 11) mod 41) - 20
+ * where w = splitmix64_finalizer(seed + stream*0xd1b54a32d192ed03 + (idx+1)*0x9e3779b97f4a7c15)
+ * and idx = row*cols + col.  FP32 and BF16 fills round the FP64 value once,
+ * to nearest even. */
+typedef enum { TSM_DIST_UNIFORM = 0, TSM_DIST_POW2 = 1 } tsm_dist;
+
+/* Explicit kernel choice (generator vocabulary).  Zero fields mean "tuned
+ * default".  Replaces the compile-time constants baked into the generated
+ * SYCL kernels: M, N, TM, TN, stride, local_size (sycl_codegen_new.py:156-160,
+ * 265-267) and the max_tile_size / global_size template arguments of
+ * tsmttsm4/6/9 (tsmttsm.hh:125, 230, 368). */
+typedef struct {
+  int tm, tn;          /* register tile: C entries owned per thread = tm x tn */
+  int rows;            /* rows per LDS tile (the n-tile streamed per stage) */
+  int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth) */
+  int wgs_per_cu;      /* persistent workgroups per compute unit */
+  int64_t grid;        /* total workgroups; overrides wgs_per_cu when > 0 (the "stride" role) */
+} tsm_kernel_config;
+
+/* ---- the hot path ------------------------------------------------------
+ * C = A^T B (accumulate == 0) or C += A^T B (accumulate != 0).
+ * Replaces: the generated tsmttsm_{M}_{N}_{TM}_{TN}[_lf|_sg|_sg_lf]
+ * (sycl_codegen_new.py:158-159, 263-264) and tsmttsm, tsmttsm2..9
+ * (tsmttsm.hh:15-586), whose contract is C += A^T B with the caller zeroing C
+ * (main.cc:67-69, 94, 119).  Unlike the reference, the result does not
+ * depend on atomic ordering: the same inputs and config give the same bits.
+ *
+ *   in_type:  TSM_F64, TSM_F32 or TSM_BF16 (A and B share it)
+ *   out_type: TSM_F64 or TSM_F32 (C).  The cross-workgroup reduction is FP64
+ *             for every input type; FP32/BF16 inputs accumulate in FP32 in
+ *             short row blocks folded into compensated (hi, lo) FP32 pairs.
+ *   workspace: NULL to use a per-(device, stream) buffer owned by the library,
+ *             or device memory of at least tsm_atb_workspace_size() bytes whose
+ *             first 256 bytes are zero before its first use (the library
+ *             leaves them zero after each call).  Calls sharing a workspace
+ *             must be ordered on one stream.
+ * Asynchronous on `stream`; no allocation when a workspace is given. */
+tsm_status tsm_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                   const void* A, int64_t lda, const void* B, int64_t ldb,
+                   void* C, int64_t ldc, int accumulate,
+                   void* workspace, size_t workspace_bytes, tsm_stream_t stream);
+
+/* As tsm_atb with an explicit kernel configuration (NULL = tuned default).
+ * Used by the autotuner and the name-compatible C++ layer (include/tsm/tsmttsm.hh). */
+tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                      const void* A, int64_t lda, const void* B, int64_t ldb,
+                      void* C, int64_t ldc, int accumulate,
+                      void* workspace, size_t workspace_bytes, tsm_stream_t stream,
+                      const tsm_kernel_config* config);
+
+/* Workspace bytes tsm_atb / tsm_atb_ex need for this shape (config may be NULL). */
+size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
+                              const tsm_kernel_config* config);
+
+/* ---- differentiation (no reference counterpart) --------------------------
+ * Backward of C = A^T B for an upstream gradient dC (k x m, row-major, lddc):
+ *   dA = B dC^T (n x k), dB = A dC (n x m).
+ * dA or dB may be NULL to skip it; both use one streaming pass over the rows.
+ * dC is in out_type (TSM_F64 or TSM_F32); A, B, dA, dB are in in_type. */
+tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, int k, int m,
+                            const void* A, int64_t lda, const void* B, int64_t ldb,
+                            const void* dC, int64_t lddc,
+                            void* dA, int64_t ldda, void* dB, int64_t lddb,
+                            tsm_stream_t stream);
+
+/* ---- batching (no reference counterpart) ----------------------------------
+ * `count` independent problems C_i (+)= A_i^T B_i in ONE persistent launch.
+ * `problems` is a HOST array; the library copies it into its workspace.
+ * All problems share in_type/out_type; shapes may differ. */
+typedef struct {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t n, lda, ldb, ldc;
+  int k, m;
+} tsm_problem;
+
+size_t tsm_atb_batched_workspace_size(tsm_dtype in_type, int count, const tsm_problem* problems);
+
+tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
+                           const tsm_problem* problems, int accumulate,
+                           void* workspace, size_t workspace_bytes, tsm_stream_t stream);
+
+/* ---- synthetic inputs (replaces main.cc:72-86, sycl_codegen_new.py:436-439) -- */
+tsm_status tsm_fill_random(tsm_dtype type, tsm_dist dist, uint64_t seed, uint64_t stream_id,
+                           void* X, int64_t rows, int64_t cols, int64_t ld,
+                           tsm_stream_t stream);
+
+/* ---- introspection ---------------------------------------------------------- */
+const char* tsm_strerror(tsm_status status);
+int tsm_last_hip_error(void);       /* hipError_t of the last TSM_ERR_HIP on this thread */
+int tsm_version(void);              /* major * 10000 + minor * 100 */
+
+typedef struct {
+  const char* name;                 /* e.g. "tsm_atb_f64_8x8_t8x8_r256_p2" */
+  int in_type;                      /* tsm_dtype */
+  int kc, mc;                       /* width class: k <= kc, m <= mc */
+  int tm, tn, rows, pf;
+  int lds_bytes;
+} tsm_kernel_desc;
+
+int tsm_num_kernels(void);
+tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out);
+/* Kernel tsm_atb would use for this shape (NULL config = tuned default);
+ * writes its index in the tsm_kernel_info table and the launch grid. */
+tsm_status tsm_atb_plan(tsm_dtype in_type, int64_t n, int k, int m,
+                        const tsm_kernel_config* config, int* kernel_index, int64_t* grid);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSM_TSM_H */

@@ -1,0 +1,439 @@
+#!/usr/bin/env python3
+"""Generator of the gfx950 HIP kernels for C = A^T B.
+
+This is the counterpart of the reference's ``sycl_codegen_new.py``: it
+enumerates kernel variants and emits concrete kernels for them.  Where the
+reference prints SYCL kernels (one per M, N, TM, TN and variant,
+``sycl_codegen_new.py:142-363``) this prints HIP translation units:
+
+* ``csrc/generated/tsm_kernels_<dtype>_k<KC>.hip`` - one ``__global__`` kernel per
+  configuration (named ``tsm_atb_<dtype>_<KC>x<MC>_t<TM>x<TN>_r<ROWS>_p<PF>``,
+  the name rocprof shows) plus its host launcher;
+* ``csrc/generated/tsm_table.cpp`` - the kernel table the C ABI dispatches
+  through and the tuned default per (dtype, width class), baked in from the
+  autotuner's JSON tile table (``tuning/gfx950.json``);
+* ``include/tsm/tsmttsm_kernels.hh`` (``--name-layer``) - the reference's
+  kernel names ``tsmttsm_{M}_{N}_{TM}_{TN}[_lf|_sg|_sg_lf]`` as thin inline
+  wrappers over the C ABI, for the M/TM set given on the command line.
+
+Configuration axes (all compile-time in the emitted kernels):
+  dtype    f64 | f32 | bf16      storage type of A and B
+  KC, MC   width class           k <= KC, m <= MC, both powers of two <= 64
+  TM, TN   register tile         C entries owned by one thread (``TM``/``TN``)
+  ROWS     rows per LDS tile     the n-tile streamed per pipeline stage
+  PF       tiles in flight       1 or 2 (the reference's "leap-frog" depth)
+
+CLI flags keep the reference's vocabulary where it maps
+(``sycl_codegen_new.py:588-604``): ``--M``, ``--tile-sizes``, ``--stride``,
+``--local-size``, ``--sg-size``, ``--unroll``.  On gfx950 the work-group size
+is fixed at 256 and the sub-group (wavefront) size at 64; other values are
+rejected, as the reference rejects inconsistent flags (``:606-620``).
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import math
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+GEN_DIR = os.path.join(HERE, "csrc", "generated")
+TUNING_JSON = os.path.join(HERE, "tuning", "gfx950.json")
+
+DTYPES = ("f64", "f32", "bf16")
+DTYPE_ID = {"f64": 0, "f32": 1, "bf16": 2}
+ELEM_BYTES = {"f64": 8, "f32": 4, "bf16": 2}
+CXX_TYPE = {"f64": "double", "f32": "float", "bf16": "tsm::bf16"}
+CLASSES = (1, 2, 4, 8, 16, 32, 64)
+THREADS = 256          # work-group size (4 wave64)
+WAVE = 64              # gfx950 wavefront size
+LDS_BUDGET = 65536     # bytes of LDS per work-group (two fit in 160 KiB)
+
+
+def lcm(a: int, b: int) -> int:
+    return a * b // math.gcd(a, b)
+
+
+def width_class(x: int) -> int:
+    """Smallest class (power of two) >= x, for 1 <= x <= 64."""
+    if not 1 <= x <= 64:
+        raise ValueError(f"width {x} outside [1, 64]")
+    c = 1
+    while c < x:
+        c *= 2
+    return c
+
+
+@dataclasses.dataclass(frozen=True, order=True)
+class KernelConfig:
+    dtype: str
+    kc: int
+    mc: int
+    tm: int
+    tn: int
+    rows: int
+    pf: int
+
+    @property
+    def name(self) -> str:
+        return (f"tsm_atb_{self.dtype}_{self.kc}x{self.mc}_t{self.tm}x{self.tn}"
+                f"_r{self.rows}_p{self.pf}")
+
+    @property
+    def ntiles(self) -> int:
+        return (self.kc // self.tm) * (self.mc // self.tn)
+
+    @property
+    def row_lanes(self) -> int:
+        return THREADS // self.ntiles
+
+    @property
+    def stage_bytes(self) -> int:
+        return self.rows * (self.kc + self.mc) * ELEM_BYTES[self.dtype]
+
+    @property
+    def lds_bytes(self) -> int:
+        return max(2 * self.stage_bytes, self.kc * self.mc * 8, THREADS * 8)
+
+    def problems(self) -> list[str]:
+        """Reasons this configuration cannot be emitted (empty when valid)."""
+        p = []
+        es = ELEM_BYTES[self.dtype]
+        if self.dtype not in DTYPES:
+            p.append("dtype")
+        if self.kc not in CLASSES or self.mc not in CLASSES:
+            p.append("width class must be a power of two <= 64")
+            return p
+        if self.tm < 1 or self.tn < 1 or self.kc % self.tm or self.mc % self.tn:
+            p.append("tile must divide the width class")
+            return p
+        if self.ntiles > WAVE:
+            p.append("more than 64 register tiles")
+        if self.pf not in (1, 2):
+            p.append("pf must be 1 or 2")
+        if self.rows < 1 or self.rows % self.row_lanes:
+            p.append(f"rows must be a multiple of {self.row_lanes} row lanes")
+        if (self.rows * self.kc * es) % 16 or (self.rows * self.mc * es) % 16:
+            p.append("tiles must be whole 16-byte chunks")
+        if self.lds_bytes > LDS_BUDGET:
+            p.append("LDS budget")
+        if self.rows // max(self.row_lanes, 1) > 64:
+            p.append("more than 64 rows per thread per tile")
+        return p
+
+
+def _pow2_floor(x: int) -> int:
+    return 1 << (max(1, x).bit_length() - 1)
+
+
+def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig:
+    """Heuristic configuration used when the tile table has no entry.
+
+    FP64 keeps up to an 8 x 8 register tile (64 FP64 accumulators); FP32/BF16
+    keep up to 8 x 4 because each entry also carries a (hi, lo) pair.  The
+    row tile aims at 32 KiB per pipeline stage."""
+    es = ELEM_BYTES[dtype]
+    tm = min(kc, 8)
+    tn = min(mc, 8 if dtype == "f64" else 4)
+    if (kc // tm) * (mc // tn) > WAVE:
+        tn = min(mc, 8)
+    ntiles = (kc // tm) * (mc // tn)
+    rg = THREADS // ntiles
+    rows = _pow2_floor(32768 // ((kc + mc) * es))
+    rows = max(rows, rg)
+    while (rows * kc * es) % 16 or (rows * mc * es) % 16:
+        rows *= 2
+    while rows // rg > 32 and rows > rg:
+        rows //= 2
+    cfg = KernelConfig(dtype, kc, mc, tm, tn, rows, pf)
+    while cfg.lds_bytes > LDS_BUDGET and cfg.rows > rg:
+        cfg = dataclasses.replace(cfg, rows=cfg.rows // 2)
+    bad = cfg.problems()
+    if bad:
+        raise ValueError(f"no default for {dtype} {kc}x{mc}: {bad}")
+    return cfg
+
+
+# Shapes of the sweep / precision / batched workloads that get extra variants
+# for the autotuner to choose from.
+TUNE_SHAPES = {
+    "f64": [(1, 1), (2, 2), (4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (4, 32), (32, 4)],
+    "f32": [(8, 8), (16, 16), (32, 32)],
+    "bf16": [(8, 8), (16, 16), (32, 32)],
+}
+
+
+def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
+    base = default_config(dtype, kc, mc)
+    out = set()
+    tiles = {(base.tm, base.tn), (min(kc, 4), min(mc, 4)), (min(kc, 8), min(mc, 8))}
+    if dtype != "f64":
+        tiles.add((min(kc, 8), min(mc, 8)))
+    for tm, tn in tiles:
+        for rows in (base.rows // 2, base.rows, base.rows * 2, base.rows * 4):
+            for pf in (1, 2):
+                c = KernelConfig(dtype, kc, mc, tm, tn, rows, pf)
+                if not c.problems():
+                    out.add(c)
+    return sorted(out)
+
+
+def reference_variants(Ms, tiles, *, dtype: str = "f64") -> list[KernelConfig]:
+    """Kernels behind the reference names tsmttsm_{M}_{M}_{TM}_{TM}[...]: basic and
+    _sg map to pf=1, _lf and _sg_lf to pf=2 (sycl_codegen_new.py:640-661)."""
+    out = []
+    for M in Ms:
+        for TM in tiles:
+            if M % TM:
+                continue  # the reference skips TM that does not divide M (:645-647)
+            kc = width_class(M)
+            base = default_config(dtype, kc, kc)
+            for pf in (1, 2):
+                c = KernelConfig(dtype, kc, kc, TM, TM, base.rows, pf)
+                while c.problems() and c.rows < 4096:
+                    c = dataclasses.replace(c, rows=c.rows * 2)
+                if not c.problems():
+                    out.append(c)
+    return out
+
+
+def load_tuning(path: str = TUNING_JSON) -> dict:
+    if not os.path.exists(path):
+        return {"device": "gfx950", "entries": []}
+    with open(path) as f:
+        return json.load(f)
+
+
+def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
+    """The full kernel set of the library and the tuned default per class."""
+    cfgs: set[KernelConfig] = set()
+    defaults: dict[tuple[str, int, int], tuple[KernelConfig, int]] = {}
+    for dt in DTYPES:
+        for kc in CLASSES:
+            for mc in CLASSES:
+                c = default_config(dt, kc, mc)
+                cfgs.add(c)
+                defaults[(dt, kc, mc)] = (c, 2)
+        for kc, mc in TUNE_SHAPES[dt]:
+            cfgs.update(tuning_variants(dt, kc, mc))
+    cfgs.update(reference_variants([4, 8, 16], [2, 4]))
+    for e in tuning.get("entries", []):
+        c = KernelConfig(e["dtype"], e["kc"], e["mc"], e["tm"], e["tn"], e["rows"], e["pf"])
+        if c.problems():
+            raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
+        cfgs.add(c)
+        defaults[(c.dtype, c.kc, c.mc)] = (c, int(e.get("wgs_per_cu", 2)))
+    return sorted(cfgs), defaults
+
+
+# ---------------------------------------------------------------------------
+# Emitters
+# ---------------------------------------------------------------------------
+BANNER = ("// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.\n"
+          "// Regenerate with: python -m tsm_benchmark_amd.codegen --library\n")
+
+
+def emit_kernel(c: KernelConfig) -> str:
+    t = CXX_TYPE[c.dtype]
+    targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}"
+    return (
+        f"// {c.name}: KC={c.kc} MC={c.mc} TM={c.tm} TN={c.tn} rows={c.rows} pf={c.pf}\n"
+        f"//   {c.ntiles} register tiles x {c.row_lanes} row lanes, "
+        f"{c.rows // c.row_lanes} rows/thread/tile, LDS {c.lds_bytes} B\n"
+        f"extern \"C\" __global__ void __launch_bounds__(256) {c.name}(tsm::AtbArgs a) {{\n"
+        f"  tsm::atb_body<{targs}>(a);\n"
+        f"}}\n"
+        f"hipError_t launch_{c.name}(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {{\n"
+        f"  hipLaunchKernelGGL({c.name}, dim3(grid), dim3(256), 0, s, a);\n"
+        f"  return hipGetLastError();\n"
+        f"}}\n")
+
+
+def emit_kernel_file(dtype: str, kc: int, cfgs: list[KernelConfig]) -> str:
+    body = [BANNER, '#include "../tsm_internal.hpp"\n']
+    for c in cfgs:
+        if c.dtype == dtype and c.kc == kc:
+            body.append(emit_kernel(c))
+    return "\n".join(body)
+
+
+def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
+    idx = {c: i for i, c in enumerate(cfgs)}
+    out = [BANNER, '#include "../tsm_internal.hpp"\n']
+    for c in cfgs:
+        out.append(f"hipError_t launch_{c.name}(const tsm::AtbArgs&, unsigned, hipStream_t);")
+    out.append("\nstatic const tsm::KernelEntry kTable[] = {")
+    for c in cfgs:
+        out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
+                   f"{c.rows}, {c.pf}, {c.lds_bytes}, launch_{c.name}}},")
+    out.append("};\n")
+    out.append("static const tsm::TunedEntry kTuned[] = {")
+    for (dt, kc, mc), (c, wpc) in sorted(defaults.items()):
+        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}}},  // {c.name}")
+    out.append("};\n")
+    out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {\n"
+               "  *count = (int)(sizeof(kTable) / sizeof(kTable[0]));\n  return kTable;\n}")
+    out.append("extern \"C\" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {\n"
+               "  *count = (int)(sizeof(kTuned) / sizeof(kTuned[0]));\n  return kTuned;\n}")
+    return "\n".join(out) + "\n"
+
+
+def _write_if_changed(path: str, text: str) -> bool:
+    if os.path.exists(path):
+        with open(path) as f:
+            if f.read() == text:
+                return False
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(text)
+    return True
+
+
+def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> list[str]:
+    """Write the library's generated sources; returns the .hip/.cpp paths."""
+    cfgs, defaults = library_kernels(load_tuning(tuning_path))
+    paths = []
+    for dt in DTYPES:
+        for kc in CLASSES:
+            p = os.path.join(out_dir, f"tsm_kernels_{dt}_k{kc}.hip")
+            _write_if_changed(p, emit_kernel_file(dt, kc, cfgs))
+            paths.append(p)
+    p = os.path.join(out_dir, "tsm_table.hip")
+    _write_if_changed(p, emit_table(cfgs, defaults))
+    paths.append(p)
+    return paths
+
+
+# ---- name-compatible layer (the reference's generated header role) ----------
+def emit_name_layer(Ms, tiles, stride, unroll, with_benchmark: bool) -> str:
+    """The reference's kernel names as inline wrappers over tsm_atb_ex.
+
+    Each wrapper keeps the reference's signature shape ``(queue, K, A, B, C)``
+    and contract ``C += A^T B`` (sycl_codegen_new.py:158-159, 218-225); the queue
+    becomes a stream and the returned value a tsm_status."""
+    lines = ["#pragma once", "// GENERATED by tsm_benchmark_amd/codegen.py --name-layer",
+             f"// M={list(Ms)} tile-sizes={list(tiles)} stride={stride} unroll={unroll}",
+             "#include \"tsm/tsmttsm.hh\"", "", "namespace tsm_compat {", ""]
+    names = []
+    for M in Ms:
+        for TM in tiles:
+            if M % TM:
+                continue
+            for suffix, pf in (("", 1), ("_lf", 2), ("_sg", 1), ("_sg_lf", 2)):
+                name = f"tsmttsm_{M}_{M}_{TM}_{TM}{suffix}"
+                names.append(name)
+                lines += [
+                    "template <typename T>",
+                    f"inline tsm_status {name}(tsm_stream_t q, int64_t K, const T* A, const T* B, T* C) {{",
+                    f"  return tsm_compat::generated_call<T>(q, K, {M}, {M}, A, B, C, {TM}, {TM}, {pf}, "
+                    f"{stride}, {unroll});",
+                    "}", ""]
+    if with_benchmark:
+        lines += ["// Every generated name, in generation order (the generated benchmark's list,",
+                  "// sycl_codegen_new.py:478-487).",
+                  "inline const char* const kGeneratedNames[] = {"]
+        lines += [f"  \"{n}\"," for n in names]
+        lines += ["};", ""]
+    lines += ["}  // namespace tsm_compat", ""]
+    return "\n".join(lines)
+
+
+def suggest_stride(Ms, tiles, local_size=THREADS, sg_size=WAVE, target_wgs=2048) -> int:
+    """The reference's stride rule (sycl_codegen_new.py:563-580): the smallest
+    multiple of lcm(local, num_tiles, sg*num_tiles) near local*target_wgs.  On
+    gfx950 the stride is the persistent grid in work-items and only sets
+    ``tsm_kernel_config.grid``."""
+    req = local_size
+    for M in Ms:
+        for TM in tiles:
+            if M % TM:
+                continue
+            nt = (M // TM) ** 2
+            req = lcm(req, nt)
+            req = lcm(req, sg_size * nt)
+    full = lcm(req, local_size)
+    n = max(1, (local_size * target_wgs + full - 1) // full)
+    return n * full
+
+
+def validate_stride(Ms, tiles, stride, local_size=THREADS, sg_size=WAVE) -> list[str]:
+    """validate_stride (sycl_codegen_new.py:530-560); returns the error lines."""
+    errs = []
+    if stride % local_size:
+        return [f"Error: --stride {stride} must be divisible by --local-size {local_size}"]
+    for M in Ms:
+        for TM in tiles:
+            if M % TM:
+                continue
+            nt = (M // TM) ** 2
+            if stride % nt:
+                errs.append(f"Warning: stride ({stride}) not divisible by num_tiles ({nt}) "
+                            f"for M={M}, TM={TM}")
+            if (stride // sg_size) % nt:
+                errs.append(f"Warning: stride/sg_size ({stride // sg_size}) not divisible by "
+                            f"num_tiles ({nt}) for M={M}, TM={TM}")
+    return errs
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="Generate gfx950 HIP kernels for C = A^T B")
+    ap.add_argument("--library", action="store_true",
+                    help="write the library kernel set into csrc/generated/")
+    ap.add_argument("--tuning", default=TUNING_JSON, help="tile-table JSON (autotuner output)")
+    ap.add_argument("--out-dir", default=GEN_DIR)
+    ap.add_argument("--name-layer", metavar="FILE",
+                    help="write the reference-name wrapper header to FILE ('-' = stdout)")
+    ap.add_argument("--M", type=int, nargs="+", default=[4, 8, 16])
+    ap.add_argument("--tile-sizes", type=int, nargs="+", default=[2, 4])
+    ap.add_argument("--stride", type=int, default=None)
+    ap.add_argument("--local-size", type=int, default=THREADS)
+    ap.add_argument("--sg-size", type=int, default=WAVE)
+    ap.add_argument("--unroll", type=int, default=1)
+    ap.add_argument("--with-benchmark", action="store_true")
+    ap.add_argument("--list", action="store_true", help="print the library kernel names")
+    args = ap.parse_args(argv)
+
+    if args.local_size != THREADS:
+        print(f"Error: --local-size must be {THREADS} on gfx950", file=sys.stderr)
+        return 1
+    if args.sg_size != WAVE:
+        print(f"Error: --sg-size must be {WAVE} (gfx950 wavefront)", file=sys.stderr)
+        return 1
+    if args.local_size % args.sg_size:
+        print("Error: --local-size must be a multiple of --sg-size", file=sys.stderr)
+        return 1
+    if args.unroll < 1:
+        print("Error: --unroll must be >= 1", file=sys.stderr)
+        return 1
+    for M in args.M:
+        width_class(M)
+    if args.stride is None:
+        args.stride = suggest_stride(args.M, args.tile_sizes, args.local_size, args.sg_size)
+    errs = validate_stride(args.M, args.tile_sizes, args.stride, args.local_size, args.sg_size)
+    if errs:
+        for e in errs:
+            print(e, file=sys.stderr)
+        return 1
+
+    if args.library:
+        for p in generate_library(args.out_dir, args.tuning):
+            print(p)
+    if args.name_layer:
+        text = emit_name_layer(args.M, args.tile_sizes, args.stride, args.unroll,
+                               args.with_benchmark)
+        if args.name_layer == "-":
+            sys.stdout.write(text)
+        else:
+            _write_if_changed(args.name_layer, text)
+    if args.list:
+        cfgs, _ = library_kernels(load_tuning(args.tuning))
+        for c in cfgs:
+            print(c.name)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,401 @@
+// C ABI of libtsm (include/tsm/tsm.h): argument checks, kernel choice,
+// workspace handling and launches.  Every entry point validates its
+// arguments before touching the HIP runtime, so argument errors are reported
+// the same way with or without a GPU, and no entry point throws.
+#include "tsm/tsm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "tsm_internal.hpp"
+
+namespace {
+
+thread_local int g_last_hip_error = 0;
+
+constexpr size_t kCounterBytes = 256;  // arrival counters at the head of a workspace
+
+tsm_status hip_fail(hipError_t e) {
+  g_last_hip_error = (int)e;
+  return TSM_ERR_HIP;
+}
+
+int elem_bytes(tsm_dtype t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
+
+bool valid_in_type(int t) { return t == TSM_F64 || t == TSM_F32 || t == TSM_BF16; }
+bool valid_out_type(int t) { return t == TSM_F64 || t == TSM_F32; }
+
+int width_class(int x) {
+  int c = 1;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+// Compute units of the current device (cached per device).
+tsm_status device_cus(int* cus) {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) {
+    *cus = it->second;
+    return TSM_SUCCESS;
+  }
+  int n = 0;
+  e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return hip_fail(e);
+  cache[dev] = n;
+  *cus = n;
+  return TSM_SUCCESS;
+}
+
+const tsm::KernelEntry* table(int* count) { return tsm_gen_kernel_table(count); }
+
+// ---- kernel choice --------------------------------------------------------------
+struct Plan {
+  int kernel = -1;
+  const tsm::KernelEntry* entry = nullptr;
+  int64_t grid = 1;
+  int64_t rows_per_wg = 0;
+  int kcmc = 0;
+  bool finish_in_kernel = true;
+  size_t workspace = 0;
+};
+
+tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_config* cfg,
+                       int* index, int* wgs_per_cu) {
+  int nt = 0, nk = 0;
+  const tsm::TunedEntry* tuned = tsm_gen_tuned_table(&nt);
+  const tsm::KernelEntry* ks = table(&nk);
+  int def = -1, def_wpc = 2;
+  for (int i = 0; i < nt; ++i)
+    if (tuned[i].dtype == (int)in_type && tuned[i].kc == kc && tuned[i].mc == mc) {
+      def = tuned[i].kernel;
+      def_wpc = tuned[i].wgs_per_cu;
+    }
+  if (def < 0) return TSM_ERR_NO_KERNEL;
+  *wgs_per_cu = (cfg && cfg->wgs_per_cu > 0) ? cfg->wgs_per_cu : def_wpc;
+  if (!cfg || (cfg->tm == 0 && cfg->tn == 0 && cfg->rows == 0 && cfg->pf == 0)) {
+    *index = def;
+    return TSM_SUCCESS;
+  }
+  // Explicit request: every non-zero field must match; zero fields take the
+  // tuned default's value.
+  const tsm::KernelEntry& d = ks[def];
+  const int tm = cfg->tm ? cfg->tm : d.tm, tn = cfg->tn ? cfg->tn : d.tn;
+  const int pf = cfg->pf ? cfg->pf : d.pf;
+  int best = -1;
+  for (int i = 0; i < nk; ++i) {
+    const tsm::KernelEntry& e = ks[i];
+    if (e.dtype != (int)in_type || e.kc != kc || e.mc != mc || e.tm != tm || e.tn != tn ||
+        e.pf != pf)
+      continue;
+    if (cfg->rows ? e.rows == cfg->rows : (best < 0 || e.rows == d.rows)) best = i;
+  }
+  if (best < 0) return TSM_ERR_NO_KERNEL;
+  *index = best;
+  return TSM_SUCCESS;
+}
+
+tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kernel_config* cfg,
+                     Plan* p) {
+  const int kc = width_class(k), mc = width_class(m);
+  int wpc = 2;
+  tsm_status st = find_kernel(in_type, kc, mc, cfg, &p->kernel, &wpc);
+  if (st != TSM_SUCCESS) return st;
+  int nk = 0;
+  p->entry = &table(&nk)[p->kernel];
+  const int64_t T = p->entry->rows;
+  int64_t want;
+  if (cfg && cfg->grid > 0) {
+    want = cfg->grid;
+  } else {
+    int cus = 0;
+    st = device_cus(&cus);
+    if (st != TSM_SUCCESS) return st;
+    want = (int64_t)wpc * cus;
+  }
+  const int64_t tiles = n > 0 ? (n + T - 1) / T : 1;
+  if (want > tiles) want = tiles;
+  if (want < 1) want = 1;
+  const int64_t tiles_per_wg = (tiles + want - 1) / want;
+  p->rows_per_wg = tiles_per_wg * T;
+  p->grid = n > 0 ? (n + p->rows_per_wg - 1) / p->rows_per_wg : 1;
+  p->kcmc = kc * mc;
+  p->finish_in_kernel = p->kcmc <= 1024;
+  p->workspace = kCounterBytes + (size_t)p->grid * p->kcmc * sizeof(double);
+  return TSM_SUCCESS;
+}
+
+tsm_status check_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                     const void* A, int64_t lda, const void* B, int64_t ldb, const void* C,
+                     int64_t ldc) {
+  if (!valid_in_type(in_type) || !valid_out_type(out_type)) return TSM_ERR_UNSUPPORTED;
+  if (n < 0 || k < 1 || m < 1 || k > TSM_MAX_WIDTH || m > TSM_MAX_WIDTH)
+    return TSM_ERR_INVALID_ARGUMENT;
+  if (lda < k || ldb < m || ldc < m) return TSM_ERR_INVALID_ARGUMENT;
+  if (!C || (n > 0 && (!A || !B))) return TSM_ERR_INVALID_ARGUMENT;
+  if (n > ((int64_t)1 << 62) / (lda > ldb ? lda : ldb)) return TSM_ERR_INVALID_ARGUMENT;
+  const uintptr_t es = (uintptr_t)elem_bytes(in_type);
+  if (((uintptr_t)A % es) || ((uintptr_t)B % es)) return TSM_ERR_MISALIGNED;
+  if ((uintptr_t)C % (uintptr_t)elem_bytes(out_type)) return TSM_ERR_MISALIGNED;
+  return TSM_SUCCESS;
+}
+
+// ---- library-owned workspaces: one per (device, stream) --------------------------------
+struct Pool {
+  std::mutex mu;
+  std::map<std::pair<int, hipStream_t>, std::pair<void*, size_t>> bufs;
+};
+Pool& pool() {
+  static Pool* p = new Pool();  // never destroyed: the HIP runtime may be gone at exit
+  return *p;
+}
+
+tsm_status pooled_workspace(size_t bytes, hipStream_t stream, void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e);
+  Pool& P = pool();
+  std::lock_guard<std::mutex> lock(P.mu);
+  auto& slot = P.bufs[{dev, stream}];
+  if (slot.second < bytes) {
+    if (slot.first) {
+      // Earlier work on this stream may still use the old buffer.
+      e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) return hip_fail(e);
+      e = hipFree(slot.first);
+      if (e != hipSuccess) return hip_fail(e);
+      slot = {nullptr, 0};
+    }
+    size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+    void* p = nullptr;
+    e = hipMalloc(&p, want);
+    if (e != hipSuccess) return hip_fail(e);
+    e = hipMemsetAsync(p, 0, kCounterBytes > want ? want : kCounterBytes, stream);
+    if (e != hipSuccess) {
+      hipFree(p);
+      return hip_fail(e);
+    }
+    slot = {p, want};
+  }
+  *out = slot.first;
+  return TSM_SUCCESS;
+}
+
+}  // namespace
+
+// ---- finisher kernel for wide classes (KC*MC > 1024) -----------------------------------
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_finish(tsm::AtbArgs a, int G, int KC, int MC) {
+  tsm::finish_body(a, G, KC, MC);
+}
+
+// ---- synthetic inputs -------------------------------------------------------------------
+namespace {
+__device__ __forceinline__ uint64_t rng_word(uint64_t seed, uint64_t stream, uint64_t idx) {
+  uint64_t z = seed + stream * 0xd1b54a32d192ed03ULL + (idx + 1) * 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double rng_value(int dist, uint64_t w) {
+  if (dist == TSM_DIST_UNIFORM) return (double)(w >> 11) * 0x1p-52 - 1.0;
+  const int e = (int)((w >> 11) % 41u) - 20;
+  const uint64_t bits = ((uint64_t)(1023 + e) << 52) | (w & 0x8000000000000000ULL);
+  return __longlong_as_double((long long)bits);
+}
+// Single round-to-nearest-even from binary64 to bfloat16 (8-bit significand).
+__device__ __forceinline__ uint16_t round_bf16(double x) {
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  if ((b & 0x7fffffffffffffffULL) == 0) return (uint16_t)((b >> 48) & 0x8000);
+  const uint64_t lsb = (b >> 45) & 1u;
+  b += 0x00000fffffffffffULL + lsb;
+  b &= ~0x00001fffffffffffULL;
+  const float f = (float)__longlong_as_double((long long)b);
+  return (uint16_t)(__float_as_uint(f) >> 16);
+}
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(256)
+    tsm_fill_kernel(int type, int dist, uint64_t seed, uint64_t stream_id, void* X, int64_t rows,
+                    int64_t cols, int64_t ld) {
+  const int64_t total = rows * cols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols, c = i - r * cols;
+    const double v = rng_value(dist, rng_word(seed, stream_id, (uint64_t)i));
+    const int64_t off = r * ld + c;
+    if (type == TSM_F64)
+      static_cast<double*>(X)[off] = v;
+    else if (type == TSM_F32)
+      static_cast<float*>(X)[off] = (float)v;
+    else
+      static_cast<uint16_t*>(X)[off] = round_bf16(v);
+  }
+}
+
+// ---- entry points ------------------------------------------------------------------------
+extern "C" {
+
+size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
+                              const tsm_kernel_config* config) {
+  if (!valid_in_type(in_type) || n < 0 || k < 1 || m < 1 || k > TSM_MAX_WIDTH ||
+      m > TSM_MAX_WIDTH)
+    return 0;
+  Plan p;
+  if (make_plan(in_type, n, k, m, config, &p) != TSM_SUCCESS) return 0;
+  return p.workspace;
+}
+
+tsm_status tsm_atb_plan(tsm_dtype in_type, int64_t n, int k, int m,
+                        const tsm_kernel_config* config, int* kernel_index, int64_t* grid) {
+  if (!valid_in_type(in_type)) return TSM_ERR_UNSUPPORTED;
+  if (n < 0 || k < 1 || m < 1 || k > TSM_MAX_WIDTH || m > TSM_MAX_WIDTH)
+    return TSM_ERR_INVALID_ARGUMENT;
+  Plan p;
+  tsm_status st = make_plan(in_type, n, k, m, config, &p);
+  if (st != TSM_SUCCESS) return st;
+  if (kernel_index) *kernel_index = p.kernel;
+  if (grid) *grid = p.grid;
+  return TSM_SUCCESS;
+}
+
+tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                      const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                      int64_t ldc, int accumulate, void* workspace, size_t workspace_bytes,
+                      tsm_stream_t stream, const tsm_kernel_config* config) {
+  tsm_status st = check_atb(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc);
+  if (st != TSM_SUCCESS) return st;
+  if (config && (config->pf < 0 || config->pf > 2 || config->tm < 0 || config->tn < 0 ||
+                 config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0))
+    return TSM_ERR_INVALID_ARGUMENT;
+  Plan p;
+  st = make_plan(in_type, n, k, m, config, &p);
+  if (st != TSM_SUCCESS) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (workspace) {
+    if (workspace_bytes < p.workspace) return TSM_ERR_WORKSPACE;
+  } else {
+    st = pooled_workspace(p.workspace, s, &workspace);
+    if (st != TSM_SUCCESS) return st;
+  }
+  const tsm::KernelEntry& ke = *p.entry;
+  tsm::AtbArgs a{};
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.counter = static_cast<unsigned int*>(workspace);
+  a.partials = reinterpret_cast<double*>(static_cast<char*>(workspace) + kCounterBytes);
+  a.n = n;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.ldc = ldc;
+  a.rows_per_wg = p.rows_per_wg;
+  a.k = k;
+  a.m = m;
+  a.accumulate = accumulate ? 1 : 0;
+  a.c_f32 = out_type == TSM_F32 ? 1 : 0;
+  a.dense = (k == ke.kc && m == ke.mc && lda == k && ldb == m && ((uintptr_t)A % 16) == 0 &&
+             ((uintptr_t)B % 16) == 0)
+                ? 1
+                : 0;
+  a.finish = p.finish_in_kernel ? 1 : 0;
+  hipError_t e = ke.launch(a, (unsigned)p.grid, s);
+  if (e != hipSuccess) return hip_fail(e);
+  if (!p.finish_in_kernel) {
+    const unsigned fg = (unsigned)((p.kcmc + 63) / 64);
+    hipLaunchKernelGGL(tsm_atb_finish, dim3(fg), dim3(256), 0, s, a, (int)p.grid, ke.kc, ke.mc);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e);
+  }
+  return TSM_SUCCESS;
+}
+
+tsm_status tsm_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m, const void* A,
+                   int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc, int accumulate,
+                   void* workspace, size_t workspace_bytes, tsm_stream_t stream) {
+  return tsm_atb_ex(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc, accumulate, workspace,
+                    workspace_bytes, stream, nullptr);
+}
+
+tsm_status tsm_fill_random(tsm_dtype type, tsm_dist dist, uint64_t seed, uint64_t stream_id,
+                           void* X, int64_t rows, int64_t cols, int64_t ld, tsm_stream_t stream) {
+  if (!valid_in_type(type)) return TSM_ERR_UNSUPPORTED;
+  if (dist != TSM_DIST_UNIFORM && dist != TSM_DIST_POW2) return TSM_ERR_INVALID_ARGUMENT;
+  if (rows < 0 || cols < 1 || ld < cols) return TSM_ERR_INVALID_ARGUMENT;
+  if (rows > 0 && !X) return TSM_ERR_INVALID_ARGUMENT;
+  if ((uintptr_t)X % (uintptr_t)elem_bytes(type)) return TSM_ERR_MISALIGNED;
+  if (rows == 0) return TSM_SUCCESS;
+  const int64_t total = rows * cols;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(tsm_fill_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), (int)type, (int)dist, seed, stream_id, X,
+                     rows, cols, ld);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e);
+  return TSM_SUCCESS;
+}
+
+const char* tsm_strerror(tsm_status s) {
+  switch (s) {
+    case TSM_SUCCESS: return "success";
+    case TSM_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case TSM_ERR_UNSUPPORTED: return "unsupported data type or request";
+    case TSM_ERR_MISALIGNED: return "misaligned pointer";
+    case TSM_ERR_WORKSPACE: return "workspace too small";
+    case TSM_ERR_HIP: return "HIP runtime error";
+    case TSM_ERR_NO_KERNEL: return "no generated kernel matches the configuration";
+  }
+  return "unknown status";
+}
+
+int tsm_last_hip_error(void) { return g_last_hip_error; }
+
+int tsm_version(void) { return TSM_VERSION_MAJOR * 10000 + TSM_VERSION_MINOR * 100; }
+
+int tsm_num_kernels(void) {
+  int n = 0;
+  table(&n);
+  return n;
+}
+
+tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out) {
+  int n = 0;
+  const tsm::KernelEntry* ks = table(&n);
+  if (!out || index < 0 || index >= n) return TSM_ERR_INVALID_ARGUMENT;
+  const tsm::KernelEntry& e = ks[index];
+  out->name = e.name;
+  out->in_type = e.dtype;
+  out->kc = e.kc;
+  out->mc = e.mc;
+  out->tm = e.tm;
+  out->tn = e.tn;
+  out->rows = e.rows;
+  out->pf = e.pf;
+  out->lds_bytes = e.lds_bytes;
+  return TSM_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---- placeholders until the backward and batched kernels land -------------------------
+extern "C" tsm_status tsm_atb_backward(tsm_dtype, tsm_dtype, int64_t, int, int, const void*,
+                                       int64_t, const void*, int64_t, const void*, int64_t, void*,
+                                       int64_t, void*, int64_t, tsm_stream_t) {
+  return TSM_ERR_UNSUPPORTED;
+}
+extern "C" size_t tsm_atb_batched_workspace_size(tsm_dtype, int, const tsm_problem*) { return 0; }
+extern "C" tsm_status tsm_atb_batched(tsm_dtype, tsm_dtype, int, const tsm_problem*, int, void*,
+                                      size_t, tsm_stream_t) {
+  return TSM_ERR_UNSUPPORTED;
+}

@@ -1,0 +1,511 @@
+// Device building blocks of the C = A^T B kernels for gfx950 (CDNA4, wave64).
+//
+// One workgroup (256 threads = 4 waves) streams a contiguous chunk of rows of
+// A (n x k) and B (n x m).  Rows arrive in tiles of T rows: each thread issues
+// 16-byte global loads whose addresses are consecutive across the workgroup
+// (a tile of a dense row-major matrix is one contiguous byte range), the
+// loaded registers are written to one of two LDS stages, and the tile is
+// consumed from LDS by a register-tiled outer product.  While a tile is being
+// consumed the loads of the next PF tiles are in flight.
+//
+// Thread -> work mapping inside a tile: the k x m result (padded to the width
+// class KC x MC) is cut into NTILES = (KC/TM) * (MC/TN) register tiles; thread
+// t owns tile p = t % NTILES (A columns [mb*TM, mb*TM+TM), B columns
+// [nb*TN, nb*TN+TN)) and the rows q, q + RG, q + 2RG, ... (q = t / NTILES,
+// RG = 256 / NTILES) of every tile.
+//
+// Reduction (deterministic, no floating-point atomics):
+//   1. wave64 recursive halving over the lanes that share a register tile
+//      (each level exchanges half of the remaining values with __shfl_xor);
+//   2. the four waves are added into LDS in wave order;
+//   3. each workgroup writes its KC x MC FP64 partial to the workspace;
+//   4. the last workgroup to arrive (an agent-scope arrival counter that the
+//      last workgroup resets to zero) adds the partials in workgroup order and
+//      writes C.  The summation tree depends only on the launch shape, so a
+//      given shape and config gives the same bits on every run.
+// FP32 and BF16 inputs accumulate in FP32 over short row blocks; each block
+// sum is folded into a compensated (hi, lo) FP32 pair with TwoSum; steps 1-4
+// run in FP64 for every input type.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsm {
+
+constexpr int kThreads = 256;  // 4 waves of 64
+
+// 16-byte chunk as a native vector (an SSA value, so register arrays of it
+// stay in VGPRs; HIP's uint4 class is copied through memory).
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+struct bf16 {
+  uint16_t bits;
+};
+
+// ---- storage / accumulator traits ------------------------------------------
+template <class S>
+struct Traits;
+template <>
+struct Traits<double> {
+  using Acc = double;
+  static constexpr bool compensated = false;
+};
+template <>
+struct Traits<float> {
+  using Acc = float;
+  static constexpr bool compensated = true;
+};
+template <>
+struct Traits<bf16> {
+  using Acc = float;
+  static constexpr bool compensated = true;
+};
+
+__device__ __forceinline__ double widen(double x) { return x; }
+__device__ __forceinline__ float widen(float x) { return x; }
+__device__ __forceinline__ float widen(bf16 x) { return __uint_as_float((uint32_t)x.bits << 16); }
+
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// Unpack one 16-byte LDS chunk into accumulator-typed values.
+template <class S>
+struct Unpack;
+template <>
+struct Unpack<double> {
+  static constexpr int N = 2;
+  __device__ static void run(const v4u& u, double* o) {
+    o[0] = __hiloint2double((int)u.y, (int)u.x);
+    o[1] = __hiloint2double((int)u.w, (int)u.z);
+  }
+};
+template <>
+struct Unpack<float> {
+  static constexpr int N = 4;
+  __device__ static void run(const v4u& u, float* o) {
+    o[0] = __uint_as_float(u.x);
+    o[1] = __uint_as_float(u.y);
+    o[2] = __uint_as_float(u.z);
+    o[3] = __uint_as_float(u.w);
+  }
+};
+template <>
+struct Unpack<bf16> {
+  static constexpr int N = 8;
+  __device__ static void run(const v4u& u, float* o) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      o[2 * i] = __uint_as_float(w[i] << 16);
+      o[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+    }
+  }
+};
+
+// Read CNT consecutive elements of a tile row from LDS, widened.
+template <class S, int CNT>
+__device__ __forceinline__ void lds_read(const S* p, typename Traits<S>::Acc* out) {
+  constexpr int BYTES = CNT * (int)sizeof(S);
+  if constexpr (BYTES % 16 == 0) {
+    const v4u* v = reinterpret_cast<const v4u*>(p);
+#pragma unroll
+    for (int c = 0; c < BYTES / 16; ++c) {
+      const v4u u = v[c];
+      Unpack<S>::run(u, out + c * Unpack<S>::N);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) out[c] = widen(p[c]);
+  }
+}
+
+// TwoSum (Knuth): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float z = s - a;
+  e = (a - (s - z)) + (b - z);
+}
+
+// ---- kernel arguments ---------------------------------------------------------
+struct AtbArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  double* partials;       // [gridDim.x][KC*MC] FP64 workgroup partials
+  unsigned int* counter;  // arrival counter, zero between calls
+  int64_t n, lda, ldb, ldc;
+  int64_t rows_per_wg;    // rows streamed by one workgroup (a multiple of T)
+  int k, m;
+  int accumulate;         // C += result (1) or C = result (0)
+  int c_f32;              // C is binary32 (1) or binary64 (0)
+  int dense;              // k == KC, m == MC, lda == k, ldb == m, 16-byte aligned A, B
+  int finish;             // the last workgroup writes C (1) or a finisher kernel does (0)
+};
+
+// ---- wave64 recursive-halving reduction -------------------------------------------
+// Lanes L and L ^ d (d = NTILES << l) share a register tile.  While more than
+// one value is left, each level keeps half of the values (the upper half on
+// the lane whose bit d is set) and adds the partner's copy of that half; the
+// remaining levels are a plain butterfly.
+template <int NTILES, int V>
+struct WaveReduce {
+  static constexpr int log2c(int x) { return x <= 1 ? 0 : 1 + log2c(x / 2); }
+  static constexpr int QB = log2c(64 / NTILES);  // lane bits that index rows
+  static constexpr int LV = log2c(V);
+  static constexpr int H = QB < LV ? QB : LV;    // halving levels
+  static constexpr int VL = V >> H;              // values each lane keeps
+
+  template <int L>
+  __device__ static __forceinline__ void halve(double (&v)[V], int lane) {
+    if constexpr (L < H) {
+      constexpr int d = NTILES << L;
+      constexpr int half = (V >> L) >> 1;
+      const bool up = (lane & d) != 0;
+#pragma unroll
+      for (int x = 0; x < half; ++x) {
+        const double send = up ? v[x] : v[x + half];
+        const double keep = up ? v[x + half] : v[x];
+        v[x] = keep + __shfl_xor(send, d, 64);
+      }
+      halve<L + 1>(v, lane);
+    }
+  }
+  __device__ static __forceinline__ void run(double (&v)[V], int lane) {
+    halve<0>(v, lane);
+#pragma unroll
+    for (int l = H; l < QB; ++l) v[0] += __shfl_xor(v[0], NTILES << l, 64);
+  }
+  // Index (in [0, V)) of the first value this lane holds after run().
+  __device__ static __forceinline__ int base(int lane) {
+    int b = 0;
+#pragma unroll
+    for (int l = 0; l < H; ++l)
+      if (lane & (NTILES << l)) b += V >> (l + 1);
+    return b;
+  }
+  // Among lanes holding identical copies (butterfly levels), lane 0 of the group writes.
+  __device__ static __forceinline__ bool writer(int lane) {
+    int mask = 0;
+#pragma unroll
+    for (int l = H; l < QB; ++l) mask |= NTILES << l;
+    return (lane & mask) == 0;
+  }
+};
+
+// ---- final reduction over workgroup partials ----------------------------------------
+// Sums partials[g][o] for g in [0, G) in a fixed order for outputs o in
+// [o0, o1) of the KC x MC class, then writes C.  Called by one workgroup.
+// `scratch` must hold kThreads doubles of LDS.
+__device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC, int o0, int o1,
+                                      double* scratch) {
+  const int tid = threadIdx.x;
+  const int nout = o1 - o0;
+  const double* P = a.partials;
+  auto store = [&](int o, double total) __attribute__((always_inline)) {
+    const int i = o / MC, j = o % MC;
+    if (i < a.k && j < a.m) {
+      const int64_t off = (int64_t)i * a.ldc + j;
+      if (a.c_f32) {
+        float* C = static_cast<float*>(a.C);
+        C[off] = a.accumulate ? (float)((double)C[off] + total) : (float)total;
+      } else {
+        double* C = static_cast<double*>(a.C);
+        C[off] = a.accumulate ? C[off] + total : total;
+      }
+    }
+  };
+  if (nout >= kThreads) {
+    for (int o = o0 + tid; o < o1; o += kThreads) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int g = 0;
+      for (; g + 4 <= G; g += 4) {
+        s0 += P[(int64_t)(g + 0) * KCMC + o];
+        s1 += P[(int64_t)(g + 1) * KCMC + o];
+        s2 += P[(int64_t)(g + 2) * KCMC + o];
+        s3 += P[(int64_t)(g + 3) * KCMC + o];
+      }
+      for (; g < G; ++g) s0 += P[(int64_t)g * KCMC + o];
+      store(o, (s0 + s1) + (s2 + s3));
+    }
+    return;
+  }
+  // nout < kThreads: SL slices of the workgroup range per output.
+  const int SL = kThreads / nout;
+  const int oi = tid % nout, s = tid / nout;
+  if (s < SL) {
+    const int g0 = (int)((int64_t)G * s / SL), g1 = (int)((int64_t)G * (s + 1) / SL);
+    const int o = o0 + oi;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int g = g0;
+    for (; g + 4 <= g1; g += 4) {
+      s0 += P[(int64_t)(g + 0) * KCMC + o];
+      s1 += P[(int64_t)(g + 1) * KCMC + o];
+      s2 += P[(int64_t)(g + 2) * KCMC + o];
+      s3 += P[(int64_t)(g + 3) * KCMC + o];
+    }
+    for (; g < g1; ++g) s0 += P[(int64_t)g * KCMC + o];
+    scratch[s * nout + oi] = (s0 + s1) + (s2 + s3);
+  }
+  __syncthreads();
+  if (tid < nout) {
+    double t = 0.0;
+    for (int q = 0; q < SL; ++q) t += scratch[q * nout + tid];
+    store(o0 + tid, t);
+  }
+}
+
+// Finisher kernel body: workgroup b handles outputs [b*64, b*64+64).
+__device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
+  __shared__ double scratch[kThreads];
+  const int KCMC = KC * MC;
+  const int o0 = blockIdx.x * 64;
+  const int o1 = o0 + 64 < KCMC ? o0 + 64 : KCMC;
+  if (o0 < KCMC) finish_outputs(a, G, KCMC, MC, o0, o1, scratch);
+}
+
+// ---- the streaming kernel body -----------------------------------------------------
+template <class S, int KC, int MC, int TM, int TN, int T, int PF>
+__device__ __forceinline__ void atb_body(const AtbArgs& a) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr bool kComp = Traits<S>::compensated;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int MB = KC / TM, NB = MC / TN, NTILES = MB * NB;
+  static_assert(KC % TM == 0 && MC % TN == 0, "tile must divide the width class");
+  static_assert(NTILES >= 1 && NTILES <= 64 && 64 % NTILES == 0, "1..64 register tiles");
+  constexpr int RG = kThreads / NTILES;  // row lanes
+  static_assert(T % RG == 0, "tile rows must be a multiple of the row lanes");
+  constexpr int RPT = T / RG;            // rows per thread per tile
+  constexpr int A_BYTES = T * KC * ES, B_BYTES = T * MC * ES;
+  static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
+  constexpr int A_VEC = A_BYTES / 16, B_VEC = B_BYTES / 16;
+  constexpr int A_PER = (A_VEC + kThreads - 1) / kThreads;
+  constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int KCMC = KC * MC;
+  constexpr int RED_BYTES = KCMC * 8;
+  constexpr int SMEM = (2 * STAGE > RED_BYTES ? 2 * STAGE : RED_BYTES) > kThreads * 8
+                           ? (2 * STAGE > RED_BYTES ? 2 * STAGE : RED_BYTES)
+                           : kThreads * 8;
+  static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
+  // Fold FP32 block sums into (hi, lo) every FOLD_TILES tiles (about 32 rows).
+  constexpr int FOLD_TILES = RPT >= 32 ? 1 : 32 / RPT;
+
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+  __shared__ int s_last;
+
+  const int tid = threadIdx.x;
+  const int p = tid % NTILES, q = tid / NTILES;
+  const int mb = p % MB, nb = p / MB;
+
+  Acc acc[TM][TN];
+  Acc hi[kComp ? TM : 1][kComp ? TN : 1];
+  Acc lo[kComp ? TM : 1][kComp ? TN : 1];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = Acc(0);
+  if constexpr (kComp) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) hi[i][j] = lo[i][j] = Acc(0);
+  }
+  int fold_count = 0;
+  auto fold = [&]() __attribute__((always_inline)) {
+    if constexpr (kComp) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          float s, e;
+          two_sum(hi[i][j], acc[i][j], s, e);
+          hi[i][j] = s;
+          lo[i][j] += e;
+          acc[i][j] = 0.0f;
+        }
+    }
+  };
+  auto after_tile = [&]() __attribute__((always_inline)) {
+    if constexpr (kComp) {
+      if (++fold_count == FOLD_TILES) {
+        fold();
+        fold_count = 0;
+      }
+    }
+  };
+
+  // Consume one LDS stage holding `T` rows (rows past the data are zero).
+  auto compute = [&](const unsigned char* stage) __attribute__((always_inline)) {
+    const S* As = reinterpret_cast<const S*>(stage);
+    const S* Bs = reinterpret_cast<const S*>(stage + A_BYTES);
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int r = q + s * RG;
+      Acc av[TM], bv[TN];
+      lds_read<S, TM>(As + r * KC + mb * TM, av);
+      lds_read<S, TN>(Bs + r * MC + nb * TN, bv);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fmadd(av[i], bv[j], acc[i][j]);
+    }
+  };
+
+  const int64_t row_begin = (int64_t)blockIdx.x * a.rows_per_wg;
+  int64_t row_end = row_begin + a.rows_per_wg;
+  if (row_end > a.n) row_end = a.n;
+  const int64_t nrows = row_end > row_begin ? row_end - row_begin : 0;
+  const int64_t nfull = a.dense ? nrows / T : 0;
+
+  // ---- pipelined dense tiles ---------------------------------------------------
+  if (nfull > 0) {
+    const v4u* gA = reinterpret_cast<const v4u*>(static_cast<const S*>(a.A) + row_begin * KC);
+    const v4u* gB = reinterpret_cast<const v4u*>(static_cast<const S*>(a.B) + row_begin * MC);
+    struct Regs {
+      v4u a[A_PER];
+      v4u b[B_PER];
+    };
+    auto load = [&](Regs& r, int64_t t) __attribute__((always_inline)) {
+      const v4u* ta = gA + t * A_VEC;
+      const v4u* tb = gB + t * B_VEC;
+#pragma unroll
+      for (int v = 0; v < A_PER; ++v) {
+        const int idx = v * kThreads + tid;
+        if (A_VEC % kThreads == 0 || idx < A_VEC) r.a[v] = ta[idx];
+      }
+#pragma unroll
+      for (int v = 0; v < B_PER; ++v) {
+        const int idx = v * kThreads + tid;
+        if (B_VEC % kThreads == 0 || idx < B_VEC) r.b[v] = tb[idx];
+      }
+    };
+    auto store = [&](const Regs& r, unsigned char* stage) __attribute__((always_inline)) {
+      v4u* la = reinterpret_cast<v4u*>(stage);
+      v4u* lb = reinterpret_cast<v4u*>(stage + A_BYTES);
+#pragma unroll
+      for (int v = 0; v < A_PER; ++v) {
+        const int idx = v * kThreads + tid;
+        if (A_VEC % kThreads == 0 || idx < A_VEC) la[idx] = r.a[v];
+      }
+#pragma unroll
+      for (int v = 0; v < B_PER; ++v) {
+        const int idx = v * kThreads + tid;
+        if (B_VEC % kThreads == 0 || idx < B_VEC) lb[idx] = r.b[v];
+      }
+    };
+    // One pipeline step on tile tt held in r: publish it to LDS, refill r with
+    // tile tt + PF, consume the LDS copy.
+    // The refill is unconditional (past the last tile it re-reads the last
+    // tile, which is in bounds and discarded) so that the number of loads in
+    // flight is the same on every path and each publish waits only for the
+    // oldest tile.
+    const int64_t last = nfull - 1;
+    auto step = [&](Regs& r, int64_t tt) __attribute__((always_inline)) {
+      unsigned char* stage = smem + (tt & 1) * STAGE;
+      // sched_barrier keeps the phases in program order: the scheduler would
+      // otherwise hoist refill loads above the publish, and since vmcnt
+      // retires in issue order the publish would then wait for them too.
+      __builtin_amdgcn_sched_barrier(0);
+      store(r, stage);
+      __syncthreads();
+      load(r, tt + PF < last ? tt + PF : last);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(stage);
+      after_tile();
+    };
+    Regs r0, r1;
+    load(r0, 0);
+    if constexpr (PF == 2) {
+      load(r1, 1 < last ? 1 : last);
+      int64_t t = 0;
+      for (; t + 1 < nfull; t += 2) {
+        step(r0, t);
+        step(r1, t + 1);
+      }
+      if (t < nfull) step(r0, t);
+    } else {
+      for (int64_t t = 0; t < nfull; ++t) step(r0, t);
+    }
+  }
+
+  // ---- remaining rows: element loads, zero padding -------------------------------
+  {
+    const S* gA = static_cast<const S*>(a.A);
+    const S* gB = static_cast<const S*>(a.B);
+    for (int64_t r0 = row_begin + nfull * T; r0 < row_end; r0 += T) {
+      const int nr = (int)((row_end - r0) < T ? (row_end - r0) : T);
+      __syncthreads();
+      S* la = reinterpret_cast<S*>(smem);
+      S* lb = reinterpret_cast<S*>(smem + A_BYTES);
+      for (int e = tid; e < T * KC; e += kThreads) {
+        const int r = e / KC, c = e % KC;
+        S v{};
+        if (r < nr && c < a.k) v = gA[(r0 + r) * a.lda + c];
+        la[e] = v;
+      }
+      for (int e = tid; e < T * MC; e += kThreads) {
+        const int r = e / MC, c = e % MC;
+        S v{};
+        if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
+        lb[e] = v;
+      }
+      __syncthreads();
+      compute(smem);
+      after_tile();
+    }
+  }
+
+  // ---- reduction -------------------------------------------------------------------
+  constexpr int V = TM * TN;
+  double v[V];
+  if constexpr (kComp) {
+    fold();
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) v[i * TN + j] = (double)hi[i][j] + (double)lo[i][j];
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) v[i * TN + j] = acc[i][j];
+  }
+  using WR = WaveReduce<NTILES, V>;
+  const int lane = tid & 63, wave = tid >> 6;
+  WR::run(v, lane);
+
+  __syncthreads();  // every wave is done with the LDS tiles
+  double* red = reinterpret_cast<double*>(smem);
+  const bool writer = WR::writer(lane);
+  const int vbase = WR::base(lane);
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    if (wave == w && writer) {
+#pragma unroll
+      for (int x = 0; x < WR::VL; ++x) {
+        const int idx = vbase + x;
+        const int o = (mb * TM + idx / TN) * MC + nb * TN + idx % TN;
+        red[o] = (w == 0) ? v[x] : red[o] + v[x];
+      }
+    }
+    __syncthreads();
+  }
+  for (int o = tid; o < KCMC; o += kThreads) a.partials[(int64_t)blockIdx.x * KCMC + o] = red[o];
+
+  if (!a.finish) return;
+  // ---- last workgroup to arrive writes C ------------------------------------------------
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned prev = atomicAdd(a.counter, 1u);
+    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  finish_outputs(a, (int)gridDim.x, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
+  if (tid == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace tsm

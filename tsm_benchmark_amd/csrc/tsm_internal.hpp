@@ -1,0 +1,32 @@
+// Host-side view of the generated kernel set (shared by the generated tables
+// and the C-ABI implementation).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tsm_device.hpp"
+
+namespace tsm {
+
+using LaunchFn = hipError_t (*)(const AtbArgs& args, unsigned grid, hipStream_t stream);
+
+struct KernelEntry {
+  const char* name;
+  int dtype;  // tsm_dtype
+  int kc, mc, tm, tn, rows, pf;
+  int lds_bytes;
+  LaunchFn launch;
+};
+
+struct TunedEntry {
+  int dtype, kc, mc;
+  int kernel;      // index into kernel_table()
+  int wgs_per_cu;  // persistent workgroups per compute unit
+};
+
+}  // namespace tsm
+
+// Defined by the generated tsm_table.cpp.
+extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count);
+extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count);

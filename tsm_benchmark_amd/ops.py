@@ -1,0 +1,160 @@
+"""PyTorch-ROCm front end of libtsm: the same entry points on torch tensors.
+
+Tensors must live on a HIP device; the current torch stream is passed to the
+library, so calls order with surrounding torch work.  ``atb`` is also
+registered as the torch operator ``torch.ops.tsm.atb`` and differentiates
+through hand-written backward kernels (``tsm_atb_backward``).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import torch
+
+from . import _lib
+
+_IN_TYPES = {torch.float64: _lib.F64, torch.float32: _lib.F32, torch.bfloat16: _lib.BF16}
+_OUT_TYPES = {torch.float64: _lib.F64, torch.float32: _lib.F32}
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _check_tall(X: torch.Tensor, name: str) -> None:
+    if X.dim() != 2:
+        raise ValueError(f"{name} must be 2-D (rows x width), got shape {tuple(X.shape)}")
+    if not X.is_cuda:
+        raise ValueError(f"{name} must be a HIP device tensor")
+    if X.stride(1) != 1 or (X.shape[0] > 1 and X.stride(0) < X.shape[1]):
+        raise ValueError(f"{name} must be row-major with unit column stride")
+    if X.dtype not in _IN_TYPES:
+        raise TypeError(f"{name}: unsupported dtype {X.dtype}")
+
+
+def _ld(X: torch.Tensor) -> int:
+    return X.stride(0) if X.shape[0] > 1 else X.shape[1]
+
+
+def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = None,
+        out: torch.Tensor | None = None, accumulate: bool = False,
+        config: _lib.KernelConfig | None = None) -> torch.Tensor:
+    """C = A^T B (or C += A^T B with ``accumulate`` and ``out``), no autograd.
+
+    A: n x k, B: n x m, same dtype (float64, float32 or bfloat16).  C is k x m
+    in ``out_dtype`` (float64 by default for float64 inputs, float32 otherwise)."""
+    _check_tall(A, "A")
+    _check_tall(B, "B")
+    if A.dtype != B.dtype or A.shape[0] != B.shape[0] or A.device != B.device:
+        raise ValueError("A and B must share dtype, row count and device")
+    n, k = A.shape
+    m = B.shape[1]
+    if out is None:
+        if out_dtype is None:
+            out_dtype = torch.float64 if A.dtype == torch.float64 else torch.float32
+        if accumulate:
+            raise ValueError("accumulate needs an existing `out`")
+        out = torch.empty((k, m), dtype=out_dtype, device=A.device)
+    if out.dtype not in _OUT_TYPES or out.shape != (k, m) or out.stride(1) != 1:
+        raise ValueError("out must be a row-major k x m float64/float32 tensor")
+    L = _lib.lib()
+    with torch.cuda.device(A.device):
+        st = L.tsm_atb_ex(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, m,
+                          A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), out.data_ptr(),
+                          out.stride(0) if k > 1 else m, 1 if accumulate else 0, None, 0,
+                          _stream(), ct.byref(config) if config is not None else None)
+    _lib.check(st, "tsm_atb")
+    return out
+
+
+def atb_backward(A: torch.Tensor, B: torch.Tensor, dC: torch.Tensor, *,
+                 need_dA: bool = True, need_dB: bool = True):
+    """(dA, dB) = (B dC^T, A dC) in the dtype of A and B; either may be skipped."""
+    _check_tall(A, "A")
+    _check_tall(B, "B")
+    n, k = A.shape
+    m = B.shape[1]
+    dC = dC.contiguous()
+    if dC.shape != (k, m) or dC.dtype not in _OUT_TYPES:
+        raise ValueError("dC must be k x m float64/float32")
+    dA = torch.empty((n, k), dtype=A.dtype, device=A.device) if need_dA else None
+    dB = torch.empty((n, m), dtype=B.dtype, device=B.device) if need_dB else None
+    L = _lib.lib()
+    with torch.cuda.device(A.device):
+        st = L.tsm_atb_backward(_IN_TYPES[A.dtype], _OUT_TYPES[dC.dtype], n, k, m,
+                                A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), dC.data_ptr(), m,
+                                dA.data_ptr() if dA is not None else None, k,
+                                dB.data_ptr() if dB is not None else None, m, _stream())
+    _lib.check(st, "tsm_atb_backward")
+    return dA, dB
+
+
+def fill_random(X: torch.Tensor, *, seed: int, stream_id: int, dist: int = _lib.DIST_UNIFORM
+                ) -> torch.Tensor:
+    """Fill a row-major device matrix with the counter-based synthetic inputs."""
+    _check_tall(X, "X")
+    rows, cols = X.shape
+    L = _lib.lib()
+    with torch.cuda.device(X.device):
+        st = L.tsm_fill_random(_IN_TYPES[X.dtype], dist, seed, stream_id, X.data_ptr(), rows,
+                               cols, _ld(X), _stream())
+    _lib.check(st, "tsm_fill_random")
+    return X
+
+
+def random_matrix(rows: int, cols: int, dtype: torch.dtype = torch.float64, *, seed: int,
+                  stream_id: int, dist: int = _lib.DIST_UNIFORM, device="cuda") -> torch.Tensor:
+    X = torch.empty((rows, cols), dtype=dtype, device=device)
+    return fill_random(X, seed=seed, stream_id=stream_id, dist=dist)
+
+
+class _AtbFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, A, B, out_dtype):
+        ctx.save_for_backward(A, B)
+        return atb(A, B, out_dtype=out_dtype)
+
+    @staticmethod
+    def backward(ctx, dC):
+        A, B = ctx.saved_tensors
+        need_dA, need_dB = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if not (need_dA or need_dB):
+            return None, None, None
+        if dC.dtype not in _OUT_TYPES:
+            dC = dC.float()
+        dA, dB = atb_backward(A, B, dC, need_dA=need_dA, need_dB=need_dB)
+        return dA, dB, None
+
+
+def atb_autograd(A: torch.Tensor, B: torch.Tensor, out_dtype: torch.dtype | None = None):
+    """Differentiable C = A^T B (gradients from the hand-written backward kernels)."""
+    if out_dtype is None:
+        out_dtype = torch.float64 if A.dtype == torch.float64 else torch.float32
+    return _AtbFunction.apply(A, B, out_dtype)
+
+
+# ---- torch operator registration ---------------------------------------------------
+@torch.library.custom_op("tsm::atb", mutates_args=())
+def _tsm_atb_op(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    return atb(A, B)
+
+
+@_tsm_atb_op.register_fake
+def _(A, B):
+    out = torch.float64 if A.dtype == torch.float64 else torch.float32
+    return A.new_empty((A.shape[1], B.shape[1]), dtype=out)
+
+
+def _op_setup_context(ctx, inputs, output):
+    ctx.save_for_backward(*inputs)
+
+
+def _op_backward(ctx, dC):
+    A, B = ctx.saved_tensors
+    if dC.dtype not in _OUT_TYPES:
+        dC = dC.float()
+    return atb_backward(A, B, dC, need_dA=ctx.needs_input_grad[0],
+                        need_dB=ctx.needs_input_grad[1])
+
+
+_tsm_atb_op.register_autograd(_op_backward, setup_context=_op_setup_context)
