@@ -129,6 +129,9 @@ int gen_sg(const oracle_ref_cfg& c, i64 K, int M, int N, const double* A, const 
   const i64 groups = stride / L;
   const int sg_per_wg = L / sg_rt;
   const i64 total_sg_rt = stride / sg_rt;  // :282
+  // When the runtime sub-group size makes total_sg_rt < num_tiles the
+  // reference's row loop never advances (k_stride == 0): refuse instead.
+  if ((total_sg_rt / nt) * sg_rt <= 0) return -1;
 #pragma omp parallel for schedule(static)
   for (i64 g = 0; g < groups; ++g) {
     for (int s = 0; s < sg_per_wg; ++s) {

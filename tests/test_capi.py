@@ -1,0 +1,129 @@
+"""The C-ABI library without a GPU: it loads, exports every function that
+include/tsm/tsm.h declares, and rejects invalid arguments with the documented
+status codes before any HIP call (so these paths behave the same on the CPU).
+"""
+import ctypes as ct
+import os
+import re
+
+import pytest
+
+from tsm_benchmark_amd import _lib, codegen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tsm", "tsm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tsm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert L.tsm_version() == 100
+
+
+def test_strerror_covers_every_status():
+    for s in range(7):
+        assert _lib.strerror(s) and _lib.strerror(s) != "unknown status"
+    assert _lib.strerror(99) == "unknown status"
+
+
+def test_kernel_table_matches_generator():
+    cfgs, defaults = codegen.library_kernels(codegen.load_tuning())
+    table = _lib.kernels()
+    assert [k["name"] for k in table] == [c.name for c in cfgs]
+    for k, c in zip(table, cfgs):
+        assert (k["kc"], k["mc"], k["tm"], k["tn"], k["rows"], k["pf"], k["lds_bytes"]) == \
+            (c.kc, c.mc, c.tm, c.tn, c.rows, c.pf, c.lds_bytes)
+    # every (dtype, k class, m class) has a kernel
+    have = {(k["in_type"], k["kc"], k["mc"]) for k in table}
+    for dt in range(3):
+        for kc in codegen.CLASSES:
+            for mc in codegen.CLASSES:
+                assert (dt, kc, mc) in have
+
+
+FAKE = 0x10000  # never dereferenced: validation fails first
+
+
+def atb(**kw):
+    a = dict(in_type=_lib.F64, out_type=_lib.F64, n=10, k=8, m=8, A=FAKE, lda=8, B=FAKE, ldb=8,
+             C=FAKE, ldc=8, acc=0, ws=None, wsb=0, stream=None)
+    a.update(kw)
+    return _lib.lib().tsm_atb(a["in_type"], a["out_type"], a["n"], a["k"], a["m"], a["A"],
+                              a["lda"], a["B"], a["ldb"], a["C"], a["ldc"], a["acc"], a["ws"],
+                              a["wsb"], a["stream"])
+
+
+@pytest.mark.parametrize("kw,status", [
+    (dict(n=-1), _lib.ERR_INVALID_ARGUMENT),
+    (dict(k=0), _lib.ERR_INVALID_ARGUMENT),
+    (dict(m=65), _lib.ERR_INVALID_ARGUMENT),
+    (dict(lda=7), _lib.ERR_INVALID_ARGUMENT),
+    (dict(ldb=4), _lib.ERR_INVALID_ARGUMENT),
+    (dict(ldc=7), _lib.ERR_INVALID_ARGUMENT),
+    (dict(A=None), _lib.ERR_INVALID_ARGUMENT),
+    (dict(C=None), _lib.ERR_INVALID_ARGUMENT),
+    (dict(A=FAKE + 4), _lib.ERR_MISALIGNED),
+    (dict(in_type=_lib.F32, B=FAKE + 2), _lib.ERR_MISALIGNED),
+    (dict(out_type=_lib.F64, C=FAKE + 4), _lib.ERR_MISALIGNED),
+    (dict(in_type=7), _lib.ERR_UNSUPPORTED),
+    (dict(out_type=_lib.BF16), _lib.ERR_UNSUPPORTED),
+    (dict(n=1 << 62), _lib.ERR_INVALID_ARGUMENT),
+])
+def test_atb_argument_errors(kw, status):
+    assert atb(**kw) == status
+
+
+def test_atb_config_errors():
+    L = _lib.lib()
+    bad = _lib.make_config(pf=3)
+    st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
+                      ct.byref(bad))
+    assert st == _lib.ERR_INVALID_ARGUMENT
+
+
+def test_backward_argument_errors():
+    L = _lib.lib()
+
+    def bwd(in_type=_lib.F64, g=_lib.F64, n=10, k=4, m=4, A=FAKE, lda=4, B=FAKE, ldb=4, dC=FAKE,
+            lddc=4, dA=FAKE, ldda=4, dB=FAKE, lddb=4):
+        return L.tsm_atb_backward(in_type, g, n, k, m, A, lda, B, ldb, dC, lddc, dA, ldda, dB,
+                                  lddb, None)
+
+    assert bwd(k=0) == _lib.ERR_INVALID_ARGUMENT
+    assert bwd(dC=None) == _lib.ERR_INVALID_ARGUMENT
+    assert bwd(ldda=3) == _lib.ERR_INVALID_ARGUMENT
+    assert bwd(dB=FAKE + 2) == _lib.ERR_MISALIGNED
+    assert bwd(g=_lib.BF16) == _lib.ERR_UNSUPPORTED
+    assert bwd(B=None) == _lib.ERR_INVALID_ARGUMENT
+    # nothing to do: succeeds without touching the device
+    assert bwd(dA=None, dB=None) == _lib.SUCCESS
+    assert bwd(n=0) == _lib.SUCCESS
+
+
+def test_fill_argument_errors():
+    L = _lib.lib()
+    assert L.tsm_fill_random(_lib.F64, 5, 1, 1, FAKE, 4, 4, 4, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_fill_random(_lib.F64, 0, 1, 1, FAKE, 4, 4, 3, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_fill_random(_lib.F32, 0, 1, 1, FAKE + 2, 4, 4, 4, None) == _lib.ERR_MISALIGNED
+    assert L.tsm_fill_random(9, 0, 1, 1, FAKE, 4, 4, 4, None) == _lib.ERR_UNSUPPORTED
+    assert L.tsm_fill_random(_lib.F64, 0, 1, 1, None, 0, 4, 4, None) == _lib.SUCCESS
+
+
+def test_kernel_info_bounds():
+    L = _lib.lib()
+    d = _lib.KernelDesc()
+    assert L.tsm_kernel_info(-1, ct.byref(d)) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_kernel_info(L.tsm_num_kernels(), ct.byref(d)) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_kernel_info(0, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_atb_workspace_size(_lib.F64, -1, 8, 8, None) == 0

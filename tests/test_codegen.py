@@ -1,0 +1,94 @@
+"""The kernel generator: configuration rules, CLI validation (the reference
+generator's flag checks, sycl_codegen_new.py:606-620), the library kernel set
+and the reference-name layer."""
+import subprocess
+import sys
+
+import pytest
+
+from tsm_benchmark_amd import codegen
+from tsm_benchmark_amd.codegen import KernelConfig
+
+
+def run_cli(*args):
+    return subprocess.run([sys.executable, "-m", "tsm_benchmark_amd.codegen", *args],
+                          capture_output=True, text=True)
+
+
+def test_width_class():
+    assert [codegen.width_class(x) for x in (1, 2, 3, 5, 8, 9, 33, 64)] == [1, 2, 4, 8, 8, 16, 64, 64]
+    with pytest.raises(ValueError):
+        codegen.width_class(65)
+    with pytest.raises(ValueError):
+        codegen.width_class(0)
+
+
+def test_config_rules():
+    ok = KernelConfig("f64", 8, 8, 8, 8, 256, 2)
+    assert ok.problems() == [] and ok.ntiles == 1 and ok.row_lanes == 256
+    assert KernelConfig("f64", 8, 8, 3, 8, 256, 2).problems()          # tile must divide
+    assert KernelConfig("f64", 64, 64, 4, 4, 256, 2).problems()        # 256 tiles > 64
+    assert KernelConfig("f64", 8, 8, 8, 8, 100, 2).problems()          # rows vs row lanes
+    assert KernelConfig("f64", 8, 8, 8, 8, 256, 3).problems()          # pf
+    assert KernelConfig("f64", 64, 64, 8, 8, 256, 2).problems()        # LDS budget
+    assert KernelConfig("bf16", 1, 1, 1, 1, 2, 1).problems()           # 16-byte chunks
+
+
+def test_every_class_has_a_valid_default():
+    for dt in codegen.DTYPES:
+        for kc in codegen.CLASSES:
+            for mc in codegen.CLASSES:
+                c = codegen.default_config(dt, kc, mc)
+                assert c.problems() == []
+                assert c.lds_bytes <= codegen.LDS_BUDGET
+
+
+def test_library_kernel_set():
+    cfgs, defaults = codegen.library_kernels({"entries": []})
+    names = [c.name for c in cfgs]
+    assert len(names) == len(set(names))
+    assert len(defaults) == 3 * 7 * 7
+    # the reference's default set (M in {4, 8, 16}, TM in {2, 4}) is present for both depths
+    for M in (4, 8, 16):
+        for TM in (2, 4):
+            for pf in (1, 2):
+                assert any(c.dtype == "f64" and c.kc == M and c.tm == TM and c.tn == TM
+                           and c.pf == pf for c in cfgs)
+
+
+def test_tuning_entry_overrides_default():
+    e = {"dtype": "f64", "kc": 8, "mc": 8, "tm": 4, "tn": 4, "rows": 128, "pf": 1, "wgs_per_cu": 3}
+    cfgs, defaults = codegen.library_kernels({"entries": [e]})
+    c, wpc = defaults[("f64", 8, 8)]
+    assert (c.tm, c.tn, c.rows, c.pf, wpc) == (4, 4, 128, 1, 3)
+    with pytest.raises(ValueError):
+        codegen.library_kernels({"entries": [dict(e, tm=3)]})
+
+
+def test_emitted_kernel_text():
+    c = KernelConfig("bf16", 32, 32, 8, 4, 256, 2)
+    text = codegen.emit_kernel(c)
+    assert "tsm_atb_bf16_32x32_t8x4_r256_p2(tsm::AtbArgs a)" in text
+    assert "tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 256, 2>(a);" in text
+    assert "hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r256_p2" in text
+
+
+def test_cli_rejects_inconsistent_flags():
+    assert run_cli("--local-size", "128").returncode == 1
+    assert run_cli("--sg-size", "32").returncode == 1
+    assert run_cli("--unroll", "0").returncode == 1
+    r = run_cli("--M", "8", "--tile-sizes", "4", "--stride", "1000")
+    assert r.returncode == 1 and "divisible" in r.stderr
+
+
+def test_cli_name_layer():
+    r = run_cli("--M", "4", "8", "--tile-sizes", "2", "4", "--name-layer", "-", "--with-benchmark")
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    for name in ("tsmttsm_4_4_2_2", "tsmttsm_4_4_4_4_lf", "tsmttsm_8_8_2_2_sg",
+                 "tsmttsm_8_8_4_4_sg_lf"):
+        assert f"inline tsm_status {name}(" in out
+    assert "tsmttsm_4_4_8_8" not in out  # TM must divide M (sycl_codegen_new.py:645-647)
+    assert "kGeneratedNames" in out
+    # default stride follows the reference's rule with the gfx950 wavefront
+    assert f"stride={codegen.suggest_stride([4, 8], [2, 4])}" in out

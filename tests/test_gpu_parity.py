@@ -1,0 +1,329 @@
+"""GPU parity of libtsm against the CPU oracle (run on an MI355X: -m gpu).
+
+Tolerances (elementwise, u = unit roundoff of the accumulation type):
+  FP64 inputs:       |C - C_exact| <= 4 sqrt(n) 2^-52 (|A|^T |B|)_ij
+  FP32/BF16 inputs:  the same form with 2^-23, and the Frobenius relative
+                     error <= 1e-6 against the FP64 oracle on the SAME rounded
+                     inputs.
+Deterministic reduction: the same call gives the same bits on every run.
+"""
+import ctypes as ct
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+from tsm_benchmark_amd import _lib, ops  # noqa: E402
+
+SEED = oracle.SEED
+HERE = os.path.dirname(os.path.abspath(__file__))
+DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+EPS = {torch.float64: 2.0**-52, torch.float32: 2.0**-23, torch.bfloat16: 2.0**-23}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    torch.cuda.set_device(0)
+    _lib.lib()
+    yield
+
+
+def host(X):
+    """Device matrix -> FP64 numpy (exact for every supported type)."""
+    if X.dtype == torch.bfloat16:
+        return oracle.bf16_bits_to_f64(X.cpu().view(torch.int16).numpy().view(np.uint16))
+    return X.cpu().double().numpy()
+
+
+def rand(n, w, dt, stream, dist=0):
+    return ops.random_matrix(n, w, dt, seed=SEED, stream_id=stream, dist=dist)
+
+
+def check_close(C, Ah, Bh, dt, exact=None):
+    """C (k x m numpy) against the oracle on the host copies of the inputs."""
+    n = Ah.shape[0]
+    ref, absC = oracle.atb_dot2(Ah, Bh, with_abs=True)
+    if exact is not None:
+        ref = exact
+    bound = 4.0 * np.sqrt(max(n, 1)) * EPS[dt] * absC
+    err = np.abs(C - ref)
+    assert np.all(err <= bound), f"max err/bound {np.max(err / np.maximum(bound, 1e-300)):.3g}"
+    if dt != torch.float64 and np.linalg.norm(ref) > 0:
+        assert np.linalg.norm(C - ref) / np.linalg.norm(ref) <= 1e-6
+    return err
+
+
+# ---- inputs ------------------------------------------------------------------------
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("dist", [0, 1])
+def test_fill_is_bit_exact(dt, dist):
+    X = rand(3001, 7, DT[dt], 3, dist)
+    ref = oracle.gen_matrix(3001, 7, dist=dist, seed=SEED, stream=3)
+    if dt == "f64":
+        assert np.array_equal(X.cpu().numpy().view(np.uint64), ref.view(np.uint64))
+    elif dt == "f32":
+        assert np.array_equal(X.cpu().numpy().view(np.uint32), oracle.round_f32(ref).view(np.uint32))
+    else:
+        got = X.cpu().view(torch.int16).numpy().view(np.uint16)
+        assert np.array_equal(got, oracle.round_bf16_bits(ref))
+
+
+# ---- golden fixtures ----------------------------------------------------------------
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "atb_golden.json")))["cases"]
+
+
+@pytest.mark.parametrize("case", GOLDEN, ids=lambda c: f"{c['storage']}-d{c['dist']}-n{c['n']}-{c['k']}x{c['m']}")
+def test_golden_fixtures(case):
+    dt = DT[case["storage"]]
+    n, k, m = case["n"], case["k"], case["m"]
+    A = rand(n, k, dt, 1, case["dist"])
+    B = rand(n, m, dt, 2, case["dist"])
+    C = ops.atb(A, B, out_dtype=torch.float64).cpu().numpy().ravel()
+    exact = np.array([float.fromhex(x) for x in case["C"]])
+    absC = np.array([float.fromhex(x) for x in case["absC"]])
+    bound = 4.0 * np.sqrt(n) * EPS[dt] * absC
+    assert np.all(np.abs(C - exact) <= bound)
+    if dt != torch.float64:
+        assert np.linalg.norm(C - exact) / np.linalg.norm(exact) <= 1e-6
+
+
+# ---- shapes and layouts -------------------------------------------------------------------
+SHAPES = [(1, 1), (2, 2), (3, 5), (8, 8), (4, 32), (32, 4), (16, 16), (33, 17), (64, 64), (1, 64),
+          (64, 1), (7, 9)]
+NS = [0, 1, 2, 255, 256, 257, 4097, 100003]
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("km", SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_shapes_and_ragged_rows(dt, km):
+    k, m = km
+    for n in NS:
+        A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+        C = ops.atb(A, B, out_dtype=torch.float64).cpu().numpy()
+        if n == 0:
+            assert np.array_equal(C, np.zeros((k, m)))
+            continue
+        check_close(C, host(A), host(B), DT[dt])
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+def test_strided_views_and_misaligned_bases(dt):
+    n = 50021
+    X = rand(n + 1, 24, DT[dt], 5)
+    Y = rand(n + 1, 16, DT[dt], 6)
+    cases = [
+        (X[:n, 3:11], Y[:n, 0:8]),   # lda = 24 > k, ldb = 16 > m
+        (X[1:, :8], Y[1:, :16]),     # base offset by one row, lda = 24 > k
+        (X[:n, 1:9], Y[:n, 1:5]),    # base offset by one element (misaligned for vector loads)
+    ]
+    for A, B in cases:
+        C = ops.atb(A, B, out_dtype=torch.float64).cpu().numpy()
+        check_close(C, host(A), host(B), DT[dt])
+
+
+def test_accumulate_and_f32_output():
+    n, k, m = 70001, 8, 8
+    A, B = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)
+    ref, absC = oracle.atb_dot2(host(A), host(B), with_abs=True)
+    C0 = torch.from_numpy(oracle.gen_matrix(k, m, seed=SEED, stream=9)).cuda()
+    C = C0.clone()
+    ops.atb(A, B, out=C, accumulate=True)
+    got = C.cpu().numpy()
+    bound = 4 * np.sqrt(n) * 2.0**-52 * absC + np.spacing(np.abs(ref + C0.cpu().numpy()))
+    assert np.all(np.abs(got - (ref + C0.cpu().numpy())) <= bound)
+    C32 = ops.atb(A, B, out_dtype=torch.float32).cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(C32 - ref) <= np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+                  + 4 * np.sqrt(n) * 2.0**-52 * absC)
+    # accumulate into an FP32 output
+    D = torch.ones((k, m), dtype=torch.float32, device="cuda")
+    ops.atb(A, B, out=D, accumulate=True)
+    assert np.all(np.abs(D.cpu().numpy() - (ref + 1.0)) <= 2 * np.spacing(np.float32(np.abs(ref).max() + 1)))
+
+
+@pytest.mark.parametrize("dt,k,m", [("f64", 8, 8), ("f64", 4, 32), ("f64", 64, 64),
+                                    ("f32", 32, 32), ("bf16", 32, 32), ("f64", 1, 1)])
+def test_every_generated_variant(dt, k, m):
+    dtype_id = {"f64": 0, "f32": 1, "bf16": 2}[dt]
+    n = 123457
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    Ah, Bh = host(A), host(B)
+    seen = 0
+    for kd in _lib.kernels():
+        if kd["in_type"] != dtype_id or kd["kc"] != k or kd["mc"] != m:
+            continue
+        cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"])
+        for grid in (0, 7):
+            cfg.grid = grid
+            C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
+            check_close(C, Ah, Bh, DT[dt])
+        seen += 1
+    assert seen >= 1
+
+
+def test_unknown_variant_is_reported():
+    A, B = rand(100, 8, torch.float64, 1), rand(100, 8, torch.float64, 2)
+    with pytest.raises(_lib.TsmError) as e:
+        ops.atb(A, B, config=_lib.make_config(tm=8, tn=8, rows=12345))
+    assert e.value.status == _lib.ERR_NO_KERNEL
+
+
+def test_determinism_bitwise():
+    A, B = rand(1 << 20, 8, torch.float64, 1), rand(1 << 20, 8, torch.float64, 2)
+    first = ops.atb(A, B)
+    s = torch.cuda.Stream()
+    for i in range(20):
+        with torch.cuda.stream(s if i % 2 else torch.cuda.current_stream()):
+            C = ops.atb(A, B)
+        torch.cuda.synchronize()
+        assert torch.equal(C, first)
+
+
+def test_size_independent_properties():
+    n = 3_000_017
+    A, B = rand(n, 8, torch.float64, 1), rand(n, 8, torch.float64, 2)
+    C = ops.atb(A, B)
+    # transposition: the same kernel with the operands swapped gives C^T bit for bit
+    assert torch.equal(ops.atb(B, A), C.T)
+    # power-of-two scaling is exact
+    assert torch.equal(ops.atb(4 * A, B), 4 * C)
+    # split-n additivity through accumulate
+    h = 1_234_567
+    D = ops.atb(A[:h], B[:h])
+    ops.atb(A[h:], B[h:], out=D, accumulate=True)
+    _, absC = oracle.atb_dot2(host(A), host(B), with_abs=True)
+    assert np.all(np.abs((D - C).cpu().numpy()) <= 8 * np.sqrt(n) * 2.0**-52 * absC)
+
+
+@pytest.mark.parametrize("n,k,m", [(1 << 25, 8, 8), (14_913_080, 4, 32), (14_913_080, 32, 4)])
+def test_full_size_sweep_against_exact_oracle(n, k, m):
+    """The FP64 sweep shapes at A + B = 4 GiB against the exact int128 oracle
+    (the host regenerates the same inputs from the counter-based generator)."""
+    A, B = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)
+    C = ops.atb(A, B).cpu().numpy()
+    del A, B
+    Ah = oracle.gen_matrix(n, k, seed=SEED, stream=1)
+    Bh = oracle.gen_matrix(n, m, seed=SEED, stream=2)
+    exact = oracle.atb_exact(Ah, Bh, 52, 52)
+    check_close(C, Ah, Bh, torch.float64, exact=exact)
+
+
+def test_row_index_beyond_2_to_31_elements():
+    """n * k > 2^31: 64-bit addressing, checked by split-n additivity over byte
+    offsets past 4 GiB (FP32 storage, 8 GiB per matrix)."""
+    n, k = (1 << 28) + 123, 8
+    A = rand(n, k, torch.float32, 1)
+    B = rand(n, k, torch.float32, 2)
+    full = ops.atb(A, B, out_dtype=torch.float64)
+    parts = sum(ops.atb(A[i:j], B[i:j], out_dtype=torch.float64)
+                for i, j in ((0, 1 << 26), (1 << 26, 3 << 26), (3 << 26, n)))
+    tail = ops.atb(A[n - 1000:], B[n - 1000:], out_dtype=torch.float64).cpu().numpy()
+    check_close(tail, host(A[n - 1000:]), host(B[n - 1000:]), torch.float32)
+    rel = (torch.linalg.norm(full - parts) / torch.linalg.norm(full)).item()
+    assert rel <= 1e-7
+
+
+# ---- precision (FP32 / BF16 storage) ---------------------------------------------------------
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("dist", [0, 1])
+def test_low_precision_storage_error(dt, dist):
+    n, k, m = 1 << 22, 32, 32
+    A, B = rand(n, k, DT[dt], 1, dist), rand(n, m, DT[dt], 2, dist)
+    C = ops.atb(A, B, out_dtype=torch.float64).cpu().numpy()
+    Ah, Bh = host(A), host(B)
+    q = 52 if dist == 0 else 20
+    exact = oracle.atb_exact(Ah, Bh, q, q)
+    _, absC = oracle.atb_dot2(Ah, Bh, with_abs=True)
+    frob = np.linalg.norm(C - exact) / np.linalg.norm(exact)
+    assert frob <= 1e-6, frob
+    assert np.max(np.abs(C - exact) / absC) <= 1e-6
+
+
+# ---- backward ------------------------------------------------------------------------------------
+@pytest.mark.parametrize("dt,k,m", [("f64", 4, 4), ("f64", 16, 16), ("f64", 64, 3), ("f64", 5, 64),
+                                    ("f32", 16, 16), ("bf16", 8, 8), ("f32", 1, 1)])
+def test_backward_against_oracle(dt, k, m):
+    n = 40001
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
+    gdt = torch.float64 if dt == "f64" else torch.float32
+    dA, dB = ops.atb_backward(A, B, G.to("cuda", gdt))
+    dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.to(gdt).double().numpy())
+    if dt == "f64":
+        tol = dict(rtol=1e-13, atol=1e-13)
+    elif dt == "f32":
+        tol = dict(rtol=1e-5, atol=1e-5)
+    else:
+        tol = dict(rtol=2**-7, atol=2**-7 * np.abs(G.numpy()).sum())
+    np.testing.assert_allclose(host(dA), dA_ref, **tol)
+    np.testing.assert_allclose(host(dB), dB_ref, **tol)
+    # one side only
+    dA2, dB2 = ops.atb_backward(A, B, G.to("cuda", gdt), need_dB=False)
+    assert dB2 is None and torch.equal(dA2, dA)
+
+
+def test_autograd_matches_torch_matmul():
+    n, k, m = 30011, 16, 16
+    A0, B0 = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3)).cuda()
+    A, B = A0.clone().requires_grad_(), B0.clone().requires_grad_()
+    (ops.atb_autograd(A, B) * G).sum().backward()
+    At, Bt = A0.clone().requires_grad_(), B0.clone().requires_grad_()
+    (torch.matmul(At.T.contiguous(), Bt) * G).sum().backward()
+    torch.testing.assert_close(A.grad, At.grad, rtol=1e-12, atol=1e-12)
+    torch.testing.assert_close(B.grad, Bt.grad, rtol=1e-12, atol=1e-12)
+    # the registered operator differentiates the same way
+    A2, B2 = A0.clone().requires_grad_(), B0.clone().requires_grad_()
+    (torch.ops.tsm.atb(A2, B2) * G).sum().backward()
+    assert torch.equal(A2.grad, A.grad) and torch.equal(B2.grad, B.grad)
+
+
+def test_gradcheck_fp64():
+    A = rand(64, 3, torch.float64, 1).requires_grad_()
+    B = rand(64, 4, torch.float64, 2).requires_grad_()
+    assert torch.autograd.gradcheck(lambda a, b: ops.atb_autograd(a, b), (A, B), eps=1e-6,
+                                    atol=1e-8)
+
+
+# ---- graph capture -------------------------------------------------------------------------------
+def test_hip_graph_replays_match_eager():
+    n, k, m = 1 << 20, 8, 8
+    A, B = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)
+    L = _lib.lib()
+    ws_bytes = L.tsm_atb_workspace_size(_lib.F64, n, k, m, None)
+    ws = torch.zeros(ws_bytes, dtype=torch.uint8, device="cuda")
+    C = torch.empty((k, m), dtype=torch.float64, device="cuda")
+
+    def call():
+        st = L.tsm_atb(_lib.F64, _lib.F64, n, k, m, A.data_ptr(), k, B.data_ptr(), m,
+                       C.data_ptr(), m, 0, ws.data_ptr(), ws_bytes,
+                       torch.cuda.current_stream().cuda_stream)
+        assert st == _lib.SUCCESS
+
+    call()
+    torch.cuda.synchronize()
+    eager = C.clone()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        call()  # warm-up on the capture stream
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        call()
+    for i in range(5):
+        C.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C, eager), f"replay {i}"
+        call()  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(C, eager)

@@ -1,0 +1,238 @@
+// Backward of C = A^T B for an upstream gradient dC (k x m):
+//   dA = B dC^T  (dA[r, i] = sum_j B[r, j] dC[i, j]),  n x k
+//   dB = A dC    (dB[r, j] = sum_i A[r, i] dC[i, j]),  n x m
+// Both are tall-skinny times small: one streaming pass reads a row of A and
+// B and writes a row of dA and dB.  dC (at most 64 x 64) is staged once per
+// workgroup in LDS, zero-padded to the width class W >= max(k, m), and read
+// with wave-uniform addresses (LDS broadcast).  Each thread owns whole rows
+// (grid-stride); a row is read with 16-byte loads when the matrices are
+// dense and aligned, and the outputs are produced in 16-byte chunks.
+// Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
+// a sum of at most 64 products).
+#include "tsm/tsm.h"
+#include "tsm_internal.hpp"
+
+namespace tsm {
+
+struct BwdArgs {
+  const void* A;
+  const void* B;
+  const void* dC;
+  void* dA;
+  void* dB;
+  int64_t n, lda, ldb, lddc, ldda, lddb;
+  int k, m;
+  int dc_f64;   // dC is binary64 (1) or binary32 (0)
+  int dense_a;  // lda == k, ldda == k, k*s % 16 == 0, 16-byte aligned A and dA
+  int dense_b;  // likewise for B and dB
+};
+
+template <class S>
+__device__ __forceinline__ S narrow(typename Traits<S>::Acc x);
+template <>
+__device__ __forceinline__ double narrow<double>(double x) { return x; }
+template <>
+__device__ __forceinline__ float narrow<float>(float x) { return x; }
+template <>
+__device__ __forceinline__ bf16 narrow<bf16>(float x) {
+  // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16{(uint16_t)(u >> 16)};
+}
+
+// Load `cnt` (<= W) elements of a row into registers, zero-padded to W.
+template <class S, int W>
+__device__ __forceinline__ void load_row(const S* row, int cnt, bool dense,
+                                         typename Traits<S>::Acc (&out)[W]) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int PER = 16 / ES;  // elements per 16-byte chunk
+  if (dense && W % PER == 0) {
+#pragma unroll
+    for (int c = 0; c < W / PER; ++c) {
+      if (c * PER < cnt) {
+        const v4u u = reinterpret_cast<const v4u*>(row)[c];
+        Unpack<S>::run(u, out + c * PER);
+      } else {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) out[c * PER + e] = Acc(0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < W; ++c) out[c] = c < cnt ? widen(row[c]) : Acc(0);
+  }
+}
+
+// out[o] = sum_t x[t] * coef(o, t) for o in [0, cnt), written to `row`.
+// coef(o, t) = G[o * W + t] (dA: G = dC) or G[t * W + o] (dB: G = dC).
+template <class S, int W, bool TRANSPOSED>
+__device__ __forceinline__ void emit_row(S* row, int cnt, bool dense,
+                                         const typename Traits<S>::Acc (&x)[W],
+                                         const typename Traits<S>::Acc* G) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int PER = 16 / ES;
+  constexpr int CH = PER < W ? PER : W;
+  for (int o0 = 0; o0 < cnt; o0 += CH) {
+    Acc acc[CH];
+#pragma unroll
+    for (int e = 0; e < CH; ++e) acc[e] = Acc(0);
+#pragma unroll
+    for (int t = 0; t < W; ++t)
+#pragma unroll
+      for (int e = 0; e < CH; ++e) {
+        const int o = o0 + e;
+        const Acc g = TRANSPOSED ? G[t * W + o] : G[o * W + t];
+        acc[e] = fmadd(x[t], g, acc[e]);
+      }
+    if (dense && CH == PER && o0 + CH <= cnt) {
+      S tmp[CH];
+#pragma unroll
+      for (int e = 0; e < CH; ++e) tmp[e] = narrow<S>(acc[e]);
+      v4u u;
+      __builtin_memcpy(&u, tmp, 16);
+      *reinterpret_cast<v4u*>(row + o0) = u;
+    } else {
+#pragma unroll
+      for (int e = 0; e < CH; ++e)
+        if (o0 + e < cnt) row[o0 + e] = narrow<S>(acc[e]);
+    }
+  }
+}
+
+template <class S, int W>
+__device__ __forceinline__ void bwd_body(const BwdArgs& a) {
+  using Acc = typename Traits<S>::Acc;
+  // dC zero-padded to W x W in LDS: G[i * W + j] = dC[i, j].
+  __shared__ Acc G[W * W + 1];
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const int i = e / W, j = e % W;
+    Acc v = Acc(0);
+    if (i < a.k && j < a.m) {
+      const int64_t off = (int64_t)i * a.lddc + j;
+      v = a.dc_f64 ? (Acc) static_cast<const double*>(a.dC)[off]
+                   : (Acc) static_cast<const float*>(a.dC)[off];
+    }
+    G[e] = v;
+  }
+  __syncthreads();
+  const S* A = static_cast<const S*>(a.A);
+  const S* B = static_cast<const S*>(a.B);
+  S* dA = static_cast<S*>(a.dA);
+  S* dB = static_cast<S*>(a.dB);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n; r += stride) {
+    Acc x[W];
+    if (dA) {
+      load_row<S, W>(B + r * a.ldb, a.m, a.dense_b != 0, x);
+      emit_row<S, W, false>(dA + r * a.ldda, a.k, a.dense_a != 0, x, G);
+    }
+    if (dB) {
+      load_row<S, W>(A + r * a.lda, a.k, a.dense_a != 0, x);
+      emit_row<S, W, true>(dB + r * a.lddb, a.m, a.dense_b != 0, x, G);
+    }
+  }
+}
+
+}  // namespace tsm
+
+#define TSM_BWD_KERNEL(NAME, S, W)                                                    \
+  extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
+    tsm::bwd_body<S, W>(a);                                                           \
+  }
+
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w1, double, 1)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w2, double, 2)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w4, double, 4)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w8, double, 8)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w16, double, 16)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w32, double, 32)
+TSM_BWD_KERNEL(tsm_atb_bwd_f64_w64, double, 64)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w1, float, 1)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w2, float, 2)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w4, float, 4)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w8, float, 8)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w16, float, 16)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w32, float, 32)
+TSM_BWD_KERNEL(tsm_atb_bwd_f32_w64, float, 64)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w1, tsm::bf16, 1)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w2, tsm::bf16, 2)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w4, tsm::bf16, 4)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w8, tsm::bf16, 8)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w16, tsm::bf16, 16)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w32, tsm::bf16, 32)
+TSM_BWD_KERNEL(tsm_atb_bwd_bf16_w64, tsm::bf16, 64)
+
+namespace {
+typedef void (*BwdKernel)(tsm::BwdArgs);
+const BwdKernel kBwd[3][7] = {
+    {tsm_atb_bwd_f64_w1, tsm_atb_bwd_f64_w2, tsm_atb_bwd_f64_w4, tsm_atb_bwd_f64_w8,
+     tsm_atb_bwd_f64_w16, tsm_atb_bwd_f64_w32, tsm_atb_bwd_f64_w64},
+    {tsm_atb_bwd_f32_w1, tsm_atb_bwd_f32_w2, tsm_atb_bwd_f32_w4, tsm_atb_bwd_f32_w8,
+     tsm_atb_bwd_f32_w16, tsm_atb_bwd_f32_w32, tsm_atb_bwd_f32_w64},
+    {tsm_atb_bwd_bf16_w1, tsm_atb_bwd_bf16_w2, tsm_atb_bwd_bf16_w4, tsm_atb_bwd_bf16_w8,
+     tsm_atb_bwd_bf16_w16, tsm_atb_bwd_bf16_w32, tsm_atb_bwd_bf16_w64},
+};
+int elem(int t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
+}  // namespace
+
+extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, int k,
+                                       int m, const void* A, int64_t lda, const void* B,
+                                       int64_t ldb, const void* dC, int64_t lddc, void* dA,
+                                       int64_t ldda, void* dB, int64_t lddb,
+                                       tsm_stream_t stream) {
+  if (in_type != TSM_F64 && in_type != TSM_F32 && in_type != TSM_BF16) return TSM_ERR_UNSUPPORTED;
+  if (grad_type != TSM_F64 && grad_type != TSM_F32) return TSM_ERR_UNSUPPORTED;
+  if (n < 0 || k < 1 || m < 1 || k > TSM_MAX_WIDTH || m > TSM_MAX_WIDTH)
+    return TSM_ERR_INVALID_ARGUMENT;
+  if (lda < k || ldb < m || lddc < m) return TSM_ERR_INVALID_ARGUMENT;
+  if (!dC) return TSM_ERR_INVALID_ARGUMENT;
+  if (dA && ldda < k) return TSM_ERR_INVALID_ARGUMENT;
+  if (dB && lddb < m) return TSM_ERR_INVALID_ARGUMENT;
+  if (n > 0 && ((dA && !B) || (dB && !A))) return TSM_ERR_INVALID_ARGUMENT;
+  const uintptr_t es = (uintptr_t)elem(in_type);
+  if ((uintptr_t)A % es || (uintptr_t)B % es || (uintptr_t)dA % es || (uintptr_t)dB % es)
+    return TSM_ERR_MISALIGNED;
+  if ((uintptr_t)dC % (uintptr_t)elem(grad_type)) return TSM_ERR_MISALIGNED;
+  if (n == 0 || (!dA && !dB)) return TSM_SUCCESS;
+  int w = 1, wi = 0;
+  while (w < (k > m ? k : m)) {
+    w <<= 1;
+    ++wi;
+  }
+  tsm::BwdArgs a{};
+  a.A = A;
+  a.B = B;
+  a.dC = dC;
+  a.dA = dA;
+  a.dB = dB;
+  a.n = n;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.lddc = lddc;
+  a.ldda = ldda;
+  a.lddb = lddb;
+  a.k = k;
+  a.m = m;
+  a.dc_f64 = grad_type == TSM_F64 ? 1 : 0;
+  const auto al16 = [](const void* p) { return ((uintptr_t)p % 16) == 0; };
+  a.dense_a = (lda == k && (!dA || ldda == k) && (k * (int)es) % 16 == 0 && al16(A) &&
+               (!dA || al16(dA)))
+                  ? 1
+                  : 0;
+  a.dense_b = (ldb == m && (!dB || lddb == m) && (m * (int)es) % 16 == 0 && al16(B) &&
+               (!dB || al16(dB)))
+                  ? 1
+                  : 0;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int64_t blocks = (n + 255) / 256;
+  const int64_t cap = (int64_t)cus * 8;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(kBwd[(int)in_type][wi], dim3((unsigned)blocks), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), a);
+  return (tsm_status)tsm_internal_hip_status(hipGetLastError());
+}

@@ -361,6 +361,12 @@ const char* tsm_strerror(tsm_status s) {
 
 int tsm_last_hip_error(void) { return g_last_hip_error; }
 
+int tsm_internal_hip_status(hipError_t e) {
+  if (e == hipSuccess) return TSM_SUCCESS;
+  g_last_hip_error = (int)e;
+  return TSM_ERR_HIP;
+}
+
 int tsm_version(void) { return TSM_VERSION_MAJOR * 10000 + TSM_VERSION_MINOR * 100; }
 
 int tsm_num_kernels(void) {
@@ -388,12 +394,7 @@ tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out) {
 
 }  // extern "C"
 
-// ---- placeholders until the backward and batched kernels land -------------------------
-extern "C" tsm_status tsm_atb_backward(tsm_dtype, tsm_dtype, int64_t, int, int, const void*,
-                                       int64_t, const void*, int64_t, const void*, int64_t, void*,
-                                       int64_t, void*, int64_t, tsm_stream_t) {
-  return TSM_ERR_UNSUPPORTED;
-}
+// ---- placeholders until the batched kernel lands --------------------------------------
 extern "C" size_t tsm_atb_batched_workspace_size(tsm_dtype, int, const tsm_problem*) { return 0; }
 extern "C" tsm_status tsm_atb_batched(tsm_dtype, tsm_dtype, int, const tsm_problem*, int, void*,
                                       size_t, tsm_stream_t) {

@@ -27,6 +27,10 @@ struct TunedEntry {
 
 }  // namespace tsm
 
-// Defined by the generated tsm_table.cpp.
+// Records a HIP error for tsm_last_hip_error() and maps it to a status
+// (TSM_SUCCESS for hipSuccess, else TSM_ERR_HIP).  Defined in tsm_capi.hip.
+extern "C" int tsm_internal_hip_status(hipError_t e);
+
+// Defined by the generated tsm_table.hip.
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count);
 extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count);

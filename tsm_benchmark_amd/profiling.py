@@ -1,0 +1,65 @@
+"""Small drivers for rocprofv3 runs (kernel traces and PMC counters).
+
+  python -m tsm_benchmark_amd.profiling atb  --n 33554432 --k 8 --m 8 --dtype f64 --reps 5
+  python -m tsm_benchmark_amd.profiling copy --bytes 1073741824 --reps 5
+  python -m tsm_benchmark_amd.profiling bwd  --n 8388608 --k 16 --m 16 --dtype f32 --reps 5
+
+`copy` runs torch's device copy of a known byte count: the calibration for the
+FETCH_SIZE / WRITE_SIZE counters (bytes counted per byte moved).  Every step
+prints one line so a long profiled run shows progress.
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+
+import torch
+
+from . import _lib, ops
+
+DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+SEED = 20261203
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["atb", "copy", "bwd"])
+    ap.add_argument("--n", type=int, default=1 << 25)
+    ap.add_argument("--k", type=int, default=8)
+    ap.add_argument("--m", type=int, default=8)
+    ap.add_argument("--dtype", default="f64", choices=list(DT))
+    ap.add_argument("--bytes", type=int, default=1 << 30)
+    ap.add_argument("--reps", type=int, default=5)
+    args = ap.parse_args(argv)
+    torch.cuda.set_device(0)
+    if args.what == "copy":
+        X = torch.ones(args.bytes // 8, dtype=torch.float64, device="cuda")
+        Y = torch.empty_like(X)
+        for i in range(args.reps):
+            Y.copy_(X)
+            torch.cuda.synchronize()
+            print(f"copy {i}: {args.bytes} bytes read, {args.bytes} bytes written", flush=True)
+        return 0
+    dt = DT[args.dtype]
+    A = ops.random_matrix(args.n, args.k, dt, seed=SEED, stream_id=1)
+    B = ops.random_matrix(args.n, args.m, dt, seed=SEED, stream_id=2)
+    torch.cuda.synchronize()
+    if args.what == "atb":
+        idx, grid = _lib.plan({"f64": 0, "f32": 1, "bf16": 2}[args.dtype], args.n, args.k, args.m)
+        print(f"kernel {_lib.kernels()[idx]['name']} grid {grid}", flush=True)
+        C = torch.empty((args.k, args.m), dtype=torch.float64, device="cuda")
+        for i in range(args.reps):
+            ops.atb(A, B, out=C)
+            torch.cuda.synchronize()
+            print(f"atb {i}: {A.element_size() * args.n * (args.k + args.m)} bytes", flush=True)
+    else:
+        G = torch.randn((args.k, args.m), dtype=torch.float32, device="cuda")
+        for i in range(args.reps):
+            ops.atb_backward(A, B, G.to(torch.float64 if dt == torch.float64 else torch.float32))
+            torch.cuda.synchronize()
+            print(f"bwd {i}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
