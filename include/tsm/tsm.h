@@ -68,6 +68,7 @@ typedef struct {
   int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth) */
   int wgs_per_cu;      /* persistent workgroups per compute unit */
   int64_t grid;        /* total workgroups; overrides wgs_per_cu when > 0 (the "stride" role) */
+  int load_policy;     /* 0 tuned default, 1 cached loads, 2 non-temporal (streaming) loads */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------
@@ -149,6 +150,7 @@ typedef struct {
   int in_type;                      /* tsm_dtype */
   int kc, mc;                       /* width class: k <= kc, m <= mc */
   int tm, tn, rows, pf;
+  int load_policy;                  /* 1 cached, 2 non-temporal */
   int lds_bytes;
 } tsm_kernel_desc;
 

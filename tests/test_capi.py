@@ -42,8 +42,8 @@ def test_kernel_table_matches_generator():
     table = _lib.kernels()
     assert [k["name"] for k in table] == [c.name for c in cfgs]
     for k, c in zip(table, cfgs):
-        assert (k["kc"], k["mc"], k["tm"], k["tn"], k["rows"], k["pf"], k["lds_bytes"]) == \
-            (c.kc, c.mc, c.tm, c.tn, c.rows, c.pf, c.lds_bytes)
+        assert (k["kc"], k["mc"], k["tm"], k["tn"], k["rows"], k["pf"], k["nt"], k["lds_bytes"]) == \
+            (c.kc, c.mc, c.tm, c.tn, c.rows, c.pf, c.nt, c.lds_bytes)
     # every (dtype, k class, m class) has a kernel
     have = {(k["in_type"], k["kc"], k["mc"]) for k in table}
     for dt in range(3):

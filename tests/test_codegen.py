@@ -69,7 +69,7 @@ def test_emitted_kernel_text():
     c = KernelConfig("bf16", 32, 32, 8, 4, 256, 2)
     text = codegen.emit_kernel(c)
     assert "tsm_atb_bf16_32x32_t8x4_r256_p2(tsm::AtbArgs a)" in text
-    assert "tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 256, 2>(a);" in text
+    assert "tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 256, 2, 0>(a);" in text
     assert "hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r256_p2" in text
 
 

@@ -159,7 +159,7 @@ def test_every_generated_variant(dt, k, m):
     for kd in _lib.kernels():
         if kd["in_type"] != dtype_id or kd["kc"] != k or kd["mc"] != m:
             continue
-        cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"])
+        cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"], nt=kd["nt"])
         for grid in (0, 7):
             cfg.grid = grid
             C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
@@ -226,8 +226,10 @@ def test_row_index_beyond_2_to_31_elements():
                 for i, j in ((0, 1 << 26), (1 << 26, 3 << 26), (3 << 26, n)))
     tail = ops.atb(A[n - 1000:], B[n - 1000:], out_dtype=torch.float64).cpu().numpy()
     check_close(tail, host(A[n - 1000:]), host(B[n - 1000:]), torch.float32)
+    # Both sums carry the FP32 block-accumulation error (about sqrt(32) * 2^-24
+    # relative for 32-row blocks), so they agree to the FP32 target, 1e-6.
     rel = (torch.linalg.norm(full - parts) / torch.linalg.norm(full)).item()
-    assert rel <= 1e-7
+    assert rel <= 1e-6
 
 
 # ---- precision (FP32 / BF16 storage) ---------------------------------------------------------

@@ -31,13 +31,13 @@ EXPORTS = (
 class KernelConfig(ct.Structure):
     """tsm_kernel_config: zero fields take the tuned default."""
     _fields_ = [("tm", ct.c_int), ("tn", ct.c_int), ("rows", ct.c_int), ("pf", ct.c_int),
-                ("wgs_per_cu", ct.c_int), ("grid", ct.c_int64)]
+                ("wgs_per_cu", ct.c_int), ("grid", ct.c_int64), ("load_policy", ct.c_int)]
 
 
 class KernelDesc(ct.Structure):
     _fields_ = [("name", ct.c_char_p), ("in_type", ct.c_int), ("kc", ct.c_int),
                 ("mc", ct.c_int), ("tm", ct.c_int), ("tn", ct.c_int), ("rows", ct.c_int),
-                ("pf", ct.c_int), ("lds_bytes", ct.c_int)]
+                ("pf", ct.c_int), ("load_policy", ct.c_int), ("lds_bytes", ct.c_int)]
 
 
 class Problem(ct.Structure):
@@ -126,12 +126,14 @@ def kernels() -> list[dict]:
         check(L.tsm_kernel_info(i, ct.byref(d)), "tsm_kernel_info")
         out.append({"index": i, "name": d.name.decode(), "in_type": d.in_type, "kc": d.kc,
                     "mc": d.mc, "tm": d.tm, "tn": d.tn, "rows": d.rows, "pf": d.pf,
-                    "lds_bytes": d.lds_bytes})
+                    "nt": 1 if d.load_policy == 2 else 0, "lds_bytes": d.lds_bytes})
     return out
 
 
-def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0) -> KernelConfig:
-    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid)
+def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0, nt=None) -> KernelConfig:
+    """nt: None = tuned default, 0 = cached loads, 1 = non-temporal loads."""
+    policy = 0 if nt is None else (2 if nt else 1)
+    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy)
 
 
 def plan(in_type: int, n: int, k: int, m: int, config: KernelConfig | None = None):

@@ -1,0 +1,121 @@
+"""Autotuner for the generated C = A^T B kernels (runs on the GPU).
+
+For each tuned shape (dtype, k class, m class) it times every compiled
+variant of that class (register tile, rows per tile, pipeline depth, load
+policy) at several persistent-grid sizes on a problem of `--gib` GiB of A + B,
+checks that the variant's result agrees with the default kernel's, and
+writes the fastest configuration per class to the tile table
+(tuning/gfx950.json).  The generator bakes that table into the library as
+the tuned defaults:
+
+    python -m tsm_benchmark_amd.autotune --out tsm_benchmark_amd/tuning/gfx950.json
+    python -m tsm_benchmark_amd.build
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import torch
+
+from . import _lib, codegen, ops
+
+SEED = 20261203
+DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+DTYPE_ID = {"f64": 0, "f32": 1, "bf16": 2}
+GRID_MULTS = (1, 2, 3, 4, 6, 8)
+
+
+def time_call(fn, reps: int) -> float:
+    """Median seconds per call, HIP events on the current stream."""
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1) * 1e-3)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict:
+    es = codegen.ELEM_BYTES[dtype]
+    n = int(gib * (1 << 30) / (es * (kc + mc)))
+    A = ops.random_matrix(n, kc, DT[dtype], seed=SEED, stream_id=1)
+    B = ops.random_matrix(n, mc, DT[dtype], seed=SEED, stream_id=2)
+    C = torch.empty((kc, mc), dtype=torch.float64, device="cuda")
+    ref = ops.atb(A, B).double()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    nbytes = es * n * (kc + mc)
+    best = None
+    default_t = time_call(lambda: ops.atb(A, B, out=C), reps)
+    for kd in _lib.kernels():
+        if kd["in_type"] != DTYPE_ID[dtype] or kd["kc"] != kc or kd["mc"] != mc:
+            continue
+        for mult in GRID_MULTS:
+            cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"],
+                                   nt=kd["nt"], wgs_per_cu=mult)
+            t = time_call(lambda: ops.atb(A, B, out=C, config=cfg), reps)
+            rel = ((C - ref).norm() / ref.norm()).item()
+            tol = 1e-13 if dtype == "f64" else 1e-6
+            if not rel <= tol:
+                raise RuntimeError(f"{kd['name']} x{mult}: result differs from the default "
+                                   f"kernel by {rel:.3g}")
+            log(f"{kd['name']:40s} wgs/cu {mult}: {t * 1e6:9.1f} us {nbytes / t / 1e9:8.1f} GB/s")
+            if best is None or t < best[0]:
+                best = (t, kd, mult)
+    t, kd, mult = best
+    del A, B
+    torch.cuda.empty_cache()
+    return {"dtype": dtype, "kc": kc, "mc": mc, "tm": kd["tm"], "tn": kd["tn"],
+            "rows": kd["rows"], "pf": kd["pf"], "nt": kd["nt"], "wgs_per_cu": mult,
+            "gbps": round(nbytes / t / 1e9, 1), "default_gbps": round(nbytes / default_t / 1e9, 1),
+            "n": n, "cus": cus}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=codegen.TUNING_JSON)
+    ap.add_argument("--gib", type=float, default=2.0, help="GiB of A + B per problem")
+    ap.add_argument("--reps", type=int, default=7)
+    ap.add_argument("--shapes", nargs="*", default=None,
+                    help="dtype:kcxmc entries (default: the generator's tuned shapes)")
+    args = ap.parse_args(argv)
+    torch.cuda.set_device(0)
+    shapes = []
+    if args.shapes:
+        for s in args.shapes:
+            dt, km = s.split(":")
+            k, m = km.split("x")
+            shapes.append((dt, int(k), int(m)))
+    else:
+        for dt, lst in codegen.TUNE_SHAPES.items():
+            shapes += [(dt, k, m) for k, m in lst]
+    table = codegen.load_tuning(args.out) if os.path.exists(args.out) else {"entries": []}
+    entries = {(e["dtype"], e["kc"], e["mc"]): e for e in table.get("entries", [])}
+    t0 = time.time()
+    for dt, k, m in shapes:
+        print(f"== {dt} {k}x{m}", flush=True)
+        e = tune_shape(dt, k, m, args.gib, args.reps, lambda s: print(s, flush=True))
+        print(f"-> {e}", flush=True)
+        entries[(dt, k, m)] = e
+    out = {"device": torch.cuda.get_device_properties(0).gcnArchName,
+           "generated_by": "python -m tsm_benchmark_amd.autotune",
+           "entries": sorted(entries.values(), key=lambda e: (e["dtype"], e["kc"], e["mc"]))}
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"wrote {args.out} ({len(out['entries'])} entries) in {time.time() - t0:.0f}s", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -20,6 +20,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsm.so")
+BENCH_BIN = os.path.join(HERE, "tsm_bench")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -50,7 +51,8 @@ def _compile(src: str) -> tuple[str, str]:
 def build(jobs: int | None = None, verbose: bool = False) -> str:
     """Compile every kernel and link libtsm.so; returns its path."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sources()
+    # largest translation units first so that the pool stays busy to the end
+    srcs = sorted(sources(), key=lambda p: -os.path.getsize(p))
     jobs = jobs or min(16, os.cpu_count() or 4)
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -61,6 +63,14 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # the C++ benchmark harness, linked against libtsm.so next to it
+    harness = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(REPO, "include"),
+               os.path.join(CSRC, "harness", "tsm_bench.cpp"), "-o", BENCH_BIN + ".tmp",
+               "-L", HERE, "-ltsm", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(harness, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness build failed:\n{r.stderr[-4000:]}")
+    os.replace(BENCH_BIN + ".tmp", BENCH_BIN)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources in {time.time() - t0:.1f}s", file=sys.stderr)
     return LIB

@@ -6,7 +6,7 @@ enumerates kernel variants and emits concrete kernels for them.  Where the
 reference prints SYCL kernels (one per M, N, TM, TN and variant,
 ``sycl_codegen_new.py:142-363``) this prints HIP translation units:
 
-* ``csrc/generated/tsm_kernels_<dtype>_k<KC>.hip`` - one ``__global__`` kernel per
+* ``csrc/generated/tsm_kernels_<dtype>_<KC>x<MC>.hip`` - one ``__global__`` kernel per
   configuration (named ``tsm_atb_<dtype>_<KC>x<MC>_t<TM>x<TN>_r<ROWS>_p<PF>``,
   the name rocprof shows) plus its host launcher;
 * ``csrc/generated/tsm_table.cpp`` - the kernel table the C ABI dispatches
@@ -22,6 +22,7 @@ Configuration axes (all compile-time in the emitted kernels):
   TM, TN   register tile         C entries owned by one thread (``TM``/``TN``)
   ROWS     rows per LDS tile     the n-tile streamed per pipeline stage
   PF       tiles in flight       1 or 2 (the reference's "leap-frog" depth)
+  NT       load cache policy     0 normal, 1 non-temporal (streaming) loads
 
 CLI flags keep the reference's vocabulary where it maps
 (``sycl_codegen_new.py:588-604``): ``--M``, ``--tile-sizes``, ``--stride``,
@@ -50,7 +51,9 @@ CXX_TYPE = {"f64": "double", "f32": "float", "bf16": "tsm::bf16"}
 CLASSES = (1, 2, 4, 8, 16, 32, 64)
 THREADS = 256          # work-group size (4 wave64)
 WAVE = 64              # gfx950 wavefront size
-LDS_BUDGET = 65536     # bytes of LDS per work-group (two fit in 160 KiB)
+LDS_BUDGET = 65536     # LDS per work-group of the defaults (two fit in 160 KiB)
+LDS_MAX = 131072       # LDS per work-group of any variant (one per compute unit)
+MAX_ROWS_PER_THREAD = 16  # the fully unrolled row loop of one tile (compile time)
 
 
 def lcm(a: int, b: int) -> int:
@@ -76,11 +79,12 @@ class KernelConfig:
     tn: int
     rows: int
     pf: int
+    nt: int = 0  # 1: non-temporal (streaming) global loads
 
     @property
     def name(self) -> str:
         return (f"tsm_atb_{self.dtype}_{self.kc}x{self.mc}_t{self.tm}x{self.tn}"
-                f"_r{self.rows}_p{self.pf}")
+                f"_r{self.rows}_p{self.pf}" + ("_nt" if self.nt else ""))
 
     @property
     def ntiles(self) -> int:
@@ -114,14 +118,16 @@ class KernelConfig:
             p.append("more than 64 register tiles")
         if self.pf not in (1, 2):
             p.append("pf must be 1 or 2")
+        if self.nt not in (0, 1):
+            p.append("nt must be 0 or 1")
         if self.rows < 1 or self.rows % self.row_lanes:
             p.append(f"rows must be a multiple of {self.row_lanes} row lanes")
         if (self.rows * self.kc * es) % 16 or (self.rows * self.mc * es) % 16:
             p.append("tiles must be whole 16-byte chunks")
-        if self.lds_bytes > LDS_BUDGET:
+        if self.lds_bytes > LDS_MAX:
             p.append("LDS budget")
-        if self.rows // max(self.row_lanes, 1) > 64:
-            p.append("more than 64 rows per thread per tile")
+        if self.rows // max(self.row_lanes, 1) > MAX_ROWS_PER_THREAD:
+            p.append(f"more than {MAX_ROWS_PER_THREAD} rows per thread per tile")
         return p
 
 
@@ -146,7 +152,7 @@ def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig
     rows = max(rows, rg)
     while (rows * kc * es) % 16 or (rows * mc * es) % 16:
         rows *= 2
-    while rows // rg > 32 and rows > rg:
+    while rows // rg > MAX_ROWS_PER_THREAD and rows > rg:
         rows //= 2
     cfg = KernelConfig(dtype, kc, mc, tm, tn, rows, pf)
     while cfg.lds_bytes > LDS_BUDGET and cfg.rows > rg:
@@ -167,17 +173,19 @@ TUNE_SHAPES = {
 
 
 def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
+    """Variants the autotuner chooses from around the heuristic default: the
+    register tile, rows per tile and pipeline depth, plus non-temporal loads
+    for the default tile."""
     base = default_config(dtype, kc, mc)
     out = set()
     tiles = {(base.tm, base.tn), (min(kc, 4), min(mc, 4)), (min(kc, 8), min(mc, 8))}
-    if dtype != "f64":
-        tiles.add((min(kc, 8), min(mc, 8)))
     for tm, tn in tiles:
-        for rows in (base.rows // 2, base.rows, base.rows * 2, base.rows * 4):
+        for rows in (base.rows // 2, base.rows, base.rows * 2):
             for pf in (1, 2):
-                c = KernelConfig(dtype, kc, mc, tm, tn, rows, pf)
-                if not c.problems():
-                    out.add(c)
+                for nt in ((0, 1) if (tm, tn) == (base.tm, base.tn) else (0,)):
+                    c = KernelConfig(dtype, kc, mc, tm, tn, rows, pf, nt)
+                    if not c.problems():
+                        out.add(c)
     return sorted(out)
 
 
@@ -192,11 +200,12 @@ def reference_variants(Ms, tiles, *, dtype: str = "f64") -> list[KernelConfig]:
             kc = width_class(M)
             base = default_config(dtype, kc, kc)
             for pf in (1, 2):
-                c = KernelConfig(dtype, kc, kc, TM, TM, base.rows, pf)
-                while c.problems() and c.rows < 4096:
-                    c = dataclasses.replace(c, rows=c.rows * 2)
-                if not c.problems():
-                    out.append(c)
+                # the largest valid row tile not above the default's, else the smallest above
+                cands = [KernelConfig(dtype, kc, kc, TM, TM, base.rows >> s, pf) for s in range(8)]
+                cands += [KernelConfig(dtype, kc, kc, TM, TM, base.rows << s, pf) for s in range(1, 6)]
+                ok = [c for c in cands if c.rows >= 1 and not c.problems()]
+                if ok:
+                    out.append(ok[0])
     return out
 
 
@@ -221,7 +230,8 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
             cfgs.update(tuning_variants(dt, kc, mc))
     cfgs.update(reference_variants([4, 8, 16], [2, 4]))
     for e in tuning.get("entries", []):
-        c = KernelConfig(e["dtype"], e["kc"], e["mc"], e["tm"], e["tn"], e["rows"], e["pf"])
+        c = KernelConfig(e["dtype"], e["kc"], e["mc"], e["tm"], e["tn"], e["rows"], e["pf"],
+                         int(e.get("nt", 0)))
         if c.problems():
             raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
         cfgs.add(c)
@@ -238,9 +248,9 @@ BANNER = ("// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.\n"
 
 def emit_kernel(c: KernelConfig) -> str:
     t = CXX_TYPE[c.dtype]
-    targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}"
+    targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}, {c.nt}"
     return (
-        f"// {c.name}: KC={c.kc} MC={c.mc} TM={c.tm} TN={c.tn} rows={c.rows} pf={c.pf}\n"
+        f"// {c.name}: KC={c.kc} MC={c.mc} TM={c.tm} TN={c.tn} rows={c.rows} pf={c.pf} nt={c.nt}\n"
         f"//   {c.ntiles} register tiles x {c.row_lanes} row lanes, "
         f"{c.rows // c.row_lanes} rows/thread/tile, LDS {c.lds_bytes} B\n"
         f"extern \"C\" __global__ void __launch_bounds__(256) {c.name}(tsm::AtbArgs a) {{\n"
@@ -252,10 +262,10 @@ def emit_kernel(c: KernelConfig) -> str:
         f"}}\n")
 
 
-def emit_kernel_file(dtype: str, kc: int, cfgs: list[KernelConfig]) -> str:
+def emit_kernel_file(dtype: str, kc: int, mc: int, cfgs: list[KernelConfig]) -> str:
     body = [BANNER, '#include "../tsm_internal.hpp"\n']
     for c in cfgs:
-        if c.dtype == dtype and c.kc == kc:
+        if c.dtype == dtype and c.kc == kc and c.mc == mc:
             body.append(emit_kernel(c))
     return "\n".join(body)
 
@@ -268,7 +278,7 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
     out.append("\nstatic const tsm::KernelEntry kTable[] = {")
     for c in cfgs:
         out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
-                   f"{c.rows}, {c.pf}, {c.lds_bytes}, launch_{c.name}}},")
+                   f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
     out.append("};\n")
     out.append("static const tsm::TunedEntry kTuned[] = {")
     for (dt, kc, mc), (c, wpc) in sorted(defaults.items()):
@@ -298,9 +308,14 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
     paths = []
     for dt in DTYPES:
         for kc in CLASSES:
-            p = os.path.join(out_dir, f"tsm_kernels_{dt}_k{kc}.hip")
-            _write_if_changed(p, emit_kernel_file(dt, kc, cfgs))
-            paths.append(p)
+            for mc in CLASSES:
+                p = os.path.join(out_dir, f"tsm_kernels_{dt}_{kc}x{mc}.hip")
+                _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs))
+                paths.append(p)
+    keep = {os.path.basename(x) for x in paths} | {"tsm_table.hip"}
+    for f in os.listdir(out_dir):  # drop files of an older split
+        if f.endswith(".hip") and f not in keep:
+            os.remove(os.path.join(out_dir, f))
     p = os.path.join(out_dir, "tsm_table.hip")
     _write_if_changed(p, emit_table(cfgs, defaults))
     paths.append(p)
@@ -324,7 +339,7 @@ def emit_name_layer(Ms, tiles, stride, unroll, with_benchmark: bool) -> str:
                 continue
             for suffix, pf in (("", 1), ("_lf", 2), ("_sg", 1), ("_sg_lf", 2)):
                 name = f"tsmttsm_{M}_{M}_{TM}_{TM}{suffix}"
-                names.append(name)
+                names.append((name, M))
                 lines += [
                     "template <typename T>",
                     f"inline tsm_status {name}(tsm_stream_t q, int64_t K, const T* A, const T* B, T* C) {{",
@@ -332,10 +347,17 @@ def emit_name_layer(Ms, tiles, stride, unroll, with_benchmark: bool) -> str:
                     f"{stride}, {unroll});",
                     "}", ""]
     if with_benchmark:
-        lines += ["// Every generated name, in generation order (the generated benchmark's list,",
-                  "// sycl_codegen_new.py:478-487).",
-                  "inline const char* const kGeneratedNames[] = {"]
-        lines += [f"  \"{n}\"," for n in names]
+        lines += ["// Every generated name with its FP64 instance, in generation order (the",
+                  "// generated benchmark's list, sycl_codegen_new.py:478-487).",
+                  "struct GeneratedKernel {",
+                  "  const char* name;",
+                  "  int M;",
+                  "  tsm_status (*fn)(tsm_stream_t, int64_t, const double*, const double*, double*);",
+                  "};",
+                  "inline const GeneratedKernel kGeneratedKernels[] = {"]
+        lines += [f"  {{\"{n}\", {M}, &{n}<double>}}," for n, M in names]
+        lines += ["};", "inline const char* const kGeneratedNames[] = {"]
+        lines += [f"  \"{n}\"," for n, _ in names]
         lines += ["};", ""]
     lines += ["}  // namespace tsm_compat", ""]
     return "\n".join(lines)

@@ -1,0 +1,164 @@
+// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.
+// Regenerate with: python -m tsm_benchmark_amd.codegen --library
+
+#include "../tsm_internal.hpp"
+
+// tsm_atb_bf16_32x32_t4x4_r64_p1: KC=32 MC=32 TM=4 TN=4 rows=64 pf=1 nt=0
+//   64 register tiles x 4 row lanes, 16 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t4x4_r64_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 4, 4, 64, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t4x4_r64_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t4x4_r64_p2: KC=32 MC=32 TM=4 TN=4 rows=64 pf=2 nt=0
+//   64 register tiles x 4 row lanes, 16 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t4x4_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 4, 4, 64, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t4x4_r64_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r64_p1: KC=32 MC=32 TM=8 TN=4 rows=64 pf=1 nt=0
+//   32 register tiles x 8 row lanes, 8 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r64_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 64, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r64_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r64_p1_nt: KC=32 MC=32 TM=8 TN=4 rows=64 pf=1 nt=1
+//   32 register tiles x 8 row lanes, 8 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r64_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 64, 1, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r64_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r64_p2: KC=32 MC=32 TM=8 TN=4 rows=64 pf=2 nt=0
+//   32 register tiles x 8 row lanes, 8 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 64, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r64_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r64_p2_nt: KC=32 MC=32 TM=8 TN=4 rows=64 pf=2 nt=1
+//   32 register tiles x 8 row lanes, 8 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 64, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r128_p1: KC=32 MC=32 TM=8 TN=4 rows=128 pf=1 nt=0
+//   32 register tiles x 8 row lanes, 16 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r128_p1_nt: KC=32 MC=32 TM=8 TN=4 rows=128 pf=1 nt=1
+//   32 register tiles x 8 row lanes, 16 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r128_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 128, 1, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r128_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r128_p2: KC=32 MC=32 TM=8 TN=4 rows=128 pf=2 nt=0
+//   32 register tiles x 8 row lanes, 16 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x4_r128_p2_nt: KC=32 MC=32 TM=8 TN=4 rows=128 pf=2 nt=1
+//   32 register tiles x 8 row lanes, 16 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r64_p1: KC=32 MC=32 TM=8 TN=8 rows=64 pf=1 nt=0
+//   16 register tiles x 16 row lanes, 4 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r64_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 64, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r64_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r64_p2: KC=32 MC=32 TM=8 TN=8 rows=64 pf=2 nt=0
+//   16 register tiles x 16 row lanes, 4 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 64, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r64_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r128_p1: KC=32 MC=32 TM=8 TN=8 rows=128 pf=1 nt=0
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r128_p2: KC=32 MC=32 TM=8 TN=8 rows=128 pf=2 nt=0
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r256_p1: KC=32 MC=32 TM=8 TN=8 rows=256 pf=1 nt=0
+//   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_32x32_t8x8_r256_p2: KC=32 MC=32 TM=8 TN=8 rows=256 pf=2 nt=0
+//   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_32x32_t8x8_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 32, 32, 8, 8, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_32x32_t8x8_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}

@@ -73,18 +73,19 @@ struct Plan {
 
 tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_config* cfg,
                        int* index, int* wgs_per_cu) {
-  int nt = 0, nk = 0;
-  const tsm::TunedEntry* tuned = tsm_gen_tuned_table(&nt);
+  int ntuned = 0, nk = 0;
+  const tsm::TunedEntry* tuned = tsm_gen_tuned_table(&ntuned);
   const tsm::KernelEntry* ks = table(&nk);
   int def = -1, def_wpc = 2;
-  for (int i = 0; i < nt; ++i)
+  for (int i = 0; i < ntuned; ++i)
     if (tuned[i].dtype == (int)in_type && tuned[i].kc == kc && tuned[i].mc == mc) {
       def = tuned[i].kernel;
       def_wpc = tuned[i].wgs_per_cu;
     }
   if (def < 0) return TSM_ERR_NO_KERNEL;
   *wgs_per_cu = (cfg && cfg->wgs_per_cu > 0) ? cfg->wgs_per_cu : def_wpc;
-  if (!cfg || (cfg->tm == 0 && cfg->tn == 0 && cfg->rows == 0 && cfg->pf == 0)) {
+  if (!cfg || (cfg->tm == 0 && cfg->tn == 0 && cfg->rows == 0 && cfg->pf == 0 &&
+               cfg->load_policy == 0)) {
     *index = def;
     return TSM_SUCCESS;
   }
@@ -93,11 +94,12 @@ tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_confi
   const tsm::KernelEntry& d = ks[def];
   const int tm = cfg->tm ? cfg->tm : d.tm, tn = cfg->tn ? cfg->tn : d.tn;
   const int pf = cfg->pf ? cfg->pf : d.pf;
+  const int nt = cfg->load_policy ? (cfg->load_policy == 2 ? 1 : 0) : d.nt;
   int best = -1;
   for (int i = 0; i < nk; ++i) {
     const tsm::KernelEntry& e = ks[i];
     if (e.dtype != (int)in_type || e.kc != kc || e.mc != mc || e.tm != tm || e.tn != tn ||
-        e.pf != pf)
+        e.pf != pf || e.nt != nt)
       continue;
     if (cfg->rows ? e.rows == cfg->rows : (best < 0 || e.rows == d.rows)) best = i;
   }
@@ -183,7 +185,7 @@ tsm_status pooled_workspace(size_t bytes, hipStream_t stream, void** out) {
     if (e != hipSuccess) return hip_fail(e);
     e = hipMemsetAsync(p, 0, kCounterBytes > want ? want : kCounterBytes, stream);
     if (e != hipSuccess) {
-      hipFree(p);
+      (void)hipFree(p);
       return hip_fail(e);
     }
     slot = {p, want};
@@ -276,7 +278,8 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   tsm_status st = check_atb(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc);
   if (st != TSM_SUCCESS) return st;
   if (config && (config->pf < 0 || config->pf > 2 || config->tm < 0 || config->tn < 0 ||
-                 config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0))
+                 config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0 ||
+                 config->load_policy < 0 || config->load_policy > 2))
     return TSM_ERR_INVALID_ARGUMENT;
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);
@@ -388,6 +391,7 @@ tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out) {
   out->tn = e.tn;
   out->rows = e.rows;
   out->pf = e.pf;
+  out->load_policy = e.nt ? 2 : 1;
   out->lds_bytes = e.lds_bytes;
   return TSM_SUCCESS;
 }

@@ -120,6 +120,14 @@ __device__ __forceinline__ void lds_read(const S* p, typename Traits<S>::Acc* ou
   }
 }
 
+// 16-byte global load of streamed data; NT = 1 marks it non-temporal (the
+// line is not kept in the caches for reuse: A and B are read exactly once).
+template <int NT>
+__device__ __forceinline__ v4u stream_load(const v4u* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
 // TwoSum (Knuth): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
   s = a + b;
@@ -265,7 +273,7 @@ __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
 }
 
 // ---- the streaming kernel body -----------------------------------------------------
-template <class S, int KC, int MC, int TM, int TN, int T, int PF>
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT = 0>
 __device__ __forceinline__ void atb_body(const AtbArgs& a) {
   using Acc = typename Traits<S>::Acc;
   constexpr bool kComp = Traits<S>::compensated;
@@ -372,12 +380,12 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
 #pragma unroll
       for (int v = 0; v < A_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (A_VEC % kThreads == 0 || idx < A_VEC) r.a[v] = ta[idx];
+        if (A_VEC % kThreads == 0 || idx < A_VEC) r.a[v] = stream_load<NT>(ta + idx);
       }
 #pragma unroll
       for (int v = 0; v < B_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (B_VEC % kThreads == 0 || idx < B_VEC) r.b[v] = tb[idx];
+        if (B_VEC % kThreads == 0 || idx < B_VEC) r.b[v] = stream_load<NT>(tb + idx);
       }
     };
     auto store = [&](const Regs& r, unsigned char* stage) __attribute__((always_inline)) {

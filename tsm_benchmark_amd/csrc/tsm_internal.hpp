@@ -15,6 +15,7 @@ struct KernelEntry {
   const char* name;
   int dtype;  // tsm_dtype
   int kc, mc, tm, tn, rows, pf;
+  int nt;  // 1: non-temporal loads
   int lds_bytes;
   LaunchFn launch;
 };
