@@ -69,6 +69,8 @@ typedef struct {
   int wgs_per_cu;      /* persistent workgroups per compute unit */
   int64_t grid;        /* total workgroups; overrides wgs_per_cu when > 0 (the "stride" role) */
   int load_policy;     /* 0 tuned default, 1 cached loads, 2 non-temporal (streaming) loads */
+  int tile_order;      /* 0 tuned default, 1 contiguous runs of tiles per workgroup,
+                          2 interleaved (tile t to workgroup t mod grid) */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------

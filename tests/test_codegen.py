@@ -59,7 +59,7 @@ def test_library_kernel_set():
 def test_tuning_entry_overrides_default():
     e = {"dtype": "f64", "kc": 8, "mc": 8, "tm": 4, "tn": 4, "rows": 128, "pf": 1, "wgs_per_cu": 3}
     cfgs, defaults = codegen.library_kernels({"entries": [e]})
-    c, wpc = defaults[("f64", 8, 8)]
+    c, wpc, _ = defaults[("f64", 8, 8)]
     assert (c.tm, c.tn, c.rows, c.pf, wpc) == (4, 4, 128, 1, 3)
     with pytest.raises(ValueError):
         codegen.library_kernels({"entries": [dict(e, tm=3)]})

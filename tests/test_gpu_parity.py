@@ -114,6 +114,18 @@ def test_shapes_and_ragged_rows(dt, km):
         check_close(C, host(A), host(B), DT[dt])
 
 
+@pytest.mark.parametrize("il", [0, 1])
+@pytest.mark.parametrize("grid", [1, 3, 1000])
+def test_tile_orders_and_grids(il, grid):
+    for n in (1, 255, 256, 257, 5000, 65536 + 300):
+        A, B = rand(n, 8, torch.float64, 1), rand(n, 8, torch.float64, 2)
+        C = ops.atb(A, B, config=_lib.make_config(grid=grid, interleave=il)).cpu().numpy()
+        check_close(C, host(A), host(B), torch.float64)
+        S = A[:, :5]  # strided (element-load path) with the same tile orders
+        C = ops.atb(S, B, config=_lib.make_config(grid=grid, interleave=il)).cpu().numpy()
+        check_close(C, host(S), host(B), torch.float64)
+
+
 @pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
 def test_strided_views_and_misaligned_bases(dt):
     n = 50021
@@ -159,11 +171,12 @@ def test_every_generated_variant(dt, k, m):
     for kd in _lib.kernels():
         if kd["in_type"] != dtype_id or kd["kc"] != k or kd["mc"] != m:
             continue
-        cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"], nt=kd["nt"])
         for grid in (0, 7):
-            cfg.grid = grid
-            C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
-            check_close(C, Ah, Bh, DT[dt])
+            for il in (0, 1):
+                cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"],
+                                       nt=kd["nt"], grid=grid, interleave=il)
+                C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
+                check_close(C, Ah, Bh, DT[dt])
         seen += 1
     assert seen >= 1
 

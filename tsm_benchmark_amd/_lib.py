@@ -31,7 +31,8 @@ EXPORTS = (
 class KernelConfig(ct.Structure):
     """tsm_kernel_config: zero fields take the tuned default."""
     _fields_ = [("tm", ct.c_int), ("tn", ct.c_int), ("rows", ct.c_int), ("pf", ct.c_int),
-                ("wgs_per_cu", ct.c_int), ("grid", ct.c_int64), ("load_policy", ct.c_int)]
+                ("wgs_per_cu", ct.c_int), ("grid", ct.c_int64), ("load_policy", ct.c_int),
+                ("tile_order", ct.c_int)]
 
 
 class KernelDesc(ct.Structure):
@@ -130,10 +131,13 @@ def kernels() -> list[dict]:
     return out
 
 
-def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0, nt=None) -> KernelConfig:
-    """nt: None = tuned default, 0 = cached loads, 1 = non-temporal loads."""
+def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0, nt=None,
+                interleave=None) -> KernelConfig:
+    """nt: None = tuned default, 0 = cached loads, 1 = non-temporal loads;
+    interleave: None = tuned default, 0 = contiguous tile runs, 1 = interleaved."""
     policy = 0 if nt is None else (2 if nt else 1)
-    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy)
+    order = 0 if interleave is None else (2 if interleave else 1)
+    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy, order)
 
 
 def plan(in_type: int, n: int, k: int, m: int, config: KernelConfig | None = None):

@@ -60,23 +60,25 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
     for kd in _lib.kernels():
         if kd["in_type"] != DTYPE_ID[dtype] or kd["kc"] != kc or kd["mc"] != mc:
             continue
-        for mult in GRID_MULTS:
+        for mult, il in [(mu, il) for mu in GRID_MULTS for il in (0, 1)]:
             cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"],
-                                   nt=kd["nt"], wgs_per_cu=mult)
+                                   nt=kd["nt"], wgs_per_cu=mult, interleave=il)
             t = time_call(lambda: ops.atb(A, B, out=C, config=cfg), reps)
             rel = ((C - ref).norm() / ref.norm()).item()
             tol = 1e-13 if dtype == "f64" else 1e-6
             if not rel <= tol:
                 raise RuntimeError(f"{kd['name']} x{mult}: result differs from the default "
                                    f"kernel by {rel:.3g}")
-            log(f"{kd['name']:40s} wgs/cu {mult}: {t * 1e6:9.1f} us {nbytes / t / 1e9:8.1f} GB/s")
+            log(f"{kd['name']:40s} wgs/cu {mult} il {il}: {t * 1e6:9.1f} us "
+                f"{nbytes / t / 1e9:8.1f} GB/s")
             if best is None or t < best[0]:
-                best = (t, kd, mult)
-    t, kd, mult = best
+                best = (t, kd, mult, il)
+    t, kd, mult, il = best
     del A, B
     torch.cuda.empty_cache()
     return {"dtype": dtype, "kc": kc, "mc": mc, "tm": kd["tm"], "tn": kd["tn"],
             "rows": kd["rows"], "pf": kd["pf"], "nt": kd["nt"], "wgs_per_cu": mult,
+            "interleave": il,
             "gbps": round(nbytes / t / 1e9, 1), "default_gbps": round(nbytes / default_t / 1e9, 1),
             "n": n, "cus": cus}
 

@@ -219,13 +219,13 @@ def load_tuning(path: str = TUNING_JSON) -> dict:
 def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
     """The full kernel set of the library and the tuned default per class."""
     cfgs: set[KernelConfig] = set()
-    defaults: dict[tuple[str, int, int], tuple[KernelConfig, int]] = {}
+    defaults: dict[tuple[str, int, int], tuple[KernelConfig, int, int]] = {}
     for dt in DTYPES:
         for kc in CLASSES:
             for mc in CLASSES:
                 c = default_config(dt, kc, mc)
                 cfgs.add(c)
-                defaults[(dt, kc, mc)] = (c, 2)
+                defaults[(dt, kc, mc)] = (c, 2, 0)
         for kc, mc in TUNE_SHAPES[dt]:
             cfgs.update(tuning_variants(dt, kc, mc))
     cfgs.update(reference_variants([4, 8, 16], [2, 4]))
@@ -235,7 +235,8 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
         if c.problems():
             raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
         cfgs.add(c)
-        defaults[(c.dtype, c.kc, c.mc)] = (c, int(e.get("wgs_per_cu", 2)))
+        defaults[(c.dtype, c.kc, c.mc)] = (c, int(e.get("wgs_per_cu", 2)),
+                                           int(e.get("interleave", 0)))
     return sorted(cfgs), defaults
 
 
@@ -281,8 +282,8 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
                    f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
     out.append("};\n")
     out.append("static const tsm::TunedEntry kTuned[] = {")
-    for (dt, kc, mc), (c, wpc) in sorted(defaults.items()):
-        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}}},  // {c.name}")
+    for (dt, kc, mc), (c, wpc, il) in sorted(defaults.items()):
+        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}, {il}}},  // {c.name}")
     out.append("};\n")
     out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {\n"
                "  *count = (int)(sizeof(kTable) / sizeof(kTable[0]));\n  return kTable;\n}")

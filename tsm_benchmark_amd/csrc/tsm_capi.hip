@@ -65,25 +65,28 @@ struct Plan {
   int kernel = -1;
   const tsm::KernelEntry* entry = nullptr;
   int64_t grid = 1;
-  int64_t rows_per_wg = 0;
+  int64_t tiles_per_wg = 1;
+  int interleave = 0;
   int kcmc = 0;
   bool finish_in_kernel = true;
   size_t workspace = 0;
 };
 
 tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_config* cfg,
-                       int* index, int* wgs_per_cu) {
+                       int* index, int* wgs_per_cu, int* interleave) {
   int ntuned = 0, nk = 0;
   const tsm::TunedEntry* tuned = tsm_gen_tuned_table(&ntuned);
   const tsm::KernelEntry* ks = table(&nk);
-  int def = -1, def_wpc = 2;
+  int def = -1, def_wpc = 2, def_il = 0;
   for (int i = 0; i < ntuned; ++i)
     if (tuned[i].dtype == (int)in_type && tuned[i].kc == kc && tuned[i].mc == mc) {
       def = tuned[i].kernel;
       def_wpc = tuned[i].wgs_per_cu;
+      def_il = tuned[i].interleave;
     }
   if (def < 0) return TSM_ERR_NO_KERNEL;
   *wgs_per_cu = (cfg && cfg->wgs_per_cu > 0) ? cfg->wgs_per_cu : def_wpc;
+  *interleave = (cfg && cfg->tile_order > 0) ? (cfg->tile_order == 2 ? 1 : 0) : def_il;
   if (!cfg || (cfg->tm == 0 && cfg->tn == 0 && cfg->rows == 0 && cfg->pf == 0 &&
                cfg->load_policy == 0)) {
     *index = def;
@@ -111,8 +114,8 @@ tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_confi
 tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kernel_config* cfg,
                      Plan* p) {
   const int kc = width_class(k), mc = width_class(m);
-  int wpc = 2;
-  tsm_status st = find_kernel(in_type, kc, mc, cfg, &p->kernel, &wpc);
+  int wpc = 2, interleave = 0;
+  tsm_status st = find_kernel(in_type, kc, mc, cfg, &p->kernel, &wpc, &interleave);
   if (st != TSM_SUCCESS) return st;
   int nk = 0;
   p->entry = &table(&nk)[p->kernel];
@@ -129,9 +132,14 @@ tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kerne
   const int64_t tiles = n > 0 ? (n + T - 1) / T : 1;
   if (want > tiles) want = tiles;
   if (want < 1) want = 1;
-  const int64_t tiles_per_wg = (tiles + want - 1) / want;
-  p->rows_per_wg = tiles_per_wg * T;
-  p->grid = n > 0 ? (n + p->rows_per_wg - 1) / p->rows_per_wg : 1;
+  p->interleave = interleave;
+  if (interleave) {
+    p->tiles_per_wg = (tiles + want - 1) / want;
+    p->grid = want;
+  } else {
+    p->tiles_per_wg = (tiles + want - 1) / want;
+    p->grid = (tiles + p->tiles_per_wg - 1) / p->tiles_per_wg;
+  }
   p->kcmc = kc * mc;
   p->finish_in_kernel = p->kcmc <= 1024;
   p->workspace = kCounterBytes + (size_t)p->grid * p->kcmc * sizeof(double);
@@ -279,7 +287,8 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   if (st != TSM_SUCCESS) return st;
   if (config && (config->pf < 0 || config->pf > 2 || config->tm < 0 || config->tn < 0 ||
                  config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0 ||
-                 config->load_policy < 0 || config->load_policy > 2))
+                 config->load_policy < 0 || config->load_policy > 2 ||
+                 config->tile_order < 0 || config->tile_order > 2))
     return TSM_ERR_INVALID_ARGUMENT;
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);
@@ -302,7 +311,8 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   a.lda = lda;
   a.ldb = ldb;
   a.ldc = ldc;
-  a.rows_per_wg = p.rows_per_wg;
+  a.tiles_per_wg = p.tiles_per_wg;
+  a.interleave = p.interleave;
   a.k = k;
   a.m = m;
   a.accumulate = accumulate ? 1 : 0;

@@ -143,7 +143,8 @@ struct AtbArgs {
   double* partials;       // [gridDim.x][KC*MC] FP64 workgroup partials
   unsigned int* counter;  // arrival counter, zero between calls
   int64_t n, lda, ldb, ldc;
-  int64_t rows_per_wg;    // rows streamed by one workgroup (a multiple of T)
+  int64_t tiles_per_wg;   // contiguous mode: consecutive tiles per workgroup
+  int interleave;         // 1: tile t goes to workgroup t % gridDim.x
   int k, m;
   int accumulate;         // C += result (1) or C = result (0)
   int c_f32;              // C is binary32 (1) or binary64 (0)
@@ -360,23 +361,40 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
     }
   };
 
-  const int64_t row_begin = (int64_t)blockIdx.x * a.rows_per_wg;
-  int64_t row_end = row_begin + a.rows_per_wg;
-  if (row_end > a.n) row_end = a.n;
-  const int64_t nrows = row_end > row_begin ? row_end - row_begin : 0;
-  const int64_t nfull = a.dense ? nrows / T : 0;
+  // Tiles of this workgroup: the full tiles first + j * step (j < count) and,
+  // for one workgroup, the partial tile at the end of the rows.  Contiguous
+  // mode gives each workgroup a run of tiles_per_wg consecutive tiles;
+  // interleaved mode deals tile t to workgroup t % G, so that the workgroups
+  // read one moving window of memory together.
+  const int64_t full = a.n / T;
+  const int64_t G = gridDim.x, g = blockIdx.x;
+  int64_t first, step, count;
+  if (a.interleave) {
+    first = g;
+    step = G;
+    count = first < full ? (full - first + G - 1) / G : 0;
+  } else {
+    first = g * a.tiles_per_wg;
+    step = 1;
+    count = first < full ? (full - first < a.tiles_per_wg ? full - first : a.tiles_per_wg) : 0;
+  }
+  const int tail_rows = (int)(a.n - full * T);
+  const bool owns_tail =
+      tail_rows > 0 && (a.interleave ? full % G == g : full / a.tiles_per_wg == g);
+  const int64_t nfull = a.dense ? count : 0;
 
   // ---- pipelined dense tiles ---------------------------------------------------
   if (nfull > 0) {
-    const v4u* gA = reinterpret_cast<const v4u*>(static_cast<const S*>(a.A) + row_begin * KC);
-    const v4u* gB = reinterpret_cast<const v4u*>(static_cast<const S*>(a.B) + row_begin * MC);
+    const v4u* gA = static_cast<const v4u*>(a.A);
+    const v4u* gB = static_cast<const v4u*>(a.B);
     struct Regs {
       v4u a[A_PER];
       v4u b[B_PER];
     };
     auto load = [&](Regs& r, int64_t t) __attribute__((always_inline)) {
-      const v4u* ta = gA + t * A_VEC;
-      const v4u* tb = gB + t * B_VEC;
+      const int64_t tile = first + t * step;
+      const v4u* ta = gA + tile * A_VEC;
+      const v4u* tb = gB + tile * B_VEC;
 #pragma unroll
       for (int v = 0; v < A_PER; ++v) {
         const int idx = v * kThreads + tid;
@@ -437,32 +455,32 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
     }
   }
 
-  // ---- remaining rows: element loads, zero padding -------------------------------
-  {
+  // ---- element loads (strided or misaligned inputs, the partial tile) -------------
+  auto elem_tile = [&](int64_t r0, int nr) __attribute__((always_inline)) {
     const S* gA = static_cast<const S*>(a.A);
     const S* gB = static_cast<const S*>(a.B);
-    for (int64_t r0 = row_begin + nfull * T; r0 < row_end; r0 += T) {
-      const int nr = (int)((row_end - r0) < T ? (row_end - r0) : T);
-      __syncthreads();
-      S* la = reinterpret_cast<S*>(smem);
-      S* lb = reinterpret_cast<S*>(smem + A_BYTES);
-      for (int e = tid; e < T * KC; e += kThreads) {
-        const int r = e / KC, c = e % KC;
-        S v{};
-        if (r < nr && c < a.k) v = gA[(r0 + r) * a.lda + c];
-        la[e] = v;
-      }
-      for (int e = tid; e < T * MC; e += kThreads) {
-        const int r = e / MC, c = e % MC;
-        S v{};
-        if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
-        lb[e] = v;
-      }
-      __syncthreads();
-      compute(smem);
-      after_tile();
+    __syncthreads();
+    S* la = reinterpret_cast<S*>(smem);
+    S* lb = reinterpret_cast<S*>(smem + A_BYTES);
+    for (int e = tid; e < T * KC; e += kThreads) {
+      const int r = e / KC, c = e % KC;
+      S v{};
+      if (r < nr && c < a.k) v = gA[(r0 + r) * a.lda + c];
+      la[e] = v;
     }
-  }
+    for (int e = tid; e < T * MC; e += kThreads) {
+      const int r = e / MC, c = e % MC;
+      S v{};
+      if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
+      lb[e] = v;
+    }
+    __syncthreads();
+    compute(smem);
+    after_tile();
+  };
+  if (!a.dense)
+    for (int64_t j = 0; j < count; ++j) elem_tile((first + j * step) * T, T);
+  if (owns_tail) elem_tile(full * T, tail_rows);
 
   // ---- reduction -------------------------------------------------------------------
   constexpr int V = TM * TN;

@@ -24,6 +24,7 @@ struct TunedEntry {
   int dtype, kc, mc;
   int kernel;      // index into kernel_table()
   int wgs_per_cu;  // persistent workgroups per compute unit
+  int interleave;  // 1: tiles dealt round-robin to workgroups
 };
 
 }  // namespace tsm
