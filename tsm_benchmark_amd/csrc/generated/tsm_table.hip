@@ -788,7 +788,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 8, 1, 21, 2, 0},  // tsm_atb_bf16_8x1_t8x1_r1024_p2
   {2, 8, 2, 22, 2, 0},  // tsm_atb_bf16_8x2_t8x2_r1024_p2
   {2, 8, 4, 23, 2, 0},  // tsm_atb_bf16_8x4_t8x4_r1024_p2
-  {2, 8, 8, 35, 1, 0},  // tsm_atb_bf16_8x8_t8x4_r1024_p2_nt
+  {2, 8, 8, 43, 1, 1},  // tsm_atb_bf16_8x8_t8x8_r1024_p2
   {2, 8, 16, 46, 2, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2
   {2, 8, 32, 47, 2, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2
   {2, 8, 64, 48, 2, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2
@@ -804,7 +804,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 32, 4, 73, 2, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
   {2, 32, 8, 74, 2, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
   {2, 32, 16, 75, 2, 0},  // tsm_atb_bf16_32x16_t8x4_r256_p2
-  {2, 32, 32, 83, 2, 0},  // tsm_atb_bf16_32x32_t8x4_r128_p1_nt
+  {2, 32, 32, 88, 2, 1},  // tsm_atb_bf16_32x32_t8x8_r128_p1
   {2, 32, 64, 92, 2, 0},  // tsm_atb_bf16_32x64_t8x4_r64_p2
   {2, 64, 1, 93, 2, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
   {2, 64, 2, 94, 2, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
@@ -837,7 +837,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 8, 1, 121, 2, 0},  // tsm_atb_f32_8x1_t8x1_r512_p2
   {1, 8, 2, 122, 2, 0},  // tsm_atb_f32_8x2_t8x2_r512_p2
   {1, 8, 4, 123, 2, 0},  // tsm_atb_f32_8x4_t8x4_r512_p2
-  {1, 8, 8, 141, 1, 0},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
+  {1, 8, 8, 141, 1, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
   {1, 8, 16, 148, 2, 0},  // tsm_atb_f32_8x16_t8x4_r256_p2
   {1, 8, 32, 149, 2, 0},  // tsm_atb_f32_8x32_t8x4_r128_p2
   {1, 8, 64, 150, 2, 0},  // tsm_atb_f32_8x64_t8x4_r64_p2
@@ -853,7 +853,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 32, 4, 181, 2, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
   {1, 32, 8, 182, 2, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
   {1, 32, 16, 183, 2, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
-  {1, 32, 32, 189, 2, 0},  // tsm_atb_f32_32x32_t8x4_r64_p2_nt
+  {1, 32, 32, 191, 2, 1},  // tsm_atb_f32_32x32_t8x4_r128_p1_nt
   {1, 32, 64, 200, 2, 0},  // tsm_atb_f32_32x64_t8x4_r64_p2
   {1, 64, 1, 201, 2, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
   {1, 64, 2, 202, 2, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
@@ -862,7 +862,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 64, 16, 205, 2, 0},  // tsm_atb_f32_64x16_t8x4_r64_p2
   {1, 64, 32, 206, 2, 0},  // tsm_atb_f32_64x32_t8x4_r64_p2
   {1, 64, 64, 207, 2, 0},  // tsm_atb_f32_64x64_t8x8_r64_p2
-  {0, 1, 1, 211, 1, 0},  // tsm_atb_f64_1x1_t1x1_r1024_p2_nt
+  {0, 1, 1, 211, 1, 1},  // tsm_atb_f64_1x1_t1x1_r1024_p2_nt
   {0, 1, 2, 220, 2, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
   {0, 1, 4, 221, 2, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
   {0, 1, 8, 222, 2, 0},  // tsm_atb_f64_1x8_t1x8_r256_p2
@@ -870,7 +870,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 1, 32, 224, 2, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
   {0, 1, 64, 225, 2, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
   {0, 2, 1, 226, 2, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
-  {0, 2, 2, 230, 1, 0},  // tsm_atb_f64_2x2_t2x2_r512_p2_nt
+  {0, 2, 2, 236, 1, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
   {0, 2, 4, 239, 2, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
   {0, 2, 8, 240, 2, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
   {0, 2, 16, 241, 2, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
@@ -878,15 +878,15 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 2, 64, 243, 2, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
   {0, 4, 1, 244, 2, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
   {0, 4, 2, 245, 2, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
-  {0, 4, 4, 255, 1, 0},  // tsm_atb_f64_4x4_t4x4_r512_p2_nt
+  {0, 4, 4, 255, 1, 1},  // tsm_atb_f64_4x4_t4x4_r512_p2_nt
   {0, 4, 8, 260, 2, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
   {0, 4, 16, 261, 2, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
-  {0, 4, 32, 271, 1, 0},  // tsm_atb_f64_4x32_t4x8_r64_p2_nt
+  {0, 4, 32, 271, 1, 1},  // tsm_atb_f64_4x32_t4x8_r64_p2_nt
   {0, 4, 64, 276, 2, 0},  // tsm_atb_f64_4x64_t4x8_r32_p2
   {0, 8, 1, 277, 2, 0},  // tsm_atb_f64_8x1_t8x1_r256_p2
   {0, 8, 2, 278, 2, 0},  // tsm_atb_f64_8x2_t8x2_r256_p2
   {0, 8, 4, 279, 2, 0},  // tsm_atb_f64_8x4_t8x4_r256_p2
-  {0, 8, 8, 291, 1, 0},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
+  {0, 8, 8, 291, 1, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
   {0, 8, 16, 296, 2, 0},  // tsm_atb_f64_8x16_t8x8_r128_p2
   {0, 8, 32, 297, 2, 0},  // tsm_atb_f64_8x32_t8x8_r64_p2
   {0, 8, 64, 298, 2, 0},  // tsm_atb_f64_8x64_t8x8_r32_p2
@@ -894,15 +894,15 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 16, 2, 300, 2, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
   {0, 16, 4, 301, 2, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
   {0, 16, 8, 302, 2, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
-  {0, 16, 16, 314, 1, 0},  // tsm_atb_f64_16x16_t8x8_r64_p2_nt
+  {0, 16, 16, 318, 1, 1},  // tsm_atb_f64_16x16_t8x8_r128_p2_nt
   {0, 16, 32, 323, 2, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
   {0, 16, 64, 324, 2, 0},  // tsm_atb_f64_16x64_t8x8_r32_p2
   {0, 32, 1, 325, 2, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
   {0, 32, 2, 326, 2, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
-  {0, 32, 4, 340, 1, 0},  // tsm_atb_f64_32x4_t8x4_r128_p2_nt
+  {0, 32, 4, 340, 1, 1},  // tsm_atb_f64_32x4_t8x4_r128_p2_nt
   {0, 32, 8, 341, 2, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
   {0, 32, 16, 342, 2, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 351, 1, 0},  // tsm_atb_f64_32x32_t8x8_r64_p1
+  {0, 32, 32, 352, 1, 1},  // tsm_atb_f64_32x32_t8x8_r64_p1_nt
   {0, 32, 64, 359, 2, 0},  // tsm_atb_f64_32x64_t8x8_r32_p2
   {0, 64, 1, 360, 2, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
   {0, 64, 2, 361, 2, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
@@ -910,7 +910,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 64, 8, 363, 2, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
   {0, 64, 16, 364, 2, 0},  // tsm_atb_f64_64x16_t8x8_r32_p2
   {0, 64, 32, 365, 2, 0},  // tsm_atb_f64_64x32_t8x8_r32_p2
-  {0, 64, 64, 370, 2, 0},  // tsm_atb_f64_64x64_t8x8_r32_p1
+  {0, 64, 64, 369, 2, 1},  // tsm_atb_f64_64x64_t8x8_r16_p2_nt
 };
 
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {
