@@ -121,8 +121,12 @@ tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, i
 
 /* ---- batching (no reference counterpart) ----------------------------------
  * `count` independent problems C_i (+)= A_i^T B_i in ONE persistent launch.
- * `problems` is a HOST array; the library copies it into its workspace.
- * All problems share in_type/out_type; shapes may differ. */
+ * `problems` is a HOST array; the library copies the work plan into device
+ * memory (the workspace, or a plan's own buffer) before the launch.  All
+ * problems share in_type/out_type; shapes may differ (1 <= k, m <= 64 for
+ * TSM_F64, <= 32 for TSM_F32 and TSM_BF16).  The results are deterministic
+ * for a given problem list.  Calls that share a workspace or a plan must be
+ * ordered on one stream. */
 typedef struct {
   const void* A;
   const void* B;
@@ -133,9 +137,24 @@ typedef struct {
 
 size_t tsm_atb_batched_workspace_size(tsm_dtype in_type, int count, const tsm_problem* problems);
 
+/* Uploads the plan (returning once the host array has been consumed) and
+ * launches; workspace NULL = a library-owned buffer for this stream. */
 tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
                            const tsm_problem* problems, int accumulate,
                            void* workspace, size_t workspace_bytes, tsm_stream_t stream);
+
+/* A batched plan owns its device buffer: create it once (allocates and
+ * uploads, synchronously), then execute it any number of times - each execute
+ * is one kernel launch with no allocation, copy or memset, so it can be
+ * captured in a HIP graph. */
+typedef struct tsm_batched_plan_s* tsm_batched_plan;
+tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int count,
+                                   const tsm_problem* problems, int accumulate,
+                                   tsm_batched_plan* plan);
+tsm_status tsm_batched_plan_execute(tsm_batched_plan plan, tsm_stream_t stream);
+/* Work items and launch grid of a plan (either pointer may be NULL). */
+tsm_status tsm_batched_plan_info(tsm_batched_plan plan, int64_t* items, int64_t* grid);
+tsm_status tsm_batched_plan_destroy(tsm_batched_plan plan);
 
 /* ---- synthetic inputs (replaces main.cc:72-86, sycl_codegen_new.py:436-439) -- */
 tsm_status tsm_fill_random(tsm_dtype type, tsm_dist dist, uint64_t seed, uint64_t stream_id,

@@ -127,3 +127,24 @@ def test_kernel_info_bounds():
     assert L.tsm_kernel_info(L.tsm_num_kernels(), ct.byref(d)) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_kernel_info(0, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_atb_workspace_size(_lib.F64, -1, 8, 8, None) == 0
+
+
+def test_batched_argument_errors():
+    L = _lib.lib()
+    P = (_lib.Problem * 2)()
+    P[0] = _lib.Problem(FAKE, FAKE, FAKE, 10, 8, 8, 8, 8, 8)
+    P[1] = _lib.Problem(FAKE, FAKE, FAKE, 10, 8, 8, 8, 8, 0)  # m = 0
+    assert L.tsm_atb_batched(_lib.F64, _lib.F64, 2, P, 0, None, 0, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_atb_batched(_lib.F64, _lib.F64, -1, P, 0, None, 0, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_atb_batched(_lib.F64, _lib.BF16, 1, P, 0, None, 0, None) == _lib.ERR_UNSUPPORTED
+    P[1] = _lib.Problem(FAKE, FAKE, FAKE, 10, 64, 8, 8, 64, 8)
+    assert L.tsm_atb_batched(_lib.F32, _lib.F64, 2, P, 0, None, 0, None) == _lib.ERR_UNSUPPORTED
+    assert L.tsm_atb_batched(_lib.F64, _lib.F64, 0, None, 0, None, 0, None) == _lib.SUCCESS
+    # the workspace size is a pure function of the problem list (no device query)
+    P[1] = _lib.Problem(FAKE, FAKE, FAKE, 10000, 8, 8, 8, 8, 8)
+    assert L.tsm_atb_batched_workspace_size(_lib.F64, 2, P) > 0
+    assert L.tsm_atb_batched_workspace_size(_lib.F64, 2, P) == L.tsm_atb_batched_workspace_size(_lib.F64, 2, P)
+    small = L.tsm_atb_batched_workspace_size(_lib.F64, 2, P) - 8
+    assert L.tsm_atb_batched(_lib.F64, _lib.F64, 2, P, 0, FAKE, small, None) == _lib.ERR_WORKSPACE
+    assert L.tsm_batched_plan_execute(None, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.tsm_batched_plan_destroy(None) == _lib.ERR_INVALID_ARGUMENT

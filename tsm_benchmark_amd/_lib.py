@@ -24,7 +24,8 @@ EXPORTS = (
     "tsm_atb", "tsm_atb_ex", "tsm_atb_workspace_size", "tsm_atb_backward",
     "tsm_atb_batched_workspace_size", "tsm_atb_batched", "tsm_fill_random",
     "tsm_strerror", "tsm_last_hip_error", "tsm_version", "tsm_num_kernels",
-    "tsm_kernel_info", "tsm_atb_plan",
+    "tsm_kernel_info", "tsm_atb_plan", "tsm_batched_plan_create", "tsm_batched_plan_execute",
+    "tsm_batched_plan_info", "tsm_batched_plan_destroy",
 )
 
 
@@ -102,6 +103,14 @@ def lib() -> ct.CDLL:
     L.tsm_num_kernels.restype = I
     L.tsm_kernel_info.restype = I
     L.tsm_kernel_info.argtypes = [I, ct.POINTER(KernelDesc)]
+    L.tsm_batched_plan_create.restype = I
+    L.tsm_batched_plan_create.argtypes = [I, I, I, ct.POINTER(Problem), I, ct.POINTER(P)]
+    L.tsm_batched_plan_execute.restype = I
+    L.tsm_batched_plan_execute.argtypes = [P, P]
+    L.tsm_batched_plan_info.restype = I
+    L.tsm_batched_plan_info.argtypes = [P, ct.POINTER(I64), ct.POINTER(I64)]
+    L.tsm_batched_plan_destroy.restype = I
+    L.tsm_batched_plan_destroy.argtypes = [P]
     L.tsm_atb_plan.restype = I
     L.tsm_atb_plan.argtypes = [I, I64, I, I, ct.POINTER(KernelConfig), ct.POINTER(I),
                                ct.POINTER(I64)]

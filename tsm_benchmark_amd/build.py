@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(REPO, "include"), "-I", CSRC,
 ]
-HAND_WRITTEN = ["tsm_capi.hip", "tsm_backward.hip"]
+HAND_WRITTEN = ["tsm_capi.hip", "tsm_backward.hip", "tsm_batched_host.hip"]
 
 
 def sources() -> list[str]:

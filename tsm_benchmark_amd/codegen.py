@@ -271,6 +271,56 @@ def emit_kernel_file(dtype: str, kc: int, mc: int, cfgs: list[KernelConfig]) -> 
     return "\n".join(body)
 
 
+BATCH_LDS = LDS_BUDGET  # LDS of the batched kernels (every class fits one budget)
+
+
+def batched_classes(defaults: dict, dtype: str) -> list[KernelConfig]:
+    """Per width class, the configuration the batched kernel uses: the tuned
+    default when it fits the batched LDS budget, else the heuristic one."""
+    out = []
+    # FP32/BF16 carry a (hi, lo) pair per accumulator: their 64-wide classes
+    # would not fit the shared register budget, so batching stops at 32.
+    widths = CLASSES if dtype == "f64" else tuple(c for c in CLASSES if c <= 32)
+    for kc in widths:
+        for mc in widths:
+            c = defaults[(dtype, kc, mc)][0]
+            if c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32):
+                c = default_config(dtype, kc, mc)
+            if c.tm * c.tn >= 32 and c.pf == 2:
+                # one register set of loads: the switch over every class shares
+                # one register budget, sized by its widest class
+                c = dataclasses.replace(c, pf=1)
+            out.append(c)
+    return out
+
+
+def emit_batched_file(dtype: str, classes: list[KernelConfig]) -> str:
+    t = CXX_TYPE[dtype]
+    lds = max(c.lds_bytes for c in classes)
+    name = f"tsm_atb_batched_{dtype}"
+    out = [BANNER, '#include "../tsm_internal.hpp"\n',
+           f"// {name}: {len(classes)} width classes, LDS {lds} B",
+           f"extern \"C\" __global__ void __launch_bounds__(256) {name}(tsm::BatchArgs ba) {{",
+           f"  __shared__ __attribute__((aligned(16))) unsigned char smem[{lds}];",
+           "  __shared__ int s_flag;",
+           "  for (int64_t i = blockIdx.x; i < ba.n_items; i += gridDim.x) {",
+           "    const tsm::BatchItem it = ba.items[i];",
+           "    switch (ba.probs[it.problem].cls) {"]
+    for i, c in enumerate(classes):
+        out.append(f"      case {i}: tsm::batched_item_call<{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, "
+                   f"{c.pf}, {c.nt}>(ba, it, smem, &s_flag); break;  // {c.name}")
+    out += ["      default: break;", "    }", "  }", "}", "",
+            f"static const tsm::BatchClass kClasses[] = {{"]
+    out += [f"  {{{c.kc}, {c.mc}, {c.rows}}}," for c in classes]
+    out += ["};", "",
+            f"hipError_t launch_{name}(const tsm::BatchArgs& a, unsigned grid, hipStream_t s) {{",
+            f"  hipLaunchKernelGGL({name}, dim3(grid), dim3(256), 0, s, a);",
+            "  return hipGetLastError();", "}", "",
+            f"const tsm::BatchClass* classes_{name}(int* count) {{",
+            f"  *count = {len(classes)};", "  return kClasses;", "}", ""]
+    return "\n".join(out)
+
+
 def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
     idx = {c: i for i, c in enumerate(cfgs)}
     out = [BANNER, '#include "../tsm_internal.hpp"\n']
@@ -285,6 +335,26 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
     for (dt, kc, mc), (c, wpc, il) in sorted(defaults.items()):
         out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}, {il}}},  // {c.name}")
     out.append("};\n")
+    for dt in DTYPES:
+        nm = f"tsm_atb_batched_{dt}"
+        out.append(f"hipError_t launch_{nm}(const tsm::BatchArgs&, unsigned, hipStream_t);")
+        out.append(f"const tsm::BatchClass* classes_{nm}(int* count);")
+    out.append("static const tsm::BatchedKernel* batched_all() {")
+    out.append("  static tsm::BatchedKernel all[3];")
+    out.append("  static bool init = [] {")
+    for i, dt in enumerate(DTYPES):
+        nm = f"tsm_atb_batched_{dt}"
+        lds = max(c.lds_bytes for c in batched_classes(defaults, dt))
+        out.append(f"    all[{i}].name = \"{nm}\"; all[{i}].dtype = {DTYPE_ID[dt]};")
+        out.append(f"    all[{i}].classes = classes_{nm}(&all[{i}].n_classes);")
+        out.append(f"    all[{i}].lds_bytes = {lds}; all[{i}].launch = launch_{nm};")
+    out.append("    return true;")
+    out.append("  }();")
+    out.append("  (void)init;")
+    out.append("  return all;")
+    out.append("}")
+    out.append("extern \"C\" const tsm::BatchedKernel* tsm_gen_batched_table(int* count) {\n"
+               f"  *count = {len(DTYPES)};\n  return batched_all();\n}}")
     out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {\n"
                "  *count = (int)(sizeof(kTable) / sizeof(kTable[0]));\n  return kTable;\n}")
     out.append("extern \"C\" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {\n"
@@ -313,10 +383,15 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
                 p = os.path.join(out_dir, f"tsm_kernels_{dt}_{kc}x{mc}.hip")
                 _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs))
                 paths.append(p)
-    keep = {os.path.basename(x) for x in paths} | {"tsm_table.hip"}
+    keep = {os.path.basename(x) for x in paths} | {"tsm_table.hip"} | \
+        {f"tsm_batched_{dt}.hip" for dt in DTYPES}
     for f in os.listdir(out_dir):  # drop files of an older split
         if f.endswith(".hip") and f not in keep:
             os.remove(os.path.join(out_dir, f))
+    for dt in DTYPES:
+        p = os.path.join(out_dir, f"tsm_batched_{dt}.hip")
+        _write_if_changed(p, emit_batched_file(dt, batched_classes(defaults, dt)))
+        paths.append(p)
     p = os.path.join(out_dir, "tsm_table.hip")
     _write_if_changed(p, emit_table(cfgs, defaults))
     paths.append(p)

@@ -913,6 +913,33 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 64, 64, 369, 2, 1},  // tsm_atb_f64_64x64_t8x8_r16_p2_nt
 };
 
+hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
+const tsm::BatchClass* classes_tsm_atb_batched_f64(int* count);
+hipError_t launch_tsm_atb_batched_f32(const tsm::BatchArgs&, unsigned, hipStream_t);
+const tsm::BatchClass* classes_tsm_atb_batched_f32(int* count);
+hipError_t launch_tsm_atb_batched_bf16(const tsm::BatchArgs&, unsigned, hipStream_t);
+const tsm::BatchClass* classes_tsm_atb_batched_bf16(int* count);
+static const tsm::BatchedKernel* batched_all() {
+  static tsm::BatchedKernel all[3];
+  static bool init = [] {
+    all[0].name = "tsm_atb_batched_f64"; all[0].dtype = 0;
+    all[0].classes = classes_tsm_atb_batched_f64(&all[0].n_classes);
+    all[0].lds_bytes = 65536; all[0].launch = launch_tsm_atb_batched_f64;
+    all[1].name = "tsm_atb_batched_f32"; all[1].dtype = 1;
+    all[1].classes = classes_tsm_atb_batched_f32(&all[1].n_classes);
+    all[1].lds_bytes = 65536; all[1].launch = launch_tsm_atb_batched_f32;
+    all[2].name = "tsm_atb_batched_bf16"; all[2].dtype = 2;
+    all[2].classes = classes_tsm_atb_batched_bf16(&all[2].n_classes);
+    all[2].lds_bytes = 65536; all[2].launch = launch_tsm_atb_batched_bf16;
+    return true;
+  }();
+  (void)init;
+  return all;
+}
+extern "C" const tsm::BatchedKernel* tsm_gen_batched_table(int* count) {
+  *count = 3;
+  return batched_all();
+}
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {
   *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
   return kTable;

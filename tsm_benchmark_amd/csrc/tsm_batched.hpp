@@ -1,0 +1,91 @@
+// Batched C_i (+)= A_i^T B_i for many independent problems in one persistent
+// launch.  The host cuts every problem into work items (runs of row tiles of
+// about 2 MiB of A + B each), sorts them by size and deals item i to
+// workgroup i % G.  A workgroup streams its item with the same body as the
+// single-problem kernel (atb_partial, specialised for the problem's width
+// class), stores the item's FP64 partial in the problem's partial slots, and
+// arrives at the problem's counter; the last item of a problem to arrive adds
+// that problem's partials in slot order and writes C_i.  The summation tree
+// depends only on the plan, so results are deterministic.
+#pragma once
+
+#include "tsm_device.hpp"
+
+namespace tsm {
+
+struct BatchProblem {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t n, lda, ldb, ldc;
+  int k, m;
+  int cls;          // width-class index of the batched kernel's class list
+  int dense;        // k == KC, m == MC, lda == k, ldb == m, 16-byte aligned A and B
+  int64_t partial_offset;  // first FP64 word of this problem's partial slots
+  unsigned int n_items;
+  unsigned int pad_;
+};
+
+struct BatchItem {
+  int problem;
+  int slot;             // partial slot within the problem
+  int64_t tile_begin;   // tiles [tile_begin, tile_end) of the problem (the last
+  int64_t tile_end;     // item may include the partial tile)
+};
+
+struct BatchArgs {
+  const BatchProblem* probs;
+  const BatchItem* items;
+  int64_t n_items;
+  double* partials;
+  unsigned int* counters;  // one per problem, zero between calls
+  int accumulate;
+  int c_f32;
+};
+
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+__device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchItem& it,
+                                             unsigned char* smem, int* s_flag) {
+  const BatchProblem& pr = ba.probs[it.problem];
+  constexpr int KCMC = KC * MC;
+  AtbArgs a{};
+  a.A = pr.A;
+  a.B = pr.B;
+  a.C = pr.C;
+  a.partials = ba.partials + pr.partial_offset;
+  a.counter = ba.counters + it.problem;
+  a.n = pr.n;
+  a.lda = pr.lda;
+  a.ldb = pr.ldb;
+  a.ldc = pr.ldc;
+  a.k = pr.k;
+  a.m = pr.m;
+  a.accumulate = ba.accumulate;
+  a.c_f32 = ba.c_f32;
+  a.dense = pr.dense;
+  a.finish = 1;
+  const int64_t full = pr.n / T;
+  const int tail_rows = (int)(pr.n - full * T);
+  const int64_t end_full = it.tile_end < full ? it.tile_end : full;
+  const int64_t count = end_full > it.tile_begin ? end_full - it.tile_begin : 0;
+  const bool owns_tail = tail_rows > 0 && it.tile_begin <= full && it.tile_end > full;
+  atb_partial<S, KC, MC, TM, TN, T, PF, NT>(a, smem, it.tile_begin, 1, count, owns_tail, full,
+                                            tail_rows, a.partials + (int64_t)it.slot * KCMC);
+  if (arrive_last(a.counter, pr.n_items, s_flag)) {
+    finish_outputs(a, (int)pr.n_items, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
+    if (threadIdx.x == 0)
+      __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();  // the next item reuses the LDS this one read last
+}
+
+// One class body behind a call boundary: the batched kernel switches over many
+// classes, and keeping each body out of line gives it its own register frame
+// (inlined together, the bodies share one allocation and spill).
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+__device__ __noinline__ void batched_item_call(const BatchArgs& ba, const BatchItem& it,
+                                               unsigned char* smem, int* s_flag) {
+  batched_item<S, KC, MC, TM, TN, T, PF, NT>(ba, it, smem, s_flag);
+}
+
+}  // namespace tsm

@@ -1,0 +1,279 @@
+// Host side of the batched C_i (+)= A_i^T B_i: validation, the work plan
+// (per-problem chunks of about 2 MiB of A + B, sorted largest first), the
+// workspace layout, and the launches (tsm_atb_batched and the plan API).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "tsm/tsm.h"
+#include "tsm_internal.hpp"
+
+namespace {
+
+constexpr size_t kAlign = 256;
+constexpr int64_t kItemBytes = 2 << 20;  // target bytes of A + B per work item
+constexpr int kWgsPerCu = 2;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+int elem_bytes(tsm_dtype t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
+int width_class(int x) {
+  int c = 1;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+const tsm::BatchedKernel* batched_kernel(tsm_dtype t) {
+  int n = 0;
+  const tsm::BatchedKernel* all = tsm_gen_batched_table(&n);
+  for (int i = 0; i < n; ++i)
+    if (all[i].dtype == (int)t) return &all[i];
+  return nullptr;
+}
+
+// The plan of one batched call: problem descriptors, work items and the
+// byte layout of the device buffer
+//   [counters | problems | items | partials]
+// whose first three parts are uploaded from `header` (counters as zeros).
+struct Layout {
+  std::vector<tsm::BatchProblem> probs;
+  std::vector<tsm::BatchItem> items;
+  size_t probs_off = 0, items_off = 0, partials_off = 0, total = 0;
+  std::vector<unsigned char> header;
+};
+
+tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
+                        const tsm_problem* problems, Layout* L) {
+  if (in_type != TSM_F64 && in_type != TSM_F32 && in_type != TSM_BF16) return TSM_ERR_UNSUPPORTED;
+  if (out_type != TSM_F64 && out_type != TSM_F32) return TSM_ERR_UNSUPPORTED;
+  if (count < 0 || (count > 0 && !problems)) return TSM_ERR_INVALID_ARGUMENT;
+  const tsm::BatchedKernel* bk = batched_kernel(in_type);
+  if (!bk) return TSM_ERR_NO_KERNEL;
+  const int es = elem_bytes(in_type);
+  int64_t partial_words = 0;
+  L->probs.resize(count);
+  for (int p = 0; p < count; ++p) {
+    const tsm_problem& q = problems[p];
+    tsm_status st = tsm_internal_check_atb(in_type, out_type, q.n, q.k, q.m, q.A, q.lda, q.B,
+                                           q.ldb, q.C, q.ldc);
+    if (st != TSM_SUCCESS) return st;
+    const int kc = width_class(q.k), mc = width_class(q.m);
+    int cls = -1;
+    for (int c = 0; c < bk->n_classes; ++c)
+      if (bk->classes[c].kc == kc && bk->classes[c].mc == mc) cls = c;
+    if (cls < 0) return TSM_ERR_UNSUPPORTED;
+    const int64_t T = bk->classes[cls].rows;
+    const int64_t tiles = (q.n + T - 1) / T;
+    const int64_t tile_bytes = T * (kc + mc) * es;
+    const int64_t chunk = std::max<int64_t>(1, kItemBytes / tile_bytes);
+    const int64_t nitems = std::max<int64_t>(1, (tiles + chunk - 1) / chunk);
+    if (nitems > 0x7fffffff) return TSM_ERR_INVALID_ARGUMENT;
+    tsm::BatchProblem& b = L->probs[p];
+    b = tsm::BatchProblem{};
+    b.A = q.A;
+    b.B = q.B;
+    b.C = q.C;
+    b.n = q.n;
+    b.lda = q.lda;
+    b.ldb = q.ldb;
+    b.ldc = q.ldc;
+    b.k = q.k;
+    b.m = q.m;
+    b.cls = cls;
+    b.dense = (q.k == kc && q.m == mc && q.lda == q.k && q.ldb == q.m &&
+               ((uintptr_t)q.A % 16) == 0 && ((uintptr_t)q.B % 16) == 0)
+                  ? 1
+                  : 0;
+    b.partial_offset = partial_words;
+    b.n_items = (unsigned)nitems;
+    partial_words += nitems * kc * mc;
+    for (int64_t s = 0; s < nitems; ++s) {
+      tsm::BatchItem it{};
+      it.problem = p;
+      it.slot = (int)s;
+      it.tile_begin = s * chunk;
+      it.tile_end = std::min(tiles, (s + 1) * chunk);
+      L->items.push_back(it);
+    }
+  }
+  // Largest items first, dealt round-robin: a static, deterministic balance.
+  std::stable_sort(L->items.begin(), L->items.end(),
+                   [](const tsm::BatchItem& x, const tsm::BatchItem& y) {
+                     return (x.tile_end - x.tile_begin) > (y.tile_end - y.tile_begin);
+                   });
+  const size_t counters = align_up(sizeof(unsigned) * (size_t)std::max(count, 1));
+  L->probs_off = counters;
+  L->items_off = L->probs_off + align_up(sizeof(tsm::BatchProblem) * (size_t)count);
+  L->partials_off = L->items_off + align_up(sizeof(tsm::BatchItem) * L->items.size());
+  L->total = L->partials_off + (size_t)partial_words * sizeof(double);
+  L->header.assign(L->partials_off, 0);
+  if (count)
+    std::memcpy(L->header.data() + L->probs_off, L->probs.data(),
+                sizeof(tsm::BatchProblem) * (size_t)count);
+  if (!L->items.empty())
+    std::memcpy(L->header.data() + L->items_off, L->items.data(),
+                sizeof(tsm::BatchItem) * L->items.size());
+  return TSM_SUCCESS;
+}
+
+tsm::BatchArgs make_args(const Layout& L, void* base, int accumulate, tsm_dtype out_type) {
+  unsigned char* b = static_cast<unsigned char*>(base);
+  tsm::BatchArgs a{};
+  a.counters = reinterpret_cast<unsigned*>(b);
+  a.probs = reinterpret_cast<const tsm::BatchProblem*>(b + L.probs_off);
+  a.items = reinterpret_cast<const tsm::BatchItem*>(b + L.items_off);
+  a.partials = reinterpret_cast<double*>(b + L.partials_off);
+  a.n_items = (int64_t)L.items.size();
+  a.accumulate = accumulate ? 1 : 0;
+  a.c_f32 = out_type == TSM_F32 ? 1 : 0;
+  return a;
+}
+
+tsm_status grid_for(int64_t n_items, unsigned* grid) {
+  int cus = 0;
+  tsm_status st = tsm_internal_device_cus(&cus);
+  if (st != TSM_SUCCESS) return st;
+  int64_t g = std::min<int64_t>(n_items, (int64_t)cus * kWgsPerCu);
+  *grid = (unsigned)std::max<int64_t>(g, 1);
+  return TSM_SUCCESS;
+}
+
+// Pinned staging for the per-call upload of tsm_atb_batched, one per stream;
+// an event guards its reuse until the previous copy out of it has finished.
+struct Staging {
+  void* host = nullptr;
+  size_t bytes = 0;
+  hipEvent_t done = nullptr;
+};
+std::mutex g_staging_mu;
+std::map<hipStream_t, Staging>& stagings() {
+  static auto* m = new std::map<hipStream_t, Staging>();  // lives to process exit
+  return *m;
+}
+
+tsm_status upload(const Layout& L, void* dst, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(g_staging_mu);
+  Staging& st = stagings()[s];
+  hipError_t e;
+  if (st.done) {
+    e = hipEventSynchronize(st.done);
+    if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  } else {
+    e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+    if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  }
+  if (st.bytes < L.header.size()) {
+    if (st.host) (void)hipHostFree(st.host);
+    st.host = nullptr;
+    st.bytes = 0;
+    e = hipHostMalloc(&st.host, L.header.size(), hipHostMallocDefault);
+    if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+    st.bytes = L.header.size();
+  }
+  std::memcpy(st.host, L.header.data(), L.header.size());
+  e = hipMemcpyAsync(dst, st.host, L.header.size(), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  return (tsm_status)tsm_internal_hip_status(hipEventRecord(st.done, s));
+}
+
+}  // namespace
+
+struct tsm_batched_plan_s {
+  tsm_dtype in_type;
+  void* buffer;
+  tsm::BatchArgs args;
+  unsigned grid;
+  const tsm::BatchedKernel* kernel;
+};
+
+extern "C" {
+
+size_t tsm_atb_batched_workspace_size(tsm_dtype in_type, int count, const tsm_problem* problems) {
+  Layout L;
+  if (build_layout(in_type, TSM_F64, count, problems, &L) != TSM_SUCCESS) return 0;
+  return L.total;
+}
+
+tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
+                           const tsm_problem* problems, int accumulate, void* workspace,
+                           size_t workspace_bytes, tsm_stream_t stream) {
+  Layout L;
+  tsm_status st = build_layout(in_type, out_type, count, problems, &L);
+  if (st != TSM_SUCCESS) return st;
+  if (count == 0) return TSM_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (workspace) {
+    if (workspace_bytes < L.total) return TSM_ERR_WORKSPACE;
+  } else {
+    st = tsm_internal_pooled_workspace(L.total, stream, &workspace);
+    if (st != TSM_SUCCESS) return st;
+  }
+  unsigned grid = 1;
+  st = grid_for((int64_t)L.items.size(), &grid);
+  if (st != TSM_SUCCESS) return st;
+  st = upload(L, workspace, s);
+  if (st != TSM_SUCCESS) return st;
+  const tsm::BatchedKernel* bk = batched_kernel(in_type);
+  return (tsm_status)tsm_internal_hip_status(
+      bk->launch(make_args(L, workspace, accumulate, out_type), grid, s));
+}
+
+tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int count,
+                                   const tsm_problem* problems, int accumulate,
+                                   tsm_batched_plan* plan) {
+  if (!plan) return TSM_ERR_INVALID_ARGUMENT;
+  *plan = nullptr;
+  Layout L;
+  tsm_status st = build_layout(in_type, out_type, count, problems, &L);
+  if (st != TSM_SUCCESS) return st;
+  unsigned grid = 1;
+  st = grid_for((int64_t)L.items.size(), &grid);
+  if (st != TSM_SUCCESS) return st;
+  void* buf = nullptr;
+  hipError_t e = hipMalloc(&buf, std::max<size_t>(L.total, kAlign));
+  if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  e = hipMemcpy(buf, L.header.data(), L.header.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(buf);
+    return (tsm_status)tsm_internal_hip_status(e);
+  }
+  auto* p = new (std::nothrow) tsm_batched_plan_s{in_type, buf,
+                                                  make_args(L, buf, accumulate, out_type), grid,
+                                                  batched_kernel(in_type)};
+  if (!p) {
+    (void)hipFree(buf);
+    return TSM_ERR_INVALID_ARGUMENT;
+  }
+  *plan = p;
+  return TSM_SUCCESS;
+}
+
+tsm_status tsm_batched_plan_execute(tsm_batched_plan plan, tsm_stream_t stream) {
+  if (!plan) return TSM_ERR_INVALID_ARGUMENT;
+  if (plan->args.n_items == 0) return TSM_SUCCESS;
+  return (tsm_status)tsm_internal_hip_status(
+      plan->kernel->launch(plan->args, plan->grid, static_cast<hipStream_t>(stream)));
+}
+
+tsm_status tsm_batched_plan_info(tsm_batched_plan plan, int64_t* items, int64_t* grid) {
+  if (!plan) return TSM_ERR_INVALID_ARGUMENT;
+  if (items) *items = plan->args.n_items;
+  if (grid) *grid = plan->grid;
+  return TSM_SUCCESS;
+}
+
+tsm_status tsm_batched_plan_destroy(tsm_batched_plan plan) {
+  if (!plan) return TSM_ERR_INVALID_ARGUMENT;
+  // Work that uses the buffer may still be queued: wait for the device.
+  hipError_t e = hipDeviceSynchronize();
+  const hipError_t f = hipFree(plan->buffer);
+  delete plan;
+  if (e == hipSuccess) e = f;
+  return (tsm_status)tsm_internal_hip_status(e);
+}
+
+}  // extern "C"

@@ -374,6 +374,18 @@ const char* tsm_strerror(tsm_status s) {
 
 int tsm_last_hip_error(void) { return g_last_hip_error; }
 
+tsm_status tsm_internal_check_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                                  const void* A, int64_t lda, const void* B, int64_t ldb,
+                                  const void* C, int64_t ldc) {
+  return check_atb(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc);
+}
+
+tsm_status tsm_internal_device_cus(int* cus) { return device_cus(cus); }
+
+tsm_status tsm_internal_pooled_workspace(size_t bytes, void* stream, void** out) {
+  return pooled_workspace(bytes, static_cast<hipStream_t>(stream), out);
+}
+
 int tsm_internal_hip_status(hipError_t e) {
   if (e == hipSuccess) return TSM_SUCCESS;
   g_last_hip_error = (int)e;
@@ -408,9 +420,3 @@ tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out) {
 
 }  // extern "C"
 
-// ---- placeholders until the batched kernel lands --------------------------------------
-extern "C" size_t tsm_atb_batched_workspace_size(tsm_dtype, int, const tsm_problem*) { return 0; }
-extern "C" tsm_status tsm_atb_batched(tsm_dtype, tsm_dtype, int, const tsm_problem*, int, void*,
-                                      size_t, tsm_stream_t) {
-  return TSM_ERR_UNSUPPORTED;
-}

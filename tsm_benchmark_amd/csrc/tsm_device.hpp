@@ -274,8 +274,25 @@ __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
 }
 
 // ---- the streaming kernel body -----------------------------------------------------
-template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT = 0>
-__device__ __forceinline__ void atb_body(const AtbArgs& a) {
+// LDS bytes a configuration needs: two pipeline stages, the KC x MC FP64
+// reduction buffer or the final-reduction scratch, whichever is largest.
+template <class S, int KC, int MC, int T>
+constexpr int atb_smem_bytes() {
+  constexpr int stage2 = 2 * T * (KC + MC) * (int)sizeof(S);
+  constexpr int red = KC * MC * 8;
+  constexpr int m1 = stage2 > red ? stage2 : red;
+  return m1 > kThreads * 8 ? m1 : kThreads * 8;
+}
+
+// Streams a sequence of row tiles of one problem - the full tiles
+// first + j * step (j < count) and, if owns_tail, the partial tile of
+// tail_rows rows that starts at row full * T - and writes this workgroup's
+// KC x MC FP64 partial sum to partial_out (global memory).  smem must hold
+// atb_smem_bytes<S, KC, MC, T>() bytes.
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+__device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* smem, int64_t first,
+                                            int64_t step, int64_t count, bool owns_tail,
+                                            int64_t full, int tail_rows, double* partial_out) {
   using Acc = typename Traits<S>::Acc;
   constexpr bool kComp = Traits<S>::compensated;
   constexpr int ES = (int)sizeof(S);
@@ -292,16 +309,9 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
   constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
   constexpr int STAGE = A_BYTES + B_BYTES;
   constexpr int KCMC = KC * MC;
-  constexpr int RED_BYTES = KCMC * 8;
-  constexpr int SMEM = (2 * STAGE > RED_BYTES ? 2 * STAGE : RED_BYTES) > kThreads * 8
-                           ? (2 * STAGE > RED_BYTES ? 2 * STAGE : RED_BYTES)
-                           : kThreads * 8;
   static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
   // Fold FP32 block sums into (hi, lo) every FOLD_TILES tiles (about 32 rows).
   constexpr int FOLD_TILES = RPT >= 32 ? 1 : 32 / RPT;
-
-  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
-  __shared__ int s_last;
 
   const int tid = threadIdx.x;
   const int p = tid % NTILES, q = tid / NTILES;
@@ -361,26 +371,6 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
     }
   };
 
-  // Tiles of this workgroup: the full tiles first + j * step (j < count) and,
-  // for one workgroup, the partial tile at the end of the rows.  Contiguous
-  // mode gives each workgroup a run of tiles_per_wg consecutive tiles;
-  // interleaved mode deals tile t to workgroup t % G, so that the workgroups
-  // read one moving window of memory together.
-  const int64_t full = a.n / T;
-  const int64_t G = gridDim.x, g = blockIdx.x;
-  int64_t first, step, count;
-  if (a.interleave) {
-    first = g;
-    step = G;
-    count = first < full ? (full - first + G - 1) / G : 0;
-  } else {
-    first = g * a.tiles_per_wg;
-    step = 1;
-    count = first < full ? (full - first < a.tiles_per_wg ? full - first : a.tiles_per_wg) : 0;
-  }
-  const int tail_rows = (int)(a.n - full * T);
-  const bool owns_tail =
-      tail_rows > 0 && (a.interleave ? full % G == g : full / a.tiles_per_wg == g);
   const int64_t nfull = a.dense ? count : 0;
 
   // ---- pipelined dense tiles ---------------------------------------------------
@@ -517,21 +507,53 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
     }
     __syncthreads();
   }
-  for (int o = tid; o < KCMC; o += kThreads) a.partials[(int64_t)blockIdx.x * KCMC + o] = red[o];
+  for (int o = tid; o < KCMC; o += kThreads) partial_out[o] = red[o];
+}
 
-  if (!a.finish) return;
-  // ---- last workgroup to arrive writes C ------------------------------------------------
-  __threadfence();
+// Arrival at a problem's counter after its partial is stored: returns true
+// (in every thread) for the last of `expected` arrivals, which then sees all
+// partials.  That workgroup resets the counter to zero after using it.
+__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int expected,
+                                            int* s_flag) {
+  __threadfence();  // release this workgroup's partial at agent scope
   __syncthreads();
-  if (tid == 0) {
-    const unsigned prev = atomicAdd(a.counter, 1u);
-    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+  if (threadIdx.x == 0) *s_flag = (atomicAdd(counter, 1u) == expected - 1) ? 1 : 0;
+  __syncthreads();
+  const bool last = *s_flag != 0;
+  if (last) __threadfence();  // acquire the other workgroups' partials
+  return last;
+}
+
+// One problem over the whole grid: contiguous runs of tiles_per_wg tiles per
+// workgroup, or tile t to workgroup t % G (interleave); then the last
+// workgroup to arrive adds the G partials in workgroup order and writes C.
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+__device__ __forceinline__ void atb_body(const AtbArgs& a) {
+  constexpr int SMEM = atb_smem_bytes<S, KC, MC, T>();
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+  __shared__ int s_last;
+  const int64_t full = a.n / T;
+  const int64_t G = gridDim.x, g = blockIdx.x;
+  int64_t first, step, count;
+  if (a.interleave) {
+    first = g;
+    step = G;
+    count = first < full ? (full - first + G - 1) / G : 0;
+  } else {
+    first = g * a.tiles_per_wg;
+    step = 1;
+    count = first < full ? (full - first < a.tiles_per_wg ? full - first : a.tiles_per_wg) : 0;
   }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
+  const int tail_rows = (int)(a.n - full * T);
+  const bool owns_tail =
+      tail_rows > 0 && (a.interleave ? full % G == g : full / a.tiles_per_wg == g);
+  constexpr int KCMC = KC * MC;
+  atb_partial<S, KC, MC, TM, TN, T, PF, NT>(a, smem, first, step, count, owns_tail, full,
+                                            tail_rows, a.partials + g * KCMC);
+  if (!a.finish) return;
+  if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
   finish_outputs(a, (int)gridDim.x, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
-  if (tid == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace tsm

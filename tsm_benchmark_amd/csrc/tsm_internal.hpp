@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tsm/tsm.h"
+
+#include "tsm_batched.hpp"
 #include "tsm_device.hpp"
 
 namespace tsm {
@@ -27,12 +30,36 @@ struct TunedEntry {
   int interleave;  // 1: tiles dealt round-robin to workgroups
 };
 
+// Width class handled by a batched kernel: the class's rows per tile fix the
+// problem's tiling on the host.
+struct BatchClass {
+  int kc, mc, rows;
+};
+using BatchLaunchFn = hipError_t (*)(const BatchArgs& args, unsigned grid, hipStream_t stream);
+struct BatchedKernel {
+  const char* name;
+  int dtype;
+  const BatchClass* classes;
+  int n_classes;
+  int lds_bytes;
+  BatchLaunchFn launch;
+};
+
 }  // namespace tsm
 
 // Records a HIP error for tsm_last_hip_error() and maps it to a status
 // (TSM_SUCCESS for hipSuccess, else TSM_ERR_HIP).  Defined in tsm_capi.hip.
 extern "C" int tsm_internal_hip_status(hipError_t e);
+// Argument checks of tsm_atb, the device's compute-unit count and the
+// library-owned per-(device, stream) workspace (all in tsm_capi.hip).
+extern "C" tsm_status tsm_internal_check_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n,
+                                             int k, int m, const void* A, int64_t lda,
+                                             const void* B, int64_t ldb, const void* C,
+                                             int64_t ldc);
+extern "C" tsm_status tsm_internal_device_cus(int* cus);
+extern "C" tsm_status tsm_internal_pooled_workspace(size_t bytes, void* stream, void** out);
 
 // Defined by the generated tsm_table.hip.
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count);
 extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count);
+extern "C" const tsm::BatchedKernel* tsm_gen_batched_table(int* count);

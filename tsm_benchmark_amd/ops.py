@@ -108,6 +108,79 @@ def random_matrix(rows: int, cols: int, dtype: torch.dtype = torch.float64, *, s
     return fill_random(X, seed=seed, stream_id=stream_id, dist=dist)
 
 
+def _problems(As, Bs, Cs):
+    if not (len(As) == len(Bs) == len(Cs)):
+        raise ValueError("As, Bs and outs must have the same length")
+    arr = (_lib.Problem * max(len(As), 1))()
+    dtype = As[0].dtype if As else torch.float64
+    for i, (A, B, C) in enumerate(zip(As, Bs, Cs)):
+        _check_tall(A, "A")
+        _check_tall(B, "B")
+        if A.dtype != dtype or B.dtype != dtype or A.shape[0] != B.shape[0]:
+            raise ValueError(f"problem {i}: A and B must share the batch dtype and row count")
+        k, m = A.shape[1], B.shape[1]
+        if C.shape != (k, m) or C.dtype != Cs[0].dtype or C.dtype not in _OUT_TYPES \
+                or C.stride(1) != 1:
+            raise ValueError(f"problem {i}: out must be a row-major k x m float64/float32 tensor")
+        arr[i] = _lib.Problem(A.data_ptr(), B.data_ptr(), C.data_ptr(), A.shape[0], _ld(A), _ld(B),
+                              C.stride(0) if k > 1 else m, k, m)
+    return arr, dtype
+
+
+def atb_batched(As, Bs, *, out_dtype: torch.dtype = torch.float64, outs=None,
+                accumulate: bool = False):
+    """C_i = A_i^T B_i for every pair in ONE persistent launch; returns the C_i."""
+    if outs is None:
+        if accumulate:
+            raise ValueError("accumulate needs existing `outs`")
+        outs = [torch.empty((A.shape[1], B.shape[1]), dtype=out_dtype, device=A.device)
+                for A, B in zip(As, Bs)]
+    if not As:
+        return outs
+    arr, dtype = _problems(As, Bs, outs)
+    L = _lib.lib()
+    with torch.cuda.device(As[0].device):
+        st = L.tsm_atb_batched(_IN_TYPES[dtype], _OUT_TYPES[outs[0].dtype], len(As), arr,
+                               1 if accumulate else 0, None, 0, _stream())
+    _lib.check(st, "tsm_atb_batched")
+    return outs
+
+
+class BatchedPlan:
+    """A batched problem list uploaded once; each run() is one kernel launch
+    (no allocation or copy), so it can be captured in a HIP graph.  The
+    tensors must stay alive and in place for the plan's lifetime."""
+
+    def __init__(self, As, Bs, outs, *, accumulate: bool = False):
+        self._keep = (list(As), list(Bs), list(outs))
+        arr, dtype = _problems(As, Bs, outs)
+        self._h = ct.c_void_p()
+        L = _lib.lib()
+        with torch.cuda.device(As[0].device):
+            st = L.tsm_batched_plan_create(_IN_TYPES[dtype], _OUT_TYPES[outs[0].dtype], len(As),
+                                           arr, 1 if accumulate else 0, ct.byref(self._h))
+        _lib.check(st, "tsm_batched_plan_create")
+        items, grid = ct.c_int64(), ct.c_int64()
+        _lib.check(L.tsm_batched_plan_info(self._h, ct.byref(items), ct.byref(grid)),
+                   "tsm_batched_plan_info")
+        self.items, self.grid = items.value, grid.value
+
+    def run(self) -> None:
+        _lib.check(_lib.lib().tsm_batched_plan_execute(self._h, _stream()),
+                   "tsm_batched_plan_execute")
+
+    def close(self) -> None:
+        if self._h:
+            _lib.check(_lib.lib().tsm_batched_plan_destroy(self._h), "tsm_batched_plan_destroy")
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class _AtbFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, A, B, out_dtype):
