@@ -86,7 +86,8 @@ typedef struct {
  *             for every input type; FP32/BF16 inputs accumulate in FP32 in
  *             short row blocks folded into compensated (hi, lo) FP32 pairs.
  *   workspace: NULL to use a per-(device, stream) buffer owned by the library,
- *             or device memory of at least tsm_atb_workspace_size() bytes whose
+ *             or 256-byte aligned device memory of at least
+ *             tsm_atb_workspace_size() bytes whose
  *             first 256 bytes are zero before its first use (the library
  *             leaves them zero after each call).  Calls sharing a workspace
  *             must be ordered on one stream.

@@ -208,6 +208,7 @@ tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (workspace) {
     if (workspace_bytes < L.total) return TSM_ERR_WORKSPACE;
+    if ((uintptr_t)workspace % kAlign) return TSM_ERR_MISALIGNED;
   } else {
     st = tsm_internal_pooled_workspace(L.total, stream, &workspace);
     if (st != TSM_SUCCESS) return st;

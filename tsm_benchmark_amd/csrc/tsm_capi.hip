@@ -296,6 +296,7 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (workspace) {
     if (workspace_bytes < p.workspace) return TSM_ERR_WORKSPACE;
+    if ((uintptr_t)workspace % kCounterBytes) return TSM_ERR_MISALIGNED;
   } else {
     st = pooled_workspace(p.workspace, s, &workspace);
     if (st != TSM_SUCCESS) return st;

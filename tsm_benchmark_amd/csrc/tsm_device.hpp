@@ -206,23 +206,28 @@ struct WaveReduce {
 // Sums partials[g][o] for g in [0, G) in a fixed order for outputs o in
 // [o0, o1) of the KC x MC class, then writes C.  Called by one workgroup.
 // `scratch` must hold kThreads doubles of LDS.
+// Stores one reduced output (class index o) into C.
+__device__ __forceinline__ void store_output(const AtbArgs& a, int MC, int o, double total) {
+  const int i = o / MC, j = o % MC;
+  if (i < a.k && j < a.m) {
+    const int64_t off = (int64_t)i * a.ldc + j;
+    if (a.c_f32) {
+      float* C = static_cast<float*>(a.C);
+      C[off] = a.accumulate ? (float)((double)C[off] + total) : (float)total;
+    } else {
+      double* C = static_cast<double*>(a.C);
+      C[off] = a.accumulate ? C[off] + total : total;
+    }
+  }
+}
+
 __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC, int o0, int o1,
                                       double* scratch) {
   const int tid = threadIdx.x;
   const int nout = o1 - o0;
   const double* P = a.partials;
   auto store = [&](int o, double total) __attribute__((always_inline)) {
-    const int i = o / MC, j = o % MC;
-    if (i < a.k && j < a.m) {
-      const int64_t off = (int64_t)i * a.ldc + j;
-      if (a.c_f32) {
-        float* C = static_cast<float*>(a.C);
-        C[off] = a.accumulate ? (float)((double)C[off] + total) : (float)total;
-      } else {
-        double* C = static_cast<double*>(a.C);
-        C[off] = a.accumulate ? C[off] + total : total;
-      }
-    }
+    store_output(a, MC, o, total);
   };
   if (nout >= kThreads) {
     for (int o = o0 + tid; o < o1; o += kThreads) {
@@ -264,6 +269,51 @@ __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC,
   }
 }
 
+// Final reduction of G partials of a KCMC-output class by one workgroup, for
+// KCMC a power of two with 2 <= KCMC <= 512.  The G x KCMC partials are read
+// as 16-byte chunks c = tid + 256 * j, so every thread always meets the same two outputs
+// (2 * tid mod KCMC and the next one) and all of its loads are independent
+// (one memory round trip instead of a dependent chain).  Each thread sums its
+// chunks in j order; the 512 / KCMC threads that share a pair of outputs are
+// then added in thread order.  `scratch` holds 2 * kThreads doubles of LDS.
+template <int KCMC>
+__device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, double* scratch) {
+  static_assert(KCMC >= 2 && KCMC <= 512 && (KCMC & (KCMC - 1)) == 0, "2..512, power of two");
+  const int tid = threadIdx.x;
+  const v4u* P = reinterpret_cast<const v4u*>(a.partials);
+  const int64_t chunks = (int64_t)G * KCMC / 2;  // 16-byte chunks
+  double s0 = 0.0, s1 = 0.0;
+  constexpr int U = 8;
+  int64_t c = tid;
+  for (; c + (U - 1) * kThreads < chunks; c += U * kThreads) {
+    v4u u[U];
+#pragma unroll
+    for (int x = 0; x < U; ++x) u[x] = P[c + x * kThreads];
+#pragma unroll
+    for (int x = 0; x < U; ++x) {
+      s0 += __hiloint2double((int)u[x].y, (int)u[x].x);
+      s1 += __hiloint2double((int)u[x].w, (int)u[x].z);
+    }
+  }
+  for (; c < chunks; c += kThreads) {
+    const v4u u = P[c];
+    s0 += __hiloint2double((int)u.y, (int)u.x);
+    s1 += __hiloint2double((int)u.w, (int)u.z);
+  }
+  scratch[2 * tid] = s0;
+  scratch[2 * tid + 1] = s1;
+  __syncthreads();
+  constexpr int PAIRS = KCMC / 2;             // threads 0..PAIRS-1 hold distinct output pairs
+  constexpr int SHARE = kThreads / PAIRS;     // threads per output pair: tid, tid + PAIRS, ...
+  if (tid < KCMC) {
+    const int pair = tid / 2, half = tid % 2;
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < SHARE; ++q) t += scratch[2 * (pair + q * PAIRS) + half];
+    store_output(a, MC, tid, t);
+  }
+}
+
 // Finisher kernel body: workgroup b handles outputs [b*64, b*64+64).
 __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
   __shared__ double scratch[kThreads];
@@ -281,7 +331,7 @@ constexpr int atb_smem_bytes() {
   constexpr int stage2 = 2 * T * (KC + MC) * (int)sizeof(S);
   constexpr int red = KC * MC * 8;
   constexpr int m1 = stage2 > red ? stage2 : red;
-  return m1 > kThreads * 8 ? m1 : kThreads * 8;
+  return m1 > kThreads * 16 ? m1 : kThreads * 16;
 }
 
 // Streams a sequence of row tiles of one problem - the full tiles
@@ -552,7 +602,10 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
                                             tail_rows, a.partials + g * KCMC);
   if (!a.finish) return;
   if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
-  finish_outputs(a, (int)gridDim.x, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
+  if constexpr (KCMC >= 2 && KCMC <= 512)
+    finish_small<KCMC>(a, (int)gridDim.x, MC, reinterpret_cast<double*>(smem));
+  else
+    finish_outputs(a, (int)gridDim.x, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
   if (threadIdx.x == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
