@@ -96,8 +96,10 @@ def test_golden_fixtures(case):
 
 
 # ---- shapes and layouts -------------------------------------------------------------------
-SHAPES = [(1, 1), (2, 2), (3, 5), (8, 8), (4, 32), (32, 4), (16, 16), (33, 17), (64, 64), (1, 64),
-          (64, 1), (7, 9)]
+# every final-reduction regime: 1 output, 2..256 (one pair per thread or fewer),
+# 512 (two outputs per thread), > 512 (per-output sums)
+SHAPES = [(1, 1), (1, 2), (2, 2), (3, 5), (8, 8), (4, 32), (32, 4), (16, 16), (16, 32), (32, 16),
+          (8, 64), (33, 17), (64, 64), (1, 64), (64, 1), (7, 9)]
 NS = [0, 1, 2, 255, 256, 257, 4097, 100003]
 
 

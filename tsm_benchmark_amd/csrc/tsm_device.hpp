@@ -305,12 +305,12 @@ __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, do
   __syncthreads();
   constexpr int PAIRS = KCMC / 2;             // threads 0..PAIRS-1 hold distinct output pairs
   constexpr int SHARE = kThreads / PAIRS;     // threads per output pair: tid, tid + PAIRS, ...
-  if (tid < KCMC) {
-    const int pair = tid / 2, half = tid % 2;
+  for (int o = tid; o < KCMC; o += kThreads) {  // KCMC may exceed the 256 threads
+    const int pair = o / 2, half = o % 2;
     double t = 0.0;
 #pragma unroll
     for (int q = 0; q < SHARE; ++q) t += scratch[2 * (pair + q * PAIRS) + half];
-    store_output(a, MC, tid, t);
+    store_output(a, MC, o, t);
   }
 }
 
