@@ -58,6 +58,36 @@ def cpu_baseline(k: int, m: int, budget_s: float = 20.0) -> dict:
                        f"{runs} timed runs, mean {t * 1e3:.1f} ms")}
 
 
+def secondary() -> dict:
+    """The metric's second half and the other workloads, measured like the
+    headline (HIP-event medians, inputs resident):
+      low_precision: k = m = 32, n = 2^25, FP32 and BF16 storage vs FP64 - time,
+        speedup and relative error against the FP64 kernel on the same rounded
+        inputs (uniform and sign*2^e entries);
+      backward: FP32 n = 2^23, k = m = 16, (forward + backward) / forward;
+      batched: 512 pairs in one launch vs one launch per pair (FP64, FP32)."""
+    from tsm_benchmark_amd import suite
+
+    prec = suite.precision(10, False)
+    low = {"n": prec["n"], "k": prec["k"], "m": prec["m"]}
+    for dist in ("uniform", "pow2"):
+        row = prec[dist]
+        low[dist] = {"fp64_us": row["fp64_us"]}
+        for dt in ("f32", "bf16"):
+            low[dist][dt] = {key: row[dt][key] for key in
+                             ("us", "speedup_vs_fp64", "frob_rel_err", "max_err_over_absAB")}
+    bwd = suite.backward(10)
+    bat = suite.batched(5)
+    return {
+        "low_precision": low,
+        "backward": {key: bwd[key] for key in ("fwd_us", "bwd_us", "fwd_plus_bwd_over_fwd",
+                                               "byte_bound_ratio")},
+        "batched": {dt: {"one_launch_GBps": bat[dt]["graph_GBps"],
+                         "per_pair_launches_GBps": bat[dt]["per_pair_GBps"]}
+                    for dt in ("f64", "f32")},
+    }
+
+
 def pmc_traffic(workload: str):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(HERE, "profiles", "pmc_traffic.json")
@@ -72,12 +102,14 @@ def pmc_traffic(workload: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1 << 25)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--m", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the low-precision / backward / batched measurements")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -146,6 +178,9 @@ def main() -> int:
     }
     del A, B
     torch.cuda.empty_cache()
+    if not args.no_secondary:
+        rec["secondary"] = secondary()
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         t0 = time.time()
         rec["cpu_baseline"] = cpu_baseline(k, m)
