@@ -71,6 +71,8 @@ typedef struct {
   int load_policy;     /* 0 tuned default, 1 cached loads, 2 non-temporal (streaming) loads */
   int tile_order;      /* 0 tuned default, 1 contiguous runs of tiles per workgroup,
                           2 interleaved (tile t to workgroup t mod grid) */
+  int engine;          /* 0 tuned default, 1 VALU register tiles (tm x tn), 2 matrix cores
+                          (MFMA; width classes that are multiples of 16) */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------
@@ -171,7 +173,7 @@ typedef struct {
   const char* name;                 /* e.g. "tsm_atb_f64_8x8_t8x8_r256_p2" */
   int in_type;                      /* tsm_dtype */
   int kc, mc;                       /* width class: k <= kc, m <= mc */
-  int tm, tn, rows, pf;
+  int tm, tn, rows, pf;              /* tm = tn = 0: matrix-core (MFMA) kernel */
   int load_policy;                  /* 1 cached, 2 non-temporal */
   int lds_bytes;
 } tsm_kernel_desc;

@@ -37,11 +37,11 @@ struct dtype_of<float> {
 template <typename T>
 inline tsm_status hinted_call(tsm_stream_t q, int64_t K, int M, int N, const T* A, const T* B,
                               T* C, int tm, int tn, int pf, int64_t grid) {
-  tsm_kernel_config cfg{tm, tn, 0, pf, 0, grid > 0 ? grid : 0, 0, 0};
+  tsm_kernel_config cfg{tm, tn, 0, pf, 0, grid > 0 ? grid : 0, 0, 0, 0};
   tsm_status s = tsm_atb_ex(dtype_of<T>::in, dtype_of<T>::out, K, M, N, A, M, B, N, C, N, 1,
                             nullptr, 0, q, &cfg);
   if (s == TSM_ERR_NO_KERNEL) {  // hint not compiled in: tuned kernel, same grid
-    tsm_kernel_config dflt{0, 0, 0, 0, 0, cfg.grid, 0, 0};
+    tsm_kernel_config dflt{0, 0, 0, 0, 0, cfg.grid, 0, 0, 0};
     s = tsm_atb_ex(dtype_of<T>::in, dtype_of<T>::out, K, M, N, A, M, B, N, C, N, 1, nullptr, 0,
                    q, &dflt);
   }

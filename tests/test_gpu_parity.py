@@ -163,7 +163,9 @@ def test_accumulate_and_f32_output():
 
 
 @pytest.mark.parametrize("dt,k,m", [("f64", 8, 8), ("f64", 4, 32), ("f64", 64, 64),
-                                    ("f32", 32, 32), ("bf16", 32, 32), ("f64", 1, 1)])
+                                    ("f64", 32, 32), ("f64", 16, 16), ("f32", 32, 32),
+                                    ("f32", 64, 64), ("bf16", 32, 32), ("bf16", 64, 64),
+                                    ("bf16", 16, 16), ("f64", 1, 1)])
 def test_every_generated_variant(dt, k, m):
     dtype_id = {"f64": 0, "f32": 1, "bf16": 2}[dt]
     n = 123457
@@ -175,8 +177,7 @@ def test_every_generated_variant(dt, k, m):
             continue
         for grid in (0, 7):
             for il in (0, 1):
-                cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"],
-                                       nt=kd["nt"], grid=grid, interleave=il)
+                cfg = _lib.config_of(kd, grid=grid, interleave=il)
                 C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
                 check_close(C, Ah, Bh, DT[dt])
         seen += 1

@@ -33,7 +33,7 @@ class KernelConfig(ct.Structure):
     """tsm_kernel_config: zero fields take the tuned default."""
     _fields_ = [("tm", ct.c_int), ("tn", ct.c_int), ("rows", ct.c_int), ("pf", ct.c_int),
                 ("wgs_per_cu", ct.c_int), ("grid", ct.c_int64), ("load_policy", ct.c_int),
-                ("tile_order", ct.c_int)]
+                ("tile_order", ct.c_int), ("engine", ct.c_int)]
 
 
 class KernelDesc(ct.Structure):
@@ -141,12 +141,20 @@ def kernels() -> list[dict]:
 
 
 def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0, nt=None,
-                interleave=None) -> KernelConfig:
+                interleave=None, mfma=None) -> KernelConfig:
     """nt: None = tuned default, 0 = cached loads, 1 = non-temporal loads;
-    interleave: None = tuned default, 0 = contiguous tile runs, 1 = interleaved."""
+    interleave: None = tuned default, 0 = contiguous tile runs, 1 = interleaved;
+    mfma: None = tuned default, False = VALU register tiles, True = matrix cores."""
     policy = 0 if nt is None else (2 if nt else 1)
     order = 0 if interleave is None else (2 if interleave else 1)
-    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy, order)
+    engine = 0 if mfma is None else (2 if mfma else 1)
+    return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy, order, engine)
+
+
+def config_of(kd: dict, **kw) -> KernelConfig:
+    """Config that selects exactly the kernel kd (an entry of kernels())."""
+    return make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"], nt=kd["nt"],
+                       mfma=kd["tm"] == 0, **kw)
 
 
 def plan(in_type: int, n: int, k: int, m: int, config: KernelConfig | None = None):

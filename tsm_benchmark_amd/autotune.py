@@ -61,8 +61,7 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
         if kd["in_type"] != DTYPE_ID[dtype] or kd["kc"] != kc or kd["mc"] != mc:
             continue
         for mult, il in [(mu, il) for mu in GRID_MULTS for il in (0, 1)]:
-            cfg = _lib.make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"],
-                                   nt=kd["nt"], wgs_per_cu=mult, interleave=il)
+            cfg = _lib.config_of(kd, wgs_per_cu=mult, interleave=il)
             t = time_call(lambda: ops.atb(A, B, out=C, config=cfg), reps)
             rel = ((C - ref).norm() / ref.norm()).item()
             tol = 1e-13 if dtype == "f64" else 1e-6

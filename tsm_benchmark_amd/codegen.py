@@ -82,13 +82,24 @@ class KernelConfig:
     nt: int = 0  # 1: non-temporal (streaming) global loads
 
     @property
+    def mfma(self) -> bool:
+        """tm = tn = 0 selects the matrix-core consumer."""
+        return self.tm == 0 and self.tn == 0
+
+    @property
     def name(self) -> str:
-        return (f"tsm_atb_{self.dtype}_{self.kc}x{self.mc}_t{self.tm}x{self.tn}"
+        tile = "mfma" if self.mfma else f"t{self.tm}x{self.tn}"
+        return (f"tsm_atb_{self.dtype}_{self.kc}x{self.mc}_{tile}"
                 f"_r{self.rows}_p{self.pf}" + ("_nt" if self.nt else ""))
 
     @property
+    def kstep(self) -> int:
+        """Rows per matrix instruction (MFMA consumer)."""
+        return 32 if self.dtype == "bf16" else 4
+
+    @property
     def ntiles(self) -> int:
-        return (self.kc // self.tm) * (self.mc // self.tn)
+        return 1 if self.mfma else (self.kc // self.tm) * (self.mc // self.tn)
 
     @property
     def row_lanes(self) -> int:
@@ -110,6 +121,20 @@ class KernelConfig:
             p.append("dtype")
         if self.kc not in CLASSES or self.mc not in CLASSES:
             p.append("width class must be a power of two <= 64")
+            return p
+        if self.mfma:
+            if self.kc % 16 or self.mc % 16:
+                p.append("MFMA needs width classes that are multiples of 16")
+            if self.rows % (4 * self.kstep):
+                p.append(f"MFMA rows must be a multiple of {4 * self.kstep}")
+            if self.rows // (4 * self.kstep) > MAX_ROWS_PER_THREAD:
+                p.append("too many MFMA steps per tile")
+            if self.pf not in (1, 2) or self.nt not in (0, 1):
+                p.append("pf must be 1 or 2, nt 0 or 1")
+            if (self.rows * self.kc * es) % 16 or (self.rows * self.mc * es) % 16:
+                p.append("tiles must be whole 16-byte chunks")
+            if self.lds_bytes > LDS_MAX:
+                p.append("LDS budget")
             return p
         if self.tm < 1 or self.tn < 1 or self.kc % self.tm or self.mc % self.tn:
             p.append("tile must divide the width class")
@@ -142,6 +167,16 @@ def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig
     keep up to 8 x 4 because each entry also carries a (hi, lo) pair.  The
     row tile aims at 32 KiB per pipeline stage."""
     es = ELEM_BYTES[dtype]
+    if kc % 16 == 0 and mc % 16 == 0 and (dtype == "bf16" or kc * mc >= 1024):
+        # matrix cores: the register tiles would be VALU-bound at these widths
+        rows = _pow2_floor(32768 // ((kc + mc) * es))
+        c = KernelConfig(dtype, kc, mc, 0, 0, max(rows, 4 * (32 if dtype == "bf16" else 4)), pf)
+        while c.lds_bytes > LDS_BUDGET and c.rows > 4 * c.kstep:
+            c = dataclasses.replace(c, rows=c.rows // 2)
+        while c.rows // (4 * c.kstep) > MAX_ROWS_PER_THREAD:
+            c = dataclasses.replace(c, rows=c.rows // 2)
+        if not c.problems():
+            return c
     tm = min(kc, 8)
     tn = min(mc, 8 if dtype == "f64" else 4)
     if (kc // tm) * (mc // tn) > WAVE:
@@ -167,8 +202,8 @@ def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig
 # for the autotuner to choose from.
 TUNE_SHAPES = {
     "f64": [(1, 1), (2, 2), (4, 4), (8, 8), (16, 16), (32, 32), (64, 64), (4, 32), (32, 4)],
-    "f32": [(8, 8), (16, 16), (32, 32)],
-    "bf16": [(8, 8), (16, 16), (32, 32)],
+    "f32": [(8, 8), (16, 16), (32, 32), (64, 64)],
+    "bf16": [(8, 8), (16, 16), (32, 32), (64, 64)],
 }
 
 
@@ -178,6 +213,20 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
     for the default tile."""
     base = default_config(dtype, kc, mc)
     out = set()
+    if kc % 16 == 0 and mc % 16 == 0:
+        mrows = _pow2_floor(32768 // ((kc + mc) * ELEM_BYTES[dtype]))
+        for rows in (mrows // 2, mrows, mrows * 2):
+            for pf in (1, 2):
+                for nt in (0, 1):
+                    c = KernelConfig(dtype, kc, mc, 0, 0, rows, pf, nt)
+                    if not c.problems():
+                        out.add(c)
+        if base.mfma:
+            base = dataclasses.replace(base, tm=min(kc, 8), tn=min(mc, 8 if dtype == "f64" else 4))
+            if (kc // base.tm) * (mc // base.tn) > WAVE:
+                base = dataclasses.replace(base, tn=min(mc, 8))
+            rg = THREADS // base.ntiles
+            base = dataclasses.replace(base, rows=max(rg, min(base.rows, rg * 8)))
     tiles = {(base.tm, base.tn), (min(kc, 4), min(mc, 4)), (min(kc, 8), min(mc, 8))}
     for tm, tn in tiles:
         for rows in (base.rows // 2, base.rows, base.rows * 2):
@@ -250,10 +299,15 @@ BANNER = ("// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.\n"
 def emit_kernel(c: KernelConfig) -> str:
     t = CXX_TYPE[c.dtype]
     targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}, {c.nt}"
+    if c.mfma:
+        shape = (f"//   matrix cores: {c.rows // c.kstep} K-steps of {c.kstep} rows per tile, "
+                 f"LDS {c.lds_bytes} B\n")
+    else:
+        shape = (f"//   {c.ntiles} register tiles x {c.row_lanes} row lanes, "
+                 f"{c.rows // c.row_lanes} rows/thread/tile, LDS {c.lds_bytes} B\n")
     return (
         f"// {c.name}: KC={c.kc} MC={c.mc} TM={c.tm} TN={c.tn} rows={c.rows} pf={c.pf} nt={c.nt}\n"
-        f"//   {c.ntiles} register tiles x {c.row_lanes} row lanes, "
-        f"{c.rows // c.row_lanes} rows/thread/tile, LDS {c.lds_bytes} B\n"
+        + shape +
         f"extern \"C\" __global__ void __launch_bounds__(256) {c.name}(tsm::AtbArgs a) {{\n"
         f"  tsm::atb_body<{targs}>(a);\n"
         f"}}\n"

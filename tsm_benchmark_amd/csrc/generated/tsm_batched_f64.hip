@@ -44,20 +44,20 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 31: tsm::batched_item_call<double, 16, 8, 8, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x8_t8x8_r128_p1
       case 32: tsm::batched_item_call<double, 16, 16, 8, 8, 128, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x16_t8x8_r128_p1_nt
       case 33: tsm::batched_item_call<double, 16, 32, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x32_t8x8_r64_p1
-      case 34: tsm::batched_item_call<double, 16, 64, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x64_t8x8_r32_p1
+      case 34: tsm::batched_item_call<double, 16, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x64_mfma_r32_p2
       case 35: tsm::batched_item_call<double, 32, 1, 8, 1, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x1_t8x1_r64_p2
       case 36: tsm::batched_item_call<double, 32, 2, 8, 2, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x2_t8x2_r64_p2
       case 37: tsm::batched_item_call<double, 32, 4, 8, 4, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x4_t8x4_r64_p1
       case 38: tsm::batched_item_call<double, 32, 8, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x8_t8x8_r64_p1
       case 39: tsm::batched_item_call<double, 32, 16, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x16_t8x8_r64_p1
       case 40: tsm::batched_item_call<double, 32, 32, 8, 8, 64, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x32_t8x8_r64_p1_nt
-      case 41: tsm::batched_item_call<double, 32, 64, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x64_t8x8_r32_p1
+      case 41: tsm::batched_item_call<double, 32, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x64_mfma_r32_p2
       case 42: tsm::batched_item_call<double, 64, 1, 8, 1, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x1_t8x1_r32_p2
       case 43: tsm::batched_item_call<double, 64, 2, 8, 2, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x2_t8x2_r32_p2
       case 44: tsm::batched_item_call<double, 64, 4, 8, 4, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x4_t8x4_r32_p1
       case 45: tsm::batched_item_call<double, 64, 8, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x8_t8x8_r32_p1
-      case 46: tsm::batched_item_call<double, 64, 16, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x16_t8x8_r32_p1
-      case 47: tsm::batched_item_call<double, 64, 32, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x32_t8x8_r32_p1
+      case 46: tsm::batched_item_call<double, 64, 16, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x16_mfma_r32_p2
+      case 47: tsm::batched_item_call<double, 64, 32, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x32_mfma_r32_p2
       case 48: tsm::batched_item_call<double, 64, 64, 8, 8, 16, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x64_t8x8_r16_p1_nt
       default: break;
     }

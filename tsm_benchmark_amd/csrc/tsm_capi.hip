@@ -88,23 +88,32 @@ tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_confi
   *wgs_per_cu = (cfg && cfg->wgs_per_cu > 0) ? cfg->wgs_per_cu : def_wpc;
   *interleave = (cfg && cfg->tile_order > 0) ? (cfg->tile_order == 2 ? 1 : 0) : def_il;
   if (!cfg || (cfg->tm == 0 && cfg->tn == 0 && cfg->rows == 0 && cfg->pf == 0 &&
-               cfg->load_policy == 0)) {
+               cfg->load_policy == 0 && cfg->engine == 0)) {
     *index = def;
     return TSM_SUCCESS;
   }
-  // Explicit request: every non-zero field must match; zero fields take the
-  // tuned default's value.
+  // Explicit request: every non-zero field filters the class's kernels; among
+  // the matches the one closest to the tuned default wins (lowest index on
+  // ties), so the choice is deterministic.
   const tsm::KernelEntry& d = ks[def];
-  const int tm = cfg->tm ? cfg->tm : d.tm, tn = cfg->tn ? cfg->tn : d.tn;
-  const int pf = cfg->pf ? cfg->pf : d.pf;
-  const int nt = cfg->load_policy ? (cfg->load_policy == 2 ? 1 : 0) : d.nt;
-  int best = -1;
+  int best = -1, best_score = -1;
   for (int i = 0; i < nk; ++i) {
     const tsm::KernelEntry& e = ks[i];
-    if (e.dtype != (int)in_type || e.kc != kc || e.mc != mc || e.tm != tm || e.tn != tn ||
-        e.pf != pf || e.nt != nt)
-      continue;
-    if (cfg->rows ? e.rows == cfg->rows : (best < 0 || e.rows == d.rows)) best = i;
+    if (e.dtype != (int)in_type || e.kc != kc || e.mc != mc) continue;
+    const bool mfma = e.tm == 0;
+    if (cfg->engine == 1 && mfma) continue;
+    if (cfg->engine == 2 && !mfma) continue;
+    if (cfg->tm > 0 && e.tm != cfg->tm) continue;
+    if (cfg->tn > 0 && e.tn != cfg->tn) continue;
+    if (cfg->rows > 0 && e.rows != cfg->rows) continue;
+    if (cfg->pf > 0 && e.pf != cfg->pf) continue;
+    if (cfg->load_policy > 0 && e.nt != (cfg->load_policy == 2 ? 1 : 0)) continue;
+    const int score = (e.tm == d.tm) + (e.tn == d.tn) + (e.rows == d.rows) + (e.pf == d.pf) +
+                      (e.nt == d.nt);
+    if (score > best_score) {
+      best = i;
+      best_score = score;
+    }
   }
   if (best < 0) return TSM_ERR_NO_KERNEL;
   *index = best;
@@ -288,7 +297,8 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   if (config && (config->pf < 0 || config->pf > 2 || config->tm < 0 || config->tn < 0 ||
                  config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0 ||
                  config->load_policy < 0 || config->load_policy > 2 ||
-                 config->tile_order < 0 || config->tile_order > 2))
+                 config->tile_order < 0 || config->tile_order > 2 || config->engine < 0 ||
+                 config->engine > 2))
     return TSM_ERR_INVALID_ARGUMENT;
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);

@@ -323,6 +323,150 @@ __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
   if (o0 < KCMC) finish_outputs(a, G, KCMC, MC, o0, o1, scratch);
 }
 
+// ---- MFMA consumer (width classes that are multiples of 16) -------------------------
+// The k x m result is a grid of 16 x 16 blocks, each accumulated by one
+// matrix instruction per K-step of rows:
+//   FP64: v_mfma_f64_16x16x4f64   4 rows / step, FP64 accumulators
+//   FP32: v_mfma_f32_16x16x4f32   4 rows / step, FP32 accumulators
+//   BF16: v_mfma_f32_16x16x32_bf16 32 rows / step, FP32 accumulators
+// Operands come from the row-major LDS tile: lane l supplies column
+// 16*block + (l % 16) of the rows that the step assigns to lane group l / 16
+// (the same rows for A and B, so any row order within a step is a valid
+// contraction).  Output layout of one block (measured on gfx950): lane l,
+// register v holds row 4*(l/16) + v (FP32, BF16) or row l/16 + 4*v (FP64),
+// column l % 16.  Wave w takes the steps w, w + 4, ... of every tile, so each
+// wave ends with a full KC x MC partial; FP32 accumulators are folded into
+// (hi, lo) pairs after every tile (32 rows per wave or fewer).
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <class S>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<double> {
+  using AccV = v4d;
+  static constexpr int KSTEP = 4;
+  static constexpr bool row_major_out = false;  // row l/16 + 4v
+};
+template <>
+struct MfmaTraits<float> {
+  using AccV = v4f;
+  static constexpr int KSTEP = 4;
+  static constexpr bool row_major_out = true;  // row 4*(l/16) + v
+};
+template <>
+struct MfmaTraits<bf16> {
+  using AccV = v4f;
+  static constexpr int KSTEP = 32;
+  static constexpr bool row_major_out = true;
+};
+
+// Operand of one 16-wide block for one K-step.
+__device__ __forceinline__ double mfma_operand(const double* X, int ld, int row0, int blk,
+                                               int lane) {
+  return X[(row0 + (lane >> 4)) * ld + blk * 16 + (lane & 15)];
+}
+__device__ __forceinline__ float mfma_operand(const float* X, int ld, int row0, int blk,
+                                              int lane) {
+  return X[(row0 + (lane >> 4)) * ld + blk * 16 + (lane & 15)];
+}
+__device__ __forceinline__ v8bf mfma_operand(const bf16* X, int ld, int row0, int blk,
+                                             int lane) {
+  // rows row0 + 8 * (l / 16) + t, t = 0..7, of column 16 * blk + l % 16
+  const uint16_t* p = reinterpret_cast<const uint16_t*>(X) + (row0 + 8 * (lane >> 4)) * ld +
+                      blk * 16 + (lane & 15);
+  uint32_t w[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    w[t] = (uint32_t)p[(2 * t) * ld] | ((uint32_t)p[(2 * t + 1) * ld] << 16);
+  return __builtin_bit_cast(v8bf, w);
+}
+__device__ __forceinline__ v4d mfma_step(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_step(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_step(v8bf a, v8bf b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <class S, int KC, int MC, int T>
+struct MfmaAcc {
+  using Tr = MfmaTraits<S>;
+  using AccV = typename Tr::AccV;
+  static constexpr bool kComp = Traits<S>::compensated;
+  static constexpr int IB = KC / 16, JB = MC / 16, KSTEP = Tr::KSTEP;
+  static constexpr int NSTEP = T / KSTEP;
+  static_assert(KC % 16 == 0 && MC % 16 == 0, "MFMA needs widths that are multiples of 16");
+  static_assert(NSTEP % 4 == 0, "each of the 4 waves takes the same number of steps");
+  AccV acc[IB][JB];
+  AccV hi[kComp ? IB : 1][kComp ? JB : 1];
+  AccV lo[kComp ? IB : 1][kComp ? JB : 1];
+
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        acc[i][j] = AccV{};
+        if constexpr (kComp) hi[i][j] = lo[i][j] = AccV{};
+      }
+  }
+  __device__ __forceinline__ void consume(const unsigned char* stage, int wave, int lane) {
+    const S* As = reinterpret_cast<const S*>(stage);
+    const S* Bs = reinterpret_cast<const S*>(stage + T * KC * (int)sizeof(S));
+#pragma unroll
+    for (int s = 0; s < NSTEP / 4; ++s) {
+      const int row0 = (wave + 4 * s) * KSTEP;
+      decltype(mfma_operand(As, KC, 0, 0, 0)) av[IB], bv[JB];
+#pragma unroll
+      for (int i = 0; i < IB; ++i) av[i] = mfma_operand(As, KC, row0, i, lane);
+#pragma unroll
+      for (int j = 0; j < JB; ++j) bv[j] = mfma_operand(Bs, MC, row0, j, lane);
+#pragma unroll
+      for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j) acc[i][j] = mfma_step(av[i], bv[j], acc[i][j]);
+    }
+  }
+  __device__ __forceinline__ void fold() {
+    if constexpr (kComp) {
+#pragma unroll
+      for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int j = 0; j < JB; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float s, e;
+            two_sum(hi[i][j][v], acc[i][j][v], s, e);
+            hi[i][j][v] = s;
+            lo[i][j][v] += e;
+            acc[i][j][v] = 0.0f;
+          }
+    }
+  }
+  // Adds this wave's KC x MC partial into red[] (FP64, row-major KC x MC).
+  __device__ __forceinline__ void emit(double* red, bool first, int lane) const {
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+      for (int j = 0; j < JB; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = Tr::row_major_out ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v;
+          const int o = (i * 16 + r) * MC + j * 16 + (lane & 15);
+          double x;
+          if constexpr (kComp)
+            x = (double)hi[i][j][v] + (double)lo[i][j][v];
+          else
+            x = acc[i][j][v];
+          red[o] = first ? x : red[o] + x;
+        }
+  }
+};
+
 // ---- the streaming kernel body -----------------------------------------------------
 // LDS bytes a configuration needs: two pipeline stages, the KC x MC FP64
 // reduction buffer or the final-reduction scratch, whichever is largest.
@@ -334,54 +478,58 @@ constexpr int atb_smem_bytes() {
   return m1 > kThreads * 16 ? m1 : kThreads * 16;
 }
 
-// Streams a sequence of row tiles of one problem - the full tiles
-// first + j * step (j < count) and, if owns_tail, the partial tile of
-// tail_rows rows that starts at row full * T - and writes this workgroup's
-// KC x MC FP64 partial sum to partial_out (global memory).  smem must hold
-// atb_smem_bytes<S, KC, MC, T>() bytes.
-template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
-__device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* smem, int64_t first,
-                                            int64_t step, int64_t count, bool owns_tail,
-                                            int64_t full, int tail_rows, double* partial_out) {
+// Register-tile consumer (VALU): the k x m result (padded to KC x MC) is cut
+// into NTILES = (KC/TM) * (MC/TN) register tiles; thread t owns tile
+// p = t % NTILES (A columns [mb*TM, mb*TM+TM), B columns [nb*TN, nb*TN+TN))
+// and the rows q, q + RG, q + 2RG, ... (q = t / NTILES, RG = 256 / NTILES) of
+// every tile.  FP32/BF16 block sums are folded into (hi, lo) about every 32
+// rows.
+template <class S, int KC, int MC, int TM, int TN, int T>
+struct RegTileAcc {
   using Acc = typename Traits<S>::Acc;
-  constexpr bool kComp = Traits<S>::compensated;
-  constexpr int ES = (int)sizeof(S);
-  constexpr int MB = KC / TM, NB = MC / TN, NTILES = MB * NB;
+  static constexpr bool kComp = Traits<S>::compensated;
+  static constexpr int MB = KC / TM, NB = MC / TN, NTILES = MB * NB;
   static_assert(KC % TM == 0 && MC % TN == 0, "tile must divide the width class");
   static_assert(NTILES >= 1 && NTILES <= 64 && 64 % NTILES == 0, "1..64 register tiles");
-  constexpr int RG = kThreads / NTILES;  // row lanes
+  static constexpr int RG = kThreads / NTILES;  // row lanes
   static_assert(T % RG == 0, "tile rows must be a multiple of the row lanes");
-  constexpr int RPT = T / RG;            // rows per thread per tile
-  constexpr int A_BYTES = T * KC * ES, B_BYTES = T * MC * ES;
-  static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
-  constexpr int A_VEC = A_BYTES / 16, B_VEC = B_BYTES / 16;
-  constexpr int A_PER = (A_VEC + kThreads - 1) / kThreads;
-  constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
-  constexpr int STAGE = A_BYTES + B_BYTES;
-  constexpr int KCMC = KC * MC;
-  static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
-  // Fold FP32 block sums into (hi, lo) every FOLD_TILES tiles (about 32 rows).
-  constexpr int FOLD_TILES = RPT >= 32 ? 1 : 32 / RPT;
-
-  const int tid = threadIdx.x;
-  const int p = tid % NTILES, q = tid / NTILES;
-  const int mb = p % MB, nb = p / MB;
-
+  static constexpr int RPT = T / RG;            // rows per thread per tile
+  static constexpr int FOLD_TILES = RPT >= 32 ? 1 : 32 / RPT;
+  int p, q, mb, nb, fold_count;
   Acc acc[TM][TN];
   Acc hi[kComp ? TM : 1][kComp ? TN : 1];
   Acc lo[kComp ? TM : 1][kComp ? TN : 1];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = Acc(0);
-  if constexpr (kComp) {
+
+  __device__ __forceinline__ void zero(int tid) {
+    p = tid % NTILES;
+    q = tid / NTILES;
+    mb = p % MB;
+    nb = p / MB;
+    fold_count = 0;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int j = 0; j < TN; ++j) hi[i][j] = lo[i][j] = Acc(0);
+      for (int j = 0; j < TN; ++j) {
+        acc[i][j] = Acc(0);
+        if constexpr (kComp) hi[i][j] = lo[i][j] = Acc(0);
+      }
   }
-  int fold_count = 0;
-  auto fold = [&]() __attribute__((always_inline)) {
+  __device__ __forceinline__ void consume(const unsigned char* stage) {
+    const S* As = reinterpret_cast<const S*>(stage);
+    const S* Bs = reinterpret_cast<const S*>(stage + T * KC * (int)sizeof(S));
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int r = q + s * RG;
+      Acc av[TM], bv[TN];
+      lds_read<S, TM>(As + r * KC + mb * TM, av);
+      lds_read<S, TN>(Bs + r * MC + nb * TN, bv);
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = fmadd(av[i], bv[j], acc[i][j]);
+    }
+  }
+  __device__ __forceinline__ void fold() {
     if constexpr (kComp) {
 #pragma unroll
       for (int i = 0; i < TM; ++i)
@@ -394,32 +542,107 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
           acc[i][j] = 0.0f;
         }
     }
-  };
-  auto after_tile = [&]() __attribute__((always_inline)) {
+  }
+  __device__ __forceinline__ void after_tile() {
     if constexpr (kComp) {
       if (++fold_count == FOLD_TILES) {
         fold();
         fold_count = 0;
       }
     }
-  };
-
-  // Consume one LDS stage holding `T` rows (rows past the data are zero).
-  auto compute = [&](const unsigned char* stage) __attribute__((always_inline)) {
-    const S* As = reinterpret_cast<const S*>(stage);
-    const S* Bs = reinterpret_cast<const S*>(stage + A_BYTES);
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const int r = q + s * RG;
-      Acc av[TM], bv[TN];
-      lds_read<S, TM>(As + r * KC + mb * TM, av);
-      lds_read<S, TN>(Bs + r * MC + nb * TN, bv);
+  }
+  // Workgroup reduction into red[] (FP64 KC x MC in LDS); ends with a barrier.
+  __device__ __forceinline__ void reduce(double* red, int tid) {
+    constexpr int V = TM * TN;
+    double v[V];
+    if constexpr (kComp) {
+      fold();
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = fmadd(av[i], bv[j], acc[i][j]);
+        for (int j = 0; j < TN; ++j) v[i * TN + j] = (double)hi[i][j] + (double)lo[i][j];
+    } else {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) v[i * TN + j] = acc[i][j];
     }
-  };
+    using WR = WaveReduce<NTILES, V>;
+    const int lane = tid & 63, wave = tid >> 6;
+    WR::run(v, lane);
+    __syncthreads();  // every wave is done with the LDS tiles
+    const bool writer = WR::writer(lane);
+    const int vbase = WR::base(lane);
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      if (wave == w && writer) {
+#pragma unroll
+        for (int x = 0; x < WR::VL; ++x) {
+          const int idx = vbase + x;
+          const int o = (mb * TM + idx / TN) * MC + nb * TN + idx % TN;
+          red[o] = (w == 0) ? v[x] : red[o] + v[x];
+        }
+      }
+      __syncthreads();
+    }
+  }
+};
+
+// The MFMA consumer with the same interface (TM = TN = 0 selects it).
+template <class S, int KC, int MC, int T>
+struct MfmaConsumer {
+  MfmaAcc<S, KC, MC, T> m;
+  int wave, lane;
+  __device__ __forceinline__ void zero(int tid) {
+    wave = tid >> 6;
+    lane = tid & 63;
+    m.zero();
+  }
+  __device__ __forceinline__ void consume(const unsigned char* stage) {
+    m.consume(stage, wave, lane);
+  }
+  __device__ __forceinline__ void after_tile() { m.fold(); }
+  __device__ __forceinline__ void reduce(double* red, int) {
+    m.fold();
+    __syncthreads();  // every wave is done with the LDS tiles
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      if (wave == w) m.emit(red, w == 0, lane);
+      __syncthreads();
+    }
+  }
+};
+
+template <class S, int KC, int MC, int TM, int TN, int T>
+struct ConsumerOf {
+  using type = RegTileAcc<S, KC, MC, TM, TN, T>;
+};
+template <class S, int KC, int MC, int T>
+struct ConsumerOf<S, KC, MC, 0, 0, T> {
+  using type = MfmaConsumer<S, KC, MC, T>;
+};
+
+// Streams a sequence of row tiles of one problem - the full tiles
+// first + j * step (j < count) and, if owns_tail, the partial tile of
+// tail_rows rows that starts at row full * T - and writes this workgroup's
+// KC x MC FP64 partial sum to partial_out (global memory).  smem must hold
+// atb_smem_bytes<S, KC, MC, T>() bytes.  TM = TN = 0 selects the MFMA consumer.
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+__device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* smem, int64_t first,
+                                            int64_t step, int64_t count, bool owns_tail,
+                                            int64_t full, int tail_rows, double* partial_out) {
+  constexpr int ES = (int)sizeof(S);
+  constexpr int A_BYTES = T * KC * ES, B_BYTES = T * MC * ES;
+  static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
+  constexpr int A_VEC = A_BYTES / 16, B_VEC = B_BYTES / 16;
+  constexpr int A_PER = (A_VEC + kThreads - 1) / kThreads;
+  constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
+  constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int KCMC = KC * MC;
+  static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
+  const int tid = threadIdx.x;
+  typename ConsumerOf<S, KC, MC, TM, TN, T>::type cons;
+  cons.zero(tid);
 
   const int64_t nfull = a.dense ? count : 0;
 
@@ -477,8 +700,8 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       __syncthreads();
       load(r, tt + PF < last ? tt + PF : last);
       __builtin_amdgcn_sched_barrier(0);
-      compute(stage);
-      after_tile();
+      cons.consume(stage);
+      cons.after_tile();
     };
     Regs r0, r1;
     load(r0, 0);
@@ -515,48 +738,16 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       lb[e] = v;
     }
     __syncthreads();
-    compute(smem);
-    after_tile();
+    cons.consume(smem);
+    cons.after_tile();
   };
   if (!a.dense)
     for (int64_t j = 0; j < count; ++j) elem_tile((first + j * step) * T, T);
   if (owns_tail) elem_tile(full * T, tail_rows);
 
   // ---- reduction -------------------------------------------------------------------
-  constexpr int V = TM * TN;
-  double v[V];
-  if constexpr (kComp) {
-    fold();
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) v[i * TN + j] = (double)hi[i][j] + (double)lo[i][j];
-  } else {
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int j = 0; j < TN; ++j) v[i * TN + j] = acc[i][j];
-  }
-  using WR = WaveReduce<NTILES, V>;
-  const int lane = tid & 63, wave = tid >> 6;
-  WR::run(v, lane);
-
-  __syncthreads();  // every wave is done with the LDS tiles
   double* red = reinterpret_cast<double*>(smem);
-  const bool writer = WR::writer(lane);
-  const int vbase = WR::base(lane);
-#pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) {
-    if (wave == w && writer) {
-#pragma unroll
-      for (int x = 0; x < WR::VL; ++x) {
-        const int idx = vbase + x;
-        const int o = (mb * TM + idx / TN) * MC + nb * TN + idx % TN;
-        red[o] = (w == 0) ? v[x] : red[o] + v[x];
-      }
-    }
-    __syncthreads();
-  }
+  cons.reduce(red, tid);
   for (int o = tid; o < KCMC; o += kThreads) partial_out[o] = red[o];
 }
 
