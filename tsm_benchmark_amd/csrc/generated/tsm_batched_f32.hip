@@ -45,7 +45,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f32(tsm::Batch
       case 32: tsm::batched_item_call<float, 32, 4, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x4_t8x4_r128_p1
       case 33: tsm::batched_item_call<float, 32, 8, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x8_t8x4_r128_p1
       case 34: tsm::batched_item_call<float, 32, 16, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x16_t8x4_r128_p1
-      case 35: tsm::batched_item_call<float, 32, 32, 8, 4, 128, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x32_t8x4_r128_p1_nt
+      case 35: tsm::batched_item_call<float, 32, 32, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x32_mfma_r128_p2_nt
       default: break;
     }
   }

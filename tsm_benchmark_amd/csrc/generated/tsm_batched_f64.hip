@@ -18,7 +18,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 5: tsm::batched_item_call<double, 1, 32, 1, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_1x32_t1x8_r64_p2
       case 6: tsm::batched_item_call<double, 1, 64, 1, 8, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_1x64_t1x8_r32_p2
       case 7: tsm::batched_item_call<double, 2, 1, 2, 1, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x1_t2x1_r1024_p2
-      case 8: tsm::batched_item_call<double, 2, 2, 2, 2, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x2_t2x2_r1024_p2
+      case 8: tsm::batched_item_call<double, 2, 2, 2, 2, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x2_t2x2_r512_p2_nt
       case 9: tsm::batched_item_call<double, 2, 4, 2, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x4_t2x4_r512_p2
       case 10: tsm::batched_item_call<double, 2, 8, 2, 8, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x8_t2x8_r256_p2
       case 11: tsm::batched_item_call<double, 2, 16, 2, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x16_t2x8_r128_p2
@@ -26,7 +26,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 13: tsm::batched_item_call<double, 2, 64, 2, 8, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_2x64_t2x8_r32_p2
       case 14: tsm::batched_item_call<double, 4, 1, 4, 1, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x1_t4x1_r512_p2
       case 15: tsm::batched_item_call<double, 4, 2, 4, 2, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x2_t4x2_r512_p2
-      case 16: tsm::batched_item_call<double, 4, 4, 4, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x4_t4x4_r512_p2_nt
+      case 16: tsm::batched_item_call<double, 4, 4, 4, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x4_t4x4_r512_p2
       case 17: tsm::batched_item_call<double, 4, 8, 4, 8, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x8_t4x8_r256_p1
       case 18: tsm::batched_item_call<double, 4, 16, 4, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x16_t4x8_r128_p1
       case 19: tsm::batched_item_call<double, 4, 32, 4, 8, 64, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
@@ -50,7 +50,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 37: tsm::batched_item_call<double, 32, 4, 8, 4, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x4_t8x4_r64_p1
       case 38: tsm::batched_item_call<double, 32, 8, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x8_t8x8_r64_p1
       case 39: tsm::batched_item_call<double, 32, 16, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x16_t8x8_r64_p1
-      case 40: tsm::batched_item_call<double, 32, 32, 8, 8, 64, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x32_t8x8_r64_p1_nt
+      case 40: tsm::batched_item_call<double, 32, 32, 0, 0, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x32_mfma_r64_p2
       case 41: tsm::batched_item_call<double, 32, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x64_mfma_r32_p2
       case 42: tsm::batched_item_call<double, 64, 1, 8, 1, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x1_t8x1_r32_p2
       case 43: tsm::batched_item_call<double, 64, 2, 8, 2, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x2_t8x2_r32_p2
@@ -58,7 +58,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 45: tsm::batched_item_call<double, 64, 8, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x8_t8x8_r32_p1
       case 46: tsm::batched_item_call<double, 64, 16, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x16_mfma_r32_p2
       case 47: tsm::batched_item_call<double, 64, 32, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x32_mfma_r32_p2
-      case 48: tsm::batched_item_call<double, 64, 64, 8, 8, 16, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x64_t8x8_r16_p1_nt
+      case 48: tsm::batched_item_call<double, 64, 64, 0, 0, 32, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x64_mfma_r32_p2_nt
       default: break;
     }
   }
@@ -73,7 +73,7 @@ static const tsm::BatchClass kClasses[] = {
   {1, 32, 64},
   {1, 64, 32},
   {2, 1, 1024},
-  {2, 2, 1024},
+  {2, 2, 512},
   {2, 4, 512},
   {2, 8, 256},
   {2, 16, 128},
@@ -113,7 +113,7 @@ static const tsm::BatchClass kClasses[] = {
   {64, 8, 32},
   {64, 16, 32},
   {64, 32, 32},
-  {64, 64, 16},
+  {64, 64, 32},
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs& a, unsigned grid, hipStream_t s) {

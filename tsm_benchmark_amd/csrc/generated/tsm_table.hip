@@ -1076,7 +1076,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 16, 2, 50, 2, 0},  // tsm_atb_bf16_16x2_t8x2_r512_p2
   {2, 16, 4, 51, 2, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2
   {2, 16, 8, 52, 2, 0},  // tsm_atb_bf16_16x8_t8x4_r512_p2
-  {2, 16, 16, 76, 2, 0},  // tsm_atb_bf16_16x16_t8x4_r256_p2_nt
+  {2, 16, 16, 56, 1, 1},  // tsm_atb_bf16_16x16_mfma_r256_p2_nt
   {2, 16, 32, 87, 2, 0},  // tsm_atb_bf16_16x32_mfma_r256_p2
   {2, 16, 64, 88, 2, 0},  // tsm_atb_bf16_16x64_mfma_r128_p2
   {2, 32, 1, 89, 2, 0},  // tsm_atb_bf16_32x1_t8x1_r256_p2
@@ -1084,7 +1084,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 32, 4, 91, 2, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
   {2, 32, 8, 92, 2, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
   {2, 32, 16, 93, 2, 0},  // tsm_atb_bf16_32x16_mfma_r256_p2
-  {2, 32, 32, 126, 2, 1},  // tsm_atb_bf16_32x32_t8x8_r128_p1
+  {2, 32, 32, 101, 1, 1},  // tsm_atb_bf16_32x32_mfma_r256_p2_nt
   {2, 32, 64, 128, 2, 0},  // tsm_atb_bf16_32x64_mfma_r128_p2
   {2, 64, 1, 129, 2, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
   {2, 64, 2, 130, 2, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
@@ -1092,7 +1092,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 64, 8, 132, 2, 0},  // tsm_atb_bf16_64x8_t8x4_r128_p2
   {2, 64, 16, 133, 2, 0},  // tsm_atb_bf16_64x16_mfma_r128_p2
   {2, 64, 32, 134, 2, 0},  // tsm_atb_bf16_64x32_mfma_r128_p2
-  {2, 64, 64, 137, 2, 0},  // tsm_atb_bf16_64x64_mfma_r128_p2
+  {2, 64, 64, 140, 1, 1},  // tsm_atb_bf16_64x64_mfma_r256_p1_nt
   {1, 1, 1, 155, 2, 0},  // tsm_atb_f32_1x1_t1x1_r4096_p2
   {1, 1, 2, 156, 2, 0},  // tsm_atb_f32_1x2_t1x2_r2048_p2
   {1, 1, 4, 157, 2, 0},  // tsm_atb_f32_1x4_t1x4_r1024_p2
@@ -1133,7 +1133,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 32, 4, 244, 2, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
   {1, 32, 8, 245, 2, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
   {1, 32, 16, 246, 2, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
-  {1, 32, 32, 272, 2, 1},  // tsm_atb_f32_32x32_t8x4_r128_p1_nt
+  {1, 32, 32, 254, 1, 0},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
   {1, 32, 64, 281, 2, 0},  // tsm_atb_f32_32x64_mfma_r64_p2
   {1, 64, 1, 282, 2, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
   {1, 64, 2, 283, 2, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
@@ -1141,7 +1141,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 64, 8, 285, 2, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
   {1, 64, 16, 286, 2, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
   {1, 64, 32, 287, 2, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
-  {1, 64, 64, 294, 2, 0},  // tsm_atb_f32_64x64_mfma_r64_p2
+  {1, 64, 64, 297, 1, 0},  // tsm_atb_f32_64x64_mfma_r128_p1_nt
   {0, 1, 1, 315, 1, 1},  // tsm_atb_f64_1x1_t1x1_r1024_p2_nt
   {0, 1, 2, 324, 2, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
   {0, 1, 4, 325, 2, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
@@ -1150,7 +1150,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 1, 32, 328, 2, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
   {0, 1, 64, 329, 2, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
   {0, 2, 1, 330, 2, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
-  {0, 2, 2, 340, 1, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
+  {0, 2, 2, 334, 1, 1},  // tsm_atb_f64_2x2_t2x2_r512_p2_nt
   {0, 2, 4, 343, 2, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
   {0, 2, 8, 344, 2, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
   {0, 2, 16, 345, 2, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
@@ -1158,7 +1158,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 2, 64, 347, 2, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
   {0, 4, 1, 348, 2, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
   {0, 4, 2, 349, 2, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
-  {0, 4, 4, 359, 1, 1},  // tsm_atb_f64_4x4_t4x4_r512_p2_nt
+  {0, 4, 4, 361, 1, 1},  // tsm_atb_f64_4x4_t4x4_r1024_p1_nt
   {0, 4, 8, 364, 2, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
   {0, 4, 16, 365, 2, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
   {0, 4, 32, 375, 1, 1},  // tsm_atb_f64_4x32_t4x8_r64_p2_nt
@@ -1182,7 +1182,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 32, 4, 456, 1, 1},  // tsm_atb_f64_32x4_t8x4_r128_p2_nt
   {0, 32, 8, 457, 2, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
   {0, 32, 16, 458, 2, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 480, 1, 1},  // tsm_atb_f64_32x32_t8x8_r64_p1_nt
+  {0, 32, 32, 470, 1, 1},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
   {0, 32, 64, 487, 2, 0},  // tsm_atb_f64_32x64_mfma_r32_p2
   {0, 64, 1, 488, 2, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
   {0, 64, 2, 489, 2, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
@@ -1190,7 +1190,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 64, 8, 491, 2, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
   {0, 64, 16, 492, 2, 0},  // tsm_atb_f64_64x16_mfma_r32_p2
   {0, 64, 32, 493, 2, 0},  // tsm_atb_f64_64x32_mfma_r32_p2
-  {0, 64, 64, 509, 2, 1},  // tsm_atb_f64_64x64_t8x8_r16_p2_nt
+  {0, 64, 64, 501, 1, 1},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
