@@ -382,6 +382,12 @@ __device__ __forceinline__ v8bf mfma_operand(const bf16* X, int ld, int row0, in
     w[t] = (uint32_t)p[(2 * t) * ld] | ((uint32_t)p[(2 * t + 1) * ld] << 16);
   return __builtin_bit_cast(v8bf, w);
 }
+// -x for eight packed BF16 values (flips the sign bits).
+__device__ __forceinline__ v8bf mfma_negate(v8bf x) {
+  v4u w = __builtin_bit_cast(v4u, x);
+  w ^= 0x80008000u;
+  return __builtin_bit_cast(v8bf, w);
+}
 __device__ __forceinline__ v4d mfma_step(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -399,9 +405,18 @@ struct MfmaAcc {
   static constexpr bool kComp = Traits<S>::compensated;
   static constexpr int IB = KC / 16, JB = MC / 16, KSTEP = Tr::KSTEP;
   static constexpr int NSTEP = T / KSTEP;
+  // The BF16 instruction rounds its sum of 32 products (and the accumulator)
+  // with a bias toward -infinity, about 1e-8 of the sum of |products| per
+  // instruction on inputs spread over many binades.  A second accumulator
+  // runs the same steps with one operand negated; its rounding bias has the
+  // opposite sign in P - Q, and the result is (P - Q) / 2.
+  static constexpr bool kPair = KSTEP == 32;
+  static constexpr bool kNegA = IB <= JB;  // negate the operand with fewer blocks
   static_assert(KC % 16 == 0 && MC % 16 == 0, "MFMA needs widths that are multiples of 16");
   static_assert(NSTEP % 4 == 0, "each of the 4 waves takes the same number of steps");
+  static_assert(!kPair || kComp, "the paired accumulators fold into (hi, lo)");
   AccV acc[IB][JB];
+  AccV accn[kPair ? IB : 1][kPair ? JB : 1];
   AccV hi[kComp ? IB : 1][kComp ? JB : 1];
   AccV lo[kComp ? IB : 1][kComp ? JB : 1];
 
@@ -411,6 +426,7 @@ struct MfmaAcc {
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         acc[i][j] = AccV{};
+        if constexpr (kPair) accn[i][j] = AccV{};
         if constexpr (kComp) hi[i][j] = lo[i][j] = AccV{};
       }
   }
@@ -429,6 +445,19 @@ struct MfmaAcc {
       for (int i = 0; i < IB; ++i)
 #pragma unroll
         for (int j = 0; j < JB; ++j) acc[i][j] = mfma_step(av[i], bv[j], acc[i][j]);
+      if constexpr (kPair) {
+        if constexpr (kNegA) {
+#pragma unroll
+          for (int i = 0; i < IB; ++i) av[i] = mfma_negate(av[i]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < JB; ++j) bv[j] = mfma_negate(bv[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < IB; ++i)
+#pragma unroll
+          for (int j = 0; j < JB; ++j) accn[i][j] = mfma_step(av[i], bv[j], accn[i][j]);
+      }
     }
   }
   __device__ __forceinline__ void fold() {
@@ -439,8 +468,13 @@ struct MfmaAcc {
         for (int j = 0; j < JB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
+            float x = acc[i][j][v];
+            if constexpr (kPair) {
+              x -= accn[i][j][v];  // 2 x (block sum)
+              accn[i][j][v] = 0.0f;
+            }
             float s, e;
-            two_sum(hi[i][j][v], acc[i][j][v], s, e);
+            two_sum(hi[i][j][v], x, s, e);
             hi[i][j][v] = s;
             lo[i][j][v] += e;
             acc[i][j][v] = 0.0f;
@@ -458,7 +492,9 @@ struct MfmaAcc {
           const int r = Tr::row_major_out ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v;
           const int o = (i * 16 + r) * MC + j * 16 + (lane & 15);
           double x;
-          if constexpr (kComp)
+          if constexpr (kPair)
+            x = 0.5 * ((double)hi[i][j][v] + (double)lo[i][j][v]);
+          else if constexpr (kComp)
             x = (double)hi[i][j][v] + (double)lo[i][j][v];
           else
             x = acc[i][j][v];
