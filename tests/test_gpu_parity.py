@@ -287,6 +287,28 @@ def test_backward_against_oracle(dt, k, m):
     assert dB2 is None and torch.equal(dA2, dA)
 
 
+@pytest.mark.parametrize("dt,k,m,n", [("f32", 16, 16, 70001), ("f32", 12, 4, 3001), ("f32", 32, 32, 300),
+                                      ("f64", 2, 16, 5), ("f64", 16, 16, 4097), ("bf16", 8, 8, 1000),
+                                      ("bf16", 64, 16, 2049)])
+def test_backward_staged_equals_rowwise(dt, k, m, n):
+    """Dense rows take the LDS-staged kernel and strided views the row-wise
+    one; both add the same products in the same order, so the bits agree."""
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
+    gdt = torch.float64 if dt == "f64" else torch.float32
+    dA, dB = ops.atb_backward(A, B, G.to("cuda", gdt))
+    Aw = torch.zeros((n, k + 3), dtype=A.dtype, device="cuda")
+    Bw = torch.zeros((n, m + 5), dtype=B.dtype, device="cuda")
+    Aw[:, :k] = A
+    Bw[:, :m] = B
+    dA2, dB2 = ops.atb_backward(Aw[:, :k], Bw[:, :m], G.to("cuda", gdt))
+    assert torch.equal(dA, dA2) and torch.equal(dB, dB2)
+    dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.to(gdt).double().numpy())
+    tol = 1e-13 if dt == "f64" else (1e-5 if dt == "f32" else 2**-7 * np.abs(G.numpy()).sum())
+    np.testing.assert_allclose(host(dA), dA_ref, rtol=tol if dt != "bf16" else 2**-7, atol=tol)
+    np.testing.assert_allclose(host(dB), dB_ref, rtol=tol if dt != "bf16" else 2**-7, atol=tol)
+
+
 def test_autograd_matches_torch_matmul():
     n, k, m = 30011, 16, 16
     A0, B0 = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)

@@ -4,11 +4,18 @@
 // Both are tall-skinny times small: one streaming pass reads a row of A and
 // B and writes a row of dA and dB.  dC (at most 64 x 64) is staged once per
 // workgroup in LDS, zero-padded to the width class W >= max(k, m), and read
-// with wave-uniform addresses (LDS broadcast).  Each thread owns whole rows
-// (grid-stride); a row is read with 16-byte loads when the matrices are
-// dense and aligned, and the outputs are produced in 16-byte chunks.
+// with wave-uniform addresses (LDS broadcast).  Each thread computes whole
+// rows.  Two kernels share that arithmetic:
+//   staged  (dense rows of 16..128 bytes): a workgroup moves a tile of 256
+//           rows between HBM and LDS with consecutive 16-byte loads and
+//           stores (each instruction of a wave covers 1 KiB of contiguous
+//           bytes); a thread reads its row from the padded LDS tile and
+//           writes its output row back there before the tile is stored;
+//   rowwise (any layout): a thread loads and stores its own rows directly,
+//           16 bytes at a time when dense.
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
-// a sum of at most 64 products).
+// a sum of at most 64 products, added in the same order by both kernels, so
+// both produce the same bits).
 #include "tsm/tsm.h"
 #include "tsm_internal.hpp"
 
@@ -136,7 +143,152 @@ __device__ __forceinline__ void bwd_body(const BwdArgs& a) {
   }
 }
 
+// ---- staged kernel ------------------------------------------------------------------
+// One phase: out (rows x out_w) = in (rows x in_w) times the dC coefficients,
+// for the tile of `rows` rows starting at row r0.  `regs` holds this thread's
+// 16-byte chunks of the input tile (chunk tid + i * 256), already loaded.
+template <class S, int W, bool TRANSPOSED, int CPT>
+__device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, int in_w,
+                                             int out_w, S* out, int64_t r0, unsigned char* tile,
+                                             const typename Traits<S>::Acc* G) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int PER = 16 / ES;             // elements per chunk
+  constexpr int PITCH = W * ES + 16;       // padded LDS row pitch (bytes)
+  const int tid = threadIdx.x;
+  const int in_cpr = in_w / PER, out_cpr = out_w / PER;
+  const int in_chunks = rows * in_cpr;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = tid + i * kThreads;
+    if (c < in_chunks) {
+      const int r = c / in_cpr;
+      *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
+    }
+  }
+  __syncthreads();
+  Acc x[W];
+  if (tid < rows) {
+#pragma unroll
+    for (int c = 0; c < W / PER; ++c) {
+      if (c < in_cpr) {
+        const v4u u = *reinterpret_cast<const v4u*>(tile + tid * PITCH + c * 16);
+        Unpack<S>::run(u, x + c * PER);
+      } else {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) x[c * PER + e] = Acc(0);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < rows) {
+#pragma unroll
+    for (int oc = 0; oc < W / PER; ++oc) {
+      if (oc < out_cpr) {
+        Acc acc[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) acc[e] = Acc(0);
+#pragma unroll
+        for (int t = 0; t < W; ++t)
+#pragma unroll
+          for (int e = 0; e < PER; ++e) {
+            const int o = oc * PER + e;
+            const Acc g = TRANSPOSED ? G[t * W + o] : G[o * W + t];
+            acc[e] = fmadd(x[t], g, acc[e]);
+          }
+        S tmp[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) tmp[e] = narrow<S>(acc[e]);
+        v4u u;
+        __builtin_memcpy(&u, tmp, 16);
+        *reinterpret_cast<v4u*>(tile + tid * PITCH + oc * 16) = u;
+      }
+    }
+  }
+  __syncthreads();
+  v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
+  const int out_chunks = rows * out_cpr;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = tid + i * kThreads;
+    if (c < out_chunks) {
+      const int r = c / out_cpr;
+      const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
+      __builtin_nontemporal_store(u, dst + c);
+    }
+  }
+  __syncthreads();  // the tile is rewritten by the next phase
+}
+
+template <class S, int W>
+__device__ __forceinline__ void load_chunks(const S* in, int64_t r0, int rows, int w,
+                                            v4u (&regs)[W * (int)sizeof(S) / 16]) {
+  constexpr int CPT = W * (int)sizeof(S) / 16;
+  const v4u* src = reinterpret_cast<const v4u*>(in + r0 * w);
+  const int chunks = rows * (w * (int)sizeof(S) / 16);
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = threadIdx.x + i * kThreads;
+    regs[i] = c < chunks ? __builtin_nontemporal_load(src + c) : v4u{0, 0, 0, 0};
+  }
+}
+
+template <class S, int W>
+__device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int CPT = W * ES / 16;  // chunks per thread per input tile
+  static_assert(W * ES >= 16 && W * ES <= 128, "staged rows are 16..128 bytes");
+  constexpr int PITCH = W * ES + 16;
+  __shared__ Acc G[W * W];
+  __shared__ __attribute__((aligned(16))) unsigned char tile[kThreads * PITCH];
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const int i = e / W, j = e % W;
+    Acc v = Acc(0);
+    if (i < a.k && j < a.m) {
+      const int64_t off = (int64_t)i * a.lddc + j;
+      v = a.dc_f64 ? (Acc) static_cast<const double*>(a.dC)[off]
+                   : (Acc) static_cast<const float*>(a.dC)[off];
+    }
+    G[e] = v;
+  }
+  __syncthreads();
+  const S* A = static_cast<const S*>(a.A);
+  const S* B = static_cast<const S*>(a.B);
+  S* dA = static_cast<S*>(a.dA);
+  S* dB = static_cast<S*>(a.dB);
+  const int64_t ntiles = (a.n + kThreads - 1) / kThreads;
+  for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+    const int64_t r0 = tt * kThreads;
+    const int rows = (int)((a.n - r0) < kThreads ? (a.n - r0) : kThreads);
+    // Both input tiles are requested before the first phase waits on one.
+    v4u rb[CPT], ra[CPT];
+    if (dA) load_chunks<S, W>(B, r0, rows, a.m, rb);
+    if (dB) load_chunks<S, W>(A, r0, rows, a.k, ra);
+    if (dA) staged_phase<S, W, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, G);
+    if (dB) staged_phase<S, W, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, G);
+  }
+}
+
 }  // namespace tsm
+
+#define TSM_BWD_STAGED(NAME, S, W)                                                    \
+  extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
+    tsm::bwd_staged_body<S, W>(a);                                                    \
+  }
+
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w2, double, 2)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w4, double, 4)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w8, double, 8)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w16, double, 16)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w4, float, 4)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w8, float, 8)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w16, float, 16)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w32, float, 32)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w8, tsm::bf16, 8)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w16, tsm::bf16, 16)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w32, tsm::bf16, 32)
+TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w64, tsm::bf16, 64)
 
 #define TSM_BWD_KERNEL(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
@@ -174,6 +326,15 @@ const BwdKernel kBwd[3][7] = {
      tsm_atb_bwd_f32_w16, tsm_atb_bwd_f32_w32, tsm_atb_bwd_f32_w64},
     {tsm_atb_bwd_bf16_w1, tsm_atb_bwd_bf16_w2, tsm_atb_bwd_bf16_w4, tsm_atb_bwd_bf16_w8,
      tsm_atb_bwd_bf16_w16, tsm_atb_bwd_bf16_w32, tsm_atb_bwd_bf16_w64},
+};
+// Staged kernels by dtype and log2(W); null where W * s is outside 16..128 bytes.
+const BwdKernel kBwdStaged[3][7] = {
+    {nullptr, tsm_atb_bwd_staged_f64_w2, tsm_atb_bwd_staged_f64_w4, tsm_atb_bwd_staged_f64_w8,
+     tsm_atb_bwd_staged_f64_w16, nullptr, nullptr},
+    {nullptr, nullptr, tsm_atb_bwd_staged_f32_w4, tsm_atb_bwd_staged_f32_w8,
+     tsm_atb_bwd_staged_f32_w16, tsm_atb_bwd_staged_f32_w32, nullptr},
+    {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, tsm_atb_bwd_staged_bf16_w16,
+     tsm_atb_bwd_staged_bf16_w32, tsm_atb_bwd_staged_bf16_w64},
 };
 int elem(int t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
 }  // namespace
@@ -230,9 +391,28 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   if (hipGetDevice(&dev) == hipSuccess)
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   int64_t blocks = (n + 255) / 256;
-  const int64_t cap = (int64_t)cus * 8;
+  // Staged when every row that moves is dense: both dense flags cover the
+  // inputs read and the outputs written (rows of k * s and m * s bytes, each
+  // a multiple of 16), and the padded width fits the staged tile.
+  // The staged kernel runs a persistent grid of as many workgroups as fit on
+  // the device at once (its LDS tile bounds the count per CU).
+  BwdKernel kern = kBwd[(int)in_type][wi];
+  int64_t cap = (int64_t)cus * 8;
+  if (a.dense_a && a.dense_b && kBwdStaged[(int)in_type][wi]) {
+    kern = kBwdStaged[(int)in_type][wi];
+    static int resident[3][7];  // workgroups per CU, filled on first use
+    int& per_cu = resident[(int)in_type][wi];
+    if (per_cu == 0) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kern),
+                                                       256, 0) != hipSuccess || nb < 1)
+        nb = 1;
+      per_cu = nb;
+    }
+    cap = (int64_t)cus * per_cu;
+  }
   if (blocks > cap) blocks = cap;
-  hipLaunchKernelGGL(kBwd[(int)in_type][wi], dim3((unsigned)blocks), dim3(256), 0,
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0,
                      static_cast<hipStream_t>(stream), a);
   return (tsm_status)tsm_internal_hip_status(hipGetLastError());
 }
