@@ -503,6 +503,25 @@ struct MfmaAcc {
   }
 };
 
+// ---- publishing workgroup partials ----------------------------------------------------
+// Each XCD of gfx950 has its own L2, so a plain store is visible to the other
+// XCDs only after an agent-scope release (a writeback of the whole L2).  The
+// partials are instead stored with agent-scope atomic stores, which write
+// through to memory; the publishing wave then only waits until its stores are
+// complete before the arrival counter is incremented.  Only the last
+// workgroup to arrive fences (acquire: it invalidates its L2) before reading.
+__device__ __forceinline__ void publish(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// s_waitcnt vmcnt(0) (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15): every
+// vector memory operation of this wave has completed.  The signal fences
+// keep the compiler from moving memory operations across the wait.
+__device__ __forceinline__ void wait_vector_memory() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // ---- the streaming kernel body -----------------------------------------------------
 // LDS bytes a configuration needs: two pipeline stages, the KC x MC FP64
 // reduction buffer or the final-reduction scratch, whichever is largest.
@@ -724,7 +743,8 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
     // The refill is unconditional (past the last tile it re-reads the last
     // tile, which is in bounds and discarded) so that the number of loads in
     // flight is the same on every path and each publish waits only for the
-    // oldest tile.
+    // oldest tile.  (Peeling the last PF steps to skip those re-reads was
+    // measured slower: the larger loop code costs more than the re-reads.)
     const int64_t last = nfull - 1;
     auto step = [&](Regs& r, int64_t tt) __attribute__((always_inline)) {
       unsigned char* stage = smem + (tt & 1) * STAGE;
@@ -784,7 +804,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   // ---- reduction -------------------------------------------------------------------
   double* red = reinterpret_cast<double*>(smem);
   cons.reduce(red, tid);
-  for (int o = tid; o < KCMC; o += kThreads) partial_out[o] = red[o];
+  for (int o = tid; o < KCMC; o += kThreads) publish(partial_out + o, red[o]);
 }
 
 // Arrival at a problem's counter after its partial is stored: returns true
@@ -792,7 +812,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
 // partials.  That workgroup resets the counter to zero after using it.
 __device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int expected,
                                             int* s_flag) {
-  __threadfence();  // release this workgroup's partial at agent scope
+  wait_vector_memory();  // this wave's write-through partial stores are complete
   __syncthreads();
   if (threadIdx.x == 0) *s_flag = (atomicAdd(counter, 1u) == expected - 1) ? 1 : 0;
   __syncthreads();
