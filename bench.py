@@ -65,7 +65,8 @@ def secondary() -> dict:
         speedup and relative error against the FP64 kernel on the same rounded
         inputs (uniform and sign*2^e entries);
       backward: FP32 n = 2^23, k = m = 16, (forward + backward) / forward;
-      batched: 512 pairs in one launch vs one launch per pair (FP64, FP32)."""
+      batched: 512 pairs in one launch vs one launch per pair (FP64, FP32);
+      siblings: C = A^T A against tsm_atb(A, A), and Y = A X."""
     from tsm_benchmark_amd import suite
 
     prec = suite.precision(10, False)
@@ -78,6 +79,7 @@ def secondary() -> dict:
                              ("us", "speedup_vs_fp64", "frob_rel_err", "max_err_over_absAB")}
     bwd = suite.backward(10)
     bat = suite.batched(5)
+    sib = suite.siblings(10)
     return {
         "low_precision": low,
         "backward": {key: bwd[key] for key in ("fwd_us", "bwd_us", "fwd_plus_bwd_over_fwd",
@@ -85,6 +87,7 @@ def secondary() -> dict:
         "batched": {dt: {"one_launch_GBps": bat[dt]["graph_GBps"],
                          "per_pair_launches_GBps": bat[dt]["per_pair_GBps"]}
                     for dt in ("f64", "f32")},
+        "siblings": sib,
     }
 
 

@@ -111,6 +111,18 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
 size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
                               const tsm_kernel_config* config);
 
+/* ---- symmetric product C = A^T A (no reference counterpart) ----------------
+ * The Gram matrix of a tall-skinny A (n x k, 1 <= k <= 64), reading A once
+ * (half the bytes of tsm_atb(A, A)).  Same accumulation types, error bound
+ * and deterministic reduction as tsm_atb; C is exactly symmetric (the
+ * matrix-core classes compute the blocks on and above the diagonal and
+ * mirror the others).  C is k x k (ldc >= k); accumulate, workspace and
+ * stream as for tsm_atb. */
+tsm_status tsm_ata(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
+                   int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
+                   size_t workspace_bytes, tsm_stream_t stream);
+size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k);
+
 /* ---- differentiation (no reference counterpart) --------------------------
  * Backward of C = A^T B for an upstream gradient dC (k x m, row-major, lddc):
  *   dA = B dC^T (n x k), dB = A dC (n x m).
@@ -121,6 +133,17 @@ tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, i
                             const void* dC, int64_t lddc,
                             void* dA, int64_t ldda, void* dB, int64_t lddb,
                             tsm_stream_t stream);
+
+/* ---- tall-skinny times small: Y = A X (no reference counterpart) ------------
+ * A is n x k (in_type, row-major, lda), X is k x m (x_type: TSM_F64 or
+ * TSM_F32, row-major, ldx), Y is n x m (in_type, ldy).  This is the
+ * dB = A dC half of tsm_atb_backward (same kernels): each output is a sum of
+ * k products in FP64 for FP64 inputs and FP32 otherwise, rounded once to
+ * in_type.  Rows of 16..128 bytes with lda == k and ldy == m take the
+ * LDS-staged kernel. */
+tsm_status tsm_tsmm(tsm_dtype in_type, tsm_dtype x_type, int64_t n, int k, int m,
+                    const void* A, int64_t lda, const void* X, int64_t ldx, void* Y,
+                    int64_t ldy, tsm_stream_t stream);
 
 /* ---- batching (no reference counterpart) ----------------------------------
  * `count` independent problems C_i (+)= A_i^T B_i in ONE persistent launch.

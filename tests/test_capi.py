@@ -111,6 +111,45 @@ def test_backward_argument_errors():
     assert bwd(n=0) == _lib.SUCCESS
 
 
+def test_ata_argument_errors():
+    L = _lib.lib()
+
+    def ata(in_type=_lib.F64, out_type=_lib.F64, n=10, k=8, A=FAKE, lda=8, C=FAKE, ldc=8):
+        return L.tsm_ata(in_type, out_type, n, k, A, lda, C, ldc, 0, None, 0, None)
+
+    assert ata(k=0) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(k=65) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(lda=7) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(ldc=7) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(A=None) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(C=None) == _lib.ERR_INVALID_ARGUMENT
+    assert ata(A=FAKE + 4) == _lib.ERR_MISALIGNED
+    assert ata(in_type=9) == _lib.ERR_UNSUPPORTED
+    assert ata(out_type=_lib.BF16) == _lib.ERR_UNSUPPORTED
+    assert L.tsm_ata_workspace_size(_lib.F64, -1, 8) == 0
+    assert L.tsm_ata_workspace_size(_lib.F64, 10, 65) == 0
+
+
+def test_tsmm_argument_errors():
+    L = _lib.lib()
+
+    def tsmm(in_type=_lib.F64, x_type=_lib.F64, n=10, k=4, m=8, A=FAKE, lda=4, X=FAKE, ldx=8,
+             Y=FAKE, ldy=8):
+        return L.tsm_tsmm(in_type, x_type, n, k, m, A, lda, X, ldx, Y, ldy, None)
+
+    assert tsmm(k=0) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(m=65) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(lda=3) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(ldx=7) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(ldy=7) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(X=None) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(A=None) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(Y=None) == _lib.ERR_INVALID_ARGUMENT
+    assert tsmm(Y=FAKE + 4) == _lib.ERR_MISALIGNED
+    assert tsmm(x_type=_lib.BF16) == _lib.ERR_UNSUPPORTED
+    assert tsmm(n=0) == _lib.SUCCESS
+
+
 def test_fill_argument_errors():
     L = _lib.lib()
     assert L.tsm_fill_random(_lib.F64, 5, 1, 1, FAKE, 4, 4, 4, None) == _lib.ERR_INVALID_ARGUMENT

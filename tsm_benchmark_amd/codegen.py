@@ -80,6 +80,7 @@ class KernelConfig:
     rows: int
     pf: int
     nt: int = 0  # 1: non-temporal (streaming) global loads
+    sym: int = 0  # 1: C = A^T A (B is A; only A's tile is staged)
 
     @property
     def mfma(self) -> bool:
@@ -89,7 +90,8 @@ class KernelConfig:
     @property
     def name(self) -> str:
         tile = "mfma" if self.mfma else f"t{self.tm}x{self.tn}"
-        return (f"tsm_atb_{self.dtype}_{self.kc}x{self.mc}_{tile}"
+        op = "ata" if self.sym else "atb"
+        return (f"tsm_{op}_{self.dtype}_{self.kc}x{self.mc}_{tile}"
                 f"_r{self.rows}_p{self.pf}" + ("_nt" if self.nt else ""))
 
     @property
@@ -107,7 +109,7 @@ class KernelConfig:
 
     @property
     def stage_bytes(self) -> int:
-        return self.rows * (self.kc + self.mc) * ELEM_BYTES[self.dtype]
+        return self.rows * (self.kc + (0 if self.sym else self.mc)) * ELEM_BYTES[self.dtype]
 
     @property
     def lds_bytes(self) -> int:
@@ -122,6 +124,8 @@ class KernelConfig:
         if self.kc not in CLASSES or self.mc not in CLASSES:
             p.append("width class must be a power of two <= 64")
             return p
+        if self.sym not in (0, 1) or (self.sym and self.kc != self.mc):
+            p.append("A^T A kernels are square")
         if self.mfma:
             if self.kc % 16 or self.mc % 16:
                 p.append("MFMA needs width classes that are multiples of 16")
@@ -265,8 +269,31 @@ def load_tuning(path: str = TUNING_JSON) -> dict:
         return json.load(f)
 
 
+def sym_kernels(defaults: dict) -> dict:
+    """The A^T A kernel per (dtype, width class), from the tuned A^T B
+    configuration of the square class (same consumer, grid and tile order).
+    A stage holds A only, so register-tile kernels take twice the rows where
+    that is valid: the same bytes in flight per tile as the A^T B kernel
+    (measured: FP64 k = 8 streams at 6.97 TB/s with 512 rows, 5.4 TB/s with
+    256).  Matrix-core kernels keep their rows: they are bound by the
+    matrix instructions, which the symmetric kernel trims instead."""
+    out = {}
+    for dt in DTYPES:
+        for kc in CLASSES:
+            c, wpc, il = defaults[(dt, kc, kc)]
+            s = dataclasses.replace(c, sym=1)
+            if not c.mfma:
+                wide = dataclasses.replace(s, rows=2 * c.rows)
+                if not wide.problems():
+                    s = wide
+            if s.problems():
+                raise ValueError(f"{s.name}: {s.problems()}")
+            out[(dt, kc)] = (s, wpc, il)
+    return out
+
+
 def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
-    """The full kernel set of the library and the tuned default per class."""
+    """The full A^T B kernel set of the library and the tuned default per class."""
     cfgs: set[KernelConfig] = set()
     defaults: dict[tuple[str, int, int], tuple[KernelConfig, int, int]] = {}
     for dt in DTYPES:
@@ -298,7 +325,7 @@ BANNER = ("// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.\n"
 
 def emit_kernel(c: KernelConfig) -> str:
     t = CXX_TYPE[c.dtype]
-    targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}, {c.nt}"
+    targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}, {c.nt}" + (", 1" if c.sym else "")
     if c.mfma:
         shape = (f"//   matrix cores: {c.rows // c.kstep} K-steps of {c.kstep} rows per tile, "
                  f"LDS {c.lds_bytes} B\n")
@@ -377,13 +404,19 @@ def emit_batched_file(dtype: str, classes: list[KernelConfig]) -> str:
 
 def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
     idx = {c: i for i, c in enumerate(cfgs)}
+    syms = sym_kernels(defaults)
     out = [BANNER, '#include "../tsm_internal.hpp"\n']
-    for c in cfgs:
+    for c in cfgs + [v[0] for v in syms.values()]:
         out.append(f"hipError_t launch_{c.name}(const tsm::AtbArgs&, unsigned, hipStream_t);")
     out.append("\nstatic const tsm::KernelEntry kTable[] = {")
     for c in cfgs:
         out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
                    f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
+    out.append("};\n")
+    out.append("static const tsm::SymEntry kSym[] = {")
+    for (dt, kc), (c, wpc, il) in sorted(syms.items()):
+        out.append(f"  {{\"{c.name}\", {DTYPE_ID[dt]}, {kc}, {c.rows}, {wpc}, {il}, "
+                   f"launch_{c.name}}},")
     out.append("};\n")
     out.append("static const tsm::TunedEntry kTuned[] = {")
     for (dt, kc, mc), (c, wpc, il) in sorted(defaults.items()):
@@ -411,6 +444,8 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
                f"  *count = {len(DTYPES)};\n  return batched_all();\n}}")
     out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {\n"
                "  *count = (int)(sizeof(kTable) / sizeof(kTable[0]));\n  return kTable;\n}")
+    out.append("extern \"C\" const tsm::SymEntry* tsm_gen_sym_table(int* count) {\n"
+               "  *count = (int)(sizeof(kSym) / sizeof(kSym[0]));\n  return kSym;\n}")
     out.append("extern \"C\" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {\n"
                "  *count = (int)(sizeof(kTuned) / sizeof(kTuned[0]));\n  return kTuned;\n}")
     return "\n".join(out) + "\n"
@@ -430,12 +465,13 @@ def _write_if_changed(path: str, text: str) -> bool:
 def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> list[str]:
     """Write the library's generated sources; returns the .hip/.cpp paths."""
     cfgs, defaults = library_kernels(load_tuning(tuning_path))
+    syms = [v[0] for v in sym_kernels(defaults).values()]
     paths = []
     for dt in DTYPES:
         for kc in CLASSES:
             for mc in CLASSES:
                 p = os.path.join(out_dir, f"tsm_kernels_{dt}_{kc}x{mc}.hip")
-                _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs))
+                _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs + syms))
                 paths.append(p)
     keep = {os.path.basename(x) for x in paths} | {"tsm_table.hip"} | \
         {f"tsm_batched_{dt}.hip" for dt in DTYPES}

@@ -242,3 +242,13 @@ hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p2(const tsm::AtbArgs& a, unsigned 
   hipLaunchKernelGGL(tsm_atb_f32_8x8_t8x8_r1024_p2, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+
+// tsm_ata_f32_8x8_t8x4_r2048_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=2048 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_8x8_t8x4_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 8, 8, 8, 4, 2048, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_8x8_t8x4_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}

@@ -322,3 +322,13 @@ hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsig
   hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+
+// tsm_ata_f64_16x16_t8x8_r256_p2_nt: KC=16 MC=16 TM=8 TN=8 rows=256 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_t8x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 8, 256, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_16x16_t8x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}

@@ -282,3 +282,13 @@ hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs& a, unsig
   hipLaunchKernelGGL(tsm_atb_f64_32x32_t8x8_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+
+// tsm_ata_f64_32x32_mfma_r128_p2_nt: KC=32 MC=32 TM=0 TN=0 rows=128 pf=2 nt=1
+//   matrix cores: 32 K-steps of 4 rows per tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 32, 32, 0, 0, 128, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}

@@ -162,3 +162,13 @@ hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigne
   hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
+
+// tsm_ata_f64_8x8_t8x8_r512_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=512 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 512, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}

@@ -521,6 +521,27 @@ hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_4x4_t4x4_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 
 static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2},
@@ -1043,6 +1064,30 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_64x64_t8x8_r64_p2_nt", 0, 64, 64, 8, 8, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt},
 };
 
+static const tsm::SymEntry kSym[] = {
+  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 4096, 2, 0, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2},
+  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 4096, 2, 0, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2},
+  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4096, 2, 0, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2},
+  {"tsm_ata_bf16_8x8_t8x8_r2048_p2", 2, 8, 2048, 1, 1, launch_tsm_ata_bf16_8x8_t8x8_r2048_p2},
+  {"tsm_ata_bf16_16x16_mfma_r256_p2_nt", 2, 16, 256, 1, 1, launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt},
+  {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 256, 1, 1, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt},
+  {"tsm_ata_bf16_64x64_mfma_r256_p1_nt", 2, 64, 256, 1, 1, launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt},
+  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 4096, 2, 0, launch_tsm_ata_f32_1x1_t1x1_r4096_p2},
+  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 4096, 2, 0, launch_tsm_ata_f32_2x2_t2x2_r4096_p2},
+  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 2048, 2, 0, launch_tsm_ata_f32_4x4_t4x4_r2048_p2},
+  {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 2048, 1, 1, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt},
+  {"tsm_ata_f32_16x16_t8x4_r512_p2_nt", 1, 16, 512, 1, 0, launch_tsm_ata_f32_16x16_t8x4_r512_p2_nt},
+  {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 128, 1, 0, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 128, 1, 0, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt},
+  {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 2048, 1, 1, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt},
+  {"tsm_ata_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 1024, 1, 1, launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt},
+  {"tsm_ata_f64_4x4_t4x4_r2048_p1_nt", 0, 4, 2048, 1, 1, launch_tsm_ata_f64_4x4_t4x4_r2048_p1_nt},
+  {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 512, 1, 1, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt},
+  {"tsm_ata_f64_16x16_t8x8_r256_p2_nt", 0, 16, 256, 1, 1, launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt},
+  {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 128, 1, 1, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 32, 1, 1, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt},
+};
+
 static const tsm::TunedEntry kTuned[] = {
   {2, 1, 1, 0, 2, 0},  // tsm_atb_bf16_1x1_t1x1_r4096_p2
   {2, 1, 2, 1, 2, 0},  // tsm_atb_bf16_1x2_t1x2_r4096_p2
@@ -1223,6 +1268,10 @@ extern "C" const tsm::BatchedKernel* tsm_gen_batched_table(int* count) {
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {
   *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
   return kTable;
+}
+extern "C" const tsm::SymEntry* tsm_gen_sym_table(int* count) {
+  *count = (int)(sizeof(kSym) / sizeof(kSym[0]));
+  return kSym;
 }
 extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {
   *count = (int)(sizeof(kTuned) / sizeof(kTuned[0]));

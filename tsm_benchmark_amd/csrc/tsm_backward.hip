@@ -416,3 +416,12 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
                      static_cast<hipStream_t>(stream), a);
   return (tsm_status)tsm_internal_hip_status(hipGetLastError());
 }
+
+extern "C" tsm_status tsm_tsmm(tsm_dtype in_type, tsm_dtype x_type, int64_t n, int k, int m,
+                               const void* A, int64_t lda, const void* X, int64_t ldx, void* Y,
+                               int64_t ldy, tsm_stream_t stream) {
+  if (!Y) return n == 0 ? TSM_SUCCESS : TSM_ERR_INVALID_ARGUMENT;
+  // Y = A X is dB = A dC with dC = X; B is not read (ldb = m passes the checks).
+  return tsm_atb_backward(in_type, x_type, n, k, m, A, lda, nullptr, m, X, ldx, nullptr, k, Y,
+                          ldy, stream);
+}

@@ -120,6 +120,15 @@ tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_confi
   return TSM_SUCCESS;
 }
 
+tsm_status run_plan(const Plan& p, tsm::LaunchFn launch, int kc, int mc, tsm_dtype out_type,
+                    int64_t n, int k, int m, const void* A, int64_t lda, const void* B,
+                    int64_t ldb, void* C, int64_t ldc, int accumulate, bool dense,
+                    void* workspace, size_t workspace_bytes, hipStream_t s);
+
+// Grid, tile split and workspace of a launch over n rows in tiles of T rows.
+tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int64_t cfg_grid,
+                    int kc, int mc);
+
 tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kernel_config* cfg,
                      Plan* p) {
   const int kc = width_class(k), mc = width_class(m);
@@ -128,13 +137,30 @@ tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kerne
   if (st != TSM_SUCCESS) return st;
   int nk = 0;
   p->entry = &table(&nk)[p->kernel];
-  const int64_t T = p->entry->rows;
+  return set_grid(p, n, p->entry->rows, wpc, interleave, cfg ? cfg->grid : 0, kc, mc);
+}
+
+// The A^T A kernel of width class width_class(k) and its launch shape.
+tsm_status make_sym_plan(tsm_dtype in_type, int64_t n, int k, Plan* p,
+                         const tsm::SymEntry** entry) {
+  const int kc = width_class(k);
+  int ns = 0;
+  const tsm::SymEntry* syms = tsm_gen_sym_table(&ns);
+  *entry = nullptr;
+  for (int i = 0; i < ns; ++i)
+    if (syms[i].dtype == (int)in_type && syms[i].kc == kc) *entry = &syms[i];
+  if (!*entry) return TSM_ERR_NO_KERNEL;
+  return set_grid(p, n, (*entry)->rows, (*entry)->wgs_per_cu, (*entry)->interleave, 0, kc, kc);
+}
+
+tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int64_t cfg_grid,
+                    int kc, int mc) {
   int64_t want;
-  if (cfg && cfg->grid > 0) {
-    want = cfg->grid;
+  if (cfg_grid > 0) {
+    want = cfg_grid;
   } else {
     int cus = 0;
-    st = device_cus(&cus);
+    tsm_status st = device_cus(&cus);
     if (st != TSM_SUCCESS) return st;
     want = (int64_t)wpc * cus;
   }
@@ -303,15 +329,29 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);
   if (st != TSM_SUCCESS) return st;
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  const tsm::KernelEntry& ke = *p.entry;
+  const bool dense = k == ke.kc && m == ke.mc && lda == k && ldb == m &&
+                     ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0;
+  return run_plan(p, ke.launch, ke.kc, ke.mc, out_type, n, k, m, A, lda, B, ldb, C, ldc,
+                  accumulate, dense, workspace, workspace_bytes,
+                  static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+namespace {
+// Launches a planned A^T B (or A^T A) kernel and, for wide classes, the finisher.
+tsm_status run_plan(const Plan& p, tsm::LaunchFn launch, int kc, int mc, tsm_dtype out_type,
+                    int64_t n, int k, int m, const void* A, int64_t lda, const void* B,
+                    int64_t ldb, void* C, int64_t ldc, int accumulate, bool dense,
+                    void* workspace, size_t workspace_bytes, hipStream_t s) {
   if (workspace) {
     if (workspace_bytes < p.workspace) return TSM_ERR_WORKSPACE;
     if ((uintptr_t)workspace % kCounterBytes) return TSM_ERR_MISALIGNED;
   } else {
-    st = pooled_workspace(p.workspace, s, &workspace);
+    tsm_status st = pooled_workspace(p.workspace, s, &workspace);
     if (st != TSM_SUCCESS) return st;
   }
-  const tsm::KernelEntry& ke = *p.entry;
   tsm::AtbArgs a{};
   a.A = A;
   a.B = B;
@@ -328,20 +368,43 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   a.m = m;
   a.accumulate = accumulate ? 1 : 0;
   a.c_f32 = out_type == TSM_F32 ? 1 : 0;
-  a.dense = (k == ke.kc && m == ke.mc && lda == k && ldb == m && ((uintptr_t)A % 16) == 0 &&
-             ((uintptr_t)B % 16) == 0)
-                ? 1
-                : 0;
+  a.dense = dense ? 1 : 0;
   a.finish = p.finish_in_kernel ? 1 : 0;
-  hipError_t e = ke.launch(a, (unsigned)p.grid, s);
+  hipError_t e = launch(a, (unsigned)p.grid, s);
   if (e != hipSuccess) return hip_fail(e);
   if (!p.finish_in_kernel) {
     const unsigned fg = (unsigned)((p.kcmc + 63) / 64);
-    hipLaunchKernelGGL(tsm_atb_finish, dim3(fg), dim3(256), 0, s, a, (int)p.grid, ke.kc, ke.mc);
+    hipLaunchKernelGGL(tsm_atb_finish, dim3(fg), dim3(256), 0, s, a, (int)p.grid, kc, mc);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e);
   }
   return TSM_SUCCESS;
+}
+}  // namespace
+
+extern "C" {
+
+size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k) {
+  if (!valid_in_type(in_type) || n < 0 || k < 1 || k > TSM_MAX_WIDTH) return 0;
+  Plan p;
+  const tsm::SymEntry* e = nullptr;
+  if (make_sym_plan(in_type, n, k, &p, &e) != TSM_SUCCESS) return 0;
+  return p.workspace;
+}
+
+tsm_status tsm_ata(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
+                   int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
+                   size_t workspace_bytes, tsm_stream_t stream) {
+  tsm_status st = check_atb(in_type, out_type, n, k, k, A, lda, A, lda, C, ldc);
+  if (st != TSM_SUCCESS) return st;
+  Plan p;
+  const tsm::SymEntry* e = nullptr;
+  st = make_sym_plan(in_type, n, k, &p, &e);
+  if (st != TSM_SUCCESS) return st;
+  const bool dense = k == e->kc && lda == k && ((uintptr_t)A % 16) == 0;
+  return run_plan(p, e->launch, e->kc, e->kc, out_type, n, k, k, A, lda, A, lda, C, ldc,
+                  accumulate, dense, workspace, workspace_bytes,
+                  static_cast<hipStream_t>(stream));
 }
 
 tsm_status tsm_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m, const void* A,

@@ -398,7 +398,10 @@ __device__ __forceinline__ v4f mfma_step(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-template <class S, int KC, int MC, int T>
+// SYM = 1 (A^T A, KC == MC): only the blocks on and above the diagonal are
+// accumulated; emit() writes each off-diagonal block and its transpose, so
+// the result is exactly symmetric outside the diagonal blocks.
+template <class S, int KC, int MC, int T, int SYM = 0>
 struct MfmaAcc {
   using Tr = MfmaTraits<S>;
   using AccV = typename Tr::AccV;
@@ -412,6 +415,8 @@ struct MfmaAcc {
   // opposite sign in P - Q, and the result is (P - Q) / 2.
   static constexpr bool kPair = KSTEP == 32;
   static constexpr bool kNegA = IB <= JB;  // negate the operand with fewer blocks
+  static_assert(!SYM || KC == MC, "A^T A is square");
+  static constexpr bool used(int i, int j) { return !SYM || i <= j; }
   static_assert(KC % 16 == 0 && MC % 16 == 0, "MFMA needs widths that are multiples of 16");
   static_assert(NSTEP % 4 == 0, "each of the 4 waves takes the same number of steps");
   static_assert(!kPair || kComp, "the paired accumulators fold into (hi, lo)");
@@ -430,9 +435,10 @@ struct MfmaAcc {
         if constexpr (kComp) hi[i][j] = lo[i][j] = AccV{};
       }
   }
-  __device__ __forceinline__ void consume(const unsigned char* stage, int wave, int lane) {
-    const S* As = reinterpret_cast<const S*>(stage);
-    const S* Bs = reinterpret_cast<const S*>(stage + T * KC * (int)sizeof(S));
+  __device__ __forceinline__ void consume(const unsigned char* a_stage,
+                                         const unsigned char* b_stage, int wave, int lane) {
+    const S* As = reinterpret_cast<const S*>(a_stage);
+    const S* Bs = reinterpret_cast<const S*>(b_stage);
 #pragma unroll
     for (int s = 0; s < NSTEP / 4; ++s) {
       const int row0 = (wave + 4 * s) * KSTEP;
@@ -444,7 +450,8 @@ struct MfmaAcc {
 #pragma unroll
       for (int i = 0; i < IB; ++i)
 #pragma unroll
-        for (int j = 0; j < JB; ++j) acc[i][j] = mfma_step(av[i], bv[j], acc[i][j]);
+        for (int j = 0; j < JB; ++j)
+          if (used(i, j)) acc[i][j] = mfma_step(av[i], bv[j], acc[i][j]);
       if constexpr (kPair) {
         if constexpr (kNegA) {
 #pragma unroll
@@ -456,7 +463,8 @@ struct MfmaAcc {
 #pragma unroll
         for (int i = 0; i < IB; ++i)
 #pragma unroll
-          for (int j = 0; j < JB; ++j) accn[i][j] = mfma_step(av[i], bv[j], accn[i][j]);
+          for (int j = 0; j < JB; ++j)
+            if (used(i, j)) accn[i][j] = mfma_step(av[i], bv[j], accn[i][j]);
       }
     }
   }
@@ -468,6 +476,7 @@ struct MfmaAcc {
         for (int j = 0; j < JB; ++j)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
+            if (!used(i, j)) continue;
             float x = acc[i][j][v];
             if constexpr (kPair) {
               x -= accn[i][j][v];  // 2 x (block sum)
@@ -489,8 +498,10 @@ struct MfmaAcc {
       for (int j = 0; j < JB; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
+          if (!used(i, j)) continue;
           const int r = Tr::row_major_out ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v;
-          const int o = (i * 16 + r) * MC + j * 16 + (lane & 15);
+          const int c = lane & 15;
+          const int o = (i * 16 + r) * MC + j * 16 + c;
           double x;
           if constexpr (kPair)
             x = 0.5 * ((double)hi[i][j][v] + (double)lo[i][j][v]);
@@ -499,6 +510,10 @@ struct MfmaAcc {
           else
             x = acc[i][j][v];
           red[o] = first ? x : red[o] + x;
+          if (SYM && i < j) {
+            const int ot = (j * 16 + c) * MC + i * 16 + r;  // the transposed element
+            red[ot] = first ? x : red[ot] + x;
+          }
         }
   }
 };
@@ -525,9 +540,9 @@ __device__ __forceinline__ void wait_vector_memory() {
 // ---- the streaming kernel body -----------------------------------------------------
 // LDS bytes a configuration needs: two pipeline stages, the KC x MC FP64
 // reduction buffer or the final-reduction scratch, whichever is largest.
-template <class S, int KC, int MC, int T>
+template <class S, int KC, int MC, int T, int SYM = 0>
 constexpr int atb_smem_bytes() {
-  constexpr int stage2 = 2 * T * (KC + MC) * (int)sizeof(S);
+  constexpr int stage2 = 2 * T * (KC + (SYM ? 0 : MC)) * (int)sizeof(S);
   constexpr int red = KC * MC * 8;
   constexpr int m1 = stage2 > red ? stage2 : red;
   return m1 > kThreads * 16 ? m1 : kThreads * 16;
@@ -569,9 +584,10 @@ struct RegTileAcc {
         if constexpr (kComp) hi[i][j] = lo[i][j] = Acc(0);
       }
   }
-  __device__ __forceinline__ void consume(const unsigned char* stage) {
-    const S* As = reinterpret_cast<const S*>(stage);
-    const S* Bs = reinterpret_cast<const S*>(stage + T * KC * (int)sizeof(S));
+  __device__ __forceinline__ void consume(const unsigned char* a_stage,
+                                         const unsigned char* b_stage) {
+    const S* As = reinterpret_cast<const S*>(a_stage);
+    const S* Bs = reinterpret_cast<const S*>(b_stage);
 #pragma unroll
     for (int s = 0; s < RPT; ++s) {
       const int r = q + s * RG;
@@ -644,17 +660,18 @@ struct RegTileAcc {
 };
 
 // The MFMA consumer with the same interface (TM = TN = 0 selects it).
-template <class S, int KC, int MC, int T>
+template <class S, int KC, int MC, int T, int SYM = 0>
 struct MfmaConsumer {
-  MfmaAcc<S, KC, MC, T> m;
+  MfmaAcc<S, KC, MC, T, SYM> m;
   int wave, lane;
   __device__ __forceinline__ void zero(int tid) {
     wave = tid >> 6;
     lane = tid & 63;
     m.zero();
   }
-  __device__ __forceinline__ void consume(const unsigned char* stage) {
-    m.consume(stage, wave, lane);
+  __device__ __forceinline__ void consume(const unsigned char* a_stage,
+                                         const unsigned char* b_stage) {
+    m.consume(a_stage, b_stage, wave, lane);
   }
   __device__ __forceinline__ void after_tile() { m.fold(); }
   __device__ __forceinline__ void reduce(double* red, int) {
@@ -668,35 +685,40 @@ struct MfmaConsumer {
   }
 };
 
-template <class S, int KC, int MC, int TM, int TN, int T>
+template <class S, int KC, int MC, int TM, int TN, int T, int SYM = 0>
 struct ConsumerOf {
   using type = RegTileAcc<S, KC, MC, TM, TN, T>;
 };
-template <class S, int KC, int MC, int T>
-struct ConsumerOf<S, KC, MC, 0, 0, T> {
-  using type = MfmaConsumer<S, KC, MC, T>;
+template <class S, int KC, int MC, int T, int SYM>
+struct ConsumerOf<S, KC, MC, 0, 0, T, SYM> {
+  using type = MfmaConsumer<S, KC, MC, T, SYM>;
 };
 
 // Streams a sequence of row tiles of one problem - the full tiles
 // first + j * step (j < count) and, if owns_tail, the partial tile of
 // tail_rows rows that starts at row full * T - and writes this workgroup's
 // KC x MC FP64 partial sum to partial_out (global memory).  smem must hold
-// atb_smem_bytes<S, KC, MC, T>() bytes.  TM = TN = 0 selects the MFMA consumer.
-template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+// atb_smem_bytes<S, KC, MC, T, SYM>() bytes.  TM = TN = 0 selects the MFMA
+// consumer.  SYM = 1 computes A^T A (B is A, KC == MC): only A's tile is
+// staged and the consumer reads both operands from it, so every product is
+// formed and added exactly as for A^T B with B = A, from half the bytes.
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT, int SYM = 0>
 __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* smem, int64_t first,
                                             int64_t step, int64_t count, bool owns_tail,
                                             int64_t full, int tail_rows, double* partial_out) {
+  static_assert(!SYM || KC == MC, "A^T A is square");
   constexpr int ES = (int)sizeof(S);
-  constexpr int A_BYTES = T * KC * ES, B_BYTES = T * MC * ES;
+  constexpr int A_BYTES = T * KC * ES, B_BYTES = SYM ? 0 : T * MC * ES;
   static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
   constexpr int A_VEC = A_BYTES / 16, B_VEC = B_BYTES / 16;
   constexpr int A_PER = (A_VEC + kThreads - 1) / kThreads;
   constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
   constexpr int STAGE = A_BYTES + B_BYTES;
+  constexpr int B_OFF = SYM ? 0 : A_BYTES;  // B's tile within a stage
   constexpr int KCMC = KC * MC;
   static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
   const int tid = threadIdx.x;
-  typename ConsumerOf<S, KC, MC, TM, TN, T>::type cons;
+  typename ConsumerOf<S, KC, MC, TM, TN, T, SYM>::type cons;
   cons.zero(tid);
 
   const int64_t nfull = a.dense ? count : 0;
@@ -707,7 +729,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
     const v4u* gB = static_cast<const v4u*>(a.B);
     struct Regs {
       v4u a[A_PER];
-      v4u b[B_PER];
+      v4u b[B_PER > 0 ? B_PER : 1];
     };
     auto load = [&](Regs& r, int64_t t) __attribute__((always_inline)) {
       const int64_t tile = first + t * step;
@@ -756,7 +778,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       __syncthreads();
       load(r, tt + PF < last ? tt + PF : last);
       __builtin_amdgcn_sched_barrier(0);
-      cons.consume(stage);
+      cons.consume(stage, stage + B_OFF);
       cons.after_tile();
     };
     Regs r0, r1;
@@ -787,14 +809,16 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       if (r < nr && c < a.k) v = gA[(r0 + r) * a.lda + c];
       la[e] = v;
     }
-    for (int e = tid; e < T * MC; e += kThreads) {
-      const int r = e / MC, c = e % MC;
-      S v{};
-      if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
-      lb[e] = v;
+    if constexpr (!SYM) {
+      for (int e = tid; e < T * MC; e += kThreads) {
+        const int r = e / MC, c = e % MC;
+        S v{};
+        if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
+        lb[e] = v;
+      }
     }
     __syncthreads();
-    cons.consume(smem);
+    cons.consume(smem, smem + B_OFF);
     cons.after_tile();
   };
   if (!a.dense)
@@ -824,9 +848,9 @@ __device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int 
 // One problem over the whole grid: contiguous runs of tiles_per_wg tiles per
 // workgroup, or tile t to workgroup t % G (interleave); then the last
 // workgroup to arrive adds the G partials in workgroup order and writes C.
-template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
+template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT, int SYM = 0>
 __device__ __forceinline__ void atb_body(const AtbArgs& a) {
-  constexpr int SMEM = atb_smem_bytes<S, KC, MC, T>();
+  constexpr int SMEM = atb_smem_bytes<S, KC, MC, T, SYM>();
   __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
   __shared__ int s_last;
   const int64_t full = a.n / T;
@@ -845,8 +869,8 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
   const bool owns_tail =
       tail_rows > 0 && (a.interleave ? full % G == g : full / a.tiles_per_wg == g);
   constexpr int KCMC = KC * MC;
-  atb_partial<S, KC, MC, TM, TN, T, PF, NT>(a, smem, first, step, count, owns_tail, full,
-                                            tail_rows, a.partials + g * KCMC);
+  atb_partial<S, KC, MC, TM, TN, T, PF, NT, SYM>(a, smem, first, step, count, owns_tail, full,
+                                                 tail_rows, a.partials + g * KCMC);
   if (!a.finish) return;
   if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
   if constexpr (KCMC >= 2 && KCMC <= 512)

@@ -67,6 +67,47 @@ def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = Non
     return out
 
 
+def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
+        out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+    """C = A^T A (or C += A^T A with ``accumulate`` and ``out``): the Gram matrix
+    of a tall-skinny A, reading A once.  Same bits as ``atb(A, A)``."""
+    _check_tall(A, "A")
+    n, k = A.shape
+    if out is None:
+        if out_dtype is None:
+            out_dtype = torch.float64 if A.dtype == torch.float64 else torch.float32
+        if accumulate:
+            raise ValueError("accumulate needs an existing `out`")
+        out = torch.empty((k, k), dtype=out_dtype, device=A.device)
+    if out.dtype not in _OUT_TYPES or out.shape != (k, k) or out.stride(1) != 1:
+        raise ValueError("out must be a row-major k x k float64/float32 tensor")
+    L = _lib.lib()
+    with torch.cuda.device(A.device):
+        st = L.tsm_ata(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, A.data_ptr(), _ld(A),
+                       out.data_ptr(), out.stride(0) if k > 1 else k, 1 if accumulate else 0,
+                       None, 0, _stream())
+    _lib.check(st, "tsm_ata")
+    return out
+
+
+def tsmm(A: torch.Tensor, X: torch.Tensor) -> torch.Tensor:
+    """Y = A X for a tall-skinny A (n x k) and a small X (k x m, float64 or
+    float32); Y has A's dtype."""
+    _check_tall(A, "A")
+    n, k = A.shape
+    X = X.contiguous()
+    if X.dim() != 2 or X.shape[0] != k or X.dtype not in _OUT_TYPES or X.device != A.device:
+        raise ValueError("X must be a k x m float64/float32 tensor on A's device")
+    m = X.shape[1]
+    Y = torch.empty((n, m), dtype=A.dtype, device=A.device)
+    L = _lib.lib()
+    with torch.cuda.device(A.device):
+        st = L.tsm_tsmm(_IN_TYPES[A.dtype], _OUT_TYPES[X.dtype], n, k, m, A.data_ptr(), _ld(A),
+                        X.data_ptr(), m, Y.data_ptr(), m, _stream())
+    _lib.check(st, "tsm_tsmm")
+    return Y
+
+
 def atb_backward(A: torch.Tensor, B: torch.Tensor, dC: torch.Tensor, *,
                  need_dA: bool = True, need_dB: bool = True):
     """(dA, dB) = (B dC^T, A dC) in the dtype of A and B; either may be skipped."""

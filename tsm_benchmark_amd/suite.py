@@ -10,6 +10,8 @@
   backward    FP32 n = 2^23, k = m = 16: forward, backward, (fwd+bwd)/fwd
   batched     512 pairs, n log-uniform in [2^10, 2^21], k, m in {2..32}, FP64
               and FP32: one persistent launch vs one launch per pair
+  siblings    C = A^T A (FP64, A = 2 GiB, k = 8 and 32) against tsm_atb(A, A);
+              Y = A X (FP64, k = m = 8, n = 2^25)
 
 Times are HIP-event medians on the launch stream with inputs resident in HBM.
 """
@@ -156,6 +158,31 @@ def batched(reps):
     return res
 
 
+def siblings(reps):
+    out = {}
+    for k in (8, 32):
+        n = (1 << 31) // (8 * k)
+        A = rand(n, k, "f64", 1)
+        C = torch.empty((k, k), dtype=torch.float64, device="cuda")
+        t_ata = timed(lambda: ops.ata(A, out=C), reps)
+        t_atb = timed(lambda: ops.atb(A, A, out=C), reps)
+        byts = 8 * n * k + 8 * k * k
+        out[f"ata_k{k}"] = {"n": n, "k": k, "ata_us": round(t_ata * 1e6, 1),
+                            "ata_GBps": round(byts / t_ata / 1e9, 1),
+                            "atb_AA_us": round(t_atb * 1e6, 1),
+                            "speedup_vs_atb_AA": round(t_atb / t_ata, 3)}
+        del A
+    n, k, m = 1 << 25, 8, 8
+    A = rand(n, k, "f64", 1)
+    X = torch.randn((k, m), dtype=torch.float64, device="cuda",
+                    generator=torch.Generator(device="cuda").manual_seed(5))
+    t = timed(lambda: ops.tsmm(A, X), reps)
+    byts = 8 * n * (k + m) + 8 * k * m
+    out["tsmm_k8_m8"] = {"n": n, "k": k, "m": m, "us": round(t * 1e6, 1),
+                         "GBps": round(byts / t / 1e9, 1)}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
@@ -166,7 +193,8 @@ def main(argv=None) -> int:
     torch.cuda.set_device(0)
     parts = {"plain": lambda: plain(args.reps), "sweep": lambda: sweep(args.reps, args.quick),
              "precision": lambda: precision(args.reps, args.quick),
-             "backward": lambda: backward(args.reps), "batched": lambda: batched(args.reps)}
+             "backward": lambda: backward(args.reps), "batched": lambda: batched(args.reps),
+             "siblings": lambda: siblings(args.reps)}
     report = {}
     for name, fn in parts.items():
         if args.only and name not in args.only:
