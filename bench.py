@@ -60,7 +60,7 @@ def cpu_baseline(k: int, m: int, budget_s: float = 20.0) -> dict:
 
 def secondary() -> dict:
     """The metric's second half and the other workloads, measured like the
-    headline (HIP-event medians, inputs resident):
+    headline (back-to-back calls between HIP events, inputs resident):
       low_precision: k = m = 32, n = 2^25, FP32 and BF16 storage vs FP64 - time,
         speedup and relative error against the FP64 kernel on the same rounded
         inputs (uniform and sign*2^e entries);

@@ -121,7 +121,14 @@ size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
 tsm_status tsm_ata(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
                    int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
                    size_t workspace_bytes, tsm_stream_t stream);
-size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k);
+/* tsm_ata with a kernel request, as tsm_atb_ex (the A^T A kernels of the
+ * tuned square classes come in several rows-per-tile and pipeline depths). */
+tsm_status tsm_ata_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
+                      int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
+                      size_t workspace_bytes, tsm_stream_t stream,
+                      const tsm_kernel_config* config);
+size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k,
+                              const tsm_kernel_config* config);
 
 /* ---- differentiation (no reference counterpart) --------------------------
  * Backward of C = A^T B for an upstream gradient dC (k x m, row-major, lddc):

@@ -126,8 +126,11 @@ def test_ata_argument_errors():
     assert ata(A=FAKE + 4) == _lib.ERR_MISALIGNED
     assert ata(in_type=9) == _lib.ERR_UNSUPPORTED
     assert ata(out_type=_lib.BF16) == _lib.ERR_UNSUPPORTED
-    assert L.tsm_ata_workspace_size(_lib.F64, -1, 8) == 0
-    assert L.tsm_ata_workspace_size(_lib.F64, 10, 65) == 0
+    assert L.tsm_ata_workspace_size(_lib.F64, -1, 8, None) == 0
+    assert L.tsm_ata_workspace_size(_lib.F64, 10, 65, None) == 0
+    bad = _lib.make_config(pf=3)
+    assert L.tsm_ata_ex(_lib.F64, _lib.F64, 10, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
+                        ct.byref(bad)) == _lib.ERR_INVALID_ARGUMENT
 
 
 def test_tsmm_argument_errors():

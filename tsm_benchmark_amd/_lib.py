@@ -25,7 +25,7 @@ EXPORTS = (
     "tsm_atb_batched_workspace_size", "tsm_atb_batched", "tsm_fill_random",
     "tsm_strerror", "tsm_last_hip_error", "tsm_version", "tsm_num_kernels",
     "tsm_kernel_info", "tsm_atb_plan", "tsm_batched_plan_create", "tsm_batched_plan_execute",
-    "tsm_batched_plan_info", "tsm_batched_plan_destroy", "tsm_ata", "tsm_ata_workspace_size",
+    "tsm_batched_plan_info", "tsm_batched_plan_destroy", "tsm_ata", "tsm_ata_ex", "tsm_ata_workspace_size",
     "tsm_tsmm",
 )
 
@@ -93,8 +93,10 @@ def lib() -> ct.CDLL:
     L.tsm_atb_backward.argtypes = [I, I, I64, I, I, P, I64, P, I64, P, I64, P, I64, P, I64, P]
     L.tsm_ata.restype = I
     L.tsm_ata.argtypes = [I, I, I64, I, P, I64, P, I64, I, P, SZ, P]
+    L.tsm_ata_ex.restype = I
+    L.tsm_ata_ex.argtypes = [I, I, I64, I, P, I64, P, I64, I, P, SZ, P, ct.POINTER(KernelConfig)]
     L.tsm_ata_workspace_size.restype = SZ
-    L.tsm_ata_workspace_size.argtypes = [I, I64, I]
+    L.tsm_ata_workspace_size.argtypes = [I, I64, I, ct.POINTER(KernelConfig)]
     L.tsm_tsmm.restype = I
     L.tsm_tsmm.argtypes = [I, I, I64, I, I, P, I64, P, I64, P, I64, P]
     L.tsm_atb_batched_workspace_size.restype = SZ

@@ -269,14 +269,29 @@ def load_tuning(path: str = TUNING_JSON) -> dict:
         return json.load(f)
 
 
-def sym_kernels(defaults: dict) -> dict:
-    """The A^T A kernel per (dtype, width class), from the tuned A^T B
-    configuration of the square class (same consumer, grid and tile order).
-    A stage holds A only, so register-tile kernels take twice the rows where
-    that is valid: the same bytes in flight per tile as the A^T B kernel
-    (measured: FP64 k = 8 streams at 6.97 TB/s with 512 rows, 5.4 TB/s with
-    256).  Matrix-core kernels keep their rows: they are bound by the
-    matrix instructions, which the symmetric kernel trims instead."""
+def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
+    """A^T A variants the autotuner chooses from for a square tuned class: the
+    tuned A^T B consumer of the class with half, one, two and four times its
+    rows per tile and one or two tiles in flight."""
+    c = defaults[(dtype, kc, kc)][0]
+    out = set()
+    for rows in (c.rows // 2, c.rows, 2 * c.rows, 4 * c.rows):
+        for pf in (1, 2):
+            v = dataclasses.replace(c, sym=1, rows=rows, pf=pf)
+            if rows >= 1 and not v.problems():
+                out.add(v)
+    return sorted(out)
+
+
+def sym_kernels(defaults: dict, tuning: dict) -> tuple[list[KernelConfig], dict]:
+    """All A^T A kernels of the library and the default per (dtype, class).
+
+    Without a tuned entry the default is the tuned A^T B configuration of the
+    square class (same consumer, grid and tile order); a stage holds A only,
+    so register-tile kernels take twice the rows where that is valid, which
+    keeps the bytes in flight per tile.  Tuned entries come from the
+    "sym_entries" of the tile table."""
+    cfgs: set[KernelConfig] = set()
     out = {}
     for dt in DTYPES:
         for kc in CLASSES:
@@ -288,8 +303,18 @@ def sym_kernels(defaults: dict) -> dict:
                     s = wide
             if s.problems():
                 raise ValueError(f"{s.name}: {s.problems()}")
+            cfgs.add(s)
             out[(dt, kc)] = (s, wpc, il)
-    return out
+            if (kc, kc) in TUNE_SHAPES[dt]:
+                cfgs.update(sym_variants(defaults, dt, kc))
+    for e in tuning.get("sym_entries", []):
+        c = KernelConfig(e["dtype"], e["kc"], e["kc"], e["tm"], e["tn"], e["rows"], e["pf"],
+                         int(e.get("nt", 0)), 1)
+        if c.problems():
+            raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
+        cfgs.add(c)
+        out[(c.dtype, c.kc)] = (c, int(e.get("wgs_per_cu", 1)), int(e.get("interleave", 0)))
+    return sorted(cfgs), out
 
 
 def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
@@ -402,21 +427,26 @@ def emit_batched_file(dtype: str, classes: list[KernelConfig]) -> str:
     return "\n".join(out)
 
 
-def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
+def emit_table(cfgs: list[KernelConfig], defaults: dict, tuning: dict) -> str:
     idx = {c: i for i, c in enumerate(cfgs)}
-    syms = sym_kernels(defaults)
+    sym_cfgs, sym_defaults = sym_kernels(defaults, tuning)
+    sidx = {c: i for i, c in enumerate(sym_cfgs)}
     out = [BANNER, '#include "../tsm_internal.hpp"\n']
-    for c in cfgs + [v[0] for v in syms.values()]:
+    for c in cfgs + sym_cfgs:
         out.append(f"hipError_t launch_{c.name}(const tsm::AtbArgs&, unsigned, hipStream_t);")
     out.append("\nstatic const tsm::KernelEntry kTable[] = {")
     for c in cfgs:
         out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
                    f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
     out.append("};\n")
-    out.append("static const tsm::SymEntry kSym[] = {")
-    for (dt, kc), (c, wpc, il) in sorted(syms.items()):
-        out.append(f"  {{\"{c.name}\", {DTYPE_ID[dt]}, {kc}, {c.rows}, {wpc}, {il}, "
-                   f"launch_{c.name}}},")
+    out.append("static const tsm::KernelEntry kSym[] = {")
+    for c in sym_cfgs:
+        out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
+                   f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
+    out.append("};\n")
+    out.append("static const tsm::TunedEntry kSymTuned[] = {")
+    for (dt, kc), (c, wpc, il) in sorted(sym_defaults.items()):
+        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {kc}, {sidx[c]}, {wpc}, {il}}},  // {c.name}")
     out.append("};\n")
     out.append("static const tsm::TunedEntry kTuned[] = {")
     for (dt, kc, mc), (c, wpc, il) in sorted(defaults.items()):
@@ -444,8 +474,10 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict) -> str:
                f"  *count = {len(DTYPES)};\n  return batched_all();\n}}")
     out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {\n"
                "  *count = (int)(sizeof(kTable) / sizeof(kTable[0]));\n  return kTable;\n}")
-    out.append("extern \"C\" const tsm::SymEntry* tsm_gen_sym_table(int* count) {\n"
+    out.append("extern \"C\" const tsm::KernelEntry* tsm_gen_sym_table(int* count) {\n"
                "  *count = (int)(sizeof(kSym) / sizeof(kSym[0]));\n  return kSym;\n}")
+    out.append("extern \"C\" const tsm::TunedEntry* tsm_gen_sym_tuned_table(int* count) {\n"
+               "  *count = (int)(sizeof(kSymTuned) / sizeof(kSymTuned[0]));\n  return kSymTuned;\n}")
     out.append("extern \"C\" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {\n"
                "  *count = (int)(sizeof(kTuned) / sizeof(kTuned[0]));\n  return kTuned;\n}")
     return "\n".join(out) + "\n"
@@ -464,8 +496,9 @@ def _write_if_changed(path: str, text: str) -> bool:
 
 def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> list[str]:
     """Write the library's generated sources; returns the .hip/.cpp paths."""
-    cfgs, defaults = library_kernels(load_tuning(tuning_path))
-    syms = [v[0] for v in sym_kernels(defaults).values()]
+    tuning = load_tuning(tuning_path)
+    cfgs, defaults = library_kernels(tuning)
+    syms, _ = sym_kernels(defaults, tuning)
     paths = []
     for dt in DTYPES:
         for kc in CLASSES:
@@ -483,7 +516,7 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
         _write_if_changed(p, emit_batched_file(dt, batched_classes(defaults, dt)))
         paths.append(p)
     p = os.path.join(out_dir, "tsm_table.hip")
-    _write_if_changed(p, emit_table(cfgs, defaults))
+    _write_if_changed(p, emit_table(cfgs, defaults, tuning))
     paths.append(p)
     return paths
 

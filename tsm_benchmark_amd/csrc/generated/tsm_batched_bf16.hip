@@ -31,7 +31,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_bf16(tsm::Batc
       case 18: tsm::batched_item_call<tsm::bf16, 8, 1, 8, 1, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x1_t8x1_r1024_p2
       case 19: tsm::batched_item_call<tsm::bf16, 8, 2, 8, 2, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x2_t8x2_r1024_p2
       case 20: tsm::batched_item_call<tsm::bf16, 8, 4, 8, 4, 1024, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x4_t8x4_r1024_p1
-      case 21: tsm::batched_item_call<tsm::bf16, 8, 8, 8, 4, 1024, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x8_t8x4_r1024_p1
+      case 21: tsm::batched_item_call<tsm::bf16, 8, 8, 8, 4, 1024, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
       case 22: tsm::batched_item_call<tsm::bf16, 8, 16, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x16_t8x4_r512_p1
       case 23: tsm::batched_item_call<tsm::bf16, 8, 32, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x32_t8x4_r256_p1
       case 24: tsm::batched_item_call<tsm::bf16, 16, 1, 8, 1, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x1_t8x1_r512_p2

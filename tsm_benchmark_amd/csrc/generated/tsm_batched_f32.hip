@@ -38,7 +38,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f32(tsm::Batch
       case 25: tsm::batched_item_call<float, 16, 2, 8, 2, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x2_t8x2_r256_p2
       case 26: tsm::batched_item_call<float, 16, 4, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x4_t8x4_r256_p1
       case 27: tsm::batched_item_call<float, 16, 8, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x8_t8x4_r256_p1
-      case 28: tsm::batched_item_call<float, 16, 16, 8, 4, 256, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x16_t8x4_r256_p1_nt
+      case 28: tsm::batched_item_call<float, 16, 16, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x16_mfma_r128_p2_nt
       case 29: tsm::batched_item_call<float, 16, 32, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x32_t8x4_r128_p1
       case 30: tsm::batched_item_call<float, 32, 1, 8, 1, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x1_t8x1_r128_p2
       case 31: tsm::batched_item_call<float, 32, 2, 8, 2, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x2_t8x2_r128_p2
@@ -80,7 +80,7 @@ static const tsm::BatchClass kClasses[] = {
   {16, 2, 256},
   {16, 4, 256},
   {16, 8, 256},
-  {16, 16, 256},
+  {16, 16, 128},
   {16, 32, 128},
   {32, 1, 128},
   {32, 2, 128},

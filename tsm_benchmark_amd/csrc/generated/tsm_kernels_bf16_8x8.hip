@@ -223,12 +223,62 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
-// tsm_ata_bf16_8x8_t8x8_r2048_p2: KC=8 MC=8 TM=8 TN=8 rows=2048 pf=2 nt=0
-//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x8_r2048_p2(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 8, 8, 8, 8, 2048, 2, 0, 1>(a);
+// tsm_ata_bf16_8x8_t8x4_r512_p1_nt: KC=8 MC=8 TM=8 TN=4 rows=512 pf=1 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r512_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 512, 1, 1, 1>(a);
 }
-hipError_t launch_tsm_ata_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x8_r2048_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r512_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_t8x4_r512_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=512 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 512, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_t8x4_r1024_p1_nt: KC=8 MC=8 TM=8 TN=4 rows=1024 pf=1 nt=1
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 1024, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r1024_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_t8x4_r1024_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=1024 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 1024, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_t8x4_r2048_p1_nt: KC=8 MC=8 TM=8 TN=4 rows=2048 pf=1 nt=1
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r2048_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 2048, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r2048_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_t8x4_r2048_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=2048 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 2048, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_t8x4_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -323,12 +323,42 @@ hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsig
   return hipGetLastError();
 }
 
-// tsm_ata_f64_16x16_t8x8_r256_p2_nt: KC=16 MC=16 TM=8 TN=8 rows=256 pf=2 nt=1
-//   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_t8x8_r256_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 16, 16, 8, 8, 256, 2, 1, 1>(a);
+// tsm_ata_f64_16x16_mfma_r128_p1_nt: KC=16 MC=16 TM=0 TN=0 rows=128 pf=1 nt=1
+//   matrix cores: 32 K-steps of 4 rows per tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_mfma_r128_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 0, 0, 128, 1, 1, 1>(a);
 }
-hipError_t launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_16x16_t8x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_16x16_mfma_r128_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_16x16_mfma_r128_p2_nt: KC=16 MC=16 TM=0 TN=0 rows=128 pf=2 nt=1
+//   matrix cores: 32 K-steps of 4 rows per tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_mfma_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 0, 0, 128, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_16x16_mfma_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_16x16_mfma_r256_p1_nt: KC=16 MC=16 TM=0 TN=0 rows=256 pf=1 nt=1
+//   matrix cores: 64 K-steps of 4 rows per tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_mfma_r256_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 0, 0, 256, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_16x16_mfma_r256_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_16x16_mfma_r256_p2_nt: KC=16 MC=16 TM=0 TN=0 rows=256 pf=2 nt=1
+//   matrix cores: 64 K-steps of 4 rows per tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_16x16_mfma_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 0, 0, 256, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_16x16_mfma_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

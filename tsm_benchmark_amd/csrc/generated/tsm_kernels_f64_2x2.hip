@@ -123,6 +123,16 @@ hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
+// tsm_ata_f64_2x2_t2x2_r1024_p1_nt: KC=2 MC=2 TM=2 TN=2 rows=1024 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r1024_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 1024, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r1024_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_ata_f64_2x2_t2x2_r1024_p2_nt: KC=2 MC=2 TM=2 TN=2 rows=1024 pf=2 nt=1
 //   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 32768 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r1024_p2_nt(tsm::AtbArgs a) {
@@ -130,5 +140,45 @@ extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r1024_p2_
 }
 hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_2x2_t2x2_r2048_p1_nt: KC=2 MC=2 TM=2 TN=2 rows=2048 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r2048_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 2048, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r2048_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r2048_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_2x2_t2x2_r2048_p2_nt: KC=2 MC=2 TM=2 TN=2 rows=2048 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 2048, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_2x2_t2x2_r4096_p1_nt: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r4096_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 4096, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r4096_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r4096_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_2x2_t2x2_r4096_p2_nt: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_2x2_t2x2_r4096_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 4096, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_2x2_t2x2_r4096_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -143,12 +143,62 @@ hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
-// tsm_ata_f64_4x4_t4x4_r2048_p1_nt: KC=4 MC=4 TM=4 TN=4 rows=2048 pf=1 nt=1
-//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r2048_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 4, 4, 4, 4, 2048, 1, 1, 1>(a);
+// tsm_ata_f64_4x4_t4x4_r256_p1_nt: KC=4 MC=4 TM=4 TN=4 rows=256 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 1 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r256_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 256, 1, 1, 1>(a);
 }
-hipError_t launch_tsm_ata_f64_4x4_t4x4_r2048_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r2048_p1_nt, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r256_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r256_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_4x4_t4x4_r256_p2_nt: KC=4 MC=4 TM=4 TN=4 rows=256 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 1 rows/thread/tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 256, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_4x4_t4x4_r512_p1_nt: KC=4 MC=4 TM=4 TN=4 rows=512 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r512_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 512, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r512_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_4x4_t4x4_r512_p2_nt: KC=4 MC=4 TM=4 TN=4 rows=512 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 512, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_4x4_t4x4_r1024_p1_nt: KC=4 MC=4 TM=4 TN=4 rows=1024 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r1024_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 1024, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r1024_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_4x4_t4x4_r1024_p2_nt: KC=4 MC=4 TM=4 TN=4 rows=1024 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_4x4_t4x4_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 1024, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_4x4_t4x4_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -163,6 +163,36 @@ hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigne
   return hipGetLastError();
 }
 
+// tsm_ata_f64_8x8_t8x8_r256_p1_nt: KC=8 MC=8 TM=8 TN=8 rows=256 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r256_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 256, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r256_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r256_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_8x8_t8x8_r256_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=256 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 256, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_8x8_t8x8_r512_p1_nt: KC=8 MC=8 TM=8 TN=8 rows=512 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r512_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 512, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r512_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_ata_f64_8x8_t8x8_r512_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=512 pf=2 nt=1
 //   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r512_p2_nt(tsm::AtbArgs a) {
@@ -170,5 +200,25 @@ extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r512_p2_n
 }
 hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_8x8_t8x8_r1024_p1_nt: KC=8 MC=8 TM=8 TN=8 rows=1024 pf=1 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r1024_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 1024, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r1024_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_8x8_t8x8_r1024_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=1024 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_8x8_t8x8_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 8, 1024, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_8x8_t8x8_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

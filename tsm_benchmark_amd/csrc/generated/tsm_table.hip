@@ -521,27 +521,106 @@ hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_4x4_t4x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_16x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_4x4_t4x4_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 
 static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2},
@@ -1064,28 +1143,131 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_64x64_t8x8_r64_p2_nt", 0, 64, 64, 8, 8, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt},
 };
 
-static const tsm::SymEntry kSym[] = {
-  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 4096, 2, 0, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2},
-  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 4096, 2, 0, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2},
-  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4096, 2, 0, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2},
-  {"tsm_ata_bf16_8x8_t8x8_r2048_p2", 2, 8, 2048, 1, 1, launch_tsm_ata_bf16_8x8_t8x8_r2048_p2},
-  {"tsm_ata_bf16_16x16_mfma_r256_p2_nt", 2, 16, 256, 1, 1, launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt},
-  {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 256, 1, 1, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt},
-  {"tsm_ata_bf16_64x64_mfma_r256_p1_nt", 2, 64, 256, 1, 1, launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt},
-  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 4096, 2, 0, launch_tsm_ata_f32_1x1_t1x1_r4096_p2},
-  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 4096, 2, 0, launch_tsm_ata_f32_2x2_t2x2_r4096_p2},
-  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 2048, 2, 0, launch_tsm_ata_f32_4x4_t4x4_r2048_p2},
-  {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 2048, 1, 1, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt},
-  {"tsm_ata_f32_16x16_t8x4_r512_p2_nt", 1, 16, 512, 1, 0, launch_tsm_ata_f32_16x16_t8x4_r512_p2_nt},
-  {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 128, 1, 0, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt},
-  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 128, 1, 0, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt},
-  {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 2048, 1, 1, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt},
-  {"tsm_ata_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 1024, 1, 1, launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt},
-  {"tsm_ata_f64_4x4_t4x4_r2048_p1_nt", 0, 4, 2048, 1, 1, launch_tsm_ata_f64_4x4_t4x4_r2048_p1_nt},
-  {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 512, 1, 1, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt},
-  {"tsm_ata_f64_16x16_t8x8_r256_p2_nt", 0, 16, 256, 1, 1, launch_tsm_ata_f64_16x16_t8x8_r256_p2_nt},
-  {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 128, 1, 1, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt},
-  {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 32, 1, 1, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt},
+static const tsm::KernelEntry kSym[] = {
+  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2},
+  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2},
+  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4, 4, 4, 4096, 2, 0, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2},
+  {"tsm_ata_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt},
+  {"tsm_ata_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt},
+  {"tsm_ata_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt},
+  {"tsm_ata_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt},
+  {"tsm_ata_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt},
+  {"tsm_ata_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt},
+  {"tsm_ata_bf16_16x16_mfma_r128_p1_nt", 2, 16, 16, 0, 0, 128, 1, 1, 8192, launch_tsm_ata_bf16_16x16_mfma_r128_p1_nt},
+  {"tsm_ata_bf16_16x16_mfma_r128_p2_nt", 2, 16, 16, 0, 0, 128, 2, 1, 8192, launch_tsm_ata_bf16_16x16_mfma_r128_p2_nt},
+  {"tsm_ata_bf16_16x16_mfma_r256_p1_nt", 2, 16, 16, 0, 0, 256, 1, 1, 16384, launch_tsm_ata_bf16_16x16_mfma_r256_p1_nt},
+  {"tsm_ata_bf16_16x16_mfma_r256_p2_nt", 2, 16, 16, 0, 0, 256, 2, 1, 16384, launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt},
+  {"tsm_ata_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt},
+  {"tsm_ata_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt},
+  {"tsm_ata_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt},
+  {"tsm_ata_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt},
+  {"tsm_ata_bf16_32x32_mfma_r128_p1_nt", 2, 32, 32, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_bf16_32x32_mfma_r128_p1_nt},
+  {"tsm_ata_bf16_32x32_mfma_r128_p2_nt", 2, 32, 32, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_bf16_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt},
+  {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt},
+  {"tsm_ata_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt},
+  {"tsm_ata_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p2_nt},
+  {"tsm_ata_bf16_32x32_mfma_r1024_p1_nt", 2, 32, 32, 0, 0, 1024, 1, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p1_nt},
+  {"tsm_ata_bf16_32x32_mfma_r1024_p2_nt", 2, 32, 32, 0, 0, 1024, 2, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p2_nt},
+  {"tsm_ata_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p1_nt},
+  {"tsm_ata_bf16_64x64_mfma_r128_p2_nt", 2, 64, 64, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p2_nt},
+  {"tsm_ata_bf16_64x64_mfma_r256_p1_nt", 2, 64, 64, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt},
+  {"tsm_ata_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt},
+  {"tsm_ata_bf16_64x64_mfma_r512_p1_nt", 2, 64, 64, 0, 0, 512, 1, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt},
+  {"tsm_ata_bf16_64x64_mfma_r512_p2_nt", 2, 64, 64, 0, 0, 512, 2, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt},
+  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2},
+  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2},
+  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2},
+  {"tsm_ata_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt},
+  {"tsm_ata_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt},
+  {"tsm_ata_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt},
+  {"tsm_ata_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt},
+  {"tsm_ata_f32_8x8_t8x4_r2048_p1_nt", 1, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt},
+  {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt},
+  {"tsm_ata_f32_16x16_mfma_r64_p1_nt", 1, 16, 16, 0, 0, 64, 1, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p1_nt},
+  {"tsm_ata_f32_16x16_mfma_r64_p2_nt", 1, 16, 16, 0, 0, 64, 2, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p2_nt},
+  {"tsm_ata_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p1_nt},
+  {"tsm_ata_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p2_nt},
+  {"tsm_ata_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p1_nt},
+  {"tsm_ata_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p2_nt},
+  {"tsm_ata_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p1_nt},
+  {"tsm_ata_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p2_nt},
+  {"tsm_ata_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p1_nt},
+  {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p1_nt},
+  {"tsm_ata_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p2_nt},
+  {"tsm_ata_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p1_nt},
+  {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt},
+  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt},
+  {"tsm_ata_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p2_nt},
+  {"tsm_ata_f32_64x64_mfma_r256_p1_nt", 1, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p1_nt},
+  {"tsm_ata_f32_64x64_mfma_r256_p2_nt", 1, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p2_nt},
+  {"tsm_ata_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt},
+  {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt},
+  {"tsm_ata_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt},
+  {"tsm_ata_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p2_nt},
+  {"tsm_ata_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p1_nt},
+  {"tsm_ata_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt},
+  {"tsm_ata_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p1_nt},
+  {"tsm_ata_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p2_nt},
+  {"tsm_ata_f64_2x2_t2x2_r4096_p1_nt", 0, 2, 2, 2, 2, 4096, 1, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p1_nt},
+  {"tsm_ata_f64_2x2_t2x2_r4096_p2_nt", 0, 2, 2, 2, 2, 4096, 2, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p2_nt},
+  {"tsm_ata_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p1_nt},
+  {"tsm_ata_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p2_nt},
+  {"tsm_ata_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt},
+  {"tsm_ata_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt},
+  {"tsm_ata_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt},
+  {"tsm_ata_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt},
+  {"tsm_ata_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p1_nt},
+  {"tsm_ata_f64_8x8_t8x8_r256_p2_nt", 0, 8, 8, 8, 8, 256, 2, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p2_nt},
+  {"tsm_ata_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt},
+  {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt},
+  {"tsm_ata_f64_8x8_t8x8_r1024_p1_nt", 0, 8, 8, 8, 8, 1024, 1, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt},
+  {"tsm_ata_f64_8x8_t8x8_r1024_p2_nt", 0, 8, 8, 8, 8, 1024, 2, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt},
+  {"tsm_ata_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p1_nt},
+  {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt},
+  {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt},
+  {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt},
+  {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt},
+  {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt},
+  {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt},
+  {"tsm_ata_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p2_nt},
+  {"tsm_ata_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p1_nt},
+  {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt},
+  {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt},
+  {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p1_nt},
+  {"tsm_ata_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p2_nt},
+  {"tsm_ata_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p1_nt},
+  {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt},
+  {"tsm_ata_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p1_nt},
+  {"tsm_ata_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p2_nt},
+  {"tsm_ata_f64_64x64_mfma_r128_p1_nt", 0, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p1_nt},
+  {"tsm_ata_f64_64x64_mfma_r128_p2_nt", 0, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p2_nt},
+};
+
+static const tsm::TunedEntry kSymTuned[] = {
+  {2, 1, 1, 0, 2, 0},  // tsm_ata_bf16_1x1_t1x1_r4096_p2
+  {2, 2, 2, 1, 2, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
+  {2, 4, 4, 2, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
+  {2, 8, 8, 5, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
+  {2, 16, 16, 16, 1, 1},  // tsm_ata_bf16_16x16_mfma_r1024_p2_nt
+  {2, 32, 32, 22, 1, 1},  // tsm_ata_bf16_32x32_mfma_r512_p2_nt
+  {2, 64, 64, 29, 1, 0},  // tsm_ata_bf16_64x64_mfma_r512_p1_nt
+  {1, 1, 1, 31, 2, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
+  {1, 2, 2, 32, 2, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
+  {1, 4, 4, 33, 2, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
+  {1, 8, 8, 34, 2, 0},  // tsm_ata_f32_8x8_t8x4_r512_p1_nt
+  {1, 16, 16, 45, 1, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
+  {1, 32, 32, 51, 1, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
+  {1, 64, 64, 54, 2, 0},  // tsm_ata_f32_64x64_mfma_r128_p1_nt
+  {0, 1, 1, 59, 1, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p2_nt
+  {0, 2, 2, 62, 2, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p1_nt
+  {0, 4, 4, 71, 1, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 75, 1, 1},  // tsm_ata_f64_8x8_t8x8_r256_p2_nt
+  {0, 16, 16, 81, 1, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
+  {0, 32, 32, 91, 1, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
+  {0, 64, 64, 97, 2, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
 };
 
 static const tsm::TunedEntry kTuned[] = {
@@ -1113,7 +1295,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 8, 1, 21, 2, 0},  // tsm_atb_bf16_8x1_t8x1_r1024_p2
   {2, 8, 2, 22, 2, 0},  // tsm_atb_bf16_8x2_t8x2_r1024_p2
   {2, 8, 4, 23, 2, 0},  // tsm_atb_bf16_8x4_t8x4_r1024_p2
-  {2, 8, 8, 43, 1, 1},  // tsm_atb_bf16_8x8_t8x8_r1024_p2
+  {2, 8, 8, 33, 2, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
   {2, 8, 16, 46, 2, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2
   {2, 8, 32, 47, 2, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2
   {2, 8, 64, 48, 2, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2
@@ -1170,7 +1352,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 16, 2, 207, 2, 0},  // tsm_atb_f32_16x2_t8x2_r256_p2
   {1, 16, 4, 208, 2, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2
   {1, 16, 8, 209, 2, 0},  // tsm_atb_f32_16x8_t8x4_r256_p2
-  {1, 16, 16, 229, 1, 0},  // tsm_atb_f32_16x16_t8x4_r256_p2_nt
+  {1, 16, 16, 213, 1, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
   {1, 16, 32, 240, 2, 0},  // tsm_atb_f32_16x32_t8x4_r128_p2
   {1, 16, 64, 241, 2, 0},  // tsm_atb_f32_16x64_mfma_r64_p2
   {1, 32, 1, 242, 2, 0},  // tsm_atb_f32_32x1_t8x1_r128_p2
@@ -1178,7 +1360,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 32, 4, 244, 2, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
   {1, 32, 8, 245, 2, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
   {1, 32, 16, 246, 2, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
-  {1, 32, 32, 254, 1, 0},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+  {1, 32, 32, 254, 1, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
   {1, 32, 64, 281, 2, 0},  // tsm_atb_f32_32x64_mfma_r64_p2
   {1, 64, 1, 282, 2, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
   {1, 64, 2, 283, 2, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
@@ -1186,8 +1368,8 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 64, 8, 285, 2, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
   {1, 64, 16, 286, 2, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
   {1, 64, 32, 287, 2, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
-  {1, 64, 64, 297, 1, 0},  // tsm_atb_f32_64x64_mfma_r128_p1_nt
-  {0, 1, 1, 315, 1, 1},  // tsm_atb_f64_1x1_t1x1_r1024_p2_nt
+  {1, 64, 64, 297, 1, 1},  // tsm_atb_f32_64x64_mfma_r128_p1_nt
+  {0, 1, 1, 321, 1, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
   {0, 1, 2, 324, 2, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
   {0, 1, 4, 325, 2, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
   {0, 1, 8, 326, 2, 0},  // tsm_atb_f64_1x8_t1x8_r256_p2
@@ -1195,7 +1377,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 1, 32, 328, 2, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
   {0, 1, 64, 329, 2, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
   {0, 2, 1, 330, 2, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
-  {0, 2, 2, 334, 1, 1},  // tsm_atb_f64_2x2_t2x2_r512_p2_nt
+  {0, 2, 2, 340, 1, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
   {0, 2, 4, 343, 2, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
   {0, 2, 8, 344, 2, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
   {0, 2, 16, 345, 2, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
@@ -1203,10 +1385,10 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 2, 64, 347, 2, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
   {0, 4, 1, 348, 2, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
   {0, 4, 2, 349, 2, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
-  {0, 4, 4, 361, 1, 1},  // tsm_atb_f64_4x4_t4x4_r1024_p1_nt
+  {0, 4, 4, 353, 2, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
   {0, 4, 8, 364, 2, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
   {0, 4, 16, 365, 2, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
-  {0, 4, 32, 375, 1, 1},  // tsm_atb_f64_4x32_t4x8_r64_p2_nt
+  {0, 4, 32, 373, 2, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
   {0, 4, 64, 380, 2, 0},  // tsm_atb_f64_4x64_t4x8_r32_p2
   {0, 8, 1, 381, 2, 0},  // tsm_atb_f64_8x1_t8x1_r256_p2
   {0, 8, 2, 382, 2, 0},  // tsm_atb_f64_8x2_t8x2_r256_p2
@@ -1219,15 +1401,15 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 16, 2, 404, 2, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
   {0, 16, 4, 405, 2, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
   {0, 16, 8, 406, 2, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
-  {0, 16, 16, 434, 1, 1},  // tsm_atb_f64_16x16_t8x8_r128_p2_nt
+  {0, 16, 16, 416, 1, 1},  // tsm_atb_f64_16x16_mfma_r256_p1_nt
   {0, 16, 32, 439, 2, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
   {0, 16, 64, 440, 2, 0},  // tsm_atb_f64_16x64_mfma_r32_p2
   {0, 32, 1, 441, 2, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
   {0, 32, 2, 442, 2, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
-  {0, 32, 4, 456, 1, 1},  // tsm_atb_f64_32x4_t8x4_r128_p2_nt
+  {0, 32, 4, 450, 2, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
   {0, 32, 8, 457, 2, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
   {0, 32, 16, 458, 2, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 470, 1, 1},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
+  {0, 32, 32, 462, 1, 1},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
   {0, 32, 64, 487, 2, 0},  // tsm_atb_f64_32x64_mfma_r32_p2
   {0, 64, 1, 488, 2, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
   {0, 64, 2, 489, 2, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
@@ -1235,7 +1417,7 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 64, 8, 491, 2, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
   {0, 64, 16, 492, 2, 0},  // tsm_atb_f64_64x16_mfma_r32_p2
   {0, 64, 32, 493, 2, 0},  // tsm_atb_f64_64x32_mfma_r32_p2
-  {0, 64, 64, 501, 1, 1},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
+  {0, 64, 64, 501, 1, 0},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
@@ -1269,9 +1451,13 @@ extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count) {
   *count = (int)(sizeof(kTable) / sizeof(kTable[0]));
   return kTable;
 }
-extern "C" const tsm::SymEntry* tsm_gen_sym_table(int* count) {
+extern "C" const tsm::KernelEntry* tsm_gen_sym_table(int* count) {
   *count = (int)(sizeof(kSym) / sizeof(kSym[0]));
   return kSym;
+}
+extern "C" const tsm::TunedEntry* tsm_gen_sym_tuned_table(int* count) {
+  *count = (int)(sizeof(kSymTuned) / sizeof(kSymTuned[0]));
+  return kSymTuned;
 }
 extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count) {
   *count = (int)(sizeof(kTuned) / sizeof(kTuned[0]));

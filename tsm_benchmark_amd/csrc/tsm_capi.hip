@@ -72,11 +72,31 @@ struct Plan {
   size_t workspace = 0;
 };
 
-tsm_status find_kernel(tsm_dtype in_type, int kc, int mc, const tsm_kernel_config* cfg,
-                       int* index, int* wgs_per_cu, int* interleave) {
-  int ntuned = 0, nk = 0;
-  const tsm::TunedEntry* tuned = tsm_gen_tuned_table(&ntuned);
-  const tsm::KernelEntry* ks = table(&nk);
+// The kernels of one operation: A^T B (sym = false) or A^T A (sym = true).
+struct Tables {
+  const tsm::KernelEntry* ks;
+  int nk;
+  const tsm::TunedEntry* tuned;
+  int ntuned;
+};
+Tables tables(bool sym) {
+  Tables t{};
+  if (sym) {
+    t.ks = tsm_gen_sym_table(&t.nk);
+    t.tuned = tsm_gen_sym_tuned_table(&t.ntuned);
+  } else {
+    t.ks = table(&t.nk);
+    t.tuned = tsm_gen_tuned_table(&t.ntuned);
+  }
+  return t;
+}
+
+tsm_status find_kernel(const Tables& tb, tsm_dtype in_type, int kc, int mc,
+                       const tsm_kernel_config* cfg, int* index, int* wgs_per_cu,
+                       int* interleave) {
+  const int ntuned = tb.ntuned, nk = tb.nk;
+  const tsm::TunedEntry* tuned = tb.tuned;
+  const tsm::KernelEntry* ks = tb.ks;
   int def = -1, def_wpc = 2, def_il = 0;
   for (int i = 0; i < ntuned; ++i)
     if (tuned[i].dtype == (int)in_type && tuned[i].kc == kc && tuned[i].mc == mc) {
@@ -129,28 +149,23 @@ tsm_status run_plan(const Plan& p, tsm::LaunchFn launch, int kc, int mc, tsm_dty
 tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int64_t cfg_grid,
                     int kc, int mc);
 
+// Kernel and launch shape of A^T B (sym = false, k x m) or A^T A (sym = true,
+// m == k).
 tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kernel_config* cfg,
-                     Plan* p) {
+                     Plan* p, bool sym = false) {
   const int kc = width_class(k), mc = width_class(m);
   int wpc = 2, interleave = 0;
-  tsm_status st = find_kernel(in_type, kc, mc, cfg, &p->kernel, &wpc, &interleave);
+  const Tables tb = tables(sym);
+  tsm_status st = find_kernel(tb, in_type, kc, mc, cfg, &p->kernel, &wpc, &interleave);
   if (st != TSM_SUCCESS) return st;
-  int nk = 0;
-  p->entry = &table(&nk)[p->kernel];
+  p->entry = &tb.ks[p->kernel];
   return set_grid(p, n, p->entry->rows, wpc, interleave, cfg ? cfg->grid : 0, kc, mc);
 }
 
-// The A^T A kernel of width class width_class(k) and its launch shape.
-tsm_status make_sym_plan(tsm_dtype in_type, int64_t n, int k, Plan* p,
-                         const tsm::SymEntry** entry) {
-  const int kc = width_class(k);
-  int ns = 0;
-  const tsm::SymEntry* syms = tsm_gen_sym_table(&ns);
-  *entry = nullptr;
-  for (int i = 0; i < ns; ++i)
-    if (syms[i].dtype == (int)in_type && syms[i].kc == kc) *entry = &syms[i];
-  if (!*entry) return TSM_ERR_NO_KERNEL;
-  return set_grid(p, n, (*entry)->rows, (*entry)->wgs_per_cu, (*entry)->interleave, 0, kc, kc);
+bool valid_config(const tsm_kernel_config* c) {
+  return !c || !(c->pf < 0 || c->pf > 2 || c->tm < 0 || c->tn < 0 || c->rows < 0 ||
+                 c->wgs_per_cu < 0 || c->grid < 0 || c->load_policy < 0 || c->load_policy > 2 ||
+                 c->tile_order < 0 || c->tile_order > 2 || c->engine < 0 || c->engine > 2);
 }
 
 tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int64_t cfg_grid,
@@ -320,12 +335,7 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
                       tsm_stream_t stream, const tsm_kernel_config* config) {
   tsm_status st = check_atb(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc);
   if (st != TSM_SUCCESS) return st;
-  if (config && (config->pf < 0 || config->pf > 2 || config->tm < 0 || config->tn < 0 ||
-                 config->rows < 0 || config->wgs_per_cu < 0 || config->grid < 0 ||
-                 config->load_policy < 0 || config->load_policy > 2 ||
-                 config->tile_order < 0 || config->tile_order > 2 || config->engine < 0 ||
-                 config->engine > 2))
-    return TSM_ERR_INVALID_ARGUMENT;
+  if (!valid_config(config)) return TSM_ERR_INVALID_ARGUMENT;
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);
   if (st != TSM_SUCCESS) return st;
@@ -384,27 +394,37 @@ tsm_status run_plan(const Plan& p, tsm::LaunchFn launch, int kc, int mc, tsm_dty
 
 extern "C" {
 
-size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k) {
-  if (!valid_in_type(in_type) || n < 0 || k < 1 || k > TSM_MAX_WIDTH) return 0;
+size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k,
+                              const tsm_kernel_config* config) {
+  if (!valid_in_type(in_type) || n < 0 || k < 1 || k > TSM_MAX_WIDTH || !valid_config(config))
+    return 0;
   Plan p;
-  const tsm::SymEntry* e = nullptr;
-  if (make_sym_plan(in_type, n, k, &p, &e) != TSM_SUCCESS) return 0;
+  if (make_plan(in_type, n, k, k, config, &p, true) != TSM_SUCCESS) return 0;
   return p.workspace;
+}
+
+tsm_status tsm_ata_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
+                      int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
+                      size_t workspace_bytes, tsm_stream_t stream,
+                      const tsm_kernel_config* config) {
+  tsm_status st = check_atb(in_type, out_type, n, k, k, A, lda, A, lda, C, ldc);
+  if (st != TSM_SUCCESS) return st;
+  if (!valid_config(config)) return TSM_ERR_INVALID_ARGUMENT;
+  Plan p;
+  st = make_plan(in_type, n, k, k, config, &p, true);
+  if (st != TSM_SUCCESS) return st;
+  const tsm::KernelEntry& e = *p.entry;
+  const bool dense = k == e.kc && lda == k && ((uintptr_t)A % 16) == 0;
+  return run_plan(p, e.launch, e.kc, e.kc, out_type, n, k, k, A, lda, A, lda, C, ldc,
+                  accumulate, dense, workspace, workspace_bytes,
+                  static_cast<hipStream_t>(stream));
 }
 
 tsm_status tsm_ata(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
                    int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
                    size_t workspace_bytes, tsm_stream_t stream) {
-  tsm_status st = check_atb(in_type, out_type, n, k, k, A, lda, A, lda, C, ldc);
-  if (st != TSM_SUCCESS) return st;
-  Plan p;
-  const tsm::SymEntry* e = nullptr;
-  st = make_sym_plan(in_type, n, k, &p, &e);
-  if (st != TSM_SUCCESS) return st;
-  const bool dense = k == e->kc && lda == k && ((uintptr_t)A % 16) == 0;
-  return run_plan(p, e->launch, e->kc, e->kc, out_type, n, k, k, A, lda, A, lda, C, ldc,
-                  accumulate, dense, workspace, workspace_bytes,
-                  static_cast<hipStream_t>(stream));
+  return tsm_ata_ex(in_type, out_type, n, k, A, lda, C, ldc, accumulate, workspace,
+                    workspace_bytes, stream, nullptr);
 }
 
 tsm_status tsm_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m, const void* A,

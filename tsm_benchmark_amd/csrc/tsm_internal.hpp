@@ -23,17 +23,6 @@ struct KernelEntry {
   LaunchFn launch;
 };
 
-// A^T A kernel of one (dtype, width class), with its launch shape.
-struct SymEntry {
-  const char* name;
-  int dtype;       // tsm_dtype
-  int kc;          // width class (k <= kc)
-  int rows;        // rows per tile
-  int wgs_per_cu;  // persistent workgroups per compute unit
-  int interleave;  // 1: tiles dealt round-robin to workgroups
-  LaunchFn launch;
-};
-
 struct TunedEntry {
   int dtype, kc, mc;
   int kernel;      // index into kernel_table()
@@ -73,5 +62,7 @@ extern "C" tsm_status tsm_internal_pooled_workspace(size_t bytes, void* stream, 
 // Defined by the generated tsm_table.hip.
 extern "C" const tsm::KernelEntry* tsm_gen_kernel_table(int* count);
 extern "C" const tsm::TunedEntry* tsm_gen_tuned_table(int* count);
-extern "C" const tsm::SymEntry* tsm_gen_sym_table(int* count);
+// A^T A kernels (kc == mc) and the default per (dtype, class).
+extern "C" const tsm::KernelEntry* tsm_gen_sym_table(int* count);
+extern "C" const tsm::TunedEntry* tsm_gen_sym_tuned_table(int* count);
 extern "C" const tsm::BatchedKernel* tsm_gen_batched_table(int* count);

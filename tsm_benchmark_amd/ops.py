@@ -68,7 +68,8 @@ def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = Non
 
 
 def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
-        out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+        out: torch.Tensor | None = None, accumulate: bool = False,
+        config: _lib.KernelConfig | None = None) -> torch.Tensor:
     """C = A^T A (or C += A^T A with ``accumulate`` and ``out``): the Gram matrix
     of a tall-skinny A, reading A once.  Same bits as ``atb(A, A)``."""
     _check_tall(A, "A")
@@ -83,9 +84,10 @@ def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
         raise ValueError("out must be a row-major k x k float64/float32 tensor")
     L = _lib.lib()
     with torch.cuda.device(A.device):
-        st = L.tsm_ata(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, A.data_ptr(), _ld(A),
-                       out.data_ptr(), out.stride(0) if k > 1 else k, 1 if accumulate else 0,
-                       None, 0, _stream())
+        st = L.tsm_ata_ex(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, A.data_ptr(),
+                          _ld(A), out.data_ptr(), out.stride(0) if k > 1 else k,
+                          1 if accumulate else 0, None, 0, _stream(),
+                          ct.byref(config) if config is not None else None)
     _lib.check(st, "tsm_ata")
     return out
 

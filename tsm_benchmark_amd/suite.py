@@ -13,7 +13,9 @@
   siblings    C = A^T A (FP64, A = 2 GiB, k = 8 and 32) against tsm_atb(A, A);
               Y = A X (FP64, k = m = 8, n = 2^25)
 
-Times are HIP-event medians on the launch stream with inputs resident in HBM.
+Times are per call over back-to-back calls after 3 warmup calls (HIP events on
+the launch stream), inputs resident in HBM: the steady-state rate, as in
+bench.py and the autotuner.
 """
 from __future__ import annotations
 
@@ -34,18 +36,16 @@ DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
 
 
 def timed(fn, reps: int = 10) -> float:
-    fn()
-    torch.cuda.synchronize()
-    ts = []
-    for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
+    for _ in range(3):
         fn()
-        e1.record()
-        e1.synchronize()
-        ts.append(e0.elapsed_time(e1) * 1e-3)
-    ts.sort()
-    return ts[len(ts) // 2]
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1e-3 / reps
 
 
 def rand(n, w, dt, sid, dist=0):
