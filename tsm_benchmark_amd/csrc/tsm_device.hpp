@@ -152,11 +152,66 @@ struct AtbArgs {
   int finish;             // the last workgroup writes C (1) or a finisher kernel does (0)
 };
 
+// ---- cross-lane exchange ------------------------------------------------------------
+// Value of lane (lane ^ D) of a wave64 without LDS: DPP within rows of 16
+// lanes for D = 1, 2, 4, 8 and v_permlane{16,32}_swap for D = 16, 32 (lane
+// mappings measured on gfx950).  `up` is (lane & D) != 0.
+template <int D>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t x, bool up) {
+  if constexpr (D == 1) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (D == 2) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (D == 4) {
+    const uint32_t from_up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x12C, 0xF, 0xF, false);  // row_ror:12: lane i + 4
+    const uint32_t from_dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x124, 0xF, 0xF, false);  // row_ror:4: lane i - 4
+    return up ? from_dn : from_up;
+  } else if constexpr (D == 8) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (D == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return up ? r[0] : r[1];
+  } else {
+    static_assert(D == 32, "xor distance within a wave64");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return up ? r[0] : r[1];
+  }
+}
+template <int D>
+__device__ __forceinline__ double lane_xor(double v, bool up) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = lane_xor<D>((uint32_t)u, up), hi = lane_xor<D>((uint32_t)(u >> 32), up);
+  return __longlong_as_double((long long)((hi << 32) | lo));
+}
+// For D = 16, 32: keep + (partner's copy of the kept half) for every lane, from
+// the lower-half value lo and the upper-half value hi of this lane: one swap
+// moves the partner's copy into place for both lane groups.
+template <int D>
+__device__ __forceinline__ double swap_add(double lo, double hi) {
+  const uint64_t a = (uint64_t)__double_as_longlong(lo), b = (uint64_t)__double_as_longlong(hi);
+  uint32_t d0, d1, s0, s1;
+  if constexpr (D == 16) {
+    const auto r0 = __builtin_amdgcn_permlane16_swap((uint32_t)a, (uint32_t)b, false, false);
+    const auto r1 = __builtin_amdgcn_permlane16_swap((uint32_t)(a >> 32), (uint32_t)(b >> 32), false, false);
+    d0 = r0[0]; s0 = r0[1]; d1 = r1[0]; s1 = r1[1];
+  } else {
+    static_assert(D == 32, "swap distance 16 or 32");
+    const auto r0 = __builtin_amdgcn_permlane32_swap((uint32_t)a, (uint32_t)b, false, false);
+    const auto r1 = __builtin_amdgcn_permlane32_swap((uint32_t)(a >> 32), (uint32_t)(b >> 32), false, false);
+    d0 = r0[0]; s0 = r0[1]; d1 = r1[0]; s1 = r1[1];
+  }
+  // lanes without bit D: dst = own lo (kept), src = partner's lo;
+  // lanes with bit D: dst = partner's hi, src = own hi (kept).
+  const double x = __longlong_as_double((long long)(((uint64_t)d1 << 32) | d0));
+  const double y = __longlong_as_double((long long)(((uint64_t)s1 << 32) | s0));
+  return x + y;
+}
+
 // ---- wave64 recursive-halving reduction -------------------------------------------
 // Lanes L and L ^ d (d = NTILES << l) share a register tile.  While more than
 // one value is left, each level keeps half of the values (the upper half on
 // the lane whose bit d is set) and adds the partner's copy of that half; the
-// remaining levels are a plain butterfly.
+// remaining levels are a plain butterfly.  All exchanges stay in registers.
 template <int NTILES, int V>
 struct WaveReduce {
   static constexpr int log2c(int x) { return x <= 1 ? 0 : 1 + log2c(x / 2); }
@@ -173,17 +228,28 @@ struct WaveReduce {
       const bool up = (lane & d) != 0;
 #pragma unroll
       for (int x = 0; x < half; ++x) {
-        const double send = up ? v[x] : v[x + half];
-        const double keep = up ? v[x + half] : v[x];
-        v[x] = keep + __shfl_xor(send, d, 64);
+        if constexpr (d >= 16) {
+          v[x] = swap_add<d>(v[x], v[x + half]);
+        } else {
+          const double send = up ? v[x] : v[x + half];
+          const double keep = up ? v[x + half] : v[x];
+          v[x] = keep + lane_xor<d>(send, up);
+        }
       }
       halve<L + 1>(v, lane);
     }
   }
+  template <int L>
+  __device__ static __forceinline__ void butterfly(double (&v)[V], int lane) {
+    if constexpr (L < QB) {
+      constexpr int d = NTILES << L;
+      v[0] += lane_xor<d>(v[0], (lane & d) != 0);
+      butterfly<L + 1>(v, lane);
+    }
+  }
   __device__ static __forceinline__ void run(double (&v)[V], int lane) {
     halve<0>(v, lane);
-#pragma unroll
-    for (int l = H; l < QB; ++l) v[0] += __shfl_xor(v[0], NTILES << l, 64);
+    butterfly<H>(v, lane);
   }
   // Index (in [0, V)) of the first value this lane holds after run().
   __device__ static __forceinline__ int base(int lane) {
@@ -272,10 +338,13 @@ __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC,
 // Final reduction of G partials of a KCMC-output class by one workgroup, for
 // KCMC a power of two with 2 <= KCMC <= 512.  The G x KCMC partials are read
 // as 16-byte chunks c = tid + 256 * j, so every thread always meets the same two outputs
-// (2 * tid mod KCMC and the next one) and all of its loads are independent
-// (one memory round trip instead of a dependent chain).  Each thread sums its
-// chunks in j order; the 512 / KCMC threads that share a pair of outputs are
-// then added in thread order.  `scratch` holds 2 * kThreads doubles of LDS.
+// (2 * tid mod KCMC and the next one) and all of its loads are independent.
+// They are issued in batches of U guarded loads, so a thread with up to U
+// chunks waits for one memory round trip (the partials were just written
+// through to memory, so each round trip is a full memory latency).  Each
+// thread sums its chunks in j order; the 512 / KCMC threads that share a pair
+// of outputs are then added in thread order.  `scratch` holds 2 * kThreads
+// doubles of LDS.
 template <int KCMC>
 __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, double* scratch) {
   static_assert(KCMC >= 2 && KCMC <= 512 && (KCMC & (KCMC - 1)) == 0, "2..512, power of two");
@@ -283,22 +352,21 @@ __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, do
   const v4u* P = reinterpret_cast<const v4u*>(a.partials);
   const int64_t chunks = (int64_t)G * KCMC / 2;  // 16-byte chunks
   double s0 = 0.0, s1 = 0.0;
-  constexpr int U = 8;
-  int64_t c = tid;
-  for (; c + (U - 1) * kThreads < chunks; c += U * kThreads) {
+  constexpr int U = 16;  // 64 VGPRs: keeps two waves per SIMD for the main loops
+  for (int64_t c = tid; c < chunks; c += U * kThreads) {
     v4u u[U];
 #pragma unroll
-    for (int x = 0; x < U; ++x) u[x] = P[c + x * kThreads];
+    for (int x = 0; x < U; ++x) {
+      const int64_t i = c + (int64_t)x * kThreads;
+      u[x] = i < chunks ? P[i] : v4u{0, 0, 0, 0};
+    }
 #pragma unroll
     for (int x = 0; x < U; ++x) {
-      s0 += __hiloint2double((int)u[x].y, (int)u[x].x);
-      s1 += __hiloint2double((int)u[x].w, (int)u[x].z);
+      if (c + (int64_t)x * kThreads < chunks) {
+        s0 += __hiloint2double((int)u[x].y, (int)u[x].x);
+        s1 += __hiloint2double((int)u[x].w, (int)u[x].z);
+      }
     }
-  }
-  for (; c < chunks; c += kThreads) {
-    const v4u u = P[c];
-    s0 += __hiloint2double((int)u.y, (int)u.x);
-    s1 += __hiloint2double((int)u.w, (int)u.z);
   }
   scratch[2 * tid] = s0;
   scratch[2 * tid + 1] = s1;
