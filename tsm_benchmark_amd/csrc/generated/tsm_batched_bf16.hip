@@ -38,14 +38,14 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_bf16(tsm::Batc
       case 25: tsm::batched_item_call<tsm::bf16, 16, 2, 8, 2, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x2_t8x2_r512_p2
       case 26: tsm::batched_item_call<tsm::bf16, 16, 4, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x4_t8x4_r512_p1
       case 27: tsm::batched_item_call<tsm::bf16, 16, 8, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x8_t8x4_r512_p1
-      case 28: tsm::batched_item_call<tsm::bf16, 16, 16, 0, 0, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x16_mfma_r256_p2_nt
+      case 28: tsm::batched_item_call<tsm::bf16, 16, 16, 0, 0, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x16_mfma_r512_p2
       case 29: tsm::batched_item_call<tsm::bf16, 16, 32, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x32_mfma_r256_p2
       case 30: tsm::batched_item_call<tsm::bf16, 32, 1, 8, 1, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x1_t8x1_r256_p2
       case 31: tsm::batched_item_call<tsm::bf16, 32, 2, 8, 2, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x2_t8x2_r256_p2
       case 32: tsm::batched_item_call<tsm::bf16, 32, 4, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x4_t8x4_r256_p1
       case 33: tsm::batched_item_call<tsm::bf16, 32, 8, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x8_t8x4_r256_p1
       case 34: tsm::batched_item_call<tsm::bf16, 32, 16, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x16_mfma_r256_p2
-      case 35: tsm::batched_item_call<tsm::bf16, 32, 32, 0, 0, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x32_mfma_r256_p2_nt
+      case 35: tsm::batched_item_call<tsm::bf16, 32, 32, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x32_mfma_r256_p2
       default: break;
     }
   }
@@ -80,7 +80,7 @@ static const tsm::BatchClass kClasses[] = {
   {16, 2, 512},
   {16, 4, 512},
   {16, 8, 512},
-  {16, 16, 256},
+  {16, 16, 512},
   {16, 32, 256},
   {32, 1, 256},
   {32, 2, 256},

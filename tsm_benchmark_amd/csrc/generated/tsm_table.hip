@@ -530,16 +530,12 @@ hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -573,6 +569,8 @@ hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1153,16 +1151,12 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt},
   {"tsm_ata_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt},
   {"tsm_ata_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt},
-  {"tsm_ata_bf16_16x16_mfma_r128_p1_nt", 2, 16, 16, 0, 0, 128, 1, 1, 8192, launch_tsm_ata_bf16_16x16_mfma_r128_p1_nt},
-  {"tsm_ata_bf16_16x16_mfma_r128_p2_nt", 2, 16, 16, 0, 0, 128, 2, 1, 8192, launch_tsm_ata_bf16_16x16_mfma_r128_p2_nt},
-  {"tsm_ata_bf16_16x16_mfma_r256_p1_nt", 2, 16, 16, 0, 0, 256, 1, 1, 16384, launch_tsm_ata_bf16_16x16_mfma_r256_p1_nt},
-  {"tsm_ata_bf16_16x16_mfma_r256_p2_nt", 2, 16, 16, 0, 0, 256, 2, 1, 16384, launch_tsm_ata_bf16_16x16_mfma_r256_p2_nt},
   {"tsm_ata_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt},
   {"tsm_ata_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt},
   {"tsm_ata_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt},
   {"tsm_ata_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt},
-  {"tsm_ata_bf16_32x32_mfma_r128_p1_nt", 2, 32, 32, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_bf16_32x32_mfma_r128_p1_nt},
-  {"tsm_ata_bf16_32x32_mfma_r128_p2_nt", 2, 32, 32, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_bf16_32x32_mfma_r128_p2_nt},
+  {"tsm_ata_bf16_16x16_mfma_r2048_p1_nt", 2, 16, 16, 0, 0, 2048, 1, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt},
+  {"tsm_ata_bf16_16x16_mfma_r2048_p2_nt", 2, 16, 16, 0, 0, 2048, 2, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt},
   {"tsm_ata_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt},
   {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt},
   {"tsm_ata_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt},
@@ -1196,6 +1190,8 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt},
   {"tsm_ata_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p1_nt},
   {"tsm_ata_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p2_nt},
+  {"tsm_ata_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p1_nt},
+  {"tsm_ata_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p2_nt},
   {"tsm_ata_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p1_nt},
   {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt},
   {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt},
@@ -1251,23 +1247,23 @@ static const tsm::TunedEntry kSymTuned[] = {
   {2, 2, 2, 1, 2, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
   {2, 4, 4, 2, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
   {2, 8, 8, 5, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 16, 16, 16, 1, 1},  // tsm_ata_bf16_16x16_mfma_r1024_p2_nt
-  {2, 32, 32, 22, 1, 1},  // tsm_ata_bf16_32x32_mfma_r512_p2_nt
-  {2, 64, 64, 29, 1, 0},  // tsm_ata_bf16_64x64_mfma_r512_p1_nt
-  {1, 1, 1, 31, 2, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
-  {1, 2, 2, 32, 2, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
-  {1, 4, 4, 33, 2, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
-  {1, 8, 8, 34, 2, 0},  // tsm_ata_f32_8x8_t8x4_r512_p1_nt
-  {1, 16, 16, 45, 1, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
-  {1, 32, 32, 51, 1, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
-  {1, 64, 64, 54, 2, 0},  // tsm_ata_f32_64x64_mfma_r128_p1_nt
-  {0, 1, 1, 59, 1, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p2_nt
-  {0, 2, 2, 62, 2, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p1_nt
-  {0, 4, 4, 71, 1, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 75, 1, 1},  // tsm_ata_f64_8x8_t8x8_r256_p2_nt
-  {0, 16, 16, 81, 1, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 91, 1, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
-  {0, 64, 64, 97, 2, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
+  {2, 16, 16, 10, 1, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
+  {2, 32, 32, 16, 1, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
+  {2, 64, 64, 24, 1, 0},  // tsm_ata_bf16_64x64_mfma_r256_p2_nt
+  {1, 1, 1, 27, 2, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
+  {1, 2, 2, 28, 2, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
+  {1, 4, 4, 29, 2, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
+  {1, 8, 8, 30, 2, 0},  // tsm_ata_f32_8x8_t8x4_r512_p1_nt
+  {1, 16, 16, 41, 1, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
+  {1, 32, 32, 45, 1, 1},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
+  {1, 64, 64, 53, 2, 0},  // tsm_ata_f32_64x64_mfma_r128_p2_nt
+  {0, 1, 1, 57, 1, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p2_nt
+  {0, 2, 2, 60, 2, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p1_nt
+  {0, 4, 4, 69, 1, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 73, 1, 1},  // tsm_ata_f64_8x8_t8x8_r256_p2_nt
+  {0, 16, 16, 79, 1, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
+  {0, 32, 32, 89, 1, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
+  {0, 64, 64, 95, 2, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
 };
 
 static const tsm::TunedEntry kTuned[] = {
@@ -1303,7 +1299,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 16, 2, 50, 2, 0},  // tsm_atb_bf16_16x2_t8x2_r512_p2
   {2, 16, 4, 51, 2, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2
   {2, 16, 8, 52, 2, 0},  // tsm_atb_bf16_16x8_t8x4_r512_p2
-  {2, 16, 16, 56, 1, 1},  // tsm_atb_bf16_16x16_mfma_r256_p2_nt
+  {2, 16, 16, 62, 1, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
   {2, 16, 32, 87, 2, 0},  // tsm_atb_bf16_16x32_mfma_r256_p2
   {2, 16, 64, 88, 2, 0},  // tsm_atb_bf16_16x64_mfma_r128_p2
   {2, 32, 1, 89, 2, 0},  // tsm_atb_bf16_32x1_t8x1_r256_p2
@@ -1311,7 +1307,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 32, 4, 91, 2, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
   {2, 32, 8, 92, 2, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
   {2, 32, 16, 93, 2, 0},  // tsm_atb_bf16_32x16_mfma_r256_p2
-  {2, 32, 32, 101, 1, 1},  // tsm_atb_bf16_32x32_mfma_r256_p2_nt
+  {2, 32, 32, 103, 1, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
   {2, 32, 64, 128, 2, 0},  // tsm_atb_bf16_32x64_mfma_r128_p2
   {2, 64, 1, 129, 2, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
   {2, 64, 2, 130, 2, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
@@ -1319,7 +1315,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 64, 8, 132, 2, 0},  // tsm_atb_bf16_64x8_t8x4_r128_p2
   {2, 64, 16, 133, 2, 0},  // tsm_atb_bf16_64x16_mfma_r128_p2
   {2, 64, 32, 134, 2, 0},  // tsm_atb_bf16_64x32_mfma_r128_p2
-  {2, 64, 64, 140, 1, 1},  // tsm_atb_bf16_64x64_mfma_r256_p1_nt
+  {2, 64, 64, 142, 1, 1},  // tsm_atb_bf16_64x64_mfma_r256_p2_nt
   {1, 1, 1, 155, 2, 0},  // tsm_atb_f32_1x1_t1x1_r4096_p2
   {1, 1, 2, 156, 2, 0},  // tsm_atb_f32_1x2_t1x2_r2048_p2
   {1, 1, 4, 157, 2, 0},  // tsm_atb_f32_1x4_t1x4_r1024_p2
@@ -1368,7 +1364,7 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 64, 8, 285, 2, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
   {1, 64, 16, 286, 2, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
   {1, 64, 32, 287, 2, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
-  {1, 64, 64, 297, 1, 1},  // tsm_atb_f32_64x64_mfma_r128_p1_nt
+  {1, 64, 64, 295, 1, 0},  // tsm_atb_f32_64x64_mfma_r64_p2_nt
   {0, 1, 1, 321, 1, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
   {0, 1, 2, 324, 2, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
   {0, 1, 4, 325, 2, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2

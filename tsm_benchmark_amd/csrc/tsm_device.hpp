@@ -404,7 +404,7 @@ __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
 // register v holds row 4*(l/16) + v (FP32, BF16) or row l/16 + 4*v (FP64),
 // column l % 16.  Wave w takes the steps w, w + 4, ... of every tile, so each
 // wave ends with a full KC x MC partial; FP32 accumulators are folded into
-// (hi, lo) pairs after every tile (32 rows per wave or fewer).
+// (hi, lo) pairs every 64 (FP32) or 128 (BF16) rows per wave.
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -439,15 +439,24 @@ __device__ __forceinline__ float mfma_operand(const float* X, int ld, int row0, 
                                               int lane) {
   return X[(row0 + (lane >> 4)) * ld + blk * 16 + (lane & 15)];
 }
+// BF16: rows row0 + 8 * (l / 16) + t, t = 0..7, of column 16 * blk + l % 16,
+// with two LDS transpose reads (ds_read_b64_tr_b16).  Within each group of
+// 16 lanes that instruction hands output lane 4q + e the element e of the
+// 8-byte reads of lanes q, q + 4, q + 8, q + 12 (measured on gfx950), so lane
+// 16g + 4j + q reads 4 columns (16 * blk + 4q ...) of row 8g + j (then of
+// row 8g + 4 + j) and lane 16g + i receives column 16 * blk + i of those rows.
+typedef short v4s_ __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s_ lds_v4s_;
 __device__ __forceinline__ v8bf mfma_operand(const bf16* X, int ld, int row0, int blk,
                                              int lane) {
-  // rows row0 + 8 * (l / 16) + t, t = 0..7, of column 16 * blk + l % 16
-  const uint16_t* p = reinterpret_cast<const uint16_t*>(X) + (row0 + 8 * (lane >> 4)) * ld +
-                      blk * 16 + (lane & 15);
-  uint32_t w[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-    w[t] = (uint32_t)p[(2 * t) * ld] | ((uint32_t)p[(2 * t + 1) * ld] << 16);
+  const int g = lane >> 4, j = (lane >> 2) & 3, q = lane & 3;
+  const bf16* p0 = X + (row0 + 8 * g + j) * ld + blk * 16 + 4 * q;
+  const v4s_ a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_*)p0);
+  const v4s_ b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s_*)(p0 + 4 * ld));
+  const uint32_t w[4] = {(uint32_t)(uint16_t)a[0] | ((uint32_t)(uint16_t)a[1] << 16),
+                         (uint32_t)(uint16_t)a[2] | ((uint32_t)(uint16_t)a[3] << 16),
+                         (uint32_t)(uint16_t)b[0] | ((uint32_t)(uint16_t)b[1] << 16),
+                         (uint32_t)(uint16_t)b[2] | ((uint32_t)(uint16_t)b[3] << 16)};
   return __builtin_bit_cast(v8bf, w);
 }
 // -x for eight packed BF16 values (flips the sign bits).
@@ -728,20 +737,32 @@ struct RegTileAcc {
 };
 
 // The MFMA consumer with the same interface (TM = TN = 0 selects it).
+// FP32/BF16 accumulators are folded into (hi, lo) every FOLD tiles: about 64
+// rows per wave for FP32 (rounded products) and 128 for BF16 (exact products,
+// summed in FP32 by the matrix instruction).
 template <class S, int KC, int MC, int T, int SYM = 0>
 struct MfmaConsumer {
   MfmaAcc<S, KC, MC, T, SYM> m;
-  int wave, lane;
+  static constexpr int kWaveRows = T / 4;  // rows of a tile each wave consumes
+  static constexpr int kFoldRows = MfmaTraits<S>::KSTEP == 32 ? 128 : 64;
+  static constexpr int FOLD = kWaveRows >= kFoldRows ? 1 : kFoldRows / kWaveRows;
+  int wave, lane, fold_count;
   __device__ __forceinline__ void zero(int tid) {
     wave = tid >> 6;
     lane = tid & 63;
+    fold_count = 0;
     m.zero();
   }
   __device__ __forceinline__ void consume(const unsigned char* a_stage,
                                          const unsigned char* b_stage) {
     m.consume(a_stage, b_stage, wave, lane);
   }
-  __device__ __forceinline__ void after_tile() { m.fold(); }
+  __device__ __forceinline__ void after_tile() {
+    if (++fold_count == FOLD) {
+      m.fold();
+      fold_count = 0;
+    }
+  }
   __device__ __forceinline__ void reduce(double* red, int) {
     m.fold();
     __syncthreads();  // every wave is done with the LDS tiles
