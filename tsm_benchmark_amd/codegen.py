@@ -392,10 +392,6 @@ def batched_classes(defaults: dict, dtype: str) -> list[KernelConfig]:
             c = defaults[(dtype, kc, mc)][0]
             if c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32):
                 c = default_config(dtype, kc, mc)
-            if c.tm * c.tn >= 32 and c.pf == 2:
-                # one register set of loads: the switch over every class shares
-                # one register budget, sized by its widest class
-                c = dataclasses.replace(c, pf=1)
             out.append(c)
     return out
 
@@ -423,7 +419,13 @@ def emit_batched_file(dtype: str, classes: list[KernelConfig]) -> str:
             f"  hipLaunchKernelGGL({name}, dim3(grid), dim3(256), 0, s, a);",
             "  return hipGetLastError();", "}", "",
             f"const tsm::BatchClass* classes_{name}(int* count) {{",
-            f"  *count = {len(classes)};", "  return kClasses;", "}", ""]
+            f"  *count = {len(classes)};", "  return kClasses;", "}", "",
+            f"int occupancy_{name}() {{",
+            "  int nb = 0;",
+            f"  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>({name}), 256, 0) !=",
+            "          hipSuccess || nb < 1)",
+            "    nb = 1;",
+            "  return nb;", "}", ""]
     return "\n".join(out)
 
 
@@ -456,6 +458,7 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict, tuning: dict) -> str:
         nm = f"tsm_atb_batched_{dt}"
         out.append(f"hipError_t launch_{nm}(const tsm::BatchArgs&, unsigned, hipStream_t);")
         out.append(f"const tsm::BatchClass* classes_{nm}(int* count);")
+        out.append(f"int occupancy_{nm}();")
     out.append("static const tsm::BatchedKernel* batched_all() {")
     out.append("  static tsm::BatchedKernel all[3];")
     out.append("  static bool init = [] {")
@@ -465,6 +468,7 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict, tuning: dict) -> str:
         out.append(f"    all[{i}].name = \"{nm}\"; all[{i}].dtype = {DTYPE_ID[dt]};")
         out.append(f"    all[{i}].classes = classes_{nm}(&all[{i}].n_classes);")
         out.append(f"    all[{i}].lds_bytes = {lds}; all[{i}].launch = launch_{nm};")
+        out.append(f"    all[{i}].occupancy = occupancy_{nm};")
     out.append("    return true;")
     out.append("  }();")
     out.append("  (void)init;")

@@ -30,20 +30,20 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_bf16(tsm::Batc
       case 17: tsm::batched_item_call<tsm::bf16, 4, 32, 4, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_4x32_t4x4_r256_p2
       case 18: tsm::batched_item_call<tsm::bf16, 8, 1, 8, 1, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x1_t8x1_r1024_p2
       case 19: tsm::batched_item_call<tsm::bf16, 8, 2, 8, 2, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x2_t8x2_r1024_p2
-      case 20: tsm::batched_item_call<tsm::bf16, 8, 4, 8, 4, 1024, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x4_t8x4_r1024_p1
+      case 20: tsm::batched_item_call<tsm::bf16, 8, 4, 8, 4, 1024, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x4_t8x4_r1024_p2
       case 21: tsm::batched_item_call<tsm::bf16, 8, 8, 8, 4, 1024, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
-      case 22: tsm::batched_item_call<tsm::bf16, 8, 16, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x16_t8x4_r512_p1
-      case 23: tsm::batched_item_call<tsm::bf16, 8, 32, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x32_t8x4_r256_p1
+      case 22: tsm::batched_item_call<tsm::bf16, 8, 16, 8, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x16_t8x4_r512_p2
+      case 23: tsm::batched_item_call<tsm::bf16, 8, 32, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_8x32_t8x4_r256_p2
       case 24: tsm::batched_item_call<tsm::bf16, 16, 1, 8, 1, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x1_t8x1_r512_p2
       case 25: tsm::batched_item_call<tsm::bf16, 16, 2, 8, 2, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x2_t8x2_r512_p2
-      case 26: tsm::batched_item_call<tsm::bf16, 16, 4, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x4_t8x4_r512_p1
-      case 27: tsm::batched_item_call<tsm::bf16, 16, 8, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x8_t8x4_r512_p1
+      case 26: tsm::batched_item_call<tsm::bf16, 16, 4, 8, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x4_t8x4_r512_p2
+      case 27: tsm::batched_item_call<tsm::bf16, 16, 8, 8, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x8_t8x4_r512_p2
       case 28: tsm::batched_item_call<tsm::bf16, 16, 16, 0, 0, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x16_mfma_r512_p2
       case 29: tsm::batched_item_call<tsm::bf16, 16, 32, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_16x32_mfma_r256_p2
       case 30: tsm::batched_item_call<tsm::bf16, 32, 1, 8, 1, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x1_t8x1_r256_p2
       case 31: tsm::batched_item_call<tsm::bf16, 32, 2, 8, 2, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x2_t8x2_r256_p2
-      case 32: tsm::batched_item_call<tsm::bf16, 32, 4, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x4_t8x4_r256_p1
-      case 33: tsm::batched_item_call<tsm::bf16, 32, 8, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x8_t8x4_r256_p1
+      case 32: tsm::batched_item_call<tsm::bf16, 32, 4, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x4_t8x4_r256_p2
+      case 33: tsm::batched_item_call<tsm::bf16, 32, 8, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x8_t8x4_r256_p2
       case 34: tsm::batched_item_call<tsm::bf16, 32, 16, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x16_mfma_r256_p2
       case 35: tsm::batched_item_call<tsm::bf16, 32, 32, 0, 0, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_bf16_32x32_mfma_r256_p2
       default: break;
@@ -98,4 +98,12 @@ hipError_t launch_tsm_atb_batched_bf16(const tsm::BatchArgs& a, unsigned grid, h
 const tsm::BatchClass* classes_tsm_atb_batched_bf16(int* count) {
   *count = 36;
   return kClasses;
+}
+
+int occupancy_tsm_atb_batched_bf16() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(tsm_atb_batched_bf16), 256, 0) !=
+          hipSuccess || nb < 1)
+    nb = 1;
+  return nb;
 }

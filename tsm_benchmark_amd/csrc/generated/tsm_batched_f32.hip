@@ -30,21 +30,21 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f32(tsm::Batch
       case 17: tsm::batched_item_call<float, 4, 32, 4, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x32_t4x4_r128_p2
       case 18: tsm::batched_item_call<float, 8, 1, 8, 1, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x1_t8x1_r512_p2
       case 19: tsm::batched_item_call<float, 8, 2, 8, 2, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x2_t8x2_r512_p2
-      case 20: tsm::batched_item_call<float, 8, 4, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x4_t8x4_r512_p1
-      case 21: tsm::batched_item_call<float, 8, 8, 8, 4, 512, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x8_t8x4_r512_p1
-      case 22: tsm::batched_item_call<float, 8, 16, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x16_t8x4_r256_p1
-      case 23: tsm::batched_item_call<float, 8, 32, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x32_t8x4_r128_p1
+      case 20: tsm::batched_item_call<float, 8, 4, 8, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x4_t8x4_r512_p2
+      case 21: tsm::batched_item_call<float, 8, 8, 8, 4, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x8_t8x4_r512_p2
+      case 22: tsm::batched_item_call<float, 8, 16, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x16_t8x4_r256_p2
+      case 23: tsm::batched_item_call<float, 8, 32, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x32_t8x4_r128_p2
       case 24: tsm::batched_item_call<float, 16, 1, 8, 1, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x1_t8x1_r256_p2
       case 25: tsm::batched_item_call<float, 16, 2, 8, 2, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x2_t8x2_r256_p2
-      case 26: tsm::batched_item_call<float, 16, 4, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x4_t8x4_r256_p1
-      case 27: tsm::batched_item_call<float, 16, 8, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x8_t8x4_r256_p1
+      case 26: tsm::batched_item_call<float, 16, 4, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x4_t8x4_r256_p2
+      case 27: tsm::batched_item_call<float, 16, 8, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x8_t8x4_r256_p2
       case 28: tsm::batched_item_call<float, 16, 16, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-      case 29: tsm::batched_item_call<float, 16, 32, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x32_t8x4_r128_p1
+      case 29: tsm::batched_item_call<float, 16, 32, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x32_t8x4_r128_p2
       case 30: tsm::batched_item_call<float, 32, 1, 8, 1, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x1_t8x1_r128_p2
       case 31: tsm::batched_item_call<float, 32, 2, 8, 2, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x2_t8x2_r128_p2
-      case 32: tsm::batched_item_call<float, 32, 4, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x4_t8x4_r128_p1
-      case 33: tsm::batched_item_call<float, 32, 8, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x8_t8x4_r128_p1
-      case 34: tsm::batched_item_call<float, 32, 16, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x16_t8x4_r128_p1
+      case 32: tsm::batched_item_call<float, 32, 4, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x4_t8x4_r128_p2
+      case 33: tsm::batched_item_call<float, 32, 8, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x8_t8x4_r128_p2
+      case 34: tsm::batched_item_call<float, 32, 16, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x16_t8x4_r128_p2
       case 35: tsm::batched_item_call<float, 32, 32, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x32_mfma_r128_p2_nt
       default: break;
     }
@@ -98,4 +98,12 @@ hipError_t launch_tsm_atb_batched_f32(const tsm::BatchArgs& a, unsigned grid, hi
 const tsm::BatchClass* classes_tsm_atb_batched_f32(int* count) {
   *count = 36;
   return kClasses;
+}
+
+int occupancy_tsm_atb_batched_f32() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(tsm_atb_batched_f32), 256, 0) !=
+          hipSuccess || nb < 1)
+    nb = 1;
+  return nb;
 }

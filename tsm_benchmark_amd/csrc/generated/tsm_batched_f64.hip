@@ -27,35 +27,35 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 14: tsm::batched_item_call<double, 4, 1, 4, 1, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x1_t4x1_r512_p2
       case 15: tsm::batched_item_call<double, 4, 2, 4, 2, 512, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x2_t4x2_r512_p2
       case 16: tsm::batched_item_call<double, 4, 4, 4, 4, 256, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-      case 17: tsm::batched_item_call<double, 4, 8, 4, 8, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x8_t4x8_r256_p1
-      case 18: tsm::batched_item_call<double, 4, 16, 4, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x16_t4x8_r128_p1
+      case 17: tsm::batched_item_call<double, 4, 8, 4, 8, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x8_t4x8_r256_p2
+      case 18: tsm::batched_item_call<double, 4, 16, 4, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x16_t4x8_r128_p2
       case 19: tsm::batched_item_call<double, 4, 32, 4, 8, 64, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-      case 20: tsm::batched_item_call<double, 4, 64, 4, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x64_t4x8_r32_p1
+      case 20: tsm::batched_item_call<double, 4, 64, 4, 8, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_4x64_t4x8_r32_p2
       case 21: tsm::batched_item_call<double, 8, 1, 8, 1, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x1_t8x1_r256_p2
       case 22: tsm::batched_item_call<double, 8, 2, 8, 2, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x2_t8x2_r256_p2
-      case 23: tsm::batched_item_call<double, 8, 4, 8, 4, 256, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x4_t8x4_r256_p1
-      case 24: tsm::batched_item_call<double, 8, 8, 8, 8, 256, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x8_t8x8_r256_p1_nt
-      case 25: tsm::batched_item_call<double, 8, 16, 8, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x16_t8x8_r128_p1
-      case 26: tsm::batched_item_call<double, 8, 32, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x32_t8x8_r64_p1
-      case 27: tsm::batched_item_call<double, 8, 64, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x64_t8x8_r32_p1
+      case 23: tsm::batched_item_call<double, 8, 4, 8, 4, 256, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x4_t8x4_r256_p2
+      case 24: tsm::batched_item_call<double, 8, 8, 8, 8, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
+      case 25: tsm::batched_item_call<double, 8, 16, 8, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x16_t8x8_r128_p2
+      case 26: tsm::batched_item_call<double, 8, 32, 8, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x32_t8x8_r64_p2
+      case 27: tsm::batched_item_call<double, 8, 64, 8, 8, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x64_t8x8_r32_p2
       case 28: tsm::batched_item_call<double, 16, 1, 8, 1, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x1_t8x1_r128_p2
       case 29: tsm::batched_item_call<double, 16, 2, 8, 2, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x2_t8x2_r128_p2
-      case 30: tsm::batched_item_call<double, 16, 4, 8, 4, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x4_t8x4_r128_p1
-      case 31: tsm::batched_item_call<double, 16, 8, 8, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x8_t8x8_r128_p1
-      case 32: tsm::batched_item_call<double, 16, 16, 8, 8, 128, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x16_t8x8_r128_p1
-      case 33: tsm::batched_item_call<double, 16, 32, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x32_t8x8_r64_p1
+      case 30: tsm::batched_item_call<double, 16, 4, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x4_t8x4_r128_p2
+      case 31: tsm::batched_item_call<double, 16, 8, 8, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x8_t8x8_r128_p2
+      case 32: tsm::batched_item_call<double, 16, 16, 8, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x16_t8x8_r128_p2
+      case 33: tsm::batched_item_call<double, 16, 32, 8, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x32_t8x8_r64_p2
       case 34: tsm::batched_item_call<double, 16, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x64_mfma_r32_p2
       case 35: tsm::batched_item_call<double, 32, 1, 8, 1, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x1_t8x1_r64_p2
       case 36: tsm::batched_item_call<double, 32, 2, 8, 2, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x2_t8x2_r64_p2
       case 37: tsm::batched_item_call<double, 32, 4, 8, 4, 64, 1, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
-      case 38: tsm::batched_item_call<double, 32, 8, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x8_t8x8_r64_p1
-      case 39: tsm::batched_item_call<double, 32, 16, 8, 8, 64, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x16_t8x8_r64_p1
+      case 38: tsm::batched_item_call<double, 32, 8, 8, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x8_t8x8_r64_p2
+      case 39: tsm::batched_item_call<double, 32, 16, 8, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x16_t8x8_r64_p2
       case 40: tsm::batched_item_call<double, 32, 32, 0, 0, 32, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x32_mfma_r32_p2_nt
       case 41: tsm::batched_item_call<double, 32, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x64_mfma_r32_p2
       case 42: tsm::batched_item_call<double, 64, 1, 8, 1, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x1_t8x1_r32_p2
       case 43: tsm::batched_item_call<double, 64, 2, 8, 2, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x2_t8x2_r32_p2
-      case 44: tsm::batched_item_call<double, 64, 4, 8, 4, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x4_t8x4_r32_p1
-      case 45: tsm::batched_item_call<double, 64, 8, 8, 8, 32, 1, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x8_t8x8_r32_p1
+      case 44: tsm::batched_item_call<double, 64, 4, 8, 4, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x4_t8x4_r32_p2
+      case 45: tsm::batched_item_call<double, 64, 8, 8, 8, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x8_t8x8_r32_p2
       case 46: tsm::batched_item_call<double, 64, 16, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x16_mfma_r32_p2
       case 47: tsm::batched_item_call<double, 64, 32, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x32_mfma_r32_p2
       case 48: tsm::batched_item_call<double, 64, 64, 0, 0, 32, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_64x64_mfma_r32_p2_nt
@@ -124,4 +124,12 @@ hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs& a, unsigned grid, hi
 const tsm::BatchClass* classes_tsm_atb_batched_f64(int* count) {
   *count = 49;
   return kClasses;
+}
+
+int occupancy_tsm_atb_batched_f64() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(tsm_atb_batched_f64), 256, 0) !=
+          hipSuccess || nb < 1)
+    nb = 1;
+  return nb;
 }

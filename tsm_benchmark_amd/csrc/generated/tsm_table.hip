@@ -1418,22 +1418,28 @@ static const tsm::TunedEntry kTuned[] = {
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
 const tsm::BatchClass* classes_tsm_atb_batched_f64(int* count);
+int occupancy_tsm_atb_batched_f64();
 hipError_t launch_tsm_atb_batched_f32(const tsm::BatchArgs&, unsigned, hipStream_t);
 const tsm::BatchClass* classes_tsm_atb_batched_f32(int* count);
+int occupancy_tsm_atb_batched_f32();
 hipError_t launch_tsm_atb_batched_bf16(const tsm::BatchArgs&, unsigned, hipStream_t);
 const tsm::BatchClass* classes_tsm_atb_batched_bf16(int* count);
+int occupancy_tsm_atb_batched_bf16();
 static const tsm::BatchedKernel* batched_all() {
   static tsm::BatchedKernel all[3];
   static bool init = [] {
     all[0].name = "tsm_atb_batched_f64"; all[0].dtype = 0;
     all[0].classes = classes_tsm_atb_batched_f64(&all[0].n_classes);
     all[0].lds_bytes = 65536; all[0].launch = launch_tsm_atb_batched_f64;
+    all[0].occupancy = occupancy_tsm_atb_batched_f64;
     all[1].name = "tsm_atb_batched_f32"; all[1].dtype = 1;
     all[1].classes = classes_tsm_atb_batched_f32(&all[1].n_classes);
     all[1].lds_bytes = 65536; all[1].launch = launch_tsm_atb_batched_f32;
+    all[1].occupancy = occupancy_tsm_atb_batched_f32;
     all[2].name = "tsm_atb_batched_bf16"; all[2].dtype = 2;
     all[2].classes = classes_tsm_atb_batched_bf16(&all[2].n_classes);
     all[2].lds_bytes = 65536; all[2].launch = launch_tsm_atb_batched_bf16;
+    all[2].occupancy = occupancy_tsm_atb_batched_bf16;
     return true;
   }();
   (void)init;

@@ -17,7 +17,6 @@ namespace {
 
 constexpr size_t kAlign = 256;
 constexpr int64_t kItemBytes = 2 << 20;  // target bytes of A + B per work item
-constexpr int kWgsPerCu = 2;
 
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 int elem_bytes(tsm_dtype t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
@@ -133,11 +132,17 @@ tsm::BatchArgs make_args(const Layout& L, void* base, int accumulate, tsm_dtype 
   return a;
 }
 
-tsm_status grid_for(int64_t n_items, unsigned* grid) {
+// Persistent grid: as many workgroups as are resident at once (the batched
+// kernels carry every class's registers, so usually one per CU), at most one
+// per item.
+tsm_status grid_for(const tsm::BatchedKernel* bk, int64_t n_items, unsigned* grid) {
   int cus = 0;
   tsm_status st = tsm_internal_device_cus(&cus);
   if (st != TSM_SUCCESS) return st;
-  int64_t g = std::min<int64_t>(n_items, (int64_t)cus * kWgsPerCu);
+  static int resident[3];  // per dtype, filled on first use
+  int& per_cu = resident[bk->dtype];
+  if (per_cu == 0) per_cu = bk->occupancy();
+  int64_t g = std::min<int64_t>(n_items, (int64_t)cus * per_cu);
   *grid = (unsigned)std::max<int64_t>(g, 1);
   return TSM_SUCCESS;
 }
@@ -213,12 +218,12 @@ tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
     st = tsm_internal_pooled_workspace(L.total, stream, &workspace);
     if (st != TSM_SUCCESS) return st;
   }
+  const tsm::BatchedKernel* bk = batched_kernel(in_type);
   unsigned grid = 1;
-  st = grid_for((int64_t)L.items.size(), &grid);
+  st = grid_for(bk, (int64_t)L.items.size(), &grid);
   if (st != TSM_SUCCESS) return st;
   st = upload(L, workspace, s);
   if (st != TSM_SUCCESS) return st;
-  const tsm::BatchedKernel* bk = batched_kernel(in_type);
   return (tsm_status)tsm_internal_hip_status(
       bk->launch(make_args(L, workspace, accumulate, out_type), grid, s));
 }
@@ -232,7 +237,7 @@ tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int co
   tsm_status st = build_layout(in_type, out_type, count, problems, &L);
   if (st != TSM_SUCCESS) return st;
   unsigned grid = 1;
-  st = grid_for((int64_t)L.items.size(), &grid);
+  st = grid_for(batched_kernel(in_type), (int64_t)L.items.size(), &grid);
   if (st != TSM_SUCCESS) return st;
   void* buf = nullptr;
   hipError_t e = hipMalloc(&buf, std::max<size_t>(L.total, kAlign));

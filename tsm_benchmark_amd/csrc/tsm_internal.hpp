@@ -43,6 +43,7 @@ struct BatchedKernel {
   int n_classes;
   int lds_bytes;
   BatchLaunchFn launch;
+  int (*occupancy)();  // workgroups of the kernel resident per CU
 };
 
 }  // namespace tsm
