@@ -105,9 +105,9 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
 
 def tune_sym(dtype: str, kc: int, gib: float, reps: int, log) -> dict:
     """A^T A of one square class: every compiled variant x grid x tile order,
-    on A of `gib` / 2 GiB (the bytes A^T B reads from A alone)."""
+    on A of `gib` GiB."""
     es = codegen.ELEM_BYTES[dtype]
-    n = int(gib / 2 * (1 << 30) / (es * kc))
+    n = int(gib * (1 << 30) / (es * kc))
     A = ops.random_matrix(n, kc, DT[dtype], seed=SEED, stream_id=1)
     C = torch.empty((kc, kc), dtype=torch.float64, device="cuda")
     ref = ops.ata(A).double()

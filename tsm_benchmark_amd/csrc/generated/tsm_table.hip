@@ -1246,21 +1246,21 @@ static const tsm::TunedEntry kSymTuned[] = {
   {2, 1, 1, 0, 2, 0},  // tsm_ata_bf16_1x1_t1x1_r4096_p2
   {2, 2, 2, 1, 2, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
   {2, 4, 4, 2, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
-  {2, 8, 8, 5, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
+  {2, 8, 8, 6, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p2_nt
   {2, 16, 16, 10, 1, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
   {2, 32, 32, 16, 1, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
-  {2, 64, 64, 24, 1, 0},  // tsm_ata_bf16_64x64_mfma_r256_p2_nt
+  {2, 64, 64, 26, 1, 0},  // tsm_ata_bf16_64x64_mfma_r512_p2_nt
   {1, 1, 1, 27, 2, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
   {1, 2, 2, 28, 2, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
   {1, 4, 4, 29, 2, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
-  {1, 8, 8, 30, 2, 0},  // tsm_ata_f32_8x8_t8x4_r512_p1_nt
-  {1, 16, 16, 41, 1, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
-  {1, 32, 32, 45, 1, 1},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
+  {1, 8, 8, 32, 2, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
+  {1, 16, 16, 40, 2, 1},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
+  {1, 32, 32, 47, 1, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
   {1, 64, 64, 53, 2, 0},  // tsm_ata_f32_64x64_mfma_r128_p2_nt
-  {0, 1, 1, 57, 1, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p2_nt
-  {0, 2, 2, 60, 2, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p1_nt
+  {0, 1, 1, 56, 2, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
+  {0, 2, 2, 61, 1, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
   {0, 4, 4, 69, 1, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 73, 1, 1},  // tsm_ata_f64_8x8_t8x8_r256_p2_nt
+  {0, 8, 8, 75, 1, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
   {0, 16, 16, 79, 1, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
   {0, 32, 32, 89, 1, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
   {0, 64, 64, 95, 2, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
