@@ -127,6 +127,16 @@ __device__ __forceinline__ v4u stream_load(const v4u* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   return *p;
 }
+// Inputs live in global memory.  The batched kernel reads A and B from its
+// problem table, where the compiler cannot prove that, and would emit flat
+// loads (which also count against lgkmcnt and serialise the pipelined LDS
+// waits); the dense tile loads take this global pointer type.
+typedef __attribute__((address_space(1))) const v4u gv4u;
+template <int NT>
+__device__ __forceinline__ v4u stream_load(gv4u* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
 
 // TwoSum (Knuth): s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
@@ -814,16 +824,16 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
 
   // ---- pipelined dense tiles ---------------------------------------------------
   if (nfull > 0) {
-    const v4u* gA = static_cast<const v4u*>(a.A);
-    const v4u* gB = static_cast<const v4u*>(a.B);
+    gv4u* gA = (gv4u*)a.A;
+    gv4u* gB = (gv4u*)a.B;
     struct Regs {
       v4u a[A_PER];
       v4u b[B_PER > 0 ? B_PER : 1];
     };
     auto load = [&](Regs& r, int64_t t) __attribute__((always_inline)) {
       const int64_t tile = first + t * step;
-      const v4u* ta = gA + tile * A_VEC;
-      const v4u* tb = gB + tile * B_VEC;
+      gv4u* ta = gA + tile * A_VEC;
+      gv4u* tb = gB + tile * B_VEC;
 #pragma unroll
       for (int v = 0; v < A_PER; ++v) {
         const int idx = v * kThreads + tid;
