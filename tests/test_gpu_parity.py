@@ -291,8 +291,10 @@ def test_backward_against_oracle(dt, k, m):
                                       ("f64", 2, 16, 5), ("f64", 16, 16, 4097), ("bf16", 8, 8, 1000),
                                       ("bf16", 64, 16, 2049)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
-    """Dense rows take the LDS-staged kernel and strided views the row-wise
-    one; both add the same products in the same order, so the bits agree."""
+    """Dense rows take the LDS-staged kernel (widths up to 16) or the
+    matrix-core kernel (32, 64), strided views the row-wise one.  The staged
+    and row-wise kernels add the same products in the same order, so their
+    bits agree; the matrix-core kernel is held to the oracle tolerance."""
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
     G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
     gdt = torch.float64 if dt == "f64" else torch.float32
@@ -302,11 +304,36 @@ def test_backward_staged_equals_rowwise(dt, k, m, n):
     Aw[:, :k] = A
     Bw[:, :m] = B
     dA2, dB2 = ops.atb_backward(Aw[:, :k], Bw[:, :m], G.to("cuda", gdt))
-    assert torch.equal(dA, dA2) and torch.equal(dB, dB2)
+    if max(k, m) <= 16:
+        assert torch.equal(dA, dA2) and torch.equal(dB, dB2)
     dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.to(gdt).double().numpy())
     tol = 1e-13 if dt == "f64" else (1e-5 if dt == "f32" else 2**-7 * np.abs(G.numpy()).sum())
     np.testing.assert_allclose(host(dA), dA_ref, rtol=tol if dt != "bf16" else 2**-7, atol=tol)
     np.testing.assert_allclose(host(dB), dB_ref, rtol=tol if dt != "bf16" else 2**-7, atol=tol)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("k,m,n", [(32, 32, 40001), (64, 64, 20011), (40, 24, 1000), (64, 8, 129),
+                                   (8, 64, 127), (48, 48, 5), (32, 64, 128)])
+def test_backward_matrix_core_against_oracle(dt, k, m, n):
+    """The 32- and 64-wide classes (dense rows) take the matrix-core kernel."""
+    if (k * DT[dt].itemsize) % 16 or (m * DT[dt].itemsize) % 16:
+        pytest.skip("rows that are not whole 16-byte chunks take the row-wise kernel")
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
+    gdt = torch.float64 if dt == "f64" else torch.float32
+    dA, dB = ops.atb_backward(A, B, G.to("cuda", gdt))
+    dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.to(gdt).double().numpy())
+    if dt == "f64":
+        tol = dict(rtol=1e-13, atol=1e-13)
+    elif dt == "f32":
+        tol = dict(rtol=1e-5, atol=1e-5)
+    else:
+        tol = dict(rtol=2**-7, atol=2**-7 * np.abs(G.numpy()).sum())
+    np.testing.assert_allclose(host(dA), dA_ref, **tol)
+    np.testing.assert_allclose(host(dB), dB_ref, **tol)
+    _, dB1 = ops.atb_backward(A, B, G.to("cuda", gdt), need_dA=False)
+    assert torch.equal(dB1, dB)
 
 
 def test_autograd_matches_torch_matmul():

@@ -289,7 +289,177 @@ __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   }
 }
 
+// ---- matrix-core kernel (width classes 32 and 64) ----------------------------------
+// Each phase is a small GEMM per tile: out (R x W) = in (R x W) * Gp (W x W),
+// with Gp[t][o] = dC[o][t] for dA and dC[t][o] for dB (dC zero-padded to W).
+// Wave w owns row blocks w, w + 4, ... of 16 rows; for each it runs W / 4
+// K-steps of 16x16x4 matrix instructions into W / 16 accumulator blocks:
+//   FP64: v_mfma_f64_16x16x4f64 (FP64 accumulation, as the VALU kernel);
+//   FP32 and BF16: v_mfma_f32_16x16x4f32 on FP32 operands (BF16 widened
+//   exactly), FP32 accumulation as the VALU kernel, one rounding to BF16.
+// Operand layout: lane l supplies in[16 rb + l % 16][4 s + l / 16] and
+// Gp[4 s + l / 16][16 nb + l % 16]; the result block holds row 4 (l / 16) + v
+// (FP32) or l / 16 + 4 v (FP64) in register v, column l % 16.  The tile is
+// staged in LDS with a padded pitch (row reads down a column are then
+// conflict-free), padding columns are zeroed, the outputs overwrite the
+// wave's own input rows, and the tile is stored as consecutive 16-byte chunks.
+template <class S>
+struct MfmaBwd;
+template <>
+struct MfmaBwd<double> {
+  using In = double;
+  using AccV = v4d;
+  static constexpr bool row_major_out = false;
+};
+template <>
+struct MfmaBwd<float> {
+  using In = float;
+  using AccV = v4f;
+  static constexpr bool row_major_out = true;
+};
+template <>
+struct MfmaBwd<bf16> {
+  using In = float;
+  using AccV = v4f;
+  static constexpr bool row_major_out = true;
+};
+
+template <class S, int W, int R>
+constexpr int mfma_bwd_pitch() { return W * (int)sizeof(S) + 16; }
+
+template <class S, int W, int R, bool TRANSPOSED, int CPT>
+__device__ __forceinline__ void mfma_phase(const v4u (&regs)[CPT], int rows, int in_w, int out_w,
+                                           S* out, int64_t r0, unsigned char* tile,
+                                           const typename Traits<S>::Acc* G) {
+  using Acc = typename Traits<S>::Acc;
+  using M = MfmaBwd<S>;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int PER = 16 / ES;
+  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
+  constexpr int GP = W + 1;        // dC pitch in LDS (Acc elements)
+  constexpr int CPR = W / PER;     // chunks per padded row
+  constexpr int NB = W / 16;       // output column blocks
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int in_cpr = in_w / PER, out_cpr = out_w / PER;
+  // stage the input tile (each thread writes the chunks it loaded) and zero
+  // the padding columns [in_w, W) of every row
+  const int in_chunks = rows * in_cpr;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = tid + i * kThreads;
+    if (c < in_chunks) {
+      const int r = c / in_cpr;
+      *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
+    }
+  }
+  const int pad_cpr = CPR - in_cpr;
+  for (int p = tid; p < R * pad_cpr; p += kThreads) {
+    const int r = p / pad_cpr;
+    *reinterpret_cast<v4u*>(tile + r * PITCH + (in_cpr + p - r * pad_cpr) * 16) =
+        v4u{0, 0, 0, 0};
+  }
+  __syncthreads();
+  for (int rb = wave; rb < R / 16; rb += 4) {
+    typename M::AccV acc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[nb] = typename M::AccV{};
+    const unsigned char* arow = tile + (rb * 16 + (lane & 15)) * PITCH;
+#pragma unroll
+    for (int st = 0; st < W / 4; ++st) {
+      const int t = 4 * st + (lane >> 4);
+      const typename M::In a = (typename M::In)widen(reinterpret_cast<const S*>(arow)[t]);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int o = 16 * nb + (lane & 15);
+        const Acc g = TRANSPOSED ? G[t * GP + o] : G[o * GP + t];
+        acc[nb] = mfma_step(a, (typename M::In)g, acc[nb]);
+      }
+    }
+    // outputs overwrite this wave's own rows of the tile
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = M::row_major_out ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v;
+        const int c = 16 * nb + (lane & 15);
+        reinterpret_cast<S*>(tile + (rb * 16 + r) * PITCH)[c] = narrow<S>((Acc)acc[nb][v]);
+      }
+  }
+  __syncthreads();
+  v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
+  const int out_chunks = rows * out_cpr;
+  for (int c = tid; c < out_chunks; c += kThreads) {
+    const int r = c / out_cpr;
+    const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
+    __builtin_nontemporal_store(u, dst + c);
+  }
+  __syncthreads();  // the tile is rewritten by the next phase
+}
+
+template <class S, int W, int R>
+__device__ __forceinline__ void load_rows(const S* in, int64_t r0, int rows, int w,
+                                          v4u (&regs)[R * W * (int)sizeof(S) / 16 / kThreads]) {
+  constexpr int CPT = R * W * (int)sizeof(S) / 16 / kThreads;
+  const v4u* src = reinterpret_cast<const v4u*>(in + r0 * w);
+  const int chunks = rows * (w * (int)sizeof(S) / 16);
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = threadIdx.x + i * kThreads;
+    regs[i] = c < chunks ? __builtin_nontemporal_load(src + c) : v4u{0, 0, 0, 0};
+  }
+}
+
+template <class S, int W, int R>
+__device__ __forceinline__ void bwd_mfma_body(const BwdArgs& a) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int CPT = R * W * ES / 16 / kThreads;  // input chunks per thread (dense rows)
+  static_assert(W % 16 == 0 && R % 64 == 0 && CPT >= 1, "16-wide blocks, 4 waves");
+  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
+  constexpr int GP = W + 1;
+  __shared__ Acc G[W * GP];
+  __shared__ __attribute__((aligned(16))) unsigned char tile[R * PITCH];
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const int i = e / W, j = e % W;
+    Acc v = Acc(0);
+    if (i < a.k && j < a.m) {
+      const int64_t off = (int64_t)i * a.lddc + j;
+      v = a.dc_f64 ? (Acc) static_cast<const double*>(a.dC)[off]
+                   : (Acc) static_cast<const float*>(a.dC)[off];
+    }
+    G[i * GP + j] = v;
+  }
+  __syncthreads();
+  const S* A = static_cast<const S*>(a.A);
+  const S* B = static_cast<const S*>(a.B);
+  S* dA = static_cast<S*>(a.dA);
+  S* dB = static_cast<S*>(a.dB);
+  const int64_t ntiles = (a.n + R - 1) / R;
+  for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+    const int64_t r0 = tt * R;
+    const int rows = (int)((a.n - r0) < R ? (a.n - r0) : R);
+    // chunk i of this thread is input chunk tid + i * 256 of the dense rows
+    v4u rb[CPT], ra[CPT];
+    if (dA) load_rows<S, W, R>(B, r0, rows, a.m, rb);
+    if (dB) load_rows<S, W, R>(A, r0, rows, a.k, ra);
+    if (dA) mfma_phase<S, W, R, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, G);
+    if (dB) mfma_phase<S, W, R, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, G);
+  }
+}
+
 }  // namespace tsm
+
+#define TSM_BWD_MFMA(NAME, S, W)                                                      \
+  extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
+    tsm::bwd_mfma_body<S, W, 128>(a);                                                 \
+  }
+
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w64, double, 64)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_bf16_w32, tsm::bf16, 32)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_bf16_w64, tsm::bf16, 64)
 
 #define TSM_BWD_STAGED(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
@@ -355,6 +525,12 @@ const BwdKernel kBwdStaged[3][7] = {
     {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, tsm_atb_bwd_staged_bf16_w16,
      tsm_atb_bwd_staged_bf16_w32, tsm_atb_bwd_staged_bf16_w64},
 };
+// Matrix-core kernels for the 32- and 64-wide classes (dense rows).
+const BwdKernel kBwdMfma[3][2] = {
+    {tsm_atb_bwd_mfma_f64_w32, tsm_atb_bwd_mfma_f64_w64},
+    {tsm_atb_bwd_mfma_f32_w32, tsm_atb_bwd_mfma_f32_w64},
+    {tsm_atb_bwd_mfma_bf16_w32, tsm_atb_bwd_mfma_bf16_w64},
+};
 int elem(int t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
 }  // namespace
 
@@ -417,8 +593,18 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   // the device at once (its LDS tile bounds the count per CU).
   BwdKernel kern = kBwd[(int)in_type][wi];
   int64_t cap = (int64_t)cus * 8;
-  if (a.dense_a && a.dense_b && kBwdStaged[(int)in_type][wi]) {
-    kern = kBwdStaged[(int)in_type][wi];
+  const bool dense = a.dense_a && a.dense_b;
+  BwdKernel fast = nullptr;
+  int64_t tile_rows = 256;
+  if (dense && wi >= 5) {
+    fast = kBwdMfma[(int)in_type][wi - 5];
+    tile_rows = 128;
+  } else if (dense) {
+    fast = kBwdStaged[(int)in_type][wi];
+  }
+  if (fast) {
+    kern = fast;
+    blocks = (n + tile_rows - 1) / tile_rows;
     static int resident[3][7];  // workgroups per CU, filled on first use
     int& per_cu = resident[(int)in_type][wi];
     if (per_cu == 0) {
