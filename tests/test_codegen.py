@@ -92,3 +92,29 @@ def test_cli_name_layer():
     assert "kGeneratedNames" in out
     # default stride follows the reference's rule with the gfx950 wavefront
     assert f"stride={codegen.suggest_stride([4, 8], [2, 4])}" in out
+
+
+def test_tile_table_format_and_device_are_checked(tmp_path):
+    """The tile table records its format, device and ROCm version; a table of
+    another format or measured on another architecture is refused."""
+    import json
+    t = codegen.load_tuning()
+    assert t["format"] == codegen.TUNING_FORMAT and t["device"].startswith("gfx950")
+    assert "rocm" in t and "library_version" in t
+    bad = dict(t, format=codegen.TUNING_FORMAT + 1)
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(bad))
+    with pytest.raises(ValueError, match="format"):
+        codegen.load_tuning(str(p))
+    p.write_text(json.dumps(dict(t, device="gfx942:sramecc+:xnack-")))
+    with pytest.raises(ValueError, match="gfx942"):
+        codegen.load_tuning(str(p))
+    # every tuned entry names a compiled kernel of the library
+    cfgs, defaults = codegen.library_kernels(t)
+    syms, sym_defaults = codegen.sym_kernels(defaults, t)
+    for e in t["entries"]:
+        c = defaults[(e["dtype"], e["kc"], e["mc"])][0]
+        assert (c.tm, c.tn, c.rows, c.pf, c.nt) == (e["tm"], e["tn"], e["rows"], e["pf"], e["nt"])
+    for e in t.get("sym_entries", []):
+        c = sym_defaults[(e["dtype"], e["kc"])][0]
+        assert c.sym == 1 and (c.rows, c.pf) == (e["rows"], e["pf"]) and c in syms

@@ -178,7 +178,10 @@ def main(argv=None) -> int:
             e = tune_sym(dt, k, args.gib, args.reps, log)
             print(f"-> {e}", flush=True)
             sym_entries[(dt, k)] = e
-    out = {"device": torch.cuda.get_device_properties(0).gcnArchName,
+    out = {"format": codegen.TUNING_FORMAT,
+           "device": torch.cuda.get_device_properties(0).gcnArchName,
+           "rocm": getattr(torch.version, "hip", None),
+           "library_version": _lib.lib().tsm_version(),
            "generated_by": "python -m tsm_benchmark_amd.autotune",
            "entries": sorted(entries.values(), key=lambda e: (e["dtype"], e["kc"], e["mc"])),
            "sym_entries": sorted(sym_entries.values(), key=lambda e: (e["dtype"], e["kc"]))}

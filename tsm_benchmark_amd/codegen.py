@@ -48,6 +48,7 @@ DTYPES = ("f64", "f32", "bf16")
 DTYPE_ID = {"f64": 0, "f32": 1, "bf16": 2}
 ELEM_BYTES = {"f64": 8, "f32": 4, "bf16": 2}
 CXX_TYPE = {"f64": "double", "f32": "float", "bf16": "tsm::bf16"}
+ARCH = "gfx950"  # the one target of the generated kernels
 CLASSES = (1, 2, 4, 8, 16, 32, 64)
 THREADS = 256          # work-group size (4 wave64)
 WAVE = 64              # gfx950 wavefront size
@@ -262,11 +263,27 @@ def reference_variants(Ms, tiles, *, dtype: str = "f64") -> list[KernelConfig]:
     return out
 
 
+TUNING_FORMAT = 1  # version of the tile-table layout below
+
+
 def load_tuning(path: str = TUNING_JSON) -> dict:
+    """The tile table: {"format", "device", "rocm", "library_version",
+    "generated_by", "entries": [A^T B class entries], "sym_entries": [A^T A]}.
+
+    A table of another format, or measured on another architecture than
+    ARCH, is rejected: its configurations would be baked into kernels tuned
+    for a different chip."""
     if not os.path.exists(path):
-        return {"device": "gfx950", "entries": []}
+        return {"format": TUNING_FORMAT, "device": ARCH, "entries": []}
     with open(path) as f:
-        return json.load(f)
+        t = json.load(f)
+    fmt = t.get("format", 1)
+    if fmt != TUNING_FORMAT:
+        raise ValueError(f"{path}: tile-table format {fmt}, this generator reads {TUNING_FORMAT}")
+    dev = t.get("device", ARCH).split(":")[0]
+    if dev != ARCH:
+        raise ValueError(f"{path}: measured on {dev}, the library is built for {ARCH}")
+    return t
 
 
 def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
