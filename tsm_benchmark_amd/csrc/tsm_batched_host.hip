@@ -100,9 +100,17 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
     }
   }
   // Largest items first, dealt round-robin: a static, deterministic balance.
+  // Among items of one size, slot-major order: the workgroups running at the
+  // same time read items of different problems.  Problem-major order had all
+  // workgroups streaming neighbouring 2 MiB items of one problem at once,
+  // which measured 3.9 instead of 6.3 TB/s for a batch of equal FP64 32 x 32
+  // problems (the summation order within a problem is by slot either way).
   std::stable_sort(L->items.begin(), L->items.end(),
                    [](const tsm::BatchItem& x, const tsm::BatchItem& y) {
-                     return (x.tile_end - x.tile_begin) > (y.tile_end - y.tile_begin);
+                     const int64_t sx = x.tile_end - x.tile_begin, sy = y.tile_end - y.tile_begin;
+                     if (sx != sy) return sx > sy;
+                     if (x.slot != y.slot) return x.slot < y.slot;
+                     return x.problem < y.problem;
                    });
   const size_t counters = align_up(sizeof(unsigned) * (size_t)std::max(count, 1));
   L->probs_off = counters;
