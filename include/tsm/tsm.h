@@ -65,14 +65,18 @@ typedef enum { TSM_DIST_UNIFORM = 0, TSM_DIST_POW2 = 1 } tsm_dist;
 typedef struct {
   int tm, tn;          /* register tile: C entries owned per thread = tm x tn */
   int rows;            /* rows per LDS tile (the n-tile streamed per stage) */
-  int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth) */
+  int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth);
+                          direct engine: 4-row steps in flight per wave, 4, 8 or 16 */
   int wgs_per_cu;      /* persistent workgroups per compute unit */
   int64_t grid;        /* total workgroups; overrides wgs_per_cu when > 0 (the "stride" role) */
   int load_policy;     /* 0 tuned default, 1 cached loads, 2 non-temporal (streaming) loads */
   int tile_order;      /* 0 tuned default, 1 contiguous runs of tiles per workgroup,
                           2 interleaved (tile t to workgroup t mod grid) */
   int engine;          /* 0 tuned default, 1 VALU register tiles (tm x tn), 2 matrix cores
-                          (MFMA; width classes that are multiples of 16) */
+                          (MFMA) fed through LDS tiles, 3 matrix cores fed straight from
+                          global memory (FP64/FP32; pf = 4-row steps in flight per wave;
+                          inputs that are not dense run the class's staged kernel).
+                          2 and 3 need width classes that are multiples of 16. */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------
@@ -206,6 +210,7 @@ typedef struct {
   int tm, tn, rows, pf;              /* tm = tn = 0: matrix-core (MFMA) kernel */
   int load_policy;                  /* 1 cached, 2 non-temporal */
   int lds_bytes;
+  int engine;                       /* as in tsm_kernel_config: 1, 2 or 3 */
 } tsm_kernel_desc;
 
 int tsm_num_kernels(void);

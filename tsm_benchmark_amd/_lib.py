@@ -40,7 +40,8 @@ class KernelConfig(ct.Structure):
 class KernelDesc(ct.Structure):
     _fields_ = [("name", ct.c_char_p), ("in_type", ct.c_int), ("kc", ct.c_int),
                 ("mc", ct.c_int), ("tm", ct.c_int), ("tn", ct.c_int), ("rows", ct.c_int),
-                ("pf", ct.c_int), ("load_policy", ct.c_int), ("lds_bytes", ct.c_int)]
+                ("pf", ct.c_int), ("load_policy", ct.c_int), ("lds_bytes", ct.c_int),
+                ("engine", ct.c_int)]
 
 
 class Problem(ct.Structure):
@@ -145,25 +146,32 @@ def kernels() -> list[dict]:
         check(L.tsm_kernel_info(i, ct.byref(d)), "tsm_kernel_info")
         out.append({"index": i, "name": d.name.decode(), "in_type": d.in_type, "kc": d.kc,
                     "mc": d.mc, "tm": d.tm, "tn": d.tn, "rows": d.rows, "pf": d.pf,
-                    "nt": 1 if d.load_policy == 2 else 0, "lds_bytes": d.lds_bytes})
+                    "nt": 1 if d.load_policy == 2 else 0, "lds_bytes": d.lds_bytes,
+                    "engine": d.engine})
     return out
 
 
+ENGINE_VALU, ENGINE_MFMA_LDS, ENGINE_DIRECT = 1, 2, 3
+
+
 def make_config(tm=0, tn=0, rows=0, pf=0, wgs_per_cu=0, grid=0, nt=None,
-                interleave=None, mfma=None) -> KernelConfig:
+                interleave=None, mfma=None, engine=None) -> KernelConfig:
     """nt: None = tuned default, 0 = cached loads, 1 = non-temporal loads;
     interleave: None = tuned default, 0 = contiguous tile runs, 1 = interleaved;
-    mfma: None = tuned default, False = VALU register tiles, True = matrix cores."""
+    mfma: None = tuned default, False = VALU register tiles, True = matrix cores
+    through LDS; engine (overrides mfma): 1 VALU, 2 matrix cores through LDS,
+    3 matrix cores fed from global memory (pf = 4-row steps in flight)."""
     policy = 0 if nt is None else (2 if nt else 1)
     order = 0 if interleave is None else (2 if interleave else 1)
-    engine = 0 if mfma is None else (2 if mfma else 1)
+    if engine is None:
+        engine = 0 if mfma is None else (ENGINE_MFMA_LDS if mfma else ENGINE_VALU)
     return KernelConfig(tm, tn, rows, pf, wgs_per_cu, grid, policy, order, engine)
 
 
 def config_of(kd: dict, **kw) -> KernelConfig:
     """Config that selects exactly the kernel kd (an entry of kernels())."""
     return make_config(tm=kd["tm"], tn=kd["tn"], rows=kd["rows"], pf=kd["pf"], nt=kd["nt"],
-                       mfma=kd["tm"] == 0, **kw)
+                       engine=kd["engine"], **kw)
 
 
 def plan(in_type: int, n: int, k: int, m: int, config: KernelConfig | None = None):

@@ -39,9 +39,16 @@ def sources() -> list[str]:
     return [os.path.join(CSRC, s) for s in HAND_WRITTEN] + gen
 
 
+# Direct-engine kernels keep MFMA accumulators in ordinary VGPRs: in the
+# default AGPR form the compiler copies them between the register files on
+# every iteration of the streaming loop.
+DIRECT_FLAGS = ["-mllvm", "-amdgpu-mfma-vgpr-form=true"]
+
+
 def _compile(src: str) -> tuple[str, str]:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
+    extra = DIRECT_FLAGS if os.path.basename(src).startswith("tsm_direct_") else []
+    cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")

@@ -54,6 +54,7 @@ THREADS = 256          # work-group size (4 wave64)
 WAVE = 64              # gfx950 wavefront size
 LDS_BUDGET = 65536     # LDS per work-group of the defaults (two fit in 160 KiB)
 LDS_MAX = 131072       # LDS per work-group of any variant (one per compute unit)
+DIRECT_DEPTHS = (4, 8, 16)  # steps in flight per wave of the direct engine (divide 16)
 MAX_ROWS_PER_THREAD = 16  # the fully unrolled row loop of one tile (compile time)
 
 
@@ -82,6 +83,9 @@ class KernelConfig:
     pf: int
     nt: int = 0  # 1: non-temporal (streaming) global loads
     sym: int = 0  # 1: C = A^T A (B is A; only A's tile is staged)
+    # 1: matrix cores fed straight from global memory, no LDS staging
+    # (rows = 4, the K-step; pf = steps in flight per wave)
+    direct: int = 0
 
     @property
     def mfma(self) -> bool:
@@ -89,9 +93,17 @@ class KernelConfig:
         return self.tm == 0 and self.tn == 0
 
     @property
+    def engine(self) -> int:
+        """1 VALU register tiles, 2 matrix cores through LDS, 3 direct matrix cores."""
+        return 3 if self.direct else (2 if self.mfma else 1)
+
+    @property
     def name(self) -> str:
-        tile = "mfma" if self.mfma else f"t{self.tm}x{self.tn}"
         op = "ata" if self.sym else "atb"
+        if self.direct:
+            return (f"tsm_{op}_{self.dtype}_{self.kc}x{self.mc}_direct_d{self.pf}"
+                    + ("_nt" if self.nt else ""))
+        tile = "mfma" if self.mfma else f"t{self.tm}x{self.tn}"
         return (f"tsm_{op}_{self.dtype}_{self.kc}x{self.mc}_{tile}"
                 f"_r{self.rows}_p{self.pf}" + ("_nt" if self.nt else ""))
 
@@ -114,6 +126,8 @@ class KernelConfig:
 
     @property
     def lds_bytes(self) -> int:
+        if self.direct:  # two KC x MC FP64 halves of the cross-wave sum
+            return max(2 * self.kc * self.mc * 8, THREADS * 16)
         return max(2 * self.stage_bytes, self.kc * self.mc * 8, THREADS * 8)
 
     def problems(self) -> list[str]:
@@ -127,6 +141,22 @@ class KernelConfig:
             return p
         if self.sym not in (0, 1) or (self.sym and self.kc != self.mc):
             p.append("A^T A kernels are square")
+        if self.direct:
+            if self.direct != 1:
+                p.append("direct must be 0 or 1")
+            if self.dtype not in ("f64", "f32"):
+                p.append("the direct engine reads FP64 and FP32 rows (BF16 needs a transpose)")
+            if self.kc % 16 or self.mc % 16:
+                p.append("MFMA needs width classes that are multiples of 16")
+            if self.tm or self.tn:
+                p.append("the direct engine has no register tile")
+            if self.rows != 4:
+                p.append("direct-engine steps are 4 rows")
+            if self.pf not in DIRECT_DEPTHS or self.nt not in (0, 1):
+                p.append(f"steps in flight must be one of {DIRECT_DEPTHS}, nt 0 or 1")
+            if self.lds_bytes > LDS_MAX:
+                p.append("LDS budget")
+            return p
         if self.mfma:
             if self.kc % 16 or self.mc % 16:
                 p.append("MFMA needs width classes that are multiples of 16")
@@ -240,7 +270,15 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
                     c = KernelConfig(dtype, kc, mc, tm, tn, rows, pf, nt)
                     if not c.problems():
                         out.add(c)
+    out.update(direct_variants(dtype, kc, mc))
     return sorted(out)
+
+
+def direct_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
+    """Direct-engine kernels of a class: every depth with streaming loads (none
+    for BF16 or widths that are not multiples of 16)."""
+    out = [KernelConfig(dtype, kc, mc, 0, 0, 4, d, 1, 0, 1) for d in DIRECT_DEPTHS]
+    return [c for c in out if not c.problems()]
 
 
 def reference_variants(Ms, tiles, *, dtype: str = "f64") -> list[KernelConfig]:
@@ -291,6 +329,8 @@ def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
     tuned A^T B consumer of the class with half, one, two and four times its
     rows per tile and one or two tiles in flight."""
     c = defaults[(dtype, kc, kc)][0]
+    if c.direct:
+        return sorted(dataclasses.replace(v, sym=1) for v in direct_variants(dtype, kc, kc))
     out = set()
     for rows in (c.rows // 2, c.rows, 2 * c.rows, 4 * c.rows):
         for pf in (1, 2):
@@ -310,28 +350,67 @@ def sym_kernels(defaults: dict, tuning: dict) -> tuple[list[KernelConfig], dict]
     "sym_entries" of the tile table."""
     cfgs: set[KernelConfig] = set()
     out = {}
+    fbs = library_fallbacks(tuning, defaults)
     for dt in DTYPES:
         for kc in CLASSES:
             c, wpc, il = defaults[(dt, kc, kc)]
-            s = dataclasses.replace(c, sym=1)
-            if not c.mfma:
-                wide = dataclasses.replace(s, rows=2 * c.rows)
-                if not wide.problems():
-                    s = wide
-            if s.problems():
-                raise ValueError(f"{s.name}: {s.problems()}")
+            s = sym_of(c)
             cfgs.add(s)
+            cfgs.add(sym_of(fbs[(dt, kc, kc)][0]))
             out[(dt, kc)] = (s, wpc, il)
             if (kc, kc) in TUNE_SHAPES[dt]:
                 cfgs.update(sym_variants(defaults, dt, kc))
     for e in tuning.get("sym_entries", []):
-        c = KernelConfig(e["dtype"], e["kc"], e["kc"], e["tm"], e["tn"], e["rows"], e["pf"],
-                         int(e.get("nt", 0)), 1)
-        if c.problems():
-            raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
+        c = config_of_entry(e, sym=1)
         cfgs.add(c)
         out[(c.dtype, c.kc)] = (c, int(e.get("wgs_per_cu", 1)), int(e.get("interleave", 0)))
     return sorted(cfgs), out
+
+
+def sym_of(c: KernelConfig) -> KernelConfig:
+    """The A^T A kernel of an A^T B configuration: same consumer; a stage
+    holds A only, so register-tile kernels take twice the rows where that is
+    valid, which keeps the bytes in flight per tile."""
+    s = dataclasses.replace(c, sym=1)
+    if not c.mfma:
+        wide = dataclasses.replace(s, rows=2 * c.rows)
+        if not wide.problems():
+            s = wide
+    if s.problems():
+        raise ValueError(f"{s.name}: {s.problems()}")
+    return s
+
+
+def config_of_entry(e: dict, sym: int = 0) -> KernelConfig:
+    """KernelConfig of a tile-table entry (or of its "fallback")."""
+    c = KernelConfig(e["dtype"], e["kc"], e.get("mc", e["kc"]) if not sym else e["kc"],
+                     e["tm"], e["tn"], e["rows"], e["pf"], int(e.get("nt", 0)), sym,
+                     int(e.get("direct", 0)))
+    if c.problems():
+        raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
+    return c
+
+
+def library_fallbacks(tuning: dict, defaults: dict) -> dict:
+    """Per class, the (kernel, interleave) that runs when the inputs do not
+    suit the class default: the direct engine reads only dense rows (k and m
+    equal to the class widths, leading dimensions k and m, 16-byte aligned
+    bases); other kernels handle every input themselves.  A direct default
+    falls back to the tuned staged kernel recorded with it, else to the
+    heuristic configuration of the class."""
+    out = {}
+    tuned = {(e["dtype"], e["kc"], e["mc"]): e for e in tuning.get("entries", [])}
+    for key, (c, _, il) in defaults.items():
+        if not c.direct:
+            out[key] = (c, il)
+            continue
+        fb = tuned.get(key, {}).get("fallback")
+        if fb:
+            out[key] = (config_of_entry(dict(fb, dtype=key[0], kc=key[1], mc=key[2])),
+                        int(fb.get("interleave", 0)))
+        else:
+            out[key] = (default_config(*key), 0)
+    return out
 
 
 def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
@@ -344,17 +423,17 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
                 c = default_config(dt, kc, mc)
                 cfgs.add(c)
                 defaults[(dt, kc, mc)] = (c, 2, 0)
+                # one direct kernel in every class the engine serves
+                cfgs.update(d for d in direct_variants(dt, kc, mc) if d.pf == 8)
         for kc, mc in TUNE_SHAPES[dt]:
             cfgs.update(tuning_variants(dt, kc, mc))
     cfgs.update(reference_variants([4, 8, 16], [2, 4]))
     for e in tuning.get("entries", []):
-        c = KernelConfig(e["dtype"], e["kc"], e["mc"], e["tm"], e["tn"], e["rows"], e["pf"],
-                         int(e.get("nt", 0)))
-        if c.problems():
-            raise ValueError(f"tuning entry {e} is invalid: {c.problems()}")
+        c = config_of_entry(e)
         cfgs.add(c)
         defaults[(c.dtype, c.kc, c.mc)] = (c, int(e.get("wgs_per_cu", 2)),
                                            int(e.get("interleave", 0)))
+    cfgs.update(c for c, _ in library_fallbacks(tuning, defaults).values())
     return sorted(cfgs), defaults
 
 
@@ -367,6 +446,18 @@ BANNER = ("// GENERATED by tsm_benchmark_amd/codegen.py - do not edit.\n"
 
 def emit_kernel(c: KernelConfig) -> str:
     t = CXX_TYPE[c.dtype]
+    if c.direct:
+        targs = f"{t}, {c.kc}, {c.mc}, {c.pf}, {c.nt}" + (", 1" if c.sym else "")
+        return (
+            f"// {c.name}: KC={c.kc} MC={c.mc}, matrix cores fed from global memory, "
+            f"{c.pf} 4-row steps in flight per wave, nt={c.nt}, LDS {c.lds_bytes} B\n"
+            f"extern \"C\" __global__ void __launch_bounds__(256) {c.name}(tsm::AtbArgs a) {{\n"
+            f"  tsm::direct_body<{targs}>(a);\n"
+            f"}}\n"
+            f"hipError_t launch_{c.name}(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {{\n"
+            f"  hipLaunchKernelGGL({c.name}, dim3(grid), dim3(256), 0, s, a);\n"
+            f"  return hipGetLastError();\n"
+            f"}}\n")
     targs = f"{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, {c.pf}, {c.nt}" + (", 1" if c.sym else "")
     if c.mfma:
         shape = (f"//   matrix cores: {c.rows // c.kstep} K-steps of {c.kstep} rows per tile, "
@@ -389,7 +480,22 @@ def emit_kernel(c: KernelConfig) -> str:
 def emit_kernel_file(dtype: str, kc: int, mc: int, cfgs: list[KernelConfig]) -> str:
     body = [BANNER, '#include "../tsm_internal.hpp"\n']
     for c in cfgs:
-        if c.dtype == dtype and c.kc == kc and c.mc == mc:
+        if c.dtype == dtype and c.kc == kc and c.mc == mc and not c.direct:
+            body.append(emit_kernel(c))
+    return "\n".join(body)
+
+
+# Direct-engine kernels are compiled with MFMA results in ordinary VGPRs
+# (build.py adds -mllvm -amdgpu-mfma-vgpr-form=true for these files): with
+# the default AGPR form the accumulators are copied between the two register
+# files on every loop iteration.
+DIRECT_FILE = "tsm_direct_{}.hip"
+
+
+def emit_direct_file(dtype: str, cfgs: list[KernelConfig]) -> str:
+    body = [BANNER, '#include "../tsm_internal.hpp"\n']
+    for c in cfgs:
+        if c.dtype == dtype and c.direct:
             body.append(emit_kernel(c))
     return "\n".join(body)
 
@@ -397,16 +503,19 @@ def emit_kernel_file(dtype: str, kc: int, mc: int, cfgs: list[KernelConfig]) -> 
 BATCH_LDS = LDS_BUDGET  # LDS of the batched kernels (every class fits one budget)
 
 
-def batched_classes(defaults: dict, dtype: str) -> list[KernelConfig]:
+def batched_classes(defaults: dict, dtype: str, tuning: dict | None = None) -> list[KernelConfig]:
     """Per width class, the configuration the batched kernel uses: the tuned
-    default when it fits the batched LDS budget, else the heuristic one."""
+    default (its staged fallback for a direct-engine default, whose lanes
+    each stream their own rows) when it fits the batched LDS budget, else the
+    heuristic one."""
+    fbs = library_fallbacks(tuning or {}, defaults)
     out = []
     # FP32/BF16 carry a (hi, lo) pair per accumulator: their 64-wide classes
     # would not fit the shared register budget, so batching stops at 32.
     widths = CLASSES if dtype == "f64" else tuple(c for c in CLASSES if c <= 32)
     for kc in widths:
         for mc in widths:
-            c = defaults[(dtype, kc, mc)][0]
+            c = fbs[(dtype, kc, mc)][0]
             if c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32):
                 c = default_config(dtype, kc, mc)
             out.append(c)
@@ -450,26 +559,31 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict, tuning: dict) -> str:
     idx = {c: i for i, c in enumerate(cfgs)}
     sym_cfgs, sym_defaults = sym_kernels(defaults, tuning)
     sidx = {c: i for i, c in enumerate(sym_cfgs)}
+    fbs = library_fallbacks(tuning, defaults)
     out = [BANNER, '#include "../tsm_internal.hpp"\n']
     for c in cfgs + sym_cfgs:
         out.append(f"hipError_t launch_{c.name}(const tsm::AtbArgs&, unsigned, hipStream_t);")
+
+    def entry(c: KernelConfig) -> str:
+        return (f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
+                f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}, {c.engine}}},")
     out.append("\nstatic const tsm::KernelEntry kTable[] = {")
-    for c in cfgs:
-        out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
-                   f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
+    out += [entry(c) for c in cfgs]
     out.append("};\n")
     out.append("static const tsm::KernelEntry kSym[] = {")
-    for c in sym_cfgs:
-        out.append(f"  {{\"{c.name}\", {DTYPE_ID[c.dtype]}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, "
-                   f"{c.rows}, {c.pf}, {c.nt}, {c.lds_bytes}, launch_{c.name}}},")
+    out += [entry(c) for c in sym_cfgs]
     out.append("};\n")
     out.append("static const tsm::TunedEntry kSymTuned[] = {")
     for (dt, kc), (c, wpc, il) in sorted(sym_defaults.items()):
-        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {kc}, {sidx[c]}, {wpc}, {il}}},  // {c.name}")
+        fc, fil = (sym_of(fbs[(dt, kc, kc)][0]), fbs[(dt, kc, kc)][1]) if c.direct else (c, il)
+        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {kc}, {sidx[c]}, {wpc}, {il}, {sidx[fc]}, {fil}}},"
+                   f"  // {c.name}" + (f" (else {fc.name})" if fc != c else ""))
     out.append("};\n")
     out.append("static const tsm::TunedEntry kTuned[] = {")
     for (dt, kc, mc), (c, wpc, il) in sorted(defaults.items()):
-        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}, {il}}},  // {c.name}")
+        fc, fil = fbs[(dt, kc, mc)]
+        out.append(f"  {{{DTYPE_ID[dt]}, {kc}, {mc}, {idx[c]}, {wpc}, {il}, {idx[fc]}, {fil}}},"
+                   f"  // {c.name}" + (f" (else {fc.name})" if fc != c else ""))
     out.append("};\n")
     for dt in DTYPES:
         nm = f"tsm_atb_batched_{dt}"
@@ -481,7 +595,7 @@ def emit_table(cfgs: list[KernelConfig], defaults: dict, tuning: dict) -> str:
     out.append("  static bool init = [] {")
     for i, dt in enumerate(DTYPES):
         nm = f"tsm_atb_batched_{dt}"
-        lds = max(c.lds_bytes for c in batched_classes(defaults, dt))
+        lds = max(c.lds_bytes for c in batched_classes(defaults, dt, tuning))
         out.append(f"    all[{i}].name = \"{nm}\"; all[{i}].dtype = {DTYPE_ID[dt]};")
         out.append(f"    all[{i}].classes = classes_{nm}(&all[{i}].n_classes);")
         out.append(f"    all[{i}].lds_bytes = {lds}; all[{i}].launch = launch_{nm};")
@@ -527,6 +641,10 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
                 p = os.path.join(out_dir, f"tsm_kernels_{dt}_{kc}x{mc}.hip")
                 _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs + syms))
                 paths.append(p)
+    for dt in ("f64", "f32"):
+        p = os.path.join(out_dir, DIRECT_FILE.format(dt))
+        _write_if_changed(p, emit_direct_file(dt, cfgs + syms))
+        paths.append(p)
     keep = {os.path.basename(x) for x in paths} | {"tsm_table.hip"} | \
         {f"tsm_batched_{dt}.hip" for dt in DTYPES}
     for f in os.listdir(out_dir):  # drop files of an older split
@@ -534,7 +652,7 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
             os.remove(os.path.join(out_dir, f))
     for dt in DTYPES:
         p = os.path.join(out_dir, f"tsm_batched_{dt}.hip")
-        _write_if_changed(p, emit_batched_file(dt, batched_classes(defaults, dt)))
+        _write_if_changed(p, emit_batched_file(dt, batched_classes(defaults, dt, tuning)))
         paths.append(p)
     p = os.path.join(out_dir, "tsm_table.hip")
     _write_if_changed(p, emit_table(cfgs, defaults, tuning))

@@ -213,6 +213,9 @@ hipError_t launch_tsm_atb_f32_16x1_t8x1_r256_p2(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f32_16x2_t8x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x4_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x8_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_mfma_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -243,13 +246,19 @@ hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x64_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x1_t8x1_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x2_t8x2_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x8_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x16_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_mfma_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -284,13 +293,19 @@ hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x64_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x1_t8x1_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x2_t8x2_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x4_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x8_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x16_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x32_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_mfma_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -410,6 +425,9 @@ hipError_t launch_tsm_atb_f64_16x1_t8x1_r128_p2(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_16x2_t8x2_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x8_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_mfma_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -442,7 +460,9 @@ hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x64_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x1_t8x1_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x2_t8x2_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -461,7 +481,11 @@ hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x8_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x16_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_mfma_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -490,13 +514,19 @@ hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x64_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x1_t8x1_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x2_t8x2_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x4_t8x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x8_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x16_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x32_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_mfma_r16_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_mfma_r16_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -621,799 +651,829 @@ hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 
 static const tsm::KernelEntry kTable[] = {
-  {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2},
-  {"tsm_atb_bf16_1x2_t1x2_r4096_p2", 2, 1, 2, 1, 2, 4096, 2, 0, 49152, launch_tsm_atb_bf16_1x2_t1x2_r4096_p2},
-  {"tsm_atb_bf16_1x4_t1x4_r2048_p2", 2, 1, 4, 1, 4, 2048, 2, 0, 40960, launch_tsm_atb_bf16_1x4_t1x4_r2048_p2},
-  {"tsm_atb_bf16_1x8_t1x4_r1024_p2", 2, 1, 8, 1, 4, 1024, 2, 0, 36864, launch_tsm_atb_bf16_1x8_t1x4_r1024_p2},
-  {"tsm_atb_bf16_1x16_t1x4_r512_p2", 2, 1, 16, 1, 4, 512, 2, 0, 34816, launch_tsm_atb_bf16_1x16_t1x4_r512_p2},
-  {"tsm_atb_bf16_1x32_t1x4_r256_p2", 2, 1, 32, 1, 4, 256, 2, 0, 33792, launch_tsm_atb_bf16_1x32_t1x4_r256_p2},
-  {"tsm_atb_bf16_1x64_t1x4_r128_p2", 2, 1, 64, 1, 4, 128, 2, 0, 33280, launch_tsm_atb_bf16_1x64_t1x4_r128_p2},
-  {"tsm_atb_bf16_2x1_t2x1_r4096_p2", 2, 2, 1, 2, 1, 4096, 2, 0, 49152, launch_tsm_atb_bf16_2x1_t2x1_r4096_p2},
-  {"tsm_atb_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_atb_bf16_2x2_t2x2_r4096_p2},
-  {"tsm_atb_bf16_2x4_t2x4_r2048_p2", 2, 2, 4, 2, 4, 2048, 2, 0, 49152, launch_tsm_atb_bf16_2x4_t2x4_r2048_p2},
-  {"tsm_atb_bf16_2x8_t2x4_r1024_p2", 2, 2, 8, 2, 4, 1024, 2, 0, 40960, launch_tsm_atb_bf16_2x8_t2x4_r1024_p2},
-  {"tsm_atb_bf16_2x16_t2x4_r512_p2", 2, 2, 16, 2, 4, 512, 2, 0, 36864, launch_tsm_atb_bf16_2x16_t2x4_r512_p2},
-  {"tsm_atb_bf16_2x32_t2x4_r256_p2", 2, 2, 32, 2, 4, 256, 2, 0, 34816, launch_tsm_atb_bf16_2x32_t2x4_r256_p2},
-  {"tsm_atb_bf16_2x64_t2x4_r128_p2", 2, 2, 64, 2, 4, 128, 2, 0, 33792, launch_tsm_atb_bf16_2x64_t2x4_r128_p2},
-  {"tsm_atb_bf16_4x1_t4x1_r2048_p2", 2, 4, 1, 4, 1, 2048, 2, 0, 40960, launch_tsm_atb_bf16_4x1_t4x1_r2048_p2},
-  {"tsm_atb_bf16_4x2_t4x2_r2048_p2", 2, 4, 2, 4, 2, 2048, 2, 0, 49152, launch_tsm_atb_bf16_4x2_t4x2_r2048_p2},
-  {"tsm_atb_bf16_4x4_t4x4_r2048_p2", 2, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_atb_bf16_4x4_t4x4_r2048_p2},
-  {"tsm_atb_bf16_4x8_t4x4_r1024_p2", 2, 4, 8, 4, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_4x8_t4x4_r1024_p2},
-  {"tsm_atb_bf16_4x16_t4x4_r512_p2", 2, 4, 16, 4, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_4x16_t4x4_r512_p2},
-  {"tsm_atb_bf16_4x32_t4x4_r256_p2", 2, 4, 32, 4, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_4x32_t4x4_r256_p2},
-  {"tsm_atb_bf16_4x64_t4x4_r128_p2", 2, 4, 64, 4, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_4x64_t4x4_r128_p2},
-  {"tsm_atb_bf16_8x1_t8x1_r1024_p2", 2, 8, 1, 8, 1, 1024, 2, 0, 36864, launch_tsm_atb_bf16_8x1_t8x1_r1024_p2},
-  {"tsm_atb_bf16_8x2_t8x2_r1024_p2", 2, 8, 2, 8, 2, 1024, 2, 0, 40960, launch_tsm_atb_bf16_8x2_t8x2_r1024_p2},
-  {"tsm_atb_bf16_8x4_t8x4_r1024_p2", 2, 8, 4, 8, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_8x4_t8x4_r1024_p2},
-  {"tsm_atb_bf16_8x8_t4x4_r512_p1", 2, 8, 8, 4, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p1},
-  {"tsm_atb_bf16_8x8_t4x4_r512_p2", 2, 8, 8, 4, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p2},
-  {"tsm_atb_bf16_8x8_t4x4_r1024_p1", 2, 8, 8, 4, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p1},
-  {"tsm_atb_bf16_8x8_t4x4_r1024_p2", 2, 8, 8, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p2},
-  {"tsm_atb_bf16_8x8_t8x4_r512_p1", 2, 8, 8, 8, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1},
-  {"tsm_atb_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1_nt},
-  {"tsm_atb_bf16_8x8_t8x4_r512_p2", 2, 8, 8, 8, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2},
-  {"tsm_atb_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2_nt},
-  {"tsm_atb_bf16_8x8_t8x4_r1024_p1", 2, 8, 8, 8, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1},
-  {"tsm_atb_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1_nt},
-  {"tsm_atb_bf16_8x8_t8x4_r1024_p2", 2, 8, 8, 8, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2},
-  {"tsm_atb_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2_nt},
-  {"tsm_atb_bf16_8x8_t8x4_r2048_p1", 2, 8, 8, 8, 4, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1},
-  {"tsm_atb_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1_nt},
-  {"tsm_atb_bf16_8x8_t8x4_r2048_p2", 2, 8, 8, 8, 4, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2},
-  {"tsm_atb_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2_nt},
-  {"tsm_atb_bf16_8x8_t8x8_r512_p1", 2, 8, 8, 8, 8, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p1},
-  {"tsm_atb_bf16_8x8_t8x8_r512_p2", 2, 8, 8, 8, 8, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p2},
-  {"tsm_atb_bf16_8x8_t8x8_r1024_p1", 2, 8, 8, 8, 8, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p1},
-  {"tsm_atb_bf16_8x8_t8x8_r1024_p2", 2, 8, 8, 8, 8, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p2},
-  {"tsm_atb_bf16_8x8_t8x8_r2048_p1", 2, 8, 8, 8, 8, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p1},
-  {"tsm_atb_bf16_8x8_t8x8_r2048_p2", 2, 8, 8, 8, 8, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p2},
-  {"tsm_atb_bf16_8x16_t8x4_r512_p2", 2, 8, 16, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_8x16_t8x4_r512_p2},
-  {"tsm_atb_bf16_8x32_t8x4_r256_p2", 2, 8, 32, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_8x32_t8x4_r256_p2},
-  {"tsm_atb_bf16_8x64_t8x4_r128_p2", 2, 8, 64, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2},
-  {"tsm_atb_bf16_16x1_t8x1_r512_p2", 2, 16, 1, 8, 1, 512, 2, 0, 34816, launch_tsm_atb_bf16_16x1_t8x1_r512_p2},
-  {"tsm_atb_bf16_16x2_t8x2_r512_p2", 2, 16, 2, 8, 2, 512, 2, 0, 36864, launch_tsm_atb_bf16_16x2_t8x2_r512_p2},
-  {"tsm_atb_bf16_16x4_t8x4_r512_p2", 2, 16, 4, 8, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_16x4_t8x4_r512_p2},
-  {"tsm_atb_bf16_16x8_t8x4_r512_p2", 2, 16, 8, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_16x8_t8x4_r512_p2},
-  {"tsm_atb_bf16_16x16_mfma_r256_p1", 2, 16, 16, 0, 0, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1},
-  {"tsm_atb_bf16_16x16_mfma_r256_p1_nt", 2, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1_nt},
-  {"tsm_atb_bf16_16x16_mfma_r256_p2", 2, 16, 16, 0, 0, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p2},
-  {"tsm_atb_bf16_16x16_mfma_r256_p2_nt", 2, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p2_nt},
-  {"tsm_atb_bf16_16x16_mfma_r512_p1", 2, 16, 16, 0, 0, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p1},
-  {"tsm_atb_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p1_nt},
-  {"tsm_atb_bf16_16x16_mfma_r512_p2", 2, 16, 16, 0, 0, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p2},
-  {"tsm_atb_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p2_nt},
-  {"tsm_atb_bf16_16x16_mfma_r1024_p1", 2, 16, 16, 0, 0, 1024, 1, 0, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p1},
-  {"tsm_atb_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p1_nt},
-  {"tsm_atb_bf16_16x16_mfma_r1024_p2", 2, 16, 16, 0, 0, 1024, 2, 0, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p2},
-  {"tsm_atb_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p2_nt},
-  {"tsm_atb_bf16_16x16_t4x4_r128_p1", 2, 16, 16, 4, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p1},
-  {"tsm_atb_bf16_16x16_t4x4_r128_p2", 2, 16, 16, 4, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p2},
-  {"tsm_atb_bf16_16x16_t4x4_r256_p1", 2, 16, 16, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p1},
-  {"tsm_atb_bf16_16x16_t4x4_r256_p2", 2, 16, 16, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p2},
-  {"tsm_atb_bf16_16x16_t8x4_r128_p1", 2, 16, 16, 8, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1},
-  {"tsm_atb_bf16_16x16_t8x4_r128_p1_nt", 2, 16, 16, 8, 4, 128, 1, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1_nt},
-  {"tsm_atb_bf16_16x16_t8x4_r128_p2", 2, 16, 16, 8, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2},
-  {"tsm_atb_bf16_16x16_t8x4_r128_p2_nt", 2, 16, 16, 8, 4, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2_nt},
-  {"tsm_atb_bf16_16x16_t8x4_r256_p1", 2, 16, 16, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1},
-  {"tsm_atb_bf16_16x16_t8x4_r256_p1_nt", 2, 16, 16, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1_nt},
-  {"tsm_atb_bf16_16x16_t8x4_r256_p2", 2, 16, 16, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2},
-  {"tsm_atb_bf16_16x16_t8x4_r256_p2_nt", 2, 16, 16, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2_nt},
-  {"tsm_atb_bf16_16x16_t8x4_r512_p1", 2, 16, 16, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1},
-  {"tsm_atb_bf16_16x16_t8x4_r512_p1_nt", 2, 16, 16, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1_nt},
-  {"tsm_atb_bf16_16x16_t8x4_r512_p2", 2, 16, 16, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2},
-  {"tsm_atb_bf16_16x16_t8x4_r512_p2_nt", 2, 16, 16, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2_nt},
-  {"tsm_atb_bf16_16x16_t8x8_r128_p1", 2, 16, 16, 8, 8, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p1},
-  {"tsm_atb_bf16_16x16_t8x8_r128_p2", 2, 16, 16, 8, 8, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p2},
-  {"tsm_atb_bf16_16x16_t8x8_r256_p1", 2, 16, 16, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p1},
-  {"tsm_atb_bf16_16x16_t8x8_r256_p2", 2, 16, 16, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p2},
-  {"tsm_atb_bf16_16x16_t8x8_r512_p1", 2, 16, 16, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p1},
-  {"tsm_atb_bf16_16x16_t8x8_r512_p2", 2, 16, 16, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p2},
-  {"tsm_atb_bf16_16x32_mfma_r256_p2", 2, 16, 32, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_16x32_mfma_r256_p2},
-  {"tsm_atb_bf16_16x64_mfma_r128_p2", 2, 16, 64, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_16x64_mfma_r128_p2},
-  {"tsm_atb_bf16_32x1_t8x1_r256_p2", 2, 32, 1, 8, 1, 256, 2, 0, 33792, launch_tsm_atb_bf16_32x1_t8x1_r256_p2},
-  {"tsm_atb_bf16_32x2_t8x2_r256_p2", 2, 32, 2, 8, 2, 256, 2, 0, 34816, launch_tsm_atb_bf16_32x2_t8x2_r256_p2},
-  {"tsm_atb_bf16_32x4_t8x4_r256_p2", 2, 32, 4, 8, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_32x4_t8x4_r256_p2},
-  {"tsm_atb_bf16_32x8_t8x4_r256_p2", 2, 32, 8, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_32x8_t8x4_r256_p2},
-  {"tsm_atb_bf16_32x16_mfma_r256_p2", 2, 32, 16, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_32x16_mfma_r256_p2},
-  {"tsm_atb_bf16_32x32_mfma_r128_p1", 2, 32, 32, 0, 0, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1},
-  {"tsm_atb_bf16_32x32_mfma_r128_p1_nt", 2, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1_nt},
-  {"tsm_atb_bf16_32x32_mfma_r128_p2", 2, 32, 32, 0, 0, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p2},
-  {"tsm_atb_bf16_32x32_mfma_r128_p2_nt", 2, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p2_nt},
-  {"tsm_atb_bf16_32x32_mfma_r256_p1", 2, 32, 32, 0, 0, 256, 1, 0, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p1},
-  {"tsm_atb_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p1_nt},
-  {"tsm_atb_bf16_32x32_mfma_r256_p2", 2, 32, 32, 0, 0, 256, 2, 0, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p2},
-  {"tsm_atb_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p2_nt},
-  {"tsm_atb_bf16_32x32_mfma_r512_p1", 2, 32, 32, 0, 0, 512, 1, 0, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p1},
-  {"tsm_atb_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p1_nt},
-  {"tsm_atb_bf16_32x32_mfma_r512_p2", 2, 32, 32, 0, 0, 512, 2, 0, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p2},
-  {"tsm_atb_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p2_nt},
-  {"tsm_atb_bf16_32x32_t4x4_r32_p1", 2, 32, 32, 4, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p1},
-  {"tsm_atb_bf16_32x32_t4x4_r32_p2", 2, 32, 32, 4, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p2},
-  {"tsm_atb_bf16_32x32_t4x4_r64_p1", 2, 32, 32, 4, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p1},
-  {"tsm_atb_bf16_32x32_t4x4_r64_p2", 2, 32, 32, 4, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p2},
-  {"tsm_atb_bf16_32x32_t8x4_r32_p1", 2, 32, 32, 8, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1},
-  {"tsm_atb_bf16_32x32_t8x4_r32_p1_nt", 2, 32, 32, 8, 4, 32, 1, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1_nt},
-  {"tsm_atb_bf16_32x32_t8x4_r32_p2", 2, 32, 32, 8, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2},
-  {"tsm_atb_bf16_32x32_t8x4_r32_p2_nt", 2, 32, 32, 8, 4, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2_nt},
-  {"tsm_atb_bf16_32x32_t8x4_r64_p1", 2, 32, 32, 8, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1},
-  {"tsm_atb_bf16_32x32_t8x4_r64_p1_nt", 2, 32, 32, 8, 4, 64, 1, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1_nt},
-  {"tsm_atb_bf16_32x32_t8x4_r64_p2", 2, 32, 32, 8, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2},
-  {"tsm_atb_bf16_32x32_t8x4_r64_p2_nt", 2, 32, 32, 8, 4, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2_nt},
-  {"tsm_atb_bf16_32x32_t8x4_r128_p1", 2, 32, 32, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1},
-  {"tsm_atb_bf16_32x32_t8x4_r128_p1_nt", 2, 32, 32, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1_nt},
-  {"tsm_atb_bf16_32x32_t8x4_r128_p2", 2, 32, 32, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2},
-  {"tsm_atb_bf16_32x32_t8x4_r128_p2_nt", 2, 32, 32, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2_nt},
-  {"tsm_atb_bf16_32x32_t8x8_r32_p1", 2, 32, 32, 8, 8, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p1},
-  {"tsm_atb_bf16_32x32_t8x8_r32_p2", 2, 32, 32, 8, 8, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p2},
-  {"tsm_atb_bf16_32x32_t8x8_r64_p1", 2, 32, 32, 8, 8, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p1},
-  {"tsm_atb_bf16_32x32_t8x8_r64_p2", 2, 32, 32, 8, 8, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p2},
-  {"tsm_atb_bf16_32x32_t8x8_r128_p1", 2, 32, 32, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p1},
-  {"tsm_atb_bf16_32x32_t8x8_r128_p2", 2, 32, 32, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p2},
-  {"tsm_atb_bf16_32x64_mfma_r128_p2", 2, 32, 64, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2},
-  {"tsm_atb_bf16_64x1_t8x1_r128_p2", 2, 64, 1, 8, 1, 128, 2, 0, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2},
-  {"tsm_atb_bf16_64x2_t8x2_r128_p2", 2, 64, 2, 8, 2, 128, 2, 0, 33792, launch_tsm_atb_bf16_64x2_t8x2_r128_p2},
-  {"tsm_atb_bf16_64x4_t8x4_r128_p2", 2, 64, 4, 8, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_64x4_t8x4_r128_p2},
-  {"tsm_atb_bf16_64x8_t8x4_r128_p2", 2, 64, 8, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_64x8_t8x4_r128_p2},
-  {"tsm_atb_bf16_64x16_mfma_r128_p2", 2, 64, 16, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_64x16_mfma_r128_p2},
-  {"tsm_atb_bf16_64x32_mfma_r128_p2", 2, 64, 32, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_64x32_mfma_r128_p2},
-  {"tsm_atb_bf16_64x64_mfma_r128_p1", 2, 64, 64, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1},
-  {"tsm_atb_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1_nt},
-  {"tsm_atb_bf16_64x64_mfma_r128_p2", 2, 64, 64, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p2},
-  {"tsm_atb_bf16_64x64_mfma_r128_p2_nt", 2, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p2_nt},
-  {"tsm_atb_bf16_64x64_mfma_r256_p1", 2, 64, 64, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p1},
-  {"tsm_atb_bf16_64x64_mfma_r256_p1_nt", 2, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p1_nt},
-  {"tsm_atb_bf16_64x64_mfma_r256_p2", 2, 64, 64, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p2},
-  {"tsm_atb_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p2_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r16_p1", 2, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p1},
-  {"tsm_atb_bf16_64x64_t8x8_r16_p1_nt", 2, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p1_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r16_p2", 2, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p2},
-  {"tsm_atb_bf16_64x64_t8x8_r16_p2_nt", 2, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p2_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r32_p1", 2, 64, 64, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p1},
-  {"tsm_atb_bf16_64x64_t8x8_r32_p1_nt", 2, 64, 64, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p1_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r32_p2", 2, 64, 64, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p2},
-  {"tsm_atb_bf16_64x64_t8x8_r32_p2_nt", 2, 64, 64, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p2_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r64_p1", 2, 64, 64, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p1},
-  {"tsm_atb_bf16_64x64_t8x8_r64_p1_nt", 2, 64, 64, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p1_nt},
-  {"tsm_atb_bf16_64x64_t8x8_r64_p2", 2, 64, 64, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2},
-  {"tsm_atb_bf16_64x64_t8x8_r64_p2_nt", 2, 64, 64, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2_nt},
-  {"tsm_atb_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 65536, launch_tsm_atb_f32_1x1_t1x1_r4096_p2},
-  {"tsm_atb_f32_1x2_t1x2_r2048_p2", 1, 1, 2, 1, 2, 2048, 2, 0, 49152, launch_tsm_atb_f32_1x2_t1x2_r2048_p2},
-  {"tsm_atb_f32_1x4_t1x4_r1024_p2", 1, 1, 4, 1, 4, 1024, 2, 0, 40960, launch_tsm_atb_f32_1x4_t1x4_r1024_p2},
-  {"tsm_atb_f32_1x8_t1x4_r512_p2", 1, 1, 8, 1, 4, 512, 2, 0, 36864, launch_tsm_atb_f32_1x8_t1x4_r512_p2},
-  {"tsm_atb_f32_1x16_t1x4_r256_p2", 1, 1, 16, 1, 4, 256, 2, 0, 34816, launch_tsm_atb_f32_1x16_t1x4_r256_p2},
-  {"tsm_atb_f32_1x32_t1x4_r128_p2", 1, 1, 32, 1, 4, 128, 2, 0, 33792, launch_tsm_atb_f32_1x32_t1x4_r128_p2},
-  {"tsm_atb_f32_1x64_t1x4_r64_p2", 1, 1, 64, 1, 4, 64, 2, 0, 33280, launch_tsm_atb_f32_1x64_t1x4_r64_p2},
-  {"tsm_atb_f32_2x1_t2x1_r2048_p2", 1, 2, 1, 2, 1, 2048, 2, 0, 49152, launch_tsm_atb_f32_2x1_t2x1_r2048_p2},
-  {"tsm_atb_f32_2x2_t2x2_r2048_p2", 1, 2, 2, 2, 2, 2048, 2, 0, 65536, launch_tsm_atb_f32_2x2_t2x2_r2048_p2},
-  {"tsm_atb_f32_2x4_t2x4_r1024_p2", 1, 2, 4, 2, 4, 1024, 2, 0, 49152, launch_tsm_atb_f32_2x4_t2x4_r1024_p2},
-  {"tsm_atb_f32_2x8_t2x4_r512_p2", 1, 2, 8, 2, 4, 512, 2, 0, 40960, launch_tsm_atb_f32_2x8_t2x4_r512_p2},
-  {"tsm_atb_f32_2x16_t2x4_r256_p2", 1, 2, 16, 2, 4, 256, 2, 0, 36864, launch_tsm_atb_f32_2x16_t2x4_r256_p2},
-  {"tsm_atb_f32_2x32_t2x4_r128_p2", 1, 2, 32, 2, 4, 128, 2, 0, 34816, launch_tsm_atb_f32_2x32_t2x4_r128_p2},
-  {"tsm_atb_f32_2x64_t2x4_r64_p2", 1, 2, 64, 2, 4, 64, 2, 0, 33792, launch_tsm_atb_f32_2x64_t2x4_r64_p2},
-  {"tsm_atb_f32_4x1_t4x1_r1024_p2", 1, 4, 1, 4, 1, 1024, 2, 0, 40960, launch_tsm_atb_f32_4x1_t4x1_r1024_p2},
-  {"tsm_atb_f32_4x2_t4x2_r1024_p2", 1, 4, 2, 4, 2, 1024, 2, 0, 49152, launch_tsm_atb_f32_4x2_t4x2_r1024_p2},
-  {"tsm_atb_f32_4x4_t4x4_r1024_p2", 1, 4, 4, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_f32_4x4_t4x4_r1024_p2},
-  {"tsm_atb_f32_4x8_t4x4_r512_p2", 1, 4, 8, 4, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_4x8_t4x4_r512_p2},
-  {"tsm_atb_f32_4x16_t4x4_r256_p2", 1, 4, 16, 4, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_4x16_t4x4_r256_p2},
-  {"tsm_atb_f32_4x32_t4x4_r128_p2", 1, 4, 32, 4, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_4x32_t4x4_r128_p2},
-  {"tsm_atb_f32_4x64_t4x4_r64_p2", 1, 4, 64, 4, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_4x64_t4x4_r64_p2},
-  {"tsm_atb_f32_8x1_t8x1_r512_p2", 1, 8, 1, 8, 1, 512, 2, 0, 36864, launch_tsm_atb_f32_8x1_t8x1_r512_p2},
-  {"tsm_atb_f32_8x2_t8x2_r512_p2", 1, 8, 2, 8, 2, 512, 2, 0, 40960, launch_tsm_atb_f32_8x2_t8x2_r512_p2},
-  {"tsm_atb_f32_8x4_t8x4_r512_p2", 1, 8, 4, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_8x4_t8x4_r512_p2},
-  {"tsm_atb_f32_8x8_t4x4_r256_p1", 1, 8, 8, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p1},
-  {"tsm_atb_f32_8x8_t4x4_r256_p2", 1, 8, 8, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p2},
-  {"tsm_atb_f32_8x8_t4x4_r512_p1", 1, 8, 8, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p1},
-  {"tsm_atb_f32_8x8_t4x4_r512_p2", 1, 8, 8, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p2},
-  {"tsm_atb_f32_8x8_t4x4_r1024_p1", 1, 8, 8, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p1},
-  {"tsm_atb_f32_8x8_t4x4_r1024_p2", 1, 8, 8, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p2},
-  {"tsm_atb_f32_8x8_t8x4_r256_p1", 1, 8, 8, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1},
-  {"tsm_atb_f32_8x8_t8x4_r256_p1_nt", 1, 8, 8, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1_nt},
-  {"tsm_atb_f32_8x8_t8x4_r256_p2", 1, 8, 8, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2},
-  {"tsm_atb_f32_8x8_t8x4_r256_p2_nt", 1, 8, 8, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2_nt},
-  {"tsm_atb_f32_8x8_t8x4_r512_p1", 1, 8, 8, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1},
-  {"tsm_atb_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1_nt},
-  {"tsm_atb_f32_8x8_t8x4_r512_p2", 1, 8, 8, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2},
-  {"tsm_atb_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2_nt},
-  {"tsm_atb_f32_8x8_t8x4_r1024_p1", 1, 8, 8, 8, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1},
-  {"tsm_atb_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1_nt},
-  {"tsm_atb_f32_8x8_t8x4_r1024_p2", 1, 8, 8, 8, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2},
-  {"tsm_atb_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2_nt},
-  {"tsm_atb_f32_8x8_t8x8_r256_p1", 1, 8, 8, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p1},
-  {"tsm_atb_f32_8x8_t8x8_r256_p2", 1, 8, 8, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p2},
-  {"tsm_atb_f32_8x8_t8x8_r512_p1", 1, 8, 8, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p1},
-  {"tsm_atb_f32_8x8_t8x8_r512_p2", 1, 8, 8, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p2},
-  {"tsm_atb_f32_8x8_t8x8_r1024_p1", 1, 8, 8, 8, 8, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p1},
-  {"tsm_atb_f32_8x8_t8x8_r1024_p2", 1, 8, 8, 8, 8, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p2},
-  {"tsm_atb_f32_8x16_t8x4_r256_p2", 1, 8, 16, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_8x16_t8x4_r256_p2},
-  {"tsm_atb_f32_8x32_t8x4_r128_p2", 1, 8, 32, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_8x32_t8x4_r128_p2},
-  {"tsm_atb_f32_8x64_t8x4_r64_p2", 1, 8, 64, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2},
-  {"tsm_atb_f32_16x1_t8x1_r256_p2", 1, 16, 1, 8, 1, 256, 2, 0, 34816, launch_tsm_atb_f32_16x1_t8x1_r256_p2},
-  {"tsm_atb_f32_16x2_t8x2_r256_p2", 1, 16, 2, 8, 2, 256, 2, 0, 36864, launch_tsm_atb_f32_16x2_t8x2_r256_p2},
-  {"tsm_atb_f32_16x4_t8x4_r256_p2", 1, 16, 4, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_16x4_t8x4_r256_p2},
-  {"tsm_atb_f32_16x8_t8x4_r256_p2", 1, 16, 8, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_16x8_t8x4_r256_p2},
-  {"tsm_atb_f32_16x16_mfma_r128_p1", 1, 16, 16, 0, 0, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p1},
-  {"tsm_atb_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p1_nt},
-  {"tsm_atb_f32_16x16_mfma_r128_p2", 1, 16, 16, 0, 0, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p2},
-  {"tsm_atb_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p2_nt},
-  {"tsm_atb_f32_16x16_mfma_r256_p1", 1, 16, 16, 0, 0, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p1},
-  {"tsm_atb_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p1_nt},
-  {"tsm_atb_f32_16x16_mfma_r256_p2", 1, 16, 16, 0, 0, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p2},
-  {"tsm_atb_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p2_nt},
-  {"tsm_atb_f32_16x16_t4x4_r128_p1", 1, 16, 16, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p1},
-  {"tsm_atb_f32_16x16_t4x4_r128_p2", 1, 16, 16, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p2},
-  {"tsm_atb_f32_16x16_t4x4_r256_p1", 1, 16, 16, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p1},
-  {"tsm_atb_f32_16x16_t4x4_r256_p2", 1, 16, 16, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p2},
-  {"tsm_atb_f32_16x16_t8x4_r128_p1", 1, 16, 16, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1},
-  {"tsm_atb_f32_16x16_t8x4_r128_p1_nt", 1, 16, 16, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1_nt},
-  {"tsm_atb_f32_16x16_t8x4_r128_p2", 1, 16, 16, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2},
-  {"tsm_atb_f32_16x16_t8x4_r128_p2_nt", 1, 16, 16, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2_nt},
-  {"tsm_atb_f32_16x16_t8x4_r256_p1", 1, 16, 16, 8, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1},
-  {"tsm_atb_f32_16x16_t8x4_r256_p1_nt", 1, 16, 16, 8, 4, 256, 1, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1_nt},
-  {"tsm_atb_f32_16x16_t8x4_r256_p2", 1, 16, 16, 8, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2},
-  {"tsm_atb_f32_16x16_t8x4_r256_p2_nt", 1, 16, 16, 8, 4, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2_nt},
-  {"tsm_atb_f32_16x16_t8x4_r512_p1", 1, 16, 16, 8, 4, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1},
-  {"tsm_atb_f32_16x16_t8x4_r512_p1_nt", 1, 16, 16, 8, 4, 512, 1, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1_nt},
-  {"tsm_atb_f32_16x16_t8x4_r512_p2", 1, 16, 16, 8, 4, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2},
-  {"tsm_atb_f32_16x16_t8x4_r512_p2_nt", 1, 16, 16, 8, 4, 512, 2, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2_nt},
-  {"tsm_atb_f32_16x16_t8x8_r128_p1", 1, 16, 16, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p1},
-  {"tsm_atb_f32_16x16_t8x8_r128_p2", 1, 16, 16, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p2},
-  {"tsm_atb_f32_16x16_t8x8_r256_p1", 1, 16, 16, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p1},
-  {"tsm_atb_f32_16x16_t8x8_r256_p2", 1, 16, 16, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p2},
-  {"tsm_atb_f32_16x16_t8x8_r512_p1", 1, 16, 16, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p1},
-  {"tsm_atb_f32_16x16_t8x8_r512_p2", 1, 16, 16, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p2},
-  {"tsm_atb_f32_16x32_t8x4_r128_p2", 1, 16, 32, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_16x32_t8x4_r128_p2},
-  {"tsm_atb_f32_16x64_mfma_r64_p2", 1, 16, 64, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_16x64_mfma_r64_p2},
-  {"tsm_atb_f32_32x1_t8x1_r128_p2", 1, 32, 1, 8, 1, 128, 2, 0, 33792, launch_tsm_atb_f32_32x1_t8x1_r128_p2},
-  {"tsm_atb_f32_32x2_t8x2_r128_p2", 1, 32, 2, 8, 2, 128, 2, 0, 34816, launch_tsm_atb_f32_32x2_t8x2_r128_p2},
-  {"tsm_atb_f32_32x4_t8x4_r128_p2", 1, 32, 4, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_32x4_t8x4_r128_p2},
-  {"tsm_atb_f32_32x8_t8x4_r128_p2", 1, 32, 8, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_32x8_t8x4_r128_p2},
-  {"tsm_atb_f32_32x16_t8x4_r128_p2", 1, 32, 16, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_32x16_t8x4_r128_p2},
-  {"tsm_atb_f32_32x32_mfma_r64_p1", 1, 32, 32, 0, 0, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p1},
-  {"tsm_atb_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p1_nt},
-  {"tsm_atb_f32_32x32_mfma_r64_p2", 1, 32, 32, 0, 0, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p2},
-  {"tsm_atb_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p2_nt},
-  {"tsm_atb_f32_32x32_mfma_r128_p1", 1, 32, 32, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p1},
-  {"tsm_atb_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p1_nt},
-  {"tsm_atb_f32_32x32_mfma_r128_p2", 1, 32, 32, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p2},
-  {"tsm_atb_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p2_nt},
-  {"tsm_atb_f32_32x32_mfma_r256_p1", 1, 32, 32, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p1},
-  {"tsm_atb_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p1_nt},
-  {"tsm_atb_f32_32x32_mfma_r256_p2", 1, 32, 32, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p2},
-  {"tsm_atb_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p2_nt},
-  {"tsm_atb_f32_32x32_t4x4_r32_p1", 1, 32, 32, 4, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p1},
-  {"tsm_atb_f32_32x32_t4x4_r32_p2", 1, 32, 32, 4, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p2},
-  {"tsm_atb_f32_32x32_t4x4_r64_p1", 1, 32, 32, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p1},
-  {"tsm_atb_f32_32x32_t4x4_r64_p2", 1, 32, 32, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p2},
-  {"tsm_atb_f32_32x32_t8x4_r32_p1", 1, 32, 32, 8, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1},
-  {"tsm_atb_f32_32x32_t8x4_r32_p1_nt", 1, 32, 32, 8, 4, 32, 1, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1_nt},
-  {"tsm_atb_f32_32x32_t8x4_r32_p2", 1, 32, 32, 8, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2},
-  {"tsm_atb_f32_32x32_t8x4_r32_p2_nt", 1, 32, 32, 8, 4, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2_nt},
-  {"tsm_atb_f32_32x32_t8x4_r64_p1", 1, 32, 32, 8, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1},
-  {"tsm_atb_f32_32x32_t8x4_r64_p1_nt", 1, 32, 32, 8, 4, 64, 1, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1_nt},
-  {"tsm_atb_f32_32x32_t8x4_r64_p2", 1, 32, 32, 8, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2},
-  {"tsm_atb_f32_32x32_t8x4_r64_p2_nt", 1, 32, 32, 8, 4, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2_nt},
-  {"tsm_atb_f32_32x32_t8x4_r128_p1", 1, 32, 32, 8, 4, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1},
-  {"tsm_atb_f32_32x32_t8x4_r128_p1_nt", 1, 32, 32, 8, 4, 128, 1, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1_nt},
-  {"tsm_atb_f32_32x32_t8x4_r128_p2", 1, 32, 32, 8, 4, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2},
-  {"tsm_atb_f32_32x32_t8x4_r128_p2_nt", 1, 32, 32, 8, 4, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2_nt},
-  {"tsm_atb_f32_32x32_t8x8_r32_p1", 1, 32, 32, 8, 8, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p1},
-  {"tsm_atb_f32_32x32_t8x8_r32_p2", 1, 32, 32, 8, 8, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p2},
-  {"tsm_atb_f32_32x32_t8x8_r64_p1", 1, 32, 32, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p1},
-  {"tsm_atb_f32_32x32_t8x8_r64_p2", 1, 32, 32, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p2},
-  {"tsm_atb_f32_32x32_t8x8_r128_p1", 1, 32, 32, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p1},
-  {"tsm_atb_f32_32x32_t8x8_r128_p2", 1, 32, 32, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p2},
-  {"tsm_atb_f32_32x64_mfma_r64_p2", 1, 32, 64, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2},
-  {"tsm_atb_f32_64x1_t8x1_r64_p2", 1, 64, 1, 8, 1, 64, 2, 0, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2},
-  {"tsm_atb_f32_64x2_t8x2_r64_p2", 1, 64, 2, 8, 2, 64, 2, 0, 33792, launch_tsm_atb_f32_64x2_t8x2_r64_p2},
-  {"tsm_atb_f32_64x4_t8x4_r64_p2", 1, 64, 4, 8, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_64x4_t8x4_r64_p2},
-  {"tsm_atb_f32_64x8_t8x4_r64_p2", 1, 64, 8, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_64x8_t8x4_r64_p2},
-  {"tsm_atb_f32_64x16_mfma_r64_p2", 1, 64, 16, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_64x16_mfma_r64_p2},
-  {"tsm_atb_f32_64x32_mfma_r64_p2", 1, 64, 32, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_64x32_mfma_r64_p2},
-  {"tsm_atb_f32_64x64_mfma_r32_p1", 1, 64, 64, 0, 0, 32, 1, 0, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p1},
-  {"tsm_atb_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p1_nt},
-  {"tsm_atb_f32_64x64_mfma_r32_p2", 1, 64, 64, 0, 0, 32, 2, 0, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p2},
-  {"tsm_atb_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p2_nt},
-  {"tsm_atb_f32_64x64_mfma_r64_p1", 1, 64, 64, 0, 0, 64, 1, 0, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p1},
-  {"tsm_atb_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p1_nt},
-  {"tsm_atb_f32_64x64_mfma_r64_p2", 1, 64, 64, 0, 0, 64, 2, 0, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p2},
-  {"tsm_atb_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p2_nt},
-  {"tsm_atb_f32_64x64_mfma_r128_p1", 1, 64, 64, 0, 0, 128, 1, 0, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p1},
-  {"tsm_atb_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p1_nt},
-  {"tsm_atb_f32_64x64_mfma_r128_p2", 1, 64, 64, 0, 0, 128, 2, 0, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p2},
-  {"tsm_atb_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p2_nt},
-  {"tsm_atb_f32_64x64_t8x8_r16_p1", 1, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p1},
-  {"tsm_atb_f32_64x64_t8x8_r16_p1_nt", 1, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p1_nt},
-  {"tsm_atb_f32_64x64_t8x8_r16_p2", 1, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p2},
-  {"tsm_atb_f32_64x64_t8x8_r16_p2_nt", 1, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p2_nt},
-  {"tsm_atb_f32_64x64_t8x8_r32_p1", 1, 64, 64, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p1},
-  {"tsm_atb_f32_64x64_t8x8_r32_p1_nt", 1, 64, 64, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p1_nt},
-  {"tsm_atb_f32_64x64_t8x8_r32_p2", 1, 64, 64, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p2},
-  {"tsm_atb_f32_64x64_t8x8_r32_p2_nt", 1, 64, 64, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p2_nt},
-  {"tsm_atb_f32_64x64_t8x8_r64_p1", 1, 64, 64, 8, 8, 64, 1, 0, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p1},
-  {"tsm_atb_f32_64x64_t8x8_r64_p1_nt", 1, 64, 64, 8, 8, 64, 1, 1, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p1_nt},
-  {"tsm_atb_f32_64x64_t8x8_r64_p2", 1, 64, 64, 8, 8, 64, 2, 0, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p2},
-  {"tsm_atb_f32_64x64_t8x8_r64_p2_nt", 1, 64, 64, 8, 8, 64, 2, 1, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p2_nt},
-  {"tsm_atb_f64_1x1_t1x1_r1024_p1", 0, 1, 1, 1, 1, 1024, 1, 0, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p1},
-  {"tsm_atb_f64_1x1_t1x1_r1024_p1_nt", 0, 1, 1, 1, 1, 1024, 1, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p1_nt},
-  {"tsm_atb_f64_1x1_t1x1_r1024_p2", 0, 1, 1, 1, 1, 1024, 2, 0, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2},
-  {"tsm_atb_f64_1x1_t1x1_r1024_p2_nt", 0, 1, 1, 1, 1, 1024, 2, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2_nt},
-  {"tsm_atb_f64_1x1_t1x1_r2048_p1", 0, 1, 1, 1, 1, 2048, 1, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1},
-  {"tsm_atb_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1_nt},
-  {"tsm_atb_f64_1x1_t1x1_r2048_p2", 0, 1, 1, 1, 1, 2048, 2, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2},
-  {"tsm_atb_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2_nt},
-  {"tsm_atb_f64_1x1_t1x1_r4096_p1", 0, 1, 1, 1, 1, 4096, 1, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1},
-  {"tsm_atb_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt},
-  {"tsm_atb_f64_1x1_t1x1_r4096_p2", 0, 1, 1, 1, 1, 4096, 2, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2},
-  {"tsm_atb_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2_nt},
-  {"tsm_atb_f64_1x2_t1x2_r1024_p2", 0, 1, 2, 1, 2, 1024, 2, 0, 49152, launch_tsm_atb_f64_1x2_t1x2_r1024_p2},
-  {"tsm_atb_f64_1x4_t1x4_r512_p2", 0, 1, 4, 1, 4, 512, 2, 0, 40960, launch_tsm_atb_f64_1x4_t1x4_r512_p2},
-  {"tsm_atb_f64_1x8_t1x8_r256_p2", 0, 1, 8, 1, 8, 256, 2, 0, 36864, launch_tsm_atb_f64_1x8_t1x8_r256_p2},
-  {"tsm_atb_f64_1x16_t1x8_r128_p2", 0, 1, 16, 1, 8, 128, 2, 0, 34816, launch_tsm_atb_f64_1x16_t1x8_r128_p2},
-  {"tsm_atb_f64_1x32_t1x8_r64_p2", 0, 1, 32, 1, 8, 64, 2, 0, 33792, launch_tsm_atb_f64_1x32_t1x8_r64_p2},
-  {"tsm_atb_f64_1x64_t1x8_r32_p2", 0, 1, 64, 1, 8, 32, 2, 0, 33280, launch_tsm_atb_f64_1x64_t1x8_r32_p2},
-  {"tsm_atb_f64_2x1_t2x1_r1024_p2", 0, 2, 1, 2, 1, 1024, 2, 0, 49152, launch_tsm_atb_f64_2x1_t2x1_r1024_p2},
-  {"tsm_atb_f64_2x2_t2x2_r512_p1", 0, 2, 2, 2, 2, 512, 1, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1},
-  {"tsm_atb_f64_2x2_t2x2_r512_p1_nt", 0, 2, 2, 2, 2, 512, 1, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt},
-  {"tsm_atb_f64_2x2_t2x2_r512_p2", 0, 2, 2, 2, 2, 512, 2, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2},
-  {"tsm_atb_f64_2x2_t2x2_r512_p2_nt", 0, 2, 2, 2, 2, 512, 2, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2_nt},
-  {"tsm_atb_f64_2x2_t2x2_r1024_p1", 0, 2, 2, 2, 2, 1024, 1, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1},
-  {"tsm_atb_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1_nt},
-  {"tsm_atb_f64_2x2_t2x2_r1024_p2", 0, 2, 2, 2, 2, 1024, 2, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2},
-  {"tsm_atb_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2_nt},
-  {"tsm_atb_f64_2x2_t2x2_r2048_p1", 0, 2, 2, 2, 2, 2048, 1, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1},
-  {"tsm_atb_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt},
-  {"tsm_atb_f64_2x2_t2x2_r2048_p2", 0, 2, 2, 2, 2, 2048, 2, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2},
-  {"tsm_atb_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt},
-  {"tsm_atb_f64_2x4_t2x4_r512_p2", 0, 2, 4, 2, 4, 512, 2, 0, 49152, launch_tsm_atb_f64_2x4_t2x4_r512_p2},
-  {"tsm_atb_f64_2x8_t2x8_r256_p2", 0, 2, 8, 2, 8, 256, 2, 0, 40960, launch_tsm_atb_f64_2x8_t2x8_r256_p2},
-  {"tsm_atb_f64_2x16_t2x8_r128_p2", 0, 2, 16, 2, 8, 128, 2, 0, 36864, launch_tsm_atb_f64_2x16_t2x8_r128_p2},
-  {"tsm_atb_f64_2x32_t2x8_r64_p2", 0, 2, 32, 2, 8, 64, 2, 0, 34816, launch_tsm_atb_f64_2x32_t2x8_r64_p2},
-  {"tsm_atb_f64_2x64_t2x8_r32_p2", 0, 2, 64, 2, 8, 32, 2, 0, 33792, launch_tsm_atb_f64_2x64_t2x8_r32_p2},
-  {"tsm_atb_f64_4x1_t4x1_r512_p2", 0, 4, 1, 4, 1, 512, 2, 0, 40960, launch_tsm_atb_f64_4x1_t4x1_r512_p2},
-  {"tsm_atb_f64_4x2_t4x2_r512_p2", 0, 4, 2, 4, 2, 512, 2, 0, 49152, launch_tsm_atb_f64_4x2_t4x2_r512_p2},
-  {"tsm_atb_f64_4x4_t2x2_r512_p1", 0, 4, 4, 2, 2, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p1},
-  {"tsm_atb_f64_4x4_t2x2_r512_p2", 0, 4, 4, 2, 2, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p2},
-  {"tsm_atb_f64_4x4_t4x4_r256_p1", 0, 4, 4, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1},
-  {"tsm_atb_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1_nt},
-  {"tsm_atb_f64_4x4_t4x4_r256_p2", 0, 4, 4, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2},
-  {"tsm_atb_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2_nt},
-  {"tsm_atb_f64_4x4_t4x4_r512_p1", 0, 4, 4, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1},
-  {"tsm_atb_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1_nt},
-  {"tsm_atb_f64_4x4_t4x4_r512_p2", 0, 4, 4, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2},
-  {"tsm_atb_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2_nt},
-  {"tsm_atb_f64_4x4_t4x4_r1024_p1", 0, 4, 4, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1},
-  {"tsm_atb_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt},
-  {"tsm_atb_f64_4x4_t4x4_r1024_p2", 0, 4, 4, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2},
-  {"tsm_atb_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt},
-  {"tsm_atb_f64_4x8_t4x8_r256_p2", 0, 4, 8, 4, 8, 256, 2, 0, 49152, launch_tsm_atb_f64_4x8_t4x8_r256_p2},
-  {"tsm_atb_f64_4x16_t4x8_r128_p2", 0, 4, 16, 4, 8, 128, 2, 0, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2},
-  {"tsm_atb_f64_4x32_t4x4_r32_p1", 0, 4, 32, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p1},
-  {"tsm_atb_f64_4x32_t4x4_r32_p2", 0, 4, 32, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2},
-  {"tsm_atb_f64_4x32_t4x4_r64_p1", 0, 4, 32, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p1},
-  {"tsm_atb_f64_4x32_t4x4_r64_p2", 0, 4, 32, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p2},
-  {"tsm_atb_f64_4x32_t4x4_r128_p1", 0, 4, 32, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p1},
-  {"tsm_atb_f64_4x32_t4x4_r128_p2", 0, 4, 32, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p2},
-  {"tsm_atb_f64_4x32_t4x8_r64_p1", 0, 4, 32, 4, 8, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1},
-  {"tsm_atb_f64_4x32_t4x8_r64_p1_nt", 0, 4, 32, 4, 8, 64, 1, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1_nt},
-  {"tsm_atb_f64_4x32_t4x8_r64_p2", 0, 4, 32, 4, 8, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2},
-  {"tsm_atb_f64_4x32_t4x8_r64_p2_nt", 0, 4, 32, 4, 8, 64, 2, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2_nt},
-  {"tsm_atb_f64_4x32_t4x8_r128_p1", 0, 4, 32, 4, 8, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1},
-  {"tsm_atb_f64_4x32_t4x8_r128_p1_nt", 0, 4, 32, 4, 8, 128, 1, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt},
-  {"tsm_atb_f64_4x32_t4x8_r128_p2", 0, 4, 32, 4, 8, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2},
-  {"tsm_atb_f64_4x32_t4x8_r128_p2_nt", 0, 4, 32, 4, 8, 128, 2, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt},
-  {"tsm_atb_f64_4x64_t4x8_r32_p2", 0, 4, 64, 4, 8, 32, 2, 0, 34816, launch_tsm_atb_f64_4x64_t4x8_r32_p2},
-  {"tsm_atb_f64_8x1_t8x1_r256_p2", 0, 8, 1, 8, 1, 256, 2, 0, 36864, launch_tsm_atb_f64_8x1_t8x1_r256_p2},
-  {"tsm_atb_f64_8x2_t8x2_r256_p2", 0, 8, 2, 8, 2, 256, 2, 0, 40960, launch_tsm_atb_f64_8x2_t8x2_r256_p2},
-  {"tsm_atb_f64_8x4_t8x4_r256_p2", 0, 8, 4, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f64_8x4_t8x4_r256_p2},
-  {"tsm_atb_f64_8x8_t2x2_r256_p1", 0, 8, 8, 2, 2, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p1},
-  {"tsm_atb_f64_8x8_t2x2_r256_p2", 0, 8, 8, 2, 2, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p2},
-  {"tsm_atb_f64_8x8_t4x4_r128_p1", 0, 8, 8, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p1},
-  {"tsm_atb_f64_8x8_t4x4_r128_p2", 0, 8, 8, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p2},
-  {"tsm_atb_f64_8x8_t4x4_r256_p1", 0, 8, 8, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p1},
-  {"tsm_atb_f64_8x8_t4x4_r256_p2", 0, 8, 8, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p2},
-  {"tsm_atb_f64_8x8_t4x4_r512_p1", 0, 8, 8, 4, 4, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p1},
-  {"tsm_atb_f64_8x8_t4x4_r512_p2", 0, 8, 8, 4, 4, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p2},
-  {"tsm_atb_f64_8x8_t8x8_r256_p1", 0, 8, 8, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1},
-  {"tsm_atb_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1_nt},
-  {"tsm_atb_f64_8x8_t8x8_r256_p2", 0, 8, 8, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p2},
-  {"tsm_atb_f64_8x8_t8x8_r256_p2_nt", 0, 8, 8, 8, 8, 256, 2, 1, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p2_nt},
-  {"tsm_atb_f64_8x8_t8x8_r512_p1", 0, 8, 8, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p1},
-  {"tsm_atb_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p1_nt},
-  {"tsm_atb_f64_8x8_t8x8_r512_p2", 0, 8, 8, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2},
-  {"tsm_atb_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt},
-  {"tsm_atb_f64_8x16_t8x8_r128_p2", 0, 8, 16, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_8x16_t8x8_r128_p2},
-  {"tsm_atb_f64_8x32_t8x8_r64_p2", 0, 8, 32, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_8x32_t8x8_r64_p2},
-  {"tsm_atb_f64_8x64_t8x8_r32_p2", 0, 8, 64, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_8x64_t8x8_r32_p2},
-  {"tsm_atb_f64_16x1_t8x1_r128_p2", 0, 16, 1, 8, 1, 128, 2, 0, 34816, launch_tsm_atb_f64_16x1_t8x1_r128_p2},
-  {"tsm_atb_f64_16x2_t8x2_r128_p2", 0, 16, 2, 8, 2, 128, 2, 0, 36864, launch_tsm_atb_f64_16x2_t8x2_r128_p2},
-  {"tsm_atb_f64_16x4_t8x4_r128_p2", 0, 16, 4, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f64_16x4_t8x4_r128_p2},
-  {"tsm_atb_f64_16x8_t8x8_r128_p2", 0, 16, 8, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_16x8_t8x8_r128_p2},
-  {"tsm_atb_f64_16x16_mfma_r64_p1", 0, 16, 16, 0, 0, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p1},
-  {"tsm_atb_f64_16x16_mfma_r64_p1_nt", 0, 16, 16, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p1_nt},
-  {"tsm_atb_f64_16x16_mfma_r64_p2", 0, 16, 16, 0, 0, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p2},
-  {"tsm_atb_f64_16x16_mfma_r64_p2_nt", 0, 16, 16, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p2_nt},
-  {"tsm_atb_f64_16x16_mfma_r128_p1", 0, 16, 16, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p1},
-  {"tsm_atb_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p1_nt},
-  {"tsm_atb_f64_16x16_mfma_r128_p2", 0, 16, 16, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p2},
-  {"tsm_atb_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p2_nt},
-  {"tsm_atb_f64_16x16_mfma_r256_p1", 0, 16, 16, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p1},
-  {"tsm_atb_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p1_nt},
-  {"tsm_atb_f64_16x16_mfma_r256_p2", 0, 16, 16, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p2},
-  {"tsm_atb_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p2_nt},
-  {"tsm_atb_f64_16x16_t2x2_r64_p1", 0, 16, 16, 2, 2, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t2x2_r64_p1},
-  {"tsm_atb_f64_16x16_t2x2_r64_p2", 0, 16, 16, 2, 2, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t2x2_r64_p2},
-  {"tsm_atb_f64_16x16_t4x4_r64_p1", 0, 16, 16, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p1},
-  {"tsm_atb_f64_16x16_t4x4_r64_p2", 0, 16, 16, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p2},
-  {"tsm_atb_f64_16x16_t4x4_r128_p1", 0, 16, 16, 4, 4, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p1},
-  {"tsm_atb_f64_16x16_t4x4_r128_p2", 0, 16, 16, 4, 4, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p2},
-  {"tsm_atb_f64_16x16_t4x4_r256_p1", 0, 16, 16, 4, 4, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p1},
-  {"tsm_atb_f64_16x16_t4x4_r256_p2", 0, 16, 16, 4, 4, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p2},
-  {"tsm_atb_f64_16x16_t8x8_r64_p1", 0, 16, 16, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1},
-  {"tsm_atb_f64_16x16_t8x8_r64_p1_nt", 0, 16, 16, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1_nt},
-  {"tsm_atb_f64_16x16_t8x8_r64_p2", 0, 16, 16, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p2},
-  {"tsm_atb_f64_16x16_t8x8_r64_p2_nt", 0, 16, 16, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p2_nt},
-  {"tsm_atb_f64_16x16_t8x8_r128_p1", 0, 16, 16, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p1},
-  {"tsm_atb_f64_16x16_t8x8_r128_p1_nt", 0, 16, 16, 8, 8, 128, 1, 1, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p1_nt},
-  {"tsm_atb_f64_16x16_t8x8_r128_p2", 0, 16, 16, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p2},
-  {"tsm_atb_f64_16x16_t8x8_r128_p2_nt", 0, 16, 16, 8, 8, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p2_nt},
-  {"tsm_atb_f64_16x16_t8x8_r256_p1", 0, 16, 16, 8, 8, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p1},
-  {"tsm_atb_f64_16x16_t8x8_r256_p1_nt", 0, 16, 16, 8, 8, 256, 1, 1, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p1_nt},
-  {"tsm_atb_f64_16x16_t8x8_r256_p2", 0, 16, 16, 8, 8, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2},
-  {"tsm_atb_f64_16x16_t8x8_r256_p2_nt", 0, 16, 16, 8, 8, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt},
-  {"tsm_atb_f64_16x32_t8x8_r64_p2", 0, 16, 32, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_16x32_t8x8_r64_p2},
-  {"tsm_atb_f64_16x64_mfma_r32_p2", 0, 16, 64, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_16x64_mfma_r32_p2},
-  {"tsm_atb_f64_32x1_t8x1_r64_p2", 0, 32, 1, 8, 1, 64, 2, 0, 33792, launch_tsm_atb_f64_32x1_t8x1_r64_p2},
-  {"tsm_atb_f64_32x2_t8x2_r64_p2", 0, 32, 2, 8, 2, 64, 2, 0, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2},
-  {"tsm_atb_f64_32x4_t4x4_r32_p1", 0, 32, 4, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p1},
-  {"tsm_atb_f64_32x4_t4x4_r32_p2", 0, 32, 4, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2},
-  {"tsm_atb_f64_32x4_t4x4_r64_p1", 0, 32, 4, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p1},
-  {"tsm_atb_f64_32x4_t4x4_r64_p2", 0, 32, 4, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p2},
-  {"tsm_atb_f64_32x4_t4x4_r128_p1", 0, 32, 4, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p1},
-  {"tsm_atb_f64_32x4_t4x4_r128_p2", 0, 32, 4, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p2},
-  {"tsm_atb_f64_32x4_t8x4_r64_p1", 0, 32, 4, 8, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1},
-  {"tsm_atb_f64_32x4_t8x4_r64_p1_nt", 0, 32, 4, 8, 4, 64, 1, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1_nt},
-  {"tsm_atb_f64_32x4_t8x4_r64_p2", 0, 32, 4, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2},
-  {"tsm_atb_f64_32x4_t8x4_r64_p2_nt", 0, 32, 4, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2_nt},
-  {"tsm_atb_f64_32x4_t8x4_r128_p1", 0, 32, 4, 8, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1},
-  {"tsm_atb_f64_32x4_t8x4_r128_p1_nt", 0, 32, 4, 8, 4, 128, 1, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt},
-  {"tsm_atb_f64_32x4_t8x4_r128_p2", 0, 32, 4, 8, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2},
-  {"tsm_atb_f64_32x4_t8x4_r128_p2_nt", 0, 32, 4, 8, 4, 128, 2, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2_nt},
-  {"tsm_atb_f64_32x8_t8x8_r64_p2", 0, 32, 8, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_32x8_t8x8_r64_p2},
-  {"tsm_atb_f64_32x16_t8x8_r64_p2", 0, 32, 16, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_32x16_t8x8_r64_p2},
-  {"tsm_atb_f64_32x32_mfma_r32_p1", 0, 32, 32, 0, 0, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p1},
-  {"tsm_atb_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p1_nt},
-  {"tsm_atb_f64_32x32_mfma_r32_p2", 0, 32, 32, 0, 0, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p2},
-  {"tsm_atb_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p2_nt},
-  {"tsm_atb_f64_32x32_mfma_r64_p1", 0, 32, 32, 0, 0, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p1},
-  {"tsm_atb_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p1_nt},
-  {"tsm_atb_f64_32x32_mfma_r64_p2", 0, 32, 32, 0, 0, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p2},
-  {"tsm_atb_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p2_nt},
-  {"tsm_atb_f64_32x32_mfma_r128_p1", 0, 32, 32, 0, 0, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p1},
-  {"tsm_atb_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p1_nt},
-  {"tsm_atb_f64_32x32_mfma_r128_p2", 0, 32, 32, 0, 0, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p2},
-  {"tsm_atb_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p2_nt},
-  {"tsm_atb_f64_32x32_t4x4_r32_p1", 0, 32, 32, 4, 4, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p1},
-  {"tsm_atb_f64_32x32_t4x4_r32_p2", 0, 32, 32, 4, 4, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p2},
-  {"tsm_atb_f64_32x32_t4x4_r64_p1", 0, 32, 32, 4, 4, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p1},
-  {"tsm_atb_f64_32x32_t4x4_r64_p2", 0, 32, 32, 4, 4, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p2},
-  {"tsm_atb_f64_32x32_t8x8_r32_p1", 0, 32, 32, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1},
-  {"tsm_atb_f64_32x32_t8x8_r32_p1_nt", 0, 32, 32, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1_nt},
-  {"tsm_atb_f64_32x32_t8x8_r32_p2", 0, 32, 32, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p2},
-  {"tsm_atb_f64_32x32_t8x8_r32_p2_nt", 0, 32, 32, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p2_nt},
-  {"tsm_atb_f64_32x32_t8x8_r64_p1", 0, 32, 32, 8, 8, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p1},
-  {"tsm_atb_f64_32x32_t8x8_r64_p1_nt", 0, 32, 32, 8, 8, 64, 1, 1, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p1_nt},
-  {"tsm_atb_f64_32x32_t8x8_r64_p2", 0, 32, 32, 8, 8, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p2},
-  {"tsm_atb_f64_32x32_t8x8_r64_p2_nt", 0, 32, 32, 8, 8, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p2_nt},
-  {"tsm_atb_f64_32x32_t8x8_r128_p1", 0, 32, 32, 8, 8, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p1},
-  {"tsm_atb_f64_32x32_t8x8_r128_p1_nt", 0, 32, 32, 8, 8, 128, 1, 1, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p1_nt},
-  {"tsm_atb_f64_32x32_t8x8_r128_p2", 0, 32, 32, 8, 8, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2},
-  {"tsm_atb_f64_32x32_t8x8_r128_p2_nt", 0, 32, 32, 8, 8, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt},
-  {"tsm_atb_f64_32x64_mfma_r32_p2", 0, 32, 64, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_32x64_mfma_r32_p2},
-  {"tsm_atb_f64_64x1_t8x1_r32_p2", 0, 64, 1, 8, 1, 32, 2, 0, 33280, launch_tsm_atb_f64_64x1_t8x1_r32_p2},
-  {"tsm_atb_f64_64x2_t8x2_r32_p2", 0, 64, 2, 8, 2, 32, 2, 0, 33792, launch_tsm_atb_f64_64x2_t8x2_r32_p2},
-  {"tsm_atb_f64_64x4_t8x4_r32_p2", 0, 64, 4, 8, 4, 32, 2, 0, 34816, launch_tsm_atb_f64_64x4_t8x4_r32_p2},
-  {"tsm_atb_f64_64x8_t8x8_r32_p2", 0, 64, 8, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_64x8_t8x8_r32_p2},
-  {"tsm_atb_f64_64x16_mfma_r32_p2", 0, 64, 16, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_64x16_mfma_r32_p2},
-  {"tsm_atb_f64_64x32_mfma_r32_p2", 0, 64, 32, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_64x32_mfma_r32_p2},
-  {"tsm_atb_f64_64x64_mfma_r16_p1", 0, 64, 64, 0, 0, 16, 1, 0, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p1},
-  {"tsm_atb_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p1_nt},
-  {"tsm_atb_f64_64x64_mfma_r16_p2", 0, 64, 64, 0, 0, 16, 2, 0, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p2},
-  {"tsm_atb_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p2_nt},
-  {"tsm_atb_f64_64x64_mfma_r32_p1", 0, 64, 64, 0, 0, 32, 1, 0, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p1},
-  {"tsm_atb_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p1_nt},
-  {"tsm_atb_f64_64x64_mfma_r32_p2", 0, 64, 64, 0, 0, 32, 2, 0, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p2},
-  {"tsm_atb_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p2_nt},
-  {"tsm_atb_f64_64x64_mfma_r64_p1", 0, 64, 64, 0, 0, 64, 1, 0, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p1},
-  {"tsm_atb_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p1_nt},
-  {"tsm_atb_f64_64x64_mfma_r64_p2", 0, 64, 64, 0, 0, 64, 2, 0, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p2},
-  {"tsm_atb_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p2_nt},
-  {"tsm_atb_f64_64x64_t8x8_r16_p1", 0, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p1},
-  {"tsm_atb_f64_64x64_t8x8_r16_p1_nt", 0, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p1_nt},
-  {"tsm_atb_f64_64x64_t8x8_r16_p2", 0, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p2},
-  {"tsm_atb_f64_64x64_t8x8_r16_p2_nt", 0, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p2_nt},
-  {"tsm_atb_f64_64x64_t8x8_r32_p1", 0, 64, 64, 8, 8, 32, 1, 0, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p1},
-  {"tsm_atb_f64_64x64_t8x8_r32_p1_nt", 0, 64, 64, 8, 8, 32, 1, 1, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p1_nt},
-  {"tsm_atb_f64_64x64_t8x8_r32_p2", 0, 64, 64, 8, 8, 32, 2, 0, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p2},
-  {"tsm_atb_f64_64x64_t8x8_r32_p2_nt", 0, 64, 64, 8, 8, 32, 2, 1, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p2_nt},
-  {"tsm_atb_f64_64x64_t8x8_r64_p1", 0, 64, 64, 8, 8, 64, 1, 0, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p1},
-  {"tsm_atb_f64_64x64_t8x8_r64_p1_nt", 0, 64, 64, 8, 8, 64, 1, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p1_nt},
-  {"tsm_atb_f64_64x64_t8x8_r64_p2", 0, 64, 64, 8, 8, 64, 2, 0, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2},
-  {"tsm_atb_f64_64x64_t8x8_r64_p2_nt", 0, 64, 64, 8, 8, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt},
+  {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2, 1},
+  {"tsm_atb_bf16_1x2_t1x2_r4096_p2", 2, 1, 2, 1, 2, 4096, 2, 0, 49152, launch_tsm_atb_bf16_1x2_t1x2_r4096_p2, 1},
+  {"tsm_atb_bf16_1x4_t1x4_r2048_p2", 2, 1, 4, 1, 4, 2048, 2, 0, 40960, launch_tsm_atb_bf16_1x4_t1x4_r2048_p2, 1},
+  {"tsm_atb_bf16_1x8_t1x4_r1024_p2", 2, 1, 8, 1, 4, 1024, 2, 0, 36864, launch_tsm_atb_bf16_1x8_t1x4_r1024_p2, 1},
+  {"tsm_atb_bf16_1x16_t1x4_r512_p2", 2, 1, 16, 1, 4, 512, 2, 0, 34816, launch_tsm_atb_bf16_1x16_t1x4_r512_p2, 1},
+  {"tsm_atb_bf16_1x32_t1x4_r256_p2", 2, 1, 32, 1, 4, 256, 2, 0, 33792, launch_tsm_atb_bf16_1x32_t1x4_r256_p2, 1},
+  {"tsm_atb_bf16_1x64_t1x4_r128_p2", 2, 1, 64, 1, 4, 128, 2, 0, 33280, launch_tsm_atb_bf16_1x64_t1x4_r128_p2, 1},
+  {"tsm_atb_bf16_2x1_t2x1_r4096_p2", 2, 2, 1, 2, 1, 4096, 2, 0, 49152, launch_tsm_atb_bf16_2x1_t2x1_r4096_p2, 1},
+  {"tsm_atb_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_atb_bf16_2x2_t2x2_r4096_p2, 1},
+  {"tsm_atb_bf16_2x4_t2x4_r2048_p2", 2, 2, 4, 2, 4, 2048, 2, 0, 49152, launch_tsm_atb_bf16_2x4_t2x4_r2048_p2, 1},
+  {"tsm_atb_bf16_2x8_t2x4_r1024_p2", 2, 2, 8, 2, 4, 1024, 2, 0, 40960, launch_tsm_atb_bf16_2x8_t2x4_r1024_p2, 1},
+  {"tsm_atb_bf16_2x16_t2x4_r512_p2", 2, 2, 16, 2, 4, 512, 2, 0, 36864, launch_tsm_atb_bf16_2x16_t2x4_r512_p2, 1},
+  {"tsm_atb_bf16_2x32_t2x4_r256_p2", 2, 2, 32, 2, 4, 256, 2, 0, 34816, launch_tsm_atb_bf16_2x32_t2x4_r256_p2, 1},
+  {"tsm_atb_bf16_2x64_t2x4_r128_p2", 2, 2, 64, 2, 4, 128, 2, 0, 33792, launch_tsm_atb_bf16_2x64_t2x4_r128_p2, 1},
+  {"tsm_atb_bf16_4x1_t4x1_r2048_p2", 2, 4, 1, 4, 1, 2048, 2, 0, 40960, launch_tsm_atb_bf16_4x1_t4x1_r2048_p2, 1},
+  {"tsm_atb_bf16_4x2_t4x2_r2048_p2", 2, 4, 2, 4, 2, 2048, 2, 0, 49152, launch_tsm_atb_bf16_4x2_t4x2_r2048_p2, 1},
+  {"tsm_atb_bf16_4x4_t4x4_r2048_p2", 2, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_atb_bf16_4x4_t4x4_r2048_p2, 1},
+  {"tsm_atb_bf16_4x8_t4x4_r1024_p2", 2, 4, 8, 4, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_4x8_t4x4_r1024_p2, 1},
+  {"tsm_atb_bf16_4x16_t4x4_r512_p2", 2, 4, 16, 4, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_4x16_t4x4_r512_p2, 1},
+  {"tsm_atb_bf16_4x32_t4x4_r256_p2", 2, 4, 32, 4, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_4x32_t4x4_r256_p2, 1},
+  {"tsm_atb_bf16_4x64_t4x4_r128_p2", 2, 4, 64, 4, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_4x64_t4x4_r128_p2, 1},
+  {"tsm_atb_bf16_8x1_t8x1_r1024_p2", 2, 8, 1, 8, 1, 1024, 2, 0, 36864, launch_tsm_atb_bf16_8x1_t8x1_r1024_p2, 1},
+  {"tsm_atb_bf16_8x2_t8x2_r1024_p2", 2, 8, 2, 8, 2, 1024, 2, 0, 40960, launch_tsm_atb_bf16_8x2_t8x2_r1024_p2, 1},
+  {"tsm_atb_bf16_8x4_t8x4_r1024_p2", 2, 8, 4, 8, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_8x4_t8x4_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r512_p1", 2, 8, 8, 4, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p1, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r512_p2", 2, 8, 8, 4, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r1024_p1", 2, 8, 8, 4, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p1, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r1024_p2", 2, 8, 8, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r512_p1", 2, 8, 8, 8, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r512_p2", 2, 8, 8, 8, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r1024_p1", 2, 8, 8, 8, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r1024_p2", 2, 8, 8, 8, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r2048_p1", 2, 8, 8, 8, 4, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r2048_p2", 2, 8, 8, 8, 4, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r512_p1", 2, 8, 8, 8, 8, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r512_p2", 2, 8, 8, 8, 8, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r1024_p1", 2, 8, 8, 8, 8, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r1024_p2", 2, 8, 8, 8, 8, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r2048_p1", 2, 8, 8, 8, 8, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p1, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r2048_p2", 2, 8, 8, 8, 8, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p2, 1},
+  {"tsm_atb_bf16_8x16_t8x4_r512_p2", 2, 8, 16, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_8x16_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_8x32_t8x4_r256_p2", 2, 8, 32, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_8x32_t8x4_r256_p2, 1},
+  {"tsm_atb_bf16_8x64_t8x4_r128_p2", 2, 8, 64, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2, 1},
+  {"tsm_atb_bf16_16x1_t8x1_r512_p2", 2, 16, 1, 8, 1, 512, 2, 0, 34816, launch_tsm_atb_bf16_16x1_t8x1_r512_p2, 1},
+  {"tsm_atb_bf16_16x2_t8x2_r512_p2", 2, 16, 2, 8, 2, 512, 2, 0, 36864, launch_tsm_atb_bf16_16x2_t8x2_r512_p2, 1},
+  {"tsm_atb_bf16_16x4_t8x4_r512_p2", 2, 16, 4, 8, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_16x4_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_16x8_t8x4_r512_p2", 2, 16, 8, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_16x8_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_16x16_mfma_r256_p1", 2, 16, 16, 0, 0, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1, 2},
+  {"tsm_atb_bf16_16x16_mfma_r256_p1_nt", 2, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_16x16_mfma_r256_p2", 2, 16, 16, 0, 0, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_16x16_mfma_r256_p2_nt", 2, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_16x16_mfma_r512_p1", 2, 16, 16, 0, 0, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p1, 2},
+  {"tsm_atb_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_16x16_mfma_r512_p2", 2, 16, 16, 0, 0, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p2, 2},
+  {"tsm_atb_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_mfma_r512_p2_nt, 2},
+  {"tsm_atb_bf16_16x16_mfma_r1024_p1", 2, 16, 16, 0, 0, 1024, 1, 0, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p1, 2},
+  {"tsm_atb_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p1_nt, 2},
+  {"tsm_atb_bf16_16x16_mfma_r1024_p2", 2, 16, 16, 0, 0, 1024, 2, 0, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p2, 2},
+  {"tsm_atb_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p2_nt, 2},
+  {"tsm_atb_bf16_16x16_t4x4_r128_p1", 2, 16, 16, 4, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p1, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r128_p2", 2, 16, 16, 4, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r256_p1", 2, 16, 16, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p1, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r256_p2", 2, 16, 16, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r128_p1", 2, 16, 16, 8, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r128_p1_nt", 2, 16, 16, 8, 4, 128, 1, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r128_p2", 2, 16, 16, 8, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r128_p2_nt", 2, 16, 16, 8, 4, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r256_p1", 2, 16, 16, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r256_p1_nt", 2, 16, 16, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r256_p2", 2, 16, 16, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r256_p2_nt", 2, 16, 16, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r512_p1", 2, 16, 16, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r512_p1_nt", 2, 16, 16, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r512_p2", 2, 16, 16, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r512_p2_nt", 2, 16, 16, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r128_p1", 2, 16, 16, 8, 8, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r128_p2", 2, 16, 16, 8, 8, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r256_p1", 2, 16, 16, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r256_p2", 2, 16, 16, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r512_p1", 2, 16, 16, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p1, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r512_p2", 2, 16, 16, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p2, 1},
+  {"tsm_atb_bf16_16x32_mfma_r256_p2", 2, 16, 32, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_16x32_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_16x64_mfma_r128_p2", 2, 16, 64, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_16x64_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_32x1_t8x1_r256_p2", 2, 32, 1, 8, 1, 256, 2, 0, 33792, launch_tsm_atb_bf16_32x1_t8x1_r256_p2, 1},
+  {"tsm_atb_bf16_32x2_t8x2_r256_p2", 2, 32, 2, 8, 2, 256, 2, 0, 34816, launch_tsm_atb_bf16_32x2_t8x2_r256_p2, 1},
+  {"tsm_atb_bf16_32x4_t8x4_r256_p2", 2, 32, 4, 8, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_32x4_t8x4_r256_p2, 1},
+  {"tsm_atb_bf16_32x8_t8x4_r256_p2", 2, 32, 8, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_32x8_t8x4_r256_p2, 1},
+  {"tsm_atb_bf16_32x16_mfma_r256_p2", 2, 32, 16, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_32x16_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_32x32_mfma_r128_p1", 2, 32, 32, 0, 0, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1, 2},
+  {"tsm_atb_bf16_32x32_mfma_r128_p1_nt", 2, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_32x32_mfma_r128_p2", 2, 32, 32, 0, 0, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_32x32_mfma_r128_p2_nt", 2, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_32x32_mfma_r256_p1", 2, 32, 32, 0, 0, 256, 1, 0, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p1, 2},
+  {"tsm_atb_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_32x32_mfma_r256_p2", 2, 32, 32, 0, 0, 256, 2, 0, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_bf16_32x32_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_32x32_mfma_r512_p1", 2, 32, 32, 0, 0, 512, 1, 0, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p1, 2},
+  {"tsm_atb_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_32x32_mfma_r512_p2", 2, 32, 32, 0, 0, 512, 2, 0, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p2, 2},
+  {"tsm_atb_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p2_nt, 2},
+  {"tsm_atb_bf16_32x32_t4x4_r32_p1", 2, 32, 32, 4, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p1, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r32_p2", 2, 32, 32, 4, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r64_p1", 2, 32, 32, 4, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p1, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r64_p2", 2, 32, 32, 4, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r32_p1", 2, 32, 32, 8, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r32_p1_nt", 2, 32, 32, 8, 4, 32, 1, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r32_p2", 2, 32, 32, 8, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r32_p2_nt", 2, 32, 32, 8, 4, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r64_p1", 2, 32, 32, 8, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r64_p1_nt", 2, 32, 32, 8, 4, 64, 1, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r64_p2", 2, 32, 32, 8, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r64_p2_nt", 2, 32, 32, 8, 4, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r128_p1", 2, 32, 32, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r128_p1_nt", 2, 32, 32, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r128_p2", 2, 32, 32, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r128_p2_nt", 2, 32, 32, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r32_p1", 2, 32, 32, 8, 8, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r32_p2", 2, 32, 32, 8, 8, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r64_p1", 2, 32, 32, 8, 8, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r64_p2", 2, 32, 32, 8, 8, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r128_p1", 2, 32, 32, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p1, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r128_p2", 2, 32, 32, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p2, 1},
+  {"tsm_atb_bf16_32x64_mfma_r128_p2", 2, 32, 64, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_64x1_t8x1_r128_p2", 2, 64, 1, 8, 1, 128, 2, 0, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2, 1},
+  {"tsm_atb_bf16_64x2_t8x2_r128_p2", 2, 64, 2, 8, 2, 128, 2, 0, 33792, launch_tsm_atb_bf16_64x2_t8x2_r128_p2, 1},
+  {"tsm_atb_bf16_64x4_t8x4_r128_p2", 2, 64, 4, 8, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_64x4_t8x4_r128_p2, 1},
+  {"tsm_atb_bf16_64x8_t8x4_r128_p2", 2, 64, 8, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_64x8_t8x4_r128_p2, 1},
+  {"tsm_atb_bf16_64x16_mfma_r128_p2", 2, 64, 16, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_64x16_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_64x32_mfma_r128_p2", 2, 64, 32, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_64x32_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_64x64_mfma_r128_p1", 2, 64, 64, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1, 2},
+  {"tsm_atb_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_64x64_mfma_r128_p2", 2, 64, 64, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_64x64_mfma_r128_p2_nt", 2, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_64x64_mfma_r256_p1", 2, 64, 64, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p1, 2},
+  {"tsm_atb_bf16_64x64_mfma_r256_p1_nt", 2, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_64x64_mfma_r256_p2", 2, 64, 64, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_bf16_64x64_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_64x64_t8x8_r16_p1", 2, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p1, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r16_p1_nt", 2, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p1_nt, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r16_p2", 2, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p2, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r16_p2_nt", 2, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r16_p2_nt, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r32_p1", 2, 64, 64, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p1, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r32_p1_nt", 2, 64, 64, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p1_nt, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r32_p2", 2, 64, 64, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p2, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r32_p2_nt", 2, 64, 64, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r32_p2_nt, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r64_p1", 2, 64, 64, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p1, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r64_p1_nt", 2, 64, 64, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p1_nt, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r64_p2", 2, 64, 64, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2, 1},
+  {"tsm_atb_bf16_64x64_t8x8_r64_p2_nt", 2, 64, 64, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2_nt, 1},
+  {"tsm_atb_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 65536, launch_tsm_atb_f32_1x1_t1x1_r4096_p2, 1},
+  {"tsm_atb_f32_1x2_t1x2_r2048_p2", 1, 1, 2, 1, 2, 2048, 2, 0, 49152, launch_tsm_atb_f32_1x2_t1x2_r2048_p2, 1},
+  {"tsm_atb_f32_1x4_t1x4_r1024_p2", 1, 1, 4, 1, 4, 1024, 2, 0, 40960, launch_tsm_atb_f32_1x4_t1x4_r1024_p2, 1},
+  {"tsm_atb_f32_1x8_t1x4_r512_p2", 1, 1, 8, 1, 4, 512, 2, 0, 36864, launch_tsm_atb_f32_1x8_t1x4_r512_p2, 1},
+  {"tsm_atb_f32_1x16_t1x4_r256_p2", 1, 1, 16, 1, 4, 256, 2, 0, 34816, launch_tsm_atb_f32_1x16_t1x4_r256_p2, 1},
+  {"tsm_atb_f32_1x32_t1x4_r128_p2", 1, 1, 32, 1, 4, 128, 2, 0, 33792, launch_tsm_atb_f32_1x32_t1x4_r128_p2, 1},
+  {"tsm_atb_f32_1x64_t1x4_r64_p2", 1, 1, 64, 1, 4, 64, 2, 0, 33280, launch_tsm_atb_f32_1x64_t1x4_r64_p2, 1},
+  {"tsm_atb_f32_2x1_t2x1_r2048_p2", 1, 2, 1, 2, 1, 2048, 2, 0, 49152, launch_tsm_atb_f32_2x1_t2x1_r2048_p2, 1},
+  {"tsm_atb_f32_2x2_t2x2_r2048_p2", 1, 2, 2, 2, 2, 2048, 2, 0, 65536, launch_tsm_atb_f32_2x2_t2x2_r2048_p2, 1},
+  {"tsm_atb_f32_2x4_t2x4_r1024_p2", 1, 2, 4, 2, 4, 1024, 2, 0, 49152, launch_tsm_atb_f32_2x4_t2x4_r1024_p2, 1},
+  {"tsm_atb_f32_2x8_t2x4_r512_p2", 1, 2, 8, 2, 4, 512, 2, 0, 40960, launch_tsm_atb_f32_2x8_t2x4_r512_p2, 1},
+  {"tsm_atb_f32_2x16_t2x4_r256_p2", 1, 2, 16, 2, 4, 256, 2, 0, 36864, launch_tsm_atb_f32_2x16_t2x4_r256_p2, 1},
+  {"tsm_atb_f32_2x32_t2x4_r128_p2", 1, 2, 32, 2, 4, 128, 2, 0, 34816, launch_tsm_atb_f32_2x32_t2x4_r128_p2, 1},
+  {"tsm_atb_f32_2x64_t2x4_r64_p2", 1, 2, 64, 2, 4, 64, 2, 0, 33792, launch_tsm_atb_f32_2x64_t2x4_r64_p2, 1},
+  {"tsm_atb_f32_4x1_t4x1_r1024_p2", 1, 4, 1, 4, 1, 1024, 2, 0, 40960, launch_tsm_atb_f32_4x1_t4x1_r1024_p2, 1},
+  {"tsm_atb_f32_4x2_t4x2_r1024_p2", 1, 4, 2, 4, 2, 1024, 2, 0, 49152, launch_tsm_atb_f32_4x2_t4x2_r1024_p2, 1},
+  {"tsm_atb_f32_4x4_t4x4_r1024_p2", 1, 4, 4, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_f32_4x4_t4x4_r1024_p2, 1},
+  {"tsm_atb_f32_4x8_t4x4_r512_p2", 1, 4, 8, 4, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_4x8_t4x4_r512_p2, 1},
+  {"tsm_atb_f32_4x16_t4x4_r256_p2", 1, 4, 16, 4, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_4x16_t4x4_r256_p2, 1},
+  {"tsm_atb_f32_4x32_t4x4_r128_p2", 1, 4, 32, 4, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_4x32_t4x4_r128_p2, 1},
+  {"tsm_atb_f32_4x64_t4x4_r64_p2", 1, 4, 64, 4, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_4x64_t4x4_r64_p2, 1},
+  {"tsm_atb_f32_8x1_t8x1_r512_p2", 1, 8, 1, 8, 1, 512, 2, 0, 36864, launch_tsm_atb_f32_8x1_t8x1_r512_p2, 1},
+  {"tsm_atb_f32_8x2_t8x2_r512_p2", 1, 8, 2, 8, 2, 512, 2, 0, 40960, launch_tsm_atb_f32_8x2_t8x2_r512_p2, 1},
+  {"tsm_atb_f32_8x4_t8x4_r512_p2", 1, 8, 4, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_8x4_t8x4_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r256_p1", 1, 8, 8, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p1, 1},
+  {"tsm_atb_f32_8x8_t4x4_r256_p2", 1, 8, 8, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r512_p1", 1, 8, 8, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p1, 1},
+  {"tsm_atb_f32_8x8_t4x4_r512_p2", 1, 8, 8, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r1024_p1", 1, 8, 8, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p1, 1},
+  {"tsm_atb_f32_8x8_t4x4_r1024_p2", 1, 8, 8, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p2, 1},
+  {"tsm_atb_f32_8x8_t8x4_r256_p1", 1, 8, 8, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1, 1},
+  {"tsm_atb_f32_8x8_t8x4_r256_p1_nt", 1, 8, 8, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r256_p2", 1, 8, 8, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t8x4_r256_p2_nt", 1, 8, 8, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r512_p1", 1, 8, 8, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1, 1},
+  {"tsm_atb_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r512_p2", 1, 8, 8, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r1024_p1", 1, 8, 8, 8, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1, 1},
+  {"tsm_atb_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r1024_p2", 1, 8, 8, 8, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2, 1},
+  {"tsm_atb_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t8x8_r256_p1", 1, 8, 8, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p1, 1},
+  {"tsm_atb_f32_8x8_t8x8_r256_p2", 1, 8, 8, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t8x8_r512_p1", 1, 8, 8, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p1, 1},
+  {"tsm_atb_f32_8x8_t8x8_r512_p2", 1, 8, 8, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t8x8_r1024_p1", 1, 8, 8, 8, 8, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p1, 1},
+  {"tsm_atb_f32_8x8_t8x8_r1024_p2", 1, 8, 8, 8, 8, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p2, 1},
+  {"tsm_atb_f32_8x16_t8x4_r256_p2", 1, 8, 16, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_8x16_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_8x32_t8x4_r128_p2", 1, 8, 32, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_8x32_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_8x64_t8x4_r64_p2", 1, 8, 64, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2, 1},
+  {"tsm_atb_f32_16x1_t8x1_r256_p2", 1, 16, 1, 8, 1, 256, 2, 0, 34816, launch_tsm_atb_f32_16x1_t8x1_r256_p2, 1},
+  {"tsm_atb_f32_16x2_t8x2_r256_p2", 1, 16, 2, 8, 2, 256, 2, 0, 36864, launch_tsm_atb_f32_16x2_t8x2_r256_p2, 1},
+  {"tsm_atb_f32_16x4_t8x4_r256_p2", 1, 16, 4, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_16x4_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_16x8_t8x4_r256_p2", 1, 16, 8, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_16x8_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_16x16_direct_d4_nt", 1, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_atb_f32_16x16_direct_d4_nt, 3},
+  {"tsm_atb_f32_16x16_direct_d8_nt", 1, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_atb_f32_16x16_direct_d8_nt, 3},
+  {"tsm_atb_f32_16x16_direct_d16_nt", 1, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_atb_f32_16x16_direct_d16_nt, 3},
+  {"tsm_atb_f32_16x16_mfma_r128_p1", 1, 16, 16, 0, 0, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p1, 2},
+  {"tsm_atb_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_16x16_mfma_r128_p2", 1, 16, 16, 0, 0, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p2, 2},
+  {"tsm_atb_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_16x16_mfma_r256_p1", 1, 16, 16, 0, 0, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p1, 2},
+  {"tsm_atb_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_16x16_mfma_r256_p2", 1, 16, 16, 0, 0, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p2, 2},
+  {"tsm_atb_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_atb_f32_16x16_t4x4_r128_p1", 1, 16, 16, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p1, 1},
+  {"tsm_atb_f32_16x16_t4x4_r128_p2", 1, 16, 16, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t4x4_r256_p1", 1, 16, 16, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p1, 1},
+  {"tsm_atb_f32_16x16_t4x4_r256_p2", 1, 16, 16, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t8x4_r128_p1", 1, 16, 16, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1, 1},
+  {"tsm_atb_f32_16x16_t8x4_r128_p1_nt", 1, 16, 16, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r128_p2", 1, 16, 16, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t8x4_r128_p2_nt", 1, 16, 16, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r256_p1", 1, 16, 16, 8, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1, 1},
+  {"tsm_atb_f32_16x16_t8x4_r256_p1_nt", 1, 16, 16, 8, 4, 256, 1, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r256_p2", 1, 16, 16, 8, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t8x4_r256_p2_nt", 1, 16, 16, 8, 4, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r512_p1", 1, 16, 16, 8, 4, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1, 1},
+  {"tsm_atb_f32_16x16_t8x4_r512_p1_nt", 1, 16, 16, 8, 4, 512, 1, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r512_p2", 1, 16, 16, 8, 4, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2, 1},
+  {"tsm_atb_f32_16x16_t8x4_r512_p2_nt", 1, 16, 16, 8, 4, 512, 2, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t8x8_r128_p1", 1, 16, 16, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p1, 1},
+  {"tsm_atb_f32_16x16_t8x8_r128_p2", 1, 16, 16, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t8x8_r256_p1", 1, 16, 16, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p1, 1},
+  {"tsm_atb_f32_16x16_t8x8_r256_p2", 1, 16, 16, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t8x8_r512_p1", 1, 16, 16, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p1, 1},
+  {"tsm_atb_f32_16x16_t8x8_r512_p2", 1, 16, 16, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p2, 1},
+  {"tsm_atb_f32_16x32_direct_d8_nt", 1, 16, 32, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f32_16x32_direct_d8_nt, 3},
+  {"tsm_atb_f32_16x32_t8x4_r128_p2", 1, 16, 32, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_16x32_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_16x64_direct_d8_nt", 1, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_16x64_direct_d8_nt, 3},
+  {"tsm_atb_f32_16x64_mfma_r64_p2", 1, 16, 64, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_16x64_mfma_r64_p2, 2},
+  {"tsm_atb_f32_32x1_t8x1_r128_p2", 1, 32, 1, 8, 1, 128, 2, 0, 33792, launch_tsm_atb_f32_32x1_t8x1_r128_p2, 1},
+  {"tsm_atb_f32_32x2_t8x2_r128_p2", 1, 32, 2, 8, 2, 128, 2, 0, 34816, launch_tsm_atb_f32_32x2_t8x2_r128_p2, 1},
+  {"tsm_atb_f32_32x4_t8x4_r128_p2", 1, 32, 4, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_32x4_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_32x8_t8x4_r128_p2", 1, 32, 8, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_32x8_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_32x16_direct_d8_nt", 1, 32, 16, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f32_32x16_direct_d8_nt, 3},
+  {"tsm_atb_f32_32x16_t8x4_r128_p2", 1, 32, 16, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_32x16_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_32x32_direct_d4_nt", 1, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_atb_f32_32x32_direct_d4_nt, 3},
+  {"tsm_atb_f32_32x32_direct_d8_nt", 1, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_32x32_direct_d8_nt, 3},
+  {"tsm_atb_f32_32x32_direct_d16_nt", 1, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_atb_f32_32x32_direct_d16_nt, 3},
+  {"tsm_atb_f32_32x32_mfma_r64_p1", 1, 32, 32, 0, 0, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p1, 2},
+  {"tsm_atb_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_32x32_mfma_r64_p2", 1, 32, 32, 0, 0, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p2, 2},
+  {"tsm_atb_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_32x32_mfma_r128_p1", 1, 32, 32, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p1, 2},
+  {"tsm_atb_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_32x32_mfma_r128_p2", 1, 32, 32, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p2, 2},
+  {"tsm_atb_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_32x32_mfma_r256_p1", 1, 32, 32, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p1, 2},
+  {"tsm_atb_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_32x32_mfma_r256_p2", 1, 32, 32, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p2, 2},
+  {"tsm_atb_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p2_nt, 2},
+  {"tsm_atb_f32_32x32_t4x4_r32_p1", 1, 32, 32, 4, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p1, 1},
+  {"tsm_atb_f32_32x32_t4x4_r32_p2", 1, 32, 32, 4, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t4x4_r64_p1", 1, 32, 32, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p1, 1},
+  {"tsm_atb_f32_32x32_t4x4_r64_p2", 1, 32, 32, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t8x4_r32_p1", 1, 32, 32, 8, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1, 1},
+  {"tsm_atb_f32_32x32_t8x4_r32_p1_nt", 1, 32, 32, 8, 4, 32, 1, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r32_p2", 1, 32, 32, 8, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t8x4_r32_p2_nt", 1, 32, 32, 8, 4, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r64_p1", 1, 32, 32, 8, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1, 1},
+  {"tsm_atb_f32_32x32_t8x4_r64_p1_nt", 1, 32, 32, 8, 4, 64, 1, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r64_p2", 1, 32, 32, 8, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t8x4_r64_p2_nt", 1, 32, 32, 8, 4, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r128_p1", 1, 32, 32, 8, 4, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1, 1},
+  {"tsm_atb_f32_32x32_t8x4_r128_p1_nt", 1, 32, 32, 8, 4, 128, 1, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r128_p2", 1, 32, 32, 8, 4, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_32x32_t8x4_r128_p2_nt", 1, 32, 32, 8, 4, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t8x8_r32_p1", 1, 32, 32, 8, 8, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p1, 1},
+  {"tsm_atb_f32_32x32_t8x8_r32_p2", 1, 32, 32, 8, 8, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t8x8_r64_p1", 1, 32, 32, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p1, 1},
+  {"tsm_atb_f32_32x32_t8x8_r64_p2", 1, 32, 32, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t8x8_r128_p1", 1, 32, 32, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p1, 1},
+  {"tsm_atb_f32_32x32_t8x8_r128_p2", 1, 32, 32, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p2, 1},
+  {"tsm_atb_f32_32x64_direct_d8_nt", 1, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f32_32x64_direct_d8_nt, 3},
+  {"tsm_atb_f32_32x64_mfma_r64_p2", 1, 32, 64, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x1_t8x1_r64_p2", 1, 64, 1, 8, 1, 64, 2, 0, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2, 1},
+  {"tsm_atb_f32_64x2_t8x2_r64_p2", 1, 64, 2, 8, 2, 64, 2, 0, 33792, launch_tsm_atb_f32_64x2_t8x2_r64_p2, 1},
+  {"tsm_atb_f32_64x4_t8x4_r64_p2", 1, 64, 4, 8, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_64x4_t8x4_r64_p2, 1},
+  {"tsm_atb_f32_64x8_t8x4_r64_p2", 1, 64, 8, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_64x8_t8x4_r64_p2, 1},
+  {"tsm_atb_f32_64x16_direct_d8_nt", 1, 64, 16, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_64x16_direct_d8_nt, 3},
+  {"tsm_atb_f32_64x16_mfma_r64_p2", 1, 64, 16, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_64x16_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x32_direct_d8_nt", 1, 64, 32, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f32_64x32_direct_d8_nt, 3},
+  {"tsm_atb_f32_64x32_mfma_r64_p2", 1, 64, 32, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_64x32_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x64_direct_d4_nt", 1, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_atb_f32_64x64_direct_d4_nt, 3},
+  {"tsm_atb_f32_64x64_direct_d8_nt", 1, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_atb_f32_64x64_direct_d8_nt, 3},
+  {"tsm_atb_f32_64x64_direct_d16_nt", 1, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_atb_f32_64x64_direct_d16_nt, 3},
+  {"tsm_atb_f32_64x64_mfma_r32_p1", 1, 64, 64, 0, 0, 32, 1, 0, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p1, 2},
+  {"tsm_atb_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f32_64x64_mfma_r32_p2", 1, 64, 64, 0, 0, 32, 2, 0, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p2, 2},
+  {"tsm_atb_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_atb_f32_64x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f32_64x64_mfma_r64_p1", 1, 64, 64, 0, 0, 64, 1, 0, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p1, 2},
+  {"tsm_atb_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_64x64_mfma_r64_p2", 1, 64, 64, 0, 0, 64, 2, 0, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_atb_f32_64x64_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_64x64_mfma_r128_p1", 1, 64, 64, 0, 0, 128, 1, 0, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p1, 2},
+  {"tsm_atb_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_64x64_mfma_r128_p2", 1, 64, 64, 0, 0, 128, 2, 0, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p2, 2},
+  {"tsm_atb_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_atb_f32_64x64_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_64x64_t8x8_r16_p1", 1, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p1, 1},
+  {"tsm_atb_f32_64x64_t8x8_r16_p1_nt", 1, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p1_nt, 1},
+  {"tsm_atb_f32_64x64_t8x8_r16_p2", 1, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p2, 1},
+  {"tsm_atb_f32_64x64_t8x8_r16_p2_nt", 1, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r16_p2_nt, 1},
+  {"tsm_atb_f32_64x64_t8x8_r32_p1", 1, 64, 64, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p1, 1},
+  {"tsm_atb_f32_64x64_t8x8_r32_p1_nt", 1, 64, 64, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p1_nt, 1},
+  {"tsm_atb_f32_64x64_t8x8_r32_p2", 1, 64, 64, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p2, 1},
+  {"tsm_atb_f32_64x64_t8x8_r32_p2_nt", 1, 64, 64, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_f32_64x64_t8x8_r32_p2_nt, 1},
+  {"tsm_atb_f32_64x64_t8x8_r64_p1", 1, 64, 64, 8, 8, 64, 1, 0, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p1, 1},
+  {"tsm_atb_f32_64x64_t8x8_r64_p1_nt", 1, 64, 64, 8, 8, 64, 1, 1, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p1_nt, 1},
+  {"tsm_atb_f32_64x64_t8x8_r64_p2", 1, 64, 64, 8, 8, 64, 2, 0, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p2, 1},
+  {"tsm_atb_f32_64x64_t8x8_r64_p2_nt", 1, 64, 64, 8, 8, 64, 2, 1, 65536, launch_tsm_atb_f32_64x64_t8x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r1024_p1", 0, 1, 1, 1, 1, 1024, 1, 0, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p1, 1},
+  {"tsm_atb_f64_1x1_t1x1_r1024_p1_nt", 0, 1, 1, 1, 1, 1024, 1, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p1_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r1024_p2", 0, 1, 1, 1, 1, 1024, 2, 0, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2, 1},
+  {"tsm_atb_f64_1x1_t1x1_r1024_p2_nt", 0, 1, 1, 1, 1, 1024, 2, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r2048_p1", 0, 1, 1, 1, 1, 2048, 1, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1, 1},
+  {"tsm_atb_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r2048_p2", 0, 1, 1, 1, 1, 2048, 2, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2, 1},
+  {"tsm_atb_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r4096_p1", 0, 1, 1, 1, 1, 4096, 1, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1, 1},
+  {"tsm_atb_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r4096_p2", 0, 1, 1, 1, 1, 4096, 2, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2, 1},
+  {"tsm_atb_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_atb_f64_1x2_t1x2_r1024_p2", 0, 1, 2, 1, 2, 1024, 2, 0, 49152, launch_tsm_atb_f64_1x2_t1x2_r1024_p2, 1},
+  {"tsm_atb_f64_1x4_t1x4_r512_p2", 0, 1, 4, 1, 4, 512, 2, 0, 40960, launch_tsm_atb_f64_1x4_t1x4_r512_p2, 1},
+  {"tsm_atb_f64_1x8_t1x8_r256_p2", 0, 1, 8, 1, 8, 256, 2, 0, 36864, launch_tsm_atb_f64_1x8_t1x8_r256_p2, 1},
+  {"tsm_atb_f64_1x16_t1x8_r128_p2", 0, 1, 16, 1, 8, 128, 2, 0, 34816, launch_tsm_atb_f64_1x16_t1x8_r128_p2, 1},
+  {"tsm_atb_f64_1x32_t1x8_r64_p2", 0, 1, 32, 1, 8, 64, 2, 0, 33792, launch_tsm_atb_f64_1x32_t1x8_r64_p2, 1},
+  {"tsm_atb_f64_1x64_t1x8_r32_p2", 0, 1, 64, 1, 8, 32, 2, 0, 33280, launch_tsm_atb_f64_1x64_t1x8_r32_p2, 1},
+  {"tsm_atb_f64_2x1_t2x1_r1024_p2", 0, 2, 1, 2, 1, 1024, 2, 0, 49152, launch_tsm_atb_f64_2x1_t2x1_r1024_p2, 1},
+  {"tsm_atb_f64_2x2_t2x2_r512_p1", 0, 2, 2, 2, 2, 512, 1, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1, 1},
+  {"tsm_atb_f64_2x2_t2x2_r512_p1_nt", 0, 2, 2, 2, 2, 512, 1, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r512_p2", 0, 2, 2, 2, 2, 512, 2, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2, 1},
+  {"tsm_atb_f64_2x2_t2x2_r512_p2_nt", 0, 2, 2, 2, 2, 512, 2, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r1024_p1", 0, 2, 2, 2, 2, 1024, 1, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1, 1},
+  {"tsm_atb_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r1024_p2", 0, 2, 2, 2, 2, 1024, 2, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2, 1},
+  {"tsm_atb_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r2048_p1", 0, 2, 2, 2, 2, 2048, 1, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1, 1},
+  {"tsm_atb_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r2048_p2", 0, 2, 2, 2, 2, 2048, 2, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2, 1},
+  {"tsm_atb_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt, 1},
+  {"tsm_atb_f64_2x4_t2x4_r512_p2", 0, 2, 4, 2, 4, 512, 2, 0, 49152, launch_tsm_atb_f64_2x4_t2x4_r512_p2, 1},
+  {"tsm_atb_f64_2x8_t2x8_r256_p2", 0, 2, 8, 2, 8, 256, 2, 0, 40960, launch_tsm_atb_f64_2x8_t2x8_r256_p2, 1},
+  {"tsm_atb_f64_2x16_t2x8_r128_p2", 0, 2, 16, 2, 8, 128, 2, 0, 36864, launch_tsm_atb_f64_2x16_t2x8_r128_p2, 1},
+  {"tsm_atb_f64_2x32_t2x8_r64_p2", 0, 2, 32, 2, 8, 64, 2, 0, 34816, launch_tsm_atb_f64_2x32_t2x8_r64_p2, 1},
+  {"tsm_atb_f64_2x64_t2x8_r32_p2", 0, 2, 64, 2, 8, 32, 2, 0, 33792, launch_tsm_atb_f64_2x64_t2x8_r32_p2, 1},
+  {"tsm_atb_f64_4x1_t4x1_r512_p2", 0, 4, 1, 4, 1, 512, 2, 0, 40960, launch_tsm_atb_f64_4x1_t4x1_r512_p2, 1},
+  {"tsm_atb_f64_4x2_t4x2_r512_p2", 0, 4, 2, 4, 2, 512, 2, 0, 49152, launch_tsm_atb_f64_4x2_t4x2_r512_p2, 1},
+  {"tsm_atb_f64_4x4_t2x2_r512_p1", 0, 4, 4, 2, 2, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p1, 1},
+  {"tsm_atb_f64_4x4_t2x2_r512_p2", 0, 4, 4, 2, 2, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p2, 1},
+  {"tsm_atb_f64_4x4_t4x4_r256_p1", 0, 4, 4, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1, 1},
+  {"tsm_atb_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r256_p2", 0, 4, 4, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2, 1},
+  {"tsm_atb_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r512_p1", 0, 4, 4, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1, 1},
+  {"tsm_atb_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r512_p2", 0, 4, 4, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2, 1},
+  {"tsm_atb_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r1024_p1", 0, 4, 4, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1, 1},
+  {"tsm_atb_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r1024_p2", 0, 4, 4, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2, 1},
+  {"tsm_atb_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_f64_4x8_t4x8_r256_p2", 0, 4, 8, 4, 8, 256, 2, 0, 49152, launch_tsm_atb_f64_4x8_t4x8_r256_p2, 1},
+  {"tsm_atb_f64_4x16_t4x8_r128_p2", 0, 4, 16, 4, 8, 128, 2, 0, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r32_p1", 0, 4, 32, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p1, 1},
+  {"tsm_atb_f64_4x32_t4x4_r32_p2", 0, 4, 32, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r64_p1", 0, 4, 32, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p1, 1},
+  {"tsm_atb_f64_4x32_t4x4_r64_p2", 0, 4, 32, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r128_p1", 0, 4, 32, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p1, 1},
+  {"tsm_atb_f64_4x32_t4x4_r128_p2", 0, 4, 32, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_4x32_t4x8_r64_p1", 0, 4, 32, 4, 8, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1, 1},
+  {"tsm_atb_f64_4x32_t4x8_r64_p1_nt", 0, 4, 32, 4, 8, 64, 1, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1_nt, 1},
+  {"tsm_atb_f64_4x32_t4x8_r64_p2", 0, 4, 32, 4, 8, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2, 1},
+  {"tsm_atb_f64_4x32_t4x8_r64_p2_nt", 0, 4, 32, 4, 8, 64, 2, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_4x32_t4x8_r128_p1", 0, 4, 32, 4, 8, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1, 1},
+  {"tsm_atb_f64_4x32_t4x8_r128_p1_nt", 0, 4, 32, 4, 8, 128, 1, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt, 1},
+  {"tsm_atb_f64_4x32_t4x8_r128_p2", 0, 4, 32, 4, 8, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_4x32_t4x8_r128_p2_nt", 0, 4, 32, 4, 8, 128, 2, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_4x64_t4x8_r32_p2", 0, 4, 64, 4, 8, 32, 2, 0, 34816, launch_tsm_atb_f64_4x64_t4x8_r32_p2, 1},
+  {"tsm_atb_f64_8x1_t8x1_r256_p2", 0, 8, 1, 8, 1, 256, 2, 0, 36864, launch_tsm_atb_f64_8x1_t8x1_r256_p2, 1},
+  {"tsm_atb_f64_8x2_t8x2_r256_p2", 0, 8, 2, 8, 2, 256, 2, 0, 40960, launch_tsm_atb_f64_8x2_t8x2_r256_p2, 1},
+  {"tsm_atb_f64_8x4_t8x4_r256_p2", 0, 8, 4, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f64_8x4_t8x4_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t2x2_r256_p1", 0, 8, 8, 2, 2, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p1, 1},
+  {"tsm_atb_f64_8x8_t2x2_r256_p2", 0, 8, 8, 2, 2, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r128_p1", 0, 8, 8, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p1, 1},
+  {"tsm_atb_f64_8x8_t4x4_r128_p2", 0, 8, 8, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r256_p1", 0, 8, 8, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p1, 1},
+  {"tsm_atb_f64_8x8_t4x4_r256_p2", 0, 8, 8, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r512_p1", 0, 8, 8, 4, 4, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p1, 1},
+  {"tsm_atb_f64_8x8_t4x4_r512_p2", 0, 8, 8, 4, 4, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_f64_8x8_t8x8_r256_p1", 0, 8, 8, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1, 1},
+  {"tsm_atb_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1_nt, 1},
+  {"tsm_atb_f64_8x8_t8x8_r256_p2", 0, 8, 8, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t8x8_r256_p2_nt", 0, 8, 8, 8, 8, 256, 2, 1, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t8x8_r512_p1", 0, 8, 8, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p1, 1},
+  {"tsm_atb_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p1_nt, 1},
+  {"tsm_atb_f64_8x8_t8x8_r512_p2", 0, 8, 8, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2, 1},
+  {"tsm_atb_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt, 1},
+  {"tsm_atb_f64_8x16_t8x8_r128_p2", 0, 8, 16, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_8x16_t8x8_r128_p2, 1},
+  {"tsm_atb_f64_8x32_t8x8_r64_p2", 0, 8, 32, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_8x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_8x64_t8x8_r32_p2", 0, 8, 64, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_8x64_t8x8_r32_p2, 1},
+  {"tsm_atb_f64_16x1_t8x1_r128_p2", 0, 16, 1, 8, 1, 128, 2, 0, 34816, launch_tsm_atb_f64_16x1_t8x1_r128_p2, 1},
+  {"tsm_atb_f64_16x2_t8x2_r128_p2", 0, 16, 2, 8, 2, 128, 2, 0, 36864, launch_tsm_atb_f64_16x2_t8x2_r128_p2, 1},
+  {"tsm_atb_f64_16x4_t8x4_r128_p2", 0, 16, 4, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f64_16x4_t8x4_r128_p2, 1},
+  {"tsm_atb_f64_16x8_t8x8_r128_p2", 0, 16, 8, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_16x8_t8x8_r128_p2, 1},
+  {"tsm_atb_f64_16x16_direct_d4_nt", 0, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_atb_f64_16x16_direct_d4_nt, 3},
+  {"tsm_atb_f64_16x16_direct_d8_nt", 0, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_atb_f64_16x16_direct_d8_nt, 3},
+  {"tsm_atb_f64_16x16_direct_d16_nt", 0, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_atb_f64_16x16_direct_d16_nt, 3},
+  {"tsm_atb_f64_16x16_mfma_r64_p1", 0, 16, 16, 0, 0, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p1, 2},
+  {"tsm_atb_f64_16x16_mfma_r64_p1_nt", 0, 16, 16, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_16x16_mfma_r64_p2", 0, 16, 16, 0, 0, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p2, 2},
+  {"tsm_atb_f64_16x16_mfma_r64_p2_nt", 0, 16, 16, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_16x16_mfma_r128_p1", 0, 16, 16, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p1, 2},
+  {"tsm_atb_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_16x16_mfma_r128_p2", 0, 16, 16, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p2, 2},
+  {"tsm_atb_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_16x16_mfma_r256_p1", 0, 16, 16, 0, 0, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p1, 2},
+  {"tsm_atb_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f64_16x16_mfma_r256_p2", 0, 16, 16, 0, 0, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p2, 2},
+  {"tsm_atb_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_atb_f64_16x16_t2x2_r64_p1", 0, 16, 16, 2, 2, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t2x2_r64_p1, 1},
+  {"tsm_atb_f64_16x16_t2x2_r64_p2", 0, 16, 16, 2, 2, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t2x2_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r64_p1", 0, 16, 16, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p1, 1},
+  {"tsm_atb_f64_16x16_t4x4_r64_p2", 0, 16, 16, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r128_p1", 0, 16, 16, 4, 4, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p1, 1},
+  {"tsm_atb_f64_16x16_t4x4_r128_p2", 0, 16, 16, 4, 4, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r256_p1", 0, 16, 16, 4, 4, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p1, 1},
+  {"tsm_atb_f64_16x16_t4x4_r256_p2", 0, 16, 16, 4, 4, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_f64_16x16_t8x8_r64_p1", 0, 16, 16, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1, 1},
+  {"tsm_atb_f64_16x16_t8x8_r64_p1_nt", 0, 16, 16, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1_nt, 1},
+  {"tsm_atb_f64_16x16_t8x8_r64_p2", 0, 16, 16, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t8x8_r64_p2_nt", 0, 16, 16, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t8x8_r128_p1", 0, 16, 16, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p1, 1},
+  {"tsm_atb_f64_16x16_t8x8_r128_p1_nt", 0, 16, 16, 8, 8, 128, 1, 1, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p1_nt, 1},
+  {"tsm_atb_f64_16x16_t8x8_r128_p2", 0, 16, 16, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p2, 1},
+  {"tsm_atb_f64_16x16_t8x8_r128_p2_nt", 0, 16, 16, 8, 8, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_t8x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t8x8_r256_p1", 0, 16, 16, 8, 8, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p1, 1},
+  {"tsm_atb_f64_16x16_t8x8_r256_p1_nt", 0, 16, 16, 8, 8, 256, 1, 1, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p1_nt, 1},
+  {"tsm_atb_f64_16x16_t8x8_r256_p2", 0, 16, 16, 8, 8, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2, 1},
+  {"tsm_atb_f64_16x16_t8x8_r256_p2_nt", 0, 16, 16, 8, 8, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_16x32_direct_d8_nt", 0, 16, 32, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f64_16x32_direct_d8_nt, 3},
+  {"tsm_atb_f64_16x32_t8x8_r64_p2", 0, 16, 32, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_16x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_16x64_direct_d8_nt", 0, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_16x64_direct_d8_nt, 3},
+  {"tsm_atb_f64_16x64_mfma_r32_p2", 0, 16, 64, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_16x64_mfma_r32_p2, 2},
+  {"tsm_atb_f64_32x1_t8x1_r64_p2", 0, 32, 1, 8, 1, 64, 2, 0, 33792, launch_tsm_atb_f64_32x1_t8x1_r64_p2, 1},
+  {"tsm_atb_f64_32x2_t8x2_r64_p2", 0, 32, 2, 8, 2, 64, 2, 0, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r32_p1", 0, 32, 4, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p1, 1},
+  {"tsm_atb_f64_32x4_t4x4_r32_p2", 0, 32, 4, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r64_p1", 0, 32, 4, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p1, 1},
+  {"tsm_atb_f64_32x4_t4x4_r64_p2", 0, 32, 4, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r128_p1", 0, 32, 4, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p1, 1},
+  {"tsm_atb_f64_32x4_t4x4_r128_p2", 0, 32, 4, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_32x4_t8x4_r64_p1", 0, 32, 4, 8, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1, 1},
+  {"tsm_atb_f64_32x4_t8x4_r64_p1_nt", 0, 32, 4, 8, 4, 64, 1, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1_nt, 1},
+  {"tsm_atb_f64_32x4_t8x4_r64_p2", 0, 32, 4, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2, 1},
+  {"tsm_atb_f64_32x4_t8x4_r64_p2_nt", 0, 32, 4, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x4_t8x4_r128_p1", 0, 32, 4, 8, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1, 1},
+  {"tsm_atb_f64_32x4_t8x4_r128_p1_nt", 0, 32, 4, 8, 4, 128, 1, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt, 1},
+  {"tsm_atb_f64_32x4_t8x4_r128_p2", 0, 32, 4, 8, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2, 1},
+  {"tsm_atb_f64_32x4_t8x4_r128_p2_nt", 0, 32, 4, 8, 4, 128, 2, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_32x8_t8x8_r64_p2", 0, 32, 8, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_32x8_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_32x16_direct_d8_nt", 0, 32, 16, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f64_32x16_direct_d8_nt, 3},
+  {"tsm_atb_f64_32x16_t8x8_r64_p2", 0, 32, 16, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_32x16_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_32x32_direct_d4_nt", 0, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_atb_f64_32x32_direct_d4_nt, 3},
+  {"tsm_atb_f64_32x32_direct_d8_nt", 0, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_32x32_direct_d8_nt, 3},
+  {"tsm_atb_f64_32x32_direct_d16_nt", 0, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_atb_f64_32x32_direct_d16_nt, 3},
+  {"tsm_atb_f64_32x32_mfma_r32_p1", 0, 32, 32, 0, 0, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p1, 2},
+  {"tsm_atb_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_32x32_mfma_r32_p2", 0, 32, 32, 0, 0, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p2, 2},
+  {"tsm_atb_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_32x32_mfma_r64_p1", 0, 32, 32, 0, 0, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p1, 2},
+  {"tsm_atb_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_32x32_mfma_r64_p2", 0, 32, 32, 0, 0, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p2, 2},
+  {"tsm_atb_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_32x32_mfma_r128_p1", 0, 32, 32, 0, 0, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p1, 2},
+  {"tsm_atb_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_32x32_mfma_r128_p2", 0, 32, 32, 0, 0, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p2, 2},
+  {"tsm_atb_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_32x32_t4x4_r32_p1", 0, 32, 32, 4, 4, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p1, 1},
+  {"tsm_atb_f64_32x32_t4x4_r32_p2", 0, 32, 32, 4, 4, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_32x32_t4x4_r64_p1", 0, 32, 32, 4, 4, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p1, 1},
+  {"tsm_atb_f64_32x32_t4x4_r64_p2", 0, 32, 32, 4, 4, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_32x32_t8x8_r32_p1", 0, 32, 32, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1, 1},
+  {"tsm_atb_f64_32x32_t8x8_r32_p1_nt", 0, 32, 32, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1_nt, 1},
+  {"tsm_atb_f64_32x32_t8x8_r32_p2", 0, 32, 32, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p2, 1},
+  {"tsm_atb_f64_32x32_t8x8_r32_p2_nt", 0, 32, 32, 8, 8, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t8x8_r64_p1", 0, 32, 32, 8, 8, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p1, 1},
+  {"tsm_atb_f64_32x32_t8x8_r64_p1_nt", 0, 32, 32, 8, 8, 64, 1, 1, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p1_nt, 1},
+  {"tsm_atb_f64_32x32_t8x8_r64_p2", 0, 32, 32, 8, 8, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_32x32_t8x8_r64_p2_nt", 0, 32, 32, 8, 8, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_t8x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t8x8_r128_p1", 0, 32, 32, 8, 8, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p1, 1},
+  {"tsm_atb_f64_32x32_t8x8_r128_p1_nt", 0, 32, 32, 8, 8, 128, 1, 1, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p1_nt, 1},
+  {"tsm_atb_f64_32x32_t8x8_r128_p2", 0, 32, 32, 8, 8, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2, 1},
+  {"tsm_atb_f64_32x32_t8x8_r128_p2_nt", 0, 32, 32, 8, 8, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_32x64_direct_d8_nt", 0, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f64_32x64_direct_d8_nt, 3},
+  {"tsm_atb_f64_32x64_mfma_r32_p2", 0, 32, 64, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_32x64_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x1_t8x1_r32_p2", 0, 64, 1, 8, 1, 32, 2, 0, 33280, launch_tsm_atb_f64_64x1_t8x1_r32_p2, 1},
+  {"tsm_atb_f64_64x2_t8x2_r32_p2", 0, 64, 2, 8, 2, 32, 2, 0, 33792, launch_tsm_atb_f64_64x2_t8x2_r32_p2, 1},
+  {"tsm_atb_f64_64x4_t8x4_r32_p2", 0, 64, 4, 8, 4, 32, 2, 0, 34816, launch_tsm_atb_f64_64x4_t8x4_r32_p2, 1},
+  {"tsm_atb_f64_64x8_t8x8_r32_p2", 0, 64, 8, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_64x8_t8x8_r32_p2, 1},
+  {"tsm_atb_f64_64x16_direct_d8_nt", 0, 64, 16, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_64x16_direct_d8_nt, 3},
+  {"tsm_atb_f64_64x16_mfma_r32_p2", 0, 64, 16, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_64x16_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x32_direct_d8_nt", 0, 64, 32, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f64_64x32_direct_d8_nt, 3},
+  {"tsm_atb_f64_64x32_mfma_r32_p2", 0, 64, 32, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_64x32_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x64_direct_d4_nt", 0, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_atb_f64_64x64_direct_d4_nt, 3},
+  {"tsm_atb_f64_64x64_direct_d8_nt", 0, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_atb_f64_64x64_direct_d8_nt, 3},
+  {"tsm_atb_f64_64x64_direct_d16_nt", 0, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_atb_f64_64x64_direct_d16_nt, 3},
+  {"tsm_atb_f64_64x64_mfma_r16_p1", 0, 64, 64, 0, 0, 16, 1, 0, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p1, 2},
+  {"tsm_atb_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p1_nt, 2},
+  {"tsm_atb_f64_64x64_mfma_r16_p2", 0, 64, 64, 0, 0, 16, 2, 0, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p2, 2},
+  {"tsm_atb_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_atb_f64_64x64_mfma_r16_p2_nt, 2},
+  {"tsm_atb_f64_64x64_mfma_r32_p1", 0, 64, 64, 0, 0, 32, 1, 0, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p1, 2},
+  {"tsm_atb_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_64x64_mfma_r32_p2", 0, 64, 64, 0, 0, 32, 2, 0, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 65536, launch_tsm_atb_f64_64x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_64x64_mfma_r64_p1", 0, 64, 64, 0, 0, 64, 1, 0, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p1, 2},
+  {"tsm_atb_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_64x64_mfma_r64_p2", 0, 64, 64, 0, 0, 64, 2, 0, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p2, 2},
+  {"tsm_atb_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_64x64_t8x8_r16_p1", 0, 64, 64, 8, 8, 16, 1, 0, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p1, 1},
+  {"tsm_atb_f64_64x64_t8x8_r16_p1_nt", 0, 64, 64, 8, 8, 16, 1, 1, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p1_nt, 1},
+  {"tsm_atb_f64_64x64_t8x8_r16_p2", 0, 64, 64, 8, 8, 16, 2, 0, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p2, 1},
+  {"tsm_atb_f64_64x64_t8x8_r16_p2_nt", 0, 64, 64, 8, 8, 16, 2, 1, 32768, launch_tsm_atb_f64_64x64_t8x8_r16_p2_nt, 1},
+  {"tsm_atb_f64_64x64_t8x8_r32_p1", 0, 64, 64, 8, 8, 32, 1, 0, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p1, 1},
+  {"tsm_atb_f64_64x64_t8x8_r32_p1_nt", 0, 64, 64, 8, 8, 32, 1, 1, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p1_nt, 1},
+  {"tsm_atb_f64_64x64_t8x8_r32_p2", 0, 64, 64, 8, 8, 32, 2, 0, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p2, 1},
+  {"tsm_atb_f64_64x64_t8x8_r32_p2_nt", 0, 64, 64, 8, 8, 32, 2, 1, 65536, launch_tsm_atb_f64_64x64_t8x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x64_t8x8_r64_p1", 0, 64, 64, 8, 8, 64, 1, 0, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p1, 1},
+  {"tsm_atb_f64_64x64_t8x8_r64_p1_nt", 0, 64, 64, 8, 8, 64, 1, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p1_nt, 1},
+  {"tsm_atb_f64_64x64_t8x8_r64_p2", 0, 64, 64, 8, 8, 64, 2, 0, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_64x64_t8x8_r64_p2_nt", 0, 64, 64, 8, 8, 64, 2, 1, 131072, launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt, 1},
 };
 
 static const tsm::KernelEntry kSym[] = {
-  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2},
-  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2},
-  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4, 4, 4, 4096, 2, 0, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2},
-  {"tsm_ata_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt},
-  {"tsm_ata_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt},
-  {"tsm_ata_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt},
-  {"tsm_ata_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt},
-  {"tsm_ata_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt},
-  {"tsm_ata_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt},
-  {"tsm_ata_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt},
-  {"tsm_ata_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt},
-  {"tsm_ata_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt},
-  {"tsm_ata_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt},
-  {"tsm_ata_bf16_16x16_mfma_r2048_p1_nt", 2, 16, 16, 0, 0, 2048, 1, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt},
-  {"tsm_ata_bf16_16x16_mfma_r2048_p2_nt", 2, 16, 16, 0, 0, 2048, 2, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt},
-  {"tsm_ata_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt},
-  {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt},
-  {"tsm_ata_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt},
-  {"tsm_ata_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p2_nt},
-  {"tsm_ata_bf16_32x32_mfma_r1024_p1_nt", 2, 32, 32, 0, 0, 1024, 1, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p1_nt},
-  {"tsm_ata_bf16_32x32_mfma_r1024_p2_nt", 2, 32, 32, 0, 0, 1024, 2, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p2_nt},
-  {"tsm_ata_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p1_nt},
-  {"tsm_ata_bf16_64x64_mfma_r128_p2_nt", 2, 64, 64, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p2_nt},
-  {"tsm_ata_bf16_64x64_mfma_r256_p1_nt", 2, 64, 64, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt},
-  {"tsm_ata_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt},
-  {"tsm_ata_bf16_64x64_mfma_r512_p1_nt", 2, 64, 64, 0, 0, 512, 1, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt},
-  {"tsm_ata_bf16_64x64_mfma_r512_p2_nt", 2, 64, 64, 0, 0, 512, 2, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt},
-  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2},
-  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2},
-  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2},
-  {"tsm_ata_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt},
-  {"tsm_ata_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt},
-  {"tsm_ata_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt},
-  {"tsm_ata_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt},
-  {"tsm_ata_f32_8x8_t8x4_r2048_p1_nt", 1, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt},
-  {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt},
-  {"tsm_ata_f32_16x16_mfma_r64_p1_nt", 1, 16, 16, 0, 0, 64, 1, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p1_nt},
-  {"tsm_ata_f32_16x16_mfma_r64_p2_nt", 1, 16, 16, 0, 0, 64, 2, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p2_nt},
-  {"tsm_ata_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p1_nt},
-  {"tsm_ata_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p2_nt},
-  {"tsm_ata_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p1_nt},
-  {"tsm_ata_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p2_nt},
-  {"tsm_ata_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p1_nt},
-  {"tsm_ata_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p2_nt},
-  {"tsm_ata_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p1_nt},
-  {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt},
-  {"tsm_ata_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p1_nt},
-  {"tsm_ata_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p2_nt},
-  {"tsm_ata_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p1_nt},
-  {"tsm_ata_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p2_nt},
-  {"tsm_ata_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p1_nt},
-  {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt},
-  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt},
-  {"tsm_ata_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p2_nt},
-  {"tsm_ata_f32_64x64_mfma_r256_p1_nt", 1, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p1_nt},
-  {"tsm_ata_f32_64x64_mfma_r256_p2_nt", 1, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p2_nt},
-  {"tsm_ata_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt},
-  {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt},
-  {"tsm_ata_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt},
-  {"tsm_ata_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p2_nt},
-  {"tsm_ata_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p1_nt},
-  {"tsm_ata_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt},
-  {"tsm_ata_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p1_nt},
-  {"tsm_ata_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p2_nt},
-  {"tsm_ata_f64_2x2_t2x2_r4096_p1_nt", 0, 2, 2, 2, 2, 4096, 1, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p1_nt},
-  {"tsm_ata_f64_2x2_t2x2_r4096_p2_nt", 0, 2, 2, 2, 2, 4096, 2, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p2_nt},
-  {"tsm_ata_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p1_nt},
-  {"tsm_ata_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p2_nt},
-  {"tsm_ata_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt},
-  {"tsm_ata_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt},
-  {"tsm_ata_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt},
-  {"tsm_ata_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt},
-  {"tsm_ata_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p1_nt},
-  {"tsm_ata_f64_8x8_t8x8_r256_p2_nt", 0, 8, 8, 8, 8, 256, 2, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p2_nt},
-  {"tsm_ata_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt},
-  {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt},
-  {"tsm_ata_f64_8x8_t8x8_r1024_p1_nt", 0, 8, 8, 8, 8, 1024, 1, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt},
-  {"tsm_ata_f64_8x8_t8x8_r1024_p2_nt", 0, 8, 8, 8, 8, 1024, 2, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt},
-  {"tsm_ata_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p1_nt},
-  {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt},
-  {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt},
-  {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt},
-  {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt},
-  {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt},
-  {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt},
-  {"tsm_ata_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p2_nt},
-  {"tsm_ata_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p1_nt},
-  {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt},
-  {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt},
-  {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt},
-  {"tsm_ata_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p1_nt},
-  {"tsm_ata_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p2_nt},
-  {"tsm_ata_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p1_nt},
-  {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt},
-  {"tsm_ata_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p1_nt},
-  {"tsm_ata_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p2_nt},
-  {"tsm_ata_f64_64x64_mfma_r128_p1_nt", 0, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p1_nt},
-  {"tsm_ata_f64_64x64_mfma_r128_p2_nt", 0, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p2_nt},
+  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2, 1},
+  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2, 1},
+  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4, 4, 4, 4096, 2, 0, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt, 1},
+  {"tsm_ata_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt, 1},
+  {"tsm_ata_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt, 2},
+  {"tsm_ata_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt, 2},
+  {"tsm_ata_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt, 2},
+  {"tsm_ata_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt, 2},
+  {"tsm_ata_bf16_16x16_mfma_r2048_p1_nt", 2, 16, 16, 0, 0, 2048, 1, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt, 2},
+  {"tsm_ata_bf16_16x16_mfma_r2048_p2_nt", 2, 16, 16, 0, 0, 2048, 2, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p2_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r1024_p1_nt", 2, 32, 32, 0, 0, 1024, 1, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p1_nt, 2},
+  {"tsm_ata_bf16_32x32_mfma_r1024_p2_nt", 2, 32, 32, 0, 0, 1024, 2, 1, 131072, launch_tsm_ata_bf16_32x32_mfma_r1024_p2_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p1_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r128_p2_nt", 2, 64, 64, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_bf16_64x64_mfma_r128_p2_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r256_p1_nt", 2, 64, 64, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r512_p1_nt", 2, 64, 64, 0, 0, 512, 1, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt, 2},
+  {"tsm_ata_bf16_64x64_mfma_r512_p2_nt", 2, 64, 64, 0, 0, 512, 2, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt, 2},
+  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2, 1},
+  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2, 1},
+  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2, 1},
+  {"tsm_ata_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt, 1},
+  {"tsm_ata_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_ata_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt, 1},
+  {"tsm_ata_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt, 1},
+  {"tsm_ata_f32_8x8_t8x4_r2048_p1_nt", 1, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt, 1},
+  {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt, 1},
+  {"tsm_ata_f32_16x16_mfma_r64_p1_nt", 1, 16, 16, 0, 0, 64, 1, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f32_16x16_mfma_r64_p2_nt", 1, 16, 16, 0, 0, 64, 2, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p2_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r256_p1_nt", 1, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r256_p2_nt", 1, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt, 1},
+  {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt, 1},
+  {"tsm_ata_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt, 1},
+  {"tsm_ata_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p1_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 32768, launch_tsm_ata_f64_2x2_t2x2_r1024_p2_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p1_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 65536, launch_tsm_ata_f64_2x2_t2x2_r2048_p2_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r4096_p1_nt", 0, 2, 2, 2, 2, 4096, 1, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p1_nt, 1},
+  {"tsm_ata_f64_2x2_t2x2_r4096_p2_nt", 0, 2, 2, 2, 2, 4096, 2, 1, 131072, launch_tsm_ata_f64_2x2_t2x2_r4096_p2_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p1_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 16384, launch_tsm_ata_f64_4x4_t4x4_r256_p2_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt, 1},
+  {"tsm_ata_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p1_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r256_p2_nt", 0, 8, 8, 8, 8, 256, 2, 1, 32768, launch_tsm_ata_f64_8x8_t8x8_r256_p2_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r1024_p1_nt", 0, 8, 8, 8, 8, 1024, 1, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt, 1},
+  {"tsm_ata_f64_8x8_t8x8_r1024_p2_nt", 0, 8, 8, 8, 8, 1024, 2, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt, 1},
+  {"tsm_ata_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p1_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p2_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r128_p1_nt", 0, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f64_64x64_mfma_r128_p2_nt", 0, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p2_nt, 2},
 };
 
 static const tsm::TunedEntry kSymTuned[] = {
-  {2, 1, 1, 0, 2, 0},  // tsm_ata_bf16_1x1_t1x1_r4096_p2
-  {2, 2, 2, 1, 2, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
-  {2, 4, 4, 2, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
-  {2, 8, 8, 6, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p2_nt
-  {2, 16, 16, 10, 1, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
-  {2, 32, 32, 16, 1, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
-  {2, 64, 64, 26, 1, 0},  // tsm_ata_bf16_64x64_mfma_r512_p2_nt
-  {1, 1, 1, 27, 2, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
-  {1, 2, 2, 28, 2, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
-  {1, 4, 4, 29, 2, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
-  {1, 8, 8, 32, 2, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
-  {1, 16, 16, 40, 2, 1},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
-  {1, 32, 32, 47, 1, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
-  {1, 64, 64, 53, 2, 0},  // tsm_ata_f32_64x64_mfma_r128_p2_nt
-  {0, 1, 1, 56, 2, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
-  {0, 2, 2, 61, 1, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
-  {0, 4, 4, 69, 1, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 75, 1, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
-  {0, 16, 16, 79, 1, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 89, 1, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
-  {0, 64, 64, 95, 2, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
+  {2, 1, 1, 0, 2, 0, 0, 0},  // tsm_ata_bf16_1x1_t1x1_r4096_p2
+  {2, 2, 2, 1, 2, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
+  {2, 4, 4, 2, 2, 0, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
+  {2, 8, 8, 6, 6, 1, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p2_nt
+  {2, 16, 16, 10, 1, 1, 10, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
+  {2, 32, 32, 16, 1, 1, 16, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
+  {2, 64, 64, 26, 1, 0, 26, 0},  // tsm_ata_bf16_64x64_mfma_r512_p2_nt
+  {1, 1, 1, 27, 2, 0, 27, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
+  {1, 2, 2, 28, 2, 0, 28, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
+  {1, 4, 4, 29, 2, 0, 29, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
+  {1, 8, 8, 32, 2, 0, 32, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
+  {1, 16, 16, 40, 2, 1, 40, 1},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
+  {1, 32, 32, 47, 1, 1, 47, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
+  {1, 64, 64, 53, 2, 0, 53, 0},  // tsm_ata_f32_64x64_mfma_r128_p2_nt
+  {0, 1, 1, 56, 2, 1, 56, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
+  {0, 2, 2, 61, 1, 1, 61, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
+  {0, 4, 4, 69, 1, 1, 69, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 75, 1, 0, 75, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
+  {0, 16, 16, 79, 1, 1, 79, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
+  {0, 32, 32, 89, 1, 1, 89, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
+  {0, 64, 64, 95, 2, 0, 95, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
 };
 
 static const tsm::TunedEntry kTuned[] = {
-  {2, 1, 1, 0, 2, 0},  // tsm_atb_bf16_1x1_t1x1_r4096_p2
-  {2, 1, 2, 1, 2, 0},  // tsm_atb_bf16_1x2_t1x2_r4096_p2
-  {2, 1, 4, 2, 2, 0},  // tsm_atb_bf16_1x4_t1x4_r2048_p2
-  {2, 1, 8, 3, 2, 0},  // tsm_atb_bf16_1x8_t1x4_r1024_p2
-  {2, 1, 16, 4, 2, 0},  // tsm_atb_bf16_1x16_t1x4_r512_p2
-  {2, 1, 32, 5, 2, 0},  // tsm_atb_bf16_1x32_t1x4_r256_p2
-  {2, 1, 64, 6, 2, 0},  // tsm_atb_bf16_1x64_t1x4_r128_p2
-  {2, 2, 1, 7, 2, 0},  // tsm_atb_bf16_2x1_t2x1_r4096_p2
-  {2, 2, 2, 8, 2, 0},  // tsm_atb_bf16_2x2_t2x2_r4096_p2
-  {2, 2, 4, 9, 2, 0},  // tsm_atb_bf16_2x4_t2x4_r2048_p2
-  {2, 2, 8, 10, 2, 0},  // tsm_atb_bf16_2x8_t2x4_r1024_p2
-  {2, 2, 16, 11, 2, 0},  // tsm_atb_bf16_2x16_t2x4_r512_p2
-  {2, 2, 32, 12, 2, 0},  // tsm_atb_bf16_2x32_t2x4_r256_p2
-  {2, 2, 64, 13, 2, 0},  // tsm_atb_bf16_2x64_t2x4_r128_p2
-  {2, 4, 1, 14, 2, 0},  // tsm_atb_bf16_4x1_t4x1_r2048_p2
-  {2, 4, 2, 15, 2, 0},  // tsm_atb_bf16_4x2_t4x2_r2048_p2
-  {2, 4, 4, 16, 2, 0},  // tsm_atb_bf16_4x4_t4x4_r2048_p2
-  {2, 4, 8, 17, 2, 0},  // tsm_atb_bf16_4x8_t4x4_r1024_p2
-  {2, 4, 16, 18, 2, 0},  // tsm_atb_bf16_4x16_t4x4_r512_p2
-  {2, 4, 32, 19, 2, 0},  // tsm_atb_bf16_4x32_t4x4_r256_p2
-  {2, 4, 64, 20, 2, 0},  // tsm_atb_bf16_4x64_t4x4_r128_p2
-  {2, 8, 1, 21, 2, 0},  // tsm_atb_bf16_8x1_t8x1_r1024_p2
-  {2, 8, 2, 22, 2, 0},  // tsm_atb_bf16_8x2_t8x2_r1024_p2
-  {2, 8, 4, 23, 2, 0},  // tsm_atb_bf16_8x4_t8x4_r1024_p2
-  {2, 8, 8, 33, 2, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 8, 16, 46, 2, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2
-  {2, 8, 32, 47, 2, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2
-  {2, 8, 64, 48, 2, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2
-  {2, 16, 1, 49, 2, 0},  // tsm_atb_bf16_16x1_t8x1_r512_p2
-  {2, 16, 2, 50, 2, 0},  // tsm_atb_bf16_16x2_t8x2_r512_p2
-  {2, 16, 4, 51, 2, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2
-  {2, 16, 8, 52, 2, 0},  // tsm_atb_bf16_16x8_t8x4_r512_p2
-  {2, 16, 16, 62, 1, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
-  {2, 16, 32, 87, 2, 0},  // tsm_atb_bf16_16x32_mfma_r256_p2
-  {2, 16, 64, 88, 2, 0},  // tsm_atb_bf16_16x64_mfma_r128_p2
-  {2, 32, 1, 89, 2, 0},  // tsm_atb_bf16_32x1_t8x1_r256_p2
-  {2, 32, 2, 90, 2, 0},  // tsm_atb_bf16_32x2_t8x2_r256_p2
-  {2, 32, 4, 91, 2, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
-  {2, 32, 8, 92, 2, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
-  {2, 32, 16, 93, 2, 0},  // tsm_atb_bf16_32x16_mfma_r256_p2
-  {2, 32, 32, 103, 1, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
-  {2, 32, 64, 128, 2, 0},  // tsm_atb_bf16_32x64_mfma_r128_p2
-  {2, 64, 1, 129, 2, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
-  {2, 64, 2, 130, 2, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
-  {2, 64, 4, 131, 2, 0},  // tsm_atb_bf16_64x4_t8x4_r128_p2
-  {2, 64, 8, 132, 2, 0},  // tsm_atb_bf16_64x8_t8x4_r128_p2
-  {2, 64, 16, 133, 2, 0},  // tsm_atb_bf16_64x16_mfma_r128_p2
-  {2, 64, 32, 134, 2, 0},  // tsm_atb_bf16_64x32_mfma_r128_p2
-  {2, 64, 64, 142, 1, 1},  // tsm_atb_bf16_64x64_mfma_r256_p2_nt
-  {1, 1, 1, 155, 2, 0},  // tsm_atb_f32_1x1_t1x1_r4096_p2
-  {1, 1, 2, 156, 2, 0},  // tsm_atb_f32_1x2_t1x2_r2048_p2
-  {1, 1, 4, 157, 2, 0},  // tsm_atb_f32_1x4_t1x4_r1024_p2
-  {1, 1, 8, 158, 2, 0},  // tsm_atb_f32_1x8_t1x4_r512_p2
-  {1, 1, 16, 159, 2, 0},  // tsm_atb_f32_1x16_t1x4_r256_p2
-  {1, 1, 32, 160, 2, 0},  // tsm_atb_f32_1x32_t1x4_r128_p2
-  {1, 1, 64, 161, 2, 0},  // tsm_atb_f32_1x64_t1x4_r64_p2
-  {1, 2, 1, 162, 2, 0},  // tsm_atb_f32_2x1_t2x1_r2048_p2
-  {1, 2, 2, 163, 2, 0},  // tsm_atb_f32_2x2_t2x2_r2048_p2
-  {1, 2, 4, 164, 2, 0},  // tsm_atb_f32_2x4_t2x4_r1024_p2
-  {1, 2, 8, 165, 2, 0},  // tsm_atb_f32_2x8_t2x4_r512_p2
-  {1, 2, 16, 166, 2, 0},  // tsm_atb_f32_2x16_t2x4_r256_p2
-  {1, 2, 32, 167, 2, 0},  // tsm_atb_f32_2x32_t2x4_r128_p2
-  {1, 2, 64, 168, 2, 0},  // tsm_atb_f32_2x64_t2x4_r64_p2
-  {1, 4, 1, 169, 2, 0},  // tsm_atb_f32_4x1_t4x1_r1024_p2
-  {1, 4, 2, 170, 2, 0},  // tsm_atb_f32_4x2_t4x2_r1024_p2
-  {1, 4, 4, 171, 2, 0},  // tsm_atb_f32_4x4_t4x4_r1024_p2
-  {1, 4, 8, 172, 2, 0},  // tsm_atb_f32_4x8_t4x4_r512_p2
-  {1, 4, 16, 173, 2, 0},  // tsm_atb_f32_4x16_t4x4_r256_p2
-  {1, 4, 32, 174, 2, 0},  // tsm_atb_f32_4x32_t4x4_r128_p2
-  {1, 4, 64, 175, 2, 0},  // tsm_atb_f32_4x64_t4x4_r64_p2
-  {1, 8, 1, 176, 2, 0},  // tsm_atb_f32_8x1_t8x1_r512_p2
-  {1, 8, 2, 177, 2, 0},  // tsm_atb_f32_8x2_t8x2_r512_p2
-  {1, 8, 4, 178, 2, 0},  // tsm_atb_f32_8x4_t8x4_r512_p2
-  {1, 8, 8, 196, 1, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
-  {1, 8, 16, 203, 2, 0},  // tsm_atb_f32_8x16_t8x4_r256_p2
-  {1, 8, 32, 204, 2, 0},  // tsm_atb_f32_8x32_t8x4_r128_p2
-  {1, 8, 64, 205, 2, 0},  // tsm_atb_f32_8x64_t8x4_r64_p2
-  {1, 16, 1, 206, 2, 0},  // tsm_atb_f32_16x1_t8x1_r256_p2
-  {1, 16, 2, 207, 2, 0},  // tsm_atb_f32_16x2_t8x2_r256_p2
-  {1, 16, 4, 208, 2, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2
-  {1, 16, 8, 209, 2, 0},  // tsm_atb_f32_16x8_t8x4_r256_p2
-  {1, 16, 16, 213, 1, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-  {1, 16, 32, 240, 2, 0},  // tsm_atb_f32_16x32_t8x4_r128_p2
-  {1, 16, 64, 241, 2, 0},  // tsm_atb_f32_16x64_mfma_r64_p2
-  {1, 32, 1, 242, 2, 0},  // tsm_atb_f32_32x1_t8x1_r128_p2
-  {1, 32, 2, 243, 2, 0},  // tsm_atb_f32_32x2_t8x2_r128_p2
-  {1, 32, 4, 244, 2, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
-  {1, 32, 8, 245, 2, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
-  {1, 32, 16, 246, 2, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
-  {1, 32, 32, 254, 1, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 281, 2, 0},  // tsm_atb_f32_32x64_mfma_r64_p2
-  {1, 64, 1, 282, 2, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
-  {1, 64, 2, 283, 2, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
-  {1, 64, 4, 284, 2, 0},  // tsm_atb_f32_64x4_t8x4_r64_p2
-  {1, 64, 8, 285, 2, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
-  {1, 64, 16, 286, 2, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
-  {1, 64, 32, 287, 2, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
-  {1, 64, 64, 295, 1, 0},  // tsm_atb_f32_64x64_mfma_r64_p2_nt
-  {0, 1, 1, 321, 1, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
-  {0, 1, 2, 324, 2, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
-  {0, 1, 4, 325, 2, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
-  {0, 1, 8, 326, 2, 0},  // tsm_atb_f64_1x8_t1x8_r256_p2
-  {0, 1, 16, 327, 2, 0},  // tsm_atb_f64_1x16_t1x8_r128_p2
-  {0, 1, 32, 328, 2, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
-  {0, 1, 64, 329, 2, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
-  {0, 2, 1, 330, 2, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
-  {0, 2, 2, 340, 1, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
-  {0, 2, 4, 343, 2, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
-  {0, 2, 8, 344, 2, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
-  {0, 2, 16, 345, 2, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
-  {0, 2, 32, 346, 2, 0},  // tsm_atb_f64_2x32_t2x8_r64_p2
-  {0, 2, 64, 347, 2, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
-  {0, 4, 1, 348, 2, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
-  {0, 4, 2, 349, 2, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
-  {0, 4, 4, 353, 2, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-  {0, 4, 8, 364, 2, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
-  {0, 4, 16, 365, 2, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
-  {0, 4, 32, 373, 2, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-  {0, 4, 64, 380, 2, 0},  // tsm_atb_f64_4x64_t4x8_r32_p2
-  {0, 8, 1, 381, 2, 0},  // tsm_atb_f64_8x1_t8x1_r256_p2
-  {0, 8, 2, 382, 2, 0},  // tsm_atb_f64_8x2_t8x2_r256_p2
-  {0, 8, 4, 383, 2, 0},  // tsm_atb_f64_8x4_t8x4_r256_p2
-  {0, 8, 8, 395, 1, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
-  {0, 8, 16, 400, 2, 0},  // tsm_atb_f64_8x16_t8x8_r128_p2
-  {0, 8, 32, 401, 2, 0},  // tsm_atb_f64_8x32_t8x8_r64_p2
-  {0, 8, 64, 402, 2, 0},  // tsm_atb_f64_8x64_t8x8_r32_p2
-  {0, 16, 1, 403, 2, 0},  // tsm_atb_f64_16x1_t8x1_r128_p2
-  {0, 16, 2, 404, 2, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
-  {0, 16, 4, 405, 2, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
-  {0, 16, 8, 406, 2, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
-  {0, 16, 16, 416, 1, 1},  // tsm_atb_f64_16x16_mfma_r256_p1_nt
-  {0, 16, 32, 439, 2, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
-  {0, 16, 64, 440, 2, 0},  // tsm_atb_f64_16x64_mfma_r32_p2
-  {0, 32, 1, 441, 2, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
-  {0, 32, 2, 442, 2, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
-  {0, 32, 4, 450, 2, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
-  {0, 32, 8, 457, 2, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
-  {0, 32, 16, 458, 2, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 462, 1, 1},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
-  {0, 32, 64, 487, 2, 0},  // tsm_atb_f64_32x64_mfma_r32_p2
-  {0, 64, 1, 488, 2, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
-  {0, 64, 2, 489, 2, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
-  {0, 64, 4, 490, 2, 0},  // tsm_atb_f64_64x4_t8x4_r32_p2
-  {0, 64, 8, 491, 2, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
-  {0, 64, 16, 492, 2, 0},  // tsm_atb_f64_64x16_mfma_r32_p2
-  {0, 64, 32, 493, 2, 0},  // tsm_atb_f64_64x32_mfma_r32_p2
-  {0, 64, 64, 501, 1, 0},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
+  {2, 1, 1, 0, 2, 0, 0, 0},  // tsm_atb_bf16_1x1_t1x1_r4096_p2
+  {2, 1, 2, 1, 2, 0, 1, 0},  // tsm_atb_bf16_1x2_t1x2_r4096_p2
+  {2, 1, 4, 2, 2, 0, 2, 0},  // tsm_atb_bf16_1x4_t1x4_r2048_p2
+  {2, 1, 8, 3, 2, 0, 3, 0},  // tsm_atb_bf16_1x8_t1x4_r1024_p2
+  {2, 1, 16, 4, 2, 0, 4, 0},  // tsm_atb_bf16_1x16_t1x4_r512_p2
+  {2, 1, 32, 5, 2, 0, 5, 0},  // tsm_atb_bf16_1x32_t1x4_r256_p2
+  {2, 1, 64, 6, 2, 0, 6, 0},  // tsm_atb_bf16_1x64_t1x4_r128_p2
+  {2, 2, 1, 7, 2, 0, 7, 0},  // tsm_atb_bf16_2x1_t2x1_r4096_p2
+  {2, 2, 2, 8, 2, 0, 8, 0},  // tsm_atb_bf16_2x2_t2x2_r4096_p2
+  {2, 2, 4, 9, 2, 0, 9, 0},  // tsm_atb_bf16_2x4_t2x4_r2048_p2
+  {2, 2, 8, 10, 2, 0, 10, 0},  // tsm_atb_bf16_2x8_t2x4_r1024_p2
+  {2, 2, 16, 11, 2, 0, 11, 0},  // tsm_atb_bf16_2x16_t2x4_r512_p2
+  {2, 2, 32, 12, 2, 0, 12, 0},  // tsm_atb_bf16_2x32_t2x4_r256_p2
+  {2, 2, 64, 13, 2, 0, 13, 0},  // tsm_atb_bf16_2x64_t2x4_r128_p2
+  {2, 4, 1, 14, 2, 0, 14, 0},  // tsm_atb_bf16_4x1_t4x1_r2048_p2
+  {2, 4, 2, 15, 2, 0, 15, 0},  // tsm_atb_bf16_4x2_t4x2_r2048_p2
+  {2, 4, 4, 16, 2, 0, 16, 0},  // tsm_atb_bf16_4x4_t4x4_r2048_p2
+  {2, 4, 8, 17, 2, 0, 17, 0},  // tsm_atb_bf16_4x8_t4x4_r1024_p2
+  {2, 4, 16, 18, 2, 0, 18, 0},  // tsm_atb_bf16_4x16_t4x4_r512_p2
+  {2, 4, 32, 19, 2, 0, 19, 0},  // tsm_atb_bf16_4x32_t4x4_r256_p2
+  {2, 4, 64, 20, 2, 0, 20, 0},  // tsm_atb_bf16_4x64_t4x4_r128_p2
+  {2, 8, 1, 21, 2, 0, 21, 0},  // tsm_atb_bf16_8x1_t8x1_r1024_p2
+  {2, 8, 2, 22, 2, 0, 22, 0},  // tsm_atb_bf16_8x2_t8x2_r1024_p2
+  {2, 8, 4, 23, 2, 0, 23, 0},  // tsm_atb_bf16_8x4_t8x4_r1024_p2
+  {2, 8, 8, 33, 2, 1, 33, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
+  {2, 8, 16, 46, 2, 0, 46, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2
+  {2, 8, 32, 47, 2, 0, 47, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2
+  {2, 8, 64, 48, 2, 0, 48, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2
+  {2, 16, 1, 49, 2, 0, 49, 0},  // tsm_atb_bf16_16x1_t8x1_r512_p2
+  {2, 16, 2, 50, 2, 0, 50, 0},  // tsm_atb_bf16_16x2_t8x2_r512_p2
+  {2, 16, 4, 51, 2, 0, 51, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2
+  {2, 16, 8, 52, 2, 0, 52, 0},  // tsm_atb_bf16_16x8_t8x4_r512_p2
+  {2, 16, 16, 62, 1, 1, 62, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
+  {2, 16, 32, 87, 2, 0, 87, 0},  // tsm_atb_bf16_16x32_mfma_r256_p2
+  {2, 16, 64, 88, 2, 0, 88, 0},  // tsm_atb_bf16_16x64_mfma_r128_p2
+  {2, 32, 1, 89, 2, 0, 89, 0},  // tsm_atb_bf16_32x1_t8x1_r256_p2
+  {2, 32, 2, 90, 2, 0, 90, 0},  // tsm_atb_bf16_32x2_t8x2_r256_p2
+  {2, 32, 4, 91, 2, 0, 91, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
+  {2, 32, 8, 92, 2, 0, 92, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
+  {2, 32, 16, 93, 2, 0, 93, 0},  // tsm_atb_bf16_32x16_mfma_r256_p2
+  {2, 32, 32, 103, 1, 1, 103, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
+  {2, 32, 64, 128, 2, 0, 128, 0},  // tsm_atb_bf16_32x64_mfma_r128_p2
+  {2, 64, 1, 129, 2, 0, 129, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
+  {2, 64, 2, 130, 2, 0, 130, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
+  {2, 64, 4, 131, 2, 0, 131, 0},  // tsm_atb_bf16_64x4_t8x4_r128_p2
+  {2, 64, 8, 132, 2, 0, 132, 0},  // tsm_atb_bf16_64x8_t8x4_r128_p2
+  {2, 64, 16, 133, 2, 0, 133, 0},  // tsm_atb_bf16_64x16_mfma_r128_p2
+  {2, 64, 32, 134, 2, 0, 134, 0},  // tsm_atb_bf16_64x32_mfma_r128_p2
+  {2, 64, 64, 142, 1, 1, 142, 1},  // tsm_atb_bf16_64x64_mfma_r256_p2_nt
+  {1, 1, 1, 155, 2, 0, 155, 0},  // tsm_atb_f32_1x1_t1x1_r4096_p2
+  {1, 1, 2, 156, 2, 0, 156, 0},  // tsm_atb_f32_1x2_t1x2_r2048_p2
+  {1, 1, 4, 157, 2, 0, 157, 0},  // tsm_atb_f32_1x4_t1x4_r1024_p2
+  {1, 1, 8, 158, 2, 0, 158, 0},  // tsm_atb_f32_1x8_t1x4_r512_p2
+  {1, 1, 16, 159, 2, 0, 159, 0},  // tsm_atb_f32_1x16_t1x4_r256_p2
+  {1, 1, 32, 160, 2, 0, 160, 0},  // tsm_atb_f32_1x32_t1x4_r128_p2
+  {1, 1, 64, 161, 2, 0, 161, 0},  // tsm_atb_f32_1x64_t1x4_r64_p2
+  {1, 2, 1, 162, 2, 0, 162, 0},  // tsm_atb_f32_2x1_t2x1_r2048_p2
+  {1, 2, 2, 163, 2, 0, 163, 0},  // tsm_atb_f32_2x2_t2x2_r2048_p2
+  {1, 2, 4, 164, 2, 0, 164, 0},  // tsm_atb_f32_2x4_t2x4_r1024_p2
+  {1, 2, 8, 165, 2, 0, 165, 0},  // tsm_atb_f32_2x8_t2x4_r512_p2
+  {1, 2, 16, 166, 2, 0, 166, 0},  // tsm_atb_f32_2x16_t2x4_r256_p2
+  {1, 2, 32, 167, 2, 0, 167, 0},  // tsm_atb_f32_2x32_t2x4_r128_p2
+  {1, 2, 64, 168, 2, 0, 168, 0},  // tsm_atb_f32_2x64_t2x4_r64_p2
+  {1, 4, 1, 169, 2, 0, 169, 0},  // tsm_atb_f32_4x1_t4x1_r1024_p2
+  {1, 4, 2, 170, 2, 0, 170, 0},  // tsm_atb_f32_4x2_t4x2_r1024_p2
+  {1, 4, 4, 171, 2, 0, 171, 0},  // tsm_atb_f32_4x4_t4x4_r1024_p2
+  {1, 4, 8, 172, 2, 0, 172, 0},  // tsm_atb_f32_4x8_t4x4_r512_p2
+  {1, 4, 16, 173, 2, 0, 173, 0},  // tsm_atb_f32_4x16_t4x4_r256_p2
+  {1, 4, 32, 174, 2, 0, 174, 0},  // tsm_atb_f32_4x32_t4x4_r128_p2
+  {1, 4, 64, 175, 2, 0, 175, 0},  // tsm_atb_f32_4x64_t4x4_r64_p2
+  {1, 8, 1, 176, 2, 0, 176, 0},  // tsm_atb_f32_8x1_t8x1_r512_p2
+  {1, 8, 2, 177, 2, 0, 177, 0},  // tsm_atb_f32_8x2_t8x2_r512_p2
+  {1, 8, 4, 178, 2, 0, 178, 0},  // tsm_atb_f32_8x4_t8x4_r512_p2
+  {1, 8, 8, 196, 1, 1, 196, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
+  {1, 8, 16, 203, 2, 0, 203, 0},  // tsm_atb_f32_8x16_t8x4_r256_p2
+  {1, 8, 32, 204, 2, 0, 204, 0},  // tsm_atb_f32_8x32_t8x4_r128_p2
+  {1, 8, 64, 205, 2, 0, 205, 0},  // tsm_atb_f32_8x64_t8x4_r64_p2
+  {1, 16, 1, 206, 2, 0, 206, 0},  // tsm_atb_f32_16x1_t8x1_r256_p2
+  {1, 16, 2, 207, 2, 0, 207, 0},  // tsm_atb_f32_16x2_t8x2_r256_p2
+  {1, 16, 4, 208, 2, 0, 208, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2
+  {1, 16, 8, 209, 2, 0, 209, 0},  // tsm_atb_f32_16x8_t8x4_r256_p2
+  {1, 16, 16, 216, 1, 1, 216, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
+  {1, 16, 32, 244, 2, 0, 244, 0},  // tsm_atb_f32_16x32_t8x4_r128_p2
+  {1, 16, 64, 246, 2, 0, 246, 0},  // tsm_atb_f32_16x64_mfma_r64_p2
+  {1, 32, 1, 247, 2, 0, 247, 0},  // tsm_atb_f32_32x1_t8x1_r128_p2
+  {1, 32, 2, 248, 2, 0, 248, 0},  // tsm_atb_f32_32x2_t8x2_r128_p2
+  {1, 32, 4, 249, 2, 0, 249, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
+  {1, 32, 8, 250, 2, 0, 250, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
+  {1, 32, 16, 252, 2, 0, 252, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
+  {1, 32, 32, 263, 1, 1, 263, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+  {1, 32, 64, 291, 2, 0, 291, 0},  // tsm_atb_f32_32x64_mfma_r64_p2
+  {1, 64, 1, 292, 2, 0, 292, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
+  {1, 64, 2, 293, 2, 0, 293, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
+  {1, 64, 4, 294, 2, 0, 294, 0},  // tsm_atb_f32_64x4_t8x4_r64_p2
+  {1, 64, 8, 295, 2, 0, 295, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
+  {1, 64, 16, 297, 2, 0, 297, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
+  {1, 64, 32, 299, 2, 0, 299, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
+  {1, 64, 64, 310, 1, 0, 310, 0},  // tsm_atb_f32_64x64_mfma_r64_p2_nt
+  {0, 1, 1, 336, 1, 1, 336, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
+  {0, 1, 2, 339, 2, 0, 339, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
+  {0, 1, 4, 340, 2, 0, 340, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
+  {0, 1, 8, 341, 2, 0, 341, 0},  // tsm_atb_f64_1x8_t1x8_r256_p2
+  {0, 1, 16, 342, 2, 0, 342, 0},  // tsm_atb_f64_1x16_t1x8_r128_p2
+  {0, 1, 32, 343, 2, 0, 343, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
+  {0, 1, 64, 344, 2, 0, 344, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
+  {0, 2, 1, 345, 2, 0, 345, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
+  {0, 2, 2, 355, 1, 1, 355, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
+  {0, 2, 4, 358, 2, 0, 358, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
+  {0, 2, 8, 359, 2, 0, 359, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
+  {0, 2, 16, 360, 2, 0, 360, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
+  {0, 2, 32, 361, 2, 0, 361, 0},  // tsm_atb_f64_2x32_t2x8_r64_p2
+  {0, 2, 64, 362, 2, 0, 362, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
+  {0, 4, 1, 363, 2, 0, 363, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
+  {0, 4, 2, 364, 2, 0, 364, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
+  {0, 4, 4, 368, 2, 1, 368, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
+  {0, 4, 8, 379, 2, 0, 379, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
+  {0, 4, 16, 380, 2, 0, 380, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
+  {0, 4, 32, 388, 2, 1, 388, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
+  {0, 4, 64, 395, 2, 0, 395, 0},  // tsm_atb_f64_4x64_t4x8_r32_p2
+  {0, 8, 1, 396, 2, 0, 396, 0},  // tsm_atb_f64_8x1_t8x1_r256_p2
+  {0, 8, 2, 397, 2, 0, 397, 0},  // tsm_atb_f64_8x2_t8x2_r256_p2
+  {0, 8, 4, 398, 2, 0, 398, 0},  // tsm_atb_f64_8x4_t8x4_r256_p2
+  {0, 8, 8, 410, 1, 1, 410, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
+  {0, 8, 16, 415, 2, 0, 415, 0},  // tsm_atb_f64_8x16_t8x8_r128_p2
+  {0, 8, 32, 416, 2, 0, 416, 0},  // tsm_atb_f64_8x32_t8x8_r64_p2
+  {0, 8, 64, 417, 2, 0, 417, 0},  // tsm_atb_f64_8x64_t8x8_r32_p2
+  {0, 16, 1, 418, 2, 0, 418, 0},  // tsm_atb_f64_16x1_t8x1_r128_p2
+  {0, 16, 2, 419, 2, 0, 419, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
+  {0, 16, 4, 420, 2, 0, 420, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
+  {0, 16, 8, 421, 2, 0, 421, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
+  {0, 16, 16, 434, 1, 1, 434, 1},  // tsm_atb_f64_16x16_mfma_r256_p1_nt
+  {0, 16, 32, 458, 2, 0, 458, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
+  {0, 16, 64, 460, 2, 0, 460, 0},  // tsm_atb_f64_16x64_mfma_r32_p2
+  {0, 32, 1, 461, 2, 0, 461, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
+  {0, 32, 2, 462, 2, 0, 462, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
+  {0, 32, 4, 470, 2, 1, 470, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
+  {0, 32, 8, 477, 2, 0, 477, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
+  {0, 32, 16, 479, 2, 0, 479, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
+  {0, 32, 32, 486, 1, 1, 486, 1},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
+  {0, 32, 64, 512, 2, 0, 512, 0},  // tsm_atb_f64_32x64_mfma_r32_p2
+  {0, 64, 1, 513, 2, 0, 513, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
+  {0, 64, 2, 514, 2, 0, 514, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
+  {0, 64, 4, 515, 2, 0, 515, 0},  // tsm_atb_f64_64x4_t8x4_r32_p2
+  {0, 64, 8, 516, 2, 0, 516, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
+  {0, 64, 16, 518, 2, 0, 518, 0},  // tsm_atb_f64_64x16_mfma_r32_p2
+  {0, 64, 32, 520, 2, 0, 520, 0},  // tsm_atb_f64_64x32_mfma_r32_p2
+  {0, 64, 64, 531, 1, 0, 531, 0},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);

@@ -64,6 +64,9 @@ const tsm::KernelEntry* table(int* count) { return tsm_gen_kernel_table(count); 
 struct Plan {
   int kernel = -1;
   const tsm::KernelEntry* entry = nullptr;
+  // Staged kernel for inputs the direct engine cannot read (== kernel otherwise).
+  int fallback = -1;
+  int fallback_interleave = 0;
   int64_t grid = 1;
   int64_t tiles_per_wg = 1;
   int interleave = 0;
@@ -93,7 +96,7 @@ Tables tables(bool sym) {
 
 tsm_status find_kernel(const Tables& tb, tsm_dtype in_type, int kc, int mc,
                        const tsm_kernel_config* cfg, int* index, int* wgs_per_cu,
-                       int* interleave) {
+                       int* interleave, int* fallback, int* fallback_interleave) {
   const int ntuned = tb.ntuned, nk = tb.nk;
   const tsm::TunedEntry* tuned = tb.tuned;
   const tsm::KernelEntry* ks = tb.ks;
@@ -103,6 +106,8 @@ tsm_status find_kernel(const Tables& tb, tsm_dtype in_type, int kc, int mc,
       def = tuned[i].kernel;
       def_wpc = tuned[i].wgs_per_cu;
       def_il = tuned[i].interleave;
+      *fallback = tuned[i].fallback;
+      *fallback_interleave = tuned[i].fallback_interleave;
     }
   if (def < 0) return TSM_ERR_NO_KERNEL;
   *wgs_per_cu = (cfg && cfg->wgs_per_cu > 0) ? cfg->wgs_per_cu : def_wpc;
@@ -120,9 +125,7 @@ tsm_status find_kernel(const Tables& tb, tsm_dtype in_type, int kc, int mc,
   for (int i = 0; i < nk; ++i) {
     const tsm::KernelEntry& e = ks[i];
     if (e.dtype != (int)in_type || e.kc != kc || e.mc != mc) continue;
-    const bool mfma = e.tm == 0;
-    if (cfg->engine == 1 && mfma) continue;
-    if (cfg->engine == 2 && !mfma) continue;
+    if (cfg->engine > 0 && e.engine != cfg->engine) continue;
     if (cfg->tm > 0 && e.tm != cfg->tm) continue;
     if (cfg->tn > 0 && e.tn != cfg->tn) continue;
     if (cfg->rows > 0 && e.rows != cfg->rows) continue;
@@ -156,16 +159,38 @@ tsm_status make_plan(tsm_dtype in_type, int64_t n, int k, int m, const tsm_kerne
   const int kc = width_class(k), mc = width_class(m);
   int wpc = 2, interleave = 0;
   const Tables tb = tables(sym);
-  tsm_status st = find_kernel(tb, in_type, kc, mc, cfg, &p->kernel, &wpc, &interleave);
+  tsm_status st = find_kernel(tb, in_type, kc, mc, cfg, &p->kernel, &wpc, &interleave,
+                              &p->fallback, &p->fallback_interleave);
   if (st != TSM_SUCCESS) return st;
   p->entry = &tb.ks[p->kernel];
+  if (p->entry->engine != 3) {
+    p->fallback = p->kernel;
+    p->fallback_interleave = interleave;
+  }
   return set_grid(p, n, p->entry->rows, wpc, interleave, cfg ? cfg->grid : 0, kc, mc);
 }
 
+// A direct-engine plan for inputs that are not dense: the class's staged
+// kernel on at most the same grid, so the workspace the plan asked for still
+// suffices.
+tsm_status use_fallback(Plan* p, int64_t n, bool sym) {
+  if (p->entry->engine != 3) return TSM_SUCCESS;
+  const Tables tb = tables(sym);
+  if (p->fallback < 0 || p->fallback >= tb.nk || tb.ks[p->fallback].engine == 3)
+    return TSM_ERR_NO_KERNEL;
+  const int64_t grid = p->grid;
+  p->kernel = p->fallback;
+  p->entry = &tb.ks[p->kernel];
+  return set_grid(p, n, p->entry->rows, 1, p->fallback_interleave, grid, p->entry->kc,
+                  p->entry->mc);
+}
+
 bool valid_config(const tsm_kernel_config* c) {
-  return !c || !(c->pf < 0 || c->pf > 2 || c->tm < 0 || c->tn < 0 || c->rows < 0 ||
+  // pf: 1 or 2 tiles in flight (staged kernels), 4, 8 or 16 steps (direct engine)
+  return !c || !((c->pf != 0 && c->pf != 1 && c->pf != 2 && c->pf != 4 && c->pf != 8 &&
+                  c->pf != 16) || c->tm < 0 || c->tn < 0 || c->rows < 0 ||
                  c->wgs_per_cu < 0 || c->grid < 0 || c->load_policy < 0 || c->load_policy > 2 ||
-                 c->tile_order < 0 || c->tile_order > 2 || c->engine < 0 || c->engine > 2);
+                 c->tile_order < 0 || c->tile_order > 2 || c->engine < 0 || c->engine > 3);
 }
 
 tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int64_t cfg_grid,
@@ -179,6 +204,24 @@ tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int6
     if (st != TSM_SUCCESS) return st;
     want = (int64_t)wpc * cus;
   }
+  p->kcmc = kc * mc;
+  // Up to 512 outputs the last workgroup adds the partials (finish_small);
+  // wider classes leave it to the finisher kernel, spread over the chip.
+  p->finish_in_kernel = p->kcmc <= 512;
+  if (p->entry && p->entry->engine == 3) {
+    // Direct engine: 4-row steps; each of the 4 waves of a workgroup takes
+    // q = tiles_per_wg steps (a multiple of the depth D), at least D when n
+    // allows; the kernel deals the rest one step at a time.
+    const int64_t F = n / 4, D = p->entry->pf;
+    const int64_t cap = F / (4 * D);
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    p->grid = want;
+    p->tiles_per_wg = F / (4 * want * D) * D;
+    p->interleave = 0;
+    p->workspace = kCounterBytes + (size_t)p->grid * p->kcmc * sizeof(double);
+    return TSM_SUCCESS;
+  }
   const int64_t tiles = n > 0 ? (n + T - 1) / T : 1;
   if (want > tiles) want = tiles;
   if (want < 1) want = 1;
@@ -191,7 +234,9 @@ tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int6
     p->grid = (tiles + p->tiles_per_wg - 1) / p->tiles_per_wg;
   }
   p->kcmc = kc * mc;
-  p->finish_in_kernel = p->kcmc <= 1024;
+  // Up to 512 outputs the last workgroup adds the partials (finish_small);
+  // wider classes leave it to the finisher kernel, spread over the chip.
+  p->finish_in_kernel = p->kcmc <= 512;
   p->workspace = kCounterBytes + (size_t)p->grid * p->kcmc * sizeof(double);
   return TSM_SUCCESS;
 }
@@ -339,9 +384,13 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   Plan p;
   st = make_plan(in_type, n, k, m, config, &p);
   if (st != TSM_SUCCESS) return st;
-  const tsm::KernelEntry& ke = *p.entry;
-  const bool dense = k == ke.kc && m == ke.mc && lda == k && ldb == m &&
+  const bool dense = k == p.entry->kc && m == p.entry->mc && lda == k && ldb == m &&
                      ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0;
+  if (!dense) {
+    st = use_fallback(&p, n, false);
+    if (st != TSM_SUCCESS) return st;
+  }
+  const tsm::KernelEntry& ke = *p.entry;
   return run_plan(p, ke.launch, ke.kc, ke.mc, out_type, n, k, m, A, lda, B, ldb, C, ldc,
                   accumulate, dense, workspace, workspace_bytes,
                   static_cast<hipStream_t>(stream));
@@ -383,7 +432,7 @@ tsm_status run_plan(const Plan& p, tsm::LaunchFn launch, int kc, int mc, tsm_dty
   hipError_t e = launch(a, (unsigned)p.grid, s);
   if (e != hipSuccess) return hip_fail(e);
   if (!p.finish_in_kernel) {
-    const unsigned fg = (unsigned)((p.kcmc + 63) / 64);
+    const unsigned fg = (unsigned)((p.kcmc + tsm::kFinishOutputs - 1) / tsm::kFinishOutputs);
     hipLaunchKernelGGL(tsm_atb_finish, dim3(fg), dim3(256), 0, s, a, (int)p.grid, kc, mc);
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e);
@@ -413,8 +462,12 @@ tsm_status tsm_ata_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, c
   Plan p;
   st = make_plan(in_type, n, k, k, config, &p, true);
   if (st != TSM_SUCCESS) return st;
+  const bool dense = k == p.entry->kc && lda == k && ((uintptr_t)A % 16) == 0;
+  if (!dense) {
+    st = use_fallback(&p, n, true);
+    if (st != TSM_SUCCESS) return st;
+  }
   const tsm::KernelEntry& e = *p.entry;
-  const bool dense = k == e.kc && lda == k && ((uintptr_t)A % 16) == 0;
   return run_plan(p, e.launch, e.kc, e.kc, out_type, n, k, k, A, lda, A, lda, C, ldc,
                   accumulate, dense, workspace, workspace_bytes,
                   static_cast<hipStream_t>(stream));
@@ -509,6 +562,7 @@ tsm_status tsm_kernel_info(int index, tsm_kernel_desc* out) {
   out->pf = e.pf;
   out->load_policy = e.nt ? 2 : 1;
   out->lds_bytes = e.lds_bytes;
+  out->engine = e.engine;
   return TSM_SUCCESS;
 }
 

@@ -392,13 +392,41 @@ __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, do
   }
 }
 
-// Finisher kernel body: workgroup b handles outputs [b*64, b*64+64).
+// Finisher kernel body for classes of more than 512 outputs (launched after
+// the streaming kernel, kFinishOutputs outputs per workgroup).  Thread t sums
+// output o0 + t % 16 over slice t / 16 of the G workgroup partials, in
+// workgroup order, with up to 16 independent loads in flight (each wave reads
+// four partial rows of 16 consecutive outputs); the 16 slice sums are then
+// added in slice order.
+constexpr int kFinishOutputs = 16;
 __device__ inline void finish_body(const AtbArgs& a, int G, int KC, int MC) {
+  constexpr int SLICES = kThreads / kFinishOutputs, U = 16;
   __shared__ double scratch[kThreads];
   const int KCMC = KC * MC;
-  const int o0 = blockIdx.x * 64;
-  const int o1 = o0 + 64 < KCMC ? o0 + 64 : KCMC;
-  if (o0 < KCMC) finish_outputs(a, G, KCMC, MC, o0, o1, scratch);
+  const int o0 = blockIdx.x * kFinishOutputs;
+  if (o0 >= KCMC) return;
+  const int oi = threadIdx.x % kFinishOutputs, sl = threadIdx.x / kFinishOutputs;
+  const int nout = KCMC - o0 < kFinishOutputs ? KCMC - o0 : kFinishOutputs;
+  double acc = 0.0;
+  if (oi < nout) {
+    const int g0 = (int)((int64_t)G * sl / SLICES), g1 = (int)((int64_t)G * (sl + 1) / SLICES);
+    const double* P = a.partials + o0 + oi;
+    for (int g = g0; g < g1; g += U) {
+      double v[U];
+#pragma unroll
+      for (int x = 0; x < U; ++x) v[x] = g + x < g1 ? P[(int64_t)(g + x) * KCMC] : 0.0;
+#pragma unroll
+      for (int x = 0; x < U; ++x) acc += v[x];
+    }
+  }
+  scratch[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < nout) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < SLICES; ++q) t += scratch[q * kFinishOutputs + threadIdx.x];
+    store_output(a, MC, o0 + threadIdx.x, t);
+  }
 }
 
 // ---- MFMA consumer (width classes that are multiples of 16) -------------------------
@@ -970,6 +998,234 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
   constexpr int KCMC = KC * MC;
   atb_partial<S, KC, MC, TM, TN, T, PF, NT, SYM>(a, smem, first, step, count, owns_tail, full,
                                                  tail_rows, a.partials + g * KCMC);
+  if (!a.finish) return;
+  if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
+  if constexpr (KCMC >= 2 && KCMC <= 512)
+    finish_small<KCMC>(a, (int)gridDim.x, MC, reinterpret_cast<double*>(smem));
+  else
+    finish_outputs(a, (int)gridDim.x, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
+  if (threadIdx.x == 0) __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- direct matrix-core engine ------------------------------------------------------
+// The FP64 and FP32 matrix instructions (16x16x4) take, per lane l, one
+// element of row l / 16 of a 4-row K-step; the 16 lanes of a row group supply
+// the 16 columns of a block.  With A row-major those 16 lanes can read the
+// whole row with one coalesced load each: lane j of the group loads
+// LB = 16 * KC * sizeof(S) / 256 bytes, the IB = KC / 16 consecutive columns
+// j * IB ... j * IB + IB - 1, and element b of its load feeds block b.  Block
+// b therefore holds the columns {j * IB + b : j = 0..15}; its output row r is
+// C row r * IB + b (likewise for B and the C columns).  Each wave streams its
+// own contiguous run of 4-row steps straight into registers, D steps in
+// flight: no LDS staging and no workgroup barrier until the final sum.
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const unsigned char gu8;
+typedef __attribute__((address_space(1))) const v2u gv2u;
+typedef __attribute__((address_space(1))) const uint32_t gu32;
+
+// LB bytes (4, 8, 16 or 32) at p into w[LB / 4].
+template <int LB, int NT>
+__device__ __forceinline__ void lane_load(uint32_t (&w)[LB / 4], gu8* p) {
+  static_assert(LB == 4 || LB == 8 || LB % 16 == 0, "lane load size");
+  if constexpr (LB >= 16) {
+#pragma unroll
+    for (int x = 0; x < LB / 16; ++x) {
+      const v4u v = stream_load<NT>((gv4u*)(p + 16 * x));
+      w[4 * x] = v.x;
+      w[4 * x + 1] = v.y;
+      w[4 * x + 2] = v.z;
+      w[4 * x + 3] = v.w;
+    }
+  } else if constexpr (LB == 8) {
+    const v2u v = NT ? __builtin_nontemporal_load((gv2u*)p) : *(gv2u*)p;
+    w[0] = v.x;
+    w[1] = v.y;
+  } else {
+    w[0] = NT ? __builtin_nontemporal_load((gu32*)p) : *(gu32*)p;
+  }
+}
+template <class S>
+__device__ __forceinline__ S lane_elem(const uint32_t* w, int e);
+template <>
+__device__ __forceinline__ double lane_elem<double>(const uint32_t* w, int e) {
+  return __hiloint2double((int)w[2 * e + 1], (int)w[2 * e]);
+}
+template <>
+__device__ __forceinline__ float lane_elem<float>(const uint32_t* w, int e) {
+  return __uint_as_float(w[e]);
+}
+
+// One wave's KC x MC partial over its steps, then the fixed-order sum of the
+// four waves' partials into partial_out (red: 2 * KC * MC doubles of LDS).
+// SYM = 1: B is A (C = A^T A); only blocks i <= j are accumulated and the
+// others mirrored.  a.tiles_per_wg is q, the pipelined steps per wave (a
+// multiple of D): wave w takes steps [w q, w q + q); the F - W q steps left
+// over (F = n / 4 full steps, W waves) go one at a time to waves in turn, and
+// the last wave adds the n % 4 final rows.  FP32 accumulators are added into
+// FP64 every 16 steps (64 rows per wave).  Needs a.dense.
+template <class S, int KC, int MC, int D, int NT, int SYM = 0>
+__device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
+                                               double* partial_out) {
+  static_assert(KC % 16 == 0 && MC % 16 == 0, "matrix-core widths");
+  static_assert(!SYM || KC == MC, "A^T A is square");
+  constexpr int IB = KC / 16, JB = MC / 16;
+  constexpr int LBA = KC * (int)sizeof(S) / 16, LBB = MC * (int)sizeof(S) / 16;
+  constexpr int RA = KC * (int)sizeof(S), RB = MC * (int)sizeof(S);  // bytes per row
+  constexpr bool kF32 = sizeof(S) == 4;
+  constexpr int FOLD = 16;  // steps between FP32 -> FP64 folds
+  static_assert(!kF32 || FOLD % D == 0, "D divides the fold interval");
+  using AccV = typename MfmaTraits<S>::AccV;
+  auto used = [](int i, int j) constexpr { return !SYM || i <= j; };
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = lane >> 4, j = lane & 15;
+  const int64_t F = a.n / 4;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t q = a.tiles_per_wg;
+  const int64_t s0 = wid * q;
+  gu8* A8 = (gu8*)a.A;
+  gu8* B8 = (gu8*)a.B;
+  const uint32_t la = (uint32_t)(g * RA + j * LBA), lb = (uint32_t)(g * RB + j * LBB);
+
+  AccV acc[IB][JB];
+  double accd[kF32 ? IB : 1][kF32 ? JB : 1][4];
+#pragma unroll
+  for (int i = 0; i < IB; ++i)
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      acc[i][jj] = AccV{};
+      if constexpr (kF32)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) accd[i][jj][v] = 0.0;
+    }
+  uint32_t wa[D][LBA / 4], wb[SYM ? 1 : D][SYM ? 1 : LBB / 4];
+  auto ld = [&](int d, int64_t step) __attribute__((always_inline)) {
+    lane_load<LBA, NT>(wa[d], A8 + step * 4 * RA + la);
+    if constexpr (!SYM) lane_load<LBB, NT>(wb[d], B8 + step * 4 * RB + lb);
+  };
+  auto consume = [&](int d) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const S av = lane_elem<S>(wa[d], i);
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj) {
+        if (!used(i, jj)) continue;
+        S bv;
+        if constexpr (SYM)
+          bv = lane_elem<S>(wa[d], jj);
+        else
+          bv = lane_elem<S>(wb[d], jj);
+        acc[i][jj] = mfma_step(av, bv, acc[i][jj]);
+      }
+    }
+  };
+  auto fold = [&]() __attribute__((always_inline)) {
+    if constexpr (kF32) {
+#pragma unroll
+      for (int i = 0; i < IB; ++i)
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            if (!used(i, jj)) continue;
+            accd[i][jj][v] += (double)acc[i][jj][v];
+            acc[i][jj][v] = 0.0f;
+          }
+    }
+  };
+
+  if (q > 0) {
+    const int64_t last = s0 + q - 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ld(d, s0 + d);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // The refill is unconditional (past the last step it re-reads that step,
+    // in bounds and discarded), so every step waits only for its own loads;
+    // sched_barrier keeps the refills behind the instructions that read the
+    // registers they overwrite.
+    int since_fold = 0;
+    for (int64_t s = 0; s < q; s += D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        consume(d);
+        __builtin_amdgcn_sched_barrier(0);
+        const int64_t nx = s0 + s + d + D;
+        ld(d, nx < last ? nx : last);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      if constexpr (kF32) {
+        since_fold += D;
+        if (since_fold == FOLD) {
+          fold();
+          since_fold = 0;
+        }
+      }
+    }
+  }
+  for (int64_t st = W * q + wid; st < F; st += W) {
+    ld(0, st);
+    consume(0);
+    fold();
+  }
+  if (wid == W - 1 && a.n > 4 * F) {  // the last n % 4 rows
+    const int64_t r = 4 * F + g;
+    if (r < a.n) {
+      lane_load<LBA, NT>(wa[0], A8 + r * RA + (uint32_t)(j * LBA));
+      if constexpr (!SYM) lane_load<LBB, NT>(wb[0], B8 + r * RB + (uint32_t)(j * LBB));
+    } else {
+#pragma unroll
+      for (int x = 0; x < LBA / 4; ++x) wa[0][x] = 0u;
+      if constexpr (!SYM)
+#pragma unroll
+        for (int x = 0; x < LBB / 4; ++x) wb[0][x] = 0u;
+    }
+    consume(0);
+  }
+  fold();
+
+  // ---- fixed-order sum of the four waves: (w0 + w2) + (w1 + w3) ----------------------
+  constexpr int KCMC = KC * MC;
+  auto value = [&](int i, int jj, int v) __attribute__((always_inline)) -> double {
+    if constexpr (kF32)
+      return accd[i][jj][v];
+    else
+      return acc[i][jj][v];
+  };
+  auto for_each = [&](auto&& f) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < IB; ++i)
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (!used(i, jj)) continue;
+          const int r = MfmaTraits<S>::row_major_out ? 4 * g + v : g + 4 * v;
+          const int o = (r * IB + i) * MC + j * JB + jj;
+          f(o, value(i, jj, v));
+          if (SYM && i < jj) f((j * JB + jj) * MC + r * IB + i, value(i, jj, v));
+        }
+  };
+  if (wave >= 2) for_each([&](int o, double x) { red[(wave - 2) * KCMC + o] = x; });
+  __syncthreads();
+  if (wave < 2) for_each([&](int o, double x) { red[wave * KCMC + o] = x + red[wave * KCMC + o]; });
+  __syncthreads();
+  for (int o = threadIdx.x; o < KCMC; o += kThreads) publish(partial_out + o, red[o] + red[KCMC + o]);
+}
+
+// Direct-engine kernel body: partial, then the last workgroup's final sum
+// (classes up to 1024 outputs) as in atb_body.
+template <class S, int KC, int MC, int D, int NT, int SYM = 0>
+__device__ __forceinline__ void direct_body(const AtbArgs& a) {
+  constexpr int KCMC = KC * MC;
+  constexpr int SMEM = 2 * KCMC * 8 > kThreads * 16 ? 2 * KCMC * 8 : kThreads * 16;
+  __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
+  __shared__ int s_last;
+  direct_partial<S, KC, MC, D, NT, SYM>(a, reinterpret_cast<double*>(smem),
+                                        a.partials + (int64_t)blockIdx.x * KCMC);
   if (!a.finish) return;
   if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
   if constexpr (KCMC >= 2 && KCMC <= 512)

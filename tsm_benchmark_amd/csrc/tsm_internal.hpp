@@ -21,6 +21,7 @@ struct KernelEntry {
   int nt;  // 1: non-temporal loads
   int lds_bytes;
   LaunchFn launch;
+  int engine;  // 1 VALU register tiles, 2 matrix cores through LDS, 3 direct matrix cores
 };
 
 struct TunedEntry {
@@ -28,6 +29,10 @@ struct TunedEntry {
   int kernel;      // index into kernel_table()
   int wgs_per_cu;  // persistent workgroups per compute unit
   int interleave;  // 1: tiles dealt round-robin to workgroups
+  // Kernel (and tile order) for inputs the default cannot read: a direct-
+  // engine default needs dense rows; for other defaults this is the default.
+  int fallback;
+  int fallback_interleave;
 };
 
 // Width class handled by a batched kernel: the class's rows per tile fix the

@@ -4,6 +4,9 @@
   python -m tsm_benchmark_amd.profiling copy --bytes 1073741824 --reps 5
   python -m tsm_benchmark_amd.profiling bwd  --n 8388608 --k 16 --m 16 --dtype f32 --reps 5
 
+`--dist pow2` draws the entries as sign * 2^e, e uniform in [-20, 20], instead
+of uniform on [-1, 1).
+
 `copy` runs torch's device copy of a known byte count: the calibration for the
 FETCH_SIZE / WRITE_SIZE counters (bytes counted per byte moved).  Every step
 prints one line so a long profiled run shows progress.
@@ -30,6 +33,7 @@ def main(argv=None) -> int:
     ap.add_argument("--dtype", default="f64", choices=list(DT))
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--dist", default="uniform", choices=["uniform", "pow2"])
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
     if args.what == "copy":
@@ -41,8 +45,9 @@ def main(argv=None) -> int:
             print(f"copy {i}: {args.bytes} bytes read, {args.bytes} bytes written", flush=True)
         return 0
     dt = DT[args.dtype]
-    A = ops.random_matrix(args.n, args.k, dt, seed=SEED, stream_id=1)
-    B = ops.random_matrix(args.n, args.m, dt, seed=SEED, stream_id=2)
+    dist = 0 if args.dist == "uniform" else 1
+    A = ops.random_matrix(args.n, args.k, dt, seed=SEED, stream_id=1, dist=dist)
+    B = ops.random_matrix(args.n, args.m, dt, seed=SEED, stream_id=2, dist=dist)
     torch.cuda.synchronize()
     if args.what == "atb":
         idx, grid = _lib.plan({"f64": 0, "f32": 1, "bf16": 2}[args.dtype], args.n, args.k, args.m)
