@@ -125,3 +125,23 @@ def test_direct_deterministic_and_accumulate():
     assert torch.equal(C, C0 + first)
     C32 = ops.atb(A, B, out_dtype=torch.float32, config=cfg)
     assert torch.equal(C32, first.float())
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_direct_ata(dt):
+    """C = A^T A on the direct engine (blocks on and above the diagonal,
+    mirrored): the oracle bound, exact symmetry, and the staged fallback for
+    strided rows."""
+    cfg = _lib.make_config(engine=_lib.ENGINE_DIRECT)
+    for n in (1, 6, 1001, 100003):
+        print(f"ata {dt} n={n}", flush=True)
+        A = rand(n, 64, dt, 9)
+        C = ops.ata(A, out_dtype=torch.float64, config=cfg).cpu().numpy()
+        Ah = host(A)
+        check(C, Ah, Ah, dt)
+        assert np.array_equal(C, C.T)
+    W = rand(30011, 66, dt, 10)
+    C = ops.ata(W[:, 1:65], out_dtype=torch.float64, config=cfg).cpu().numpy()
+    Ah = host(W[:, 1:65])
+    check(C, Ah, Ah, dt)
+    assert np.array_equal(C, C.T)

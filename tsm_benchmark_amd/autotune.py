@@ -73,7 +73,9 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
     for kd in _lib.kernels():
         if kd["in_type"] != DTYPE_ID[dtype] or kd["kc"] != kc or kd["mc"] != mc:
             continue
-        for mult, il in [(mu, il) for mu in GRID_MULTS for il in (0, 1)]:
+        # the direct engine streams each wave's own rows: no tile order to choose
+        orders = (0,) if kd["engine"] == _lib.ENGINE_DIRECT else (0, 1)
+        for mult, il in [(mu, il) for mu in GRID_MULTS for il in orders]:
             cfg = _lib.config_of(kd, wgs_per_cu=mult, interleave=il)
             t = time_call(lambda: ops.atb(A, B, out=C, config=cfg), reps)
             rel = ((C - ref).norm() / ref.norm()).item()
@@ -85,22 +87,32 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
                 f"{nbytes / t / 1e9:8.1f} GB/s")
             results.append((t, kd["name"], kd, mult, il))
     results.sort(key=lambda r: (r[0], r[1]))
-    best = None
-    for t, _, kd, mult, il in results:
-        if accurate_on_pow2(dtype, kc, mc, _lib.config_of(kd, wgs_per_cu=mult, interleave=il)):
-            best = (t, kd, mult, il)
-            break
-        log(f"{kd['name']} misses 1e-6 on sign * 2^e inputs: skipped")
-    if best is None:
+
+    def fastest(direct_ok: bool):
+        for t, _, kd, mult, il in results:
+            if not direct_ok and kd["engine"] == _lib.ENGINE_DIRECT:
+                continue
+            if accurate_on_pow2(dtype, kc, mc, _lib.config_of(kd, wgs_per_cu=mult, interleave=il)):
+                return t, kd, mult, il
+            log(f"{kd['name']} misses 1e-6 on sign * 2^e inputs: skipped")
         raise RuntimeError(f"no {dtype} {kc}x{mc} kernel meets 1e-6 on sign * 2^e inputs")
-    t, kd, mult, il = best
+
+    t, kd, mult, il = fastest(True)
+    entry = {"dtype": dtype, "kc": kc, "mc": mc, "tm": kd["tm"], "tn": kd["tn"],
+             "rows": kd["rows"], "pf": kd["pf"], "nt": kd["nt"], "wgs_per_cu": mult,
+             "interleave": il,
+             "gbps": round(nbytes / t / 1e9, 1), "default_gbps": round(nbytes / default_t / 1e9, 1),
+             "n": n, "cus": cus}
+    if kd["engine"] == _lib.ENGINE_DIRECT:
+        # the staged kernel for inputs the direct engine cannot read
+        entry["direct"] = 1
+        ft, fkd, _, fil = fastest(False)
+        entry["fallback"] = {"tm": fkd["tm"], "tn": fkd["tn"], "rows": fkd["rows"],
+                             "pf": fkd["pf"], "nt": fkd["nt"], "interleave": fil,
+                             "gbps": round(nbytes / ft / 1e9, 1)}
     del A, B
     torch.cuda.empty_cache()
-    return {"dtype": dtype, "kc": kc, "mc": mc, "tm": kd["tm"], "tn": kd["tn"],
-            "rows": kd["rows"], "pf": kd["pf"], "nt": kd["nt"], "wgs_per_cu": mult,
-            "interleave": il,
-            "gbps": round(nbytes / t / 1e9, 1), "default_gbps": round(nbytes / default_t / 1e9, 1),
-            "n": n, "cus": cus}
+    return entry
 
 
 def tune_sym(dtype: str, kc: int, gib: float, reps: int, log) -> dict:
@@ -118,9 +130,10 @@ def tune_sym(dtype: str, kc: int, gib: float, reps: int, log) -> dict:
     results = []
     default_t = time_call(lambda: ops.ata(A, out=C), reps)
     for c in variants:
-        for mult, il in [(mu, il) for mu in GRID_MULTS for il in (0, 1)]:
+        orders = (0,) if c.direct else (0, 1)
+        for mult, il in [(mu, il) for mu in GRID_MULTS for il in orders]:
             cfg = _lib.make_config(tm=c.tm, tn=c.tn, rows=c.rows, pf=c.pf, nt=c.nt,
-                                   wgs_per_cu=mult, interleave=il, mfma=c.mfma)
+                                   wgs_per_cu=mult, interleave=il, engine=c.engine)
             t = time_call(lambda: ops.ata(A, out=C, config=cfg), reps)
             rel = ((C - ref).norm() / ref.norm()).item()
             tol = 1e-13 if dtype == "f64" else 1e-6
@@ -134,7 +147,7 @@ def tune_sym(dtype: str, kc: int, gib: float, reps: int, log) -> dict:
     del A
     torch.cuda.empty_cache()
     return {"dtype": dtype, "kc": kc, "tm": c.tm, "tn": c.tn, "rows": c.rows, "pf": c.pf,
-            "nt": c.nt, "wgs_per_cu": mult, "interleave": il,
+            "nt": c.nt, "wgs_per_cu": mult, "interleave": il, "direct": c.direct,
             "gbps": round(nbytes / t / 1e9, 1), "default_gbps": round(nbytes / default_t / 1e9, 1),
             "n": n}
 

@@ -423,8 +423,13 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
                 c = default_config(dt, kc, mc)
                 cfgs.add(c)
                 defaults[(dt, kc, mc)] = (c, 2, 0)
-                # one direct kernel in every class the engine serves
-                cfgs.update(d for d in direct_variants(dt, kc, mc) if d.pf == 8)
+                # one direct kernel in every class the engine serves; without
+                # a tuned entry it is the default from 1024 outputs on (one
+                # workgroup per CU), the staged kernel its fallback
+                direct = [d for d in direct_variants(dt, kc, mc) if d.pf == 8]
+                cfgs.update(direct)
+                if direct and kc * mc >= 1024:
+                    defaults[(dt, kc, mc)] = (direct[0], 1, 0)
         for kc, mc in TUNE_SHAPES[dt]:
             cfgs.update(tuning_variants(dt, kc, mc))
     cfgs.update(reference_variants([4, 8, 16], [2, 4]))

@@ -42,7 +42,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 29: tsm::batched_item_call<double, 16, 2, 8, 2, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x2_t8x2_r128_p2
       case 30: tsm::batched_item_call<double, 16, 4, 8, 4, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x4_t8x4_r128_p2
       case 31: tsm::batched_item_call<double, 16, 8, 8, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x8_t8x8_r128_p2
-      case 32: tsm::batched_item_call<double, 16, 16, 8, 8, 128, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x16_t8x8_r128_p2
+      case 32: tsm::batched_item_call<double, 16, 16, 0, 0, 64, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x16_mfma_r64_p2_nt
       case 33: tsm::batched_item_call<double, 16, 32, 8, 8, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x32_t8x8_r64_p2
       case 34: tsm::batched_item_call<double, 16, 64, 0, 0, 32, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_16x64_mfma_r32_p2
       case 35: tsm::batched_item_call<double, 32, 1, 8, 1, 64, 2, 0>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_32x1_t8x1_r64_p2
@@ -97,7 +97,7 @@ static const tsm::BatchClass kClasses[] = {
   {16, 2, 128},
   {16, 4, 128},
   {16, 8, 128},
-  {16, 16, 128},
+  {16, 16, 64},
   {16, 32, 64},
   {16, 64, 32},
   {32, 1, 64},

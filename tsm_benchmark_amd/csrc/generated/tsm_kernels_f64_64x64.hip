@@ -243,36 +243,6 @@ hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
-// tsm_ata_f64_64x64_mfma_r16_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=16 pf=1 nt=1
-//   matrix cores: 4 K-steps of 4 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r16_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 16, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r16_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r16_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=16 pf=2 nt=1
-//   matrix cores: 4 K-steps of 4 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r16_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 16, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r16_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r32_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=32 pf=1 nt=1
-//   matrix cores: 8 K-steps of 4 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r32_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 32, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r32_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
 // tsm_ata_f64_64x64_mfma_r32_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=32 pf=2 nt=1
 //   matrix cores: 8 K-steps of 4 rows per tile, LDS 32768 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r32_p2_nt(tsm::AtbArgs a) {
@@ -280,45 +250,5 @@ extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r32_p2_
 }
 hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r32_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r64_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=64 pf=1 nt=1
-//   matrix cores: 16 K-steps of 4 rows per tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r64_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 64, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r64_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r64_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=64 pf=2 nt=1
-//   matrix cores: 16 K-steps of 4 rows per tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r64_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 64, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r128_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=128 pf=1 nt=1
-//   matrix cores: 32 K-steps of 4 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r128_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 128, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r128_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_64x64_mfma_r128_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=128 pf=2 nt=1
-//   matrix cores: 32 K-steps of 4 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_64x64_mfma_r128_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 64, 64, 0, 0, 128, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_64x64_mfma_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

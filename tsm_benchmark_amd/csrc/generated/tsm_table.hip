@@ -599,14 +599,10 @@ hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -629,6 +625,10 @@ hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p1_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -641,14 +641,10 @@ hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_64x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 
 static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2, 1},
@@ -1250,14 +1246,10 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f32_32x32_mfma_r128_p2_nt", 1, 32, 32, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p2_nt, 2},
   {"tsm_ata_f32_32x32_mfma_r256_p1_nt", 1, 32, 32, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p1_nt, 2},
   {"tsm_ata_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f32_32x32_mfma_r256_p2_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r32_p1_nt", 1, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p1_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r32_p2_nt", 1, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r32_p2_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r64_p1_nt", 1, 64, 64, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f32_64x64_direct_d4_nt", 1, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_ata_f32_64x64_direct_d4_nt, 3},
+  {"tsm_ata_f32_64x64_direct_d8_nt", 1, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_ata_f32_64x64_direct_d8_nt, 3},
+  {"tsm_ata_f32_64x64_direct_d16_nt", 1, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_ata_f32_64x64_direct_d16_nt, 3},
   {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r128_p1_nt", 1, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p1_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r128_p2_nt", 1, 64, 64, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f32_64x64_mfma_r128_p2_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r256_p1_nt", 1, 64, 64, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p1_nt, 2},
-  {"tsm_ata_f32_64x64_mfma_r256_p2_nt", 1, 64, 64, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f32_64x64_mfma_r256_p2_nt, 2},
   {"tsm_ata_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt, 1},
   {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt, 1},
   {"tsm_ata_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt, 1},
@@ -1280,6 +1272,10 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt, 1},
   {"tsm_ata_f64_8x8_t8x8_r1024_p1_nt", 0, 8, 8, 8, 8, 1024, 1, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p1_nt, 1},
   {"tsm_ata_f64_8x8_t8x8_r1024_p2_nt", 0, 8, 8, 8, 8, 1024, 2, 1, 131072, launch_tsm_ata_f64_8x8_t8x8_r1024_p2_nt, 1},
+  {"tsm_ata_f64_16x16_mfma_r32_p1_nt", 0, 16, 16, 0, 0, 32, 1, 1, 8192, launch_tsm_ata_f64_16x16_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r32_p2_nt", 0, 16, 16, 0, 0, 32, 2, 1, 8192, launch_tsm_ata_f64_16x16_mfma_r32_p2_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r64_p1_nt", 0, 16, 16, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f64_16x16_mfma_r64_p1_nt, 2},
+  {"tsm_ata_f64_16x16_mfma_r64_p2_nt", 0, 16, 16, 0, 0, 64, 2, 1, 16384, launch_tsm_ata_f64_16x16_mfma_r64_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r128_p1_nt", 0, 16, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p1_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt, 2},
@@ -1292,14 +1288,10 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r16_p1_nt", 0, 64, 64, 0, 0, 16, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p1_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r16_p2_nt", 0, 64, 64, 0, 0, 16, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r16_p2_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r32_p1_nt", 0, 64, 64, 0, 0, 32, 1, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f64_64x64_direct_d4_nt", 0, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_ata_f64_64x64_direct_d4_nt, 3},
+  {"tsm_ata_f64_64x64_direct_d8_nt", 0, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_ata_f64_64x64_direct_d8_nt, 3},
+  {"tsm_ata_f64_64x64_direct_d16_nt", 0, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_ata_f64_64x64_direct_d16_nt, 3},
   {"tsm_ata_f64_64x64_mfma_r32_p2_nt", 0, 64, 64, 0, 0, 32, 2, 1, 32768, launch_tsm_ata_f64_64x64_mfma_r32_p2_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r64_p1_nt", 0, 64, 64, 0, 0, 64, 1, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p1_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r64_p2_nt", 0, 64, 64, 0, 0, 64, 2, 1, 65536, launch_tsm_ata_f64_64x64_mfma_r64_p2_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r128_p1_nt", 0, 64, 64, 0, 0, 128, 1, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p1_nt, 2},
-  {"tsm_ata_f64_64x64_mfma_r128_p2_nt", 0, 64, 64, 0, 0, 128, 2, 1, 131072, launch_tsm_ata_f64_64x64_mfma_r128_p2_nt, 2},
 };
 
 static const tsm::TunedEntry kSymTuned[] = {
@@ -1314,16 +1306,16 @@ static const tsm::TunedEntry kSymTuned[] = {
   {1, 2, 2, 28, 2, 0, 28, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
   {1, 4, 4, 29, 2, 0, 29, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
   {1, 8, 8, 32, 2, 0, 32, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
-  {1, 16, 16, 40, 2, 1, 40, 1},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
-  {1, 32, 32, 47, 1, 1, 47, 1},  // tsm_ata_f32_32x32_mfma_r256_p2_nt
-  {1, 64, 64, 53, 2, 0, 53, 0},  // tsm_ata_f32_64x64_mfma_r128_p2_nt
-  {0, 1, 1, 56, 2, 1, 56, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
-  {0, 2, 2, 61, 1, 1, 61, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
-  {0, 4, 4, 69, 1, 1, 69, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 75, 1, 0, 75, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
-  {0, 16, 16, 79, 1, 1, 79, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 89, 1, 1, 89, 1},  // tsm_ata_f64_32x32_mfma_r128_p2_nt
-  {0, 64, 64, 95, 2, 0, 95, 0},  // tsm_ata_f64_64x64_mfma_r64_p2_nt
+  {1, 16, 16, 41, 1, 0, 41, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
+  {1, 32, 32, 45, 3, 0, 45, 0},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
+  {1, 64, 64, 50, 2, 0, 51, 0},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2_nt)
+  {0, 1, 1, 52, 2, 1, 52, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
+  {0, 2, 2, 57, 1, 1, 57, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
+  {0, 4, 4, 65, 1, 1, 65, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 71, 1, 0, 71, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
+  {0, 16, 16, 81, 1, 0, 81, 0},  // tsm_ata_f64_16x16_mfma_r256_p2_nt
+  {0, 32, 32, 88, 2, 0, 88, 0},  // tsm_ata_f64_32x32_mfma_r128_p1_nt
+  {0, 64, 64, 92, 2, 0, 93, 1},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
 };
 
 static const tsm::TunedEntry kTuned[] = {
@@ -1410,21 +1402,21 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 16, 8, 209, 2, 0, 209, 0},  // tsm_atb_f32_16x8_t8x4_r256_p2
   {1, 16, 16, 216, 1, 1, 216, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
   {1, 16, 32, 244, 2, 0, 244, 0},  // tsm_atb_f32_16x32_t8x4_r128_p2
-  {1, 16, 64, 246, 2, 0, 246, 0},  // tsm_atb_f32_16x64_mfma_r64_p2
+  {1, 16, 64, 245, 1, 0, 246, 0},  // tsm_atb_f32_16x64_direct_d8_nt (else tsm_atb_f32_16x64_mfma_r64_p2)
   {1, 32, 1, 247, 2, 0, 247, 0},  // tsm_atb_f32_32x1_t8x1_r128_p2
   {1, 32, 2, 248, 2, 0, 248, 0},  // tsm_atb_f32_32x2_t8x2_r128_p2
   {1, 32, 4, 249, 2, 0, 249, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
   {1, 32, 8, 250, 2, 0, 250, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
   {1, 32, 16, 252, 2, 0, 252, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
   {1, 32, 32, 263, 1, 1, 263, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 291, 2, 0, 291, 0},  // tsm_atb_f32_32x64_mfma_r64_p2
+  {1, 32, 64, 290, 1, 0, 291, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2)
   {1, 64, 1, 292, 2, 0, 292, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
   {1, 64, 2, 293, 2, 0, 293, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
   {1, 64, 4, 294, 2, 0, 294, 0},  // tsm_atb_f32_64x4_t8x4_r64_p2
   {1, 64, 8, 295, 2, 0, 295, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
-  {1, 64, 16, 297, 2, 0, 297, 0},  // tsm_atb_f32_64x16_mfma_r64_p2
-  {1, 64, 32, 299, 2, 0, 299, 0},  // tsm_atb_f32_64x32_mfma_r64_p2
-  {1, 64, 64, 310, 1, 0, 310, 0},  // tsm_atb_f32_64x64_mfma_r64_p2_nt
+  {1, 64, 16, 296, 1, 0, 297, 0},  // tsm_atb_f32_64x16_direct_d8_nt (else tsm_atb_f32_64x16_mfma_r64_p2)
+  {1, 64, 32, 298, 1, 0, 299, 0},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2)
+  {1, 64, 64, 301, 1, 0, 310, 0},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
   {0, 1, 1, 336, 1, 1, 336, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
   {0, 1, 2, 339, 2, 0, 339, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
   {0, 1, 4, 340, 2, 0, 340, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
@@ -1457,23 +1449,23 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 16, 2, 419, 2, 0, 419, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
   {0, 16, 4, 420, 2, 0, 420, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
   {0, 16, 8, 421, 2, 0, 421, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
-  {0, 16, 16, 434, 1, 1, 434, 1},  // tsm_atb_f64_16x16_mfma_r256_p1_nt
+  {0, 16, 16, 428, 1, 0, 428, 0},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
   {0, 16, 32, 458, 2, 0, 458, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
-  {0, 16, 64, 460, 2, 0, 460, 0},  // tsm_atb_f64_16x64_mfma_r32_p2
+  {0, 16, 64, 459, 1, 0, 460, 0},  // tsm_atb_f64_16x64_direct_d8_nt (else tsm_atb_f64_16x64_mfma_r32_p2)
   {0, 32, 1, 461, 2, 0, 461, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
   {0, 32, 2, 462, 2, 0, 462, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
   {0, 32, 4, 470, 2, 1, 470, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
   {0, 32, 8, 477, 2, 0, 477, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
   {0, 32, 16, 479, 2, 0, 479, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 486, 1, 1, 486, 1},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
-  {0, 32, 64, 512, 2, 0, 512, 0},  // tsm_atb_f64_32x64_mfma_r32_p2
+  {0, 32, 32, 486, 1, 0, 486, 0},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
+  {0, 32, 64, 511, 1, 0, 512, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2)
   {0, 64, 1, 513, 2, 0, 513, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
   {0, 64, 2, 514, 2, 0, 514, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
   {0, 64, 4, 515, 2, 0, 515, 0},  // tsm_atb_f64_64x4_t8x4_r32_p2
   {0, 64, 8, 516, 2, 0, 516, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
-  {0, 64, 16, 518, 2, 0, 518, 0},  // tsm_atb_f64_64x16_mfma_r32_p2
-  {0, 64, 32, 520, 2, 0, 520, 0},  // tsm_atb_f64_64x32_mfma_r32_p2
-  {0, 64, 64, 531, 1, 0, 531, 0},  // tsm_atb_f64_64x64_mfma_r32_p2_nt
+  {0, 64, 16, 517, 1, 0, 518, 0},  // tsm_atb_f64_64x16_direct_d8_nt (else tsm_atb_f64_64x16_mfma_r32_p2)
+  {0, 64, 32, 519, 1, 0, 520, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2)
+  {0, 64, 64, 522, 1, 0, 531, 1},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
