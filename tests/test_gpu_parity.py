@@ -289,10 +289,13 @@ def test_backward_against_oracle(dt, k, m):
 
 @pytest.mark.parametrize("dt,k,m,n", [("f32", 16, 16, 70001), ("f32", 12, 4, 3001), ("f32", 32, 32, 300),
                                       ("f64", 2, 16, 5), ("f64", 16, 16, 4097), ("bf16", 8, 8, 1000),
-                                      ("bf16", 64, 16, 2049)])
+                                      ("bf16", 64, 16, 2049), ("bf16", 4, 4, 100003), ("bf16", 4, 4, 3),
+                                      ("bf16", 2, 2, 1001), ("bf16", 1, 1, 37), ("f32", 2, 2, 70001),
+                                      ("f32", 1, 1, 5), ("f64", 1, 1, 3001)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
-    """Dense rows take the LDS-staged kernel (widths up to 16) or the
-    matrix-core kernel (32, 64), strided views the row-wise one.  The staged
+    """Dense rows take the LDS-staged kernel (widths up to 16), the packed
+    kernel (square classes with rows under 16 bytes) or the matrix-core
+    kernel (32, 64); strided views take the row-wise one.  The staged, packed
     and row-wise kernels add the same products in the same order, so their
     bits agree; the matrix-core kernel is held to the oracle tolerance."""
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)

@@ -11,6 +11,9 @@
 //           stores (each instruction of a wave covers 1 KiB of contiguous
 //           bytes); a thread reads its row from the padded LDS tile and
 //           writes its output row back there before the tile is stored;
+//   packed  (k == m == W, dense rows under 16 bytes): a thread moves 16
+//           consecutive bytes, 16 / (W s) whole rows, in one load and one
+//           store;
 //   rowwise (any layout): a thread loads and stores its own rows directly,
 //           16 bytes at a time when dense.
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
@@ -289,6 +292,73 @@ __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   }
 }
 
+// ---- packed kernel (square classes with rows under 16 bytes) ------------------------
+// k == m == W, W * s < 16, dense: chunk c (16 bytes) of the input holds rows
+// c * PR ... c * PR + PR - 1 (PR = 16 / (W s)), and so does chunk c of the
+// output.  One thread per chunk: a non-temporal 16-byte load, the PR rows'
+// products (the same products in the same order as emit_row), one
+// non-temporal 16-byte store; a wave covers 1 KiB of consecutive bytes.
+template <class S, int W, bool TRANSPOSED>
+__device__ __forceinline__ void packed_chunk(const S* in, S* out, int64_t c,
+                                             const typename Traits<S>::Acc* G) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int PER = 16 / (int)sizeof(S), PR = PER / W;
+  static_assert(PR >= 2 && PR * W == PER, "rows under 16 bytes, whole rows per chunk");
+  const v4u u = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(in) + c);
+  Acc x[PER];
+  Unpack<S>::run(u, x);
+  S y[PER];
+#pragma unroll
+  for (int p = 0; p < PR; ++p)
+#pragma unroll
+    for (int o = 0; o < W; ++o) {
+      Acc acc = Acc(0);
+#pragma unroll
+      for (int t = 0; t < W; ++t)
+        acc = fmadd(x[p * W + t], TRANSPOSED ? G[t * W + o] : G[o * W + t], acc);
+      y[p * W + o] = narrow<S>(acc);
+    }
+  v4u v;
+  __builtin_memcpy(&v, y, 16);
+  __builtin_nontemporal_store(v, reinterpret_cast<v4u*>(out) + c);
+}
+
+template <class S, int W>
+__device__ __forceinline__ void bwd_packed_body(const BwdArgs& a) {
+  using Acc = typename Traits<S>::Acc;
+  constexpr int PR = 16 / ((int)sizeof(S) * W);
+  __shared__ Acc G[W * W];
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const int i = e / W, j = e % W;
+    const int64_t off = (int64_t)i * a.lddc + j;
+    G[e] = a.dc_f64 ? (Acc) static_cast<const double*>(a.dC)[off]
+                    : (Acc) static_cast<const float*>(a.dC)[off];
+  }
+  __syncthreads();
+  const S* A = static_cast<const S*>(a.A);
+  const S* B = static_cast<const S*>(a.B);
+  S* dA = static_cast<S*>(a.dA);
+  S* dB = static_cast<S*>(a.dB);
+  const int64_t chunks = a.n / PR;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t c = first; c < chunks; c += stride) {
+    if (dA) packed_chunk<S, W, false>(B, dA, c, G);
+    if (dB) packed_chunk<S, W, true>(A, dB, c, G);
+  }
+  for (int64_t r = chunks * PR + first; r < a.n; r += stride) {  // the n % PR final rows
+    Acc x[W];
+    if (dA) {
+      load_row<S, W>(B + r * W, W, false, x);
+      emit_row<S, W, false>(dA + r * W, W, false, x, G);
+    }
+    if (dB) {
+      load_row<S, W>(A + r * W, W, false, x);
+      emit_row<S, W, true>(dB + r * W, W, false, x, G);
+    }
+  }
+}
+
 // ---- matrix-core kernel (width classes 32 and 64) ----------------------------------
 // Each phase is a small GEMM per tile: out (R x W) = in (R x W) * Gp (W x W),
 // with Gp[t][o] = dC[o][t] for dA and dC[t][o] for dB (dC zero-padded to W).
@@ -479,6 +549,18 @@ TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w16, tsm::bf16, 16)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w32, tsm::bf16, 32)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w64, tsm::bf16, 64)
 
+#define TSM_BWD_PACKED(NAME, S, W)                                                    \
+  extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
+    tsm::bwd_packed_body<S, W>(a);                                                    \
+  }
+
+TSM_BWD_PACKED(tsm_atb_bwd_packed_f64_w1, double, 1)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_f32_w1, float, 1)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_f32_w2, float, 2)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w1, tsm::bf16, 1)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w2, tsm::bf16, 2)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w4, tsm::bf16, 4)
+
 #define TSM_BWD_KERNEL(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_body<S, W>(a);                                                           \
@@ -524,6 +606,12 @@ const BwdKernel kBwdStaged[3][7] = {
      tsm_atb_bwd_staged_f32_w16, tsm_atb_bwd_staged_f32_w32, nullptr},
     {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, tsm_atb_bwd_staged_bf16_w16,
      tsm_atb_bwd_staged_bf16_w32, tsm_atb_bwd_staged_bf16_w64},
+};
+// Packed kernels by dtype and log2(W); null where W * s is 16 bytes or more.
+const BwdKernel kBwdPacked[3][3] = {
+    {tsm_atb_bwd_packed_f64_w1, nullptr, nullptr},
+    {tsm_atb_bwd_packed_f32_w1, tsm_atb_bwd_packed_f32_w2, nullptr},
+    {tsm_atb_bwd_packed_bf16_w1, tsm_atb_bwd_packed_bf16_w2, tsm_atb_bwd_packed_bf16_w4},
 };
 // Matrix-core kernels for the 32- and 64-wide classes (dense rows).
 const BwdKernel kBwdMfma[3][2] = {
@@ -594,6 +682,18 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   BwdKernel kern = kBwd[(int)in_type][wi];
   int64_t cap = (int64_t)cus * 8;
   const bool dense = a.dense_a && a.dense_b;
+  // Square classes whose rows are under 16 bytes, every moving matrix
+  // contiguous (leading dimension = width) and 16-byte aligned.
+  const bool packed = k == m && k == w && k * (int)es < 16 && wi < 3 &&
+                      kBwdPacked[(int)in_type][wi] && (!dA || (ldb == m && ldda == k && al16(B) &&
+                                                               al16(dA))) &&
+                      (!dB || (lda == k && lddb == m && al16(A) && al16(dB)));
+  if (packed) {
+    kern = kBwdPacked[(int)in_type][wi];
+    const int64_t chunks = n / (16 / (k * (int)es));
+    blocks = (chunks > 0 ? chunks : 1) + 255;
+    blocks /= 256;
+  }
   BwdKernel fast = nullptr;
   int64_t tile_rows = 256;
   if (dense && wi >= 5) {

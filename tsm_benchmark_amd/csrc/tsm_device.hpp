@@ -305,19 +305,23 @@ __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC,
   auto store = [&](int o, double total) __attribute__((always_inline)) {
     store_output(a, MC, o, total);
   };
-  if (nout >= kThreads) {
-    for (int o = o0 + tid; o < o1; o += kThreads) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int g = 0;
-      for (; g + 4 <= G; g += 4) {
-        s0 += P[(int64_t)(g + 0) * KCMC + o];
-        s1 += P[(int64_t)(g + 1) * KCMC + o];
-        s2 += P[(int64_t)(g + 2) * KCMC + o];
-        s3 += P[(int64_t)(g + 3) * KCMC + o];
-      }
-      for (; g < G; ++g) s0 += P[(int64_t)g * KCMC + o];
-      store(o, (s0 + s1) + (s2 + s3));
+  // Partials g0 .. g1-1 of output o, added in g order, up to U loads in flight
+  // (the partials were just written through to memory: every load is a full
+  // memory round trip).
+  constexpr int U = 16;
+  auto sum = [&](int o, int g0, int g1) __attribute__((always_inline)) {
+    double acc = 0.0;
+    for (int g = g0; g < g1; g += U) {
+      double v[U];
+#pragma unroll
+      for (int x = 0; x < U; ++x) v[x] = g + x < g1 ? P[(int64_t)(g + x) * KCMC + o] : 0.0;
+#pragma unroll
+      for (int x = 0; x < U; ++x) acc += v[x];
     }
+    return acc;
+  };
+  if (nout >= kThreads) {
+    for (int o = o0 + tid; o < o1; o += kThreads) store(o, sum(o, 0, G));
     return;
   }
   // nout < kThreads: SL slices of the workgroup range per output.
@@ -325,17 +329,7 @@ __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC,
   const int oi = tid % nout, s = tid / nout;
   if (s < SL) {
     const int g0 = (int)((int64_t)G * s / SL), g1 = (int)((int64_t)G * (s + 1) / SL);
-    const int o = o0 + oi;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int g = g0;
-    for (; g + 4 <= g1; g += 4) {
-      s0 += P[(int64_t)(g + 0) * KCMC + o];
-      s1 += P[(int64_t)(g + 1) * KCMC + o];
-      s2 += P[(int64_t)(g + 2) * KCMC + o];
-      s3 += P[(int64_t)(g + 3) * KCMC + o];
-    }
-    for (; g < g1; ++g) s0 += P[(int64_t)g * KCMC + o];
-    scratch[s * nout + oi] = (s0 + s1) + (s2 + s3);
+    scratch[s * nout + oi] = sum(o0 + oi, g0, g1);
   }
   __syncthreads();
   if (tid < nout) {
