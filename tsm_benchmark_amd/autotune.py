@@ -59,7 +59,8 @@ def accurate_on_pow2(dtype: str, kc: int, mc: int, cfg) -> bool:
     return ((got - ref).norm() / ref.norm()).item() <= 1e-6
 
 
-def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict:
+def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log,
+               mults: tuple = GRID_MULTS) -> dict:
     es = codegen.ELEM_BYTES[dtype]
     n = int(gib * (1 << 30) / (es * (kc + mc)))
     A = ops.random_matrix(n, kc, DT[dtype], seed=SEED, stream_id=1)
@@ -75,7 +76,7 @@ def tune_shape(dtype: str, kc: int, mc: int, gib: float, reps: int, log) -> dict
             continue
         # the direct engine streams each wave's own rows: no tile order to choose
         orders = (0,) if kd["engine"] == _lib.ENGINE_DIRECT else (0, 1)
-        for mult, il in [(mu, il) for mu in GRID_MULTS for il in orders]:
+        for mult, il in [(mu, il) for mu in mults for il in orders]:
             cfg = _lib.config_of(kd, wgs_per_cu=mult, interleave=il)
             t = time_call(lambda: ops.atb(A, B, out=C, config=cfg), reps)
             rel = ((C - ref).norm() / ref.norm()).item()
@@ -161,6 +162,9 @@ def main(argv=None) -> int:
                     help="dtype:kcxmc entries (default: the generator's tuned shapes)")
     ap.add_argument("--ops", nargs="*", default=["atb", "ata"], choices=["atb", "ata"],
                     help="tune the A^T B kernels, the A^T A kernels, or both")
+    ap.add_argument("--all-classes", action="store_true",
+                    help="also every other width class (its compiled kernels, 1 or 2 "
+                         "workgroups per CU, both tile orders)")
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
     shapes = []
@@ -183,6 +187,16 @@ def main(argv=None) -> int:
             e = tune_shape(dt, k, m, args.gib, args.reps, log)
             print(f"-> {e}", flush=True)
             entries[(dt, k, m)] = e
+        if args.all_classes:
+            for dt in codegen.DTYPES:
+                for k in codegen.CLASSES:
+                    for m in codegen.CLASSES:
+                        if (dt, k, m) in shapes or (k, m) in codegen.TUNE_SHAPES[dt]:
+                            continue
+                        print(f"== {dt} {k}x{m} (grid and order)", flush=True)
+                        e = tune_shape(dt, k, m, args.gib, args.reps, log, mults=(1, 2))
+                        print(f"-> {e}", flush=True)
+                        entries[(dt, k, m)] = e
     if "ata" in args.ops:
         for dt, k, m in shapes:
             if k != m:

@@ -195,7 +195,7 @@ def _pow2_floor(x: int) -> int:
     return 1 << (max(1, x).bit_length() - 1)
 
 
-def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig:
+def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2, nt: int = 1) -> KernelConfig:
     """Heuristic configuration used when the tile table has no entry.
 
     FP64 keeps up to an 8 x 8 register tile (64 FP64 accumulators); FP32/BF16
@@ -205,7 +205,7 @@ def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig
     if kc % 16 == 0 and mc % 16 == 0 and (dtype == "bf16" or kc * mc >= 1024):
         # matrix cores: the register tiles would be VALU-bound at these widths
         rows = _pow2_floor(32768 // ((kc + mc) * es))
-        c = KernelConfig(dtype, kc, mc, 0, 0, max(rows, 4 * (32 if dtype == "bf16" else 4)), pf)
+        c = KernelConfig(dtype, kc, mc, 0, 0, max(rows, 4 * (32 if dtype == "bf16" else 4)), pf, nt)
         while c.lds_bytes > LDS_BUDGET and c.rows > 4 * c.kstep:
             c = dataclasses.replace(c, rows=c.rows // 2)
         while c.rows // (4 * c.kstep) > MAX_ROWS_PER_THREAD:
@@ -224,7 +224,7 @@ def default_config(dtype: str, kc: int, mc: int, *, pf: int = 2) -> KernelConfig
         rows *= 2
     while rows // rg > MAX_ROWS_PER_THREAD and rows > rg:
         rows //= 2
-    cfg = KernelConfig(dtype, kc, mc, tm, tn, rows, pf)
+    cfg = KernelConfig(dtype, kc, mc, tm, tn, rows, pf, nt)
     while cfg.lds_bytes > LDS_BUDGET and cfg.rows > rg:
         cfg = dataclasses.replace(cfg, rows=cfg.rows // 2)
     bad = cfg.problems()
@@ -505,7 +505,12 @@ def emit_direct_file(dtype: str, cfgs: list[KernelConfig]) -> str:
     return "\n".join(body)
 
 
-BATCH_LDS = LDS_BUDGET  # LDS of the batched kernels (every class fits one budget)
+# The batched kernel carries every class's registers and runs one workgroup
+# per CU, so LDS does not bound its occupancy: its FP64 and FP32 classes take
+# row tiles of up to BATCH_STAGE bytes (the bytes in flight per CU that a
+# second workgroup would otherwise add).
+BATCH_LDS = 81920
+BATCH_STAGE = 40960
 
 
 def batched_classes(defaults: dict, dtype: str, tuning: dict | None = None) -> list[KernelConfig]:
@@ -523,6 +528,12 @@ def batched_classes(defaults: dict, dtype: str, tuning: dict | None = None) -> l
             c = fbs[(dtype, kc, mc)][0]
             if c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32):
                 c = default_config(dtype, kc, mc)
+            while dtype != "bf16":  # BF16's wider class bodies would spill
+                wider = dataclasses.replace(c, rows=2 * c.rows)
+                if (wider.stage_bytes > BATCH_STAGE or wider.lds_bytes > BATCH_LDS
+                        or wider.problems()):
+                    break
+                c = wider
             out.append(c)
     return out
 

@@ -3,22 +3,22 @@
 
 #include "../tsm_internal.hpp"
 
-// tsm_atb_bf16_2x2_t2x2_r4096_p2: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=2 nt=0
+// tsm_atb_bf16_2x2_t2x2_r4096_p2_nt: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=2 nt=1
 //   1 register tiles x 256 row lanes, 16 rows/thread/tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_2x2_t2x2_r4096_p2(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 2, 2, 2, 2, 4096, 2, 0>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_2x2_t2x2_r4096_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 2, 2, 2, 2, 4096, 2, 1>(a);
 }
-hipError_t launch_tsm_atb_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_atb_bf16_2x2_t2x2_r4096_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_atb_bf16_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_2x2_t2x2_r4096_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-// tsm_ata_bf16_2x2_t2x2_r4096_p2: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=2 nt=0
+// tsm_ata_bf16_2x2_t2x2_r4096_p2_nt: KC=2 MC=2 TM=2 TN=2 rows=4096 pf=2 nt=1
 //   1 register tiles x 256 row lanes, 16 rows/thread/tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_2x2_t2x2_r4096_p2(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 2, 2, 2, 2, 4096, 2, 0, 1>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_2x2_t2x2_r4096_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 2, 2, 2, 2, 4096, 2, 1, 1>(a);
 }
-hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_2x2_t2x2_r4096_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_2x2_t2x2_r4096_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

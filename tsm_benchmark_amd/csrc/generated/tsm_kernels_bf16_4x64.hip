@@ -3,12 +3,12 @@
 
 #include "../tsm_internal.hpp"
 
-// tsm_atb_bf16_4x64_t4x4_r128_p2: KC=4 MC=64 TM=4 TN=4 rows=128 pf=2 nt=0
+// tsm_atb_bf16_4x64_t4x4_r128_p2_nt: KC=4 MC=64 TM=4 TN=4 rows=128 pf=2 nt=1
 //   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 34816 B
-extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_4x64_t4x4_r128_p2(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 4, 64, 4, 4, 128, 2, 0>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_4x64_t4x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 4, 64, 4, 4, 128, 2, 1>(a);
 }
-hipError_t launch_tsm_atb_bf16_4x64_t4x4_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_atb_bf16_4x64_t4x4_r128_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_atb_bf16_4x64_t4x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_4x64_t4x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

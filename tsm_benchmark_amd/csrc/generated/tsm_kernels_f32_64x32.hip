@@ -3,12 +3,12 @@
 
 #include "../tsm_internal.hpp"
 
-// tsm_atb_f32_64x32_mfma_r64_p2: KC=64 MC=32 TM=0 TN=0 rows=64 pf=2 nt=0
+// tsm_atb_f32_64x32_mfma_r64_p2_nt: KC=64 MC=32 TM=0 TN=0 rows=64 pf=2 nt=1
 //   matrix cores: 16 K-steps of 4 rows per tile, LDS 49152 B
-extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_64x32_mfma_r64_p2(tsm::AtbArgs a) {
-  tsm::atb_body<float, 64, 32, 0, 0, 64, 2, 0>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_64x32_mfma_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 64, 32, 0, 0, 64, 2, 1>(a);
 }
-hipError_t launch_tsm_atb_f32_64x32_mfma_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_atb_f32_64x32_mfma_r64_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_atb_f32_64x32_mfma_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_64x32_mfma_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -3,12 +3,12 @@
 
 #include "../tsm_internal.hpp"
 
-// tsm_atb_f64_4x64_t4x8_r32_p2: KC=4 MC=64 TM=4 TN=8 rows=32 pf=2 nt=0
+// tsm_atb_f64_4x64_t4x8_r32_p2_nt: KC=4 MC=64 TM=4 TN=8 rows=32 pf=2 nt=1
 //   8 register tiles x 32 row lanes, 1 rows/thread/tile, LDS 34816 B
-extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x64_t4x8_r32_p2(tsm::AtbArgs a) {
-  tsm::atb_body<double, 4, 64, 4, 8, 32, 2, 0>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x64_t4x8_r32_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 64, 4, 8, 32, 2, 1>(a);
 }
-hipError_t launch_tsm_atb_f64_4x64_t4x8_r32_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_atb_f64_4x64_t4x8_r32_p2, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_atb_f64_4x64_t4x8_r32_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x64_t4x8_r32_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -3,30 +3,30 @@
 
 #include "../tsm_internal.hpp"
 
-hipError_t launch_tsm_atb_bf16_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x2_t1x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x4_t1x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x8_t1x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x16_t1x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x32_t1x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_1x64_t1x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x1_t2x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x4_t2x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x8_t2x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x16_t2x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x32_t2x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_2x64_t2x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x1_t4x1_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x2_t4x2_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x4_t4x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x8_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x16_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x32_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_4x64_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x1_t8x1_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x2_t8x2_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x4_t8x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x2_t1x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x4_t1x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x8_t1x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x16_t1x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x32_t1x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_1x64_t1x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x1_t2x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x4_t2x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x8_t2x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x16_t2x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x32_t2x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_2x64_t2x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x1_t4x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x2_t4x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x4_t4x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x8_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x64_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x1_t8x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x2_t8x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x4_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -49,13 +49,13 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x16_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x32_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_8x64_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x1_t8x1_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x2_t8x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x4_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x8_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x32_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x1_t8x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x2_t8x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_mfma_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_mfma_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -90,13 +90,13 @@ hipError_t launch_tsm_atb_bf16_16x16_t8x8_r256_p1(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x32_mfma_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_16x64_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x1_t8x1_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x2_t8x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x4_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x8_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x16_mfma_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x8_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_mfma_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -131,13 +131,13 @@ hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_32x64_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x1_t8x1_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x2_t8x2_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x8_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x16_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_bf16_64x32_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x8_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x64_mfma_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x64_mfma_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -158,30 +158,30 @@ hipError_t launch_tsm_atb_bf16_64x64_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_bf16_64x64_t8x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x64_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x2_t1x2_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x4_t1x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x8_t1x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x16_t1x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x32_t1x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_1x64_t1x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x1_t2x1_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x2_t2x2_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x4_t2x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x8_t2x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x16_t2x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x32_t2x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_2x64_t2x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x1_t4x1_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x2_t4x2_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x4_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x8_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x16_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x32_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_4x64_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x1_t8x1_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x2_t8x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x4_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x2_t1x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x4_t1x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x8_t1x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x16_t1x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x32_t1x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_1x64_t1x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x1_t2x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x4_t2x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x8_t2x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x16_t2x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x32_t2x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_2x64_t2x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x1_t4x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x2_t4x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x16_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x1_t8x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x2_t8x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x4_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -206,13 +206,13 @@ hipError_t launch_tsm_atb_f32_8x8_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hi
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x16_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_8x64_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x1_t8x1_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x2_t8x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x4_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x8_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x16_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x4_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x8_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -247,15 +247,15 @@ hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_16x64_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x1_t8x1_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x2_t8x2_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x8_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x8_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x16_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x16_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -294,15 +294,15 @@ hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_32x64_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x1_t8x1_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x2_t8x2_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x4_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x8_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x2_t8x2_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x8_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x16_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f32_64x32_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -342,13 +342,13 @@ hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x2_t1x2_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x4_t1x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x8_t1x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x16_t1x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x32_t1x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_1x64_t1x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x1_t2x1_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x2_t1x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x4_t1x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x8_t1x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x16_t1x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x32_t1x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x64_t1x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x1_t2x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -361,13 +361,13 @@ hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x4_t2x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x8_t2x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x16_t2x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x32_t2x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_2x64_t2x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_4x1_t4x1_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_4x2_t4x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x4_t2x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x8_t2x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_t2x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_t2x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_t2x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x1_t4x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x2_t4x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t2x2_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t2x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -382,8 +382,8 @@ hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_4x8_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_4x16_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x8_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -398,10 +398,10 @@ hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_4x64_t4x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x1_t8x1_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x2_t8x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x4_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_t4x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x4_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t2x2_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t2x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -418,13 +418,13 @@ hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hi
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x16_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_8x64_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x1_t8x1_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x2_t8x2_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x8_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x16_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x32_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x64_t8x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x8_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -461,11 +461,11 @@ hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x32_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_16x64_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_32x1_t8x1_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_32x2_t8x2_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -480,9 +480,9 @@ hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_32x8_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x8_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_32x16_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x16_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -515,15 +515,15 @@ hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_32x64_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x1_t8x1_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x2_t8x2_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x4_t8x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x8_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x1_t8x1_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_t8x2_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_t8x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x8_t8x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x16_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x16_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_atb_f64_64x32_mfma_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -551,9 +551,9 @@ hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_bf16_4x4_t4x4_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_4x4_t4x4_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -578,9 +578,9 @@ hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsign
 hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -633,44 +633,42 @@ hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 
 static const tsm::KernelEntry kTable[] = {
-  {"tsm_atb_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2, 1},
-  {"tsm_atb_bf16_1x2_t1x2_r4096_p2", 2, 1, 2, 1, 2, 4096, 2, 0, 49152, launch_tsm_atb_bf16_1x2_t1x2_r4096_p2, 1},
-  {"tsm_atb_bf16_1x4_t1x4_r2048_p2", 2, 1, 4, 1, 4, 2048, 2, 0, 40960, launch_tsm_atb_bf16_1x4_t1x4_r2048_p2, 1},
-  {"tsm_atb_bf16_1x8_t1x4_r1024_p2", 2, 1, 8, 1, 4, 1024, 2, 0, 36864, launch_tsm_atb_bf16_1x8_t1x4_r1024_p2, 1},
-  {"tsm_atb_bf16_1x16_t1x4_r512_p2", 2, 1, 16, 1, 4, 512, 2, 0, 34816, launch_tsm_atb_bf16_1x16_t1x4_r512_p2, 1},
-  {"tsm_atb_bf16_1x32_t1x4_r256_p2", 2, 1, 32, 1, 4, 256, 2, 0, 33792, launch_tsm_atb_bf16_1x32_t1x4_r256_p2, 1},
-  {"tsm_atb_bf16_1x64_t1x4_r128_p2", 2, 1, 64, 1, 4, 128, 2, 0, 33280, launch_tsm_atb_bf16_1x64_t1x4_r128_p2, 1},
-  {"tsm_atb_bf16_2x1_t2x1_r4096_p2", 2, 2, 1, 2, 1, 4096, 2, 0, 49152, launch_tsm_atb_bf16_2x1_t2x1_r4096_p2, 1},
-  {"tsm_atb_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_atb_bf16_2x2_t2x2_r4096_p2, 1},
-  {"tsm_atb_bf16_2x4_t2x4_r2048_p2", 2, 2, 4, 2, 4, 2048, 2, 0, 49152, launch_tsm_atb_bf16_2x4_t2x4_r2048_p2, 1},
-  {"tsm_atb_bf16_2x8_t2x4_r1024_p2", 2, 2, 8, 2, 4, 1024, 2, 0, 40960, launch_tsm_atb_bf16_2x8_t2x4_r1024_p2, 1},
-  {"tsm_atb_bf16_2x16_t2x4_r512_p2", 2, 2, 16, 2, 4, 512, 2, 0, 36864, launch_tsm_atb_bf16_2x16_t2x4_r512_p2, 1},
-  {"tsm_atb_bf16_2x32_t2x4_r256_p2", 2, 2, 32, 2, 4, 256, 2, 0, 34816, launch_tsm_atb_bf16_2x32_t2x4_r256_p2, 1},
-  {"tsm_atb_bf16_2x64_t2x4_r128_p2", 2, 2, 64, 2, 4, 128, 2, 0, 33792, launch_tsm_atb_bf16_2x64_t2x4_r128_p2, 1},
-  {"tsm_atb_bf16_4x1_t4x1_r2048_p2", 2, 4, 1, 4, 1, 2048, 2, 0, 40960, launch_tsm_atb_bf16_4x1_t4x1_r2048_p2, 1},
-  {"tsm_atb_bf16_4x2_t4x2_r2048_p2", 2, 4, 2, 4, 2, 2048, 2, 0, 49152, launch_tsm_atb_bf16_4x2_t4x2_r2048_p2, 1},
-  {"tsm_atb_bf16_4x4_t4x4_r2048_p2", 2, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_atb_bf16_4x4_t4x4_r2048_p2, 1},
-  {"tsm_atb_bf16_4x8_t4x4_r1024_p2", 2, 4, 8, 4, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_4x8_t4x4_r1024_p2, 1},
-  {"tsm_atb_bf16_4x16_t4x4_r512_p2", 2, 4, 16, 4, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_4x16_t4x4_r512_p2, 1},
-  {"tsm_atb_bf16_4x32_t4x4_r256_p2", 2, 4, 32, 4, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_4x32_t4x4_r256_p2, 1},
-  {"tsm_atb_bf16_4x64_t4x4_r128_p2", 2, 4, 64, 4, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_4x64_t4x4_r128_p2, 1},
-  {"tsm_atb_bf16_8x1_t8x1_r1024_p2", 2, 8, 1, 8, 1, 1024, 2, 0, 36864, launch_tsm_atb_bf16_8x1_t8x1_r1024_p2, 1},
-  {"tsm_atb_bf16_8x2_t8x2_r1024_p2", 2, 8, 2, 8, 2, 1024, 2, 0, 40960, launch_tsm_atb_bf16_8x2_t8x2_r1024_p2, 1},
-  {"tsm_atb_bf16_8x4_t8x4_r1024_p2", 2, 8, 4, 8, 4, 1024, 2, 0, 49152, launch_tsm_atb_bf16_8x4_t8x4_r1024_p2, 1},
+  {"tsm_atb_bf16_1x1_t1x1_r4096_p2_nt", 2, 1, 1, 1, 1, 4096, 2, 1, 32768, launch_tsm_atb_bf16_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_atb_bf16_1x2_t1x2_r4096_p2_nt", 2, 1, 2, 1, 2, 4096, 2, 1, 49152, launch_tsm_atb_bf16_1x2_t1x2_r4096_p2_nt, 1},
+  {"tsm_atb_bf16_1x4_t1x4_r2048_p2_nt", 2, 1, 4, 1, 4, 2048, 2, 1, 40960, launch_tsm_atb_bf16_1x4_t1x4_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_1x8_t1x4_r1024_p2_nt", 2, 1, 8, 1, 4, 1024, 2, 1, 36864, launch_tsm_atb_bf16_1x8_t1x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_1x16_t1x4_r512_p2_nt", 2, 1, 16, 1, 4, 512, 2, 1, 34816, launch_tsm_atb_bf16_1x16_t1x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_1x32_t1x4_r256_p2_nt", 2, 1, 32, 1, 4, 256, 2, 1, 33792, launch_tsm_atb_bf16_1x32_t1x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_1x64_t1x4_r128_p2_nt", 2, 1, 64, 1, 4, 128, 2, 1, 33280, launch_tsm_atb_bf16_1x64_t1x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_2x1_t2x1_r4096_p2_nt", 2, 2, 1, 2, 1, 4096, 2, 1, 49152, launch_tsm_atb_bf16_2x1_t2x1_r4096_p2_nt, 1},
+  {"tsm_atb_bf16_2x2_t2x2_r4096_p2_nt", 2, 2, 2, 2, 2, 4096, 2, 1, 65536, launch_tsm_atb_bf16_2x2_t2x2_r4096_p2_nt, 1},
+  {"tsm_atb_bf16_2x4_t2x4_r2048_p2_nt", 2, 2, 4, 2, 4, 2048, 2, 1, 49152, launch_tsm_atb_bf16_2x4_t2x4_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_2x8_t2x4_r1024_p2_nt", 2, 2, 8, 2, 4, 1024, 2, 1, 40960, launch_tsm_atb_bf16_2x8_t2x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_2x16_t2x4_r512_p2_nt", 2, 2, 16, 2, 4, 512, 2, 1, 36864, launch_tsm_atb_bf16_2x16_t2x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_2x32_t2x4_r256_p2_nt", 2, 2, 32, 2, 4, 256, 2, 1, 34816, launch_tsm_atb_bf16_2x32_t2x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_2x64_t2x4_r128_p2_nt", 2, 2, 64, 2, 4, 128, 2, 1, 33792, launch_tsm_atb_bf16_2x64_t2x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_4x1_t4x1_r2048_p2_nt", 2, 4, 1, 4, 1, 2048, 2, 1, 40960, launch_tsm_atb_bf16_4x1_t4x1_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_4x2_t4x2_r2048_p2_nt", 2, 4, 2, 4, 2, 2048, 2, 1, 49152, launch_tsm_atb_bf16_4x2_t4x2_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_4x4_t4x4_r2048_p2_nt", 2, 4, 4, 4, 4, 2048, 2, 1, 65536, launch_tsm_atb_bf16_4x4_t4x4_r2048_p2_nt, 1},
+  {"tsm_atb_bf16_4x8_t4x4_r1024_p2_nt", 2, 4, 8, 4, 4, 1024, 2, 1, 49152, launch_tsm_atb_bf16_4x8_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_4x16_t4x4_r512_p2_nt", 2, 4, 16, 4, 4, 512, 2, 1, 40960, launch_tsm_atb_bf16_4x16_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_4x32_t4x4_r256_p2_nt", 2, 4, 32, 4, 4, 256, 2, 1, 36864, launch_tsm_atb_bf16_4x32_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_4x64_t4x4_r128_p2_nt", 2, 4, 64, 4, 4, 128, 2, 1, 34816, launch_tsm_atb_bf16_4x64_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_8x1_t8x1_r1024_p2_nt", 2, 8, 1, 8, 1, 1024, 2, 1, 36864, launch_tsm_atb_bf16_8x1_t8x1_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x2_t8x2_r1024_p2_nt", 2, 8, 2, 8, 2, 1024, 2, 1, 40960, launch_tsm_atb_bf16_8x2_t8x2_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x4_t8x4_r1024_p2_nt", 2, 8, 4, 8, 4, 1024, 2, 1, 49152, launch_tsm_atb_bf16_8x4_t8x4_r1024_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t4x4_r512_p1", 2, 8, 8, 4, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p1, 1},
   {"tsm_atb_bf16_8x8_t4x4_r512_p2", 2, 8, 8, 4, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p2, 1},
   {"tsm_atb_bf16_8x8_t4x4_r1024_p1", 2, 8, 8, 4, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p1, 1},
@@ -693,13 +691,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_8x8_t8x8_r1024_p2", 2, 8, 8, 8, 8, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p2, 1},
   {"tsm_atb_bf16_8x8_t8x8_r2048_p1", 2, 8, 8, 8, 8, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p1, 1},
   {"tsm_atb_bf16_8x8_t8x8_r2048_p2", 2, 8, 8, 8, 8, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p2, 1},
-  {"tsm_atb_bf16_8x16_t8x4_r512_p2", 2, 8, 16, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_8x16_t8x4_r512_p2, 1},
-  {"tsm_atb_bf16_8x32_t8x4_r256_p2", 2, 8, 32, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_8x32_t8x4_r256_p2, 1},
-  {"tsm_atb_bf16_8x64_t8x4_r128_p2", 2, 8, 64, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2, 1},
-  {"tsm_atb_bf16_16x1_t8x1_r512_p2", 2, 16, 1, 8, 1, 512, 2, 0, 34816, launch_tsm_atb_bf16_16x1_t8x1_r512_p2, 1},
-  {"tsm_atb_bf16_16x2_t8x2_r512_p2", 2, 16, 2, 8, 2, 512, 2, 0, 36864, launch_tsm_atb_bf16_16x2_t8x2_r512_p2, 1},
-  {"tsm_atb_bf16_16x4_t8x4_r512_p2", 2, 16, 4, 8, 4, 512, 2, 0, 40960, launch_tsm_atb_bf16_16x4_t8x4_r512_p2, 1},
-  {"tsm_atb_bf16_16x8_t8x4_r512_p2", 2, 16, 8, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_bf16_16x8_t8x4_r512_p2, 1},
+  {"tsm_atb_bf16_8x16_t8x4_r512_p2_nt", 2, 8, 16, 8, 4, 512, 2, 1, 49152, launch_tsm_atb_bf16_8x16_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_8x32_t8x4_r256_p2_nt", 2, 8, 32, 8, 4, 256, 2, 1, 40960, launch_tsm_atb_bf16_8x32_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_8x64_t8x4_r128_p2_nt", 2, 8, 64, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_16x1_t8x1_r512_p2_nt", 2, 16, 1, 8, 1, 512, 2, 1, 34816, launch_tsm_atb_bf16_16x1_t8x1_r512_p2_nt, 1},
+  {"tsm_atb_bf16_16x2_t8x2_r512_p2_nt", 2, 16, 2, 8, 2, 512, 2, 1, 36864, launch_tsm_atb_bf16_16x2_t8x2_r512_p2_nt, 1},
+  {"tsm_atb_bf16_16x4_t8x4_r512_p2_nt", 2, 16, 4, 8, 4, 512, 2, 1, 40960, launch_tsm_atb_bf16_16x4_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_16x8_t8x4_r512_p2_nt", 2, 16, 8, 8, 4, 512, 2, 1, 49152, launch_tsm_atb_bf16_16x8_t8x4_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x16_mfma_r256_p1", 2, 16, 16, 0, 0, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1, 2},
   {"tsm_atb_bf16_16x16_mfma_r256_p1_nt", 2, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p1_nt, 2},
   {"tsm_atb_bf16_16x16_mfma_r256_p2", 2, 16, 16, 0, 0, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_mfma_r256_p2, 2},
@@ -734,13 +732,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_16x16_t8x8_r256_p2", 2, 16, 16, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p2, 1},
   {"tsm_atb_bf16_16x16_t8x8_r512_p1", 2, 16, 16, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p1, 1},
   {"tsm_atb_bf16_16x16_t8x8_r512_p2", 2, 16, 16, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p2, 1},
-  {"tsm_atb_bf16_16x32_mfma_r256_p2", 2, 16, 32, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_16x32_mfma_r256_p2, 2},
-  {"tsm_atb_bf16_16x64_mfma_r128_p2", 2, 16, 64, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_16x64_mfma_r128_p2, 2},
-  {"tsm_atb_bf16_32x1_t8x1_r256_p2", 2, 32, 1, 8, 1, 256, 2, 0, 33792, launch_tsm_atb_bf16_32x1_t8x1_r256_p2, 1},
-  {"tsm_atb_bf16_32x2_t8x2_r256_p2", 2, 32, 2, 8, 2, 256, 2, 0, 34816, launch_tsm_atb_bf16_32x2_t8x2_r256_p2, 1},
-  {"tsm_atb_bf16_32x4_t8x4_r256_p2", 2, 32, 4, 8, 4, 256, 2, 0, 36864, launch_tsm_atb_bf16_32x4_t8x4_r256_p2, 1},
-  {"tsm_atb_bf16_32x8_t8x4_r256_p2", 2, 32, 8, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_bf16_32x8_t8x4_r256_p2, 1},
-  {"tsm_atb_bf16_32x16_mfma_r256_p2", 2, 32, 16, 0, 0, 256, 2, 0, 49152, launch_tsm_atb_bf16_32x16_mfma_r256_p2, 2},
+  {"tsm_atb_bf16_16x32_mfma_r256_p2_nt", 2, 16, 32, 0, 0, 256, 2, 1, 49152, launch_tsm_atb_bf16_16x32_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_16x64_mfma_r128_p2_nt", 2, 16, 64, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_16x64_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_32x1_t8x1_r256_p2_nt", 2, 32, 1, 8, 1, 256, 2, 1, 33792, launch_tsm_atb_bf16_32x1_t8x1_r256_p2_nt, 1},
+  {"tsm_atb_bf16_32x2_t8x2_r256_p2_nt", 2, 32, 2, 8, 2, 256, 2, 1, 34816, launch_tsm_atb_bf16_32x2_t8x2_r256_p2_nt, 1},
+  {"tsm_atb_bf16_32x4_t8x4_r256_p2_nt", 2, 32, 4, 8, 4, 256, 2, 1, 36864, launch_tsm_atb_bf16_32x4_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_32x8_t8x4_r256_p2_nt", 2, 32, 8, 8, 4, 256, 2, 1, 40960, launch_tsm_atb_bf16_32x8_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_32x16_mfma_r256_p2_nt", 2, 32, 16, 0, 0, 256, 2, 1, 49152, launch_tsm_atb_bf16_32x16_mfma_r256_p2_nt, 2},
   {"tsm_atb_bf16_32x32_mfma_r128_p1", 2, 32, 32, 0, 0, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1, 2},
   {"tsm_atb_bf16_32x32_mfma_r128_p1_nt", 2, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p1_nt, 2},
   {"tsm_atb_bf16_32x32_mfma_r128_p2", 2, 32, 32, 0, 0, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_mfma_r128_p2, 2},
@@ -775,13 +773,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_32x32_t8x8_r64_p2", 2, 32, 32, 8, 8, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p2, 1},
   {"tsm_atb_bf16_32x32_t8x8_r128_p1", 2, 32, 32, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p1, 1},
   {"tsm_atb_bf16_32x32_t8x8_r128_p2", 2, 32, 32, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p2, 1},
-  {"tsm_atb_bf16_32x64_mfma_r128_p2", 2, 32, 64, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2, 2},
-  {"tsm_atb_bf16_64x1_t8x1_r128_p2", 2, 64, 1, 8, 1, 128, 2, 0, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2, 1},
-  {"tsm_atb_bf16_64x2_t8x2_r128_p2", 2, 64, 2, 8, 2, 128, 2, 0, 33792, launch_tsm_atb_bf16_64x2_t8x2_r128_p2, 1},
-  {"tsm_atb_bf16_64x4_t8x4_r128_p2", 2, 64, 4, 8, 4, 128, 2, 0, 34816, launch_tsm_atb_bf16_64x4_t8x4_r128_p2, 1},
-  {"tsm_atb_bf16_64x8_t8x4_r128_p2", 2, 64, 8, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_bf16_64x8_t8x4_r128_p2, 1},
-  {"tsm_atb_bf16_64x16_mfma_r128_p2", 2, 64, 16, 0, 0, 128, 2, 0, 40960, launch_tsm_atb_bf16_64x16_mfma_r128_p2, 2},
-  {"tsm_atb_bf16_64x32_mfma_r128_p2", 2, 64, 32, 0, 0, 128, 2, 0, 49152, launch_tsm_atb_bf16_64x32_mfma_r128_p2, 2},
+  {"tsm_atb_bf16_32x64_mfma_r128_p2_nt", 2, 32, 64, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_64x1_t8x1_r128_p2_nt", 2, 64, 1, 8, 1, 128, 2, 1, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt, 1},
+  {"tsm_atb_bf16_64x2_t8x2_r128_p2_nt", 2, 64, 2, 8, 2, 128, 2, 1, 33792, launch_tsm_atb_bf16_64x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_bf16_64x4_t8x4_r128_p2_nt", 2, 64, 4, 8, 4, 128, 2, 1, 34816, launch_tsm_atb_bf16_64x4_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_64x8_t8x4_r128_p2_nt", 2, 64, 8, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_bf16_64x8_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_64x16_mfma_r128_p2_nt", 2, 64, 16, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_64x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_64x32_mfma_r128_p2_nt", 2, 64, 32, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_bf16_64x32_mfma_r128_p2_nt, 2},
   {"tsm_atb_bf16_64x64_mfma_r128_p1", 2, 64, 64, 0, 0, 128, 1, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1, 2},
   {"tsm_atb_bf16_64x64_mfma_r128_p1_nt", 2, 64, 64, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p1_nt, 2},
   {"tsm_atb_bf16_64x64_mfma_r128_p2", 2, 64, 64, 0, 0, 128, 2, 0, 65536, launch_tsm_atb_bf16_64x64_mfma_r128_p2, 2},
@@ -802,30 +800,30 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_64x64_t8x8_r64_p1_nt", 2, 64, 64, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p1_nt, 1},
   {"tsm_atb_bf16_64x64_t8x8_r64_p2", 2, 64, 64, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2, 1},
   {"tsm_atb_bf16_64x64_t8x8_r64_p2_nt", 2, 64, 64, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_bf16_64x64_t8x8_r64_p2_nt, 1},
-  {"tsm_atb_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 65536, launch_tsm_atb_f32_1x1_t1x1_r4096_p2, 1},
-  {"tsm_atb_f32_1x2_t1x2_r2048_p2", 1, 1, 2, 1, 2, 2048, 2, 0, 49152, launch_tsm_atb_f32_1x2_t1x2_r2048_p2, 1},
-  {"tsm_atb_f32_1x4_t1x4_r1024_p2", 1, 1, 4, 1, 4, 1024, 2, 0, 40960, launch_tsm_atb_f32_1x4_t1x4_r1024_p2, 1},
-  {"tsm_atb_f32_1x8_t1x4_r512_p2", 1, 1, 8, 1, 4, 512, 2, 0, 36864, launch_tsm_atb_f32_1x8_t1x4_r512_p2, 1},
-  {"tsm_atb_f32_1x16_t1x4_r256_p2", 1, 1, 16, 1, 4, 256, 2, 0, 34816, launch_tsm_atb_f32_1x16_t1x4_r256_p2, 1},
-  {"tsm_atb_f32_1x32_t1x4_r128_p2", 1, 1, 32, 1, 4, 128, 2, 0, 33792, launch_tsm_atb_f32_1x32_t1x4_r128_p2, 1},
-  {"tsm_atb_f32_1x64_t1x4_r64_p2", 1, 1, 64, 1, 4, 64, 2, 0, 33280, launch_tsm_atb_f32_1x64_t1x4_r64_p2, 1},
-  {"tsm_atb_f32_2x1_t2x1_r2048_p2", 1, 2, 1, 2, 1, 2048, 2, 0, 49152, launch_tsm_atb_f32_2x1_t2x1_r2048_p2, 1},
-  {"tsm_atb_f32_2x2_t2x2_r2048_p2", 1, 2, 2, 2, 2, 2048, 2, 0, 65536, launch_tsm_atb_f32_2x2_t2x2_r2048_p2, 1},
-  {"tsm_atb_f32_2x4_t2x4_r1024_p2", 1, 2, 4, 2, 4, 1024, 2, 0, 49152, launch_tsm_atb_f32_2x4_t2x4_r1024_p2, 1},
-  {"tsm_atb_f32_2x8_t2x4_r512_p2", 1, 2, 8, 2, 4, 512, 2, 0, 40960, launch_tsm_atb_f32_2x8_t2x4_r512_p2, 1},
-  {"tsm_atb_f32_2x16_t2x4_r256_p2", 1, 2, 16, 2, 4, 256, 2, 0, 36864, launch_tsm_atb_f32_2x16_t2x4_r256_p2, 1},
-  {"tsm_atb_f32_2x32_t2x4_r128_p2", 1, 2, 32, 2, 4, 128, 2, 0, 34816, launch_tsm_atb_f32_2x32_t2x4_r128_p2, 1},
-  {"tsm_atb_f32_2x64_t2x4_r64_p2", 1, 2, 64, 2, 4, 64, 2, 0, 33792, launch_tsm_atb_f32_2x64_t2x4_r64_p2, 1},
-  {"tsm_atb_f32_4x1_t4x1_r1024_p2", 1, 4, 1, 4, 1, 1024, 2, 0, 40960, launch_tsm_atb_f32_4x1_t4x1_r1024_p2, 1},
-  {"tsm_atb_f32_4x2_t4x2_r1024_p2", 1, 4, 2, 4, 2, 1024, 2, 0, 49152, launch_tsm_atb_f32_4x2_t4x2_r1024_p2, 1},
-  {"tsm_atb_f32_4x4_t4x4_r1024_p2", 1, 4, 4, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_f32_4x4_t4x4_r1024_p2, 1},
-  {"tsm_atb_f32_4x8_t4x4_r512_p2", 1, 4, 8, 4, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_4x8_t4x4_r512_p2, 1},
-  {"tsm_atb_f32_4x16_t4x4_r256_p2", 1, 4, 16, 4, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_4x16_t4x4_r256_p2, 1},
-  {"tsm_atb_f32_4x32_t4x4_r128_p2", 1, 4, 32, 4, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_4x32_t4x4_r128_p2, 1},
-  {"tsm_atb_f32_4x64_t4x4_r64_p2", 1, 4, 64, 4, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_4x64_t4x4_r64_p2, 1},
-  {"tsm_atb_f32_8x1_t8x1_r512_p2", 1, 8, 1, 8, 1, 512, 2, 0, 36864, launch_tsm_atb_f32_8x1_t8x1_r512_p2, 1},
-  {"tsm_atb_f32_8x2_t8x2_r512_p2", 1, 8, 2, 8, 2, 512, 2, 0, 40960, launch_tsm_atb_f32_8x2_t8x2_r512_p2, 1},
-  {"tsm_atb_f32_8x4_t8x4_r512_p2", 1, 8, 4, 8, 4, 512, 2, 0, 49152, launch_tsm_atb_f32_8x4_t8x4_r512_p2, 1},
+  {"tsm_atb_f32_1x1_t1x1_r4096_p2_nt", 1, 1, 1, 1, 1, 4096, 2, 1, 65536, launch_tsm_atb_f32_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_atb_f32_1x2_t1x2_r2048_p2_nt", 1, 1, 2, 1, 2, 2048, 2, 1, 49152, launch_tsm_atb_f32_1x2_t1x2_r2048_p2_nt, 1},
+  {"tsm_atb_f32_1x4_t1x4_r1024_p2_nt", 1, 1, 4, 1, 4, 1024, 2, 1, 40960, launch_tsm_atb_f32_1x4_t1x4_r1024_p2_nt, 1},
+  {"tsm_atb_f32_1x8_t1x4_r512_p2_nt", 1, 1, 8, 1, 4, 512, 2, 1, 36864, launch_tsm_atb_f32_1x8_t1x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_1x16_t1x4_r256_p2_nt", 1, 1, 16, 1, 4, 256, 2, 1, 34816, launch_tsm_atb_f32_1x16_t1x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_1x32_t1x4_r128_p2_nt", 1, 1, 32, 1, 4, 128, 2, 1, 33792, launch_tsm_atb_f32_1x32_t1x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_1x64_t1x4_r64_p2_nt", 1, 1, 64, 1, 4, 64, 2, 1, 33280, launch_tsm_atb_f32_1x64_t1x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_2x1_t2x1_r2048_p2_nt", 1, 2, 1, 2, 1, 2048, 2, 1, 49152, launch_tsm_atb_f32_2x1_t2x1_r2048_p2_nt, 1},
+  {"tsm_atb_f32_2x2_t2x2_r2048_p2_nt", 1, 2, 2, 2, 2, 2048, 2, 1, 65536, launch_tsm_atb_f32_2x2_t2x2_r2048_p2_nt, 1},
+  {"tsm_atb_f32_2x4_t2x4_r1024_p2_nt", 1, 2, 4, 2, 4, 1024, 2, 1, 49152, launch_tsm_atb_f32_2x4_t2x4_r1024_p2_nt, 1},
+  {"tsm_atb_f32_2x8_t2x4_r512_p2_nt", 1, 2, 8, 2, 4, 512, 2, 1, 40960, launch_tsm_atb_f32_2x8_t2x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_2x16_t2x4_r256_p2_nt", 1, 2, 16, 2, 4, 256, 2, 1, 36864, launch_tsm_atb_f32_2x16_t2x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_2x32_t2x4_r128_p2_nt", 1, 2, 32, 2, 4, 128, 2, 1, 34816, launch_tsm_atb_f32_2x32_t2x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_2x64_t2x4_r64_p2_nt", 1, 2, 64, 2, 4, 64, 2, 1, 33792, launch_tsm_atb_f32_2x64_t2x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_4x1_t4x1_r1024_p2_nt", 1, 4, 1, 4, 1, 1024, 2, 1, 40960, launch_tsm_atb_f32_4x1_t4x1_r1024_p2_nt, 1},
+  {"tsm_atb_f32_4x2_t4x2_r1024_p2_nt", 1, 4, 2, 4, 2, 1024, 2, 1, 49152, launch_tsm_atb_f32_4x2_t4x2_r1024_p2_nt, 1},
+  {"tsm_atb_f32_4x4_t4x4_r1024_p2_nt", 1, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_atb_f32_4x4_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_f32_4x8_t4x4_r512_p2_nt", 1, 4, 8, 4, 4, 512, 2, 1, 49152, launch_tsm_atb_f32_4x8_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_4x16_t4x4_r256_p2_nt", 1, 4, 16, 4, 4, 256, 2, 1, 40960, launch_tsm_atb_f32_4x16_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_4x32_t4x4_r128_p2_nt", 1, 4, 32, 4, 4, 128, 2, 1, 36864, launch_tsm_atb_f32_4x32_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_4x64_t4x4_r64_p2_nt", 1, 4, 64, 4, 4, 64, 2, 1, 34816, launch_tsm_atb_f32_4x64_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_8x1_t8x1_r512_p2_nt", 1, 8, 1, 8, 1, 512, 2, 1, 36864, launch_tsm_atb_f32_8x1_t8x1_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x2_t8x2_r512_p2_nt", 1, 8, 2, 8, 2, 512, 2, 1, 40960, launch_tsm_atb_f32_8x2_t8x2_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x4_t8x4_r512_p2_nt", 1, 8, 4, 8, 4, 512, 2, 1, 49152, launch_tsm_atb_f32_8x4_t8x4_r512_p2_nt, 1},
   {"tsm_atb_f32_8x8_t4x4_r256_p1", 1, 8, 8, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p1, 1},
   {"tsm_atb_f32_8x8_t4x4_r256_p2", 1, 8, 8, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p2, 1},
   {"tsm_atb_f32_8x8_t4x4_r512_p1", 1, 8, 8, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p1, 1},
@@ -850,13 +848,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_8x8_t8x8_r512_p2", 1, 8, 8, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p2, 1},
   {"tsm_atb_f32_8x8_t8x8_r1024_p1", 1, 8, 8, 8, 8, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p1, 1},
   {"tsm_atb_f32_8x8_t8x8_r1024_p2", 1, 8, 8, 8, 8, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p2, 1},
-  {"tsm_atb_f32_8x16_t8x4_r256_p2", 1, 8, 16, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_8x16_t8x4_r256_p2, 1},
-  {"tsm_atb_f32_8x32_t8x4_r128_p2", 1, 8, 32, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_8x32_t8x4_r128_p2, 1},
-  {"tsm_atb_f32_8x64_t8x4_r64_p2", 1, 8, 64, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2, 1},
-  {"tsm_atb_f32_16x1_t8x1_r256_p2", 1, 16, 1, 8, 1, 256, 2, 0, 34816, launch_tsm_atb_f32_16x1_t8x1_r256_p2, 1},
-  {"tsm_atb_f32_16x2_t8x2_r256_p2", 1, 16, 2, 8, 2, 256, 2, 0, 36864, launch_tsm_atb_f32_16x2_t8x2_r256_p2, 1},
-  {"tsm_atb_f32_16x4_t8x4_r256_p2", 1, 16, 4, 8, 4, 256, 2, 0, 40960, launch_tsm_atb_f32_16x4_t8x4_r256_p2, 1},
-  {"tsm_atb_f32_16x8_t8x4_r256_p2", 1, 16, 8, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f32_16x8_t8x4_r256_p2, 1},
+  {"tsm_atb_f32_8x16_t8x4_r256_p2_nt", 1, 8, 16, 8, 4, 256, 2, 1, 49152, launch_tsm_atb_f32_8x16_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_8x32_t8x4_r128_p2_nt", 1, 8, 32, 8, 4, 128, 2, 1, 40960, launch_tsm_atb_f32_8x32_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_8x64_t8x4_r64_p2_nt", 1, 8, 64, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_16x1_t8x1_r256_p2_nt", 1, 16, 1, 8, 1, 256, 2, 1, 34816, launch_tsm_atb_f32_16x1_t8x1_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x2_t8x2_r256_p2_nt", 1, 16, 2, 8, 2, 256, 2, 1, 36864, launch_tsm_atb_f32_16x2_t8x2_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x4_t8x4_r256_p2_nt", 1, 16, 4, 8, 4, 256, 2, 1, 40960, launch_tsm_atb_f32_16x4_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x8_t8x4_r256_p2_nt", 1, 16, 8, 8, 4, 256, 2, 1, 49152, launch_tsm_atb_f32_16x8_t8x4_r256_p2_nt, 1},
   {"tsm_atb_f32_16x16_direct_d4_nt", 1, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_atb_f32_16x16_direct_d4_nt, 3},
   {"tsm_atb_f32_16x16_direct_d8_nt", 1, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_atb_f32_16x16_direct_d8_nt, 3},
   {"tsm_atb_f32_16x16_direct_d16_nt", 1, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_atb_f32_16x16_direct_d16_nt, 3},
@@ -891,15 +889,15 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_16x16_t8x8_r512_p1", 1, 16, 16, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p1, 1},
   {"tsm_atb_f32_16x16_t8x8_r512_p2", 1, 16, 16, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p2, 1},
   {"tsm_atb_f32_16x32_direct_d8_nt", 1, 16, 32, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f32_16x32_direct_d8_nt, 3},
-  {"tsm_atb_f32_16x32_t8x4_r128_p2", 1, 16, 32, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_16x32_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_16x32_t8x4_r128_p2_nt", 1, 16, 32, 8, 4, 128, 2, 1, 49152, launch_tsm_atb_f32_16x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_16x64_direct_d8_nt", 1, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_16x64_direct_d8_nt, 3},
-  {"tsm_atb_f32_16x64_mfma_r64_p2", 1, 16, 64, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_16x64_mfma_r64_p2, 2},
-  {"tsm_atb_f32_32x1_t8x1_r128_p2", 1, 32, 1, 8, 1, 128, 2, 0, 33792, launch_tsm_atb_f32_32x1_t8x1_r128_p2, 1},
-  {"tsm_atb_f32_32x2_t8x2_r128_p2", 1, 32, 2, 8, 2, 128, 2, 0, 34816, launch_tsm_atb_f32_32x2_t8x2_r128_p2, 1},
-  {"tsm_atb_f32_32x4_t8x4_r128_p2", 1, 32, 4, 8, 4, 128, 2, 0, 36864, launch_tsm_atb_f32_32x4_t8x4_r128_p2, 1},
-  {"tsm_atb_f32_32x8_t8x4_r128_p2", 1, 32, 8, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f32_32x8_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_16x64_mfma_r64_p2_nt", 1, 16, 64, 0, 0, 64, 2, 1, 40960, launch_tsm_atb_f32_16x64_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_32x1_t8x1_r128_p2_nt", 1, 32, 1, 8, 1, 128, 2, 1, 33792, launch_tsm_atb_f32_32x1_t8x1_r128_p2_nt, 1},
+  {"tsm_atb_f32_32x2_t8x2_r128_p2_nt", 1, 32, 2, 8, 2, 128, 2, 1, 34816, launch_tsm_atb_f32_32x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_f32_32x4_t8x4_r128_p2_nt", 1, 32, 4, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_f32_32x4_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_32x8_t8x4_r128_p2_nt", 1, 32, 8, 8, 4, 128, 2, 1, 40960, launch_tsm_atb_f32_32x8_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_32x16_direct_d8_nt", 1, 32, 16, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f32_32x16_direct_d8_nt, 3},
-  {"tsm_atb_f32_32x16_t8x4_r128_p2", 1, 32, 16, 8, 4, 128, 2, 0, 49152, launch_tsm_atb_f32_32x16_t8x4_r128_p2, 1},
+  {"tsm_atb_f32_32x16_t8x4_r128_p2_nt", 1, 32, 16, 8, 4, 128, 2, 1, 49152, launch_tsm_atb_f32_32x16_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_32x32_direct_d4_nt", 1, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_atb_f32_32x32_direct_d4_nt, 3},
   {"tsm_atb_f32_32x32_direct_d8_nt", 1, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_32x32_direct_d8_nt, 3},
   {"tsm_atb_f32_32x32_direct_d16_nt", 1, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_atb_f32_32x32_direct_d16_nt, 3},
@@ -938,15 +936,15 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_32x32_t8x8_r128_p1", 1, 32, 32, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p1, 1},
   {"tsm_atb_f32_32x32_t8x8_r128_p2", 1, 32, 32, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p2, 1},
   {"tsm_atb_f32_32x64_direct_d8_nt", 1, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f32_32x64_direct_d8_nt, 3},
-  {"tsm_atb_f32_32x64_mfma_r64_p2", 1, 32, 64, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2, 2},
-  {"tsm_atb_f32_64x1_t8x1_r64_p2", 1, 64, 1, 8, 1, 64, 2, 0, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2, 1},
-  {"tsm_atb_f32_64x2_t8x2_r64_p2", 1, 64, 2, 8, 2, 64, 2, 0, 33792, launch_tsm_atb_f32_64x2_t8x2_r64_p2, 1},
-  {"tsm_atb_f32_64x4_t8x4_r64_p2", 1, 64, 4, 8, 4, 64, 2, 0, 34816, launch_tsm_atb_f32_64x4_t8x4_r64_p2, 1},
-  {"tsm_atb_f32_64x8_t8x4_r64_p2", 1, 64, 8, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f32_64x8_t8x4_r64_p2, 1},
+  {"tsm_atb_f32_32x64_mfma_r64_p2_nt", 1, 32, 64, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_64x1_t8x1_r64_p2_nt", 1, 64, 1, 8, 1, 64, 2, 1, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt, 1},
+  {"tsm_atb_f32_64x2_t8x2_r64_p2_nt", 1, 64, 2, 8, 2, 64, 2, 1, 33792, launch_tsm_atb_f32_64x2_t8x2_r64_p2_nt, 1},
+  {"tsm_atb_f32_64x4_t8x4_r64_p2_nt", 1, 64, 4, 8, 4, 64, 2, 1, 34816, launch_tsm_atb_f32_64x4_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_64x8_t8x4_r64_p2_nt", 1, 64, 8, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f32_64x8_t8x4_r64_p2_nt, 1},
   {"tsm_atb_f32_64x16_direct_d8_nt", 1, 64, 16, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_64x16_direct_d8_nt, 3},
-  {"tsm_atb_f32_64x16_mfma_r64_p2", 1, 64, 16, 0, 0, 64, 2, 0, 40960, launch_tsm_atb_f32_64x16_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x16_mfma_r64_p2_nt", 1, 64, 16, 0, 0, 64, 2, 1, 40960, launch_tsm_atb_f32_64x16_mfma_r64_p2_nt, 2},
   {"tsm_atb_f32_64x32_direct_d8_nt", 1, 64, 32, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f32_64x32_direct_d8_nt, 3},
-  {"tsm_atb_f32_64x32_mfma_r64_p2", 1, 64, 32, 0, 0, 64, 2, 0, 49152, launch_tsm_atb_f32_64x32_mfma_r64_p2, 2},
+  {"tsm_atb_f32_64x32_mfma_r64_p2_nt", 1, 64, 32, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f32_64x32_mfma_r64_p2_nt, 2},
   {"tsm_atb_f32_64x64_direct_d4_nt", 1, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_atb_f32_64x64_direct_d4_nt, 3},
   {"tsm_atb_f32_64x64_direct_d8_nt", 1, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_atb_f32_64x64_direct_d8_nt, 3},
   {"tsm_atb_f32_64x64_direct_d16_nt", 1, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_atb_f32_64x64_direct_d16_nt, 3},
@@ -986,13 +984,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r4096_p2", 0, 1, 1, 1, 1, 4096, 2, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2, 1},
   {"tsm_atb_f64_1x1_t1x1_r4096_p2_nt", 0, 1, 1, 1, 1, 4096, 2, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2_nt, 1},
-  {"tsm_atb_f64_1x2_t1x2_r1024_p2", 0, 1, 2, 1, 2, 1024, 2, 0, 49152, launch_tsm_atb_f64_1x2_t1x2_r1024_p2, 1},
-  {"tsm_atb_f64_1x4_t1x4_r512_p2", 0, 1, 4, 1, 4, 512, 2, 0, 40960, launch_tsm_atb_f64_1x4_t1x4_r512_p2, 1},
-  {"tsm_atb_f64_1x8_t1x8_r256_p2", 0, 1, 8, 1, 8, 256, 2, 0, 36864, launch_tsm_atb_f64_1x8_t1x8_r256_p2, 1},
-  {"tsm_atb_f64_1x16_t1x8_r128_p2", 0, 1, 16, 1, 8, 128, 2, 0, 34816, launch_tsm_atb_f64_1x16_t1x8_r128_p2, 1},
-  {"tsm_atb_f64_1x32_t1x8_r64_p2", 0, 1, 32, 1, 8, 64, 2, 0, 33792, launch_tsm_atb_f64_1x32_t1x8_r64_p2, 1},
-  {"tsm_atb_f64_1x64_t1x8_r32_p2", 0, 1, 64, 1, 8, 32, 2, 0, 33280, launch_tsm_atb_f64_1x64_t1x8_r32_p2, 1},
-  {"tsm_atb_f64_2x1_t2x1_r1024_p2", 0, 2, 1, 2, 1, 1024, 2, 0, 49152, launch_tsm_atb_f64_2x1_t2x1_r1024_p2, 1},
+  {"tsm_atb_f64_1x2_t1x2_r1024_p2_nt", 0, 1, 2, 1, 2, 1024, 2, 1, 49152, launch_tsm_atb_f64_1x2_t1x2_r1024_p2_nt, 1},
+  {"tsm_atb_f64_1x4_t1x4_r512_p2_nt", 0, 1, 4, 1, 4, 512, 2, 1, 40960, launch_tsm_atb_f64_1x4_t1x4_r512_p2_nt, 1},
+  {"tsm_atb_f64_1x8_t1x8_r256_p2_nt", 0, 1, 8, 1, 8, 256, 2, 1, 36864, launch_tsm_atb_f64_1x8_t1x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_1x16_t1x8_r128_p2_nt", 0, 1, 16, 1, 8, 128, 2, 1, 34816, launch_tsm_atb_f64_1x16_t1x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_1x32_t1x8_r64_p2_nt", 0, 1, 32, 1, 8, 64, 2, 1, 33792, launch_tsm_atb_f64_1x32_t1x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_1x64_t1x8_r32_p2_nt", 0, 1, 64, 1, 8, 32, 2, 1, 33280, launch_tsm_atb_f64_1x64_t1x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_2x1_t2x1_r1024_p2_nt", 0, 2, 1, 2, 1, 1024, 2, 1, 49152, launch_tsm_atb_f64_2x1_t2x1_r1024_p2_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r512_p1", 0, 2, 2, 2, 2, 512, 1, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1, 1},
   {"tsm_atb_f64_2x2_t2x2_r512_p1_nt", 0, 2, 2, 2, 2, 512, 1, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r512_p2", 0, 2, 2, 2, 2, 512, 2, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2, 1},
@@ -1005,13 +1003,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r2048_p2", 0, 2, 2, 2, 2, 2048, 2, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2, 1},
   {"tsm_atb_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt, 1},
-  {"tsm_atb_f64_2x4_t2x4_r512_p2", 0, 2, 4, 2, 4, 512, 2, 0, 49152, launch_tsm_atb_f64_2x4_t2x4_r512_p2, 1},
-  {"tsm_atb_f64_2x8_t2x8_r256_p2", 0, 2, 8, 2, 8, 256, 2, 0, 40960, launch_tsm_atb_f64_2x8_t2x8_r256_p2, 1},
-  {"tsm_atb_f64_2x16_t2x8_r128_p2", 0, 2, 16, 2, 8, 128, 2, 0, 36864, launch_tsm_atb_f64_2x16_t2x8_r128_p2, 1},
-  {"tsm_atb_f64_2x32_t2x8_r64_p2", 0, 2, 32, 2, 8, 64, 2, 0, 34816, launch_tsm_atb_f64_2x32_t2x8_r64_p2, 1},
-  {"tsm_atb_f64_2x64_t2x8_r32_p2", 0, 2, 64, 2, 8, 32, 2, 0, 33792, launch_tsm_atb_f64_2x64_t2x8_r32_p2, 1},
-  {"tsm_atb_f64_4x1_t4x1_r512_p2", 0, 4, 1, 4, 1, 512, 2, 0, 40960, launch_tsm_atb_f64_4x1_t4x1_r512_p2, 1},
-  {"tsm_atb_f64_4x2_t4x2_r512_p2", 0, 4, 2, 4, 2, 512, 2, 0, 49152, launch_tsm_atb_f64_4x2_t4x2_r512_p2, 1},
+  {"tsm_atb_f64_2x4_t2x4_r512_p2_nt", 0, 2, 4, 2, 4, 512, 2, 1, 49152, launch_tsm_atb_f64_2x4_t2x4_r512_p2_nt, 1},
+  {"tsm_atb_f64_2x8_t2x8_r256_p2_nt", 0, 2, 8, 2, 8, 256, 2, 1, 40960, launch_tsm_atb_f64_2x8_t2x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_2x16_t2x8_r128_p2_nt", 0, 2, 16, 2, 8, 128, 2, 1, 36864, launch_tsm_atb_f64_2x16_t2x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_2x32_t2x8_r64_p2_nt", 0, 2, 32, 2, 8, 64, 2, 1, 34816, launch_tsm_atb_f64_2x32_t2x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_2x64_t2x8_r32_p2_nt", 0, 2, 64, 2, 8, 32, 2, 1, 33792, launch_tsm_atb_f64_2x64_t2x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_4x1_t4x1_r512_p2_nt", 0, 4, 1, 4, 1, 512, 2, 1, 40960, launch_tsm_atb_f64_4x1_t4x1_r512_p2_nt, 1},
+  {"tsm_atb_f64_4x2_t4x2_r512_p2_nt", 0, 4, 2, 4, 2, 512, 2, 1, 49152, launch_tsm_atb_f64_4x2_t4x2_r512_p2_nt, 1},
   {"tsm_atb_f64_4x4_t2x2_r512_p1", 0, 4, 4, 2, 2, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p1, 1},
   {"tsm_atb_f64_4x4_t2x2_r512_p2", 0, 4, 4, 2, 2, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t2x2_r512_p2, 1},
   {"tsm_atb_f64_4x4_t4x4_r256_p1", 0, 4, 4, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1, 1},
@@ -1026,8 +1024,8 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p2", 0, 4, 4, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt, 1},
-  {"tsm_atb_f64_4x8_t4x8_r256_p2", 0, 4, 8, 4, 8, 256, 2, 0, 49152, launch_tsm_atb_f64_4x8_t4x8_r256_p2, 1},
-  {"tsm_atb_f64_4x16_t4x8_r128_p2", 0, 4, 16, 4, 8, 128, 2, 0, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_4x8_t4x8_r256_p2_nt", 0, 4, 8, 4, 8, 256, 2, 1, 49152, launch_tsm_atb_f64_4x8_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_4x16_t4x8_r128_p2_nt", 0, 4, 16, 4, 8, 128, 2, 1, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p1", 0, 4, 32, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p1, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p2", 0, 4, 32, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2, 1},
   {"tsm_atb_f64_4x32_t4x4_r64_p1", 0, 4, 32, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p1, 1},
@@ -1042,10 +1040,10 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x32_t4x8_r128_p1_nt", 0, 4, 32, 4, 8, 128, 1, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p2", 0, 4, 32, 4, 8, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p2_nt", 0, 4, 32, 4, 8, 128, 2, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt, 1},
-  {"tsm_atb_f64_4x64_t4x8_r32_p2", 0, 4, 64, 4, 8, 32, 2, 0, 34816, launch_tsm_atb_f64_4x64_t4x8_r32_p2, 1},
-  {"tsm_atb_f64_8x1_t8x1_r256_p2", 0, 8, 1, 8, 1, 256, 2, 0, 36864, launch_tsm_atb_f64_8x1_t8x1_r256_p2, 1},
-  {"tsm_atb_f64_8x2_t8x2_r256_p2", 0, 8, 2, 8, 2, 256, 2, 0, 40960, launch_tsm_atb_f64_8x2_t8x2_r256_p2, 1},
-  {"tsm_atb_f64_8x4_t8x4_r256_p2", 0, 8, 4, 8, 4, 256, 2, 0, 49152, launch_tsm_atb_f64_8x4_t8x4_r256_p2, 1},
+  {"tsm_atb_f64_4x64_t4x8_r32_p2_nt", 0, 4, 64, 4, 8, 32, 2, 1, 34816, launch_tsm_atb_f64_4x64_t4x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_8x1_t8x1_r256_p2_nt", 0, 8, 1, 8, 1, 256, 2, 1, 36864, launch_tsm_atb_f64_8x1_t8x1_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x2_t8x2_r256_p2_nt", 0, 8, 2, 8, 2, 256, 2, 1, 40960, launch_tsm_atb_f64_8x2_t8x2_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x4_t8x4_r256_p2_nt", 0, 8, 4, 8, 4, 256, 2, 1, 49152, launch_tsm_atb_f64_8x4_t8x4_r256_p2_nt, 1},
   {"tsm_atb_f64_8x8_t2x2_r256_p1", 0, 8, 8, 2, 2, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p1, 1},
   {"tsm_atb_f64_8x8_t2x2_r256_p2", 0, 8, 8, 2, 2, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p2, 1},
   {"tsm_atb_f64_8x8_t4x4_r128_p1", 0, 8, 8, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p1, 1},
@@ -1062,13 +1060,13 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_8x8_t8x8_r512_p1_nt", 0, 8, 8, 8, 8, 512, 1, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p1_nt, 1},
   {"tsm_atb_f64_8x8_t8x8_r512_p2", 0, 8, 8, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2, 1},
   {"tsm_atb_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t8x8_r512_p2_nt, 1},
-  {"tsm_atb_f64_8x16_t8x8_r128_p2", 0, 8, 16, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_8x16_t8x8_r128_p2, 1},
-  {"tsm_atb_f64_8x32_t8x8_r64_p2", 0, 8, 32, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_8x32_t8x8_r64_p2, 1},
-  {"tsm_atb_f64_8x64_t8x8_r32_p2", 0, 8, 64, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_8x64_t8x8_r32_p2, 1},
-  {"tsm_atb_f64_16x1_t8x1_r128_p2", 0, 16, 1, 8, 1, 128, 2, 0, 34816, launch_tsm_atb_f64_16x1_t8x1_r128_p2, 1},
-  {"tsm_atb_f64_16x2_t8x2_r128_p2", 0, 16, 2, 8, 2, 128, 2, 0, 36864, launch_tsm_atb_f64_16x2_t8x2_r128_p2, 1},
-  {"tsm_atb_f64_16x4_t8x4_r128_p2", 0, 16, 4, 8, 4, 128, 2, 0, 40960, launch_tsm_atb_f64_16x4_t8x4_r128_p2, 1},
-  {"tsm_atb_f64_16x8_t8x8_r128_p2", 0, 16, 8, 8, 8, 128, 2, 0, 49152, launch_tsm_atb_f64_16x8_t8x8_r128_p2, 1},
+  {"tsm_atb_f64_8x16_t8x8_r128_p2_nt", 0, 8, 16, 8, 8, 128, 2, 1, 49152, launch_tsm_atb_f64_8x16_t8x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_8x32_t8x8_r64_p2_nt", 0, 8, 32, 8, 8, 64, 2, 1, 40960, launch_tsm_atb_f64_8x32_t8x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_8x64_t8x8_r32_p2_nt", 0, 8, 64, 8, 8, 32, 2, 1, 36864, launch_tsm_atb_f64_8x64_t8x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_16x1_t8x1_r128_p2_nt", 0, 16, 1, 8, 1, 128, 2, 1, 34816, launch_tsm_atb_f64_16x1_t8x1_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x2_t8x2_r128_p2_nt", 0, 16, 2, 8, 2, 128, 2, 1, 36864, launch_tsm_atb_f64_16x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x4_t8x4_r128_p2_nt", 0, 16, 4, 8, 4, 128, 2, 1, 40960, launch_tsm_atb_f64_16x4_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x8_t8x8_r128_p2_nt", 0, 16, 8, 8, 8, 128, 2, 1, 49152, launch_tsm_atb_f64_16x8_t8x8_r128_p2_nt, 1},
   {"tsm_atb_f64_16x16_direct_d4_nt", 0, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_atb_f64_16x16_direct_d4_nt, 3},
   {"tsm_atb_f64_16x16_direct_d8_nt", 0, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_atb_f64_16x16_direct_d8_nt, 3},
   {"tsm_atb_f64_16x16_direct_d16_nt", 0, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_atb_f64_16x16_direct_d16_nt, 3},
@@ -1105,11 +1103,11 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_16x16_t8x8_r256_p2", 0, 16, 16, 8, 8, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2, 1},
   {"tsm_atb_f64_16x16_t8x8_r256_p2_nt", 0, 16, 16, 8, 8, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t8x8_r256_p2_nt, 1},
   {"tsm_atb_f64_16x32_direct_d8_nt", 0, 16, 32, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f64_16x32_direct_d8_nt, 3},
-  {"tsm_atb_f64_16x32_t8x8_r64_p2", 0, 16, 32, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_16x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_16x32_t8x8_r64_p2_nt", 0, 16, 32, 8, 8, 64, 2, 1, 49152, launch_tsm_atb_f64_16x32_t8x8_r64_p2_nt, 1},
   {"tsm_atb_f64_16x64_direct_d8_nt", 0, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_16x64_direct_d8_nt, 3},
-  {"tsm_atb_f64_16x64_mfma_r32_p2", 0, 16, 64, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_16x64_mfma_r32_p2, 2},
-  {"tsm_atb_f64_32x1_t8x1_r64_p2", 0, 32, 1, 8, 1, 64, 2, 0, 33792, launch_tsm_atb_f64_32x1_t8x1_r64_p2, 1},
-  {"tsm_atb_f64_32x2_t8x2_r64_p2", 0, 32, 2, 8, 2, 64, 2, 0, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2, 1},
+  {"tsm_atb_f64_16x64_mfma_r32_p2_nt", 0, 16, 64, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_16x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_32x1_t8x1_r64_p2_nt", 0, 32, 1, 8, 1, 64, 2, 1, 33792, launch_tsm_atb_f64_32x1_t8x1_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x2_t8x2_r64_p2_nt", 0, 32, 2, 8, 2, 64, 2, 1, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p1", 0, 32, 4, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p1, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p2", 0, 32, 4, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2, 1},
   {"tsm_atb_f64_32x4_t4x4_r64_p1", 0, 32, 4, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p1, 1},
@@ -1124,9 +1122,9 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_32x4_t8x4_r128_p1_nt", 0, 32, 4, 8, 4, 128, 1, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt, 1},
   {"tsm_atb_f64_32x4_t8x4_r128_p2", 0, 32, 4, 8, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2, 1},
   {"tsm_atb_f64_32x4_t8x4_r128_p2_nt", 0, 32, 4, 8, 4, 128, 2, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2_nt, 1},
-  {"tsm_atb_f64_32x8_t8x8_r64_p2", 0, 32, 8, 8, 8, 64, 2, 0, 40960, launch_tsm_atb_f64_32x8_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_32x8_t8x8_r64_p2_nt", 0, 32, 8, 8, 8, 64, 2, 1, 40960, launch_tsm_atb_f64_32x8_t8x8_r64_p2_nt, 1},
   {"tsm_atb_f64_32x16_direct_d8_nt", 0, 32, 16, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f64_32x16_direct_d8_nt, 3},
-  {"tsm_atb_f64_32x16_t8x8_r64_p2", 0, 32, 16, 8, 8, 64, 2, 0, 49152, launch_tsm_atb_f64_32x16_t8x8_r64_p2, 1},
+  {"tsm_atb_f64_32x16_t8x8_r64_p2_nt", 0, 32, 16, 8, 8, 64, 2, 1, 49152, launch_tsm_atb_f64_32x16_t8x8_r64_p2_nt, 1},
   {"tsm_atb_f64_32x32_direct_d4_nt", 0, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_atb_f64_32x32_direct_d4_nt, 3},
   {"tsm_atb_f64_32x32_direct_d8_nt", 0, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_32x32_direct_d8_nt, 3},
   {"tsm_atb_f64_32x32_direct_d16_nt", 0, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_atb_f64_32x32_direct_d16_nt, 3},
@@ -1159,15 +1157,15 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_32x32_t8x8_r128_p2", 0, 32, 32, 8, 8, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2, 1},
   {"tsm_atb_f64_32x32_t8x8_r128_p2_nt", 0, 32, 32, 8, 8, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt, 1},
   {"tsm_atb_f64_32x64_direct_d8_nt", 0, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f64_32x64_direct_d8_nt, 3},
-  {"tsm_atb_f64_32x64_mfma_r32_p2", 0, 32, 64, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_32x64_mfma_r32_p2, 2},
-  {"tsm_atb_f64_64x1_t8x1_r32_p2", 0, 64, 1, 8, 1, 32, 2, 0, 33280, launch_tsm_atb_f64_64x1_t8x1_r32_p2, 1},
-  {"tsm_atb_f64_64x2_t8x2_r32_p2", 0, 64, 2, 8, 2, 32, 2, 0, 33792, launch_tsm_atb_f64_64x2_t8x2_r32_p2, 1},
-  {"tsm_atb_f64_64x4_t8x4_r32_p2", 0, 64, 4, 8, 4, 32, 2, 0, 34816, launch_tsm_atb_f64_64x4_t8x4_r32_p2, 1},
-  {"tsm_atb_f64_64x8_t8x8_r32_p2", 0, 64, 8, 8, 8, 32, 2, 0, 36864, launch_tsm_atb_f64_64x8_t8x8_r32_p2, 1},
+  {"tsm_atb_f64_32x64_mfma_r32_p2_nt", 0, 32, 64, 0, 0, 32, 2, 1, 49152, launch_tsm_atb_f64_32x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_64x1_t8x1_r32_p2_nt", 0, 64, 1, 8, 1, 32, 2, 1, 33280, launch_tsm_atb_f64_64x1_t8x1_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x2_t8x2_r32_p2_nt", 0, 64, 2, 8, 2, 32, 2, 1, 33792, launch_tsm_atb_f64_64x2_t8x2_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x4_t8x4_r32_p2_nt", 0, 64, 4, 8, 4, 32, 2, 1, 34816, launch_tsm_atb_f64_64x4_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x8_t8x8_r32_p2_nt", 0, 64, 8, 8, 8, 32, 2, 1, 36864, launch_tsm_atb_f64_64x8_t8x8_r32_p2_nt, 1},
   {"tsm_atb_f64_64x16_direct_d8_nt", 0, 64, 16, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_64x16_direct_d8_nt, 3},
-  {"tsm_atb_f64_64x16_mfma_r32_p2", 0, 64, 16, 0, 0, 32, 2, 0, 40960, launch_tsm_atb_f64_64x16_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x16_mfma_r32_p2_nt", 0, 64, 16, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_64x16_mfma_r32_p2_nt, 2},
   {"tsm_atb_f64_64x32_direct_d8_nt", 0, 64, 32, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f64_64x32_direct_d8_nt, 3},
-  {"tsm_atb_f64_64x32_mfma_r32_p2", 0, 64, 32, 0, 0, 32, 2, 0, 49152, launch_tsm_atb_f64_64x32_mfma_r32_p2, 2},
+  {"tsm_atb_f64_64x32_mfma_r32_p2_nt", 0, 64, 32, 0, 0, 32, 2, 1, 49152, launch_tsm_atb_f64_64x32_mfma_r32_p2_nt, 2},
   {"tsm_atb_f64_64x64_direct_d4_nt", 0, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_atb_f64_64x64_direct_d4_nt, 3},
   {"tsm_atb_f64_64x64_direct_d8_nt", 0, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_atb_f64_64x64_direct_d8_nt, 3},
   {"tsm_atb_f64_64x64_direct_d16_nt", 0, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_atb_f64_64x64_direct_d16_nt, 3},
@@ -1198,9 +1196,9 @@ static const tsm::KernelEntry kTable[] = {
 };
 
 static const tsm::KernelEntry kSym[] = {
-  {"tsm_ata_bf16_1x1_t1x1_r4096_p2", 2, 1, 1, 1, 1, 4096, 2, 0, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2, 1},
-  {"tsm_ata_bf16_2x2_t2x2_r4096_p2", 2, 2, 2, 2, 2, 4096, 2, 0, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2, 1},
-  {"tsm_ata_bf16_4x4_t4x4_r4096_p2", 2, 4, 4, 4, 4, 4096, 2, 0, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2, 1},
+  {"tsm_ata_bf16_1x1_t1x1_r4096_p2_nt", 2, 1, 1, 1, 1, 4096, 2, 1, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_ata_bf16_2x2_t2x2_r4096_p2_nt", 2, 2, 2, 2, 2, 4096, 2, 1, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2_nt, 1},
+  {"tsm_ata_bf16_4x4_t4x4_r4096_p2_nt", 2, 4, 4, 4, 4, 4096, 2, 1, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt, 1},
@@ -1225,9 +1223,9 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_bf16_64x64_mfma_r256_p2_nt", 2, 64, 64, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt, 2},
   {"tsm_ata_bf16_64x64_mfma_r512_p1_nt", 2, 64, 64, 0, 0, 512, 1, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt, 2},
   {"tsm_ata_bf16_64x64_mfma_r512_p2_nt", 2, 64, 64, 0, 0, 512, 2, 1, 131072, launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt, 2},
-  {"tsm_ata_f32_1x1_t1x1_r4096_p2", 1, 1, 1, 1, 1, 4096, 2, 0, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2, 1},
-  {"tsm_ata_f32_2x2_t2x2_r4096_p2", 1, 2, 2, 2, 2, 4096, 2, 0, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2, 1},
-  {"tsm_ata_f32_4x4_t4x4_r2048_p2", 1, 4, 4, 4, 4, 2048, 2, 0, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2, 1},
+  {"tsm_ata_f32_1x1_t1x1_r4096_p2_nt", 1, 1, 1, 1, 1, 4096, 2, 1, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2_nt, 1},
+  {"tsm_ata_f32_2x2_t2x2_r4096_p2_nt", 1, 2, 2, 2, 2, 4096, 2, 1, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2_nt, 1},
+  {"tsm_ata_f32_4x4_t4x4_r2048_p2_nt", 1, 4, 4, 4, 4, 2048, 2, 1, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt, 1},
@@ -1280,14 +1278,12 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r256_p1_nt", 0, 32, 32, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f64_32x32_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r256_p2_nt", 0, 32, 32, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f64_32x32_mfma_r256_p2_nt, 2},
   {"tsm_ata_f64_64x64_direct_d4_nt", 0, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_ata_f64_64x64_direct_d4_nt, 3},
   {"tsm_ata_f64_64x64_direct_d8_nt", 0, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_ata_f64_64x64_direct_d8_nt, 3},
   {"tsm_ata_f64_64x64_direct_d16_nt", 0, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_ata_f64_64x64_direct_d16_nt, 3},
@@ -1295,177 +1291,177 @@ static const tsm::KernelEntry kSym[] = {
 };
 
 static const tsm::TunedEntry kSymTuned[] = {
-  {2, 1, 1, 0, 2, 0, 0, 0},  // tsm_ata_bf16_1x1_t1x1_r4096_p2
-  {2, 2, 2, 1, 2, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2
-  {2, 4, 4, 2, 2, 0, 2, 0},  // tsm_ata_bf16_4x4_t4x4_r4096_p2
+  {2, 1, 1, 0, 1, 1, 0, 1},  // tsm_ata_bf16_1x1_t1x1_r4096_p2_nt
+  {2, 2, 2, 1, 1, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2_nt
+  {2, 4, 4, 2, 1, 1, 2, 1},  // tsm_ata_bf16_4x4_t4x4_r4096_p2_nt
   {2, 8, 8, 6, 6, 1, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p2_nt
   {2, 16, 16, 10, 1, 1, 10, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
   {2, 32, 32, 16, 1, 1, 16, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
   {2, 64, 64, 26, 1, 0, 26, 0},  // tsm_ata_bf16_64x64_mfma_r512_p2_nt
-  {1, 1, 1, 27, 2, 0, 27, 0},  // tsm_ata_f32_1x1_t1x1_r4096_p2
-  {1, 2, 2, 28, 2, 0, 28, 0},  // tsm_ata_f32_2x2_t2x2_r4096_p2
-  {1, 4, 4, 29, 2, 0, 29, 0},  // tsm_ata_f32_4x4_t4x4_r2048_p2
+  {1, 1, 1, 27, 1, 1, 27, 1},  // tsm_ata_f32_1x1_t1x1_r4096_p2_nt
+  {1, 2, 2, 28, 1, 1, 28, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
+  {1, 4, 4, 29, 1, 1, 29, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
   {1, 8, 8, 32, 2, 0, 32, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
   {1, 16, 16, 41, 1, 0, 41, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
   {1, 32, 32, 45, 3, 0, 45, 0},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
-  {1, 64, 64, 50, 2, 0, 51, 0},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2_nt)
+  {1, 64, 64, 50, 2, 0, 51, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2_nt)
   {0, 1, 1, 52, 2, 1, 52, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
   {0, 2, 2, 57, 1, 1, 57, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
   {0, 4, 4, 65, 1, 1, 65, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
   {0, 8, 8, 71, 1, 0, 71, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
   {0, 16, 16, 81, 1, 0, 81, 0},  // tsm_ata_f64_16x16_mfma_r256_p2_nt
-  {0, 32, 32, 88, 2, 0, 88, 0},  // tsm_ata_f64_32x32_mfma_r128_p1_nt
-  {0, 64, 64, 92, 2, 0, 93, 1},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
+  {0, 32, 32, 84, 2, 0, 84, 0},  // tsm_ata_f64_32x32_mfma_r128_p1_nt
+  {0, 64, 64, 90, 2, 0, 91, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
 };
 
 static const tsm::TunedEntry kTuned[] = {
-  {2, 1, 1, 0, 2, 0, 0, 0},  // tsm_atb_bf16_1x1_t1x1_r4096_p2
-  {2, 1, 2, 1, 2, 0, 1, 0},  // tsm_atb_bf16_1x2_t1x2_r4096_p2
-  {2, 1, 4, 2, 2, 0, 2, 0},  // tsm_atb_bf16_1x4_t1x4_r2048_p2
-  {2, 1, 8, 3, 2, 0, 3, 0},  // tsm_atb_bf16_1x8_t1x4_r1024_p2
-  {2, 1, 16, 4, 2, 0, 4, 0},  // tsm_atb_bf16_1x16_t1x4_r512_p2
-  {2, 1, 32, 5, 2, 0, 5, 0},  // tsm_atb_bf16_1x32_t1x4_r256_p2
-  {2, 1, 64, 6, 2, 0, 6, 0},  // tsm_atb_bf16_1x64_t1x4_r128_p2
-  {2, 2, 1, 7, 2, 0, 7, 0},  // tsm_atb_bf16_2x1_t2x1_r4096_p2
-  {2, 2, 2, 8, 2, 0, 8, 0},  // tsm_atb_bf16_2x2_t2x2_r4096_p2
-  {2, 2, 4, 9, 2, 0, 9, 0},  // tsm_atb_bf16_2x4_t2x4_r2048_p2
-  {2, 2, 8, 10, 2, 0, 10, 0},  // tsm_atb_bf16_2x8_t2x4_r1024_p2
-  {2, 2, 16, 11, 2, 0, 11, 0},  // tsm_atb_bf16_2x16_t2x4_r512_p2
-  {2, 2, 32, 12, 2, 0, 12, 0},  // tsm_atb_bf16_2x32_t2x4_r256_p2
-  {2, 2, 64, 13, 2, 0, 13, 0},  // tsm_atb_bf16_2x64_t2x4_r128_p2
-  {2, 4, 1, 14, 2, 0, 14, 0},  // tsm_atb_bf16_4x1_t4x1_r2048_p2
-  {2, 4, 2, 15, 2, 0, 15, 0},  // tsm_atb_bf16_4x2_t4x2_r2048_p2
-  {2, 4, 4, 16, 2, 0, 16, 0},  // tsm_atb_bf16_4x4_t4x4_r2048_p2
-  {2, 4, 8, 17, 2, 0, 17, 0},  // tsm_atb_bf16_4x8_t4x4_r1024_p2
-  {2, 4, 16, 18, 2, 0, 18, 0},  // tsm_atb_bf16_4x16_t4x4_r512_p2
-  {2, 4, 32, 19, 2, 0, 19, 0},  // tsm_atb_bf16_4x32_t4x4_r256_p2
-  {2, 4, 64, 20, 2, 0, 20, 0},  // tsm_atb_bf16_4x64_t4x4_r128_p2
-  {2, 8, 1, 21, 2, 0, 21, 0},  // tsm_atb_bf16_8x1_t8x1_r1024_p2
-  {2, 8, 2, 22, 2, 0, 22, 0},  // tsm_atb_bf16_8x2_t8x2_r1024_p2
-  {2, 8, 4, 23, 2, 0, 23, 0},  // tsm_atb_bf16_8x4_t8x4_r1024_p2
+  {2, 1, 1, 0, 1, 1, 0, 1},  // tsm_atb_bf16_1x1_t1x1_r4096_p2_nt
+  {2, 1, 2, 1, 1, 1, 1, 1},  // tsm_atb_bf16_1x2_t1x2_r4096_p2_nt
+  {2, 1, 4, 2, 1, 1, 2, 1},  // tsm_atb_bf16_1x4_t1x4_r2048_p2_nt
+  {2, 1, 8, 3, 2, 1, 3, 1},  // tsm_atb_bf16_1x8_t1x4_r1024_p2_nt
+  {2, 1, 16, 4, 1, 1, 4, 1},  // tsm_atb_bf16_1x16_t1x4_r512_p2_nt
+  {2, 1, 32, 5, 1, 1, 5, 1},  // tsm_atb_bf16_1x32_t1x4_r256_p2_nt
+  {2, 1, 64, 6, 1, 1, 6, 1},  // tsm_atb_bf16_1x64_t1x4_r128_p2_nt
+  {2, 2, 1, 7, 1, 0, 7, 0},  // tsm_atb_bf16_2x1_t2x1_r4096_p2_nt
+  {2, 2, 2, 8, 1, 0, 8, 0},  // tsm_atb_bf16_2x2_t2x2_r4096_p2_nt
+  {2, 2, 4, 9, 1, 1, 9, 1},  // tsm_atb_bf16_2x4_t2x4_r2048_p2_nt
+  {2, 2, 8, 10, 1, 1, 10, 1},  // tsm_atb_bf16_2x8_t2x4_r1024_p2_nt
+  {2, 2, 16, 11, 2, 1, 11, 1},  // tsm_atb_bf16_2x16_t2x4_r512_p2_nt
+  {2, 2, 32, 12, 2, 1, 12, 1},  // tsm_atb_bf16_2x32_t2x4_r256_p2_nt
+  {2, 2, 64, 13, 2, 1, 13, 1},  // tsm_atb_bf16_2x64_t2x4_r128_p2_nt
+  {2, 4, 1, 14, 1, 1, 14, 1},  // tsm_atb_bf16_4x1_t4x1_r2048_p2_nt
+  {2, 4, 2, 15, 1, 1, 15, 1},  // tsm_atb_bf16_4x2_t4x2_r2048_p2_nt
+  {2, 4, 4, 16, 1, 1, 16, 1},  // tsm_atb_bf16_4x4_t4x4_r2048_p2_nt
+  {2, 4, 8, 17, 1, 1, 17, 1},  // tsm_atb_bf16_4x8_t4x4_r1024_p2_nt
+  {2, 4, 16, 18, 2, 0, 18, 0},  // tsm_atb_bf16_4x16_t4x4_r512_p2_nt
+  {2, 4, 32, 19, 2, 1, 19, 1},  // tsm_atb_bf16_4x32_t4x4_r256_p2_nt
+  {2, 4, 64, 20, 2, 1, 20, 1},  // tsm_atb_bf16_4x64_t4x4_r128_p2_nt
+  {2, 8, 1, 21, 2, 1, 21, 1},  // tsm_atb_bf16_8x1_t8x1_r1024_p2_nt
+  {2, 8, 2, 22, 1, 1, 22, 1},  // tsm_atb_bf16_8x2_t8x2_r1024_p2_nt
+  {2, 8, 4, 23, 1, 1, 23, 1},  // tsm_atb_bf16_8x4_t8x4_r1024_p2_nt
   {2, 8, 8, 33, 2, 1, 33, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 8, 16, 46, 2, 0, 46, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2
-  {2, 8, 32, 47, 2, 0, 47, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2
-  {2, 8, 64, 48, 2, 0, 48, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2
-  {2, 16, 1, 49, 2, 0, 49, 0},  // tsm_atb_bf16_16x1_t8x1_r512_p2
-  {2, 16, 2, 50, 2, 0, 50, 0},  // tsm_atb_bf16_16x2_t8x2_r512_p2
-  {2, 16, 4, 51, 2, 0, 51, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2
-  {2, 16, 8, 52, 2, 0, 52, 0},  // tsm_atb_bf16_16x8_t8x4_r512_p2
+  {2, 8, 16, 46, 2, 0, 46, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2_nt
+  {2, 8, 32, 47, 2, 0, 47, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2_nt
+  {2, 8, 64, 48, 2, 0, 48, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2_nt
+  {2, 16, 1, 49, 2, 1, 49, 1},  // tsm_atb_bf16_16x1_t8x1_r512_p2_nt
+  {2, 16, 2, 50, 2, 1, 50, 1},  // tsm_atb_bf16_16x2_t8x2_r512_p2_nt
+  {2, 16, 4, 51, 1, 0, 51, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2_nt
+  {2, 16, 8, 52, 2, 1, 52, 1},  // tsm_atb_bf16_16x8_t8x4_r512_p2_nt
   {2, 16, 16, 62, 1, 1, 62, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
-  {2, 16, 32, 87, 2, 0, 87, 0},  // tsm_atb_bf16_16x32_mfma_r256_p2
-  {2, 16, 64, 88, 2, 0, 88, 0},  // tsm_atb_bf16_16x64_mfma_r128_p2
-  {2, 32, 1, 89, 2, 0, 89, 0},  // tsm_atb_bf16_32x1_t8x1_r256_p2
-  {2, 32, 2, 90, 2, 0, 90, 0},  // tsm_atb_bf16_32x2_t8x2_r256_p2
-  {2, 32, 4, 91, 2, 0, 91, 0},  // tsm_atb_bf16_32x4_t8x4_r256_p2
-  {2, 32, 8, 92, 2, 0, 92, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2
-  {2, 32, 16, 93, 2, 0, 93, 0},  // tsm_atb_bf16_32x16_mfma_r256_p2
+  {2, 16, 32, 87, 1, 1, 87, 1},  // tsm_atb_bf16_16x32_mfma_r256_p2_nt
+  {2, 16, 64, 88, 1, 1, 88, 1},  // tsm_atb_bf16_16x64_mfma_r128_p2_nt
+  {2, 32, 1, 89, 2, 1, 89, 1},  // tsm_atb_bf16_32x1_t8x1_r256_p2_nt
+  {2, 32, 2, 90, 2, 1, 90, 1},  // tsm_atb_bf16_32x2_t8x2_r256_p2_nt
+  {2, 32, 4, 91, 2, 1, 91, 1},  // tsm_atb_bf16_32x4_t8x4_r256_p2_nt
+  {2, 32, 8, 92, 2, 0, 92, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2_nt
+  {2, 32, 16, 93, 1, 1, 93, 1},  // tsm_atb_bf16_32x16_mfma_r256_p2_nt
   {2, 32, 32, 103, 1, 1, 103, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
-  {2, 32, 64, 128, 2, 0, 128, 0},  // tsm_atb_bf16_32x64_mfma_r128_p2
-  {2, 64, 1, 129, 2, 0, 129, 0},  // tsm_atb_bf16_64x1_t8x1_r128_p2
-  {2, 64, 2, 130, 2, 0, 130, 0},  // tsm_atb_bf16_64x2_t8x2_r128_p2
-  {2, 64, 4, 131, 2, 0, 131, 0},  // tsm_atb_bf16_64x4_t8x4_r128_p2
-  {2, 64, 8, 132, 2, 0, 132, 0},  // tsm_atb_bf16_64x8_t8x4_r128_p2
-  {2, 64, 16, 133, 2, 0, 133, 0},  // tsm_atb_bf16_64x16_mfma_r128_p2
-  {2, 64, 32, 134, 2, 0, 134, 0},  // tsm_atb_bf16_64x32_mfma_r128_p2
+  {2, 32, 64, 128, 1, 1, 128, 1},  // tsm_atb_bf16_32x64_mfma_r128_p2_nt
+  {2, 64, 1, 129, 2, 1, 129, 1},  // tsm_atb_bf16_64x1_t8x1_r128_p2_nt
+  {2, 64, 2, 130, 2, 1, 130, 1},  // tsm_atb_bf16_64x2_t8x2_r128_p2_nt
+  {2, 64, 4, 131, 2, 1, 131, 1},  // tsm_atb_bf16_64x4_t8x4_r128_p2_nt
+  {2, 64, 8, 132, 2, 1, 132, 1},  // tsm_atb_bf16_64x8_t8x4_r128_p2_nt
+  {2, 64, 16, 133, 1, 1, 133, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
+  {2, 64, 32, 134, 2, 1, 134, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
   {2, 64, 64, 142, 1, 1, 142, 1},  // tsm_atb_bf16_64x64_mfma_r256_p2_nt
-  {1, 1, 1, 155, 2, 0, 155, 0},  // tsm_atb_f32_1x1_t1x1_r4096_p2
-  {1, 1, 2, 156, 2, 0, 156, 0},  // tsm_atb_f32_1x2_t1x2_r2048_p2
-  {1, 1, 4, 157, 2, 0, 157, 0},  // tsm_atb_f32_1x4_t1x4_r1024_p2
-  {1, 1, 8, 158, 2, 0, 158, 0},  // tsm_atb_f32_1x8_t1x4_r512_p2
-  {1, 1, 16, 159, 2, 0, 159, 0},  // tsm_atb_f32_1x16_t1x4_r256_p2
-  {1, 1, 32, 160, 2, 0, 160, 0},  // tsm_atb_f32_1x32_t1x4_r128_p2
-  {1, 1, 64, 161, 2, 0, 161, 0},  // tsm_atb_f32_1x64_t1x4_r64_p2
-  {1, 2, 1, 162, 2, 0, 162, 0},  // tsm_atb_f32_2x1_t2x1_r2048_p2
-  {1, 2, 2, 163, 2, 0, 163, 0},  // tsm_atb_f32_2x2_t2x2_r2048_p2
-  {1, 2, 4, 164, 2, 0, 164, 0},  // tsm_atb_f32_2x4_t2x4_r1024_p2
-  {1, 2, 8, 165, 2, 0, 165, 0},  // tsm_atb_f32_2x8_t2x4_r512_p2
-  {1, 2, 16, 166, 2, 0, 166, 0},  // tsm_atb_f32_2x16_t2x4_r256_p2
-  {1, 2, 32, 167, 2, 0, 167, 0},  // tsm_atb_f32_2x32_t2x4_r128_p2
-  {1, 2, 64, 168, 2, 0, 168, 0},  // tsm_atb_f32_2x64_t2x4_r64_p2
-  {1, 4, 1, 169, 2, 0, 169, 0},  // tsm_atb_f32_4x1_t4x1_r1024_p2
-  {1, 4, 2, 170, 2, 0, 170, 0},  // tsm_atb_f32_4x2_t4x2_r1024_p2
-  {1, 4, 4, 171, 2, 0, 171, 0},  // tsm_atb_f32_4x4_t4x4_r1024_p2
-  {1, 4, 8, 172, 2, 0, 172, 0},  // tsm_atb_f32_4x8_t4x4_r512_p2
-  {1, 4, 16, 173, 2, 0, 173, 0},  // tsm_atb_f32_4x16_t4x4_r256_p2
-  {1, 4, 32, 174, 2, 0, 174, 0},  // tsm_atb_f32_4x32_t4x4_r128_p2
-  {1, 4, 64, 175, 2, 0, 175, 0},  // tsm_atb_f32_4x64_t4x4_r64_p2
-  {1, 8, 1, 176, 2, 0, 176, 0},  // tsm_atb_f32_8x1_t8x1_r512_p2
-  {1, 8, 2, 177, 2, 0, 177, 0},  // tsm_atb_f32_8x2_t8x2_r512_p2
-  {1, 8, 4, 178, 2, 0, 178, 0},  // tsm_atb_f32_8x4_t8x4_r512_p2
+  {1, 1, 1, 155, 1, 1, 155, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
+  {1, 1, 2, 156, 1, 1, 156, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
+  {1, 1, 4, 157, 1, 1, 157, 1},  // tsm_atb_f32_1x4_t1x4_r1024_p2_nt
+  {1, 1, 8, 158, 1, 1, 158, 1},  // tsm_atb_f32_1x8_t1x4_r512_p2_nt
+  {1, 1, 16, 159, 1, 1, 159, 1},  // tsm_atb_f32_1x16_t1x4_r256_p2_nt
+  {1, 1, 32, 160, 1, 1, 160, 1},  // tsm_atb_f32_1x32_t1x4_r128_p2_nt
+  {1, 1, 64, 161, 1, 1, 161, 1},  // tsm_atb_f32_1x64_t1x4_r64_p2_nt
+  {1, 2, 1, 162, 1, 1, 162, 1},  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
+  {1, 2, 2, 163, 1, 1, 163, 1},  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
+  {1, 2, 4, 164, 1, 1, 164, 1},  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
+  {1, 2, 8, 165, 1, 1, 165, 1},  // tsm_atb_f32_2x8_t2x4_r512_p2_nt
+  {1, 2, 16, 166, 1, 1, 166, 1},  // tsm_atb_f32_2x16_t2x4_r256_p2_nt
+  {1, 2, 32, 167, 1, 1, 167, 1},  // tsm_atb_f32_2x32_t2x4_r128_p2_nt
+  {1, 2, 64, 168, 1, 1, 168, 1},  // tsm_atb_f32_2x64_t2x4_r64_p2_nt
+  {1, 4, 1, 169, 1, 1, 169, 1},  // tsm_atb_f32_4x1_t4x1_r1024_p2_nt
+  {1, 4, 2, 170, 1, 1, 170, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
+  {1, 4, 4, 171, 1, 1, 171, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
+  {1, 4, 8, 172, 1, 1, 172, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
+  {1, 4, 16, 173, 1, 1, 173, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
+  {1, 4, 32, 174, 1, 1, 174, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
+  {1, 4, 64, 175, 1, 1, 175, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
+  {1, 8, 1, 176, 2, 1, 176, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
+  {1, 8, 2, 177, 1, 1, 177, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
+  {1, 8, 4, 178, 1, 1, 178, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
   {1, 8, 8, 196, 1, 1, 196, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
-  {1, 8, 16, 203, 2, 0, 203, 0},  // tsm_atb_f32_8x16_t8x4_r256_p2
-  {1, 8, 32, 204, 2, 0, 204, 0},  // tsm_atb_f32_8x32_t8x4_r128_p2
-  {1, 8, 64, 205, 2, 0, 205, 0},  // tsm_atb_f32_8x64_t8x4_r64_p2
-  {1, 16, 1, 206, 2, 0, 206, 0},  // tsm_atb_f32_16x1_t8x1_r256_p2
-  {1, 16, 2, 207, 2, 0, 207, 0},  // tsm_atb_f32_16x2_t8x2_r256_p2
-  {1, 16, 4, 208, 2, 0, 208, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2
-  {1, 16, 8, 209, 2, 0, 209, 0},  // tsm_atb_f32_16x8_t8x4_r256_p2
+  {1, 8, 16, 203, 1, 1, 203, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
+  {1, 8, 32, 204, 1, 1, 204, 1},  // tsm_atb_f32_8x32_t8x4_r128_p2_nt
+  {1, 8, 64, 205, 2, 1, 205, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
+  {1, 16, 1, 206, 2, 1, 206, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
+  {1, 16, 2, 207, 2, 1, 207, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
+  {1, 16, 4, 208, 1, 0, 208, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
+  {1, 16, 8, 209, 1, 1, 209, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
   {1, 16, 16, 216, 1, 1, 216, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-  {1, 16, 32, 244, 2, 0, 244, 0},  // tsm_atb_f32_16x32_t8x4_r128_p2
-  {1, 16, 64, 245, 1, 0, 246, 0},  // tsm_atb_f32_16x64_direct_d8_nt (else tsm_atb_f32_16x64_mfma_r64_p2)
-  {1, 32, 1, 247, 2, 0, 247, 0},  // tsm_atb_f32_32x1_t8x1_r128_p2
-  {1, 32, 2, 248, 2, 0, 248, 0},  // tsm_atb_f32_32x2_t8x2_r128_p2
-  {1, 32, 4, 249, 2, 0, 249, 0},  // tsm_atb_f32_32x4_t8x4_r128_p2
-  {1, 32, 8, 250, 2, 0, 250, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2
-  {1, 32, 16, 252, 2, 0, 252, 0},  // tsm_atb_f32_32x16_t8x4_r128_p2
+  {1, 16, 32, 243, 1, 0, 244, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
+  {1, 16, 64, 246, 1, 1, 246, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
+  {1, 32, 1, 247, 2, 1, 247, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
+  {1, 32, 2, 248, 2, 1, 248, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
+  {1, 32, 4, 249, 2, 1, 249, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
+  {1, 32, 8, 250, 1, 0, 250, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2_nt
+  {1, 32, 16, 251, 1, 0, 252, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
   {1, 32, 32, 263, 1, 1, 263, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 290, 1, 0, 291, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2)
-  {1, 64, 1, 292, 2, 0, 292, 0},  // tsm_atb_f32_64x1_t8x1_r64_p2
-  {1, 64, 2, 293, 2, 0, 293, 0},  // tsm_atb_f32_64x2_t8x2_r64_p2
-  {1, 64, 4, 294, 2, 0, 294, 0},  // tsm_atb_f32_64x4_t8x4_r64_p2
-  {1, 64, 8, 295, 2, 0, 295, 0},  // tsm_atb_f32_64x8_t8x4_r64_p2
-  {1, 64, 16, 296, 1, 0, 297, 0},  // tsm_atb_f32_64x16_direct_d8_nt (else tsm_atb_f32_64x16_mfma_r64_p2)
-  {1, 64, 32, 298, 1, 0, 299, 0},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2)
-  {1, 64, 64, 301, 1, 0, 310, 0},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
+  {1, 32, 64, 291, 2, 1, 291, 1},  // tsm_atb_f32_32x64_mfma_r64_p2_nt
+  {1, 64, 1, 292, 2, 1, 292, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
+  {1, 64, 2, 293, 2, 1, 293, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
+  {1, 64, 4, 294, 2, 1, 294, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
+  {1, 64, 8, 295, 2, 1, 295, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
+  {1, 64, 16, 297, 1, 1, 297, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
+  {1, 64, 32, 298, 1, 0, 299, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
+  {1, 64, 64, 302, 1, 0, 310, 1},  // tsm_atb_f32_64x64_direct_d16_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
   {0, 1, 1, 336, 1, 1, 336, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
-  {0, 1, 2, 339, 2, 0, 339, 0},  // tsm_atb_f64_1x2_t1x2_r1024_p2
-  {0, 1, 4, 340, 2, 0, 340, 0},  // tsm_atb_f64_1x4_t1x4_r512_p2
-  {0, 1, 8, 341, 2, 0, 341, 0},  // tsm_atb_f64_1x8_t1x8_r256_p2
-  {0, 1, 16, 342, 2, 0, 342, 0},  // tsm_atb_f64_1x16_t1x8_r128_p2
-  {0, 1, 32, 343, 2, 0, 343, 0},  // tsm_atb_f64_1x32_t1x8_r64_p2
-  {0, 1, 64, 344, 2, 0, 344, 0},  // tsm_atb_f64_1x64_t1x8_r32_p2
-  {0, 2, 1, 345, 2, 0, 345, 0},  // tsm_atb_f64_2x1_t2x1_r1024_p2
+  {0, 1, 2, 339, 1, 1, 339, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
+  {0, 1, 4, 340, 1, 1, 340, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
+  {0, 1, 8, 341, 1, 1, 341, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
+  {0, 1, 16, 342, 1, 1, 342, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
+  {0, 1, 32, 343, 1, 1, 343, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
+  {0, 1, 64, 344, 1, 1, 344, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
+  {0, 2, 1, 345, 1, 1, 345, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
   {0, 2, 2, 355, 1, 1, 355, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
-  {0, 2, 4, 358, 2, 0, 358, 0},  // tsm_atb_f64_2x4_t2x4_r512_p2
-  {0, 2, 8, 359, 2, 0, 359, 0},  // tsm_atb_f64_2x8_t2x8_r256_p2
-  {0, 2, 16, 360, 2, 0, 360, 0},  // tsm_atb_f64_2x16_t2x8_r128_p2
-  {0, 2, 32, 361, 2, 0, 361, 0},  // tsm_atb_f64_2x32_t2x8_r64_p2
-  {0, 2, 64, 362, 2, 0, 362, 0},  // tsm_atb_f64_2x64_t2x8_r32_p2
-  {0, 4, 1, 363, 2, 0, 363, 0},  // tsm_atb_f64_4x1_t4x1_r512_p2
-  {0, 4, 2, 364, 2, 0, 364, 0},  // tsm_atb_f64_4x2_t4x2_r512_p2
+  {0, 2, 4, 358, 1, 1, 358, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
+  {0, 2, 8, 359, 1, 1, 359, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
+  {0, 2, 16, 360, 1, 1, 360, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
+  {0, 2, 32, 361, 1, 1, 361, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
+  {0, 2, 64, 362, 1, 1, 362, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
+  {0, 4, 1, 363, 1, 1, 363, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
+  {0, 4, 2, 364, 1, 1, 364, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
   {0, 4, 4, 368, 2, 1, 368, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-  {0, 4, 8, 379, 2, 0, 379, 0},  // tsm_atb_f64_4x8_t4x8_r256_p2
-  {0, 4, 16, 380, 2, 0, 380, 0},  // tsm_atb_f64_4x16_t4x8_r128_p2
+  {0, 4, 8, 379, 1, 1, 379, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
+  {0, 4, 16, 380, 1, 1, 380, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
   {0, 4, 32, 388, 2, 1, 388, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-  {0, 4, 64, 395, 2, 0, 395, 0},  // tsm_atb_f64_4x64_t4x8_r32_p2
-  {0, 8, 1, 396, 2, 0, 396, 0},  // tsm_atb_f64_8x1_t8x1_r256_p2
-  {0, 8, 2, 397, 2, 0, 397, 0},  // tsm_atb_f64_8x2_t8x2_r256_p2
-  {0, 8, 4, 398, 2, 0, 398, 0},  // tsm_atb_f64_8x4_t8x4_r256_p2
+  {0, 4, 64, 395, 1, 1, 395, 1},  // tsm_atb_f64_4x64_t4x8_r32_p2_nt
+  {0, 8, 1, 396, 2, 1, 396, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
+  {0, 8, 2, 397, 1, 1, 397, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
+  {0, 8, 4, 398, 1, 1, 398, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
   {0, 8, 8, 410, 1, 1, 410, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
-  {0, 8, 16, 415, 2, 0, 415, 0},  // tsm_atb_f64_8x16_t8x8_r128_p2
-  {0, 8, 32, 416, 2, 0, 416, 0},  // tsm_atb_f64_8x32_t8x8_r64_p2
-  {0, 8, 64, 417, 2, 0, 417, 0},  // tsm_atb_f64_8x64_t8x8_r32_p2
-  {0, 16, 1, 418, 2, 0, 418, 0},  // tsm_atb_f64_16x1_t8x1_r128_p2
-  {0, 16, 2, 419, 2, 0, 419, 0},  // tsm_atb_f64_16x2_t8x2_r128_p2
-  {0, 16, 4, 420, 2, 0, 420, 0},  // tsm_atb_f64_16x4_t8x4_r128_p2
-  {0, 16, 8, 421, 2, 0, 421, 0},  // tsm_atb_f64_16x8_t8x8_r128_p2
-  {0, 16, 16, 428, 1, 0, 428, 0},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
-  {0, 16, 32, 458, 2, 0, 458, 0},  // tsm_atb_f64_16x32_t8x8_r64_p2
-  {0, 16, 64, 459, 1, 0, 460, 0},  // tsm_atb_f64_16x64_direct_d8_nt (else tsm_atb_f64_16x64_mfma_r32_p2)
-  {0, 32, 1, 461, 2, 0, 461, 0},  // tsm_atb_f64_32x1_t8x1_r64_p2
-  {0, 32, 2, 462, 2, 0, 462, 0},  // tsm_atb_f64_32x2_t8x2_r64_p2
+  {0, 8, 16, 415, 1, 1, 415, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
+  {0, 8, 32, 416, 1, 1, 416, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
+  {0, 8, 64, 417, 2, 1, 417, 1},  // tsm_atb_f64_8x64_t8x8_r32_p2_nt
+  {0, 16, 1, 418, 2, 1, 418, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
+  {0, 16, 2, 419, 2, 1, 419, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
+  {0, 16, 4, 420, 1, 1, 420, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
+  {0, 16, 8, 421, 1, 1, 421, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
+  {0, 16, 16, 428, 1, 1, 428, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
+  {0, 16, 32, 457, 1, 0, 458, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
+  {0, 16, 64, 460, 1, 1, 460, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
+  {0, 32, 1, 461, 2, 1, 461, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
+  {0, 32, 2, 462, 2, 1, 462, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
   {0, 32, 4, 470, 2, 1, 470, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
-  {0, 32, 8, 477, 2, 0, 477, 0},  // tsm_atb_f64_32x8_t8x8_r64_p2
-  {0, 32, 16, 479, 2, 0, 479, 0},  // tsm_atb_f64_32x16_t8x8_r64_p2
-  {0, 32, 32, 486, 1, 0, 486, 0},  // tsm_atb_f64_32x32_mfma_r32_p2_nt
-  {0, 32, 64, 511, 1, 0, 512, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2)
-  {0, 64, 1, 513, 2, 0, 513, 0},  // tsm_atb_f64_64x1_t8x1_r32_p2
-  {0, 64, 2, 514, 2, 0, 514, 0},  // tsm_atb_f64_64x2_t8x2_r32_p2
-  {0, 64, 4, 515, 2, 0, 515, 0},  // tsm_atb_f64_64x4_t8x4_r32_p2
-  {0, 64, 8, 516, 2, 0, 516, 0},  // tsm_atb_f64_64x8_t8x8_r32_p2
-  {0, 64, 16, 517, 1, 0, 518, 0},  // tsm_atb_f64_64x16_direct_d8_nt (else tsm_atb_f64_64x16_mfma_r32_p2)
-  {0, 64, 32, 519, 1, 0, 520, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2)
-  {0, 64, 64, 522, 1, 0, 531, 1},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
+  {0, 32, 8, 477, 1, 1, 477, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
+  {0, 32, 16, 478, 1, 0, 479, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
+  {0, 32, 32, 494, 1, 0, 494, 0},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
+  {0, 32, 64, 512, 2, 0, 512, 0},  // tsm_atb_f64_32x64_mfma_r32_p2_nt
+  {0, 64, 1, 513, 2, 1, 513, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
+  {0, 64, 2, 514, 2, 1, 514, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
+  {0, 64, 4, 515, 2, 1, 515, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
+  {0, 64, 8, 516, 2, 1, 516, 1},  // tsm_atb_f64_64x8_t8x8_r32_p2_nt
+  {0, 64, 16, 518, 1, 1, 518, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
+  {0, 64, 32, 520, 1, 0, 520, 0},  // tsm_atb_f64_64x32_mfma_r32_p2_nt
+  {0, 64, 64, 523, 2, 0, 531, 0},  // tsm_atb_f64_64x64_direct_d16_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
@@ -1482,11 +1478,11 @@ static const tsm::BatchedKernel* batched_all() {
   static bool init = [] {
     all[0].name = "tsm_atb_batched_f64"; all[0].dtype = 0;
     all[0].classes = classes_tsm_atb_batched_f64(&all[0].n_classes);
-    all[0].lds_bytes = 65536; all[0].launch = launch_tsm_atb_batched_f64;
+    all[0].lds_bytes = 81920; all[0].launch = launch_tsm_atb_batched_f64;
     all[0].occupancy = occupancy_tsm_atb_batched_f64;
     all[1].name = "tsm_atb_batched_f32"; all[1].dtype = 1;
     all[1].classes = classes_tsm_atb_batched_f32(&all[1].n_classes);
-    all[1].lds_bytes = 65536; all[1].launch = launch_tsm_atb_batched_f32;
+    all[1].lds_bytes = 81920; all[1].launch = launch_tsm_atb_batched_f32;
     all[1].occupancy = occupancy_tsm_atb_batched_f32;
     all[2].name = "tsm_atb_batched_bf16"; all[2].dtype = 2;
     all[2].classes = classes_tsm_atb_batched_bf16(&all[2].n_classes);
