@@ -42,8 +42,8 @@ def test_kernel_table_matches_generator():
     table = _lib.kernels()
     assert [k["name"] for k in table] == [c.name for c in cfgs]
     for k, c in zip(table, cfgs):
-        assert (k["kc"], k["mc"], k["tm"], k["tn"], k["rows"], k["pf"], k["nt"], k["lds_bytes"]) == \
-            (c.kc, c.mc, c.tm, c.tn, c.rows, c.pf, c.nt, c.lds_bytes)
+        assert (k["kc"], k["mc"], k["tm"], k["tn"], k["rows"], k["pf"], k["nt"], k["lds_bytes"],
+                k["engine"]) == (c.kc, c.mc, c.tm, c.tn, c.rows, c.pf, c.nt, c.lds_bytes, c.engine)
     # every (dtype, k class, m class) has a kernel
     have = {(k["in_type"], k["kc"], k["mc"]) for k in table}
     for dt in range(3):
@@ -86,10 +86,15 @@ def test_atb_argument_errors(kw, status):
 
 def test_atb_config_errors():
     L = _lib.lib()
-    bad = _lib.make_config(pf=3)
-    st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
-                      ct.byref(bad))
-    assert st == _lib.ERR_INVALID_ARGUMENT
+    for bad in (_lib.make_config(pf=3), _lib.make_config(pf=5), _lib.make_config(engine=4)):
+        st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
+                          ct.byref(bad))
+        assert st == _lib.ERR_INVALID_ARGUMENT
+    # a valid request no kernel satisfies: the direct engine serves FP64 and FP32 only
+    direct = _lib.make_config(engine=_lib.ENGINE_DIRECT)
+    st = L.tsm_atb_ex(_lib.BF16, _lib.F64, 10, 64, 64, FAKE, 64, FAKE, 64, FAKE, 64, 0, None, 0,
+                      None, ct.byref(direct))
+    assert st == _lib.ERR_NO_KERNEL
 
 
 def test_backward_argument_errors():

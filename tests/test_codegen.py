@@ -118,3 +118,60 @@ def test_tile_table_format_and_device_are_checked(tmp_path):
     for e in t.get("sym_entries", []):
         c = sym_defaults[(e["dtype"], e["kc"])][0]
         assert c.sym == 1 and (c.rows, c.pf) == (e["rows"], e["pf"]) and c in syms
+
+
+def test_direct_engine_rules_and_names():
+    ok = KernelConfig("f64", 64, 64, 0, 0, 4, 8, 1, 0, 1)
+    assert ok.problems() == [] and ok.engine == 3 and ok.mfma
+    assert ok.name == "tsm_atb_f64_64x64_direct_d8_nt"
+    assert ok.lds_bytes == 2 * 64 * 64 * 8
+    assert KernelConfig("bf16", 64, 64, 0, 0, 4, 8, 1, 0, 1).problems()   # BF16 needs a transpose
+    assert KernelConfig("f64", 8, 8, 0, 0, 4, 8, 1, 0, 1).problems()      # multiples of 16
+    assert KernelConfig("f32", 32, 32, 0, 0, 8, 8, 1, 0, 1).problems()    # steps are 4 rows
+    assert KernelConfig("f32", 32, 32, 0, 0, 4, 6, 1, 0, 1).problems()    # depth divides 16
+    assert KernelConfig("f64", 32, 32, 8, 8, 4, 8, 1, 0, 1).problems()    # no register tile
+    assert KernelConfig("f64", 32, 32, 0, 0, 4, 8, 1, 1, 1).name.startswith("tsm_ata_f64_32x32_direct")
+    assert [c.pf for c in codegen.direct_variants("f32", 16, 64)] == list(codegen.DIRECT_DEPTHS)
+    assert codegen.direct_variants("bf16", 32, 32) == [] and codegen.direct_variants("f64", 8, 16) == []
+
+
+def test_direct_defaults_and_fallbacks():
+    """Untuned FP64/FP32 classes of 1024+ outputs default to the direct engine
+    with the staged heuristic kernel as fallback; a recorded fallback wins."""
+    cfgs, defaults = codegen.library_kernels({"entries": []})
+    fbs = codegen.library_fallbacks({"entries": []}, defaults)
+    for dt in ("f64", "f32"):
+        for kc, mc in ((32, 32), (16, 64), (64, 64)):
+            c, wpc, il = defaults[(dt, kc, mc)]
+            assert c.direct and wpc == 1 and il == 0 and c in cfgs
+            fb, fil = fbs[(dt, kc, mc)]
+            assert not fb.direct and fb == codegen.default_config(dt, kc, mc) and fb in cfgs
+    for key in (("bf16", 64, 64), ("f64", 16, 16), ("f32", 8, 8)):
+        assert not defaults[key][0].direct and fbs[key][0] == defaults[key][0]
+    fb = {"tm": 0, "tn": 0, "rows": 64, "pf": 2, "nt": 1, "interleave": 1}
+    e = {"dtype": "f64", "kc": 64, "mc": 64, "tm": 0, "tn": 0, "rows": 4, "pf": 16, "nt": 1,
+         "direct": 1, "fallback": fb}
+    t = {"entries": [e]}
+    cfgs, defaults = codegen.library_kernels(t)
+    c = defaults[("f64", 64, 64)][0]
+    assert c.direct and c.pf == 16
+    f, fil = codegen.library_fallbacks(t, defaults)[("f64", 64, 64)]
+    assert (f.rows, f.pf, f.nt, f.direct, fil) == (64, 2, 1, 0, 1) and f in cfgs
+    # A^T A: the direct default gives a direct A^T A kernel and its staged fallback
+    syms, sym_defaults = codegen.sym_kernels(defaults, t)
+    s = sym_defaults[("f64", 64)][0]
+    assert s.direct and s.sym and codegen.sym_of(f) in syms
+    # the generated table names the fallback beside the default
+    table = codegen.emit_table(cfgs, defaults, t)
+    assert "tsm_atb_f64_64x64_direct_d16_nt (else tsm_atb_f64_64x64_mfma_r64_p2_nt)" in table
+    assert f'launch_{c.name}, 3}}' in table
+
+
+def test_batched_classes_are_staged_with_wide_tiles():
+    cfgs, defaults = codegen.library_kernels(codegen.load_tuning())
+    for dt in codegen.DTYPES:
+        for c in codegen.batched_classes(defaults, dt, codegen.load_tuning()):
+            assert not c.direct and c.problems() == []
+            assert c.lds_bytes <= codegen.BATCH_LDS
+            if dt != "bf16":
+                assert c.stage_bytes <= codegen.BATCH_STAGE
