@@ -66,7 +66,9 @@ def secondary() -> dict:
         inputs (uniform and sign*2^e entries);
       backward: FP32 n = 2^23, k = m = 16, (forward + backward) / forward;
       batched: 512 pairs in one launch vs one launch per pair (FP64, FP32);
-      siblings: C = A^T A against tsm_atb(A, A), and Y = A X."""
+      siblings: C = A^T A against tsm_atb(A, A), and Y = A X;
+      matrix_cores: FP64 and FP32 k = m = 64 (A+B = 4 GiB), GFLOP/s and the
+        fraction of the dense matrix peak (78.6 / 157.3 TFLOP/s)."""
     from tsm_benchmark_amd import suite
 
     prec = suite.precision(10, False)
@@ -80,6 +82,7 @@ def secondary() -> dict:
     bwd = suite.backward(10)
     bat = suite.batched(5)
     sib = suite.siblings(10)
+    mat = suite.matrix(10)
     return {
         "low_precision": low,
         "backward": {key: bwd[key] for key in ("fwd_us", "bwd_us", "fwd_plus_bwd_over_fwd",
@@ -88,6 +91,7 @@ def secondary() -> dict:
                          "per_pair_launches_GBps": bat[dt]["per_pair_GBps"]}
                     for dt in ("f64", "f32")},
         "siblings": sib,
+        "matrix_cores": mat,
     }
 
 

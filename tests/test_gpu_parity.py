@@ -363,8 +363,11 @@ def test_gradcheck_fp64():
 
 
 # ---- graph capture -------------------------------------------------------------------------------
-def test_hip_graph_replays_match_eager():
-    n, k, m = 1 << 20, 8, 8
+@pytest.mark.parametrize("k,m", [(8, 8), (32, 32), (64, 64)])
+def test_hip_graph_replays_match_eager(k, m):
+    """In-kernel final sum (8 x 8), the finisher kernel after the staged
+    matrix-core kernel (32 x 32) and after the direct engine (64 x 64)."""
+    n = 1 << 20
     A, B = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)
     L = _lib.lib()
     ws_bytes = L.tsm_atb_workspace_size(_lib.F64, n, k, m, None)

@@ -12,6 +12,9 @@
               and FP32: one persistent launch vs one launch per pair
   siblings    C = A^T A (FP64, A = 2 GiB, k = 8 and 32) against tsm_atb(A, A);
               Y = A X (FP64, k = m = 8, n = 2^25)
+  matrix      FP64 and FP32 k = m = 64, A+B = 4 GiB, both distributions: the
+              matrix-core bound classes, GB/s and GFLOP/s with the fraction
+              of the dtype's dense matrix peak
 
 Times are per call over back-to-back calls after 3 warmup calls (HIP events on
 the launch stream), inputs resident in HBM: the steady-state rate, as in
@@ -183,6 +186,26 @@ def siblings(reps):
     return out
 
 
+def matrix(reps):
+    out = {}
+    for dt, es, peak in (("f64", 8, FP64_PEAK), ("f32", 4, FP32_PEAK)):
+        k = m = 64
+        n = (1 << 32) // (es * (k + m))
+        idx, _ = _lib.plan({"f64": _lib.F64, "f32": _lib.F32}[dt], n, k, m)
+        row = {"n": n, "k": k, "m": m, "kernel": _lib.kernels()[idx]["name"]}
+        for dist in (0, 1):
+            A, B = rand(n, k, dt, 1, dist), rand(n, m, dt, 2, dist)
+            C = torch.empty((k, m), dtype=torch.float64, device="cuda")
+            t = timed(lambda: ops.atb(A, B, out=C), reps)
+            flops = 2 * n * k * m
+            row[["uniform", "pow2"][dist]] = {
+                "us": round(t * 1e6, 1), "GBps": round(es * n * (k + m) / t / 1e9, 1),
+                "GFLOPs": round(flops / t / 1e9, 1), "matrix_peak_frac": round(flops / t / peak, 4)}
+            del A, B
+        out[dt] = row
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
@@ -194,7 +217,7 @@ def main(argv=None) -> int:
     parts = {"plain": lambda: plain(args.reps), "sweep": lambda: sweep(args.reps, args.quick),
              "precision": lambda: precision(args.reps, args.quick),
              "backward": lambda: backward(args.reps), "batched": lambda: batched(args.reps),
-             "siblings": lambda: siblings(args.reps)}
+             "siblings": lambda: siblings(args.reps), "matrix": lambda: matrix(args.reps)}
     report = {}
     for name, fn in parts.items():
         if args.only and name not in args.only:
