@@ -145,3 +145,20 @@ def test_direct_ata(dt):
     Ah = host(W[:, 1:65])
     check(C, Ah, Ah, dt)
     assert np.array_equal(C, C.T)
+
+
+def test_direct_64bit_offsets():
+    """Byte offsets past 2^34 (FP32 64 x 64, n = 2^26 + 5): the whole equals
+    the sum of its two halves (split-n additivity) to the FP32 partials'
+    rounding."""
+    n = (1 << 26) + 5
+    A, B = rand(n, 64, "f32", 11), rand(n, 64, "f32", 12)
+    cfg = _lib.make_config(engine=_lib.ENGINE_DIRECT)
+    C = ops.atb(A, B, out_dtype=torch.float64, config=cfg)
+    h = n // 2 + 3
+    C1 = ops.atb(A[:h], B[:h], out_dtype=torch.float64, config=cfg)
+    C2 = ops.atb(A[h:], B[h:], out_dtype=torch.float64, config=cfg)
+    ref = C1 + C2
+    # both sides carry FP32-accumulated 64-row partials: relative 1e-6 of the scale
+    assert torch.allclose(C, ref, rtol=1e-6, atol=1e-6 * ref.abs().max().item())
+    del A, B
