@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
 // element of row l / 16 of a 4-row K-step; the 16 lanes of a row group supply
 // the 16 columns of a block.  With A row-major those 16 lanes can read the
 // whole row with one coalesced load each: lane j of the group loads
-// LB = 16 * KC * sizeof(S) / 256 bytes, the IB = KC / 16 consecutive columns
+// LB = KC * sizeof(S) / 16 bytes, the IB = KC / 16 consecutive columns
 // j * IB ... j * IB + IB - 1, and element b of its load feeds block b.  Block
 // b therefore holds the columns {j * IB + b : j = 0..15}; its output row r is
 // C row r * IB + b (likewise for B and the C columns).  Each wave streams its
