@@ -1055,8 +1055,13 @@ __device__ __forceinline__ float lane_elem<float>(const uint32_t* w, int e) {
 // others mirrored.  a.tiles_per_wg is q, the pipelined steps per wave (a
 // multiple of D): wave w takes steps [w q, w q + q); the F - W q steps left
 // over (F = n / 4 full steps, W waves) go one at a time to waves in turn, and
-// the last wave adds the n % 4 final rows.  FP32 accumulators are added into
-// FP64 every 16 steps (64 rows per wave).  Needs a.dense.
+// the last wave adds the n % 4 final rows.  FP32 accumulators are folded
+// into TwoSum (hi, lo) FP32 pairs every 32 steps (128 rows per wave); FP64
+// appears only in the cross-wave and cross-workgroup sums.  (Folding every
+// 16 steps cost 7-10% at 64 x 64: the fold's dependent chain stalls the
+// matrix instructions that feed the accumulators.  At 128 rows the
+// Frobenius error on 4 GiB inputs is 2.1e-7 uniform, 6.1e-8 sign * 2^e.)
+// Needs a.dense.
 template <class S, int KC, int MC, int D, int NT, int SYM = 0>
 __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
                                                double* partial_out) {
@@ -1066,7 +1071,7 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   constexpr int LBA = KC * (int)sizeof(S) / 16, LBB = MC * (int)sizeof(S) / 16;
   constexpr int RA = KC * (int)sizeof(S), RB = MC * (int)sizeof(S);  // bytes per row
   constexpr bool kF32 = sizeof(S) == 4;
-  constexpr int FOLD = 16;  // steps between FP32 -> FP64 folds
+  constexpr int FOLD = 32;  // steps between FP32 (hi, lo) folds (D divides it)
   static_assert(!kF32 || FOLD % D == 0, "D divides the fold interval");
   using AccV = typename MfmaTraits<S>::AccV;
   auto used = [](int i, int j) constexpr { return !SYM || i <= j; };
@@ -1084,15 +1089,13 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   const uint32_t la = (uint32_t)(g * RA + j * LBA), lb = (uint32_t)(g * RB + j * LBB);
 
   AccV acc[IB][JB];
-  double accd[kF32 ? IB : 1][kF32 ? JB : 1][4];
+  AccV hi[kF32 ? IB : 1][kF32 ? JB : 1], lo[kF32 ? IB : 1][kF32 ? JB : 1];
 #pragma unroll
   for (int i = 0; i < IB; ++i)
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) {
       acc[i][jj] = AccV{};
-      if constexpr (kF32)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) accd[i][jj][v] = 0.0;
+      if constexpr (kF32) hi[i][jj] = lo[i][jj] = AccV{};
     }
   uint32_t wa[D][LBA / 4], wb[SYM ? 1 : D][SYM ? 1 : LBB / 4];
   auto ld = [&](int d, int64_t step) __attribute__((always_inline)) {
@@ -1124,7 +1127,10 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             if (!used(i, jj)) continue;
-            accd[i][jj][v] += (double)acc[i][jj][v];
+            float s, e;
+            two_sum(hi[i][jj][v], acc[i][jj][v], s, e);
+            hi[i][jj][v] = s;
+            lo[i][jj][v] += e;
             acc[i][jj][v] = 0.0f;
           }
     }
@@ -1185,7 +1191,7 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   constexpr int KCMC = KC * MC;
   auto value = [&](int i, int jj, int v) __attribute__((always_inline)) -> double {
     if constexpr (kF32)
-      return accd[i][jj][v];
+      return (double)hi[i][jj][v] + (double)lo[i][jj][v];
     else
       return acc[i][jj][v];
   };
