@@ -110,7 +110,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=20)
+    # about 30 ms: the first passes over freshly allocated inputs run slower
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1 << 25)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--m", type=int, default=8)

@@ -16,15 +16,16 @@
               matrix-core bound classes, GB/s and GFLOP/s with the fraction
               of the dtype's dense matrix peak
 
-Times are per call over back-to-back calls after 3 warmup calls (HIP events on
-the launch stream), inputs resident in HBM: the steady-state rate, as in
-bench.py and the autotuner.
+Times are per call over back-to-back calls after at least 0.1 s of warmup
+calls (HIP events on the launch stream), inputs resident in HBM: the
+steady-state rate, as in bench.py and the autotuner.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import sys
+import time
 
 import numpy as np
 import torch
@@ -38,10 +39,17 @@ FP32_PEAK = 157.3e12
 DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
 
 
-def timed(fn, reps: int = 10) -> float:
-    for _ in range(3):
+def timed(fn, reps: int = 10, warm_s: float = 0.1) -> float:
+    """Seconds per call over `reps` back-to-back calls, after at least three
+    warmup calls and `warm_s` seconds of them: right after its inputs are
+    allocated a pass over a few GiB runs up to 25% slower for the first
+    several milliseconds."""
+    t0 = time.perf_counter()
+    calls = 0
+    while calls < 3 or time.perf_counter() - t0 < warm_s:
         fn()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        calls += 1
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
