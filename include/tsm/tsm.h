@@ -66,7 +66,8 @@ typedef struct {
   int tm, tn;          /* register tile: C entries owned per thread = tm x tn */
   int rows;            /* rows per LDS tile (the n-tile streamed per stage) */
   int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth);
-                          direct engine: 4-row steps in flight per wave, 4, 8 or 16 */
+                          direct engine: steps in flight per wave, 4, 8 or 16 (FP64/FP32)
+                          or 2 or 4 (BF16) */
   int wgs_per_cu;      /* persistent workgroups per compute unit */
   int64_t grid;        /* total workgroups; overrides wgs_per_cu when > 0 (the "stride" role) */
   int load_policy;     /* 0 tuned default, 1 cached loads, 2 non-temporal (streaming) loads */
@@ -74,7 +75,8 @@ typedef struct {
                           2 interleaved (tile t to workgroup t mod grid) */
   int engine;          /* 0 tuned default, 1 VALU register tiles (tm x tn), 2 matrix cores
                           (MFMA) fed through LDS tiles, 3 matrix cores fed straight from
-                          global memory (FP64/FP32; pf = 4-row steps in flight per wave;
+                          global memory (BF16 through a wave-private LDS tile; pf = steps
+                          in flight per wave, of 4 rows (FP64/FP32) or 32 rows (BF16);
                           inputs that are not dense run the class's staged kernel).
                           2 and 3 need width classes that are multiples of 16. */
 } tsm_kernel_config;

@@ -90,9 +90,9 @@ def test_atb_config_errors():
         st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
                           ct.byref(bad))
         assert st == _lib.ERR_INVALID_ARGUMENT
-    # a valid request no kernel satisfies: the direct engine serves FP64 and FP32 only
+    # a valid request no kernel satisfies: the direct engine needs widths of 16 or more
     direct = _lib.make_config(engine=_lib.ENGINE_DIRECT)
-    st = L.tsm_atb_ex(_lib.BF16, _lib.F64, 10, 64, 64, FAKE, 64, FAKE, 64, FAKE, 64, 0, None, 0,
+    st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0,
                       None, ct.byref(direct))
     assert st == _lib.ERR_NO_KERNEL
 

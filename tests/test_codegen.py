@@ -125,14 +125,18 @@ def test_direct_engine_rules_and_names():
     assert ok.problems() == [] and ok.engine == 3 and ok.mfma
     assert ok.name == "tsm_atb_f64_64x64_direct_d8_nt"
     assert ok.lds_bytes == 2 * 64 * 64 * 8
-    assert KernelConfig("bf16", 64, 64, 0, 0, 4, 8, 1, 0, 1).problems()   # BF16 needs a transpose
+    assert KernelConfig("bf16", 64, 64, 0, 0, 4, 8, 1, 0, 1).problems()   # BF16 steps are 32 rows
+    bf = KernelConfig("bf16", 64, 64, 0, 0, 32, 4, 1, 0, 1)
+    assert bf.problems() == [] and bf.lds_bytes == 4 * 2 * 32 * 128 * 2
+    assert KernelConfig("bf16", 64, 64, 0, 0, 32, 8, 1, 0, 1).problems()  # depth divides the fold
     assert KernelConfig("f64", 8, 8, 0, 0, 4, 8, 1, 0, 1).problems()      # multiples of 16
     assert KernelConfig("f32", 32, 32, 0, 0, 8, 8, 1, 0, 1).problems()    # steps are 4 rows
     assert KernelConfig("f32", 32, 32, 0, 0, 4, 6, 1, 0, 1).problems()    # depth divides 16
     assert KernelConfig("f64", 32, 32, 8, 8, 4, 8, 1, 0, 1).problems()    # no register tile
     assert KernelConfig("f64", 32, 32, 0, 0, 4, 8, 1, 1, 1).name.startswith("tsm_ata_f64_32x32_direct")
     assert [c.pf for c in codegen.direct_variants("f32", 16, 64)] == list(codegen.DIRECT_DEPTHS)
-    assert codegen.direct_variants("bf16", 32, 32) == [] and codegen.direct_variants("f64", 8, 16) == []
+    assert [c.pf for c in codegen.direct_variants("bf16", 32, 32)] == list(codegen.DIRECT_DEPTHS_BF16)
+    assert codegen.direct_variants("f64", 8, 16) == []
 
 
 def test_direct_defaults_and_fallbacks():
@@ -146,7 +150,7 @@ def test_direct_defaults_and_fallbacks():
             assert c.direct and wpc == 1 and il == 0 and c in cfgs
             fb, fil = fbs[(dt, kc, mc)]
             assert not fb.direct and fb == codegen.default_config(dt, kc, mc) and fb in cfgs
-    for key in (("bf16", 64, 64), ("f64", 16, 16), ("f32", 8, 8)):
+    for key in (("bf16", 64, 64), ("f64", 16, 16), ("f32", 8, 8)):  # BF16: tuned entries only
         assert not defaults[key][0].direct and fbs[key][0] == defaults[key][0]
     fb = {"tm": 0, "tn": 0, "rows": 64, "pf": 2, "nt": 1, "interleave": 1}
     e = {"dtype": "f64", "kc": 64, "mc": 64, "tm": 0, "tn": 0, "rows": 4, "pf": 16, "nt": 1,

@@ -1,10 +1,11 @@
 """GPU parity of the direct matrix-core engine (engine 3; run on an MI355X: -m gpu).
 
 The engine streams each wave's rows straight into matrix-instruction
-operands.  Every direct kernel of the library is checked against the oracle
-with the tolerances of test_gpu_parity.py:
-  FP64:  |C - C_exact| <= 4 sqrt(n) 2^-52 (|A|^T |B|)_ij
-  FP32:  the same form with 2^-23 and Frobenius relative error <= 1e-6.
+operands (FP64, FP32) or through a wave-private LDS tile (BF16).  Every
+direct kernel of the library is checked against the oracle with the
+tolerances of test_gpu_parity.py:
+  FP64:       |C - C_exact| <= 4 sqrt(n) 2^-52 (|A|^T |B|)_ij
+  FP32/BF16:  the same form with 2^-23 and Frobenius relative error <= 1e-6.
 Row counts cover the pipelined steps, the steps dealt one at a time after
 them, and the n % 4 final rows; requests for the engine on inputs it cannot
 read (strided rows, k or m below the class width) run the class's staged
@@ -21,14 +22,16 @@ torch = pytest.importorskip("torch")
 from tsm_benchmark_amd import _lib, ops  # noqa: E402
 
 SEED = oracle.SEED
-DT = {"f64": torch.float64, "f32": torch.float32}
-EPS = {"f64": 2.0**-52, "f32": 2.0**-23}
-DID = {"f64": _lib.F64, "f32": _lib.F32}
+DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
+EPS = {"f64": 2.0**-52, "f32": 2.0**-23, "bf16": 2.0**-23}
+DID = {"f64": _lib.F64, "f32": _lib.F32, "bf16": _lib.BF16}
 CLASSES = [("f64", 16, 16), ("f64", 32, 32), ("f64", 64, 64), ("f64", 16, 64), ("f64", 64, 32),
-           ("f32", 16, 16), ("f32", 32, 32), ("f32", 64, 64), ("f32", 32, 64)]
+           ("f32", 16, 16), ("f32", 32, 32), ("f32", 64, 64), ("f32", 32, 64),
+           ("bf16", 16, 16), ("bf16", 32, 32), ("bf16", 64, 64), ("bf16", 16, 64)]
 # 100003 = 25000 steps + 3 rows: with 256 workgroups and 8 steps in flight,
-# 24 pipelined steps per wave, 424 dealt one at a time, 3 final rows
-NS = [0, 1, 3, 4, 5, 127, 128, 1001, 100003]
+# 24 pipelined steps per wave, 424 dealt one at a time, 3 final rows (BF16:
+# 32-row steps, so 3125 steps + 3 rows)
+NS = [0, 1, 3, 4, 5, 31, 33, 127, 128, 1001, 100003]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -54,7 +57,7 @@ def check(C, Ah, Bh, dt):
     bound = 4.0 * np.sqrt(max(n, 1)) * EPS[dt] * absC
     err = np.abs(C - ref)
     assert np.all(err <= bound), f"max err/bound {np.max(err / np.maximum(bound, 1e-300)):.3g}"
-    if dt == "f32" and np.linalg.norm(ref) > 0:
+    if dt != "f64" and np.linalg.norm(ref) > 0:
         assert np.linalg.norm(C - ref) / np.linalg.norm(ref) <= 1e-6
 
 
@@ -82,7 +85,7 @@ def test_direct_against_oracle(cls):
             check(C, host(A), host(B), dt)
 
 
-@pytest.mark.parametrize("cls", [("f64", 64, 64), ("f32", 32, 32), ("f64", 16, 16)],
+@pytest.mark.parametrize("cls", [("f64", 64, 64), ("f32", 32, 32), ("f64", 16, 16), ("bf16", 64, 64)],
                          ids=lambda c: f"{c[0]}-{c[1]}x{c[2]}")
 @pytest.mark.parametrize("grid", [1, 3, 77])
 def test_direct_grids_and_pow2(cls, grid):

@@ -54,7 +54,8 @@ THREADS = 256          # work-group size (4 wave64)
 WAVE = 64              # gfx950 wavefront size
 LDS_BUDGET = 65536     # LDS per work-group of the defaults (two fit in 160 KiB)
 LDS_MAX = 131072       # LDS per work-group of any variant (one per compute unit)
-DIRECT_DEPTHS = (4, 8, 16)  # steps in flight per wave of the direct engine (divide 16)
+DIRECT_DEPTHS = (4, 8, 16)  # steps in flight per wave of the direct engine (FP64, FP32)
+DIRECT_DEPTHS_BF16 = (2, 4)  # BF16: even, dividing the 4-step (128-row) fold
 MAX_ROWS_PER_THREAD = 16  # the fully unrolled row loop of one tile (compile time)
 
 
@@ -125,7 +126,14 @@ class KernelConfig:
         return self.rows * (self.kc + (0 if self.sym else self.mc)) * ELEM_BYTES[self.dtype]
 
     @property
+    def direct_depths(self) -> tuple:
+        return DIRECT_DEPTHS_BF16 if self.dtype == "bf16" else DIRECT_DEPTHS
+
+    @property
     def lds_bytes(self) -> int:
+        if self.direct and self.dtype == "bf16":  # four waves' double-buffered 32-row steps
+            staging = 4 * 2 * 32 * (self.kc + (0 if self.sym else self.mc)) * 2
+            return max(staging, self.kc * self.mc * 8, THREADS * 16)
         if self.direct:  # two KC x MC FP64 halves of the cross-wave sum
             return max(2 * self.kc * self.mc * 8, THREADS * 16)
         return max(2 * self.stage_bytes, self.kc * self.mc * 8, THREADS * 8)
@@ -144,16 +152,16 @@ class KernelConfig:
         if self.direct:
             if self.direct != 1:
                 p.append("direct must be 0 or 1")
-            if self.dtype not in ("f64", "f32"):
-                p.append("the direct engine reads FP64 and FP32 rows (BF16 needs a transpose)")
+            if self.dtype not in DTYPES:
+                p.append("dtype")
             if self.kc % 16 or self.mc % 16:
                 p.append("MFMA needs width classes that are multiples of 16")
             if self.tm or self.tn:
                 p.append("the direct engine has no register tile")
-            if self.rows != 4:
-                p.append("direct-engine steps are 4 rows")
-            if self.pf not in DIRECT_DEPTHS or self.nt not in (0, 1):
-                p.append(f"steps in flight must be one of {DIRECT_DEPTHS}, nt 0 or 1")
+            if self.rows != self.kstep:
+                p.append(f"direct-engine steps are {self.kstep} rows")
+            if self.pf not in self.direct_depths or self.nt not in (0, 1):
+                p.append(f"steps in flight must be one of {self.direct_depths}, nt 0 or 1")
             if self.lds_bytes > LDS_MAX:
                 p.append("LDS budget")
             return p
@@ -276,8 +284,10 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
 
 def direct_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
     """Direct-engine kernels of a class: every depth with streaming loads (none
-    for BF16 or widths that are not multiples of 16)."""
-    out = [KernelConfig(dtype, kc, mc, 0, 0, 4, d, 1, 0, 1) for d in DIRECT_DEPTHS]
+    for widths that are not multiples of 16)."""
+    rows = 32 if dtype == "bf16" else 4
+    depths = DIRECT_DEPTHS_BF16 if dtype == "bf16" else DIRECT_DEPTHS
+    out = [KernelConfig(dtype, kc, mc, 0, 0, rows, d, 1, 0, 1) for d in depths]
     return [c for c in out if not c.problems()]
 
 
@@ -426,9 +436,9 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
                 # one direct kernel in every class the engine serves; without
                 # a tuned entry it is the default from 1024 outputs on (one
                 # workgroup per CU), the staged kernel its fallback
-                direct = [d for d in direct_variants(dt, kc, mc) if d.pf == 8]
+                direct = [d for d in direct_variants(dt, kc, mc) if d.pf == (4 if dt == "bf16" else 8)]
                 cfgs.update(direct)
-                if direct and kc * mc >= 1024:
+                if direct and kc * mc >= 1024 and dt != "bf16":
                     defaults[(dt, kc, mc)] = (direct[0], 1, 0)
         for kc, mc in TUNE_SHAPES[dt]:
             cfgs.update(tuning_variants(dt, kc, mc))
@@ -657,7 +667,7 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
                 p = os.path.join(out_dir, f"tsm_kernels_{dt}_{kc}x{mc}.hip")
                 _write_if_changed(p, emit_kernel_file(dt, kc, mc, cfgs + syms))
                 paths.append(p)
-    for dt in ("f64", "f32"):
+    for dt in DTYPES:
         p = os.path.join(out_dir, DIRECT_FILE.format(dt))
         _write_if_changed(p, emit_direct_file(dt, cfgs + syms))
         paths.append(p)

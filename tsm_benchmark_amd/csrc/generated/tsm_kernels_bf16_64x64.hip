@@ -203,36 +203,6 @@ hipError_t launch_tsm_atb_bf16_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs& a, unsig
   return hipGetLastError();
 }
 
-// tsm_ata_bf16_64x64_mfma_r128_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=128 pf=1 nt=1
-//   matrix cores: 4 K-steps of 32 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r128_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 64, 64, 0, 0, 128, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_bf16_64x64_mfma_r128_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r128_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_bf16_64x64_mfma_r128_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=128 pf=2 nt=1
-//   matrix cores: 4 K-steps of 32 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r128_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 64, 64, 0, 0, 128, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_bf16_64x64_mfma_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_bf16_64x64_mfma_r256_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=256 pf=1 nt=1
-//   matrix cores: 8 K-steps of 32 rows per tile, LDS 65536 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r256_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 64, 64, 0, 0, 256, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r256_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
 // tsm_ata_bf16_64x64_mfma_r256_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=256 pf=2 nt=1
 //   matrix cores: 8 K-steps of 32 rows per tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r256_p2_nt(tsm::AtbArgs a) {
@@ -240,25 +210,5 @@ extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r256_p
 }
 hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_bf16_64x64_mfma_r512_p1_nt: KC=64 MC=64 TM=0 TN=0 rows=512 pf=1 nt=1
-//   matrix cores: 16 K-steps of 32 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r512_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 64, 64, 0, 0, 512, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r512_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_bf16_64x64_mfma_r512_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=512 pf=2 nt=1
-//   matrix cores: 16 K-steps of 32 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_64x64_mfma_r512_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<tsm::bf16, 64, 64, 0, 0, 512, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_bf16_64x64_mfma_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_bf16_64x64_mfma_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

@@ -209,10 +209,11 @@ tsm_status set_grid(Plan* p, int64_t n, int64_t T, int wpc, int interleave, int6
   // wider classes leave it to the finisher kernel, spread over the chip.
   p->finish_in_kernel = p->kcmc <= 512;
   if (p->entry && p->entry->engine == 3) {
-    // Direct engine: 4-row steps; each of the 4 waves of a workgroup takes
-    // q = tiles_per_wg steps (a multiple of the depth D), at least D when n
-    // allows; the kernel deals the rest one step at a time.
-    const int64_t F = n / 4, D = p->entry->pf;
+    // Direct engine: steps of T rows (4 for FP64/FP32, 32 for BF16); each
+    // of the 4 waves of a workgroup takes q = tiles_per_wg steps (a multiple
+    // of the depth D), at least D when n allows; the kernel deals the rest
+    // one step at a time.
+    const int64_t F = n / T, D = p->entry->pf;
     const int64_t cap = F / (4 * D);
     if (want > cap) want = cap;
     if (want < 1) want = 1;

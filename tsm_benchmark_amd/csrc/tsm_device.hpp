@@ -549,8 +549,12 @@ struct MfmaAcc {
     const S* As = reinterpret_cast<const S*>(a_stage);
     const S* Bs = reinterpret_cast<const S*>(b_stage);
 #pragma unroll
-    for (int s = 0; s < NSTEP / 4; ++s) {
-      const int row0 = (wave + 4 * s) * KSTEP;
+    for (int s = 0; s < NSTEP / 4; ++s) step(As, Bs, (wave + 4 * s) * KSTEP, lane);
+  }
+  // One K-step: rows row0 .. row0 + KSTEP - 1 of LDS tiles As (pitch KC) and
+  // Bs (pitch MC).
+  __device__ __forceinline__ void step(const S* As, const S* Bs, int row0, int lane) {
+    {
       decltype(mfma_operand(As, KC, 0, 0, 0)) av[IB], bv[JB];
 #pragma unroll
       for (int i = 0; i < IB; ++i) av[i] = mfma_operand(As, KC, row0, i, lane);
@@ -1216,16 +1220,164 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   for (int o = threadIdx.x; o < KCMC; o += kThreads) publish(partial_out + o, red[o] + red[KCMC + o]);
 }
 
+// BF16 direct engine.  The 16x16x32 BF16 instruction takes eight rows of one
+// column per lane, which row-major loads cannot supply, so each wave copies
+// its own 32-row steps (the same contiguous bytes as in memory) into a
+// wave-private, double-buffered LDS tile and reads the operands from it with
+// the transposing reads of the staged consumer (MfmaAcc::step: paired
+// accumulators against the instruction's rounding bias, (hi, lo) folds every
+// 128 rows per wave).  Only the wave's own lanes touch its tile, so no
+// workgroup barrier stands between the loads and the matrix instructions.
+// Steps, work split and final sum as in direct_partial (steps of 32 rows).
+template <int KC, int MC, int SYM>
+constexpr int direct_bf16_smem() {
+  constexpr int staging = 4 * 2 * 32 * (KC + (SYM ? 0 : MC)) * 2;
+  constexpr int red = KC * MC * 8;
+  constexpr int m1 = staging > red ? staging : red;
+  return m1 > kThreads * 16 ? m1 : kThreads * 16;
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int KC, int MC, int D, int NT, int SYM = 0>
+__device__ __forceinline__ void direct_bf16_partial(const AtbArgs& a, unsigned char* smem,
+                                                    double* partial_out) {
+  constexpr int KS = 32;                        // rows per step
+  constexpr int RA = KC * 2, RB = MC * 2;       // bytes per row
+  constexpr int CA = KS * RA / 1024;            // 16-byte chunks per lane per step (A)
+  constexpr int CB = SYM ? 0 : KS * RB / 1024;  // (B)
+  static_assert(CA >= 1 && (SYM || CB >= 1), "widths of 16 or more");
+  constexpr int STAGE = KS * (RA + (SYM ? 0 : RB));
+  constexpr int FOLD = 4;                       // steps between (hi, lo) folds: 128 rows
+  static_assert(FOLD % D == 0 && D % 2 == 0, "D even and dividing the fold interval");
+  MfmaAcc<bf16, KC, MC, 4 * KS, SYM> acc;
+  acc.zero();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned char* wbuf = smem + wave * 2 * STAGE;
+  const int64_t F = a.n / KS;
+  const int64_t W = (int64_t)gridDim.x * 4;
+  const int64_t wid = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t q = a.tiles_per_wg;
+  const int64_t s0 = wid * q;
+  gu8* A8 = (gu8*)a.A;
+  gu8* B8 = (gu8*)a.B;
+  v4u ra[D][CA], rb[D][CB > 0 ? CB : 1];
+  auto ld = [&](int d, int64_t st) __attribute__((always_inline)) {
+    gv4u* pa = (gv4u*)(A8 + st * KS * RA);
+#pragma unroll
+    for (int c = 0; c < CA; ++c) ra[d][c] = stream_load<NT>(pa + lane + 64 * c);
+    if constexpr (!SYM) {
+      gv4u* pb = (gv4u*)(B8 + st * KS * RB);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) rb[d][c] = stream_load<NT>(pb + lane + 64 * c);
+    }
+  };
+  auto put = [&](int d, int buf) __attribute__((always_inline)) {
+    v4u* la = reinterpret_cast<v4u*>(wbuf + buf * STAGE);
+#pragma unroll
+    for (int c = 0; c < CA; ++c) la[lane + 64 * c] = ra[d][c];
+    if constexpr (!SYM) {
+      v4u* lb = reinterpret_cast<v4u*>(wbuf + buf * STAGE + KS * RA);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) lb[lane + 64 * c] = rb[d][c];
+    }
+  };
+  auto compute = [&](int buf) __attribute__((always_inline)) {
+    const bf16* As = reinterpret_cast<const bf16*>(wbuf + buf * STAGE);
+    const bf16* Bs = SYM ? As : reinterpret_cast<const bf16*>(wbuf + buf * STAGE + KS * RA);
+    acc.step(As, Bs, 0, lane);
+  };
+  if (q > 0) {
+    const int64_t last = s0 + q - 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ld(d, s0 + d);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // Slot d's step goes to LDS buffer d % 2 (D is even): the buffer written
+    // was last read two steps earlier, and one wave's LDS operations run in
+    // order.  The refill is unconditional and clamped, as in direct_partial.
+    int since_fold = 0;
+    for (int64_t s = 0; s < q; s += D) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        put(d, d & 1);
+        wave_sync();
+        const int64_t nx = s0 + s + d + D;
+        ld(d, nx < last ? nx : last);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(d & 1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      since_fold += D;
+      if (since_fold == FOLD) {
+        acc.fold();
+        since_fold = 0;
+      }
+    }
+  }
+  wave_sync();
+  for (int64_t st = W * q + wid; st < F; st += W) {  // leftover steps, one at a time
+    ld(0, st);
+    put(0, 0);
+    wave_sync();
+    compute(0);
+    acc.fold();
+    wave_sync();
+  }
+  if (wid == W - 1 && a.n > KS * F) {  // the last n % 32 rows, zero-padded
+#pragma unroll
+    for (int c = 0; c < CA; ++c) {
+      const int ci = lane + 64 * c;
+      const int64_t row = KS * F + ci / (RA / 16);
+      ra[0][c] = row < a.n ? stream_load<NT>((gv4u*)(A8 + KS * F * RA) + ci) : v4u{0, 0, 0, 0};
+    }
+    if constexpr (!SYM) {
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        const int ci = lane + 64 * c;
+        const int64_t row = KS * F + ci / (RB / 16);
+        rb[0][c] = row < a.n ? stream_load<NT>((gv4u*)(B8 + KS * F * RB) + ci) : v4u{0, 0, 0, 0};
+      }
+    }
+    put(0, 0);
+    wave_sync();
+    compute(0);
+  }
+  acc.fold();
+
+  // ---- the four waves' partials in wave order, then publication -----------------------
+  constexpr int KCMC = KC * MC;
+  double* red = reinterpret_cast<double*>(smem);
+  __syncthreads();  // every wave is done with its staging tile
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) acc.emit(red, w == 0, lane);
+    __syncthreads();
+  }
+  for (int o = threadIdx.x; o < KCMC; o += kThreads) publish(partial_out + o, red[o]);
+}
+
 // Direct-engine kernel body: partial, then the last workgroup's final sum
-// (classes up to 1024 outputs) as in atb_body.
+// (classes up to 512 outputs) as in atb_body.
 template <class S, int KC, int MC, int D, int NT, int SYM = 0>
 __device__ __forceinline__ void direct_body(const AtbArgs& a) {
   constexpr int KCMC = KC * MC;
-  constexpr int SMEM = 2 * KCMC * 8 > kThreads * 16 ? 2 * KCMC * 8 : kThreads * 16;
+  constexpr bool kBf16 = sizeof(S) == 2;
+  constexpr int SMEM_F = 2 * KCMC * 8 > kThreads * 16 ? 2 * KCMC * 8 : kThreads * 16;
+  constexpr int SMEM = kBf16 ? direct_bf16_smem<KC, MC, SYM>() : SMEM_F;
   __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
   __shared__ int s_last;
-  direct_partial<S, KC, MC, D, NT, SYM>(a, reinterpret_cast<double*>(smem),
-                                        a.partials + (int64_t)blockIdx.x * KCMC);
+  if constexpr (kBf16)
+    direct_bf16_partial<KC, MC, D, NT, SYM>(a, smem, a.partials + (int64_t)blockIdx.x * KCMC);
+  else
+    direct_partial<S, KC, MC, D, NT, SYM>(a, reinterpret_cast<double*>(smem),
+                                          a.partials + (int64_t)blockIdx.x * KCMC);
   if (!a.finish) return;
   if (!arrive_last(a.counter, gridDim.x, &s_last)) return;
   if constexpr (KCMC >= 2 && KCMC <= 512)
