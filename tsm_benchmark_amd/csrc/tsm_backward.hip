@@ -6,11 +6,12 @@
 // workgroup in LDS, zero-padded to the width class W >= max(k, m), and read
 // with wave-uniform addresses (LDS broadcast).  Each thread computes whole
 // rows.  Two kernels share that arithmetic:
-//   staged  (dense rows of 16..128 bytes): a workgroup moves a tile of 256
-//           rows between HBM and LDS with consecutive 16-byte loads and
-//           stores (each instruction of a wave covers 1 KiB of contiguous
-//           bytes); a thread reads its row from the padded LDS tile and
-//           writes its output row back there before the tile is stored;
+//   staged  (dense rows of 16..128 bytes): each wave moves its own tiles of
+//           64 rows between HBM and its part of LDS with consecutive 16-byte
+//           loads and stores (each instruction covers 1 KiB of contiguous
+//           bytes); a lane reads its row from the padded LDS tile and writes
+//           its output row back there before the tile is stored, and no
+//           workgroup barrier is needed;
 //   packed  (k == m == W, dense rows under 16 bytes): a thread moves 16
 //           consecutive bytes, 16 / (W s) whole rows, in one load and one
 //           store;
@@ -147,9 +148,14 @@ __device__ __forceinline__ void bwd_body(const BwdArgs& a) {
 }
 
 // ---- staged kernel ------------------------------------------------------------------
-// One phase: out (rows x out_w) = in (rows x in_w) times the dC coefficients,
-// for the tile of `rows` rows starting at row r0.  `regs` holds this thread's
-// 16-byte chunks of the input tile (chunk tid + i * 256), already loaded.
+// Each wave stages its own tiles of 64 rows in its own part of LDS, so only
+// wave-level ordering is needed: the wave writes the input chunks it loaded,
+// each lane reads its row, computes its output row and writes it back over
+// its own row (no other lane reads that row), and the wave then stores the
+// tile as consecutive 16-byte chunks.  One phase: out (rows x out_w) =
+// in (rows x in_w) times the dC coefficients for the wave tile of `rows` rows
+// starting at row r0; `regs` holds this lane's 16-byte chunks of the input
+// tile (chunk lane + i * 64), already loaded.
 template <class S, int W, bool TRANSPOSED, int CPT>
 __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, int in_w,
                                              int out_w, S* out, int64_t r0, unsigned char* tile,
@@ -158,33 +164,30 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
   constexpr int ES = (int)sizeof(S);
   constexpr int PER = 16 / ES;             // elements per chunk
   constexpr int PITCH = W * ES + 16;       // padded LDS row pitch (bytes)
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int in_cpr = in_w / PER, out_cpr = out_w / PER;
   const int in_chunks = rows * in_cpr;
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
-    const int c = tid + i * kThreads;
+    const int c = lane + i * 64;
     if (c < in_chunks) {
       const int r = c / in_cpr;
       *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
     }
   }
-  __syncthreads();
-  Acc x[W];
-  if (tid < rows) {
+  wave_sync();
+  if (lane < rows) {
+    Acc x[W];
 #pragma unroll
     for (int c = 0; c < W / PER; ++c) {
       if (c < in_cpr) {
-        const v4u u = *reinterpret_cast<const v4u*>(tile + tid * PITCH + c * 16);
+        const v4u u = *reinterpret_cast<const v4u*>(tile + lane * PITCH + c * 16);
         Unpack<S>::run(u, x + c * PER);
       } else {
 #pragma unroll
         for (int e = 0; e < PER; ++e) x[c * PER + e] = Acc(0);
       }
     }
-  }
-  __syncthreads();
-  if (tid < rows) {
 #pragma unroll
     for (int oc = 0; oc < W / PER; ++oc) {
       if (oc < out_cpr) {
@@ -204,23 +207,23 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
         for (int e = 0; e < PER; ++e) tmp[e] = narrow<S>(acc[e]);
         v4u u;
         __builtin_memcpy(&u, tmp, 16);
-        *reinterpret_cast<v4u*>(tile + tid * PITCH + oc * 16) = u;
+        *reinterpret_cast<v4u*>(tile + lane * PITCH + oc * 16) = u;
       }
     }
   }
-  __syncthreads();
+  wave_sync();
   v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
   const int out_chunks = rows * out_cpr;
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
-    const int c = tid + i * kThreads;
+    const int c = lane + i * 64;
     if (c < out_chunks) {
       const int r = c / out_cpr;
       const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
       __builtin_nontemporal_store(u, dst + c);
     }
   }
-  __syncthreads();  // the tile is rewritten by the next phase
+  wave_sync();  // the tile is rewritten by the next phase
 }
 
 template <class S, int W>
@@ -231,7 +234,7 @@ __device__ __forceinline__ void load_chunks(const S* in, int64_t r0, int rows, i
   const int chunks = rows * (w * (int)sizeof(S) / 16);
 #pragma unroll
   for (int i = 0; i < CPT; ++i) {
-    const int c = threadIdx.x + i * kThreads;
+    const int c = (threadIdx.x & 63) + i * 64;
     regs[i] = c < chunks ? __builtin_nontemporal_load(src + c) : v4u{0, 0, 0, 0};
   }
 }
@@ -240,11 +243,12 @@ template <class S, int W>
 __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   using Acc = typename Traits<S>::Acc;
   constexpr int ES = (int)sizeof(S);
-  constexpr int CPT = W * ES / 16;  // chunks per thread per input tile
+  constexpr int CPT = W * ES / 16;  // chunks per lane per input tile
   static_assert(W * ES >= 16 && W * ES <= 128, "staged rows are 16..128 bytes");
   constexpr int PITCH = W * ES + 16;
+  constexpr int WT = 64;            // rows per wave tile
   __shared__ Acc G[W * W];
-  __shared__ __attribute__((aligned(16))) unsigned char tile[kThreads * PITCH];
+  __shared__ __attribute__((aligned(16))) unsigned char tiles[kThreads / 64][WT * PITCH];
   for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
     const int i = e / W, j = e % W;
     Acc v = Acc(0);
@@ -260,27 +264,30 @@ __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   const S* B = static_cast<const S*>(a.B);
   S* dA = static_cast<S*>(a.dA);
   S* dB = static_cast<S*>(a.dB);
-  const int64_t ntiles = (a.n + kThreads - 1) / kThreads;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned char* tile = tiles[wave];
+  const int64_t ntiles = (a.n + WT - 1) / WT;
+  const int64_t stride = (int64_t)gridDim.x * (kThreads / 64);
   auto rows_of = [&](int64_t t) {
-    const int64_t r0 = t * kThreads;
-    return (int)((a.n - r0) < kThreads ? (a.n - r0) : kThreads);
+    const int64_t r0 = t * WT;
+    return (int)((a.n - r0) < WT ? (a.n - r0) : WT);
   };
-  // Both input tiles of the next tile are requested before this tile is
-  // processed (measured: +5% at FP32 k = m = 16).
+  // Both input tiles of the wave's next tile are requested before this tile
+  // is processed.
   v4u rb[CPT], ra[CPT];
-  int64_t tt = blockIdx.x;
+  int64_t tt = (int64_t)blockIdx.x * (kThreads / 64) + wave;
   if (tt < ntiles) {
-    if (dA) load_chunks<S, W>(B, tt * kThreads, rows_of(tt), a.m, rb);
-    if (dB) load_chunks<S, W>(A, tt * kThreads, rows_of(tt), a.k, ra);
+    if (dA) load_chunks<S, W>(B, tt * WT, rows_of(tt), a.m, rb);
+    if (dB) load_chunks<S, W>(A, tt * WT, rows_of(tt), a.k, ra);
   }
-  for (; tt < ntiles; tt += gridDim.x) {
-    const int64_t r0 = tt * kThreads;
+  for (; tt < ntiles; tt += stride) {
+    const int64_t r0 = tt * WT;
     const int rows = rows_of(tt);
-    const int64_t nt = tt + gridDim.x;
+    const int64_t nt = tt + stride;
     v4u nb[CPT], na[CPT];
     if (nt < ntiles) {
-      if (dA) load_chunks<S, W>(B, nt * kThreads, rows_of(nt), a.m, nb);
-      if (dB) load_chunks<S, W>(A, nt * kThreads, rows_of(nt), a.k, na);
+      if (dA) load_chunks<S, W>(B, nt * WT, rows_of(nt), a.m, nb);
+      if (dB) load_chunks<S, W>(A, nt * WT, rows_of(nt), a.k, na);
     }
     if (dA) staged_phase<S, W, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, G);
     if (dB) staged_phase<S, W, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, G);
