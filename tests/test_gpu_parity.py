@@ -339,6 +339,38 @@ def test_backward_matrix_core_against_oracle(dt, k, m, n):
     assert torch.equal(dB1, dB)
 
 
+@pytest.mark.parametrize("k,m,n", [(64, 64, 20011), (32, 32, 40001), (40, 24, 1000), (64, 8, 129),
+                                   (8, 64, 127), (48, 48, 5)])
+def test_backward_bf16_matrix_core_against_fp32_path(k, m, n):
+    """The BF16 matrix-core kernel (dC split exactly into three BF16 pieces,
+    FP32 accumulation in the matrix instruction) against the row-wise kernel
+    on a strided view (FP32 products added in column order).  Both FP32 sums
+    are within 64 * 2^-24 * S of the exact one (S = sum of |products|), so
+    the BF16 outputs differ by at most one BF16 rounding (2^-7 relative) plus
+    2^-17 S, and almost all are equal.  Dropping the middle piece would move
+    outputs by about 2^-8 S."""
+    A, B = rand(n, k, torch.bfloat16, 1), rand(n, m, torch.bfloat16, 2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
+    Gd = G.to("cuda", torch.float32)
+    dA, dB = ops.atb_backward(A, B, Gd)
+    Aw = torch.zeros((n, k + 3), dtype=A.dtype, device="cuda")
+    Bw = torch.zeros((n, m + 5), dtype=B.dtype, device="cuda")
+    Aw[:, :k] = A
+    Bw[:, :m] = B
+    dA2, dB2 = ops.atb_backward(Aw[:, :k], Bw[:, :m], Gd)
+    Gabs = np.abs(Gd.cpu().double().numpy())
+    S_A = np.abs(host(B)) @ Gabs.T   # dA = B dC^T
+    S_B = np.abs(host(A)) @ Gabs     # dB = A dC
+    for name, X, Y, S in (("dA", dA, dA2, S_A), ("dB", dB, dB2, S_B)):
+        x, y = host(X), host(Y)
+        err = np.abs(x - y)
+        tol = 2.0**-7 * np.maximum(np.abs(x), np.abs(y)) + 2.0**-17 * S
+        worst = np.unravel_index(np.argmax(err / np.maximum(tol, 1e-300)), err.shape)
+        assert np.all(err <= tol), (name, worst, x[worst], y[worst], S[worst])
+        same = np.mean(x == y)
+        assert same >= 0.995, (name, same)
+
+
 def test_autograd_matches_torch_matmul():
     n, k, m = 30011, 16, 16
     A0, B0 = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)

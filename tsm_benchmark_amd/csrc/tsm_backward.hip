@@ -395,12 +395,6 @@ struct MfmaBwd<float> {
   using AccV = v4f;
   static constexpr bool row_major_out = true;
 };
-template <>
-struct MfmaBwd<bf16> {
-  using In = float;
-  using AccV = v4f;
-  static constexpr bool row_major_out = true;
-};
 
 template <class S, int W, int R>
 constexpr int mfma_bwd_pitch() { return W * (int)sizeof(S) + 16; }
@@ -525,6 +519,184 @@ __device__ __forceinline__ void bwd_mfma_body(const BwdArgs& a) {
   }
 }
 
+// ---- BF16 matrix-core backward ----------------------------------------------------
+// dC (FP32) is split exactly into three BF16 pieces, g = h + m + l with
+// h = bf16(g), m = bf16(g - h), l = bf16(g - h - m) (each difference is exact
+// in FP32), so every product of a BF16 input and a piece is exact and the
+// sums run in the FP32 accumulation of v_mfma_f32_16x16x32_bf16.  A 16-row
+// block times 16 output columns takes, per K-step of 32 input columns, three
+// instructions (pieces l, m, h, in that order).  Operands of lane l:
+//   input  row l % 16 of the block, columns 32 ks + 8 (l / 16) + e, e = 0..7:
+//          16 consecutive bytes of the LDS tile row;
+//   piece  output column 16 nb + l % 16, input columns 32 ks + 8 (l / 16) + e:
+//          built once per workgroup in LDS, in lane order (1 KiB each).
+// Output of one block: lane l, register v holds row 4 (l / 16) + v, column
+// l % 16.  The instruction's FP32 rounding of its 32-product sums is biased
+// toward -infinity by about 1e-8 of the sum of |products|, far below the
+// BF16 rounding of the outputs.
+constexpr int kBf16Pieces = 3;
+
+template <int W>
+constexpr int bf16_bwd_operands() { return 2 * (W / 32) * (W / 16) * kBf16Pieces; }
+
+template <int W, int R, bool TRANSPOSED, int CPT>
+__device__ __forceinline__ void mfma_bf16_phase(const v4u (&regs)[CPT], int rows, int in_w,
+                                                int out_w, bf16* out, int64_t r0,
+                                                unsigned char* tile, const v8bf* ops) {
+  constexpr int PITCH = mfma_bwd_pitch<bf16, W, R>();
+  constexpr int PER = 8;           // BF16 elements per chunk
+  constexpr int CPR = W / PER;     // chunks per padded row
+  constexpr int NB = W / 16, KS = W / 32, RBW = R / 64;  // row blocks per wave
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int in_cpr = in_w / PER, out_cpr = out_w / PER;
+  // stage the input tile and zero the padding columns [in_w, W) of every row
+  const int in_chunks = rows * in_cpr;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = tid + i * kThreads;
+    if (c < in_chunks) {
+      const int r = c / in_cpr;
+      *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
+    }
+  }
+  const int pad_cpr = CPR - in_cpr;
+  for (int p = tid; p < R * pad_cpr; p += kThreads) {
+    const int r = p / pad_cpr;
+    *reinterpret_cast<v4u*>(tile + r * PITCH + (in_cpr + p - r * pad_cpr) * 16) =
+        v4u{0, 0, 0, 0};
+  }
+  __syncthreads();
+  // wave w owns row blocks w, w + 4, ...; it reads and then overwrites only those rows
+  v8bf av[RBW][KS];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      av[j][ks] = *reinterpret_cast<const v8bf*>(
+          tile + ((wave + 4 * j) * 16 + (lane & 15)) * PITCH + (32 * ks + 8 * (lane >> 4)) * 2);
+  v4f acc[RBW][NB];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[j][nb] = v4f{};
+  const v8bf* po = ops + (TRANSPOSED ? KS * NB * kBf16Pieces * 64 : 0);
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int p = kBf16Pieces - 1; p >= 0; --p) {
+        const v8bf bv = po[((ks * NB + nb) * kBf16Pieces + p) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < RBW; ++j) acc[j][nb] = mfma_step(av[j][ks], bv, acc[j][nb]);
+      }
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = (wave + 4 * j) * 16 + 4 * (lane >> 4) + v;
+        reinterpret_cast<bf16*>(tile + r * PITCH)[16 * nb + (lane & 15)] =
+            narrow<bf16>(acc[j][nb][v]);
+      }
+  __syncthreads();
+  v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
+  const int out_chunks = rows * out_cpr;
+  for (int c = tid; c < out_chunks; c += kThreads) {
+    const int r = c / out_cpr;
+    const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
+    __builtin_nontemporal_store(u, dst + c);
+  }
+  __syncthreads();  // the tile is rewritten by the next phase
+}
+
+template <int W, int R>
+__device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
+  constexpr int CPT = R * W * 2 / 16 / kThreads;  // input chunks per thread (dense rows)
+  static_assert(W % 32 == 0 && R % 64 == 0 && CPT >= 1, "32-wide K-steps, 4 waves");
+  constexpr int PITCH = mfma_bwd_pitch<bf16, W, R>();
+  constexpr int GP = W + 1;
+  constexpr int NB = W / 16, KS = W / 32;
+  static_assert(R * PITCH >= W * GP * 4, "dC is staged in the tile's memory");
+  __shared__ v8bf ops[bf16_bwd_operands<W>() * 64];
+  __shared__ __attribute__((aligned(16))) unsigned char tile[R * PITCH];
+  float* Gs = reinterpret_cast<float*>(tile);
+  for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
+    const int i = e / W, j = e % W;
+    float v = 0.0f;
+    if (i < a.k && j < a.m) {
+      const int64_t off = (int64_t)i * a.lddc + j;
+      v = a.dc_f64 ? (float)static_cast<const double*>(a.dC)[off]
+                   : static_cast<const float*>(a.dC)[off];
+    }
+    Gs[i * GP + j] = v;
+  }
+  __syncthreads();
+  // pieces: phase 0 (dA = B dC^T) takes dC[o][t], phase 1 (dB = A dC) dC[t][o]
+  for (int idx = threadIdx.x; idx < 2 * KS * NB * 64; idx += kThreads) {
+    const int ln = idx & 63, q = idx >> 6;  // q = (phase * KS + ks) * NB + nb
+    const int nb = q % NB, ks = (q / NB) % KS, ph = q / (NB * KS);
+    uint32_t w[kBf16Pieces][4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+      uint32_t h[kBf16Pieces][2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int t = 32 * ks + 8 * (ln >> 4) + e + x, o = 16 * nb + (ln & 15);
+        float g = ph ? Gs[t * GP + o] : Gs[o * GP + t];
+#pragma unroll
+        for (int p = 0; p < kBf16Pieces; ++p) {
+          const bf16 b = narrow<bf16>(g);
+          h[p][x] = b.bits;
+          g -= widen(b);
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < kBf16Pieces; ++p) w[p][e / 2] = h[p][0] | (h[p][1] << 16);
+    }
+#pragma unroll
+    for (int p = 0; p < kBf16Pieces; ++p)
+      ops[(q * kBf16Pieces + p) * 64 + ln] =
+          __builtin_bit_cast(v8bf, v4u{w[p][0], w[p][1], w[p][2], w[p][3]});
+  }
+  __syncthreads();
+  const bf16* A = static_cast<const bf16*>(a.A);
+  const bf16* B = static_cast<const bf16*>(a.B);
+  bf16* dA = static_cast<bf16*>(a.dA);
+  bf16* dB = static_cast<bf16*>(a.dB);
+  const int64_t ntiles = (a.n + R - 1) / R;
+  auto rows_of = [&](int64_t t) {
+    const int64_t r0 = t * R;
+    return (int)((a.n - r0) < R ? (a.n - r0) : R);
+  };
+  // both input tiles of the next tile are requested before this tile is
+  // processed (the tile stride stays an unsigned expression of gridDim.x)
+  v4u rb[CPT], ra[CPT];
+  int64_t tt = blockIdx.x;
+  if (tt < ntiles) {
+    if (dA) load_rows<bf16, W, R>(B, tt * R, rows_of(tt), a.m, rb);
+    if (dB) load_rows<bf16, W, R>(A, tt * R, rows_of(tt), a.k, ra);
+  }
+  for (; tt < ntiles; tt += gridDim.x) {
+    const int64_t r0 = tt * R;
+    const int rows = rows_of(tt);
+    const int64_t nt = tt + gridDim.x;
+    v4u nb_[CPT], na_[CPT];
+    if (nt < ntiles) {
+      if (dA) load_rows<bf16, W, R>(B, nt * R, rows_of(nt), a.m, nb_);
+      if (dB) load_rows<bf16, W, R>(A, nt * R, rows_of(nt), a.k, na_);
+    }
+    if (dA) mfma_bf16_phase<W, R, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, ops);
+    if (dB) mfma_bf16_phase<W, R, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, ops);
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      rb[i] = nb_[i];
+      ra[i] = na_[i];
+    }
+  }
+}
+
 }  // namespace tsm
 
 #define TSM_BWD_MFMA(NAME, S, W)                                                      \
@@ -536,8 +708,13 @@ TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w64, double, 64)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64)
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_bf16_w32, tsm::bf16, 32)
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_bf16_w64, tsm::bf16, 64)
+#define TSM_BWD_MFMA_BF16(NAME, W)                                                    \
+  extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
+    tsm::bwd_mfma_bf16_body<W, 128>(a);                                               \
+  }
+
+TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w32, 32)
+TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w64, 64)
 
 #define TSM_BWD_STAGED(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
