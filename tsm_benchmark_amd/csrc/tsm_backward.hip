@@ -373,100 +373,35 @@ __device__ __forceinline__ void bwd_packed_body(const BwdArgs& a) {
 // Wave w owns row blocks w, w + 4, ... of 16 rows; for each it runs W / 4
 // K-steps of 16x16x4 matrix instructions into W / 16 accumulator blocks:
 //   FP64: v_mfma_f64_16x16x4f64 (FP64 accumulation, as the VALU kernel);
-//   FP32 and BF16: v_mfma_f32_16x16x4f32 on FP32 operands (BF16 widened
-//   exactly), FP32 accumulation as the VALU kernel, one rounding to BF16.
-// Operand layout: lane l supplies in[16 rb + l % 16][4 s + l / 16] and
-// Gp[4 s + l / 16][16 nb + l % 16]; the result block holds row 4 (l / 16) + v
-// (FP32) or l / 16 + 4 v (FP64) in register v, column l % 16.  The tile is
-// staged in LDS with a padded pitch (row reads down a column are then
-// conflict-free), padding columns are zeroed, the outputs overwrite the
-// wave's own input rows, and the tile is stored as consecutive 16-byte chunks.
+//   FP32: v_mfma_f32_16x16x4f32 (FP32 accumulation, as the VALU kernel).
+// (BF16 has its own kernel on BF16 matrix instructions, below.)
+// K order: in K-step st, lane group g = l / 16 takes input column
+// t = g * W/4 + st, so a lane's input operands for all steps are W/4
+// consecutive elements of its tile row (16-byte LDS reads).  Lane l supplies
+// in[16 rb + l % 16][t] and Gp[t][16 nb + l % 16]; the Gp operands are
+// built once per workgroup in LDS in lane order (one 16-byte read per lane
+// covers 16 / sizeof(S) steps, consecutive lanes reading consecutive bytes).
+// The result block holds row 4 (l / 16) + v (FP32) or l / 16 + 4 v (FP64) in
+// register v, column l % 16.  The tile is staged in LDS with a padded pitch,
+// padding columns are zeroed, the outputs overwrite the wave's own input
+// rows, and the tile is stored as consecutive 16-byte chunks.  Once a phase
+// has staged its input, its registers take the next tile's rows of the same
+// input, so those loads are in flight through this phase and the next.
 template <class S>
 struct MfmaBwd;
 template <>
 struct MfmaBwd<double> {
-  using In = double;
   using AccV = v4d;
   static constexpr bool row_major_out = false;
 };
 template <>
 struct MfmaBwd<float> {
-  using In = float;
   using AccV = v4f;
   static constexpr bool row_major_out = true;
 };
 
 template <class S, int W, int R>
 constexpr int mfma_bwd_pitch() { return W * (int)sizeof(S) + 16; }
-
-template <class S, int W, int R, bool TRANSPOSED, int CPT>
-__device__ __forceinline__ void mfma_phase(const v4u (&regs)[CPT], int rows, int in_w, int out_w,
-                                           S* out, int64_t r0, unsigned char* tile,
-                                           const typename Traits<S>::Acc* G) {
-  using Acc = typename Traits<S>::Acc;
-  using M = MfmaBwd<S>;
-  constexpr int ES = (int)sizeof(S);
-  constexpr int PER = 16 / ES;
-  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
-  constexpr int GP = W + 1;        // dC pitch in LDS (Acc elements)
-  constexpr int CPR = W / PER;     // chunks per padded row
-  constexpr int NB = W / 16;       // output column blocks
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int in_cpr = in_w / PER, out_cpr = out_w / PER;
-  // stage the input tile (each thread writes the chunks it loaded) and zero
-  // the padding columns [in_w, W) of every row
-  const int in_chunks = rows * in_cpr;
-#pragma unroll
-  for (int i = 0; i < CPT; ++i) {
-    const int c = tid + i * kThreads;
-    if (c < in_chunks) {
-      const int r = c / in_cpr;
-      *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
-    }
-  }
-  const int pad_cpr = CPR - in_cpr;
-  for (int p = tid; p < R * pad_cpr; p += kThreads) {
-    const int r = p / pad_cpr;
-    *reinterpret_cast<v4u*>(tile + r * PITCH + (in_cpr + p - r * pad_cpr) * 16) =
-        v4u{0, 0, 0, 0};
-  }
-  __syncthreads();
-  for (int rb = wave; rb < R / 16; rb += 4) {
-    typename M::AccV acc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) acc[nb] = typename M::AccV{};
-    const unsigned char* arow = tile + (rb * 16 + (lane & 15)) * PITCH;
-#pragma unroll
-    for (int st = 0; st < W / 4; ++st) {
-      const int t = 4 * st + (lane >> 4);
-      const typename M::In a = (typename M::In)widen(reinterpret_cast<const S*>(arow)[t]);
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int o = 16 * nb + (lane & 15);
-        const Acc g = TRANSPOSED ? G[t * GP + o] : G[o * GP + t];
-        acc[nb] = mfma_step(a, (typename M::In)g, acc[nb]);
-      }
-    }
-    // outputs overwrite this wave's own rows of the tile
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = M::row_major_out ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v;
-        const int c = 16 * nb + (lane & 15);
-        reinterpret_cast<S*>(tile + (rb * 16 + r) * PITCH)[c] = narrow<S>((Acc)acc[nb][v]);
-      }
-  }
-  __syncthreads();
-  v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
-  const int out_chunks = rows * out_cpr;
-  for (int c = tid; c < out_chunks; c += kThreads) {
-    const int r = c / out_cpr;
-    const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
-    __builtin_nontemporal_store(u, dst + c);
-  }
-  __syncthreads();  // the tile is rewritten by the next phase
-}
 
 template <class S, int W, int R>
 __device__ __forceinline__ void load_rows(const S* in, int64_t r0, int rows, int w,
@@ -481,25 +416,142 @@ __device__ __forceinline__ void load_rows(const S* in, int64_t r0, int rows, int
   }
 }
 
+// Stage the input tile (each thread writes the chunks it loaded), zero the
+// padding columns [in_w, W) of every row, then refill `regs` (next tile).
+template <class S, int W, int R, int CPT, class Refill>
+__device__ __forceinline__ void stage_rows(v4u (&regs)[CPT], int rows, int in_w,
+                                           unsigned char* tile, Refill refill) {
+  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
+  constexpr int PER = 16 / (int)sizeof(S);
+  constexpr int CPR = W / PER;
+  const int tid = threadIdx.x;
+  const int in_cpr = in_w / PER;
+  const int in_chunks = rows * in_cpr;
+#pragma unroll
+  for (int i = 0; i < CPT; ++i) {
+    const int c = tid + i * kThreads;
+    if (c < in_chunks) {
+      const int r = c / in_cpr;
+      *reinterpret_cast<v4u*>(tile + r * PITCH + (c - r * in_cpr) * 16) = regs[i];
+    }
+  }
+  refill(regs);
+  const int pad_cpr = CPR - in_cpr;
+  for (int p = tid; p < R * pad_cpr; p += kThreads) {
+    const int r = p / pad_cpr;
+    *reinterpret_cast<v4u*>(tile + r * PITCH + (in_cpr + p - r * pad_cpr) * 16) =
+        v4u{0, 0, 0, 0};
+  }
+  __syncthreads();
+}
+
+// Store the tile's `rows` output rows of out_w elements as 16-byte chunks.
+template <class S, int W, int R>
+__device__ __forceinline__ void store_rows(const unsigned char* tile, int rows, int out_w,
+                                           S* out, int64_t r0) {
+  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
+  constexpr int PER = 16 / (int)sizeof(S);
+  const int out_cpr = out_w / PER;
+  v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
+  const int out_chunks = rows * out_cpr;
+  for (int c = threadIdx.x; c < out_chunks; c += kThreads) {
+    const int r = c / out_cpr;
+    const v4u u = *reinterpret_cast<const v4u*>(tile + r * PITCH + (c - r * out_cpr) * 16);
+    __builtin_nontemporal_store(u, dst + c);
+  }
+}
+
+template <class S, int W, int R, bool TRANSPOSED, int CPT, class Refill>
+__device__ __forceinline__ void mfma_phase(v4u (&regs)[CPT], int rows, int in_w, int out_w,
+                                           S* out, int64_t r0, unsigned char* tile,
+                                           const S* gop, Refill refill) {
+  using M = MfmaBwd<S>;
+  constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
+  constexpr int V = 16 / (int)sizeof(S);  // steps per 16-byte operand read
+  constexpr int NST = W / 4, NB = W / 16, RBW = R / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+  stage_rows<S, W, R>(regs, rows, in_w, tile, refill);
+  S av[RBW][NST];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int c = 0; c < NST / V; ++c) {
+      const v4u u = *reinterpret_cast<const v4u*>(
+          tile + ((wave + 4 * j) * 16 + (lane & 15)) * PITCH + (g * NST + c * V) * (int)sizeof(S));
+      S x[V];
+      __builtin_memcpy(x, &u, 16);
+#pragma unroll
+      for (int v = 0; v < V; ++v) av[j][c * V + v] = x[v];
+    }
+  typename M::AccV acc[RBW][NB];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[j][nb] = typename M::AccV{};
+  const S* po = gop + (TRANSPOSED ? W * W : 0);
+#pragma unroll
+  for (int c = 0; c < NST / V; ++c) {
+    S gv[NB][V];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const v4u u = *reinterpret_cast<const v4u*>(po + ((nb * (NST / V) + c) * 64 + lane) * V);
+      __builtin_memcpy(gv[nb], &u, 16);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int j = 0; j < RBW; ++j)
+          acc[j][nb] = mfma_step(av[j][c * V + v], gv[nb][v], acc[j][nb]);
+  }
+  // outputs overwrite this wave's own rows of the tile
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = M::row_major_out ? 4 * g + v : g + 4 * v;
+        reinterpret_cast<S*>(tile + ((wave + 4 * j) * 16 + r) * PITCH)[16 * nb + (lane & 15)] =
+            acc[j][nb][v];
+      }
+  __syncthreads();
+  store_rows<S, W, R>(tile, rows, out_w, out, r0);
+  __syncthreads();  // the tile is rewritten by the next phase
+}
+
 template <class S, int W, int R>
 __device__ __forceinline__ void bwd_mfma_body(const BwdArgs& a) {
-  using Acc = typename Traits<S>::Acc;
   constexpr int ES = (int)sizeof(S);
   constexpr int CPT = R * W * ES / 16 / kThreads;  // input chunks per thread (dense rows)
   static_assert(W % 16 == 0 && R % 64 == 0 && CPT >= 1, "16-wide blocks, 4 waves");
   constexpr int PITCH = mfma_bwd_pitch<S, W, R>();
   constexpr int GP = W + 1;
-  __shared__ Acc G[W * GP];
+  constexpr int V = 16 / ES, NST = W / 4, NB = W / 16;
+  static_assert(R * PITCH >= W * GP * ES, "dC is staged in the tile's memory");
+  __shared__ __attribute__((aligned(16))) S gop[2 * W * W];
   __shared__ __attribute__((aligned(16))) unsigned char tile[R * PITCH];
+  S* Gs = reinterpret_cast<S*>(tile);
   for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
     const int i = e / W, j = e % W;
-    Acc v = Acc(0);
+    S v = S(0);
     if (i < a.k && j < a.m) {
       const int64_t off = (int64_t)i * a.lddc + j;
-      v = a.dc_f64 ? (Acc) static_cast<const double*>(a.dC)[off]
-                   : (Acc) static_cast<const float*>(a.dC)[off];
+      v = a.dc_f64 ? (S) static_cast<const double*>(a.dC)[off]
+                   : (S) static_cast<const float*>(a.dC)[off];
     }
-    G[i * GP + j] = v;
+    Gs[i * GP + j] = v;
+  }
+  __syncthreads();
+  // operand element (phase, nb, c, lane, v): step st = c V + v, input column
+  // t = (lane / 16) NST + st, output column o = 16 nb + lane % 16; phase 0
+  // (dA = B dC^T) takes dC[o][t], phase 1 (dB = A dC) dC[t][o]
+  for (int idx = threadIdx.x; idx < 2 * W * W; idx += kThreads) {
+    const int v = idx % V, ln = (idx / V) % 64, q = idx / (V * 64);
+    const int c = q % (NST / V), nb = (q / (NST / V)) % NB, ph = q / ((NST / V) * NB);
+    const int t = (ln >> 4) * NST + c * V + v, o = 16 * nb + (ln & 15);
+    gop[idx] = ph ? Gs[t * GP + o] : Gs[o * GP + t];
   }
   __syncthreads();
   const S* A = static_cast<const S*>(a.A);
@@ -507,15 +559,33 @@ __device__ __forceinline__ void bwd_mfma_body(const BwdArgs& a) {
   S* dA = static_cast<S*>(a.dA);
   S* dB = static_cast<S*>(a.dB);
   const int64_t ntiles = (a.n + R - 1) / R;
-  for (int64_t tt = blockIdx.x; tt < ntiles; tt += gridDim.x) {
+  auto rows_of = [&](int64_t t) {
+    const int64_t r0 = t * R;
+    return (int)((a.n - r0) < R ? (a.n - r0) : R);
+  };
+  v4u rb[CPT], ra[CPT];
+  int64_t tt = blockIdx.x;
+  if (tt < ntiles) {
+    if (dA) load_rows<S, W, R>(B, tt * R, rows_of(tt), a.m, rb);
+    if (dB) load_rows<S, W, R>(A, tt * R, rows_of(tt), a.k, ra);
+  }
+  // the tile stride stays an unsigned expression of gridDim.x
+  for (; tt < ntiles; tt += gridDim.x) {
     const int64_t r0 = tt * R;
-    const int rows = (int)((a.n - r0) < R ? (a.n - r0) : R);
-    // chunk i of this thread is input chunk tid + i * 256 of the dense rows
-    v4u rb[CPT], ra[CPT];
-    if (dA) load_rows<S, W, R>(B, r0, rows, a.m, rb);
-    if (dB) load_rows<S, W, R>(A, r0, rows, a.k, ra);
-    if (dA) mfma_phase<S, W, R, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, G);
-    if (dB) mfma_phase<S, W, R, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, G);
+    const int rows = rows_of(tt);
+    const int64_t nt = tt + gridDim.x;
+    const bool more = nt < ntiles;
+    const int nrows = more ? rows_of(nt) : 0;
+    if (dA)
+      mfma_phase<S, W, R, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, gop,
+                                      [&](v4u (&r)[CPT]) {
+                                        if (more) load_rows<S, W, R>(B, nt * R, nrows, a.m, r);
+                                      });
+    if (dB)
+      mfma_phase<S, W, R, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, gop,
+                                     [&](v4u (&r)[CPT]) {
+                                       if (more) load_rows<S, W, R>(A, nt * R, nrows, a.k, r);
+                                     });
   }
 }
 
@@ -699,15 +769,17 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
 
 }  // namespace tsm
 
-#define TSM_BWD_MFMA(NAME, S, W)                                                      \
+// Rows per tile: 64 for FP64 W = 32 (measured 5.61 TB/s against 5.36 with
+// 128), 128 otherwise (FP32 W = 32: 5.59 against 5.33 with 64).
+#define TSM_BWD_MFMA(NAME, S, W, R)                                                   \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
-    tsm::bwd_mfma_body<S, W, 128>(a);                                                 \
+    tsm::bwd_mfma_body<S, W, R>(a);                                                   \
   }
 
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32)
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w64, double, 64)
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32)
-TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32, 64)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w64, double, 64, 128)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32, 128)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64, 128)
 #define TSM_BWD_MFMA_BF16(NAME, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_mfma_bf16_body<W, 128>(a);                                               \
