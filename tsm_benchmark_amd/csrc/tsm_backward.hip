@@ -770,7 +770,8 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
 }  // namespace tsm
 
 // Rows per tile: 64 for FP64 W = 32 (measured 5.61 TB/s against 5.36 with
-// 128), 128 otherwise (FP32 W = 32: 5.59 against 5.33 with 64).
+// 128), 128 otherwise (FP32 W = 32: 5.59 against 5.33 with 64 and 5.35 with
+// 256; FP32 W = 64: 5.23 against 4.57 with 256).
 #define TSM_BWD_MFMA(NAME, S, W, R)                                                   \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_mfma_body<S, W, R>(a);                                                   \
@@ -780,13 +781,15 @@ TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32, 64)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w64, double, 64, 128)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32, 128)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64, 128)
-#define TSM_BWD_MFMA_BF16(NAME, W)                                                    \
+// Rows per tile: 256 at W = 32 (5.70 TB/s against 5.26 with 128), 128 at
+// W = 64 (5.32 against 3.83 with 256).
+#define TSM_BWD_MFMA_BF16(NAME, W, R)                                                 \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
-    tsm::bwd_mfma_bf16_body<W, 128>(a);                                               \
+    tsm::bwd_mfma_bf16_body<W, R>(a);                                                 \
   }
 
-TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w32, 32)
-TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w64, 64)
+TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w32, 32, 256)
+TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w64, 64, 128)
 
 #define TSM_BWD_STAGED(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
