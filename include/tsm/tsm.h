@@ -140,7 +140,12 @@ size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k,
  * Backward of C = A^T B for an upstream gradient dC (k x m, row-major, lddc):
  *   dA = B dC^T (n x k), dB = A dC (n x m).
  * dA or dB may be NULL to skip it; both use one streaming pass over the rows.
- * dC is in out_type (TSM_F64 or TSM_F32); A, B, dA, dB are in in_type. */
+ * dC is in out_type (TSM_F64 or TSM_F32); A, B, dA, dB are in in_type.
+ * Each output is a sum of products accumulated in FP64 for FP64 inputs and in
+ * FP32 otherwise, rounded once to in_type.  Dense 32- and 64-wide classes run
+ * on matrix instructions (another summation order than the narrower
+ * classes); for BF16 they split dC exactly into three BF16 pieces, so every
+ * product stays exact. */
 tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, int k, int m,
                             const void* A, int64_t lda, const void* B, int64_t ldb,
                             const void* dC, int64_t lddc,
