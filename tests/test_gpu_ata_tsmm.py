@@ -148,3 +148,26 @@ def test_tsmm_against_oracle_and_backward(dt, k, m):
         assert torch.equal(Yw, Y)
     else:
         np.testing.assert_allclose(host(Yw), ref, **tol)
+
+
+@pytest.mark.parametrize("w", [1, 2, 4, 8, 16])
+def test_tsmm_bf16_rounding_is_nearest_even(w):
+    """With a diagonal X every output is one FP32 product a * x rounded once
+    to BF16 (adding the FP32 zero products is exact), so the outputs of the
+    packed (w <= 4) and staged kernels must equal torch's CPU conversion
+    (round to nearest even) of the FP32 product, bit for bit; a strided view
+    (row-wise kernel) must give the same bits."""
+    n = 50021
+    A = rand(n, w, torch.bfloat16, 11)
+    rng = np.random.default_rng(SEED + w)
+    d = (rng.uniform(0.5, 2.0, w) * np.where(rng.random(w) < 0.5, -1.0, 1.0)).astype(np.float32)
+    X = torch.diag(torch.from_numpy(d))
+    Y = ops.tsmm(A, X.cuda())
+    prod = A.cpu().float() * torch.from_numpy(d)[None, :]  # FP32 products, rounded once
+    ref = prod.to(torch.bfloat16)
+    assert torch.equal(Y.cpu().view(torch.int16), ref.view(torch.int16))
+    W = torch.zeros((n, w + 3), dtype=torch.bfloat16, device="cuda")
+    W[:, :w] = A
+    Yw = ops.tsmm(W[:, :w], X.cuda())
+    assert torch.equal(Yw.cpu().view(torch.int16), ref.view(torch.int16))
+

@@ -46,7 +46,9 @@ template <>
 __device__ __forceinline__ float narrow<float>(float x) { return x; }
 template <>
 __device__ __forceinline__ bf16 narrow<bf16>(float x) {
-  // round to nearest even (finite inputs)
+  // round to nearest even (finite inputs).  v_cvt_pk_bf16_f32 does the same
+  // in one instruction per pair but measured 7% slower at BF16 W = 8 (staged)
+  // and 1-2% faster at W = 16..64.
   uint32_t u = __float_as_uint(x);
   u += 0x7fffu + ((u >> 16) & 1u);
   return bf16{(uint16_t)(u >> 16)};
