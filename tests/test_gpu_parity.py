@@ -317,9 +317,13 @@ def test_backward_staged_equals_rowwise(dt, k, m, n):
 
 @pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
 @pytest.mark.parametrize("k,m,n", [(32, 32, 40001), (64, 64, 20011), (40, 24, 1000), (64, 8, 129),
-                                   (8, 64, 127), (48, 48, 5), (32, 64, 128)])
+                                   (8, 64, 127), (48, 48, 5), (32, 64, 128), (32, 32, 63),
+                                   (24, 32, 257), (32, 16, 5), (32, 32, 513)])
 def test_backward_matrix_core_against_oracle(dt, k, m, n):
-    """The 32- and 64-wide classes (dense rows) take the matrix-core kernel."""
+    """The 32- and 64-wide classes (dense rows) take the matrix-core kernel
+    (tiles of 64 rows for FP64 W = 32, 256 for BF16 W = 32, 128 otherwise:
+    the row counts cover one partial tile, a tile plus one row and several
+    tiles plus a partial one)."""
     if (k * DT[dt].itemsize) % 16 or (m * DT[dt].itemsize) % 16:
         pytest.skip("rows that are not whole 16-byte chunks take the row-wise kernel")
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
@@ -340,7 +344,8 @@ def test_backward_matrix_core_against_oracle(dt, k, m, n):
 
 
 @pytest.mark.parametrize("k,m,n", [(64, 64, 20011), (32, 32, 40001), (40, 24, 1000), (64, 8, 129),
-                                   (8, 64, 127), (48, 48, 5)])
+                                   (8, 64, 127), (48, 48, 5), (32, 32, 255), (24, 32, 257),
+                                   (32, 16, 5)])
 def test_backward_bf16_matrix_core_against_fp32_path(k, m, n):
     """The BF16 matrix-core kernel (dC split exactly into three BF16 pieces,
     FP32 accumulation in the matrix instruction) against the row-wise kernel
