@@ -5,7 +5,7 @@
 // B and writes a row of dA and dB.  dC (at most 64 x 64) is staged once per
 // workgroup in LDS, zero-padded to the width class W >= max(k, m), and read
 // with wave-uniform addresses (LDS broadcast).  Each thread computes whole
-// rows.  Two kernels share that arithmetic:
+// rows.  Three kernels share that arithmetic:
 //   staged  (dense rows of 16..128 bytes): each wave moves its own tiles of
 //           64 rows between HBM and its part of LDS with consecutive 16-byte
 //           loads and stores (each instruction covers 1 KiB of contiguous
@@ -18,8 +18,10 @@
 //   rowwise (any layout): a thread loads and stores its own rows directly,
 //           16 bytes at a time when dense.
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
-// a sum of at most 64 products, added in the same order by both kernels, so
-// both produce the same bits).
+// a sum of at most 64 products, added in the same order by these three
+// kernels, so they produce the same bits).  Dense rows of the 32- and 64-wide
+// classes take the matrix-core kernels further down (another summation
+// order; for BF16, dC split exactly into three BF16 pieces).
 #include "tsm/tsm.h"
 #include "tsm_internal.hpp"
 
