@@ -22,6 +22,8 @@
 // kernels, so they produce the same bits).  Dense rows of the 32- and 64-wide
 // classes take the matrix-core kernels further down (another summation
 // order; for BF16, dC split exactly into three BF16 pieces).
+#include <atomic>
+
 #include "tsm/tsm.h"
 #include "tsm_internal.hpp"
 
@@ -1101,14 +1103,17 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   if (fast) {
     kern = fast;
     blocks = (n + tile_rows - 1) / tile_rows;
-    static int resident[3][7];  // workgroups per CU, filled on first use
-    int& per_cu = resident[(int)in_type][wi];
+    // workgroups per CU, filled on first use (concurrent first calls store
+    // the same value)
+    static std::atomic<int> resident[3][7];
+    int per_cu = resident[(int)in_type][wi].load(std::memory_order_relaxed);
     if (per_cu == 0) {
       int nb = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kern),
                                                        256, 0) != hipSuccess || nb < 1)
         nb = 1;
       per_cu = nb;
+      resident[(int)in_type][wi].store(per_cu, std::memory_order_relaxed);
     }
     cap = (int64_t)cus * per_cu;
   }

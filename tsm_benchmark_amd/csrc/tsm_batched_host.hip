@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -147,9 +148,13 @@ tsm_status grid_for(const tsm::BatchedKernel* bk, int64_t n_items, unsigned* gri
   int cus = 0;
   tsm_status st = tsm_internal_device_cus(&cus);
   if (st != TSM_SUCCESS) return st;
-  static int resident[3];  // per dtype, filled on first use
-  int& per_cu = resident[bk->dtype];
-  if (per_cu == 0) per_cu = bk->occupancy();
+  // per dtype, filled on first use (concurrent first calls store the same value)
+  static std::atomic<int> resident[3];
+  int per_cu = resident[bk->dtype].load(std::memory_order_relaxed);
+  if (per_cu == 0) {
+    per_cu = bk->occupancy();
+    resident[bk->dtype].store(per_cu, std::memory_order_relaxed);
+  }
   int64_t g = std::min<int64_t>(n_items, (int64_t)cus * per_cu);
   *grid = (unsigned)std::max<int64_t>(g, 1);
   return TSM_SUCCESS;
