@@ -291,7 +291,11 @@ def test_backward_against_oracle(dt, k, m):
                                       ("f64", 2, 16, 5), ("f64", 16, 16, 4097), ("bf16", 8, 8, 1000),
                                       ("bf16", 64, 16, 2049), ("bf16", 4, 4, 100003), ("bf16", 4, 4, 3),
                                       ("bf16", 2, 2, 1001), ("bf16", 1, 1, 37), ("f32", 2, 2, 70001),
-                                      ("f32", 1, 1, 5), ("f64", 1, 1, 3001)])
+                                      ("f32", 1, 1, 5), ("f64", 1, 1, 3001),
+                                      # 32-byte rows: two rows per lane in the staged kernel
+                                      ("f64", 4, 4, 70001), ("f64", 4, 2, 129), ("f32", 8, 8, 100003),
+                                      ("f32", 8, 4, 127), ("bf16", 16, 16, 40001), ("bf16", 16, 8, 257),
+                                      ("bf16", 8, 16, 64)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
     """Dense rows take the LDS-staged kernel (widths up to 16), the packed
     kernel (square classes with rows under 16 bytes) or the matrix-core

@@ -162,8 +162,8 @@ __device__ __forceinline__ void bwd_body(const BwdArgs& a) {
 // in (rows x in_w) times the dC coefficients for the wave tile of `rows` rows
 // starting at row r0; `regs` holds this lane's 16-byte chunks of the input
 // tile (chunk lane + i * 64), already loaded.
-template <class S, int W, bool TRANSPOSED, int CPT>
-__device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, int in_w,
+template <class S, int W, bool TRANSPOSED, int RPL, int NCH>
+__device__ __forceinline__ void staged_phase(const v4u (&regs)[NCH], int rows, int in_w,
                                              int out_w, S* out, int64_t r0, unsigned char* tile,
                                              const typename Traits<S>::Acc* G) {
   using Acc = typename Traits<S>::Acc;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
   const int in_cpr = in_w / PER, out_cpr = out_w / PER;
   const int in_chunks = rows * in_cpr;
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
+  for (int i = 0; i < NCH; ++i) {
     const int c = lane + i * 64;
     if (c < in_chunks) {
       const int r = c / in_cpr;
@@ -182,38 +182,46 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
     }
   }
   wave_sync();
-  if (lane < rows) {
-    Acc x[W];
+  // lane l owns rows l, l + 64, ... of the tile; rows past `rows` are
+  // computed from stale tile contents and never stored
+  Acc x[RPL][W];
+#pragma unroll
+  for (int j = 0; j < RPL; ++j)
 #pragma unroll
     for (int c = 0; c < W / PER; ++c) {
       if (c < in_cpr) {
-        const v4u u = *reinterpret_cast<const v4u*>(tile + lane * PITCH + c * 16);
-        Unpack<S>::run(u, x + c * PER);
+        const v4u u = *reinterpret_cast<const v4u*>(tile + (lane + 64 * j) * PITCH + c * 16);
+        Unpack<S>::run(u, x[j] + c * PER);
       } else {
 #pragma unroll
-        for (int e = 0; e < PER; ++e) x[c * PER + e] = Acc(0);
+        for (int e = 0; e < PER; ++e) x[j][c * PER + e] = Acc(0);
       }
     }
 #pragma unroll
-    for (int oc = 0; oc < W / PER; ++oc) {
-      if (oc < out_cpr) {
-        Acc acc[PER];
+  for (int oc = 0; oc < W / PER; ++oc) {
+    if (oc < out_cpr) {
+      Acc acc[RPL][PER];
 #pragma unroll
-        for (int e = 0; e < PER; ++e) acc[e] = Acc(0);
+      for (int j = 0; j < RPL; ++j)
 #pragma unroll
-        for (int t = 0; t < W; ++t)
+        for (int e = 0; e < PER; ++e) acc[j][e] = Acc(0);
 #pragma unroll
-          for (int e = 0; e < PER; ++e) {
-            const int o = oc * PER + e;
-            const Acc g = TRANSPOSED ? G[t * W + o] : G[o * W + t];
-            acc[e] = fmadd(x[t], g, acc[e]);
-          }
+      for (int t = 0; t < W; ++t)
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+          const int o = oc * PER + e;
+          const Acc g = TRANSPOSED ? G[t * W + o] : G[o * W + t];  // one read, RPL rows
+#pragma unroll
+          for (int j = 0; j < RPL; ++j) acc[j][e] = fmadd(x[j][t], g, acc[j][e]);
+        }
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
         S tmp[PER];
 #pragma unroll
-        for (int e = 0; e < PER; ++e) tmp[e] = narrow<S>(acc[e]);
+        for (int e = 0; e < PER; ++e) tmp[e] = narrow<S>(acc[j][e]);
         v4u u;
         __builtin_memcpy(&u, tmp, 16);
-        *reinterpret_cast<v4u*>(tile + lane * PITCH + oc * 16) = u;
+        *reinterpret_cast<v4u*>(tile + (lane + 64 * j) * PITCH + oc * 16) = u;
       }
     }
   }
@@ -221,7 +229,7 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
   v4u* dst = reinterpret_cast<v4u*>(out + r0 * out_w);
   const int out_chunks = rows * out_cpr;
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
+  for (int i = 0; i < NCH; ++i) {
     const int c = lane + i * 64;
     if (c < out_chunks) {
       const int r = c / out_cpr;
@@ -232,27 +240,36 @@ __device__ __forceinline__ void staged_phase(const v4u (&regs)[CPT], int rows, i
   wave_sync();  // the tile is rewritten by the next phase
 }
 
-template <class S, int W>
+template <class S, int NCH>
 __device__ __forceinline__ void load_chunks(const S* in, int64_t r0, int rows, int w,
-                                            v4u (&regs)[W * (int)sizeof(S) / 16]) {
-  constexpr int CPT = W * (int)sizeof(S) / 16;
+                                            v4u (&regs)[NCH]) {
   const v4u* src = reinterpret_cast<const v4u*>(in + r0 * w);
   const int chunks = rows * (w * (int)sizeof(S) / 16);
 #pragma unroll
-  for (int i = 0; i < CPT; ++i) {
+  for (int i = 0; i < NCH; ++i) {
     const int c = (threadIdx.x & 63) + i * 64;
     regs[i] = c < chunks ? __builtin_nontemporal_load(src + c) : v4u{0, 0, 0, 0};
   }
 }
 
+// Rows per lane of the staged kernel.  A lane that owns two rows reads each
+// dC coefficient from LDS once for both; the reads are wave-uniform, and each
+// still moves 64 copies through the LDS return path, which 32-byte rows with
+// one row per lane nearly saturate (BF16 W = 16: 256 coefficients per 32-byte
+// row).  Measured with two rows per lane (TB/s, dA and dB): 32-byte rows
+// +3..6% (FP64 W = 4 5.38 -> 5.68, FP32 W = 8 5.36 -> 5.66, BF16 W = 16
+// 4.91 -> 5.05); other widths -1..15% (more VGPRs, fewer waves).
+constexpr int staged_rows_per_lane(int row_bytes) { return row_bytes == 32 ? 2 : 1; }
+
 template <class S, int W>
 __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   using Acc = typename Traits<S>::Acc;
   constexpr int ES = (int)sizeof(S);
-  constexpr int CPT = W * ES / 16;  // chunks per lane per input tile
   static_assert(W * ES >= 16 && W * ES <= 128, "staged rows are 16..128 bytes");
   constexpr int PITCH = W * ES + 16;
-  constexpr int WT = 64;            // rows per wave tile
+  constexpr int RPL = staged_rows_per_lane(W * ES);
+  constexpr int WT = 64 * RPL;                // rows per wave tile
+  constexpr int NCH = W * ES / 16 * RPL;      // chunks per lane per input tile
   __shared__ Acc G[W * W];
   __shared__ __attribute__((aligned(16))) unsigned char tiles[kThreads / 64][WT * PITCH];
   for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
@@ -281,25 +298,25 @@ __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   // is processed.  The tile stride stays an unsigned expression of gridDim.x
   // in the loop: held in a 64-bit variable it cost 18..20 VGPRs in every
   // class (FP64 W = 2: 36 -> 60) and 8% at 16-byte rows.
-  v4u rb[CPT], ra[CPT];
+  v4u rb[NCH], ra[NCH];
   int64_t tt = (int64_t)blockIdx.x * (kThreads / 64) + wave;
   if (tt < ntiles) {
-    if (dA) load_chunks<S, W>(B, tt * WT, rows_of(tt), a.m, rb);
-    if (dB) load_chunks<S, W>(A, tt * WT, rows_of(tt), a.k, ra);
+    if (dA) load_chunks<S, NCH>(B, tt * WT, rows_of(tt), a.m, rb);
+    if (dB) load_chunks<S, NCH>(A, tt * WT, rows_of(tt), a.k, ra);
   }
   for (; tt < ntiles; tt += gridDim.x * (kThreads / 64)) {
     const int64_t r0 = tt * WT;
     const int rows = rows_of(tt);
     const int64_t nt = tt + gridDim.x * (kThreads / 64);
-    v4u nb[CPT], na[CPT];
+    v4u nb[NCH], na[NCH];
     if (nt < ntiles) {
-      if (dA) load_chunks<S, W>(B, nt * WT, rows_of(nt), a.m, nb);
-      if (dB) load_chunks<S, W>(A, nt * WT, rows_of(nt), a.k, na);
+      if (dA) load_chunks<S, NCH>(B, nt * WT, rows_of(nt), a.m, nb);
+      if (dB) load_chunks<S, NCH>(A, nt * WT, rows_of(nt), a.k, na);
     }
-    if (dA) staged_phase<S, W, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, G);
-    if (dB) staged_phase<S, W, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, G);
+    if (dA) staged_phase<S, W, false, RPL, NCH>(rb, rows, a.m, a.k, dA, r0, tile, G);
+    if (dB) staged_phase<S, W, true, RPL, NCH>(ra, rows, a.k, a.m, dB, r0, tile, G);
 #pragma unroll
-    for (int i = 0; i < CPT; ++i) {
+    for (int i = 0; i < NCH; ++i) {
       rb[i] = nb[i];
       ra[i] = na[i];
     }
