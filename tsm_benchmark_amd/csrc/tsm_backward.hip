@@ -621,7 +621,7 @@ __device__ __forceinline__ void bwd_mfma_body(const BwdArgs& a) {
 // reads every input row (two 16-row blocks at a time, one accumulator each,
 // interleaved so that no instruction waits on the previous one's result);
 // after a barrier the waves overwrite the tile with their output columns.
-template <class S, int R, bool TRANSPOSED, int CPT, class Refill>
+template <class S, int R, int CPT, class Refill>
 __device__ __forceinline__ void mfma_cb_phase(v4u (&regs)[CPT], int rows, int in_w, int out_w,
                                               S* out, int64_t r0, unsigned char* tile,
                                               const S (&gv)[16], Refill refill) {
@@ -725,12 +725,12 @@ __device__ __forceinline__ void bwd_mfma_cb_body(const BwdArgs& a) {
     const bool more = nt < ntiles;
     const int nrows = more ? rows_of(nt) : 0;
     if (dA)
-      mfma_cb_phase<S, R, false, CPT>(rb, rows, a.m, a.k, dA, r0, tile, g0,
+      mfma_cb_phase<S, R, CPT>(rb, rows, a.m, a.k, dA, r0, tile, g0,
                                       [&](v4u (&r)[CPT]) {
                                         if (more) load_rows<S, W, R>(B, nt * R, nrows, a.m, r);
                                       });
     if (dB)
-      mfma_cb_phase<S, R, true, CPT>(ra, rows, a.k, a.m, dB, r0, tile, g1,
+      mfma_cb_phase<S, R, CPT>(ra, rows, a.k, a.m, dB, r0, tile, g1,
                                      [&](v4u (&r)[CPT]) {
                                        if (more) load_rows<S, W, R>(A, nt * R, nrows, a.k, r);
                                      });
