@@ -167,10 +167,12 @@ tsm_status tsm_tsmm(tsm_dtype in_type, tsm_dtype x_type, int64_t n, int k, int m
  * `count` independent problems C_i (+)= A_i^T B_i in ONE persistent launch.
  * `problems` is a HOST array; the library copies the work plan into device
  * memory (the workspace, or a plan's own buffer) before the launch.  All
- * problems share in_type/out_type; shapes may differ (1 <= k, m <= 64 for
- * TSM_F64, <= 32 for TSM_F32 and TSM_BF16).  The results are deterministic
- * for a given problem list.  Calls that share a workspace or a plan must be
- * ordered on one stream. */
+ * problems share in_type/out_type; shapes may differ (1 <= k, m <= 64).
+ * FP32 and BF16 problems with k or m above 32 run, inside the same launch, as
+ * column blocks of at most 32 x 32 (A and B are then read once per block of
+ * C's columns and rows).  The results are deterministic for a given problem
+ * list.  Calls that share a workspace or a plan must be ordered on one
+ * stream. */
 typedef struct {
   const void* A;
   const void* B;

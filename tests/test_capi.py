@@ -184,8 +184,17 @@ def test_batched_argument_errors():
     assert L.tsm_atb_batched(_lib.F64, _lib.F64, 2, P, 0, None, 0, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_atb_batched(_lib.F64, _lib.F64, -1, P, 0, None, 0, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_atb_batched(_lib.F64, _lib.BF16, 1, P, 0, None, 0, None) == _lib.ERR_UNSUPPORTED
-    P[1] = _lib.Problem(FAKE, FAKE, FAKE, 10, 64, 8, 8, 64, 8)
-    assert L.tsm_atb_batched(_lib.F32, _lib.F64, 2, P, 0, None, 0, None) == _lib.ERR_UNSUPPORTED
+    # FP32/BF16 problems wider than 32 run as column blocks: the layout of a
+    # 64 x 8 problem is that of its two 32 x 8 blocks (no device query)
+    W1 = (_lib.Problem * 1)()
+    W1[0] = _lib.Problem(FAKE, FAKE, FAKE, 10000, 64, 8, 8, 64, 8)
+    W2 = (_lib.Problem * 2)()
+    W2[0] = _lib.Problem(FAKE, FAKE, FAKE, 10000, 64, 8, 8, 32, 8)
+    W2[1] = _lib.Problem(FAKE + 128, FAKE, FAKE + 32 * 8 * 8, 10000, 64, 8, 8, 32, 8)
+    for dt in (_lib.F32, _lib.BF16):
+        assert L.tsm_atb_batched_workspace_size(dt, 1, W1) == L.tsm_atb_batched_workspace_size(dt, 2, W2) > 0
+    # FP64 has a 64-wide class: no split, another layout
+    assert L.tsm_atb_batched_workspace_size(_lib.F64, 1, W1) != L.tsm_atb_batched_workspace_size(_lib.F64, 2, W2)
     assert L.tsm_atb_batched(_lib.F64, _lib.F64, 0, None, 0, None, 0, None) == _lib.SUCCESS
     # the workspace size is a pure function of the problem list (no device query)
     P[1] = _lib.Problem(FAKE, FAKE, FAKE, 10000, 8, 8, 8, 8, 8)

@@ -116,9 +116,29 @@ def test_plan_api_and_graph_replay():
     plan.close()
 
 
-def test_batched_rejects_unsupported_width():
-    A = ops.random_matrix(100, 64, torch.float32, seed=SEED, stream_id=1)
-    B = ops.random_matrix(100, 8, torch.float32, seed=SEED, stream_id=2)
-    with pytest.raises(_lib.TsmError) as e:
-        ops.atb_batched([A], [B])
-    assert e.value.status == _lib.ERR_UNSUPPORTED
+@pytest.mark.parametrize("dt", [torch.float32, torch.bfloat16])
+def test_batched_wide_low_precision_as_column_blocks(dt):
+    """FP32/BF16 problems with k or m above 32 run as column blocks of at most
+    32 x 32 in the same launch: each C against the oracle, deterministic, and
+    accumulate / FP32 output / strided views on the split problems too."""
+    rng = np.random.default_rng(SEED + 7)
+    As, Bs = [], []
+    for i, (k, m) in enumerate([(64, 64), (33, 8), (8, 40), (48, 64), (16, 16), (64, 1), (2, 33)]):
+        n = int(rng.integers(1, 40000)) if i else 0
+        As.append(ops.random_matrix(n, k, dt, seed=SEED, stream_id=2 * i + 1))
+        Bs.append(ops.random_matrix(n, m, dt, seed=SEED, stream_id=2 * i + 2))
+    outs = ops.atb_batched(As, Bs, out_dtype=torch.float64)
+    again = ops.atb_batched(As, Bs, out_dtype=torch.float64)
+    for A, B, C, C2 in zip(As, Bs, outs, again):
+        assert torch.equal(C, C2)
+        check(C.cpu().numpy(), A, B, dt)
+    # views with lda > k, C += A^T B into FP32 outputs
+    W = ops.random_matrix(30011, 67, dt, seed=SEED, stream_id=40)
+    As2, Bs2 = [W[:, :64], W[:, 2:50]], [W[:, 3:67], W[:, :40]]
+    base = [torch.full((A.shape[1], B.shape[1]), 0.25, dtype=torch.float32, device="cuda")
+            for A, B in zip(As2, Bs2)]
+    outs2 = [b.clone() for b in base]
+    ops.atb_batched(As2, Bs2, outs=outs2, accumulate=True)
+    for A, B, C in zip(As2, Bs2, outs2):
+        ref = oracle.atb_dot2(host(A), host(B)) + 0.25
+        np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=2e-6, atol=2e-6 * (1 + np.abs(ref).max()))

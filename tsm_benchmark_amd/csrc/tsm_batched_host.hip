@@ -53,7 +53,39 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
   if (count < 0 || (count > 0 && !problems)) return TSM_ERR_INVALID_ARGUMENT;
   const tsm::BatchedKernel* bk = batched_kernel(in_type);
   if (!bk) return TSM_ERR_NO_KERNEL;
-  const int es = elem_bytes(in_type);
+  const int es = elem_bytes(in_type), out_es = elem_bytes(out_type);
+  // Problems wider than the kernel's widest class (FP32 and BF16 stop at 32:
+  // their (hi, lo) accumulators for 64 would not fit its register budget) run
+  // as column blocks of at most `widest` x `widest`: block (i, j) is
+  // A[:, i w ...]^T B[:, j w ...] into C's block (i, j), read through strided
+  // views, so A and B are read once per block column of C and row of C.
+  int widest = 0;
+  for (int c = 0; c < bk->n_classes; ++c) widest = std::max(widest, bk->classes[c].kc);
+  std::vector<tsm_problem> parts;
+  parts.reserve((size_t)std::max(count, 0));
+  for (int p = 0; p < count; ++p) {
+    const tsm_problem& q = problems[p];
+    tsm_status st = tsm_internal_check_atb(in_type, out_type, q.n, q.k, q.m, q.A, q.lda, q.B,
+                                           q.ldb, q.C, q.ldc);
+    if (st != TSM_SUCCESS) return st;
+    if (q.k <= widest && q.m <= widest) {
+      parts.push_back(q);
+      continue;
+    }
+    for (int i0 = 0; i0 < q.k; i0 += widest)
+      for (int j0 = 0; j0 < q.m; j0 += widest) {
+        tsm_problem b = q;
+        b.A = q.n > 0 ? static_cast<const unsigned char*>(q.A) + (size_t)i0 * es : q.A;
+        b.B = q.n > 0 ? static_cast<const unsigned char*>(q.B) + (size_t)j0 * es : q.B;
+        b.C = static_cast<unsigned char*>(q.C) + ((size_t)i0 * q.ldc + j0) * out_es;
+        b.k = std::min(widest, q.k - i0);
+        b.m = std::min(widest, q.m - j0);
+        parts.push_back(b);
+      }
+  }
+  if (parts.size() > (size_t)0x7fffffff) return TSM_ERR_INVALID_ARGUMENT;
+  count = (int)parts.size();
+  problems = parts.data();
   int64_t partial_words = 0;
   L->probs.resize(count);
   for (int p = 0; p < count; ++p) {
