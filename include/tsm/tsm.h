@@ -142,10 +142,10 @@ size_t tsm_ata_workspace_size(tsm_dtype in_type, int64_t n, int k,
  * dA or dB may be NULL to skip it; both use one streaming pass over the rows.
  * dC is in out_type (TSM_F64 or TSM_F32); A, B, dA, dB are in in_type.
  * Each output is a sum of products accumulated in FP64 for FP64 inputs and in
- * FP32 otherwise, rounded once to in_type.  Dense 32- and 64-wide classes run
- * on matrix instructions (another summation order than the narrower
- * classes); for BF16 they split dC exactly into three BF16 pieces, so every
- * product stays exact. */
+ * FP32 otherwise, rounded once to in_type.  Dense 32- and 64-wide classes,
+ * and the FP64/FP32 16-wide ones, run on matrix instructions (another
+ * summation order than the narrower classes); for BF16 they split dC exactly
+ * into three BF16 pieces, so every product stays exact. */
 tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, int k, int m,
                             const void* A, int64_t lda, const void* B, int64_t ldb,
                             const void* dC, int64_t lddc,
@@ -158,7 +158,8 @@ tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, int64_t n, i
  * dB = A dC half of tsm_atb_backward (same kernels): each output is a sum of
  * k products in FP64 for FP64 inputs and FP32 otherwise, rounded once to
  * in_type.  Rows of 16..128 bytes with lda == k and ldy == m take the
- * LDS-staged kernel. */
+ * LDS-staged kernel, or a matrix-core kernel for the FP64/FP32 16-wide and
+ * all 32- and 64-wide classes. */
 tsm_status tsm_tsmm(tsm_dtype in_type, tsm_dtype x_type, int64_t n, int k, int m,
                     const void* A, int64_t lda, const void* X, int64_t ldx, void* Y,
                     int64_t ldy, tsm_stream_t stream);

@@ -287,6 +287,15 @@ def test_backward_against_oracle(dt, k, m):
     assert dB2 is None and torch.equal(dA2, dA)
 
 
+def valu_class(dt, k, m):
+    """Dense rows of this shape take a VALU kernel (staged or packed), whose
+    bits equal the row-wise kernel's."""
+    w = 1
+    while w < max(k, m):
+        w *= 2
+    return w <= (16 if dt == "bf16" else 8)
+
+
 @pytest.mark.parametrize("dt,k,m,n", [("f32", 16, 16, 70001), ("f32", 12, 4, 3001), ("f32", 32, 32, 300),
                                       ("f64", 2, 16, 5), ("f64", 16, 16, 4097), ("bf16", 8, 8, 1000),
                                       ("bf16", 64, 16, 2049), ("bf16", 4, 4, 100003), ("bf16", 4, 4, 3),
@@ -297,11 +306,12 @@ def test_backward_against_oracle(dt, k, m):
                                       ("f32", 8, 4, 127), ("bf16", 16, 16, 40001), ("bf16", 16, 8, 257),
                                       ("bf16", 8, 16, 64)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
-    """Dense rows take the LDS-staged kernel (widths up to 16), the packed
-    kernel (square classes with rows under 16 bytes) or the matrix-core
-    kernel (32, 64); strided views take the row-wise one.  The staged, packed
-    and row-wise kernels add the same products in the same order, so their
-    bits agree; the matrix-core kernel is held to the oracle tolerance."""
+    """Dense rows take the LDS-staged kernel (widths up to 8, BF16 up to 16),
+    the packed kernel (square classes with rows under 16 bytes) or a
+    matrix-core kernel (FP64/FP32 16, all 32 and 64); strided views take the
+    row-wise one.  The staged, packed and row-wise kernels add the same
+    products in the same order, so their bits agree; the matrix-core kernels
+    are held to the oracle tolerance."""
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
     G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
     gdt = torch.float64 if dt == "f64" else torch.float32
@@ -311,7 +321,7 @@ def test_backward_staged_equals_rowwise(dt, k, m, n):
     Aw[:, :k] = A
     Bw[:, :m] = B
     dA2, dB2 = ops.atb_backward(Aw[:, :k], Bw[:, :m], G.to("cuda", gdt))
-    if max(k, m) <= 16:
+    if valu_class(dt, k, m):
         assert torch.equal(dA, dA2) and torch.equal(dB, dB2)
     dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.to(gdt).double().numpy())
     tol = 1e-13 if dt == "f64" else (1e-5 if dt == "f32" else 2**-7 * np.abs(G.numpy()).sum())
@@ -322,12 +332,13 @@ def test_backward_staged_equals_rowwise(dt, k, m, n):
 @pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
 @pytest.mark.parametrize("k,m,n", [(32, 32, 40001), (64, 64, 20011), (40, 24, 1000), (64, 8, 129),
                                    (8, 64, 127), (48, 48, 5), (32, 64, 128), (32, 32, 63),
-                                   (24, 32, 257), (32, 16, 5), (32, 32, 513)])
+                                   (24, 32, 257), (32, 16, 5), (32, 32, 513), (16, 16, 257),
+                                   (16, 4, 1000), (12, 16, 129), (16, 8, 255)])
 def test_backward_matrix_core_against_oracle(dt, k, m, n):
-    """The 32- and 64-wide classes (dense rows) take the matrix-core kernel
-    (tiles of 64 rows for FP64 W = 32, 256 for BF16 W = 32, 128 otherwise:
-    the row counts cover one partial tile, a tile plus one row and several
-    tiles plus a partial one)."""
+    """The 32- and 64-wide classes, and the FP64/FP32 16-wide ones (dense
+    rows), take a matrix-core kernel (tiles of 64 rows for FP64 W = 32, 256
+    for BF16 W = 32 and FP32 W = 16, 128 otherwise: the row counts cover one
+    partial tile, a tile plus one row and several tiles plus a partial one)."""
     if (k * DT[dt].itemsize) % 16 or (m * DT[dt].itemsize) % 16:
         pytest.skip("rows that are not whole 16-byte chunks take the row-wise kernel")
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)

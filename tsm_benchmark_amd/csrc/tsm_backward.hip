@@ -20,8 +20,9 @@
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
 // a sum of at most 64 products, added in the same order by these three
 // kernels, so they produce the same bits).  Dense rows of the 32- and 64-wide
-// classes take the matrix-core kernels further down (another summation
-// order; for BF16, dC split exactly into three BF16 pieces).
+// classes, and of the FP64 and FP32 16-wide ones, take the matrix-core
+// kernels further down (another summation order; for BF16, dC split exactly
+// into three BF16 pieces).
 #include <atomic>
 
 #include "tsm/tsm.h"
@@ -919,12 +920,18 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
 
 // Rows per tile: 64 for FP64 W = 32 (measured 5.61 TB/s against 5.36 with
 // 128), 128 otherwise (FP32 W = 32: 5.59 against 5.33 with 64 and 5.35 with
-// 256; FP32 W = 64: 5.23 against 4.57 with 256).
+// 256; FP32 W = 64: 5.23 against 4.57 with 256).  The 16-wide classes run
+// here too: FP64 W = 16 with 128 rows 5.71-5.75 TB/s (64 rows: 5.40-5.42,
+// 256: 5.48-5.50; the staged VALU kernel 5.17-5.26), FP32 W = 16 with 256
+// rows 5.73-5.77 (128: 5.39-5.40, 512: 5.45; staged 5.46-5.50).  Routing the
+// 8-wide classes here (zero-padded to 16) measured 6-7% slower than staged.
 #define TSM_BWD_MFMA(NAME, S, W, R)                                                   \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_mfma_body<S, W, R>(a);                                                   \
   }
 
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w16, double, 16, 128)
+TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w16, float, 16, 256)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f64_w32, double, 32, 64)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w32, float, 32, 128)
 TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64, 128)
@@ -1111,7 +1118,10 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   }
   BwdKernel fast = nullptr;
   int64_t tile_rows = 256;
-  if (dense && wi >= 5) {
+  if (dense && wi == 4 && in_type != TSM_BF16) {
+    fast = in_type == TSM_F64 ? tsm_atb_bwd_mfma_f64_w16 : tsm_atb_bwd_mfma_f32_w16;
+    tile_rows = 128;
+  } else if (dense && wi >= 5) {
     fast = kBwdMfma[(int)in_type][wi - 5];
     tile_rows = 128;
   } else if (dense) {
