@@ -140,24 +140,25 @@ def test_tsmm_against_oracle_and_backward(dt, k, m):
     _, dB = ops.atb_backward(A, B, X.to("cuda", xdt), need_dA=False)
     assert torch.equal(Y, dB)
     # a strided view of A takes the row-wise kernel: the same bits as the
-    # staged kernel (widths up to 8, BF16 up to 16), within the tolerance for
-    # the matrix-core classes
+    # staged kernel (widths up to 8), within the tolerance for the
+    # matrix-core classes
     W = torch.zeros((n, k + 2), dtype=DT[dt], device="cuda")
     W[:, :k] = A
     Yw = ops.tsmm(W[:, :k], X.to("cuda", xdt))
-    if max(k, m) <= (16 if dt == "bf16" else 8):
+    if max(k, m) <= 8:
         assert torch.equal(Yw, Y)
     else:
         np.testing.assert_allclose(host(Yw), ref, **tol)
 
 
-@pytest.mark.parametrize("w", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("w", [1, 2, 4, 8])
 def test_tsmm_bf16_rounding_is_nearest_even(w):
     """With a diagonal X every output is one FP32 product a * x rounded once
     to BF16 (adding the FP32 zero products is exact), so the outputs of the
     packed (w <= 4) and staged kernels must equal torch's CPU conversion
     (round to nearest even) of the FP32 product, bit for bit; a strided view
-    (row-wise kernel) must give the same bits."""
+    (row-wise kernel) must give the same bits.  (Wider classes run on matrix
+    instructions, whose FP32 sums are not rounded to nearest.)"""
     n = 50021
     A = rand(n, w, torch.bfloat16, 11)
     rng = np.random.default_rng(SEED + w)

@@ -288,12 +288,9 @@ def test_backward_against_oracle(dt, k, m):
 
 
 def valu_class(dt, k, m):
-    """Dense rows of this shape take a VALU kernel (staged or packed), whose
-    bits equal the row-wise kernel's."""
-    w = 1
-    while w < max(k, m):
-        w *= 2
-    return w <= (16 if dt == "bf16" else 8)
+    """Dense rows of this shape take a VALU kernel (staged or packed, widths
+    up to 8), whose bits equal the row-wise kernel's."""
+    return max(k, m) <= 8
 
 
 @pytest.mark.parametrize("dt,k,m,n", [("f32", 16, 16, 70001), ("f32", 12, 4, 3001), ("f32", 32, 32, 300),
@@ -306,10 +303,9 @@ def valu_class(dt, k, m):
                                       ("f32", 8, 4, 127), ("bf16", 16, 16, 40001), ("bf16", 16, 8, 257),
                                       ("bf16", 8, 16, 64)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
-    """Dense rows take the LDS-staged kernel (widths up to 8, BF16 up to 16),
-    the packed kernel (square classes with rows under 16 bytes) or a
-    matrix-core kernel (FP64/FP32 16, all 32 and 64); strided views take the
-    row-wise one.  The staged, packed and row-wise kernels add the same
+    """Dense rows take the LDS-staged kernel (widths up to 8), the packed
+    kernel (square classes with rows under 16 bytes) or a matrix-core kernel
+    (16, 32 and 64); strided views take the row-wise one.  The staged, packed and row-wise kernels add the same
     products in the same order, so their bits agree; the matrix-core kernels
     are held to the oracle tolerance."""
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
@@ -360,7 +356,7 @@ def test_backward_matrix_core_against_oracle(dt, k, m, n):
 
 @pytest.mark.parametrize("k,m,n", [(64, 64, 20011), (32, 32, 40001), (40, 24, 1000), (64, 8, 129),
                                    (8, 64, 127), (48, 48, 5), (32, 32, 255), (24, 32, 257),
-                                   (32, 16, 5)])
+                                   (32, 16, 5), (16, 16, 40001), (16, 8, 513), (8, 16, 511)])
 def test_backward_bf16_matrix_core_against_fp32_path(k, m, n):
     """The BF16 matrix-core kernel (dC split exactly into three BF16 pieces,
     FP32 accumulation in the matrix instruction) against the row-wise kernel

@@ -6,12 +6,13 @@
 // workgroup in LDS, zero-padded to the width class W >= max(k, m), and read
 // with wave-uniform addresses (LDS broadcast).  Each thread computes whole
 // rows.  Three kernels share that arithmetic:
-//   staged  (dense rows of 16..128 bytes): each wave moves its own tiles of
-//           64 rows between HBM and its part of LDS with consecutive 16-byte
-//           loads and stores (each instruction covers 1 KiB of contiguous
-//           bytes); a lane reads its row from the padded LDS tile and writes
-//           its output row back there before the tile is stored, and no
-//           workgroup barrier is needed;
+//   staged  (dense rows of 16..64 bytes, widths up to 8): each wave moves
+//           its own tiles of 64 rows (128 for 32-byte rows) between HBM and
+//           its part of LDS with consecutive 16-byte loads and stores (each
+//           instruction covers 1 KiB of contiguous bytes); a lane reads its
+//           rows from the padded LDS tile and writes its output rows back
+//           there before the tile is stored, and no workgroup barrier is
+//           needed;
 //   packed  (k == m == W, dense rows under 16 bytes): a thread moves 16
 //           consecutive bytes, 16 / (W s) whole rows, in one load and one
 //           store;
@@ -19,10 +20,9 @@
 //           16 bytes at a time when dense.
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
 // a sum of at most 64 products, added in the same order by these three
-// kernels, so they produce the same bits).  Dense rows of the 32- and 64-wide
-// classes, and of the FP64 and FP32 16-wide ones, take the matrix-core
-// kernels further down (another summation order; for BF16, dC split exactly
-// into three BF16 pieces).
+// kernels, so they produce the same bits).  Dense rows of the 16-, 32- and
+// 64-wide classes take the matrix-core kernels further down (another
+// summation order; for BF16, dC split exactly into three BF16 pieces).
 #include <atomic>
 
 #include "tsm/tsm.h"
@@ -742,30 +742,55 @@ __device__ __forceinline__ void bwd_mfma_cb_body(const BwdArgs& a) {
 // dC (FP32) is split exactly into three BF16 pieces, g = h + m + l with
 // h = bf16(g), m = bf16(g - h), l = bf16(g - h - m) (each difference is exact
 // in FP32), so every product of a BF16 input and a piece is exact and the
-// sums run in the FP32 accumulation of v_mfma_f32_16x16x32_bf16.  A 16-row
-// block times 16 output columns takes, per K-step of 32 input columns, three
-// instructions (pieces l, m, h, in that order).  Operands of lane l:
-//   input  row l % 16 of the block, columns 32 ks + 8 (l / 16) + e, e = 0..7:
-//          16 consecutive bytes of the LDS tile row;
-//   piece  output column 16 nb + l % 16, input columns 32 ks + 8 (l / 16) + e:
-//          built once per workgroup in LDS, in lane order (1 KiB each).
+// sums run in the FP32 accumulation of the matrix instruction: for W = 32, 64
+// v_mfma_f32_16x16x32_bf16 (K-steps of K = 32 input columns, E = 8 elements
+// per lane), for W = 16 v_mfma_f32_16x16x16_bf16 (K = 16, E = 4).  A 16-row
+// block times 16 output columns takes, per K-step, three instructions
+// (pieces l, m, h, in that order).  Operands of lane l:
+//   input  row l % 16 of the block, columns K ks + E (l / 16) + e, e < E:
+//          2 E consecutive bytes of the LDS tile row;
+//   piece  output column 16 nb + l % 16, input columns K ks + E (l / 16) + e:
+//          built once per workgroup in LDS, in lane order.
 // Output of one block: lane l, register v holds row 4 (l / 16) + v, column
 // l % 16.  The instruction's FP32 rounding of its 32-product sums is biased
 // toward -infinity by about 1e-8 of the sum of |products|, far below the
 // BF16 rounding of the outputs.
 constexpr int kBf16Pieces = 3;
 
+typedef short v4s __attribute__((ext_vector_type(4)));
+template <int K>
+struct Bf16Step;
+template <>
+struct Bf16Step<32> {
+  using Op = v8bf;
+  static constexpr int E = 8;
+  __device__ static v4f mma(Op a, Op b, v4f c) { return mfma_step(a, b, c); }
+};
+template <>
+struct Bf16Step<16> {
+  using Op = v4s;
+  static constexpr int E = 4;
+  __device__ static v4f mma(Op a, Op b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+};
 template <int W>
-constexpr int bf16_bwd_operands() { return 2 * (W / 32) * (W / 16) * kBf16Pieces; }
+constexpr int bf16_kstep() { return W >= 32 ? 32 : 16; }
+
+template <int W>
+constexpr int bf16_bwd_operands() { return 2 * (W / bf16_kstep<W>()) * (W / 16) * kBf16Pieces; }
 
 template <int W, int R, bool TRANSPOSED, int CPT>
-__device__ __forceinline__ void mfma_bf16_phase(const v4u (&regs)[CPT], int rows, int in_w,
-                                                int out_w, bf16* out, int64_t r0,
-                                                unsigned char* tile, const v8bf* ops) {
+__device__ __forceinline__ void mfma_bf16_phase(
+    const v4u (&regs)[CPT], int rows, int in_w, int out_w, bf16* out, int64_t r0,
+    unsigned char* tile, const typename Bf16Step<bf16_kstep<W>()>::Op* ops) {
+  using St = Bf16Step<bf16_kstep<W>()>;
+  using Op = typename St::Op;
+  constexpr int K = bf16_kstep<W>(), E = St::E;
   constexpr int PITCH = mfma_bwd_pitch<bf16, W, R>();
   constexpr int PER = 8;           // BF16 elements per chunk
   constexpr int CPR = W / PER;     // chunks per padded row
-  constexpr int NB = W / 16, KS = W / 32, RBW = R / 64;  // row blocks per wave
+  constexpr int NB = W / 16, KS = W / K, RBW = R / 64;  // row blocks per wave
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int in_cpr = in_w / PER, out_cpr = out_w / PER;
   // stage the input tile and zero the padding columns [in_w, W) of every row
@@ -786,28 +811,28 @@ __device__ __forceinline__ void mfma_bf16_phase(const v4u (&regs)[CPT], int rows
   }
   __syncthreads();
   // wave w owns row blocks w, w + 4, ...; it reads and then overwrites only those rows
-  v8bf av[RBW][KS];
+  Op av[RBW][KS];
 #pragma unroll
   for (int j = 0; j < RBW; ++j)
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
-      av[j][ks] = *reinterpret_cast<const v8bf*>(
-          tile + ((wave + 4 * j) * 16 + (lane & 15)) * PITCH + (32 * ks + 8 * (lane >> 4)) * 2);
+      av[j][ks] = *reinterpret_cast<const Op*>(
+          tile + ((wave + 4 * j) * 16 + (lane & 15)) * PITCH + (K * ks + E * (lane >> 4)) * 2);
   v4f acc[RBW][NB];
 #pragma unroll
   for (int j = 0; j < RBW; ++j)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) acc[j][nb] = v4f{};
-  const v8bf* po = ops + (TRANSPOSED ? KS * NB * kBf16Pieces * 64 : 0);
+  const Op* po = ops + (TRANSPOSED ? KS * NB * kBf16Pieces * 64 : 0);
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
       for (int p = kBf16Pieces - 1; p >= 0; --p) {
-        const v8bf bv = po[((ks * NB + nb) * kBf16Pieces + p) * 64 + lane];
+        const Op bv = po[((ks * NB + nb) * kBf16Pieces + p) * 64 + lane];
 #pragma unroll
-        for (int j = 0; j < RBW; ++j) acc[j][nb] = mfma_step(av[j][ks], bv, acc[j][nb]);
+        for (int j = 0; j < RBW; ++j) acc[j][nb] = St::mma(av[j][ks], bv, acc[j][nb]);
       }
 #pragma unroll
   for (int j = 0; j < RBW; ++j)
@@ -832,13 +857,16 @@ __device__ __forceinline__ void mfma_bf16_phase(const v4u (&regs)[CPT], int rows
 
 template <int W, int R>
 __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
+  using St = Bf16Step<bf16_kstep<W>()>;
+  using Op = typename St::Op;
+  constexpr int K = bf16_kstep<W>(), E = St::E;
   constexpr int CPT = R * W * 2 / 16 / kThreads;  // input chunks per thread (dense rows)
-  static_assert(W % 32 == 0 && R % 64 == 0 && CPT >= 1, "32-wide K-steps, 4 waves");
+  static_assert(W % K == 0 && R % 64 == 0 && CPT >= 1, "whole K-steps, 4 waves");
   constexpr int PITCH = mfma_bwd_pitch<bf16, W, R>();
   constexpr int GP = W + 1;
-  constexpr int NB = W / 16, KS = W / 32;
+  constexpr int NB = W / 16, KS = W / K;
   static_assert(R * PITCH >= W * GP * 4, "dC is staged in the tile's memory");
-  __shared__ v8bf ops[bf16_bwd_operands<W>() * 64];
+  __shared__ Op ops[bf16_bwd_operands<W>() * 64];
   __shared__ __attribute__((aligned(16))) unsigned char tile[R * PITCH];
   float* Gs = reinterpret_cast<float*>(tile);
   for (int e = threadIdx.x; e < W * W; e += blockDim.x) {
@@ -856,13 +884,13 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
   for (int idx = threadIdx.x; idx < 2 * KS * NB * 64; idx += kThreads) {
     const int ln = idx & 63, q = idx >> 6;  // q = (phase * KS + ks) * NB + nb
     const int nb = q % NB, ks = (q / NB) % KS, ph = q / (NB * KS);
-    uint32_t w[kBf16Pieces][4];
+    uint32_t w[kBf16Pieces][E / 2];
 #pragma unroll
-    for (int e = 0; e < 8; e += 2) {
+    for (int e = 0; e < E; e += 2) {
       uint32_t h[kBf16Pieces][2];
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
-        const int t = 32 * ks + 8 * (ln >> 4) + e + x, o = 16 * nb + (ln & 15);
+        const int t = K * ks + E * (ln >> 4) + e + x, o = 16 * nb + (ln & 15);
         float g = ph ? Gs[t * GP + o] : Gs[o * GP + t];
 #pragma unroll
         for (int p = 0; p < kBf16Pieces; ++p) {
@@ -875,9 +903,11 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
       for (int p = 0; p < kBf16Pieces; ++p) w[p][e / 2] = h[p][0] | (h[p][1] << 16);
     }
 #pragma unroll
-    for (int p = 0; p < kBf16Pieces; ++p)
-      ops[(q * kBf16Pieces + p) * 64 + ln] =
-          __builtin_bit_cast(v8bf, v4u{w[p][0], w[p][1], w[p][2], w[p][3]});
+    for (int p = 0; p < kBf16Pieces; ++p) {
+      Op v;
+      __builtin_memcpy(&v, w[p], sizeof(Op));
+      ops[(q * kBf16Pieces + p) * 64 + ln] = v;
+    }
   }
   __syncthreads();
   const bf16* A = static_cast<const bf16*>(a.A);
@@ -943,13 +973,17 @@ TSM_BWD_MFMA(tsm_atb_bwd_mfma_f32_w64, float, 64, 128)
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_bwd_mfma_f64_w64(tsm::BwdArgs a) {
   tsm::bwd_mfma_cb_body<double, 64>(a);
 }
-// Rows per tile: 256 at W = 32 (5.70 TB/s against 5.26 with 128), 128 at
-// W = 64 (5.32 against 3.83 with 256).
+// Rows per tile: 512 at W = 16 (5.79-5.84 TB/s against 5.32 with 128, 5.23
+// with 256, 5.21-5.31 with 1024, and 4.85-4.91 for the staged VALU kernel;
+// routing W = 8 here measured 4.75-4.80 against 5.65 staged), 256 at W = 32
+// (5.70 TB/s against 5.26 with 128), 128 at W = 64 (5.32 against 3.83 with
+// 256).
 #define TSM_BWD_MFMA_BF16(NAME, W, R)                                                 \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_mfma_bf16_body<W, R>(a);                                                 \
   }
 
+TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w16, 16, 512)
 TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w32, 32, 256)
 TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w64, 64, 128)
 
@@ -958,18 +992,14 @@ TSM_BWD_MFMA_BF16(tsm_atb_bwd_mfma_bf16_w64, 64, 128)
     tsm::bwd_staged_body<S, W>(a);                                                    \
   }
 
+// Dense rows of 16..64 bytes in the classes up to 8 wide (16 and wider run on
+// the matrix-core kernels).
 TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w2, double, 2)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w4, double, 4)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w8, double, 8)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_f64_w16, double, 16)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w4, float, 4)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w8, float, 8)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w16, float, 16)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_f32_w32, float, 32)
 TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w8, tsm::bf16, 8)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w16, tsm::bf16, 16)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w32, tsm::bf16, 32)
-TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w64, tsm::bf16, 64)
 
 #define TSM_BWD_PACKED(NAME, S, W)                                                    \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
@@ -1023,11 +1053,10 @@ const BwdKernel kBwd[3][7] = {
 // Staged kernels by dtype and log2(W); null where W * s is outside 16..128 bytes.
 const BwdKernel kBwdStaged[3][7] = {
     {nullptr, tsm_atb_bwd_staged_f64_w2, tsm_atb_bwd_staged_f64_w4, tsm_atb_bwd_staged_f64_w8,
-     tsm_atb_bwd_staged_f64_w16, nullptr, nullptr},
-    {nullptr, nullptr, tsm_atb_bwd_staged_f32_w4, tsm_atb_bwd_staged_f32_w8,
-     tsm_atb_bwd_staged_f32_w16, tsm_atb_bwd_staged_f32_w32, nullptr},
-    {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, tsm_atb_bwd_staged_bf16_w16,
-     tsm_atb_bwd_staged_bf16_w32, tsm_atb_bwd_staged_bf16_w64},
+     nullptr, nullptr, nullptr},
+    {nullptr, nullptr, tsm_atb_bwd_staged_f32_w4, tsm_atb_bwd_staged_f32_w8, nullptr, nullptr,
+     nullptr},
+    {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, nullptr, nullptr, nullptr},
 };
 // Packed kernels by dtype and log2(W); null where W * s is 16 bytes or more.
 const BwdKernel kBwdPacked[3][3] = {
@@ -1118,8 +1147,10 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   }
   BwdKernel fast = nullptr;
   int64_t tile_rows = 256;
-  if (dense && wi == 4 && in_type != TSM_BF16) {
-    fast = in_type == TSM_F64 ? tsm_atb_bwd_mfma_f64_w16 : tsm_atb_bwd_mfma_f32_w16;
+  if (dense && wi == 4) {
+    fast = in_type == TSM_F64   ? tsm_atb_bwd_mfma_f64_w16
+           : in_type == TSM_F32 ? tsm_atb_bwd_mfma_f32_w16
+                                : tsm_atb_bwd_mfma_bf16_w16;
     tile_rows = 128;
   } else if (dense && wi >= 5) {
     fast = kBwdMfma[(int)in_type][wi - 5];
