@@ -256,10 +256,9 @@ __device__ __forceinline__ void load_chunks(const S* in, int64_t r0, int rows, i
 // Rows per lane of the staged kernel.  A lane that owns two rows reads each
 // dC coefficient from LDS once for both; the reads are wave-uniform, and each
 // still moves 64 copies through the LDS return path, which 32-byte rows with
-// one row per lane nearly saturate (BF16 W = 16: 256 coefficients per 32-byte
-// row).  Measured with two rows per lane (TB/s, dA and dB): 32-byte rows
-// +3..6% (FP64 W = 4 5.38 -> 5.68, FP32 W = 8 5.36 -> 5.66, BF16 W = 16
-// 4.91 -> 5.05); other widths -1..15% (more VGPRs, fewer waves).
+// one row per lane load most.  Measured with two rows per lane (TB/s, dA and
+// dB): 32-byte rows +5..6% (FP64 W = 4 5.38 -> 5.68, FP32 W = 8 5.36 ->
+// 5.66); other widths -1..15% (more VGPRs, fewer waves).
 constexpr int staged_rows_per_lane(int row_bytes) { return row_bytes == 32 ? 2 : 1; }
 
 template <class S, int W>
