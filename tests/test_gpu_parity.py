@@ -301,7 +301,9 @@ def valu_class(dt, k, m):
                                       # 32-byte rows: two rows per lane in the staged kernel
                                       ("f64", 4, 4, 70001), ("f64", 4, 2, 129), ("f32", 8, 8, 100003),
                                       ("f32", 8, 4, 127), ("bf16", 16, 16, 40001), ("bf16", 16, 8, 257),
-                                      ("bf16", 8, 16, 64)])
+                                      ("bf16", 8, 16, 64),
+                                      # one 16-byte row per chunk in the packed kernel
+                                      ("f64", 2, 2, 70001), ("f64", 2, 2, 3), ("f32", 4, 4, 5001)])
 def test_backward_staged_equals_rowwise(dt, k, m, n):
     """Dense rows take the LDS-staged kernel (widths up to 8), the packed
     kernel (square classes with rows under 16 bytes) or a matrix-core kernel

@@ -13,9 +13,9 @@
 //           rows from the padded LDS tile and writes its output rows back
 //           there before the tile is stored, and no workgroup barrier is
 //           needed;
-//   packed  (k == m == W, dense rows under 16 bytes): a thread moves 16
-//           consecutive bytes, 16 / (W s) whole rows, in one load and one
-//           store;
+//   packed  (k == m == W, dense rows of at most 16 bytes; BF16 W = 8 is
+//           staged): a thread moves 16 consecutive bytes, 16 / (W s) whole
+//           rows, in one load and one store;
 //   rowwise (any layout): a thread loads and stores its own rows directly,
 //           16 bytes at a time when dense.
 // Accumulation is in FP64 for FP64 inputs and FP32 otherwise (each output is
@@ -323,8 +323,8 @@ __device__ __forceinline__ void bwd_staged_body(const BwdArgs& a) {
   }
 }
 
-// ---- packed kernel (square classes with rows under 16 bytes) ------------------------
-// k == m == W, W * s < 16, dense: chunk c (16 bytes) of the input holds rows
+// ---- packed kernel (square classes with rows of at most 16 bytes) --------------------
+// k == m == W, W * s <= 16, dense: chunk c (16 bytes) of the input holds rows
 // c * PR ... c * PR + PR - 1 (PR = 16 / (W s)), and so does chunk c of the
 // output.  One thread per chunk (grid-strided): it issues the non-temporal
 // 16-byte loads of both phases first, then forms each chunk's PR rows (the
@@ -339,7 +339,7 @@ __device__ __forceinline__ void packed_chunk(const v4u u, S* out, int64_t c,
                                              const typename Traits<S>::Acc* G) {
   using Acc = typename Traits<S>::Acc;
   constexpr int PER = 16 / (int)sizeof(S), PR = PER / W;
-  static_assert(PR >= 2 && PR * W == PER, "rows under 16 bytes, whole rows per chunk");
+  static_assert(PR >= 1 && PR * W == PER, "rows of at most 16 bytes, whole rows per chunk");
   Acc x[PER];
   Unpack<S>::run(u, x);
   S y[PER];
@@ -1013,9 +1013,14 @@ TSM_BWD_STAGED(tsm_atb_bwd_staged_bf16_w8, tsm::bf16, 8)
     tsm::bwd_packed_body<S, W>(a);                                                    \
   }
 
+// One 16-byte row per chunk for FP64 W = 2 and FP32 W = 4: 5.61-5.64 TB/s
+// against 5.46-5.48 for the staged kernel; BF16 W = 8 stays staged (5.66
+// against 5.17 packed).
 TSM_BWD_PACKED(tsm_atb_bwd_packed_f64_w1, double, 1)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_f64_w2, double, 2)
 TSM_BWD_PACKED(tsm_atb_bwd_packed_f32_w1, float, 1)
 TSM_BWD_PACKED(tsm_atb_bwd_packed_f32_w2, float, 2)
+TSM_BWD_PACKED(tsm_atb_bwd_packed_f32_w4, float, 4)
 TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w1, tsm::bf16, 1)
 TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w2, tsm::bf16, 2)
 TSM_BWD_PACKED(tsm_atb_bwd_packed_bf16_w4, tsm::bf16, 4)
@@ -1065,10 +1070,11 @@ const BwdKernel kBwdStaged[3][7] = {
      nullptr},
     {nullptr, nullptr, nullptr, tsm_atb_bwd_staged_bf16_w8, nullptr, nullptr, nullptr},
 };
-// Packed kernels by dtype and log2(W); null where W * s is 16 bytes or more.
+// Packed kernels by dtype and log2(W); null where W * s is over 16 bytes, and
+// for BF16 W = 8 (staged).
 const BwdKernel kBwdPacked[3][3] = {
-    {tsm_atb_bwd_packed_f64_w1, nullptr, nullptr},
-    {tsm_atb_bwd_packed_f32_w1, tsm_atb_bwd_packed_f32_w2, nullptr},
+    {tsm_atb_bwd_packed_f64_w1, tsm_atb_bwd_packed_f64_w2, nullptr},
+    {tsm_atb_bwd_packed_f32_w1, tsm_atb_bwd_packed_f32_w2, tsm_atb_bwd_packed_f32_w4},
     {tsm_atb_bwd_packed_bf16_w1, tsm_atb_bwd_packed_bf16_w2, tsm_atb_bwd_packed_bf16_w4},
 };
 // Matrix-core kernels for the 32- and 64-wide classes (dense rows).
@@ -1140,9 +1146,10 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   BwdKernel kern = kBwd[(int)in_type][wi];
   int64_t cap = (int64_t)cus * 8;
   const bool dense = a.dense_a && a.dense_b;
-  // Square classes whose rows are under 16 bytes, every moving matrix
-  // contiguous (leading dimension = width) and 16-byte aligned.
-  const bool packed = k == m && k == w && k * (int)es < 16 && wi < 3 &&
+  // Square classes whose rows are at most 16 bytes (a packed kernel exists),
+  // every moving matrix contiguous (leading dimension = width) and 16-byte
+  // aligned.
+  const bool packed = k == m && k == w && k * (int)es <= 16 && wi < 3 &&
                       kBwdPacked[(int)in_type][wi] && (!dA || (ldb == m && ldda == k && al16(B) &&
                                                                al16(dA))) &&
                       (!dB || (lda == k && lddb == m && al16(A) && al16(dB)));
@@ -1162,7 +1169,7 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   } else if (dense && wi >= 5) {
     fast = kBwdMfma[(int)in_type][wi - 5];
     tile_rows = 128;
-  } else if (dense) {
+  } else if (dense && !packed) {
     fast = kBwdStaged[(int)in_type][wi];
   }
   if (fast) {
