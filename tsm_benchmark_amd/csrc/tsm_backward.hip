@@ -957,11 +957,12 @@ __device__ __forceinline__ void bwd_mfma_bf16_body(const BwdArgs& a) {
 
 // Rows per tile: 64 for FP64 W = 32 (measured 5.61 TB/s against 5.36 with
 // 128), 128 otherwise (FP32 W = 32: 5.59 against 5.33 with 64 and 5.35 with
-// 256; FP32 W = 64: 5.23 against 4.57 with 256).  The 16-wide classes run
-// here too: FP64 W = 16 with 128 rows 5.71-5.75 TB/s (64 rows: 5.40-5.42,
-// 256: 5.48-5.50; the staged VALU kernel 5.17-5.26), FP32 W = 16 with 256
-// rows 5.73-5.77 (128: 5.39-5.40, 512: 5.45; staged 5.46-5.50).  Routing the
-// 8-wide classes here (zero-padded to 16) measured 6-7% slower than staged.
+// 256; FP32 W = 64: 5.23 against 4.57 with 256 and 5.08-5.20 with 64).  The
+// 16-wide classes run here too: FP64 W = 16 with 128 rows 5.71-5.75 TB/s (64
+// rows: 5.40-5.42, 256: 5.48-5.50; the staged VALU kernel 5.17-5.26), FP32
+// W = 16 with 256 rows 5.73-5.77 (128: 5.39-5.40, 512: 5.45; staged
+// 5.46-5.50).  Routing the 8-wide classes here (zero-padded to 16) measured
+// 6-7% slower than staged.
 #define TSM_BWD_MFMA(NAME, S, W, R)                                                   \
   extern "C" __global__ void __launch_bounds__(256) NAME(tsm::BwdArgs a) {          \
     tsm::bwd_mfma_body<S, W, R>(a);                                                   \
