@@ -132,6 +132,21 @@ def test_batched_wide_low_precision_as_column_blocks(dt):
     for A, B, C, C2 in zip(As, Bs, outs, again):
         assert torch.equal(C, C2)
         check(C.cpu().numpy(), A, B, dt)
+    # the plan API and a graph replay of it give the same bits
+    pouts = [torch.empty_like(C) for C in outs]
+    plan = ops.BatchedPlan(As, Bs, pouts)
+    plan.run()
+    torch.cuda.synchronize()
+    assert all(torch.equal(C, P) for C, P in zip(outs, pouts))
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run()
+    for P in pouts:
+        P.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    assert all(torch.equal(C, P) for C, P in zip(outs, pouts))
+    plan.close()
     # views with lda > k, C += A^T B into FP32 outputs
     W = ops.random_matrix(30011, 67, dt, seed=SEED, stream_id=40)
     As2, Bs2 = [W[:, :64], W[:, 2:50]], [W[:, 3:67], W[:, :40]]
