@@ -18,6 +18,14 @@ namespace {
 
 constexpr size_t kAlign = 256;
 constexpr int64_t kItemBytes = 2 << 20;  // target bytes of A + B per work item
+// At most this many items per problem (larger items beyond; the persistent
+// grid has 256 workgroups on MI355X): the last item of a problem adds all of
+// the problem's item partials in one workgroup, so a problem cut into
+// thousands of items spent hundreds of microseconds in that final sum.  A
+// batch holding one 4 GiB FP64 32 x 32 problem (2048 items of 2 MiB, 16 MiB of
+// partials) ran at 3.9 TB/s; with 256 items 6.2 (128 items: 4.4, the grid
+// then idles half the CUs; 512: 5.6).
+constexpr int64_t kMaxItemsPerProblem = 256;
 
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 int elem_bytes(tsm_dtype t) { return t == TSM_F64 ? 8 : t == TSM_F32 ? 4 : 2; }
@@ -101,7 +109,9 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
     const int64_t T = bk->classes[cls].rows;
     const int64_t tiles = (q.n + T - 1) / T;
     const int64_t tile_bytes = T * (kc + mc) * es;
-    const int64_t chunk = std::max<int64_t>(1, kItemBytes / tile_bytes);
+    const int64_t chunk = std::max<int64_t>(
+        std::max<int64_t>(1, kItemBytes / tile_bytes),
+        (tiles + kMaxItemsPerProblem - 1) / kMaxItemsPerProblem);
     const int64_t nitems = std::max<int64_t>(1, (tiles + chunk - 1) / chunk);
     if (nitems > 0x7fffffff) return TSM_ERR_INVALID_ARGUMENT;
     tsm::BatchProblem& b = L->probs[p];
