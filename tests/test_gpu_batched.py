@@ -90,6 +90,30 @@ def test_batched_matches_single_calls_at_the_batched_workload_shape():
         assert rel <= 1e-13, rel
 
 
+def test_batched_large_problem_with_capped_items():
+    """A problem that would be cut into more than 256 items of 2 MiB runs as
+    256 larger items (next to a small problem): the same results as a single
+    call within 1e-13 relative, deterministic."""
+    A = ops.random_matrix((1 << 21) + 77, 32, torch.float64, seed=SEED, stream_id=1)  # 1 GiB + A
+    B = ops.random_matrix((1 << 21) + 77, 32, torch.float64, seed=SEED, stream_id=2)
+    a2 = ops.random_matrix(1001, 8, torch.float64, seed=SEED, stream_id=3)
+    b2 = ops.random_matrix(1001, 4, torch.float64, seed=SEED, stream_id=4)
+    plan_outs = [torch.empty((32, 32), dtype=torch.float64, device="cuda"),
+                 torch.empty((8, 4), dtype=torch.float64, device="cuda")]
+    plan = ops.BatchedPlan([A, a2], [B, b2], plan_outs)
+    assert plan.items <= 256 + 1
+    plan.run()
+    torch.cuda.synchronize()
+    for C, X, Y in zip(plan_outs, (A, a2), (B, b2)):
+        single = ops.atb(X, Y)
+        assert ((C - single).norm() / single.norm()).item() <= 1e-13
+    first = [C.clone() for C in plan_outs]
+    plan.run()
+    torch.cuda.synchronize()
+    assert all(torch.equal(C, F) for C, F in zip(plan_outs, first))
+    plan.close()
+
+
 def test_plan_api_and_graph_replay():
     As, Bs = make_batch(torch.float32, 40, [2, 4, 8, 16, 32], 16, 3)
     outs = [torch.empty((A.shape[1], B.shape[1]), dtype=torch.float64, device="cuda")
