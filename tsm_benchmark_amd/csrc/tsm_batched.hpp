@@ -43,6 +43,36 @@ struct BatchArgs {
   int c_f32;
 };
 
+// The last item's sum of a problem's item partials for classes of R kThreads
+// outputs (R = 2, 4, 8, 16): thread t takes outputs t + r kThreads and loads
+// UG = 64 / R slots of all R at once, so 64 loads are in flight instead of 16
+// (each output is still added in slot order: the bits of finish_outputs).  A
+// separate function so that only the batched kernels carry its registers.
+template <int KCMC>
+__device__ __forceinline__ void finish_outputs_joint(const AtbArgs& a, int G, int MC) {
+  constexpr int R = KCMC / kThreads, UG = 64 / R;
+  static_assert(R * kThreads == KCMC && R >= 2 && R <= 16, "2..16 outputs per thread");
+  const int tid = threadIdx.x;
+  const double* P = a.partials;
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int g = 0; g < G; g += UG) {
+    double v[UG][R];
+#pragma unroll
+    for (int x = 0; x < UG; ++x)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        v[x][r] = g + x < G ? P[(int64_t)(g + x) * KCMC + tid + r * kThreads] : 0.0;
+#pragma unroll
+    for (int x = 0; x < UG; ++x)
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] += v[x][r];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) store_output(a, MC, tid + r * kThreads, acc[r]);
+}
+
 template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
 __device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchItem& it,
                                              unsigned char* smem, int* s_flag) {
@@ -72,7 +102,10 @@ __device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchIte
   atb_partial<S, KC, MC, TM, TN, T, PF, NT>(a, smem, it.tile_begin, 1, count, owns_tail, full,
                                             tail_rows, a.partials + (int64_t)it.slot * KCMC);
   if (arrive_last(a.counter, pr.n_items, s_flag)) {
-    finish_outputs(a, (int)pr.n_items, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
+    if constexpr (KCMC >= 2 * kThreads)
+      finish_outputs_joint<KCMC>(a, (int)pr.n_items, MC);
+    else
+      finish_outputs(a, (int)pr.n_items, KCMC, MC, 0, KCMC, reinterpret_cast<double*>(smem));
     if (threadIdx.x == 0)
       __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
