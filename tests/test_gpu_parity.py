@@ -389,6 +389,38 @@ def test_backward_bf16_matrix_core_against_fp32_path(k, m, n):
         assert same >= 0.995, (name, same)
 
 
+@pytest.mark.parametrize("dt,k,m", [("f64", 8, 8), ("f64", 32, 32), ("f64", 64, 64), ("f32", 16, 48),
+                                    ("bf16", 64, 64)])
+def test_torch_front_end_in_a_captured_graph(dt, k, m):
+    """ops.atb and ops.ata with a caller workspace (ops.workspace) captured in a
+    torch CUDA graph: every replay, with eager calls between them, gives the
+    eager bits (in-kernel final sum, finisher kernel and direct engine)."""
+    n = 300007
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    ws = ops.workspace(n, k, m, DT[dt])
+    ws2 = ops.workspace(n, k, None, DT[dt])
+    C = torch.empty((k, m), dtype=torch.float64, device="cuda")
+    S = torch.empty((k, k), dtype=torch.float64, device="cuda")
+    ops.atb(A, B, out=C, workspace=ws)
+    ops.ata(A, out=S, workspace=ws2)
+    torch.cuda.synchronize()
+    eager_c, eager_s = C.clone(), S.clone()
+    assert torch.equal(eager_c, ops.atb(A, B, out_dtype=torch.float64))  # the pool gives the same bits
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ops.atb(A, B, out=C, workspace=ws)
+        ops.ata(A, out=S, workspace=ws2)
+    for i in range(4):
+        C.zero_()
+        S.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C, eager_c) and torch.equal(S, eager_s), f"replay {i}"
+        ops.atb(A, B, out=C, workspace=ws)  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(C, eager_c)
+
+
 def test_autograd_matches_torch_matmul():
     n, k, m = 30011, 16, 16
     A0, B0 = rand(n, k, torch.float64, 1), rand(n, m, torch.float64, 2)

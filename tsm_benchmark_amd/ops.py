@@ -36,13 +36,43 @@ def _ld(X: torch.Tensor) -> int:
     return X.stride(0) if X.shape[0] > 1 else X.shape[1]
 
 
+def workspace(n: int, k: int, m: int | None = None, dtype: torch.dtype = torch.float64, *,
+              config: _lib.KernelConfig | None = None, device="cuda") -> torch.Tensor:
+    """A zeroed workspace for ``atb`` (``m`` given) or ``ata`` (``m`` None) on
+    inputs of n rows, k (and m) columns and ``dtype``: pass it as
+    ``workspace=`` to run those calls without the library's per-stream pool,
+    for instance inside a captured graph.  Calls that share one must be
+    ordered on one stream; it stays valid for any call of at most these
+    sizes and the same kernel request."""
+    L = _lib.lib()
+    cfg = ct.byref(config) if config is not None else None
+    if m is None:
+        size = L.tsm_ata_workspace_size(_IN_TYPES[dtype], n, k, cfg)
+    else:
+        size = L.tsm_atb_workspace_size(_IN_TYPES[dtype], n, k, m, cfg)
+    if size == 0:
+        raise ValueError("invalid shape or kernel request for a workspace")
+    return torch.zeros(size, dtype=torch.uint8, device=device)
+
+
+def _ws(ws: torch.Tensor | None, device):
+    if ws is None:
+        return None, 0
+    if ws.device != device or not ws.is_contiguous():
+        raise ValueError("workspace must be a contiguous tensor on the inputs' device")
+    return ws.data_ptr(), ws.numel() * ws.element_size()
+
+
 def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = None,
         out: torch.Tensor | None = None, accumulate: bool = False,
-        config: _lib.KernelConfig | None = None) -> torch.Tensor:
+        config: _lib.KernelConfig | None = None,
+        workspace: torch.Tensor | None = None) -> torch.Tensor:
     """C = A^T B (or C += A^T B with ``accumulate`` and ``out``), no autograd.
 
     A: n x k, B: n x m, same dtype (float64, float32 or bfloat16).  C is k x m
-    in ``out_dtype`` (float64 by default for float64 inputs, float32 otherwise)."""
+    in ``out_dtype`` (float64 by default for float64 inputs, float32 otherwise).
+    ``workspace`` (see ``ops.workspace``) replaces the library's per-stream
+    pool, which allocates on a stream's first call."""
     _check_tall(A, "A")
     _check_tall(B, "B")
     if A.dtype != B.dtype or A.shape[0] != B.shape[0] or A.device != B.device:
@@ -58,20 +88,23 @@ def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = Non
     if out.dtype not in _OUT_TYPES or out.shape != (k, m) or out.stride(1) != 1:
         raise ValueError("out must be a row-major k x m float64/float32 tensor")
     L = _lib.lib()
+    ws_ptr, ws_bytes = _ws(workspace, A.device)
     with torch.cuda.device(A.device):
         st = L.tsm_atb_ex(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, m,
                           A.data_ptr(), _ld(A), B.data_ptr(), _ld(B), out.data_ptr(),
-                          out.stride(0) if k > 1 else m, 1 if accumulate else 0, None, 0,
-                          _stream(), ct.byref(config) if config is not None else None)
+                          out.stride(0) if k > 1 else m, 1 if accumulate else 0, ws_ptr,
+                          ws_bytes, _stream(), ct.byref(config) if config is not None else None)
     _lib.check(st, "tsm_atb")
     return out
 
 
 def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
         out: torch.Tensor | None = None, accumulate: bool = False,
-        config: _lib.KernelConfig | None = None) -> torch.Tensor:
+        config: _lib.KernelConfig | None = None,
+        workspace: torch.Tensor | None = None) -> torch.Tensor:
     """C = A^T A (or C += A^T A with ``accumulate`` and ``out``): the Gram matrix
-    of a tall-skinny A, reading A once.  Same bits as ``atb(A, A)``."""
+    of a tall-skinny A, reading A once.  Same bits as ``atb(A, A)``.
+    ``workspace`` as for ``atb``."""
     _check_tall(A, "A")
     n, k = A.shape
     if out is None:
@@ -83,10 +116,11 @@ def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
     if out.dtype not in _OUT_TYPES or out.shape != (k, k) or out.stride(1) != 1:
         raise ValueError("out must be a row-major k x k float64/float32 tensor")
     L = _lib.lib()
+    ws_ptr, ws_bytes = _ws(workspace, A.device)
     with torch.cuda.device(A.device):
         st = L.tsm_ata_ex(_IN_TYPES[A.dtype], _OUT_TYPES[out.dtype], n, k, A.data_ptr(),
                           _ld(A), out.data_ptr(), out.stride(0) if k > 1 else k,
-                          1 if accumulate else 0, None, 0, _stream(),
+                          1 if accumulate else 0, ws_ptr, ws_bytes, _stream(),
                           ct.byref(config) if config is not None else None)
     _lib.check(st, "tsm_ata")
     return out
