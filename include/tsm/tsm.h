@@ -65,7 +65,7 @@ typedef enum { TSM_DIST_UNIFORM = 0, TSM_DIST_POW2 = 1 } tsm_dist;
 typedef struct {
   int tm, tn;          /* register tile: C entries owned per thread = tm x tn */
   int rows;            /* rows per LDS tile (the n-tile streamed per stage) */
-  int pf;              /* tiles in flight per thread: 1 or 2 (the "leap-frog" depth);
+  int pf;              /* tiles in flight per thread: 1 to 4 (the "leap-frog" depth);
                           direct engine: steps in flight per wave, 4, 8 or 16 (FP64/FP32)
                           or 2 or 4 (BF16) */
   int wgs_per_cu;      /* persistent workgroups per compute unit */

@@ -86,7 +86,7 @@ def test_atb_argument_errors(kw, status):
 
 def test_atb_config_errors():
     L = _lib.lib()
-    for bad in (_lib.make_config(pf=3), _lib.make_config(pf=5), _lib.make_config(engine=4)):
+    for bad in (_lib.make_config(pf=5), _lib.make_config(pf=-1), _lib.make_config(engine=4)):
         st = L.tsm_atb_ex(_lib.F64, _lib.F64, 10, 8, 8, FAKE, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
                           ct.byref(bad))
         assert st == _lib.ERR_INVALID_ARGUMENT
@@ -133,7 +133,7 @@ def test_ata_argument_errors():
     assert ata(out_type=_lib.BF16) == _lib.ERR_UNSUPPORTED
     assert L.tsm_ata_workspace_size(_lib.F64, -1, 8, None) == 0
     assert L.tsm_ata_workspace_size(_lib.F64, 10, 65, None) == 0
-    bad = _lib.make_config(pf=3)
+    bad = _lib.make_config(pf=5)
     assert L.tsm_ata_ex(_lib.F64, _lib.F64, 10, 8, FAKE, 8, FAKE, 8, 0, None, 0, None,
                         ct.byref(bad)) == _lib.ERR_INVALID_ARGUMENT
 

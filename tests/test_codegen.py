@@ -29,7 +29,8 @@ def test_config_rules():
     assert KernelConfig("f64", 8, 8, 3, 8, 256, 2).problems()          # tile must divide
     assert KernelConfig("f64", 64, 64, 4, 4, 256, 2).problems()        # 256 tiles > 64
     assert KernelConfig("f64", 8, 8, 8, 8, 100, 2).problems()          # rows vs row lanes
-    assert KernelConfig("f64", 8, 8, 8, 8, 256, 3).problems()          # pf
+    assert KernelConfig("f64", 8, 8, 8, 8, 256, 5).problems()          # pf
+    assert KernelConfig("f64", 8, 8, 4, 4, 128, 4).problems() == []    # four in flight
     assert KernelConfig("f64", 64, 64, 8, 8, 256, 2).problems()        # LDS budget
     assert KernelConfig("bf16", 1, 1, 1, 1, 2, 1).problems()           # 16-byte chunks
 

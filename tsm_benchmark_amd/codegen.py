@@ -184,8 +184,8 @@ class KernelConfig:
             return p
         if self.ntiles > WAVE:
             p.append("more than 64 register tiles")
-        if self.pf not in (1, 2):
-            p.append("pf must be 1 or 2")
+        if self.pf not in (1, 2, 3, 4):
+            p.append("pf must be 1 to 4")
         if self.nt not in (0, 1):
             p.append("nt must be 0 or 1")
         if self.rows < 1 or self.rows % self.row_lanes:
@@ -270,14 +270,24 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
                 base = dataclasses.replace(base, tn=min(mc, 8))
             rg = THREADS // base.ntiles
             base = dataclasses.replace(base, rows=max(rg, min(base.rows, rg * 8)))
-    tiles = {(base.tm, base.tn), (min(kc, 4), min(mc, 4)), (min(kc, 8), min(mc, 8))}
+    tiles = {(base.tm, base.tn), (min(kc, 4), min(mc, 4)), (min(kc, 8), min(mc, 8)),
+             (min(kc, 4), min(mc, 8)), (min(kc, 8), min(mc, 4))}
     for tm, tn in tiles:
         for rows in (base.rows // 2, base.rows, base.rows * 2):
             for pf in (1, 2):
-                for nt in ((0, 1) if (tm, tn) == (base.tm, base.tn) else (0,)):
+                # streaming loads for every tile at two tiles in flight, and
+                # for the default tile at either depth
+                nts = (0, 1) if (tm, tn) == (base.tm, base.tn) or pf == 2 else (0,)
+                for nt in nts:
                     c = KernelConfig(dtype, kc, mc, tm, tn, rows, pf, nt)
                     if not c.problems():
                         out.add(c)
+            # four tiles in flight (streaming loads) for the register tiles of
+            # at most 32 accumulators, whose registers leave room for them
+            if tm * tn <= 32 and rows <= base.rows:
+                c = KernelConfig(dtype, kc, mc, tm, tn, rows, 4, 1)
+                if not c.problems():
+                    out.add(c)
     out.update(direct_variants(dtype, kc, mc))
     return sorted(out)
 

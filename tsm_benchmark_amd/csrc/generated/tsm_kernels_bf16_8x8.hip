@@ -23,6 +23,26 @@ hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p2(const tsm::AtbArgs& a, unsigned 
   return hipGetLastError();
 }
 
+// tsm_atb_bf16_8x8_t4x4_r512_p2_nt: KC=8 MC=8 TM=4 TN=4 rows=512 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 4, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x4_r512_p4_nt: KC=8 MC=8 TM=4 TN=4 rows=512 pf=4 nt=1
+//   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r512_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 4, 512, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x4_r512_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_bf16_8x8_t4x4_r1024_p1: KC=8 MC=8 TM=4 TN=4 rows=1024 pf=1 nt=0
 //   4 register tiles x 64 row lanes, 16 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r1024_p1(tsm::AtbArgs a) {
@@ -40,6 +60,136 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r1024_p2
 }
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x4_r1024_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x4_r1024_p2_nt: KC=8 MC=8 TM=4 TN=4 rows=1024 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 4, 1024, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x4_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x4_r1024_p4_nt: KC=8 MC=8 TM=4 TN=4 rows=1024 pf=4 nt=1
+//   4 register tiles x 64 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x4_r1024_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 4, 1024, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x4_r1024_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r512_p1: KC=8 MC=8 TM=4 TN=8 rows=512 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r512_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 512, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r512_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r512_p2: KC=8 MC=8 TM=4 TN=8 rows=512 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r512_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 512, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r512_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=512 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r512_p4_nt: KC=8 MC=8 TM=4 TN=8 rows=512 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r512_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 512, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r512_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r1024_p1: KC=8 MC=8 TM=4 TN=8 rows=1024 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r1024_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 1024, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r1024_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r1024_p2: KC=8 MC=8 TM=4 TN=8 rows=1024 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r1024_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 1024, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r1024_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r1024_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=1024 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 1024, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r1024_p4_nt: KC=8 MC=8 TM=4 TN=8 rows=1024 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r1024_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 1024, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r1024_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r2048_p1: KC=8 MC=8 TM=4 TN=8 rows=2048 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r2048_p1(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 2048, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r2048_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r2048_p2: KC=8 MC=8 TM=4 TN=8 rows=2048 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r2048_p2(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 2048, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r2048_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t4x8_r2048_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=2048 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t4x8_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 4, 8, 2048, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t4x8_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -83,6 +233,16 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
+// tsm_atb_bf16_8x8_t8x4_r512_p4_nt: KC=8 MC=8 TM=8 TN=4 rows=512 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x4_r512_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 512, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x4_r512_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_bf16_8x8_t8x4_r1024_p1: KC=8 MC=8 TM=8 TN=4 rows=1024 pf=1 nt=0
 //   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x4_r1024_p1(tsm::AtbArgs a) {
@@ -120,6 +280,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x4_r1024_p2
 }
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x4_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t8x4_r1024_p4_nt: KC=8 MC=8 TM=8 TN=4 rows=1024 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x4_r1024_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 4, 1024, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x4_r1024_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -183,6 +353,16 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x8_r512_p2(const tsm::AtbArgs& a, unsigned 
   return hipGetLastError();
 }
 
+// tsm_atb_bf16_8x8_t8x8_r512_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=512 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 8, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_bf16_8x8_t8x8_r1024_p1: KC=8 MC=8 TM=8 TN=8 rows=1024 pf=1 nt=0
 //   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r1024_p1(tsm::AtbArgs a) {
@@ -203,6 +383,16 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
+// tsm_atb_bf16_8x8_t8x8_r1024_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=1024 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 8, 1024, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x8_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_bf16_8x8_t8x8_r2048_p1: KC=8 MC=8 TM=8 TN=8 rows=2048 pf=1 nt=0
 //   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 131072 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r2048_p1(tsm::AtbArgs a) {
@@ -220,6 +410,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r2048_p2
 }
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x8_r2048_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_bf16_8x8_t8x8_r2048_p2_nt: KC=8 MC=8 TM=8 TN=8 rows=2048 pf=2 nt=1
+//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_bf16_8x8_t8x8_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 8, 8, 2048, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_bf16_8x8_t8x8_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -103,6 +103,26 @@ hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
+// tsm_atb_f32_16x16_t4x4_r128_p2_nt: KC=16 MC=16 TM=4 TN=4 rows=128 pf=2 nt=1
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x4_r128_p4_nt: KC=16 MC=16 TM=4 TN=4 rows=128 pf=4 nt=1
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f32_16x16_t4x4_r256_p1: KC=16 MC=16 TM=4 TN=4 rows=256 pf=1 nt=0
 //   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r256_p1(tsm::AtbArgs a) {
@@ -120,6 +140,136 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r256_p2
 }
 hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x4_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x4_r256_p2_nt: KC=16 MC=16 TM=4 TN=4 rows=256 pf=2 nt=1
+//   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 4, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x4_r256_p4_nt: KC=16 MC=16 TM=4 TN=4 rows=256 pf=4 nt=1
+//   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x4_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 4, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x4_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r128_p1: KC=16 MC=16 TM=4 TN=8 rows=128 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r128_p2: KC=16 MC=16 TM=4 TN=8 rows=128 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r128_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=128 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r128_p4_nt: KC=16 MC=16 TM=4 TN=8 rows=128 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r256_p1: KC=16 MC=16 TM=4 TN=8 rows=256 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r256_p2: KC=16 MC=16 TM=4 TN=8 rows=256 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r256_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=256 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r256_p4_nt: KC=16 MC=16 TM=4 TN=8 rows=256 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r512_p1: KC=16 MC=16 TM=4 TN=8 rows=512 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r512_p1(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 512, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r512_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r512_p2: KC=16 MC=16 TM=4 TN=8 rows=512 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r512_p2(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 512, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t4x8_r512_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=512 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t4x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 4, 8, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t4x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -163,6 +313,16 @@ hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p2_nt(const tsm::AtbArgs& a, unsig
   return hipGetLastError();
 }
 
+// tsm_atb_f32_16x16_t8x4_r128_p4_nt: KC=16 MC=16 TM=8 TN=4 rows=128 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 8, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f32_16x16_t8x4_r256_p1: KC=16 MC=16 TM=8 TN=4 rows=256 pf=1 nt=0
 //   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x4_r256_p1(tsm::AtbArgs a) {
@@ -200,6 +360,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x4_r256_p2
 }
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t8x4_r256_p4_nt: KC=16 MC=16 TM=8 TN=4 rows=256 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x4_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 8, 4, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x4_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -263,6 +433,16 @@ hipError_t launch_tsm_atb_f32_16x16_t8x8_r128_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
+// tsm_atb_f32_16x16_t8x8_r128_p2_nt: KC=16 MC=16 TM=8 TN=8 rows=128 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 8, 8, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x8_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f32_16x16_t8x8_r256_p1: KC=16 MC=16 TM=8 TN=8 rows=256 pf=1 nt=0
 //   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r256_p1(tsm::AtbArgs a) {
@@ -283,6 +463,16 @@ hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
+// tsm_atb_f32_16x16_t8x8_r256_p2_nt: KC=16 MC=16 TM=8 TN=8 rows=256 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 8, 8, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f32_16x16_t8x8_r512_p1: KC=16 MC=16 TM=8 TN=8 rows=512 pf=1 nt=0
 //   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 131072 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r512_p1(tsm::AtbArgs a) {
@@ -300,6 +490,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r512_p2
 }
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x8_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f32_16x16_t8x8_r512_p2_nt: KC=16 MC=16 TM=8 TN=8 rows=512 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f32_16x16_t8x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<float, 16, 16, 8, 8, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f32_16x16_t8x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -163,6 +163,26 @@ hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p2(const tsm::AtbArgs& a, unsigned 
   return hipGetLastError();
 }
 
+// tsm_atb_f64_16x16_t4x4_r64_p2_nt: KC=16 MC=16 TM=4 TN=4 rows=64 pf=2 nt=1
+//   16 register tiles x 16 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 4, 64, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x4_r64_p4_nt: KC=16 MC=16 TM=4 TN=4 rows=64 pf=4 nt=1
+//   16 register tiles x 16 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r64_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 4, 64, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r64_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_16x16_t4x4_r128_p1: KC=16 MC=16 TM=4 TN=4 rows=128 pf=1 nt=0
 //   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r128_p1(tsm::AtbArgs a) {
@@ -183,6 +203,26 @@ hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p2(const tsm::AtbArgs& a, unsigned
   return hipGetLastError();
 }
 
+// tsm_atb_f64_16x16_t4x4_r128_p2_nt: KC=16 MC=16 TM=4 TN=4 rows=128 pf=2 nt=1
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x4_r128_p4_nt: KC=16 MC=16 TM=4 TN=4 rows=128 pf=4 nt=1
+//   16 register tiles x 16 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_16x16_t4x4_r256_p1: KC=16 MC=16 TM=4 TN=4 rows=256 pf=1 nt=0
 //   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 131072 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r256_p1(tsm::AtbArgs a) {
@@ -200,6 +240,236 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r256_p2
 }
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x4_r256_p2_nt: KC=16 MC=16 TM=4 TN=4 rows=256 pf=2 nt=1
+//   16 register tiles x 16 row lanes, 16 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 4, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r64_p1: KC=16 MC=16 TM=4 TN=8 rows=64 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r64_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 64, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r64_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r64_p2: KC=16 MC=16 TM=4 TN=8 rows=64 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 64, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r64_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r64_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=64 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 64, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r64_p4_nt: KC=16 MC=16 TM=4 TN=8 rows=64 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r64_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 64, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r64_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r128_p1: KC=16 MC=16 TM=4 TN=8 rows=128 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r128_p2: KC=16 MC=16 TM=4 TN=8 rows=128 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r128_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=128 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r128_p4_nt: KC=16 MC=16 TM=4 TN=8 rows=128 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r256_p1: KC=16 MC=16 TM=4 TN=8 rows=256 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r256_p2: KC=16 MC=16 TM=4 TN=8 rows=256 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t4x8_r256_p2_nt: KC=16 MC=16 TM=4 TN=8 rows=256 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t4x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 4, 8, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t4x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r64_p1: KC=16 MC=16 TM=8 TN=4 rows=64 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r64_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 64, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r64_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r64_p2: KC=16 MC=16 TM=8 TN=4 rows=64 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 64, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r64_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r64_p2_nt: KC=16 MC=16 TM=8 TN=4 rows=64 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 64, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r64_p4_nt: KC=16 MC=16 TM=8 TN=4 rows=64 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r64_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 64, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r64_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r128_p1: KC=16 MC=16 TM=8 TN=4 rows=128 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r128_p2: KC=16 MC=16 TM=8 TN=4 rows=128 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r128_p2_nt: KC=16 MC=16 TM=8 TN=4 rows=128 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r128_p4_nt: KC=16 MC=16 TM=8 TN=4 rows=128 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r256_p1: KC=16 MC=16 TM=8 TN=4 rows=256 pf=1 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r256_p2: KC=16 MC=16 TM=8 TN=4 rows=256 pf=2 nt=0
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_16x16_t8x4_r256_p2_nt: KC=16 MC=16 TM=8 TN=4 rows=256 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_16x16_t8x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 16, 16, 8, 4, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_16x16_t8x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

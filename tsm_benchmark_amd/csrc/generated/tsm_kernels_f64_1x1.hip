@@ -43,6 +43,16 @@ hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
+// tsm_atb_f64_1x1_t1x1_r1024_p4_nt: KC=1 MC=1 TM=1 TN=1 rows=1024 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_1x1_t1x1_r1024_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 1, 1, 1, 1, 1024, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_1x1_t1x1_r1024_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_1x1_t1x1_r2048_p1: KC=1 MC=1 TM=1 TN=1 rows=2048 pf=1 nt=0
 //   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_1x1_t1x1_r2048_p1(tsm::AtbArgs a) {
@@ -80,6 +90,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_1x1_t1x1_r2048_p2_
 }
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_1x1_t1x1_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_1x1_t1x1_r2048_p4_nt: KC=1 MC=1 TM=1 TN=1 rows=2048 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 8 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_1x1_t1x1_r2048_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 1, 1, 1, 1, 2048, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_1x1_t1x1_r2048_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

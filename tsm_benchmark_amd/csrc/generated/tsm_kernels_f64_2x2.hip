@@ -43,6 +43,16 @@ hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p2_nt(const tsm::AtbArgs& a, unsigne
   return hipGetLastError();
 }
 
+// tsm_atb_f64_2x2_t2x2_r512_p4_nt: KC=2 MC=2 TM=2 TN=2 rows=512 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_2x2_t2x2_r512_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 512, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_2x2_t2x2_r512_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_2x2_t2x2_r1024_p1: KC=2 MC=2 TM=2 TN=2 rows=1024 pf=1 nt=0
 //   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_2x2_t2x2_r1024_p1(tsm::AtbArgs a) {
@@ -80,6 +90,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_2x2_t2x2_r1024_p2_
 }
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_2x2_t2x2_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_2x2_t2x2_r1024_p4_nt: KC=2 MC=2 TM=2 TN=2 rows=1024 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_2x2_t2x2_r1024_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 2, 2, 2, 2, 1024, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_2x2_t2x2_r1024_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

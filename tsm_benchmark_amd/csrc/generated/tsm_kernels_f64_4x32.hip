@@ -23,6 +23,26 @@ hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2(const tsm::AtbArgs& a, unsigned g
   return hipGetLastError();
 }
 
+// tsm_atb_f64_4x32_t4x4_r32_p2_nt: KC=4 MC=32 TM=4 TN=4 rows=32 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 1 rows/thread/tile, LDS 18432 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r32_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 4, 32, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r32_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_4x32_t4x4_r32_p4_nt: KC=4 MC=32 TM=4 TN=4 rows=32 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 1 rows/thread/tile, LDS 18432 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r32_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 4, 32, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r32_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_4x32_t4x4_r64_p1: KC=4 MC=32 TM=4 TN=4 rows=64 pf=1 nt=0
 //   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 36864 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r64_p1(tsm::AtbArgs a) {
@@ -43,6 +63,26 @@ hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p2(const tsm::AtbArgs& a, unsigned g
   return hipGetLastError();
 }
 
+// tsm_atb_f64_4x32_t4x4_r64_p2_nt: KC=4 MC=32 TM=4 TN=4 rows=64 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 36864 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r64_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 4, 64, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_4x32_t4x4_r64_p4_nt: KC=4 MC=32 TM=4 TN=4 rows=64 pf=4 nt=1
+//   8 register tiles x 32 row lanes, 2 rows/thread/tile, LDS 36864 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r64_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 4, 64, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r64_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_4x32_t4x4_r128_p1: KC=4 MC=32 TM=4 TN=4 rows=128 pf=1 nt=0
 //   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 73728 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r128_p1(tsm::AtbArgs a) {
@@ -60,6 +100,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r128_p2(
 }
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_4x32_t4x4_r128_p2_nt: KC=4 MC=32 TM=4 TN=4 rows=128 pf=2 nt=1
+//   8 register tiles x 32 row lanes, 4 rows/thread/tile, LDS 73728 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -100,6 +150,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x8_r64_p2_n
 }
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x8_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_4x32_t4x8_r64_p4_nt: KC=4 MC=32 TM=4 TN=8 rows=64 pf=4 nt=1
+//   4 register tiles x 64 row lanes, 1 rows/thread/tile, LDS 36864 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x32_t4x8_r64_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 32, 4, 8, 64, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x32_t4x8_r64_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -63,6 +63,16 @@ hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p2_nt(const tsm::AtbArgs& a, unsigne
   return hipGetLastError();
 }
 
+// tsm_atb_f64_4x4_t4x4_r256_p4_nt: KC=4 MC=4 TM=4 TN=4 rows=256 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x4_t4x4_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x4_t4x4_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_4x4_t4x4_r512_p1: KC=4 MC=4 TM=4 TN=4 rows=512 pf=1 nt=0
 //   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x4_t4x4_r512_p1(tsm::AtbArgs a) {
@@ -100,6 +110,16 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x4_t4x4_r512_p2_n
 }
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_4x4_t4x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_4x4_t4x4_r512_p4_nt: KC=4 MC=4 TM=4 TN=4 rows=512 pf=4 nt=1
+//   1 register tiles x 256 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_4x4_t4x4_r512_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 4, 4, 4, 4, 512, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_4x4_t4x4_r512_p4_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -43,6 +43,26 @@ hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p2(const tsm::AtbArgs& a, unsigned g
   return hipGetLastError();
 }
 
+// tsm_atb_f64_8x8_t4x4_r128_p2_nt: KC=8 MC=8 TM=4 TN=4 rows=128 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x4_r128_p4_nt: KC=8 MC=8 TM=4 TN=4 rows=128 pf=4 nt=1
+//   4 register tiles x 64 row lanes, 2 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_8x8_t4x4_r256_p1: KC=8 MC=8 TM=4 TN=4 rows=256 pf=1 nt=0
 //   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r256_p1(tsm::AtbArgs a) {
@@ -63,6 +83,26 @@ hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p2(const tsm::AtbArgs& a, unsigned g
   return hipGetLastError();
 }
 
+// tsm_atb_f64_8x8_t4x4_r256_p2_nt: KC=8 MC=8 TM=4 TN=4 rows=256 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 4, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x4_r256_p4_nt: KC=8 MC=8 TM=4 TN=4 rows=256 pf=4 nt=1
+//   4 register tiles x 64 row lanes, 4 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 4, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_atb_f64_8x8_t4x4_r512_p1: KC=8 MC=8 TM=4 TN=4 rows=512 pf=1 nt=0
 //   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 131072 B
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r512_p1(tsm::AtbArgs a) {
@@ -80,6 +120,236 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r512_p2(t
 }
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x4_r512_p2_nt: KC=8 MC=8 TM=4 TN=4 rows=512 pf=2 nt=1
+//   4 register tiles x 64 row lanes, 8 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x4_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 4, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r128_p1: KC=8 MC=8 TM=4 TN=8 rows=128 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r128_p2: KC=8 MC=8 TM=4 TN=8 rows=128 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r128_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=128 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r128_p4_nt: KC=8 MC=8 TM=4 TN=8 rows=128 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r256_p1: KC=8 MC=8 TM=4 TN=8 rows=256 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r256_p2: KC=8 MC=8 TM=4 TN=8 rows=256 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r256_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=256 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r256_p4_nt: KC=8 MC=8 TM=4 TN=8 rows=256 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r512_p1: KC=8 MC=8 TM=4 TN=8 rows=512 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r512_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 512, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r512_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r512_p2: KC=8 MC=8 TM=4 TN=8 rows=512 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r512_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 512, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t4x8_r512_p2_nt: KC=8 MC=8 TM=4 TN=8 rows=512 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t4x8_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 4, 8, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t4x8_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r128_p1: KC=8 MC=8 TM=8 TN=4 rows=128 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r128_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 128, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r128_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r128_p2: KC=8 MC=8 TM=8 TN=4 rows=128 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r128_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 128, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r128_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r128_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=128 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r128_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 128, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r128_p4_nt: KC=8 MC=8 TM=8 TN=4 rows=128 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 1 rows/thread/tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r128_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 128, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r128_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r256_p1: KC=8 MC=8 TM=8 TN=4 rows=256 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r256_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 256, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r256_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r256_p2: KC=8 MC=8 TM=8 TN=4 rows=256 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r256_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 256, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r256_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r256_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=256 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r256_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 256, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r256_p4_nt: KC=8 MC=8 TM=8 TN=4 rows=256 pf=4 nt=1
+//   2 register tiles x 128 row lanes, 2 rows/thread/tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r256_p4_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 256, 4, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r256_p4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r512_p1: KC=8 MC=8 TM=8 TN=4 rows=512 pf=1 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r512_p1(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 512, 1, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p1(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r512_p1, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r512_p2: KC=8 MC=8 TM=8 TN=4 rows=512 pf=2 nt=0
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r512_p2(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 512, 2, 0>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r512_p2, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_atb_f64_8x8_t8x4_r512_p2_nt: KC=8 MC=8 TM=8 TN=4 rows=512 pf=2 nt=1
+//   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_atb_f64_8x8_t8x4_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 8, 8, 8, 4, 512, 2, 1>(a);
+}
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_atb_f64_8x8_t8x4_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

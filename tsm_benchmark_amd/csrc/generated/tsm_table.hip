@@ -29,26 +29,46 @@ hipError_t launch_tsm_atb_bf16_8x2_t8x2_r1024_p2_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_bf16_8x4_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x4_r1024_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r1024_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t4x8_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t8x4_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t8x4_r1024_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x32_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -72,26 +92,46 @@ hipError_t launch_tsm_atb_bf16_16x16_mfma_r1024_p2(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_bf16_16x16_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t4x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t8x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t8x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x16_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x16_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -118,26 +158,46 @@ hipError_t launch_tsm_atb_bf16_32x32_mfma_r512_p2(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_bf16_32x32_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t8x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -196,28 +256,49 @@ hipError_t launch_tsm_atb_f32_8x2_t8x2_r512_p2_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_f32_8x4_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x4_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t4x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t8x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t8x4_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_8x8_t8x8_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x16_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -238,26 +319,46 @@ hipError_t launch_tsm_atb_f32_16x16_mfma_r256_p2(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t4x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t8x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t8x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x16_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -285,26 +386,46 @@ hipError_t launch_tsm_atb_f32_32x32_mfma_r256_p2(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t8x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t8x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t8x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -346,10 +467,12 @@ hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x1_t1x1_r1024_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_1x1_t1x1_r2048_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_1x1_t1x1_r4096_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -365,10 +488,12 @@ hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p1(const tsm::AtbArgs&, unsigned, hi
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x2_t2x2_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x2_t2x2_r1024_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -386,10 +511,12 @@ hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hi
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x4_t4x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x4_t4x4_r512_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -398,14 +525,20 @@ hipError_t launch_tsm_atb_f64_4x8_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_t4x8_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -418,10 +551,37 @@ hipError_t launch_tsm_atb_f64_8x8_t2x2_r256_p1(const tsm::AtbArgs&, unsigned, hi
 hipError_t launch_tsm_atb_f64_8x8_t2x2_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t4x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t4x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r256_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x8_t8x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -456,10 +616,37 @@ hipError_t launch_tsm_atb_f64_16x16_t2x2_r64_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_16x16_t2x2_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t4x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r128_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x16_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x16_t8x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -480,14 +667,20 @@ hipError_t launch_tsm_atb_f64_32x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_t8x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -512,8 +705,34 @@ hipError_t launch_tsm_atb_f64_32x32_mfma_r128_p2(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t4x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t4x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r32_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r64_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r64_p4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r128_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x32_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x32_t8x8_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -680,26 +899,46 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_8x4_t8x4_r1024_p2_nt", 2, 8, 4, 8, 4, 1024, 2, 1, 49152, launch_tsm_atb_bf16_8x4_t8x4_r1024_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t4x4_r512_p1", 2, 8, 8, 4, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p1, 1},
   {"tsm_atb_bf16_8x8_t4x4_r512_p2", 2, 8, 8, 4, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r512_p2_nt", 2, 8, 8, 4, 4, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r512_p4_nt", 2, 8, 8, 4, 4, 512, 4, 1, 32768, launch_tsm_atb_bf16_8x8_t4x4_r512_p4_nt, 1},
   {"tsm_atb_bf16_8x8_t4x4_r1024_p1", 2, 8, 8, 4, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p1, 1},
   {"tsm_atb_bf16_8x8_t4x4_r1024_p2", 2, 8, 8, 4, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r1024_p2_nt", 2, 8, 8, 4, 4, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x4_r1024_p4_nt", 2, 8, 8, 4, 4, 1024, 4, 1, 65536, launch_tsm_atb_bf16_8x8_t4x4_r1024_p4_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r512_p1", 2, 8, 8, 4, 8, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t4x8_r512_p1, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r512_p2", 2, 8, 8, 4, 8, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t4x8_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r512_p2_nt", 2, 8, 8, 4, 8, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t4x8_r512_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r512_p4_nt", 2, 8, 8, 4, 8, 512, 4, 1, 32768, launch_tsm_atb_bf16_8x8_t4x8_r512_p4_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r1024_p1", 2, 8, 8, 4, 8, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t4x8_r1024_p1, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r1024_p2", 2, 8, 8, 4, 8, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t4x8_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r1024_p2_nt", 2, 8, 8, 4, 8, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t4x8_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r1024_p4_nt", 2, 8, 8, 4, 8, 1024, 4, 1, 65536, launch_tsm_atb_bf16_8x8_t4x8_r1024_p4_nt, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r2048_p1", 2, 8, 8, 4, 8, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t4x8_r2048_p1, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r2048_p2", 2, 8, 8, 4, 8, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t4x8_r2048_p2, 1},
+  {"tsm_atb_bf16_8x8_t4x8_r2048_p2_nt", 2, 8, 8, 4, 8, 2048, 2, 1, 131072, launch_tsm_atb_bf16_8x8_t4x8_r2048_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r512_p1", 2, 8, 8, 8, 4, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1, 1},
   {"tsm_atb_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r512_p2", 2, 8, 8, 8, 4, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2, 1},
   {"tsm_atb_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r512_p4_nt", 2, 8, 8, 8, 4, 512, 4, 1, 32768, launch_tsm_atb_bf16_8x8_t8x4_r512_p4_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r1024_p1", 2, 8, 8, 8, 4, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1, 1},
   {"tsm_atb_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p1_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r1024_p2", 2, 8, 8, 8, 4, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2, 1},
   {"tsm_atb_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_8x8_t8x4_r1024_p4_nt", 2, 8, 8, 8, 4, 1024, 4, 1, 65536, launch_tsm_atb_bf16_8x8_t8x4_r1024_p4_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r2048_p1", 2, 8, 8, 8, 4, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1, 1},
   {"tsm_atb_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p1_nt, 1},
   {"tsm_atb_bf16_8x8_t8x4_r2048_p2", 2, 8, 8, 8, 4, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2, 1},
   {"tsm_atb_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_atb_bf16_8x8_t8x4_r2048_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t8x8_r512_p1", 2, 8, 8, 8, 8, 512, 1, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p1, 1},
   {"tsm_atb_bf16_8x8_t8x8_r512_p2", 2, 8, 8, 8, 8, 512, 2, 0, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r512_p2_nt", 2, 8, 8, 8, 8, 512, 2, 1, 32768, launch_tsm_atb_bf16_8x8_t8x8_r512_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t8x8_r1024_p1", 2, 8, 8, 8, 8, 1024, 1, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p1, 1},
   {"tsm_atb_bf16_8x8_t8x8_r1024_p2", 2, 8, 8, 8, 8, 1024, 2, 0, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r1024_p2_nt", 2, 8, 8, 8, 8, 1024, 2, 1, 65536, launch_tsm_atb_bf16_8x8_t8x8_r1024_p2_nt, 1},
   {"tsm_atb_bf16_8x8_t8x8_r2048_p1", 2, 8, 8, 8, 8, 2048, 1, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p1, 1},
   {"tsm_atb_bf16_8x8_t8x8_r2048_p2", 2, 8, 8, 8, 8, 2048, 2, 0, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p2, 1},
+  {"tsm_atb_bf16_8x8_t8x8_r2048_p2_nt", 2, 8, 8, 8, 8, 2048, 2, 1, 131072, launch_tsm_atb_bf16_8x8_t8x8_r2048_p2_nt, 1},
   {"tsm_atb_bf16_8x16_t8x4_r512_p2_nt", 2, 8, 16, 8, 4, 512, 2, 1, 49152, launch_tsm_atb_bf16_8x16_t8x4_r512_p2_nt, 1},
   {"tsm_atb_bf16_8x32_t8x4_r256_p2_nt", 2, 8, 32, 8, 4, 256, 2, 1, 40960, launch_tsm_atb_bf16_8x32_t8x4_r256_p2_nt, 1},
   {"tsm_atb_bf16_8x64_t8x4_r128_p2_nt", 2, 8, 64, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt, 1},
@@ -723,26 +962,46 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 131072, launch_tsm_atb_bf16_16x16_mfma_r1024_p2_nt, 2},
   {"tsm_atb_bf16_16x16_t4x4_r128_p1", 2, 16, 16, 4, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p1, 1},
   {"tsm_atb_bf16_16x16_t4x4_r128_p2", 2, 16, 16, 4, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r128_p2_nt", 2, 16, 16, 4, 4, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r128_p4_nt", 2, 16, 16, 4, 4, 128, 4, 1, 16384, launch_tsm_atb_bf16_16x16_t4x4_r128_p4_nt, 1},
   {"tsm_atb_bf16_16x16_t4x4_r256_p1", 2, 16, 16, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p1, 1},
   {"tsm_atb_bf16_16x16_t4x4_r256_p2", 2, 16, 16, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r256_p2_nt", 2, 16, 16, 4, 4, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x4_r256_p4_nt", 2, 16, 16, 4, 4, 256, 4, 1, 32768, launch_tsm_atb_bf16_16x16_t4x4_r256_p4_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r128_p1", 2, 16, 16, 4, 8, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t4x8_r128_p1, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r128_p2", 2, 16, 16, 4, 8, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t4x8_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r128_p2_nt", 2, 16, 16, 4, 8, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r128_p4_nt", 2, 16, 16, 4, 8, 128, 4, 1, 16384, launch_tsm_atb_bf16_16x16_t4x8_r128_p4_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r256_p1", 2, 16, 16, 4, 8, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t4x8_r256_p1, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r256_p2", 2, 16, 16, 4, 8, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t4x8_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r256_p2_nt", 2, 16, 16, 4, 8, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r256_p4_nt", 2, 16, 16, 4, 8, 256, 4, 1, 32768, launch_tsm_atb_bf16_16x16_t4x8_r256_p4_nt, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r512_p1", 2, 16, 16, 4, 8, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t4x8_r512_p1, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r512_p2", 2, 16, 16, 4, 8, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t4x8_r512_p2, 1},
+  {"tsm_atb_bf16_16x16_t4x8_r512_p2_nt", 2, 16, 16, 4, 8, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_t4x8_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r128_p1", 2, 16, 16, 8, 4, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1, 1},
   {"tsm_atb_bf16_16x16_t8x4_r128_p1_nt", 2, 16, 16, 8, 4, 128, 1, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p1_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r128_p2", 2, 16, 16, 8, 4, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2, 1},
   {"tsm_atb_bf16_16x16_t8x4_r128_p2_nt", 2, 16, 16, 8, 4, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r128_p4_nt", 2, 16, 16, 8, 4, 128, 4, 1, 16384, launch_tsm_atb_bf16_16x16_t8x4_r128_p4_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r256_p1", 2, 16, 16, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1, 1},
   {"tsm_atb_bf16_16x16_t8x4_r256_p1_nt", 2, 16, 16, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p1_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r256_p2", 2, 16, 16, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2, 1},
   {"tsm_atb_bf16_16x16_t8x4_r256_p2_nt", 2, 16, 16, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_16x16_t8x4_r256_p4_nt", 2, 16, 16, 8, 4, 256, 4, 1, 32768, launch_tsm_atb_bf16_16x16_t8x4_r256_p4_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r512_p1", 2, 16, 16, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1, 1},
   {"tsm_atb_bf16_16x16_t8x4_r512_p1_nt", 2, 16, 16, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p1_nt, 1},
   {"tsm_atb_bf16_16x16_t8x4_r512_p2", 2, 16, 16, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2, 1},
   {"tsm_atb_bf16_16x16_t8x4_r512_p2_nt", 2, 16, 16, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_t8x4_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x16_t8x8_r128_p1", 2, 16, 16, 8, 8, 128, 1, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p1, 1},
   {"tsm_atb_bf16_16x16_t8x8_r128_p2", 2, 16, 16, 8, 8, 128, 2, 0, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r128_p2_nt", 2, 16, 16, 8, 8, 128, 2, 1, 16384, launch_tsm_atb_bf16_16x16_t8x8_r128_p2_nt, 1},
   {"tsm_atb_bf16_16x16_t8x8_r256_p1", 2, 16, 16, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p1, 1},
   {"tsm_atb_bf16_16x16_t8x8_r256_p2", 2, 16, 16, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r256_p2_nt", 2, 16, 16, 8, 8, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x16_t8x8_r256_p2_nt, 1},
   {"tsm_atb_bf16_16x16_t8x8_r512_p1", 2, 16, 16, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p1, 1},
   {"tsm_atb_bf16_16x16_t8x8_r512_p2", 2, 16, 16, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p2, 1},
+  {"tsm_atb_bf16_16x16_t8x8_r512_p2_nt", 2, 16, 16, 8, 8, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x16_t8x8_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x32_direct_d4_nt", 2, 16, 32, 0, 0, 32, 4, 1, 24576, launch_tsm_atb_bf16_16x32_direct_d4_nt, 3},
   {"tsm_atb_bf16_16x32_mfma_r256_p2_nt", 2, 16, 32, 0, 0, 256, 2, 1, 49152, launch_tsm_atb_bf16_16x32_mfma_r256_p2_nt, 2},
   {"tsm_atb_bf16_16x64_direct_d4_nt", 2, 16, 64, 0, 0, 32, 4, 1, 40960, launch_tsm_atb_bf16_16x64_direct_d4_nt, 3},
@@ -769,26 +1028,46 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_32x32_mfma_r512_p2_nt", 2, 32, 32, 0, 0, 512, 2, 1, 131072, launch_tsm_atb_bf16_32x32_mfma_r512_p2_nt, 2},
   {"tsm_atb_bf16_32x32_t4x4_r32_p1", 2, 32, 32, 4, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p1, 1},
   {"tsm_atb_bf16_32x32_t4x4_r32_p2", 2, 32, 32, 4, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r32_p2_nt", 2, 32, 32, 4, 4, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r32_p4_nt", 2, 32, 32, 4, 4, 32, 4, 1, 8192, launch_tsm_atb_bf16_32x32_t4x4_r32_p4_nt, 1},
   {"tsm_atb_bf16_32x32_t4x4_r64_p1", 2, 32, 32, 4, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p1, 1},
   {"tsm_atb_bf16_32x32_t4x4_r64_p2", 2, 32, 32, 4, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r64_p2_nt", 2, 32, 32, 4, 4, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x4_r64_p4_nt", 2, 32, 32, 4, 4, 64, 4, 1, 16384, launch_tsm_atb_bf16_32x32_t4x4_r64_p4_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r32_p1", 2, 32, 32, 4, 8, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t4x8_r32_p1, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r32_p2", 2, 32, 32, 4, 8, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t4x8_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r32_p2_nt", 2, 32, 32, 4, 8, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t4x8_r32_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r32_p4_nt", 2, 32, 32, 4, 8, 32, 4, 1, 8192, launch_tsm_atb_bf16_32x32_t4x8_r32_p4_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r64_p1", 2, 32, 32, 4, 8, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t4x8_r64_p1, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r64_p2", 2, 32, 32, 4, 8, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t4x8_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r64_p2_nt", 2, 32, 32, 4, 8, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r64_p4_nt", 2, 32, 32, 4, 8, 64, 4, 1, 16384, launch_tsm_atb_bf16_32x32_t4x8_r64_p4_nt, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r128_p1", 2, 32, 32, 4, 8, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t4x8_r128_p1, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r128_p2", 2, 32, 32, 4, 8, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t4x8_r128_p2, 1},
+  {"tsm_atb_bf16_32x32_t4x8_r128_p2_nt", 2, 32, 32, 4, 8, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_t4x8_r128_p2_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r32_p1", 2, 32, 32, 8, 4, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1, 1},
   {"tsm_atb_bf16_32x32_t8x4_r32_p1_nt", 2, 32, 32, 8, 4, 32, 1, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p1_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r32_p2", 2, 32, 32, 8, 4, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2, 1},
   {"tsm_atb_bf16_32x32_t8x4_r32_p2_nt", 2, 32, 32, 8, 4, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r32_p4_nt", 2, 32, 32, 8, 4, 32, 4, 1, 8192, launch_tsm_atb_bf16_32x32_t8x4_r32_p4_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r64_p1", 2, 32, 32, 8, 4, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1, 1},
   {"tsm_atb_bf16_32x32_t8x4_r64_p1_nt", 2, 32, 32, 8, 4, 64, 1, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p1_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r64_p2", 2, 32, 32, 8, 4, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2, 1},
   {"tsm_atb_bf16_32x32_t8x4_r64_p2_nt", 2, 32, 32, 8, 4, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_bf16_32x32_t8x4_r64_p4_nt", 2, 32, 32, 8, 4, 64, 4, 1, 16384, launch_tsm_atb_bf16_32x32_t8x4_r64_p4_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r128_p1", 2, 32, 32, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1, 1},
   {"tsm_atb_bf16_32x32_t8x4_r128_p1_nt", 2, 32, 32, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p1_nt, 1},
   {"tsm_atb_bf16_32x32_t8x4_r128_p2", 2, 32, 32, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2, 1},
   {"tsm_atb_bf16_32x32_t8x4_r128_p2_nt", 2, 32, 32, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_bf16_32x32_t8x8_r32_p1", 2, 32, 32, 8, 8, 32, 1, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p1, 1},
   {"tsm_atb_bf16_32x32_t8x8_r32_p2", 2, 32, 32, 8, 8, 32, 2, 0, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r32_p2_nt", 2, 32, 32, 8, 8, 32, 2, 1, 8192, launch_tsm_atb_bf16_32x32_t8x8_r32_p2_nt, 1},
   {"tsm_atb_bf16_32x32_t8x8_r64_p1", 2, 32, 32, 8, 8, 64, 1, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p1, 1},
   {"tsm_atb_bf16_32x32_t8x8_r64_p2", 2, 32, 32, 8, 8, 64, 2, 0, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r64_p2_nt", 2, 32, 32, 8, 8, 64, 2, 1, 16384, launch_tsm_atb_bf16_32x32_t8x8_r64_p2_nt, 1},
   {"tsm_atb_bf16_32x32_t8x8_r128_p1", 2, 32, 32, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p1, 1},
   {"tsm_atb_bf16_32x32_t8x8_r128_p2", 2, 32, 32, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p2, 1},
+  {"tsm_atb_bf16_32x32_t8x8_r128_p2_nt", 2, 32, 32, 8, 8, 128, 2, 1, 32768, launch_tsm_atb_bf16_32x32_t8x8_r128_p2_nt, 1},
   {"tsm_atb_bf16_32x64_direct_d4_nt", 2, 32, 64, 0, 0, 32, 4, 1, 49152, launch_tsm_atb_bf16_32x64_direct_d4_nt, 3},
   {"tsm_atb_bf16_32x64_mfma_r128_p2_nt", 2, 32, 64, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt, 2},
   {"tsm_atb_bf16_64x1_t8x1_r128_p2_nt", 2, 64, 1, 8, 1, 128, 2, 1, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt, 1},
@@ -847,28 +1126,49 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_8x4_t8x4_r512_p2_nt", 1, 8, 4, 8, 4, 512, 2, 1, 49152, launch_tsm_atb_f32_8x4_t8x4_r512_p2_nt, 1},
   {"tsm_atb_f32_8x8_t4x4_r256_p1", 1, 8, 8, 4, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p1, 1},
   {"tsm_atb_f32_8x8_t4x4_r256_p2", 1, 8, 8, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r256_p2_nt", 1, 8, 8, 4, 4, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t4x4_r256_p4_nt", 1, 8, 8, 4, 4, 256, 4, 1, 32768, launch_tsm_atb_f32_8x8_t4x4_r256_p4_nt, 1},
   {"tsm_atb_f32_8x8_t4x4_r512_p1", 1, 8, 8, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p1, 1},
   {"tsm_atb_f32_8x8_t4x4_r512_p2", 1, 8, 8, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r512_p2_nt", 1, 8, 8, 4, 4, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t4x4_r512_p4_nt", 1, 8, 8, 4, 4, 512, 4, 1, 65536, launch_tsm_atb_f32_8x8_t4x4_r512_p4_nt, 1},
   {"tsm_atb_f32_8x8_t4x4_r1024_p1", 1, 8, 8, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p1, 1},
   {"tsm_atb_f32_8x8_t4x4_r1024_p2", 1, 8, 8, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p2, 1},
+  {"tsm_atb_f32_8x8_t4x4_r1024_p2_nt", 1, 8, 8, 4, 4, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t4x8_r256_p1", 1, 8, 8, 4, 8, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t4x8_r256_p1, 1},
+  {"tsm_atb_f32_8x8_t4x8_r256_p2", 1, 8, 8, 4, 8, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t4x8_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t4x8_r256_p2_nt", 1, 8, 8, 4, 8, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t4x8_r256_p4_nt", 1, 8, 8, 4, 8, 256, 4, 1, 32768, launch_tsm_atb_f32_8x8_t4x8_r256_p4_nt, 1},
+  {"tsm_atb_f32_8x8_t4x8_r512_p1", 1, 8, 8, 4, 8, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t4x8_r512_p1, 1},
+  {"tsm_atb_f32_8x8_t4x8_r512_p2", 1, 8, 8, 4, 8, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t4x8_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t4x8_r512_p2_nt", 1, 8, 8, 4, 8, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t4x8_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t4x8_r512_p4_nt", 1, 8, 8, 4, 8, 512, 4, 1, 65536, launch_tsm_atb_f32_8x8_t4x8_r512_p4_nt, 1},
+  {"tsm_atb_f32_8x8_t4x8_r1024_p1", 1, 8, 8, 4, 8, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t4x8_r1024_p1, 1},
+  {"tsm_atb_f32_8x8_t4x8_r1024_p2", 1, 8, 8, 4, 8, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t4x8_r1024_p2, 1},
+  {"tsm_atb_f32_8x8_t4x8_r1024_p2_nt", 1, 8, 8, 4, 8, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t4x8_r1024_p2_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r256_p1", 1, 8, 8, 8, 4, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1, 1},
   {"tsm_atb_f32_8x8_t8x4_r256_p1_nt", 1, 8, 8, 8, 4, 256, 1, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p1_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r256_p2", 1, 8, 8, 8, 4, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2, 1},
   {"tsm_atb_f32_8x8_t8x4_r256_p2_nt", 1, 8, 8, 8, 4, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r256_p4_nt", 1, 8, 8, 8, 4, 256, 4, 1, 32768, launch_tsm_atb_f32_8x8_t8x4_r256_p4_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r512_p1", 1, 8, 8, 8, 4, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1, 1},
   {"tsm_atb_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r512_p2", 1, 8, 8, 8, 4, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2, 1},
   {"tsm_atb_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_8x8_t8x4_r512_p4_nt", 1, 8, 8, 8, 4, 512, 4, 1, 65536, launch_tsm_atb_f32_8x8_t8x4_r512_p4_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r1024_p1", 1, 8, 8, 8, 4, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1, 1},
   {"tsm_atb_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p1_nt, 1},
   {"tsm_atb_f32_8x8_t8x4_r1024_p2", 1, 8, 8, 8, 4, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2, 1},
   {"tsm_atb_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t8x4_r1024_p2_nt, 1},
   {"tsm_atb_f32_8x8_t8x8_r256_p1", 1, 8, 8, 8, 8, 256, 1, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p1, 1},
   {"tsm_atb_f32_8x8_t8x8_r256_p2", 1, 8, 8, 8, 8, 256, 2, 0, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p2, 1},
+  {"tsm_atb_f32_8x8_t8x8_r256_p2_nt", 1, 8, 8, 8, 8, 256, 2, 1, 32768, launch_tsm_atb_f32_8x8_t8x8_r256_p2_nt, 1},
   {"tsm_atb_f32_8x8_t8x8_r512_p1", 1, 8, 8, 8, 8, 512, 1, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p1, 1},
   {"tsm_atb_f32_8x8_t8x8_r512_p2", 1, 8, 8, 8, 8, 512, 2, 0, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p2, 1},
+  {"tsm_atb_f32_8x8_t8x8_r512_p2_nt", 1, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_atb_f32_8x8_t8x8_r512_p2_nt, 1},
   {"tsm_atb_f32_8x8_t8x8_r1024_p1", 1, 8, 8, 8, 8, 1024, 1, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p1, 1},
   {"tsm_atb_f32_8x8_t8x8_r1024_p2", 1, 8, 8, 8, 8, 1024, 2, 0, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p2, 1},
+  {"tsm_atb_f32_8x8_t8x8_r1024_p2_nt", 1, 8, 8, 8, 8, 1024, 2, 1, 131072, launch_tsm_atb_f32_8x8_t8x8_r1024_p2_nt, 1},
   {"tsm_atb_f32_8x16_t8x4_r256_p2_nt", 1, 8, 16, 8, 4, 256, 2, 1, 49152, launch_tsm_atb_f32_8x16_t8x4_r256_p2_nt, 1},
   {"tsm_atb_f32_8x32_t8x4_r128_p2_nt", 1, 8, 32, 8, 4, 128, 2, 1, 40960, launch_tsm_atb_f32_8x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_8x64_t8x4_r64_p2_nt", 1, 8, 64, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt, 1},
@@ -889,26 +1189,46 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_16x16_t4x4_r128_p1", 1, 16, 16, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p1, 1},
   {"tsm_atb_f32_16x16_t4x4_r128_p2", 1, 16, 16, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t4x4_r128_p2_nt", 1, 16, 16, 4, 4, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t4x4_r128_p4_nt", 1, 16, 16, 4, 4, 128, 4, 1, 32768, launch_tsm_atb_f32_16x16_t4x4_r128_p4_nt, 1},
   {"tsm_atb_f32_16x16_t4x4_r256_p1", 1, 16, 16, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p1, 1},
   {"tsm_atb_f32_16x16_t4x4_r256_p2", 1, 16, 16, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t4x4_r256_p2_nt", 1, 16, 16, 4, 4, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t4x4_r256_p4_nt", 1, 16, 16, 4, 4, 256, 4, 1, 65536, launch_tsm_atb_f32_16x16_t4x4_r256_p4_nt, 1},
+  {"tsm_atb_f32_16x16_t4x8_r128_p1", 1, 16, 16, 4, 8, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t4x8_r128_p1, 1},
+  {"tsm_atb_f32_16x16_t4x8_r128_p2", 1, 16, 16, 4, 8, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t4x8_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t4x8_r128_p2_nt", 1, 16, 16, 4, 8, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t4x8_r128_p4_nt", 1, 16, 16, 4, 8, 128, 4, 1, 32768, launch_tsm_atb_f32_16x16_t4x8_r128_p4_nt, 1},
+  {"tsm_atb_f32_16x16_t4x8_r256_p1", 1, 16, 16, 4, 8, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t4x8_r256_p1, 1},
+  {"tsm_atb_f32_16x16_t4x8_r256_p2", 1, 16, 16, 4, 8, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t4x8_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t4x8_r256_p2_nt", 1, 16, 16, 4, 8, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t4x8_r256_p4_nt", 1, 16, 16, 4, 8, 256, 4, 1, 65536, launch_tsm_atb_f32_16x16_t4x8_r256_p4_nt, 1},
+  {"tsm_atb_f32_16x16_t4x8_r512_p1", 1, 16, 16, 4, 8, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t4x8_r512_p1, 1},
+  {"tsm_atb_f32_16x16_t4x8_r512_p2", 1, 16, 16, 4, 8, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t4x8_r512_p2, 1},
+  {"tsm_atb_f32_16x16_t4x8_r512_p2_nt", 1, 16, 16, 4, 8, 512, 2, 1, 131072, launch_tsm_atb_f32_16x16_t4x8_r512_p2_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r128_p1", 1, 16, 16, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1, 1},
   {"tsm_atb_f32_16x16_t8x4_r128_p1_nt", 1, 16, 16, 8, 4, 128, 1, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p1_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r128_p2", 1, 16, 16, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2, 1},
   {"tsm_atb_f32_16x16_t8x4_r128_p2_nt", 1, 16, 16, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r128_p4_nt", 1, 16, 16, 8, 4, 128, 4, 1, 32768, launch_tsm_atb_f32_16x16_t8x4_r128_p4_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r256_p1", 1, 16, 16, 8, 4, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1, 1},
   {"tsm_atb_f32_16x16_t8x4_r256_p1_nt", 1, 16, 16, 8, 4, 256, 1, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p1_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r256_p2", 1, 16, 16, 8, 4, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2, 1},
   {"tsm_atb_f32_16x16_t8x4_r256_p2_nt", 1, 16, 16, 8, 4, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x16_t8x4_r256_p4_nt", 1, 16, 16, 8, 4, 256, 4, 1, 65536, launch_tsm_atb_f32_16x16_t8x4_r256_p4_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r512_p1", 1, 16, 16, 8, 4, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1, 1},
   {"tsm_atb_f32_16x16_t8x4_r512_p1_nt", 1, 16, 16, 8, 4, 512, 1, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p1_nt, 1},
   {"tsm_atb_f32_16x16_t8x4_r512_p2", 1, 16, 16, 8, 4, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2, 1},
   {"tsm_atb_f32_16x16_t8x4_r512_p2_nt", 1, 16, 16, 8, 4, 512, 2, 1, 131072, launch_tsm_atb_f32_16x16_t8x4_r512_p2_nt, 1},
   {"tsm_atb_f32_16x16_t8x8_r128_p1", 1, 16, 16, 8, 8, 128, 1, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p1, 1},
   {"tsm_atb_f32_16x16_t8x8_r128_p2", 1, 16, 16, 8, 8, 128, 2, 0, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p2, 1},
+  {"tsm_atb_f32_16x16_t8x8_r128_p2_nt", 1, 16, 16, 8, 8, 128, 2, 1, 32768, launch_tsm_atb_f32_16x16_t8x8_r128_p2_nt, 1},
   {"tsm_atb_f32_16x16_t8x8_r256_p1", 1, 16, 16, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p1, 1},
   {"tsm_atb_f32_16x16_t8x8_r256_p2", 1, 16, 16, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p2, 1},
+  {"tsm_atb_f32_16x16_t8x8_r256_p2_nt", 1, 16, 16, 8, 8, 256, 2, 1, 65536, launch_tsm_atb_f32_16x16_t8x8_r256_p2_nt, 1},
   {"tsm_atb_f32_16x16_t8x8_r512_p1", 1, 16, 16, 8, 8, 512, 1, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p1, 1},
   {"tsm_atb_f32_16x16_t8x8_r512_p2", 1, 16, 16, 8, 8, 512, 2, 0, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p2, 1},
+  {"tsm_atb_f32_16x16_t8x8_r512_p2_nt", 1, 16, 16, 8, 8, 512, 2, 1, 131072, launch_tsm_atb_f32_16x16_t8x8_r512_p2_nt, 1},
   {"tsm_atb_f32_16x32_direct_d8_nt", 1, 16, 32, 0, 0, 4, 8, 1, 8192, launch_tsm_atb_f32_16x32_direct_d8_nt, 3},
   {"tsm_atb_f32_16x32_t8x4_r128_p2_nt", 1, 16, 32, 8, 4, 128, 2, 1, 49152, launch_tsm_atb_f32_16x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_16x64_direct_d8_nt", 1, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f32_16x64_direct_d8_nt, 3},
@@ -936,26 +1256,46 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_32x32_mfma_r256_p2_nt", 1, 32, 32, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f32_32x32_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_32x32_t4x4_r32_p1", 1, 32, 32, 4, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p1, 1},
   {"tsm_atb_f32_32x32_t4x4_r32_p2", 1, 32, 32, 4, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t4x4_r32_p2_nt", 1, 32, 32, 4, 4, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t4x4_r32_p4_nt", 1, 32, 32, 4, 4, 32, 4, 1, 16384, launch_tsm_atb_f32_32x32_t4x4_r32_p4_nt, 1},
   {"tsm_atb_f32_32x32_t4x4_r64_p1", 1, 32, 32, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p1, 1},
   {"tsm_atb_f32_32x32_t4x4_r64_p2", 1, 32, 32, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t4x4_r64_p2_nt", 1, 32, 32, 4, 4, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t4x4_r64_p4_nt", 1, 32, 32, 4, 4, 64, 4, 1, 32768, launch_tsm_atb_f32_32x32_t4x4_r64_p4_nt, 1},
+  {"tsm_atb_f32_32x32_t4x8_r32_p1", 1, 32, 32, 4, 8, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t4x8_r32_p1, 1},
+  {"tsm_atb_f32_32x32_t4x8_r32_p2", 1, 32, 32, 4, 8, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t4x8_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t4x8_r32_p2_nt", 1, 32, 32, 4, 8, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t4x8_r32_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t4x8_r32_p4_nt", 1, 32, 32, 4, 8, 32, 4, 1, 16384, launch_tsm_atb_f32_32x32_t4x8_r32_p4_nt, 1},
+  {"tsm_atb_f32_32x32_t4x8_r64_p1", 1, 32, 32, 4, 8, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t4x8_r64_p1, 1},
+  {"tsm_atb_f32_32x32_t4x8_r64_p2", 1, 32, 32, 4, 8, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t4x8_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t4x8_r64_p2_nt", 1, 32, 32, 4, 8, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t4x8_r64_p4_nt", 1, 32, 32, 4, 8, 64, 4, 1, 32768, launch_tsm_atb_f32_32x32_t4x8_r64_p4_nt, 1},
+  {"tsm_atb_f32_32x32_t4x8_r128_p1", 1, 32, 32, 4, 8, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t4x8_r128_p1, 1},
+  {"tsm_atb_f32_32x32_t4x8_r128_p2", 1, 32, 32, 4, 8, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t4x8_r128_p2, 1},
+  {"tsm_atb_f32_32x32_t4x8_r128_p2_nt", 1, 32, 32, 4, 8, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_t4x8_r128_p2_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r32_p1", 1, 32, 32, 8, 4, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1, 1},
   {"tsm_atb_f32_32x32_t8x4_r32_p1_nt", 1, 32, 32, 8, 4, 32, 1, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p1_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r32_p2", 1, 32, 32, 8, 4, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2, 1},
   {"tsm_atb_f32_32x32_t8x4_r32_p2_nt", 1, 32, 32, 8, 4, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r32_p4_nt", 1, 32, 32, 8, 4, 32, 4, 1, 16384, launch_tsm_atb_f32_32x32_t8x4_r32_p4_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r64_p1", 1, 32, 32, 8, 4, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1, 1},
   {"tsm_atb_f32_32x32_t8x4_r64_p1_nt", 1, 32, 32, 8, 4, 64, 1, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p1_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r64_p2", 1, 32, 32, 8, 4, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2, 1},
   {"tsm_atb_f32_32x32_t8x4_r64_p2_nt", 1, 32, 32, 8, 4, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f32_32x32_t8x4_r64_p4_nt", 1, 32, 32, 8, 4, 64, 4, 1, 32768, launch_tsm_atb_f32_32x32_t8x4_r64_p4_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r128_p1", 1, 32, 32, 8, 4, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1, 1},
   {"tsm_atb_f32_32x32_t8x4_r128_p1_nt", 1, 32, 32, 8, 4, 128, 1, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p1_nt, 1},
   {"tsm_atb_f32_32x32_t8x4_r128_p2", 1, 32, 32, 8, 4, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2, 1},
   {"tsm_atb_f32_32x32_t8x4_r128_p2_nt", 1, 32, 32, 8, 4, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_32x32_t8x8_r32_p1", 1, 32, 32, 8, 8, 32, 1, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p1, 1},
   {"tsm_atb_f32_32x32_t8x8_r32_p2", 1, 32, 32, 8, 8, 32, 2, 0, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p2, 1},
+  {"tsm_atb_f32_32x32_t8x8_r32_p2_nt", 1, 32, 32, 8, 8, 32, 2, 1, 16384, launch_tsm_atb_f32_32x32_t8x8_r32_p2_nt, 1},
   {"tsm_atb_f32_32x32_t8x8_r64_p1", 1, 32, 32, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p1, 1},
   {"tsm_atb_f32_32x32_t8x8_r64_p2", 1, 32, 32, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p2, 1},
+  {"tsm_atb_f32_32x32_t8x8_r64_p2_nt", 1, 32, 32, 8, 8, 64, 2, 1, 32768, launch_tsm_atb_f32_32x32_t8x8_r64_p2_nt, 1},
   {"tsm_atb_f32_32x32_t8x8_r128_p1", 1, 32, 32, 8, 8, 128, 1, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p1, 1},
   {"tsm_atb_f32_32x32_t8x8_r128_p2", 1, 32, 32, 8, 8, 128, 2, 0, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p2, 1},
+  {"tsm_atb_f32_32x32_t8x8_r128_p2_nt", 1, 32, 32, 8, 8, 128, 2, 1, 65536, launch_tsm_atb_f32_32x32_t8x8_r128_p2_nt, 1},
   {"tsm_atb_f32_32x64_direct_d8_nt", 1, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f32_32x64_direct_d8_nt, 3},
   {"tsm_atb_f32_32x64_mfma_r64_p2_nt", 1, 32, 64, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f32_64x1_t8x1_r64_p2_nt", 1, 64, 1, 8, 1, 64, 2, 1, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt, 1},
@@ -997,10 +1337,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_1x1_t1x1_r1024_p1_nt", 0, 1, 1, 1, 1, 1024, 1, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p1_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r1024_p2", 0, 1, 1, 1, 1, 1024, 2, 0, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2, 1},
   {"tsm_atb_f64_1x1_t1x1_r1024_p2_nt", 0, 1, 1, 1, 1, 1024, 2, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p2_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r1024_p4_nt", 0, 1, 1, 1, 1, 1024, 4, 1, 32768, launch_tsm_atb_f64_1x1_t1x1_r1024_p4_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r2048_p1", 0, 1, 1, 1, 1, 2048, 1, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1, 1},
   {"tsm_atb_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p1_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r2048_p2", 0, 1, 1, 1, 1, 2048, 2, 0, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2, 1},
   {"tsm_atb_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p2_nt, 1},
+  {"tsm_atb_f64_1x1_t1x1_r2048_p4_nt", 0, 1, 1, 1, 1, 2048, 4, 1, 65536, launch_tsm_atb_f64_1x1_t1x1_r2048_p4_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r4096_p1", 0, 1, 1, 1, 1, 4096, 1, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1, 1},
   {"tsm_atb_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p1_nt, 1},
   {"tsm_atb_f64_1x1_t1x1_r4096_p2", 0, 1, 1, 1, 1, 4096, 2, 0, 131072, launch_tsm_atb_f64_1x1_t1x1_r4096_p2, 1},
@@ -1016,10 +1358,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_2x2_t2x2_r512_p1_nt", 0, 2, 2, 2, 2, 512, 1, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p1_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r512_p2", 0, 2, 2, 2, 2, 512, 2, 0, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2, 1},
   {"tsm_atb_f64_2x2_t2x2_r512_p2_nt", 0, 2, 2, 2, 2, 512, 2, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p2_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r512_p4_nt", 0, 2, 2, 2, 2, 512, 4, 1, 32768, launch_tsm_atb_f64_2x2_t2x2_r512_p4_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r1024_p1", 0, 2, 2, 2, 2, 1024, 1, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1, 1},
   {"tsm_atb_f64_2x2_t2x2_r1024_p1_nt", 0, 2, 2, 2, 2, 1024, 1, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p1_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r1024_p2", 0, 2, 2, 2, 2, 1024, 2, 0, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2, 1},
   {"tsm_atb_f64_2x2_t2x2_r1024_p2_nt", 0, 2, 2, 2, 2, 1024, 2, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p2_nt, 1},
+  {"tsm_atb_f64_2x2_t2x2_r1024_p4_nt", 0, 2, 2, 2, 2, 1024, 4, 1, 65536, launch_tsm_atb_f64_2x2_t2x2_r1024_p4_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r2048_p1", 0, 2, 2, 2, 2, 2048, 1, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1, 1},
   {"tsm_atb_f64_2x2_t2x2_r2048_p1_nt", 0, 2, 2, 2, 2, 2048, 1, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p1_nt, 1},
   {"tsm_atb_f64_2x2_t2x2_r2048_p2", 0, 2, 2, 2, 2, 2048, 2, 0, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2, 1},
@@ -1037,10 +1381,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x4_t4x4_r256_p1_nt", 0, 4, 4, 4, 4, 256, 1, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p1_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r256_p2", 0, 4, 4, 4, 4, 256, 2, 0, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2, 1},
   {"tsm_atb_f64_4x4_t4x4_r256_p2_nt", 0, 4, 4, 4, 4, 256, 2, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r256_p4_nt", 0, 4, 4, 4, 4, 256, 4, 1, 32768, launch_tsm_atb_f64_4x4_t4x4_r256_p4_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r512_p1", 0, 4, 4, 4, 4, 512, 1, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1, 1},
   {"tsm_atb_f64_4x4_t4x4_r512_p1_nt", 0, 4, 4, 4, 4, 512, 1, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p1_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r512_p2", 0, 4, 4, 4, 4, 512, 2, 0, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2, 1},
   {"tsm_atb_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f64_4x4_t4x4_r512_p4_nt", 0, 4, 4, 4, 4, 512, 4, 1, 65536, launch_tsm_atb_f64_4x4_t4x4_r512_p4_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p1", 0, 4, 4, 4, 4, 1024, 1, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p2", 0, 4, 4, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2, 1},
@@ -1049,14 +1395,20 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x16_t4x8_r128_p2_nt", 0, 4, 16, 4, 8, 128, 2, 1, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p1", 0, 4, 32, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p1, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p2", 0, 4, 32, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r32_p2_nt", 0, 4, 32, 4, 4, 32, 2, 1, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2_nt, 1},
+  {"tsm_atb_f64_4x32_t4x4_r32_p4_nt", 0, 4, 32, 4, 4, 32, 4, 1, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p4_nt, 1},
   {"tsm_atb_f64_4x32_t4x4_r64_p1", 0, 4, 32, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p1, 1},
   {"tsm_atb_f64_4x32_t4x4_r64_p2", 0, 4, 32, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r64_p2_nt", 0, 4, 32, 4, 4, 64, 2, 1, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_4x32_t4x4_r64_p4_nt", 0, 4, 32, 4, 4, 64, 4, 1, 36864, launch_tsm_atb_f64_4x32_t4x4_r64_p4_nt, 1},
   {"tsm_atb_f64_4x32_t4x4_r128_p1", 0, 4, 32, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p1, 1},
   {"tsm_atb_f64_4x32_t4x4_r128_p2", 0, 4, 32, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_4x32_t4x4_r128_p2_nt", 0, 4, 32, 4, 4, 128, 2, 1, 73728, launch_tsm_atb_f64_4x32_t4x4_r128_p2_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r64_p1", 0, 4, 32, 4, 8, 64, 1, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1, 1},
   {"tsm_atb_f64_4x32_t4x8_r64_p1_nt", 0, 4, 32, 4, 8, 64, 1, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p1_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r64_p2", 0, 4, 32, 4, 8, 64, 2, 0, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2, 1},
   {"tsm_atb_f64_4x32_t4x8_r64_p2_nt", 0, 4, 32, 4, 8, 64, 2, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_4x32_t4x8_r64_p4_nt", 0, 4, 32, 4, 8, 64, 4, 1, 36864, launch_tsm_atb_f64_4x32_t4x8_r64_p4_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p1", 0, 4, 32, 4, 8, 128, 1, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p1_nt", 0, 4, 32, 4, 8, 128, 1, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p2", 0, 4, 32, 4, 8, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2, 1},
@@ -1069,10 +1421,37 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_8x8_t2x2_r256_p2", 0, 8, 8, 2, 2, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t2x2_r256_p2, 1},
   {"tsm_atb_f64_8x8_t4x4_r128_p1", 0, 8, 8, 4, 4, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p1, 1},
   {"tsm_atb_f64_8x8_t4x4_r128_p2", 0, 8, 8, 4, 4, 128, 2, 0, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r128_p2_nt", 0, 8, 8, 4, 4, 128, 2, 1, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t4x4_r128_p4_nt", 0, 8, 8, 4, 4, 128, 4, 1, 32768, launch_tsm_atb_f64_8x8_t4x4_r128_p4_nt, 1},
   {"tsm_atb_f64_8x8_t4x4_r256_p1", 0, 8, 8, 4, 4, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p1, 1},
   {"tsm_atb_f64_8x8_t4x4_r256_p2", 0, 8, 8, 4, 4, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r256_p2_nt", 0, 8, 8, 4, 4, 256, 2, 1, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t4x4_r256_p4_nt", 0, 8, 8, 4, 4, 256, 4, 1, 65536, launch_tsm_atb_f64_8x8_t4x4_r256_p4_nt, 1},
   {"tsm_atb_f64_8x8_t4x4_r512_p1", 0, 8, 8, 4, 4, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p1, 1},
   {"tsm_atb_f64_8x8_t4x4_r512_p2", 0, 8, 8, 4, 4, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p2, 1},
+  {"tsm_atb_f64_8x8_t4x4_r512_p2_nt", 0, 8, 8, 4, 4, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t4x8_r128_p1", 0, 8, 8, 4, 8, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t4x8_r128_p1, 1},
+  {"tsm_atb_f64_8x8_t4x8_r128_p2", 0, 8, 8, 4, 8, 128, 2, 0, 32768, launch_tsm_atb_f64_8x8_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_8x8_t4x8_r128_p2_nt", 0, 8, 8, 4, 8, 128, 2, 1, 32768, launch_tsm_atb_f64_8x8_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t4x8_r128_p4_nt", 0, 8, 8, 4, 8, 128, 4, 1, 32768, launch_tsm_atb_f64_8x8_t4x8_r128_p4_nt, 1},
+  {"tsm_atb_f64_8x8_t4x8_r256_p1", 0, 8, 8, 4, 8, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t4x8_r256_p1, 1},
+  {"tsm_atb_f64_8x8_t4x8_r256_p2", 0, 8, 8, 4, 8, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t4x8_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t4x8_r256_p2_nt", 0, 8, 8, 4, 8, 256, 2, 1, 65536, launch_tsm_atb_f64_8x8_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t4x8_r256_p4_nt", 0, 8, 8, 4, 8, 256, 4, 1, 65536, launch_tsm_atb_f64_8x8_t4x8_r256_p4_nt, 1},
+  {"tsm_atb_f64_8x8_t4x8_r512_p1", 0, 8, 8, 4, 8, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t4x8_r512_p1, 1},
+  {"tsm_atb_f64_8x8_t4x8_r512_p2", 0, 8, 8, 4, 8, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t4x8_r512_p2, 1},
+  {"tsm_atb_f64_8x8_t4x8_r512_p2_nt", 0, 8, 8, 4, 8, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t4x8_r512_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t8x4_r128_p1", 0, 8, 8, 8, 4, 128, 1, 0, 32768, launch_tsm_atb_f64_8x8_t8x4_r128_p1, 1},
+  {"tsm_atb_f64_8x8_t8x4_r128_p2", 0, 8, 8, 8, 4, 128, 2, 0, 32768, launch_tsm_atb_f64_8x8_t8x4_r128_p2, 1},
+  {"tsm_atb_f64_8x8_t8x4_r128_p2_nt", 0, 8, 8, 8, 4, 128, 2, 1, 32768, launch_tsm_atb_f64_8x8_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t8x4_r128_p4_nt", 0, 8, 8, 8, 4, 128, 4, 1, 32768, launch_tsm_atb_f64_8x8_t8x4_r128_p4_nt, 1},
+  {"tsm_atb_f64_8x8_t8x4_r256_p1", 0, 8, 8, 8, 4, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t8x4_r256_p1, 1},
+  {"tsm_atb_f64_8x8_t8x4_r256_p2", 0, 8, 8, 8, 4, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t8x4_r256_p2, 1},
+  {"tsm_atb_f64_8x8_t8x4_r256_p2_nt", 0, 8, 8, 8, 4, 256, 2, 1, 65536, launch_tsm_atb_f64_8x8_t8x4_r256_p2_nt, 1},
+  {"tsm_atb_f64_8x8_t8x4_r256_p4_nt", 0, 8, 8, 8, 4, 256, 4, 1, 65536, launch_tsm_atb_f64_8x8_t8x4_r256_p4_nt, 1},
+  {"tsm_atb_f64_8x8_t8x4_r512_p1", 0, 8, 8, 8, 4, 512, 1, 0, 131072, launch_tsm_atb_f64_8x8_t8x4_r512_p1, 1},
+  {"tsm_atb_f64_8x8_t8x4_r512_p2", 0, 8, 8, 8, 4, 512, 2, 0, 131072, launch_tsm_atb_f64_8x8_t8x4_r512_p2, 1},
+  {"tsm_atb_f64_8x8_t8x4_r512_p2_nt", 0, 8, 8, 8, 4, 512, 2, 1, 131072, launch_tsm_atb_f64_8x8_t8x4_r512_p2_nt, 1},
   {"tsm_atb_f64_8x8_t8x8_r256_p1", 0, 8, 8, 8, 8, 256, 1, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1, 1},
   {"tsm_atb_f64_8x8_t8x8_r256_p1_nt", 0, 8, 8, 8, 8, 256, 1, 1, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p1_nt, 1},
   {"tsm_atb_f64_8x8_t8x8_r256_p2", 0, 8, 8, 8, 8, 256, 2, 0, 65536, launch_tsm_atb_f64_8x8_t8x8_r256_p2, 1},
@@ -1107,10 +1486,37 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_16x16_t2x2_r64_p2", 0, 16, 16, 2, 2, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t2x2_r64_p2, 1},
   {"tsm_atb_f64_16x16_t4x4_r64_p1", 0, 16, 16, 4, 4, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p1, 1},
   {"tsm_atb_f64_16x16_t4x4_r64_p2", 0, 16, 16, 4, 4, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r64_p2_nt", 0, 16, 16, 4, 4, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t4x4_r64_p4_nt", 0, 16, 16, 4, 4, 64, 4, 1, 32768, launch_tsm_atb_f64_16x16_t4x4_r64_p4_nt, 1},
   {"tsm_atb_f64_16x16_t4x4_r128_p1", 0, 16, 16, 4, 4, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p1, 1},
   {"tsm_atb_f64_16x16_t4x4_r128_p2", 0, 16, 16, 4, 4, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r128_p2_nt", 0, 16, 16, 4, 4, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t4x4_r128_p4_nt", 0, 16, 16, 4, 4, 128, 4, 1, 65536, launch_tsm_atb_f64_16x16_t4x4_r128_p4_nt, 1},
   {"tsm_atb_f64_16x16_t4x4_r256_p1", 0, 16, 16, 4, 4, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p1, 1},
   {"tsm_atb_f64_16x16_t4x4_r256_p2", 0, 16, 16, 4, 4, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p2, 1},
+  {"tsm_atb_f64_16x16_t4x4_r256_p2_nt", 0, 16, 16, 4, 4, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t4x8_r64_p1", 0, 16, 16, 4, 8, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t4x8_r64_p1, 1},
+  {"tsm_atb_f64_16x16_t4x8_r64_p2", 0, 16, 16, 4, 8, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t4x8_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t4x8_r64_p2_nt", 0, 16, 16, 4, 8, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t4x8_r64_p4_nt", 0, 16, 16, 4, 8, 64, 4, 1, 32768, launch_tsm_atb_f64_16x16_t4x8_r64_p4_nt, 1},
+  {"tsm_atb_f64_16x16_t4x8_r128_p1", 0, 16, 16, 4, 8, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t4x8_r128_p1, 1},
+  {"tsm_atb_f64_16x16_t4x8_r128_p2", 0, 16, 16, 4, 8, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_16x16_t4x8_r128_p2_nt", 0, 16, 16, 4, 8, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t4x8_r128_p4_nt", 0, 16, 16, 4, 8, 128, 4, 1, 65536, launch_tsm_atb_f64_16x16_t4x8_r128_p4_nt, 1},
+  {"tsm_atb_f64_16x16_t4x8_r256_p1", 0, 16, 16, 4, 8, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t4x8_r256_p1, 1},
+  {"tsm_atb_f64_16x16_t4x8_r256_p2", 0, 16, 16, 4, 8, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t4x8_r256_p2, 1},
+  {"tsm_atb_f64_16x16_t4x8_r256_p2_nt", 0, 16, 16, 4, 8, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t8x4_r64_p1", 0, 16, 16, 8, 4, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t8x4_r64_p1, 1},
+  {"tsm_atb_f64_16x16_t8x4_r64_p2", 0, 16, 16, 8, 4, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t8x4_r64_p2, 1},
+  {"tsm_atb_f64_16x16_t8x4_r64_p2_nt", 0, 16, 16, 8, 4, 64, 2, 1, 32768, launch_tsm_atb_f64_16x16_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t8x4_r64_p4_nt", 0, 16, 16, 8, 4, 64, 4, 1, 32768, launch_tsm_atb_f64_16x16_t8x4_r64_p4_nt, 1},
+  {"tsm_atb_f64_16x16_t8x4_r128_p1", 0, 16, 16, 8, 4, 128, 1, 0, 65536, launch_tsm_atb_f64_16x16_t8x4_r128_p1, 1},
+  {"tsm_atb_f64_16x16_t8x4_r128_p2", 0, 16, 16, 8, 4, 128, 2, 0, 65536, launch_tsm_atb_f64_16x16_t8x4_r128_p2, 1},
+  {"tsm_atb_f64_16x16_t8x4_r128_p2_nt", 0, 16, 16, 8, 4, 128, 2, 1, 65536, launch_tsm_atb_f64_16x16_t8x4_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x16_t8x4_r128_p4_nt", 0, 16, 16, 8, 4, 128, 4, 1, 65536, launch_tsm_atb_f64_16x16_t8x4_r128_p4_nt, 1},
+  {"tsm_atb_f64_16x16_t8x4_r256_p1", 0, 16, 16, 8, 4, 256, 1, 0, 131072, launch_tsm_atb_f64_16x16_t8x4_r256_p1, 1},
+  {"tsm_atb_f64_16x16_t8x4_r256_p2", 0, 16, 16, 8, 4, 256, 2, 0, 131072, launch_tsm_atb_f64_16x16_t8x4_r256_p2, 1},
+  {"tsm_atb_f64_16x16_t8x4_r256_p2_nt", 0, 16, 16, 8, 4, 256, 2, 1, 131072, launch_tsm_atb_f64_16x16_t8x4_r256_p2_nt, 1},
   {"tsm_atb_f64_16x16_t8x8_r64_p1", 0, 16, 16, 8, 8, 64, 1, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1, 1},
   {"tsm_atb_f64_16x16_t8x8_r64_p1_nt", 0, 16, 16, 8, 8, 64, 1, 1, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p1_nt, 1},
   {"tsm_atb_f64_16x16_t8x8_r64_p2", 0, 16, 16, 8, 8, 64, 2, 0, 32768, launch_tsm_atb_f64_16x16_t8x8_r64_p2, 1},
@@ -1131,14 +1537,20 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_32x2_t8x2_r64_p2_nt", 0, 32, 2, 8, 2, 64, 2, 1, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p1", 0, 32, 4, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p1, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p2", 0, 32, 4, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r32_p2_nt", 0, 32, 4, 4, 4, 32, 2, 1, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2_nt, 1},
+  {"tsm_atb_f64_32x4_t4x4_r32_p4_nt", 0, 32, 4, 4, 4, 32, 4, 1, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p4_nt, 1},
   {"tsm_atb_f64_32x4_t4x4_r64_p1", 0, 32, 4, 4, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p1, 1},
   {"tsm_atb_f64_32x4_t4x4_r64_p2", 0, 32, 4, 4, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r64_p2_nt", 0, 32, 4, 4, 4, 64, 2, 1, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x4_t4x4_r64_p4_nt", 0, 32, 4, 4, 4, 64, 4, 1, 36864, launch_tsm_atb_f64_32x4_t4x4_r64_p4_nt, 1},
   {"tsm_atb_f64_32x4_t4x4_r128_p1", 0, 32, 4, 4, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p1, 1},
   {"tsm_atb_f64_32x4_t4x4_r128_p2", 0, 32, 4, 4, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p2, 1},
+  {"tsm_atb_f64_32x4_t4x4_r128_p2_nt", 0, 32, 4, 4, 4, 128, 2, 1, 73728, launch_tsm_atb_f64_32x4_t4x4_r128_p2_nt, 1},
   {"tsm_atb_f64_32x4_t8x4_r64_p1", 0, 32, 4, 8, 4, 64, 1, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1, 1},
   {"tsm_atb_f64_32x4_t8x4_r64_p1_nt", 0, 32, 4, 8, 4, 64, 1, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p1_nt, 1},
   {"tsm_atb_f64_32x4_t8x4_r64_p2", 0, 32, 4, 8, 4, 64, 2, 0, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2, 1},
   {"tsm_atb_f64_32x4_t8x4_r64_p2_nt", 0, 32, 4, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x4_t8x4_r64_p4_nt", 0, 32, 4, 8, 4, 64, 4, 1, 36864, launch_tsm_atb_f64_32x4_t8x4_r64_p4_nt, 1},
   {"tsm_atb_f64_32x4_t8x4_r128_p1", 0, 32, 4, 8, 4, 128, 1, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1, 1},
   {"tsm_atb_f64_32x4_t8x4_r128_p1_nt", 0, 32, 4, 8, 4, 128, 1, 1, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p1_nt, 1},
   {"tsm_atb_f64_32x4_t8x4_r128_p2", 0, 32, 4, 8, 4, 128, 2, 0, 73728, launch_tsm_atb_f64_32x4_t8x4_r128_p2, 1},
@@ -1163,8 +1575,34 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_mfma_r128_p2_nt, 2},
   {"tsm_atb_f64_32x32_t4x4_r32_p1", 0, 32, 32, 4, 4, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p1, 1},
   {"tsm_atb_f64_32x32_t4x4_r32_p2", 0, 32, 32, 4, 4, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p2, 1},
+  {"tsm_atb_f64_32x32_t4x4_r32_p2_nt", 0, 32, 32, 4, 4, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t4x4_r32_p4_nt", 0, 32, 32, 4, 4, 32, 4, 1, 32768, launch_tsm_atb_f64_32x32_t4x4_r32_p4_nt, 1},
   {"tsm_atb_f64_32x32_t4x4_r64_p1", 0, 32, 32, 4, 4, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p1, 1},
   {"tsm_atb_f64_32x32_t4x4_r64_p2", 0, 32, 32, 4, 4, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p2, 1},
+  {"tsm_atb_f64_32x32_t4x4_r64_p2_nt", 0, 32, 32, 4, 4, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t4x4_r64_p4_nt", 0, 32, 32, 4, 4, 64, 4, 1, 65536, launch_tsm_atb_f64_32x32_t4x4_r64_p4_nt, 1},
+  {"tsm_atb_f64_32x32_t4x8_r32_p1", 0, 32, 32, 4, 8, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t4x8_r32_p1, 1},
+  {"tsm_atb_f64_32x32_t4x8_r32_p2", 0, 32, 32, 4, 8, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t4x8_r32_p2, 1},
+  {"tsm_atb_f64_32x32_t4x8_r32_p2_nt", 0, 32, 32, 4, 8, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_t4x8_r32_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t4x8_r32_p4_nt", 0, 32, 32, 4, 8, 32, 4, 1, 32768, launch_tsm_atb_f64_32x32_t4x8_r32_p4_nt, 1},
+  {"tsm_atb_f64_32x32_t4x8_r64_p1", 0, 32, 32, 4, 8, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t4x8_r64_p1, 1},
+  {"tsm_atb_f64_32x32_t4x8_r64_p2", 0, 32, 32, 4, 8, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t4x8_r64_p2, 1},
+  {"tsm_atb_f64_32x32_t4x8_r64_p2_nt", 0, 32, 32, 4, 8, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_t4x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t4x8_r64_p4_nt", 0, 32, 32, 4, 8, 64, 4, 1, 65536, launch_tsm_atb_f64_32x32_t4x8_r64_p4_nt, 1},
+  {"tsm_atb_f64_32x32_t4x8_r128_p1", 0, 32, 32, 4, 8, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_t4x8_r128_p1, 1},
+  {"tsm_atb_f64_32x32_t4x8_r128_p2", 0, 32, 32, 4, 8, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_t4x8_r128_p2, 1},
+  {"tsm_atb_f64_32x32_t4x8_r128_p2_nt", 0, 32, 32, 4, 8, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t8x4_r32_p1", 0, 32, 32, 8, 4, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t8x4_r32_p1, 1},
+  {"tsm_atb_f64_32x32_t8x4_r32_p2", 0, 32, 32, 8, 4, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t8x4_r32_p2, 1},
+  {"tsm_atb_f64_32x32_t8x4_r32_p2_nt", 0, 32, 32, 8, 4, 32, 2, 1, 32768, launch_tsm_atb_f64_32x32_t8x4_r32_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t8x4_r32_p4_nt", 0, 32, 32, 8, 4, 32, 4, 1, 32768, launch_tsm_atb_f64_32x32_t8x4_r32_p4_nt, 1},
+  {"tsm_atb_f64_32x32_t8x4_r64_p1", 0, 32, 32, 8, 4, 64, 1, 0, 65536, launch_tsm_atb_f64_32x32_t8x4_r64_p1, 1},
+  {"tsm_atb_f64_32x32_t8x4_r64_p2", 0, 32, 32, 8, 4, 64, 2, 0, 65536, launch_tsm_atb_f64_32x32_t8x4_r64_p2, 1},
+  {"tsm_atb_f64_32x32_t8x4_r64_p2_nt", 0, 32, 32, 8, 4, 64, 2, 1, 65536, launch_tsm_atb_f64_32x32_t8x4_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x32_t8x4_r64_p4_nt", 0, 32, 32, 8, 4, 64, 4, 1, 65536, launch_tsm_atb_f64_32x32_t8x4_r64_p4_nt, 1},
+  {"tsm_atb_f64_32x32_t8x4_r128_p1", 0, 32, 32, 8, 4, 128, 1, 0, 131072, launch_tsm_atb_f64_32x32_t8x4_r128_p1, 1},
+  {"tsm_atb_f64_32x32_t8x4_r128_p2", 0, 32, 32, 8, 4, 128, 2, 0, 131072, launch_tsm_atb_f64_32x32_t8x4_r128_p2, 1},
+  {"tsm_atb_f64_32x32_t8x4_r128_p2_nt", 0, 32, 32, 8, 4, 128, 2, 1, 131072, launch_tsm_atb_f64_32x32_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f64_32x32_t8x8_r32_p1", 0, 32, 32, 8, 8, 32, 1, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1, 1},
   {"tsm_atb_f64_32x32_t8x8_r32_p1_nt", 0, 32, 32, 8, 8, 32, 1, 1, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p1_nt, 1},
   {"tsm_atb_f64_32x32_t8x8_r32_p2", 0, 32, 32, 8, 8, 32, 2, 0, 32768, launch_tsm_atb_f64_32x32_t8x8_r32_p2, 1},
@@ -1357,129 +1795,129 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 8, 1, 21, 2, 1, 21, 1},  // tsm_atb_bf16_8x1_t8x1_r1024_p2_nt
   {2, 8, 2, 22, 1, 1, 22, 1},  // tsm_atb_bf16_8x2_t8x2_r1024_p2_nt
   {2, 8, 4, 23, 1, 1, 23, 1},  // tsm_atb_bf16_8x4_t8x4_r1024_p2_nt
-  {2, 8, 8, 33, 2, 1, 33, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 8, 16, 46, 2, 0, 46, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2_nt
-  {2, 8, 32, 47, 2, 0, 47, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2_nt
-  {2, 8, 64, 48, 2, 0, 48, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2_nt
-  {2, 16, 1, 49, 2, 1, 49, 1},  // tsm_atb_bf16_16x1_t8x1_r512_p2_nt
-  {2, 16, 2, 50, 2, 1, 50, 1},  // tsm_atb_bf16_16x2_t8x2_r512_p2_nt
-  {2, 16, 4, 51, 1, 0, 51, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2_nt
-  {2, 16, 8, 52, 2, 1, 52, 1},  // tsm_atb_bf16_16x8_t8x4_r512_p2_nt
-  {2, 16, 16, 64, 1, 1, 64, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
-  {2, 16, 32, 90, 1, 1, 90, 1},  // tsm_atb_bf16_16x32_mfma_r256_p2_nt
-  {2, 16, 64, 92, 1, 1, 92, 1},  // tsm_atb_bf16_16x64_mfma_r128_p2_nt
-  {2, 32, 1, 93, 2, 1, 93, 1},  // tsm_atb_bf16_32x1_t8x1_r256_p2_nt
-  {2, 32, 2, 94, 2, 1, 94, 1},  // tsm_atb_bf16_32x2_t8x2_r256_p2_nt
-  {2, 32, 4, 95, 2, 1, 95, 1},  // tsm_atb_bf16_32x4_t8x4_r256_p2_nt
-  {2, 32, 8, 96, 2, 0, 96, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2_nt
-  {2, 32, 16, 98, 1, 1, 98, 1},  // tsm_atb_bf16_32x16_mfma_r256_p2_nt
-  {2, 32, 32, 110, 1, 1, 110, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
-  {2, 32, 64, 136, 1, 1, 136, 1},  // tsm_atb_bf16_32x64_mfma_r128_p2_nt
-  {2, 64, 1, 137, 2, 1, 137, 1},  // tsm_atb_bf16_64x1_t8x1_r128_p2_nt
-  {2, 64, 2, 138, 2, 1, 138, 1},  // tsm_atb_bf16_64x2_t8x2_r128_p2_nt
-  {2, 64, 4, 139, 2, 1, 139, 1},  // tsm_atb_bf16_64x4_t8x4_r128_p2_nt
-  {2, 64, 8, 140, 2, 1, 140, 1},  // tsm_atb_bf16_64x8_t8x4_r128_p2_nt
-  {2, 64, 16, 142, 1, 1, 142, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
-  {2, 64, 32, 144, 2, 1, 144, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
-  {2, 64, 64, 146, 1, 0, 154, 1},  // tsm_atb_bf16_64x64_direct_d4_nt (else tsm_atb_bf16_64x64_mfma_r256_p2_nt)
-  {1, 1, 1, 167, 1, 1, 167, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
-  {1, 1, 2, 168, 1, 1, 168, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
-  {1, 1, 4, 169, 1, 1, 169, 1},  // tsm_atb_f32_1x4_t1x4_r1024_p2_nt
-  {1, 1, 8, 170, 1, 1, 170, 1},  // tsm_atb_f32_1x8_t1x4_r512_p2_nt
-  {1, 1, 16, 171, 1, 1, 171, 1},  // tsm_atb_f32_1x16_t1x4_r256_p2_nt
-  {1, 1, 32, 172, 1, 1, 172, 1},  // tsm_atb_f32_1x32_t1x4_r128_p2_nt
-  {1, 1, 64, 173, 1, 1, 173, 1},  // tsm_atb_f32_1x64_t1x4_r64_p2_nt
-  {1, 2, 1, 174, 1, 1, 174, 1},  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
-  {1, 2, 2, 175, 1, 1, 175, 1},  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
-  {1, 2, 4, 176, 1, 1, 176, 1},  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
-  {1, 2, 8, 177, 1, 1, 177, 1},  // tsm_atb_f32_2x8_t2x4_r512_p2_nt
-  {1, 2, 16, 178, 1, 1, 178, 1},  // tsm_atb_f32_2x16_t2x4_r256_p2_nt
-  {1, 2, 32, 179, 1, 1, 179, 1},  // tsm_atb_f32_2x32_t2x4_r128_p2_nt
-  {1, 2, 64, 180, 1, 1, 180, 1},  // tsm_atb_f32_2x64_t2x4_r64_p2_nt
-  {1, 4, 1, 181, 1, 1, 181, 1},  // tsm_atb_f32_4x1_t4x1_r1024_p2_nt
-  {1, 4, 2, 182, 1, 1, 182, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
-  {1, 4, 4, 183, 1, 1, 183, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
-  {1, 4, 8, 184, 1, 1, 184, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
-  {1, 4, 16, 185, 1, 1, 185, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
-  {1, 4, 32, 186, 1, 1, 186, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
-  {1, 4, 64, 187, 1, 1, 187, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
-  {1, 8, 1, 188, 2, 1, 188, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
-  {1, 8, 2, 189, 1, 1, 189, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
-  {1, 8, 4, 190, 1, 1, 190, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
-  {1, 8, 8, 208, 1, 1, 208, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
-  {1, 8, 16, 215, 1, 1, 215, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
-  {1, 8, 32, 216, 1, 1, 216, 1},  // tsm_atb_f32_8x32_t8x4_r128_p2_nt
-  {1, 8, 64, 217, 2, 1, 217, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
-  {1, 16, 1, 218, 2, 1, 218, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
-  {1, 16, 2, 219, 2, 1, 219, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
-  {1, 16, 4, 220, 1, 0, 220, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
-  {1, 16, 8, 221, 1, 1, 221, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
-  {1, 16, 16, 228, 1, 1, 228, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-  {1, 16, 32, 255, 1, 0, 256, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
-  {1, 16, 64, 258, 1, 1, 258, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
-  {1, 32, 1, 259, 2, 1, 259, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
-  {1, 32, 2, 260, 2, 1, 260, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
-  {1, 32, 4, 261, 2, 1, 261, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
-  {1, 32, 8, 262, 1, 0, 262, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2_nt
-  {1, 32, 16, 263, 1, 0, 264, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
-  {1, 32, 32, 275, 1, 1, 275, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 303, 2, 1, 303, 1},  // tsm_atb_f32_32x64_mfma_r64_p2_nt
-  {1, 64, 1, 304, 2, 1, 304, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
-  {1, 64, 2, 305, 2, 1, 305, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
-  {1, 64, 4, 306, 2, 1, 306, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
-  {1, 64, 8, 307, 2, 1, 307, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
-  {1, 64, 16, 309, 1, 1, 309, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
-  {1, 64, 32, 310, 1, 0, 311, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
-  {1, 64, 64, 314, 1, 0, 322, 1},  // tsm_atb_f32_64x64_direct_d16_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
-  {0, 1, 1, 348, 1, 1, 348, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
-  {0, 1, 2, 351, 1, 1, 351, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
-  {0, 1, 4, 352, 1, 1, 352, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
-  {0, 1, 8, 353, 1, 1, 353, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
-  {0, 1, 16, 354, 1, 1, 354, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
-  {0, 1, 32, 355, 1, 1, 355, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
-  {0, 1, 64, 356, 1, 1, 356, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
-  {0, 2, 1, 357, 1, 1, 357, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
-  {0, 2, 2, 367, 1, 1, 367, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
-  {0, 2, 4, 370, 1, 1, 370, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
-  {0, 2, 8, 371, 1, 1, 371, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
-  {0, 2, 16, 372, 1, 1, 372, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
-  {0, 2, 32, 373, 1, 1, 373, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
-  {0, 2, 64, 374, 1, 1, 374, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
-  {0, 4, 1, 375, 1, 1, 375, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
-  {0, 4, 2, 376, 1, 1, 376, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
-  {0, 4, 4, 380, 2, 1, 380, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-  {0, 4, 8, 391, 1, 1, 391, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
-  {0, 4, 16, 392, 1, 1, 392, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
-  {0, 4, 32, 400, 2, 1, 400, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-  {0, 4, 64, 407, 1, 1, 407, 1},  // tsm_atb_f64_4x64_t4x8_r32_p2_nt
-  {0, 8, 1, 408, 2, 1, 408, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
-  {0, 8, 2, 409, 1, 1, 409, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
-  {0, 8, 4, 410, 1, 1, 410, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
-  {0, 8, 8, 422, 1, 1, 422, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
-  {0, 8, 16, 427, 1, 1, 427, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
-  {0, 8, 32, 428, 1, 1, 428, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
-  {0, 8, 64, 429, 2, 1, 429, 1},  // tsm_atb_f64_8x64_t8x8_r32_p2_nt
-  {0, 16, 1, 430, 2, 1, 430, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
-  {0, 16, 2, 431, 2, 1, 431, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
-  {0, 16, 4, 432, 1, 1, 432, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
-  {0, 16, 8, 433, 1, 1, 433, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
-  {0, 16, 16, 440, 1, 1, 440, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
-  {0, 16, 32, 469, 1, 0, 470, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
-  {0, 16, 64, 472, 1, 1, 472, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
-  {0, 32, 1, 473, 2, 1, 473, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
-  {0, 32, 2, 474, 2, 1, 474, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
-  {0, 32, 4, 482, 2, 1, 482, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
-  {0, 32, 8, 489, 1, 1, 489, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
-  {0, 32, 16, 490, 1, 0, 491, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
-  {0, 32, 32, 506, 1, 0, 506, 0},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
-  {0, 32, 64, 524, 2, 0, 524, 0},  // tsm_atb_f64_32x64_mfma_r32_p2_nt
-  {0, 64, 1, 525, 2, 1, 525, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
-  {0, 64, 2, 526, 2, 1, 526, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
-  {0, 64, 4, 527, 2, 1, 527, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
-  {0, 64, 8, 528, 2, 1, 528, 1},  // tsm_atb_f64_64x8_t8x8_r32_p2_nt
-  {0, 64, 16, 530, 1, 1, 530, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
-  {0, 64, 32, 532, 1, 0, 532, 0},  // tsm_atb_f64_64x32_mfma_r32_p2_nt
-  {0, 64, 64, 535, 2, 0, 543, 0},  // tsm_atb_f64_64x64_direct_d16_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
+  {2, 8, 8, 49, 2, 1, 49, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
+  {2, 8, 16, 66, 2, 0, 66, 0},  // tsm_atb_bf16_8x16_t8x4_r512_p2_nt
+  {2, 8, 32, 67, 2, 0, 67, 0},  // tsm_atb_bf16_8x32_t8x4_r256_p2_nt
+  {2, 8, 64, 68, 2, 0, 68, 0},  // tsm_atb_bf16_8x64_t8x4_r128_p2_nt
+  {2, 16, 1, 69, 2, 1, 69, 1},  // tsm_atb_bf16_16x1_t8x1_r512_p2_nt
+  {2, 16, 2, 70, 2, 1, 70, 1},  // tsm_atb_bf16_16x2_t8x2_r512_p2_nt
+  {2, 16, 4, 71, 1, 0, 71, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2_nt
+  {2, 16, 8, 72, 2, 1, 72, 1},  // tsm_atb_bf16_16x8_t8x4_r512_p2_nt
+  {2, 16, 16, 84, 1, 1, 84, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
+  {2, 16, 32, 130, 1, 1, 130, 1},  // tsm_atb_bf16_16x32_mfma_r256_p2_nt
+  {2, 16, 64, 132, 1, 1, 132, 1},  // tsm_atb_bf16_16x64_mfma_r128_p2_nt
+  {2, 32, 1, 133, 2, 1, 133, 1},  // tsm_atb_bf16_32x1_t8x1_r256_p2_nt
+  {2, 32, 2, 134, 2, 1, 134, 1},  // tsm_atb_bf16_32x2_t8x2_r256_p2_nt
+  {2, 32, 4, 135, 2, 1, 135, 1},  // tsm_atb_bf16_32x4_t8x4_r256_p2_nt
+  {2, 32, 8, 136, 2, 0, 136, 0},  // tsm_atb_bf16_32x8_t8x4_r256_p2_nt
+  {2, 32, 16, 138, 1, 1, 138, 1},  // tsm_atb_bf16_32x16_mfma_r256_p2_nt
+  {2, 32, 32, 150, 1, 1, 150, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
+  {2, 32, 64, 196, 1, 1, 196, 1},  // tsm_atb_bf16_32x64_mfma_r128_p2_nt
+  {2, 64, 1, 197, 2, 1, 197, 1},  // tsm_atb_bf16_64x1_t8x1_r128_p2_nt
+  {2, 64, 2, 198, 2, 1, 198, 1},  // tsm_atb_bf16_64x2_t8x2_r128_p2_nt
+  {2, 64, 4, 199, 2, 1, 199, 1},  // tsm_atb_bf16_64x4_t8x4_r128_p2_nt
+  {2, 64, 8, 200, 2, 1, 200, 1},  // tsm_atb_bf16_64x8_t8x4_r128_p2_nt
+  {2, 64, 16, 202, 1, 1, 202, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
+  {2, 64, 32, 204, 2, 1, 204, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
+  {2, 64, 64, 206, 1, 0, 214, 1},  // tsm_atb_bf16_64x64_direct_d4_nt (else tsm_atb_bf16_64x64_mfma_r256_p2_nt)
+  {1, 1, 1, 227, 1, 1, 227, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
+  {1, 1, 2, 228, 1, 1, 228, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
+  {1, 1, 4, 229, 1, 1, 229, 1},  // tsm_atb_f32_1x4_t1x4_r1024_p2_nt
+  {1, 1, 8, 230, 1, 1, 230, 1},  // tsm_atb_f32_1x8_t1x4_r512_p2_nt
+  {1, 1, 16, 231, 1, 1, 231, 1},  // tsm_atb_f32_1x16_t1x4_r256_p2_nt
+  {1, 1, 32, 232, 1, 1, 232, 1},  // tsm_atb_f32_1x32_t1x4_r128_p2_nt
+  {1, 1, 64, 233, 1, 1, 233, 1},  // tsm_atb_f32_1x64_t1x4_r64_p2_nt
+  {1, 2, 1, 234, 1, 1, 234, 1},  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
+  {1, 2, 2, 235, 1, 1, 235, 1},  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
+  {1, 2, 4, 236, 1, 1, 236, 1},  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
+  {1, 2, 8, 237, 1, 1, 237, 1},  // tsm_atb_f32_2x8_t2x4_r512_p2_nt
+  {1, 2, 16, 238, 1, 1, 238, 1},  // tsm_atb_f32_2x16_t2x4_r256_p2_nt
+  {1, 2, 32, 239, 1, 1, 239, 1},  // tsm_atb_f32_2x32_t2x4_r128_p2_nt
+  {1, 2, 64, 240, 1, 1, 240, 1},  // tsm_atb_f32_2x64_t2x4_r64_p2_nt
+  {1, 4, 1, 241, 1, 1, 241, 1},  // tsm_atb_f32_4x1_t4x1_r1024_p2_nt
+  {1, 4, 2, 242, 1, 1, 242, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
+  {1, 4, 4, 243, 1, 1, 243, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
+  {1, 4, 8, 244, 1, 1, 244, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
+  {1, 4, 16, 245, 1, 1, 245, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
+  {1, 4, 32, 246, 1, 1, 246, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
+  {1, 4, 64, 247, 1, 1, 247, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
+  {1, 8, 1, 248, 2, 1, 248, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
+  {1, 8, 2, 249, 1, 1, 249, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
+  {1, 8, 4, 250, 1, 1, 250, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
+  {1, 8, 8, 286, 1, 1, 286, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
+  {1, 8, 16, 296, 1, 1, 296, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
+  {1, 8, 32, 297, 1, 1, 297, 1},  // tsm_atb_f32_8x32_t8x4_r128_p2_nt
+  {1, 8, 64, 298, 2, 1, 298, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
+  {1, 16, 1, 299, 2, 1, 299, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
+  {1, 16, 2, 300, 2, 1, 300, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
+  {1, 16, 4, 301, 1, 0, 301, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
+  {1, 16, 8, 302, 1, 1, 302, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
+  {1, 16, 16, 309, 1, 1, 309, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
+  {1, 16, 32, 356, 1, 0, 357, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
+  {1, 16, 64, 359, 1, 1, 359, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
+  {1, 32, 1, 360, 2, 1, 360, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
+  {1, 32, 2, 361, 2, 1, 361, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
+  {1, 32, 4, 362, 2, 1, 362, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
+  {1, 32, 8, 363, 1, 0, 363, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2_nt
+  {1, 32, 16, 364, 1, 0, 365, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
+  {1, 32, 32, 376, 1, 1, 376, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+  {1, 32, 64, 424, 2, 1, 424, 1},  // tsm_atb_f32_32x64_mfma_r64_p2_nt
+  {1, 64, 1, 425, 2, 1, 425, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
+  {1, 64, 2, 426, 2, 1, 426, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
+  {1, 64, 4, 427, 2, 1, 427, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
+  {1, 64, 8, 428, 2, 1, 428, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
+  {1, 64, 16, 430, 1, 1, 430, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
+  {1, 64, 32, 431, 1, 0, 432, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
+  {1, 64, 64, 435, 1, 0, 443, 1},  // tsm_atb_f32_64x64_direct_d16_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
+  {0, 1, 1, 471, 1, 1, 471, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
+  {0, 1, 2, 474, 1, 1, 474, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
+  {0, 1, 4, 475, 1, 1, 475, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
+  {0, 1, 8, 476, 1, 1, 476, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
+  {0, 1, 16, 477, 1, 1, 477, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
+  {0, 1, 32, 478, 1, 1, 478, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
+  {0, 1, 64, 479, 1, 1, 479, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
+  {0, 2, 1, 480, 1, 1, 480, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
+  {0, 2, 2, 492, 1, 1, 492, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
+  {0, 2, 4, 495, 1, 1, 495, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
+  {0, 2, 8, 496, 1, 1, 496, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
+  {0, 2, 16, 497, 1, 1, 497, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
+  {0, 2, 32, 498, 1, 1, 498, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
+  {0, 2, 64, 499, 1, 1, 499, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
+  {0, 4, 1, 500, 1, 1, 500, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
+  {0, 4, 2, 501, 1, 1, 501, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
+  {0, 4, 4, 505, 2, 1, 505, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
+  {0, 4, 8, 518, 1, 1, 518, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
+  {0, 4, 16, 519, 1, 1, 519, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
+  {0, 4, 32, 532, 2, 1, 532, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
+  {0, 4, 64, 540, 1, 1, 540, 1},  // tsm_atb_f64_4x64_t4x8_r32_p2_nt
+  {0, 8, 1, 541, 2, 1, 541, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
+  {0, 8, 2, 542, 1, 1, 542, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
+  {0, 8, 4, 543, 1, 1, 543, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
+  {0, 8, 8, 582, 1, 1, 582, 1},  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
+  {0, 8, 16, 587, 1, 1, 587, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
+  {0, 8, 32, 588, 1, 1, 588, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
+  {0, 8, 64, 589, 2, 1, 589, 1},  // tsm_atb_f64_8x64_t8x8_r32_p2_nt
+  {0, 16, 1, 590, 2, 1, 590, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
+  {0, 16, 2, 591, 2, 1, 591, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
+  {0, 16, 4, 592, 1, 1, 592, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
+  {0, 16, 8, 593, 1, 1, 593, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
+  {0, 16, 16, 600, 1, 1, 600, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
+  {0, 16, 32, 656, 1, 0, 657, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
+  {0, 16, 64, 659, 1, 1, 659, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
+  {0, 32, 1, 660, 2, 1, 660, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
+  {0, 32, 2, 661, 2, 1, 661, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
+  {0, 32, 4, 674, 2, 1, 674, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
+  {0, 32, 8, 682, 1, 1, 682, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
+  {0, 32, 16, 683, 1, 0, 684, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
+  {0, 32, 32, 699, 1, 0, 699, 0},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
+  {0, 32, 64, 743, 2, 0, 743, 0},  // tsm_atb_f64_32x64_mfma_r32_p2_nt
+  {0, 64, 1, 744, 2, 1, 744, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
+  {0, 64, 2, 745, 2, 1, 745, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
+  {0, 64, 4, 746, 2, 1, 746, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
+  {0, 64, 8, 747, 2, 1, 747, 1},  // tsm_atb_f64_64x8_t8x8_r32_p2_nt
+  {0, 64, 16, 749, 1, 1, 749, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
+  {0, 64, 32, 751, 1, 0, 751, 0},  // tsm_atb_f64_64x32_mfma_r32_p2_nt
+  {0, 64, 64, 754, 2, 0, 762, 0},  // tsm_atb_f64_64x64_direct_d16_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);

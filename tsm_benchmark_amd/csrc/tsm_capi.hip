@@ -186,9 +186,8 @@ tsm_status use_fallback(Plan* p, int64_t n, bool sym) {
 }
 
 bool valid_config(const tsm_kernel_config* c) {
-  // pf: 1 or 2 tiles in flight (staged kernels), 4, 8 or 16 steps (direct engine)
-  return !c || !((c->pf != 0 && c->pf != 1 && c->pf != 2 && c->pf != 4 && c->pf != 8 &&
-                  c->pf != 16) || c->tm < 0 || c->tn < 0 || c->rows < 0 ||
+  // pf: 1 to 4 tiles in flight (staged kernels), 4, 8 or 16 steps (direct engine)
+  return !c || !((c->pf < 0 || (c->pf > 4 && c->pf != 8 && c->pf != 16)) || c->tm < 0 || c->tn < 0 || c->rows < 0 ||
                  c->wgs_per_cu < 0 || c->grid < 0 || c->load_policy < 0 || c->load_policy > 2 ||
                  c->tile_order < 0 || c->tile_order > 2 || c->engine < 0 || c->engine > 3);
 }

@@ -343,9 +343,11 @@ __device__ inline void finish_outputs(const AtbArgs& a, int G, int KCMC, int MC,
 // KCMC a power of two with 2 <= KCMC <= 512.  The G x KCMC partials are read
 // as 16-byte chunks c = tid + 256 * j, so every thread always meets the same two outputs
 // (2 * tid mod KCMC and the next one) and all of its loads are independent.
-// They are issued in batches of U guarded loads, so a thread with up to U
-// chunks waits for one memory round trip (the partials were just written
-// through to memory, so each round trip is a full memory latency).  Each
+// They are issued in batches of U = 32 guarded loads, so a thread with up to
+// 32 chunks (the 256 workgroup partials of an 8 x 8 class) waits for one
+// memory round trip (the partials were just written through to memory, so
+// each round trip is a full memory latency).  The load registers are free
+// by then: the streaming loop's registers are dead.  Each
 // thread sums its chunks in j order; the 512 / KCMC threads that share a pair
 // of outputs are then added in thread order.  `scratch` holds 2 * kThreads
 // doubles of LDS.
@@ -356,7 +358,7 @@ __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, do
   const v4u* P = reinterpret_cast<const v4u*>(a.partials);
   const int64_t chunks = (int64_t)G * KCMC / 2;  // 16-byte chunks
   double s0 = 0.0, s1 = 0.0;
-  constexpr int U = 16;  // 64 VGPRs: keeps two waves per SIMD for the main loops
+  constexpr int U = 32;
   for (int64_t c = tid; c < chunks; c += U * kThreads) {
     v4u u[U];
 #pragma unroll
@@ -735,7 +737,21 @@ struct RegTileAcc {
       }
     }
   }
-  // Workgroup reduction into red[] (FP64 KC x MC in LDS); ends with a barrier.
+  // Workgroup reduction into red[] (FP64 in LDS); ends with a barrier, after
+  // which total(red, o) is the workgroup's sum for output o.  When four
+  // KC x MC slots fit in the two stages of A's tiles (a lower bound of the
+  // LDS of every kernel, A^T A included), each wave writes its slot at once
+  // and total() adds the slots in wave order (one barrier); otherwise the
+  // waves add into one slot in turn.  Both give ((w0 + w1) + w2) + w3.
+  static constexpr bool kSlots = 4 * KC * MC * 8 <= 2 * T * KC * (int)sizeof(S);
+  __device__ __forceinline__ double total(const double* red, int o) const {
+    if constexpr (kSlots) {
+      constexpr int K = KC * MC;
+      return ((red[o] + red[K + o]) + red[2 * K + o]) + red[3 * K + o];
+    } else {
+      return red[o];
+    }
+  }
   __device__ __forceinline__ void reduce(double* red, int tid) {
     constexpr int V = TM * TN;
     double v[V];
@@ -757,6 +773,18 @@ struct RegTileAcc {
     __syncthreads();  // every wave is done with the LDS tiles
     const bool writer = WR::writer(lane);
     const int vbase = WR::base(lane);
+    if constexpr (kSlots) {
+      if (writer) {
+#pragma unroll
+        for (int x = 0; x < WR::VL; ++x) {
+          const int idx = vbase + x;
+          const int o = (mb * TM + idx / TN) * MC + nb * TN + idx % TN;
+          red[wave * KC * MC + o] = v[x];
+        }
+      }
+      __syncthreads();
+      return;
+    }
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) {
       if (wave == w && writer) {
@@ -799,6 +827,7 @@ struct MfmaConsumer {
       fold_count = 0;
     }
   }
+  __device__ __forceinline__ double total(const double* red, int o) const { return red[o]; }
   __device__ __forceinline__ void reduce(double* red, int) {
     m.fold();
     __syncthreads();  // every wave is done with the LDS tiles
@@ -841,7 +870,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   constexpr int STAGE = A_BYTES + B_BYTES;
   constexpr int B_OFF = SYM ? 0 : A_BYTES;  // B's tile within a stage
   constexpr int KCMC = KC * MC;
-  static_assert(PF == 1 || PF == 2, "one or two tiles in flight");
+  static_assert(PF >= 1 && PF <= 4, "one to four tiles in flight");
   const int tid = threadIdx.x;
   typename ConsumerOf<S, KC, MC, TM, TN, T, SYM>::type cons;
   cons.zero(tid);
@@ -906,19 +935,18 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       cons.consume(stage, stage + B_OFF);
       cons.after_tile();
     };
-    Regs r0, r1;
-    load(r0, 0);
-    if constexpr (PF == 2) {
-      load(r1, 1 < last ? 1 : last);
-      int64_t t = 0;
-      for (; t + 1 < nfull; t += 2) {
-        step(r0, t);
-        step(r1, t + 1);
-      }
-      if (t < nfull) step(r0, t);
-    } else {
-      for (int64_t t = 0; t < nfull; ++t) step(r0, t);
+    // PF register sets, used in rotation: set p holds tiles p, p + PF, ...
+    Regs r[PF];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) load(r[p], p < last ? p : last);
+    int64_t t = 0;
+    for (; t + PF <= nfull; t += PF) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) step(r[p], t + p);
     }
+#pragma unroll
+    for (int p = 0; p < PF - 1; ++p)
+      if (t + p < nfull) step(r[p], t + p);
   }
 
   // ---- element loads (strided or misaligned inputs, the partial tile) -------------
@@ -953,7 +981,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   // ---- reduction -------------------------------------------------------------------
   double* red = reinterpret_cast<double*>(smem);
   cons.reduce(red, tid);
-  for (int o = tid; o < KCMC; o += kThreads) publish(partial_out + o, red[o]);
+  for (int o = tid; o < KCMC; o += kThreads) publish(partial_out + o, cons.total(red, o));
 }
 
 // Arrival at a problem's counter after its partial is stored: returns true
