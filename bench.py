@@ -1,16 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the tall-skinny product C = A^T B on one MI355X.
 
-Workload (the FP64 sweep at k = m = 8): A is n x 8 and B is n x 8, FP64,
-row-major, n = 2^25 so that A + B = 4 GiB, entries uniform on [-1, 1) from the
-counter-based generator (seed 20261203, generated on the device).  A step is
-one tsm_atb call (one kernel launch) over resident inputs.
+Headline workload (plain FP64): A is n x 8 and B is n x 8, FP64, row-major,
+n = 2^22 (A + B = 512 MiB), entries uniform on [-1, 1) from the counter-based
+generator (seed 20261203, generated on the device).  A step is one tsm_atb
+call (one kernel launch) over resident inputs.  Three such input sets (1.5 GiB,
+six times the 256 MB of last-level cache) are used in rotation, so no step
+reads data that the previous one left in the cache; the same measurement on
+one set is reported beside it.
 
 Prints ONE JSON line with GB/s (algorithmic bytes s*n*(k+m) + 8*k*m per
 call), the roofline of the kernel (achieved / 8 TB/s, timed with HIP events on
-the launch stream) and a CPU baseline: the reference's generated kernels
-restated for host cores (oracle/ref_kernels.cc) timed on a bounded sample of
-the same workload.
+the launch stream) and a CPU baseline: the reference's kernels restated for
+host cores (oracle/ref_kernels.cc) timed on the same n = 2^22 workload.
 """
 from __future__ import annotations
 
@@ -28,49 +30,114 @@ SEED = 20261203
 METRIC = "GB/s & % HBM peak, FP64 k=m=8; low-prec speedup at ≤1e-6 rel err vs FP64"
 
 
-def cpu_baseline(k: int, m: int, budget_s: float = 20.0) -> dict:
-    """Time the restated reference kernels (generated basic / _lf / _sg at the
-    generator's default stride 524288, local size 256, TM = TN = 4) on a
-    2^24-row sample of the workload; report the fastest variant."""
+def cpu_baseline(n: int, k: int, m: int, budget_s: float = 24.0) -> dict:
+    """Time the reference's kernels restated for host cores on the same n x 8
+    FP64 inputs (n = 2^22 for the headline):
+      - the generated basic / _lf / _sg kernels at the generator's default
+        stride 524288, local size 256, TM = TN = 4 (sycl_codegen_new.py:142-363);
+      - the hand-written tsmttsm4, tsmttsm6 and tsmttsm9 at main.cc's M = 8
+        configurations: stride 4096, 8192, 16384, local size 32, tile 2 or 4
+        (main.cc:103-105,171-177).
+    Each variant gets an equal share of the budget (at most 30 timed runs after
+    2 warmup runs, C zeroed outside the timer); the fastest is the value."""
     import numpy as np
 
     import oracle
 
-    rows = 1 << 24
-    A = oracle.gen_matrix(rows, k, seed=SEED, stream=1)
-    B = oracle.gen_matrix(rows, m, seed=SEED, stream=2)
-    best = None
-    per_variant = budget_s / 3
-    for name, variant in (("basic", oracle.REF_GEN_BASIC), ("lf", oracle.REF_GEN_LF),
-                          ("sg", oracle.REF_GEN_SG)):
-        cfg = oracle.ref_cfg(variant, tm=4, stride=524288, local_size=256, sg_size_gen=32,
-                             sg_size_rt=32)
+    A = oracle.gen_matrix(n, k, seed=SEED, stream=1)
+    B = oracle.gen_matrix(n, m, seed=SEED, stream=2)
+    cands = [(f"generated {nm} stride=524288 TM=TN=4",
+              oracle.ref_cfg(v, tm=4, stride=524288, local_size=256, sg_size_gen=32, sg_size_rt=32))
+             for nm, v in (("basic", oracle.REF_GEN_BASIC), ("_lf", oracle.REF_GEN_LF),
+                           ("_sg", oracle.REF_GEN_SG))]
+    if k == m:
+        for nm, v in (("tsmttsm4", oracle.REF_V4), ("tsmttsm6", oracle.REF_V6),
+                      ("tsmttsm9", oracle.REF_V9)):
+            for stride in (4096, 8192, 16384):
+                for tile in (2, 4):
+                    cands.append((f"{nm} stride={stride} local=32 tile={tile}",
+                                  oracle.ref_cfg(v, tm=tile, stride=stride, local_size=32)))
+    per_variant = budget_s / len(cands)
+    rows = []
+    for name, cfg in cands:
         times = oracle.ref_time(cfg, A, B, warmup=2, runs=30, budget_s=per_variant)
         t = float(np.mean(times)) * 1e-9
-        gbps = (8 * rows * (k + m) + 8 * k * m) / t / 1e9
-        if best is None or gbps > best[1]:
-            best = (name, gbps, t, len(times))
-    name, gbps, t, runs = best
+        rows.append((name, (8 * n * (k + m) + 8 * k * m) / t / 1e9, t, len(times)))
+    name, gbps, t, runs = max(rows, key=lambda r: r[1])
     return {"value": round(gbps, 2), "unit": "GB/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": (f"n=2^24 rows of the same FP64 k=m=8 workload; fastest of the restated "
-                       f"generated basic/_lf/_sg kernels (stride 524288, TM=TN=4): {name}, "
-                       f"{runs} timed runs, mean {t * 1e3:.1f} ms")}
+            "sample": (f"the full n=2^22 FP64 k=m=8 workload (one input set); fastest of "
+                       f"{len(rows)} restated reference kernels: {name}, {runs} timed runs, "
+                       f"mean {t * 1e3:.2f} ms"),
+            "variants_GBps": {r[0]: round(r[1], 2) for r in rows}}
 
 
-def secondary() -> dict:
-    """The metric's second half and the other workloads, measured like the
-    headline (back-to-back calls between HIP events, inputs resident):
+def fp64_rate(n: int, k: int, m: int, sets: int, steps: int, warmup: int) -> dict:
+    """GB/s of tsm_atb over `sets` FP64 input sets used in rotation: `warmup`
+    untimed steps, then `steps` steps between two HIP events on the launch
+    stream (one kernel launch per step)."""
+    import torch
+
+    from tsm_benchmark_amd import _lib, ops
+
+    dev = torch.device("cuda", 0)
+    data = [(ops.random_matrix(n, k, torch.float64, seed=SEED + i, stream_id=1, device=dev),
+             ops.random_matrix(n, m, torch.float64, seed=SEED + i, stream_id=2, device=dev))
+            for i in range(sets)]
+    C = torch.empty((k, m), dtype=torch.float64, device=dev)
+    L = _lib.lib()
+    ws_bytes = L.tsm_atb_workspace_size(_lib.F64, n, k, m, None)
+    ws = torch.zeros(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.Stream(device=dev)
+    sh = stream.cuda_stream
+    ptrs = [(A.data_ptr(), B.data_ptr()) for A, B in data]
+    torch.cuda.synchronize()
+
+    def step(i):
+        pa, pb = ptrs[i % sets]
+        st = L.tsm_atb(_lib.F64, _lib.F64, n, k, m, pa, k, pb, m, C.data_ptr(), m, 0,
+                       ws.data_ptr(), ws_bytes, sh)
+        if st != _lib.SUCCESS:
+            raise _lib.TsmError(st, "tsm_atb")
+
+    for i in range(warmup):
+        step(i)
+    stream.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(steps):
+        step(warmup + i)
+    e1.record(stream)
+    e1.synchronize()
+    t_step = e0.elapsed_time(e1) * 1e-3 / steps
+    algo_bytes = 8 * n * (k + m) + 8 * k * m
+    del data
+    torch.cuda.empty_cache()
+    return {"t_step": t_step, "GBps": algo_bytes / t_step / 1e9, "algo_bytes": algo_bytes}
+
+
+def secondary(n: int, k: int, m: int, steps: int, warmup: int) -> dict:
+    """The metric's second half and the other workloads:
+      resident_one_set: the headline workload on one input set (the data of
+        the previous step may partly be in the last-level cache);
+      fp64_k8_n2^25: FP64 k = m = 8 at n = 2^25 (A + B = 4 GiB), one set;
       low_precision: k = m = 32, n = 2^25, FP32 and BF16 storage vs FP64 - time,
         speedup and relative error against the FP64 kernel on the same rounded
         inputs (uniform and sign*2^e entries);
-      backward: FP32 n = 2^23, k = m = 16, (forward + backward) / forward;
+      backward: FP32 n = 2^23, k = m = 16, (forward + backward) / forward with
+        both gradients (byte bound 3.0) and with dA only (byte bound 2.0);
       batched: 512 pairs in one launch vs one launch per pair (FP64, FP32);
       siblings: C = A^T A against tsm_atb(A, A), and Y = A X;
       matrix_cores: FP64 and FP32 k = m = 64 (A+B = 4 GiB), GFLOP/s and the
-        fraction of the dense matrix peak (78.6 / 157.3 TFLOP/s)."""
+        fraction of the dense matrix peak (78.6 / 157.3 TFLOP/s).
+    The first two are timed like the headline; the others as back-to-back
+    calls between HIP events after 0.1 s of warmup calls."""
     from tsm_benchmark_amd import suite
 
+    one = fp64_rate(n, k, m, 1, steps, warmup)
+    big_n = 1 << 25
+    big = fp64_rate(big_n, k, m, 1, steps, warmup)
     prec = suite.precision(10, False)
     low = {"n": prec["n"], "k": prec["k"], "m": prec["m"]}
     for dist in ("uniform", "pow2"):
@@ -84,9 +151,19 @@ def secondary() -> dict:
     sib = suite.siblings(10)
     mat = suite.matrix(10)
     return {
+        "resident_one_set": {"GBps": round(one["GBps"], 2),
+                             "ms_per_step": round(one["t_step"] * 1e3, 4),
+                             "hbm_frac": round(one["GBps"] / HBM_PEAK_GBPS, 4)},
+        "fp64_k8_n2^25": {"n": big_n, "GBps": round(big["GBps"], 2),
+                          "ms_per_step": round(big["t_step"] * 1e3, 4),
+                          "hbm_frac": round(big["GBps"] / HBM_PEAK_GBPS, 4)},
         "low_precision": low,
-        "backward": {key: bwd[key] for key in ("fwd_us", "bwd_us", "fwd_plus_bwd_over_fwd",
-                                               "byte_bound_ratio")},
+        "backward": {
+            **{key: bwd[key] for key in ("n", "k", "m", "fwd_us", "bwd_us", "bwd_dA_only_us",
+                                         "fwd_plus_bwd_over_fwd", "byte_bound_ratio")},
+            "fwd_plus_bwd_dA_only_over_fwd": round(
+                (bwd["fwd_us"] + bwd["bwd_dA_only_us"]) / bwd["fwd_us"], 3),
+            "byte_bound_ratio_dA_only": 2.0},
         "batched": {dt: {"one_launch_GBps": bat[dt]["graph_GBps"],
                          "per_pair_launches_GBps": bat[dt]["per_pair_GBps"]}
                     for dt in ("f64", "f32")},
@@ -109,12 +186,13 @@ def pmc_traffic(workload: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    # about 30 ms: the first passes over freshly allocated inputs run slower
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--n", type=int, default=1 << 25)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--m", type=int, default=8)
+    ap.add_argument("--sets", type=int, default=3,
+                    help="input sets used in rotation (3 x 512 MiB exceeds the caches)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true",
                     help="skip the low-precision / backward / batched measurements")
@@ -126,42 +204,12 @@ def main() -> int:
 
     import torch
 
-    from tsm_benchmark_amd import _lib, ops
+    from tsm_benchmark_amd import _lib
 
     n, k, m = args.n, args.k, args.m
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    A = ops.random_matrix(n, k, torch.float64, seed=SEED, stream_id=1, device=dev)
-    B = ops.random_matrix(n, m, torch.float64, seed=SEED, stream_id=2, device=dev)
-    C = torch.empty((k, m), dtype=torch.float64, device=dev)
-    L = _lib.lib()
-    ws_bytes = L.tsm_atb_workspace_size(_lib.F64, n, k, m, None)
-    ws = torch.zeros(ws_bytes, dtype=torch.uint8, device=dev)
-    stream = torch.cuda.Stream(device=dev)
-    sh = stream.cuda_stream
-    torch.cuda.synchronize()
-
-    def step():
-        st = L.tsm_atb(_lib.F64, _lib.F64, n, k, m, A.data_ptr(), k, B.data_ptr(), m,
-                       C.data_ptr(), m, 0, ws.data_ptr(), ws_bytes, sh)
-        if st != _lib.SUCCESS:
-            raise _lib.TsmError(st, "tsm_atb")
-
-    with torch.cuda.stream(stream):
-        for _ in range(args.warmup):
-            step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    t_step = e0.elapsed_time(e1) * 1e-3 / args.steps
-
-    algo_bytes = 8 * n * (k + m) + 8 * k * m
-    gbps = algo_bytes / t_step / 1e9
+    torch.cuda.set_device(0)
+    head = fp64_rate(n, k, m, args.sets, args.steps, args.warmup)
+    gbps, t_step = head["GBps"], head["t_step"]
     idx, grid = _lib.plan(_lib.F64, n, k, m)
     kname = _lib.kernels()[idx]["name"]
     workload = f"fp64_atb_k{k}_m{m}_n{n}"
@@ -177,21 +225,20 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: counter-based uniform[-1,1) (seed 20261203), generated on the device",
+        "data": "synthetic: counter-based uniform[-1,1) (seed 20261203 + set), generated on the device",
         "config": {"workload": workload, "n": n, "k": k, "m": m, "layout": "row-major",
-                   "A_plus_B_bytes": 8 * n * (k + m), "kernel": kname, "grid": grid},
+                   "A_plus_B_bytes": 8 * n * (k + m), "input_sets_rotated": args.sets,
+                   "kernel": kname, "grid": grid},
         "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
                      "traffic": pmc_traffic(workload)},
     }
-    del A, B
-    torch.cuda.empty_cache()
     if not args.no_secondary:
-        rec["secondary"] = secondary()
+        rec["secondary"] = secondary(n, k, m, args.steps, args.warmup)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         t0 = time.time()
-        rec["cpu_baseline"] = cpu_baseline(k, m)
+        rec["cpu_baseline"] = cpu_baseline(n, k, m)
         rec["cpu_baseline"]["wall_s"] = round(time.time() - t0, 1)
     else:
         rec["cpu_baseline"] = None

@@ -44,7 +44,8 @@ typedef enum {
   TSM_ERR_INVALID_ARGUMENT = 1, /* null pointer, n < 0, k or m outside [1,64], ld too small */
   TSM_ERR_UNSUPPORTED = 2,      /* dtype combination or kernel request not provided */
   TSM_ERR_MISALIGNED = 3,       /* pointer not aligned to its element size */
-  TSM_ERR_WORKSPACE = 4,        /* caller workspace smaller than tsm_atb_workspace_size() */
+  TSM_ERR_WORKSPACE = 4,        /* caller workspace smaller than tsm_atb_workspace_size(), or
+                                   none given on a stream that is being captured */
   TSM_ERR_HIP = 5,              /* a HIP runtime call failed: see tsm_last_hip_error() */
   TSM_ERR_NO_KERNEL = 6         /* no generated kernel matches the requested config */
 } tsm_status;

@@ -174,6 +174,10 @@ def test_kernel_info_bounds():
     assert L.tsm_kernel_info(L.tsm_num_kernels(), ct.byref(d)) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_kernel_info(0, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_atb_workspace_size(_lib.F64, -1, 8, 8, None) == 0
+    # a kernel request that tsm_atb_ex would reject has no workspace size either
+    for bad in (_lib.make_config(pf=5), _lib.make_config(engine=9), _lib.make_config(rows=-1)):
+        assert L.tsm_atb_workspace_size(_lib.F64, 10, 8, 8, ct.byref(bad)) == 0
+        assert L.tsm_ata_workspace_size(_lib.F64, 10, 8, ct.byref(bad)) == 0
 
 
 def test_batched_argument_errors():

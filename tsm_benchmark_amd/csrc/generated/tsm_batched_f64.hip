@@ -34,7 +34,7 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 21: tsm::batched_item_call<double, 8, 1, 8, 1, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x1_t8x1_r512_p2_nt
       case 22: tsm::batched_item_call<double, 8, 2, 8, 2, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x2_t8x2_r512_p2_nt
       case 23: tsm::batched_item_call<double, 8, 4, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
-      case 24: tsm::batched_item_call<double, 8, 8, 8, 8, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x8_t8x8_r256_p2_nt
+      case 24: tsm::batched_item_call<double, 8, 8, 4, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x8_t4x4_r256_p2_nt
       case 25: tsm::batched_item_call<double, 8, 16, 8, 8, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
       case 26: tsm::batched_item_call<double, 8, 32, 8, 8, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x32_t8x8_r128_p2_nt
       case 27: tsm::batched_item_call<double, 8, 64, 8, 8, 64, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f64_8x64_t8x8_r64_p2_nt

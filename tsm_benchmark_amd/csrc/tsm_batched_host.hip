@@ -216,9 +216,14 @@ std::map<hipStream_t, Staging>& stagings() {
 }
 
 tsm_status upload(const Layout& L, void* dst, hipStream_t s) {
+  // A captured copy would replay from the staging buffer, which later calls
+  // overwrite: a graph takes a plan (tsm_batched_plan_execute) instead.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &cs);
+  if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  if (cs != hipStreamCaptureStatusNone) return TSM_ERR_UNSUPPORTED;
   std::lock_guard<std::mutex> lock(g_staging_mu);
   Staging& st = stagings()[s];
-  hipError_t e;
   if (st.done) {
     e = hipEventSynchronize(st.done);
     if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
@@ -248,6 +253,10 @@ struct tsm_batched_plan_s {
   tsm::BatchArgs args;
   unsigned grid;
   const tsm::BatchedKernel* kernel;
+  // One event per stream the plan ran on (outside graph capture), recorded
+  // after each launch: destroy waits for these launches only.
+  std::mutex mu;
+  std::map<hipStream_t, hipEvent_t> launched;
 };
 
 extern "C" {
@@ -302,13 +311,16 @@ tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int co
     (void)hipFree(buf);
     return (tsm_status)tsm_internal_hip_status(e);
   }
-  auto* p = new (std::nothrow) tsm_batched_plan_s{in_type, buf,
-                                                  make_args(L, buf, accumulate, out_type), grid,
-                                                  batched_kernel(in_type)};
+  auto* p = new (std::nothrow) tsm_batched_plan_s;
   if (!p) {
     (void)hipFree(buf);
     return TSM_ERR_INVALID_ARGUMENT;
   }
+  p->in_type = in_type;
+  p->buffer = buf;
+  p->args = make_args(L, buf, accumulate, out_type);
+  p->grid = grid;
+  p->kernel = batched_kernel(in_type);
   *plan = p;
   return TSM_SUCCESS;
 }
@@ -316,8 +328,26 @@ tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int co
 tsm_status tsm_batched_plan_execute(tsm_batched_plan plan, tsm_stream_t stream) {
   if (!plan) return TSM_ERR_INVALID_ARGUMENT;
   if (plan->args.n_items == 0) return TSM_SUCCESS;
-  return (tsm_status)tsm_internal_hip_status(
-      plan->kernel->launch(plan->args, plan->grid, static_cast<hipStream_t>(stream)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = plan->kernel->launch(plan->args, plan->grid, s);
+  if (e != hipSuccess) return (tsm_status)tsm_internal_hip_status(e);
+  // A launch captured into a graph runs when the graph does: the caller keeps
+  // the plan alive as long as such a graph may replay.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  e = hipStreamIsCapturing(s, &cs);
+  if (e != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return (tsm_status)tsm_internal_hip_status(e);
+  std::lock_guard<std::mutex> lock(plan->mu);
+  hipEvent_t& ev = plan->launched[s];
+  if (!ev) {
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      ev = nullptr;
+      plan->launched.erase(s);
+      return (tsm_status)tsm_internal_hip_status(e);
+    }
+  }
+  return (tsm_status)tsm_internal_hip_status(hipEventRecord(ev, s));
 }
 
 tsm_status tsm_batched_plan_info(tsm_batched_plan plan, int64_t* items, int64_t* grid) {
@@ -329,8 +359,14 @@ tsm_status tsm_batched_plan_info(tsm_batched_plan plan, int64_t* items, int64_t*
 
 tsm_status tsm_batched_plan_destroy(tsm_batched_plan plan) {
   if (!plan) return TSM_ERR_INVALID_ARGUMENT;
-  // Work that uses the buffer may still be queued: wait for the device.
-  hipError_t e = hipDeviceSynchronize();
+  // The plan's own launches may still be queued: wait for each stream's last
+  // one (not for the whole device).
+  hipError_t e = hipSuccess;
+  for (auto& kv : plan->launched) {
+    const hipError_t w = hipEventSynchronize(kv.second);
+    if (e == hipSuccess) e = w;
+    (void)hipEventDestroy(kv.second);
+  }
   const hipError_t f = hipFree(plan->buffer);
   delete plan;
   if (e == hipSuccess) e = f;

@@ -267,8 +267,15 @@ Pool& pool() {
 }
 
 tsm_status pooled_workspace(size_t bytes, hipStream_t stream, void** out) {
+  // A graph captured now would keep the pool's buffer, which a later, larger
+  // call on this stream frees (and the first call allocates and clears it,
+  // which has no place in a graph): capture needs a caller workspace.
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(stream, &cs);
+  if (e != hipSuccess) return hip_fail(e);
+  if (cs != hipStreamCaptureStatusNone) return TSM_ERR_WORKSPACE;
   int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
+  e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e);
   Pool& P = pool();
   std::lock_guard<std::mutex> lock(P.mu);
@@ -354,7 +361,7 @@ extern "C" {
 size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
                               const tsm_kernel_config* config) {
   if (!valid_in_type(in_type) || n < 0 || k < 1 || m < 1 || k > TSM_MAX_WIDTH ||
-      m > TSM_MAX_WIDTH)
+      m > TSM_MAX_WIDTH || !valid_config(config))
     return 0;
   Plan p;
   if (make_plan(in_type, n, k, m, config, &p) != TSM_SUCCESS) return 0;
@@ -512,7 +519,8 @@ const char* tsm_strerror(tsm_status s) {
     case TSM_ERR_INVALID_ARGUMENT: return "invalid argument";
     case TSM_ERR_UNSUPPORTED: return "unsupported data type or request";
     case TSM_ERR_MISALIGNED: return "misaligned pointer";
-    case TSM_ERR_WORKSPACE: return "workspace too small";
+    case TSM_ERR_WORKSPACE:
+      return "workspace too small, or none given for a call on a capturing stream";
     case TSM_ERR_HIP: return "HIP runtime error";
     case TSM_ERR_NO_KERNEL: return "no generated kernel matches the configuration";
   }

@@ -87,6 +87,8 @@ def atb(A: torch.Tensor, B: torch.Tensor, *, out_dtype: torch.dtype | None = Non
         out = torch.empty((k, m), dtype=out_dtype, device=A.device)
     if out.dtype not in _OUT_TYPES or out.shape != (k, m) or out.stride(1) != 1:
         raise ValueError("out must be a row-major k x m float64/float32 tensor")
+    if out.device != A.device:
+        raise ValueError("out must be on the inputs' device")
     L = _lib.lib()
     ws_ptr, ws_bytes = _ws(workspace, A.device)
     with torch.cuda.device(A.device):
@@ -115,6 +117,8 @@ def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
         out = torch.empty((k, k), dtype=out_dtype, device=A.device)
     if out.dtype not in _OUT_TYPES or out.shape != (k, k) or out.stride(1) != 1:
         raise ValueError("out must be a row-major k x k float64/float32 tensor")
+    if out.device != A.device:
+        raise ValueError("out must be on the input's device")
     L = _lib.lib()
     ws_ptr, ws_bytes = _ws(workspace, A.device)
     with torch.cuda.device(A.device):
@@ -149,8 +153,12 @@ def atb_backward(A: torch.Tensor, B: torch.Tensor, dC: torch.Tensor, *,
     """(dA, dB) = (B dC^T, A dC) in the dtype of A and B; either may be skipped."""
     _check_tall(A, "A")
     _check_tall(B, "B")
+    if A.dtype != B.dtype or A.shape[0] != B.shape[0] or A.device != B.device:
+        raise ValueError("A and B must share dtype, row count and device")
     n, k = A.shape
     m = B.shape[1]
+    if not isinstance(dC, torch.Tensor) or dC.device != A.device:
+        raise ValueError("dC must be a tensor on the inputs' device")
     dC = dC.contiguous()
     if dC.shape != (k, m) or dC.dtype not in _OUT_TYPES:
         raise ValueError("dC must be k x m float64/float32")
@@ -195,6 +203,8 @@ def _problems(As, Bs, Cs):
         _check_tall(B, "B")
         if A.dtype != dtype or B.dtype != dtype or A.shape[0] != B.shape[0]:
             raise ValueError(f"problem {i}: A and B must share the batch dtype and row count")
+        if A.device != As[0].device or B.device != A.device or C.device != A.device:
+            raise ValueError(f"problem {i}: A, B and out must be on the batch's device")
         k, m = A.shape[1], B.shape[1]
         if C.shape != (k, m) or C.dtype != Cs[0].dtype or C.dtype not in _OUT_TYPES \
                 or C.stride(1) != 1:
