@@ -114,6 +114,26 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
                       void* workspace, size_t workspace_bytes, tsm_stream_t stream,
                       const tsm_kernel_config* config);
 
+/* ---- an entry point that allocates nothing --------------------------------
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  As tsm_atb with the tuned default kernel and no
+ * workspace argument: the workspace is a static device array of
+ * TSM_NOALLOC_SCRATCH_BYTES in the library (one per GPU, held until the
+ * process ends), never a caller buffer or the per-stream pool, so the call
+ * makes no allocation.  A shape whose default plan needs more
+ * (tsm_atb_workspace_size() > TSM_NOALLOC_SCRATCH_BYTES; with 16 MiB no
+ * default plan does on a 256-CU MI355X) returns TSM_ERR_WORKSPACE.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other host threads) race, so order them on one stream.
+ * The first call on a device clears the array's arrival counters with a
+ * kernel on `stream` and returns TSM_ERR_UNSUPPORTED if `stream` is being
+ * captured; later calls may be captured. */
+#define TSM_NOALLOC_SCRATCH_BYTES ((size_t)256 + ((size_t)16 << 20))
+tsm_status tsm_atb_noalloc(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                           const void* A, int64_t lda, const void* B, int64_t ldb,
+                           void* C, int64_t ldc, int accumulate, tsm_stream_t stream);
+
 /* Workspace bytes tsm_atb / tsm_atb_ex need for this shape (config may be NULL). */
 size_t tsm_atb_workspace_size(tsm_dtype in_type, int64_t n, int k, int m,
                               const tsm_kernel_config* config);

@@ -26,8 +26,10 @@ EXPORTS = (
     "tsm_strerror", "tsm_last_hip_error", "tsm_version", "tsm_num_kernels",
     "tsm_kernel_info", "tsm_atb_plan", "tsm_batched_plan_create", "tsm_batched_plan_execute",
     "tsm_batched_plan_info", "tsm_batched_plan_destroy", "tsm_ata", "tsm_ata_ex", "tsm_ata_workspace_size",
-    "tsm_tsmm",
+    "tsm_tsmm", "tsm_atb_noalloc",
 )
+# include/tsm/tsm.h TSM_NOALLOC_SCRATCH_BYTES: the static workspace of tsm_atb_noalloc
+NOALLOC_SCRATCH_BYTES = 256 + (16 << 20)
 
 
 class KernelConfig(ct.Structure):
@@ -88,6 +90,8 @@ def lib() -> ct.CDLL:
     L.tsm_atb_ex.restype = I
     L.tsm_atb_ex.argtypes = [I, I, I64, I, I, P, I64, P, I64, P, I64, I, P, SZ, P,
                              ct.POINTER(KernelConfig)]
+    L.tsm_atb_noalloc.restype = I
+    L.tsm_atb_noalloc.argtypes = [I, I, I64, I, I, P, I64, P, I64, P, I64, I, P]
     L.tsm_atb_workspace_size.restype = SZ
     L.tsm_atb_workspace_size.argtypes = [I, I64, I, I, ct.POINTER(KernelConfig)]
     L.tsm_atb_backward.restype = I

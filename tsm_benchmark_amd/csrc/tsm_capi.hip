@@ -304,6 +304,52 @@ tsm_status pooled_workspace(size_t bytes, hipStream_t stream, void** out) {
   return TSM_SUCCESS;
 }
 
+// ---- the workspace of tsm_atb_noalloc: a static array per GPU --------------------
+// Arrival counters (the first 256 bytes) and workgroup partials of every
+// default plan fit in it.  In host code `scratch` names a host-side stand-in,
+// not device memory: host code uses only the per-device address that
+// hipGetSymbolAddress returns for it.
+static __device__ __align__(256) unsigned char scratch[TSM_NOALLOC_SCRATCH_BYTES];
+
+__global__ void tsm_scratch_clear(unsigned int* p, int words) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < words) p[i] = 0u;
+}
+
+// Device address of the current device's copy of `scratch`, looked up once
+// per device with that device current.  The lookup also clears the arrival
+// counters with a kernel on `stream` (the kernels leave them zero after every
+// call), so it is refused on a stream that is being captured.
+tsm_status scratch_for_device(hipStream_t stream, void** out) {
+  static std::mutex mu;
+  static std::map<int, void*> ptrs;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) {
+    *out = it->second;
+    return TSM_SUCCESS;
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  e = hipStreamIsCapturing(stream, &cs);
+  if (e != hipSuccess) return hip_fail(e);
+  if (cs != hipStreamCaptureStatusNone) return TSM_ERR_UNSUPPORTED;
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return hip_fail(e);
+  if (!ptr || (uintptr_t)ptr % kCounterBytes) return TSM_ERR_MISALIGNED;
+  const int words = (int)(kCounterBytes / sizeof(unsigned int));
+  hipLaunchKernelGGL(tsm_scratch_clear, dim3(1), dim3(words), 0, stream,
+                     static_cast<unsigned int*>(ptr), words);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e);
+  ptrs[dev] = ptr;
+  *out = ptr;
+  return TSM_SUCCESS;
+}
+
 }  // namespace
 
 // ---- finisher kernel for wide classes (KC*MC > 1024) -----------------------------------
@@ -401,6 +447,30 @@ tsm_status tsm_atb_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, i
   return run_plan(p, ke.launch, ke.kc, ke.mc, out_type, n, k, m, A, lda, B, ldb, C, ldc,
                   accumulate, dense, workspace, workspace_bytes,
                   static_cast<hipStream_t>(stream));
+}
+
+tsm_status tsm_atb_noalloc(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
+                           const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                           int64_t ldc, int accumulate, tsm_stream_t stream) {
+  tsm_status st = check_atb(in_type, out_type, n, k, m, A, lda, B, ldb, C, ldc);
+  if (st != TSM_SUCCESS) return st;
+  Plan p;
+  st = make_plan(in_type, n, k, m, nullptr, &p);
+  if (st != TSM_SUCCESS) return st;
+  const bool dense = k == p.entry->kc && m == p.entry->mc && lda == k && ldb == m &&
+                     ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0;
+  if (!dense) {
+    st = use_fallback(&p, n, false);
+    if (st != TSM_SUCCESS) return st;
+  }
+  if (p.workspace > TSM_NOALLOC_SCRATCH_BYTES) return TSM_ERR_WORKSPACE;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  void* ws = nullptr;
+  st = scratch_for_device(s, &ws);
+  if (st != TSM_SUCCESS) return st;
+  const tsm::KernelEntry& ke = *p.entry;
+  return run_plan(p, ke.launch, ke.kc, ke.mc, out_type, n, k, m, A, lda, B, ldb, C, ldc,
+                  accumulate, dense, ws, TSM_NOALLOC_SCRATCH_BYTES, s);
 }
 
 }  // extern "C"
