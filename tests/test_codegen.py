@@ -79,7 +79,7 @@ def test_cli_rejects_inconsistent_flags():
     assert run_cli("--sg-size", "32").returncode == 1
     assert run_cli("--unroll", "0").returncode == 1
     r = run_cli("--M", "8", "--tile-sizes", "4", "--stride", "1000")
-    assert r.returncode == 1 and "divisible" in r.stderr
+    assert r.returncode == 1 and "whole number of work-groups" in r.stderr
 
 
 def test_cli_name_layer():

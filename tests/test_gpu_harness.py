@@ -24,17 +24,37 @@ def run(*args):
     return rows
 
 
+def speed_ratios(rows, base_of):
+    """min-time GB/s of each reference name over the library entry point's on
+    the same problem (base_of maps a row name to its tsm_atb row name)."""
+    by = {row["kernel"]: row for row in rows}
+    return {name: by[base_of(name)]["min_ms"] / row["min_ms"]
+            for name, row in by.items() if not name.startswith("tsm_atb")}
+
+
 def test_main_suite():
-    """tsmttsm4 / 6 / 9 over main.cc's 18 configurations (M in {4, 8, 16})."""
-    rows = run("--suite", "main", "--K", "65536", "--warmup", "1", "--runs", "2")
-    assert len(rows) == 3 * 18
+    """Every hand-written name (tsmttsm, tsmttsm2 ... tsmttsm9) over main.cc's
+    18 configurations (M in {4, 8, 16}) at main.cc's K = 2^20 rows: correct,
+    and at least 90% of the library entry point's GB/s on the same problem."""
+    rows = run("--suite", "main", "--K", str(1 << 20), "--warmup", "5", "--runs", "30")
+    assert len(rows) == 10 * 18
+    ratios = speed_ratios(rows, lambda name: "tsm_atb_" + name.split("_", 1)[1])
+    assert len(ratios) == 9 * 18
+    slow = {k: round(v, 3) for k, v in ratios.items() if v < 0.9}
+    assert not slow, slow
 
 
 def test_generated_suite():
-    """Every generated reference name tsmttsm_{M}_{M}_{TM}_{TM}[_lf|_sg|_sg_lf]."""
-    rows = run("--suite", "generated", "--K", "65536", "--warmup", "1", "--runs", "2")
+    """Every generated reference name tsmttsm_{M}_{M}_{TM}_{TM}[_lf|_sg|_sg_lf]
+    at K = 2^22 rows: correct, and at least 90% of the library entry point's
+    GB/s on the same problem."""
+    rows = run("--suite", "generated", "--K", str(1 << 22), "--warmup", "5", "--runs", "30")
     names = {row["kernel"] for row in rows}
     assert "tsmttsm_8_8_4_4" in names and "tsmttsm_16_16_2_2_sg_lf" in names
+    ratios = speed_ratios(rows, lambda name: "tsm_atb_M" + name.split("_")[1])
+    assert len(ratios) == 24
+    slow = {k: round(v, 3) for k, v in ratios.items() if v < 0.9}
+    assert not slow, slow
 
 
 @pytest.mark.parametrize("dtype,M,N", [("f64", 64, 64), ("f32", 64, 64), ("bf16", 64, 64),

@@ -698,11 +698,13 @@ def generate_library(out_dir: str = GEN_DIR, tuning_path: str = TUNING_JSON) -> 
 
 # ---- name-compatible layer (the reference's generated header role) ----------
 def emit_name_layer(Ms, tiles, stride, unroll, with_benchmark: bool) -> str:
-    """The reference's kernel names as inline wrappers over tsm_atb_ex.
+    """The reference's kernel names as inline wrappers over the tuned launch.
 
     Each wrapper keeps the reference's signature shape ``(queue, K, A, B, C)``
     and contract ``C += A^T B`` (sycl_codegen_new.py:158-159, 218-225); the queue
-    becomes a stream and the returned value a tsm_status."""
+    becomes a stream and the returned value a tsm_status.  The register tile,
+    leap-frog depth, stride and unroll of each name are recorded in
+    ``kGeneratedKernels``; the launch is the library's tuned one."""
     lines = ["#pragma once", "// GENERATED by tsm_benchmark_amd/codegen.py --name-layer",
              f"// M={list(Ms)} tile-sizes={list(tiles)} stride={stride} unroll={unroll}",
              "#include \"tsm/tsmttsm.hh\"", "", "namespace tsm_compat {", ""]
@@ -711,27 +713,31 @@ def emit_name_layer(Ms, tiles, stride, unroll, with_benchmark: bool) -> str:
         for TM in tiles:
             if M % TM:
                 continue
-            for suffix, pf in (("", 1), ("_lf", 2), ("_sg", 1), ("_sg_lf", 2)):
+            for suffix, lf, sg in (("", 0, 0), ("_lf", 1, 0), ("_sg", 0, 1), ("_sg_lf", 1, 1)):
                 name = f"tsmttsm_{M}_{M}_{TM}_{TM}{suffix}"
-                names.append((name, M))
+                names.append((name, M, TM, lf, sg))
                 lines += [
                     "template <typename T>",
                     f"inline tsm_status {name}(tsm_stream_t q, int64_t K, const T* A, const T* B, T* C) {{",
-                    f"  return tsm_compat::generated_call<T>(q, K, {M}, {M}, A, B, C, {TM}, {TM}, {pf}, "
-                    f"{stride}, {unroll});",
+                    f"  return tsm_compat::generated_call<T>(q, K, {M}, {M}, A, B, C);",
                     "}", ""]
     if with_benchmark:
         lines += ["// Every generated name with its FP64 instance, in generation order (the",
                   "// generated benchmark's list, sycl_codegen_new.py:478-487).",
+                  "// The reference's decomposition of each name is recorded (tile TM x TM,",
+                  "// leap-frog prefetch, sub-group reduction, stride, unroll).",
                   "struct GeneratedKernel {",
                   "  const char* name;",
-                  "  int M;",
+                  "  int M, TM, leap_frog, sub_group;",
+                  "  long long stride;",
+                  "  int unroll;",
                   "  tsm_status (*fn)(tsm_stream_t, int64_t, const double*, const double*, double*);",
                   "};",
                   "inline const GeneratedKernel kGeneratedKernels[] = {"]
-        lines += [f"  {{\"{n}\", {M}, &{n}<double>}}," for n, M in names]
+        lines += [f"  {{\"{n}\", {M}, {TM}, {lf}, {sg}, {stride}, {unroll}, &{n}<double>}},"
+                  for n, M, TM, lf, sg in names]
         lines += ["};", "inline const char* const kGeneratedNames[] = {"]
-        lines += [f"  \"{n}\"," for n, _ in names]
+        lines += [f"  \"{n}\"," for n, *_ in names]
         lines += ["};", ""]
     lines += ["}  // namespace tsm_compat", ""]
     return "\n".join(lines)
@@ -759,18 +765,19 @@ def validate_stride(Ms, tiles, stride, local_size=THREADS, sg_size=WAVE) -> list
     """validate_stride (sycl_codegen_new.py:530-560); returns the error lines."""
     errs = []
     if stride % local_size:
-        return [f"Error: --stride {stride} must be divisible by --local-size {local_size}"]
+        return [f"error: the stride {stride} is not a whole number of work-groups of "
+                f"{local_size} items"]
     for M in Ms:
         for TM in tiles:
             if M % TM:
                 continue
             nt = (M // TM) ** 2
             if stride % nt:
-                errs.append(f"Warning: stride ({stride}) not divisible by num_tiles ({nt}) "
-                            f"for M={M}, TM={TM}")
+                errs.append(f"note: M={M}, TM={TM} has {nt} register tiles, which do not "
+                            f"split the stride {stride} evenly")
             if (stride // sg_size) % nt:
-                errs.append(f"Warning: stride/sg_size ({stride // sg_size}) not divisible by "
-                            f"num_tiles ({nt}) for M={M}, TM={TM}")
+                errs.append(f"note: M={M}, TM={TM} has {nt} register tiles, which do not "
+                            f"split the {stride // sg_size} sub-groups of the stride evenly")
     return errs
 
 

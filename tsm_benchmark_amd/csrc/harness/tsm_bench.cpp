@@ -20,7 +20,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <iterator>
 #include <numeric>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -128,10 +130,10 @@ struct Problem {
     }
   }
   ~Problem() {
-    hipFree(A);
-    hipFree(B);
-    hipFree(C);
-    hipStreamDestroy(s);
+    (void)hipFree(A);
+    (void)hipFree(B);
+    (void)hipFree(C);
+    (void)hipStreamDestroy(s);
   }
   std::vector<double> to_host(const void* d, size_t count) const {
     std::vector<double> out(count);
@@ -191,8 +193,8 @@ Row run(Problem& P, const std::string& name, const std::function<tsm_status()>& 
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
     t[i] = ms * 1e6;
   }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   const double sum = std::accumulate(t.begin(), t.end(), 0.0);
   const double avg = sum / runs;
   const double sq = std::inner_product(t.begin(), t.end(), t.begin(), 0.0);
@@ -228,7 +230,9 @@ void print(const std::vector<Row>& rows, bool json) {
   }
 }
 
-// main.cc:171-192: (M, stride, local, tile) for variants 4, 6 and 9.
+// main.cc:171-192: (M, stride, local, tile).  The reference runs variants 4,
+// 6 and 9 there; every hand-written name (tsmttsm.hh:15-586) runs here, with
+// the library entry point on the same problem as the speed reference.
 template <int M, int stride, int local, int tile>
 void main_suite_one(std::vector<Row>& rows, int64_t K, int warmup, int runs) {
   Problem P(K, M, M, TSM_F64);
@@ -236,15 +240,22 @@ void main_suite_one(std::vector<Row>& rows, int64_t K, int warmup, int runs) {
   const double* B = static_cast<const double*>(P.B);
   const std::string tag = "M" + std::to_string(M) + "_s" + std::to_string(stride) + "_t" +
                           std::to_string(tile);
-  rows.push_back(run(P, "Variant4_" + tag, [&] {
-    return tsm_compat::tsmttsm4<double, M, stride, tile>(P.s, K, A, B, P.C);
-  }, warmup, runs));
-  rows.push_back(run(P, "Variant6_" + tag, [&] {
-    return tsm_compat::tsmttsm6<double, M, stride, local, tile>(P.s, K, A, B, P.C);
-  }, warmup, runs));
-  rows.push_back(run(P, "Variant9_" + tag, [&] {
-    return tsm_compat::tsmttsm9<double, M, stride, local, tile>(P.s, K, A, B, P.C);
-  }, warmup, runs));
+  using namespace tsm_compat;
+  const std::pair<const char*, std::function<tsm_status()>> names[] = {
+      {"Variant1_", [&] { return tsmttsm<double, M, stride>(P.s, K, A, B, P.C); }},
+      {"Variant2_", [&] { return tsmttsm2<double, M, stride, tile>(P.s, K, A, B, P.C); }},
+      {"Variant3_", [&] { return tsmttsm3<double, M, stride, tile>(P.s, K, A, B, P.C); }},
+      {"Variant4_", [&] { return tsmttsm4<double, M, stride, tile>(P.s, K, A, B, P.C); }},
+      {"Variant5_", [&] { return tsmttsm5<double, M, stride, tile>(P.s, K, A, B, P.C); }},
+      {"Variant6_", [&] { return tsmttsm6<double, M, stride, local, tile>(P.s, K, A, B, P.C); }},
+      {"Variant7_", [&] { return tsmttsm7<double, M, stride, tile>(P.s, K, A, B, P.C); }},
+      {"Variant8_", [&] { return tsmttsm8<double, M>(P.s, K, A, B, P.C); }},
+      {"Variant9_", [&] { return tsmttsm9<double, M, stride, local, tile>(P.s, K, A, B, P.C); }},
+      {"tsm_atb_", [&] {
+         return tsm_atb(TSM_F64, TSM_F64, K, M, M, A, M, B, M, P.C, M, 1, nullptr, 0, P.s);
+       }},
+  };
+  for (const auto& nm : names) rows.push_back(run(P, nm.first + tag, nm.second, warmup, runs));
 }
 
 void main_suite(std::vector<Row>& rows, int64_t K, int warmup, int runs) {
@@ -281,6 +292,12 @@ void generated_suite(std::vector<Row>& rows, int64_t K, int warmup, int runs) {
     const double* A = static_cast<const double*>(P->A);
     const double* B = static_cast<const double*>(P->B);
     rows.push_back(run(*P, g.name, [&] { return g.fn(P->s, K, A, B, P->C); }, warmup, runs));
+    const bool last_of_M = (&g + 1 == std::end(tsm_compat::kGeneratedKernels)) || (&g + 1)->M != g.M;
+    if (last_of_M)  // the library entry point on the same problem, as the speed reference
+      rows.push_back(run(*P, "tsm_atb_M" + std::to_string(g.M), [&] {
+        return tsm_atb(TSM_F64, TSM_F64, K, g.M, g.M, A, g.M, B, g.M, P->C, g.M, 1, nullptr, 0,
+                       P->s);
+      }, warmup, runs));
   }
   delete P;
 }
