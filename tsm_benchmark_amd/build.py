@@ -21,6 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtsm.so")
 BENCH_BIN = os.path.join(HERE, "tsm_bench")
+CLOCK_BIN = os.path.join(HERE, "tsm_clock")
 ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -78,6 +79,15 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"harness build failed:\n{r.stderr[-4000:]}")
     os.replace(BENCH_BIN + ".tmp", BENCH_BIN)
+    # the shader-clock timeline tool (a sampler kernel beside tsm_atb launches)
+    clock = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+             "-Xclang", "-packed-fp32-ops", "-I", os.path.join(REPO, "include"),
+             os.path.join(CSRC, "harness", "tsm_clock.hip"), "-o", CLOCK_BIN + ".tmp",
+             "-L", HERE, "-ltsm", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(clock, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"tsm_clock build failed:\n{r.stderr[-4000:]}")
+    os.replace(CLOCK_BIN + ".tmp", CLOCK_BIN)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources in {time.time() - t0:.1f}s", file=sys.stderr)
     return LIB
