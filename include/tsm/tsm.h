@@ -36,7 +36,12 @@ typedef void* tsm_stream_t; /* hipStream_t */
 typedef enum {
   TSM_F64 = 0,  /* IEEE binary64 */
   TSM_F32 = 1,  /* IEEE binary32 */
-  TSM_BF16 = 2  /* bfloat16 (upper half of a binary32) */
+  TSM_BF16 = 2, /* bfloat16 (upper half of a binary32) */
+  /* Not values: they make every int a value of the type, so that an
+   * argument outside the list (a C caller may pass any int) is rejected by
+   * the checks instead of being undefined in the library's C++. */
+  TSM_DTYPE_RANGE_MIN_ = -2147483647 - 1,
+  TSM_DTYPE_RANGE_MAX_ = 2147483647
 } tsm_dtype;
 
 typedef enum {
@@ -47,7 +52,9 @@ typedef enum {
   TSM_ERR_WORKSPACE = 4,        /* caller workspace smaller than tsm_atb_workspace_size(), or
                                    none given on a stream that is being captured */
   TSM_ERR_HIP = 5,              /* a HIP runtime call failed: see tsm_last_hip_error() */
-  TSM_ERR_NO_KERNEL = 6         /* no generated kernel matches the requested config */
+  TSM_ERR_NO_KERNEL = 6,        /* no generated kernel matches the requested config */
+  TSM_STATUS_RANGE_MIN_ = -2147483647 - 1, /* not values: every int is one of the type */
+  TSM_STATUS_RANGE_MAX_ = 2147483647
 } tsm_status;
 
 /* Distributions of tsm_fill_random (the synthetic inputs):
@@ -56,7 +63,12 @@ typedef enum {
  * where w = splitmix64_finalizer(seed + stream*0xd1b54a32d192ed03 + (idx+1)*0x9e3779b97f4a7c15)
  * and idx = row*cols + col.  FP32 and BF16 fills round the FP64 value once,
  * to nearest even. */
-typedef enum { TSM_DIST_UNIFORM = 0, TSM_DIST_POW2 = 1 } tsm_dist;
+typedef enum {
+  TSM_DIST_UNIFORM = 0,
+  TSM_DIST_POW2 = 1,
+  TSM_DIST_RANGE_MIN_ = -2147483647 - 1, /* not values: every int is one of the type */
+  TSM_DIST_RANGE_MAX_ = 2147483647
+} tsm_dist;
 
 /* Explicit kernel choice (generator vocabulary).  Zero fields mean "tuned
  * default".  Replaces the compile-time constants baked into the generated

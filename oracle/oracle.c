@@ -59,12 +59,17 @@ float oracle_round_f32(double x) { return (float)x; /* IEEE cast: RNE */ }
 uint16_t oracle_round_bf16(double x) {
   /* One rounding from FP64 to an 8-bit significand: round the FP64 pattern
    * to nearest-even at bit 45 (52 - 7 kept fraction bits), then the value is
-   * exact in FP32 and its top 16 bits are the bf16 pattern.  Valid for the
-   * finite values of FP32 normal range and zero, which covers every input
-   * this oracle produces. */
+   * exact in FP32 and its top 16 bits are the bf16 pattern.  Below 2^-126
+   * (the bf16 subnormals, multiples of 2^-133) the value is rounded to the
+   * nearest multiple of 2^-133 instead, ties to even.  Finite inputs. */
   uint64_t b;
   memcpy(&b, &x, 8);
   if ((b & 0x7fffffffffffffffULL) == 0) return (uint16_t)((b >> 48) & 0x8000);
+  if (fabs(x) < 0x1p-126) {
+    /* the scaling is exact; nearbyint rounds to nearest even (default mode) */
+    const double q = nearbyint(fabs(x) * 0x1p133); /* 0 .. 128; 128 = 2^-126 */
+    return (uint16_t)(((b >> 48) & 0x8000) | (uint16_t)q);
+  }
   uint64_t lsb = (b >> 45) & 1u;
   b += 0x00000fffffffffffULL + lsb;
   b &= ~0x00001fffffffffffULL;

@@ -242,6 +242,9 @@ int sq_range(const oracle_ref_cfg& c, i64 K, int M, const double* A, const doubl
   if ((M * M) % (TS * TS)) return -1;  // static_assert, tsmttsm.hh:43
   const int nt = (M * M) / (TS * TS);
   const i64 stride = c.stride;
+  // fewer work-items than tiles: the row stride stride / nt is 0 and the
+  // reference's row loop would never end (an invalid configuration)
+  if (stride / nt == 0) return -1;
 #pragma omp parallel for schedule(static)
   for (i64 tid = 0; tid < stride; ++tid) {
     double cl[TS][TS] = {};
@@ -262,6 +265,9 @@ int sq_ndrange(const oracle_ref_cfg& c, i64 K, int M, const double* A, const dou
   const int L = c.local_size;
   if (L <= 0 || c.stride % L) return -1;
   const i64 num_groups = c.stride / L;
+  // fewer work-groups than tiles: the row stride (num_groups / nt) * L is 0
+  // and the reference's row loop would never end (an invalid configuration)
+  if (num_groups / nt == 0) return -1;
 #pragma omp parallel for schedule(static)
   for (i64 g = 0; g < num_groups; ++g) {
     const auto t = SqTile<TS>::make(M, g % nt, transposed);  // tsmttsm.hh:248-250

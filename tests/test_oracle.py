@@ -220,3 +220,33 @@ def test_oracle_reproduces_golden_fixtures(orc):
         q = 52 if case["dist"] == 0 else 20
         C = orc.atb_exact(A, B, q, q).ravel()
         assert [float(x).hex() for x in C] == case["C"]
+
+
+def test_normal_generator_moments(orc):
+    """The Box-Muller normal(0, 1) draws (used for the upstream gradient dC):
+    mean, variance, skewness and kurtosis within 6 standard errors over 2^20
+    draws, and a Kolmogorov-Smirnov test against the normal CDF."""
+    from scipy import stats
+
+    x = orc.gen_matrix(1 << 20, 1, dist=orc.DIST_NORMAL, seed=orc.SEED, stream=77).ravel()
+    N = x.size
+    assert abs(x.mean()) <= 6 / np.sqrt(N)
+    assert abs(x.var() - 1) <= 6 * np.sqrt(2 / N)
+    assert abs(stats.skew(x)) <= 6 * np.sqrt(6 / N)
+    assert abs(stats.kurtosis(x)) <= 6 * np.sqrt(24 / N)  # excess kurtosis 0
+    assert stats.kstest(x, "norm").pvalue > 1e-4
+    # a planted mistake (the cosine's argument without 2 pi) fails the same checks
+    u1 = (np.arange(N) + 0.5) / N
+    bad = np.sqrt(-2 * np.log(np.random.default_rng(0).random(N) + 1e-300)) * np.cos(u1)
+    assert stats.kstest(bad, "norm").pvalue < 1e-4
+
+
+def test_bf16_rounding_of_subnormals(orc):
+    """Below 2^-126 bf16 values are multiples of 2^-133: known answers of the
+    one rounding (to the nearest multiple, ties to even), where rounding to
+    FP32 first and truncating would give the neighbour below."""
+    u = 2.0**-133
+    cases = {-1.5e-40: 0x8002, 1e-39: 0x000B, 3e-39: 0x0021, 2.5 * u: 0x0002, 3.5 * u: 0x0004,
+             0.49 * u: 0x0000, 127.5 * u: 0x0080, 2.0**-126: 0x0080, -0.5 * u: 0x8000}
+    got = orc.round_bf16_bits(np.array(list(cases)))
+    assert [int(g) for g in got] == list(cases.values())

@@ -371,10 +371,15 @@ __device__ __forceinline__ double rng_value(int dist, uint64_t w) {
   const uint64_t bits = ((uint64_t)(1023 + e) << 52) | (w & 0x8000000000000000ULL);
   return __longlong_as_double((long long)bits);
 }
-// Single round-to-nearest-even from binary64 to bfloat16 (8-bit significand).
+// Single round-to-nearest-even from binary64 to bfloat16 (8-bit significand);
+// below 2^-126 to the nearest multiple of 2^-133 (the bf16 subnormals).
 __device__ __forceinline__ uint16_t round_bf16(double x) {
   uint64_t b = (uint64_t)__double_as_longlong(x);
   if ((b & 0x7fffffffffffffffULL) == 0) return (uint16_t)((b >> 48) & 0x8000);
+  if (__builtin_fabs(x) < 0x1p-126) {
+    const double q = __builtin_rint(__builtin_fabs(x) * 0x1p133);  // exact scaling, ties to even
+    return (uint16_t)(((b >> 48) & 0x8000) | (uint16_t)q);
+  }
   const uint64_t lsb = (b >> 45) & 1u;
   b += 0x00000fffffffffffULL + lsb;
   b &= ~0x00001fffffffffffULL;
@@ -593,6 +598,7 @@ const char* tsm_strerror(tsm_status s) {
       return "workspace too small, or none given for a call on a capturing stream";
     case TSM_ERR_HIP: return "HIP runtime error";
     case TSM_ERR_NO_KERNEL: return "no generated kernel matches the configuration";
+    default: break;
   }
   return "unknown status";
 }

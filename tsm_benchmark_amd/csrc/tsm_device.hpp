@@ -640,6 +640,18 @@ struct MfmaAcc {
 // through to memory; the publishing wave then only waits until its stores are
 // complete before the arrival counter is incremented.  Only the last
 // workgroup to arrive fences (acquire: it invalidates its L2) before reading.
+//
+// The ordering rests on the gfx942/gfx950 memory model of the AMDGPU backend
+// (LLVM AMDGPUUsage, "Memory Model GFX942"): an agent-scope atomic store is a
+// global_store with sc1 set, which writes through the issuing XCD's L2 to
+// memory, and vmcnt counts it until the write is acknowledged there.  So after
+// `s_waitcnt vmcnt(0)` the partial is in memory, where the counter atomic (also
+// performed at memory at agent scope) cannot overtake it; the last arriver's
+// acquire fence invalidates its L2 before it reads.  The relaxed store plus the
+// explicit wait replace the release fence a C++ release store would emit
+// (buffer_wbl2, a writeback of the whole L2).  A compiler that stopped
+// lowering agent-scope stores as write-through would break this: the
+// determinism and parity tests of every class exercise it.
 __device__ __forceinline__ void publish(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
