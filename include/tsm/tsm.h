@@ -218,7 +218,9 @@ typedef struct {
 size_t tsm_atb_batched_workspace_size(tsm_dtype in_type, int count, const tsm_problem* problems);
 
 /* Uploads the plan (returning once the host array has been consumed) and
- * launches; workspace NULL = a library-owned buffer for this stream. */
+ * launches; workspace NULL = a library-owned buffer for this stream.  The
+ * upload goes through a reused host buffer, so this call cannot be captured in
+ * a graph (TSM_ERR_UNSUPPORTED on a capturing stream): use a plan there. */
 tsm_status tsm_atb_batched(tsm_dtype in_type, tsm_dtype out_type, int count,
                            const tsm_problem* problems, int accumulate,
                            void* workspace, size_t workspace_bytes, tsm_stream_t stream);
@@ -232,6 +234,10 @@ tsm_status tsm_batched_plan_create(tsm_dtype in_type, tsm_dtype out_type, int co
                                    const tsm_problem* problems, int accumulate,
                                    tsm_batched_plan* plan);
 tsm_status tsm_batched_plan_execute(tsm_batched_plan plan, tsm_stream_t stream);
+/* (Executions of one plan share its buffer: order them on one stream or with
+ * events.  A graph that captured an execution uses the plan's buffer when it
+ * replays, so the plan must outlive such graphs.  Destroy waits for the
+ * plan's launches outside graphs, stream by stream, then frees the buffer.) */
 /* Work items and launch grid of a plan (either pointer may be NULL). */
 tsm_status tsm_batched_plan_info(tsm_batched_plan plan, int64_t* items, int64_t* grid);
 tsm_status tsm_batched_plan_destroy(tsm_batched_plan plan);
