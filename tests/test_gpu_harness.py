@@ -5,6 +5,7 @@ the reference's acceptance rule against the naive host loop over all K rows
 import json
 import os
 import subprocess
+import sys
 
 import pytest
 
@@ -65,3 +66,36 @@ def test_atb_suite(dtype, M, N):
     rows = run("--suite", "atb", "--dtype", dtype, "--M", str(M), "--N", str(N), "--K", "100003",
                "--warmup", "1", "--runs", "3")
     assert len(rows) == 1
+
+
+def test_bench_line_keys():
+    """bench.py prints one JSON line with the contract's keys (a short run)."""
+    r = subprocess.run([sys.executable, os.path.join(HERE, "..", "bench.py"), "--steps", "3",
+                        "--warmup", "1", "--n", str(1 << 16), "--no-secondary",
+                        "--no-cpu-baseline"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    assert len(lines) == 1
+    rec = json.loads(lines[0])
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+                "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
+                "roofline", "cpu_baseline"):
+        assert key in rec, key
+    assert rec["n_gpus"] == 1 and rec["steps"] == 3 and rec["dtype"] == "f64"
+    assert rec["value"] > 0 and rec["roofline"]["bound"] == "hbm"
+    assert abs(rec["roofline"]["frac"] - rec["value"] / 8000.0) < 1e-3
+
+
+def test_clock_tool_runs():
+    """tsm_clock: per-launch durations and a shader-clock timeline in GHz."""
+    tool = os.path.join(HERE, "..", "tsm_benchmark_amd", "tsm_clock")
+    if not os.path.exists(tool):
+        pytest.fail("tsm_clock is not built (python -m tsm_benchmark_amd.build)")
+    r = subprocess.run([tool, "--n", str(1 << 20), "--launches", "20", "--idle-ms", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    launches = [ln for ln in r.stdout.splitlines() if ln.startswith("launch ")]
+    clocks = [float(ln.split()[-1]) for ln in r.stdout.splitlines() if ln.startswith("sample ")]
+    assert len(launches) == 20 and clocks
+    busy = [c for c in clocks if c > 0]
+    assert busy and all(0.3 < c < 3.5 for c in busy)
