@@ -91,7 +91,7 @@ def test_batched_matches_single_calls_at_the_batched_workload_shape():
 
 
 def test_batched_large_problem_with_capped_items():
-    """A problem that would be cut into more than 256 items of 2 MiB runs as
+    """A problem that would be cut into more than 256 items of 4 MiB runs as
     256 larger items (next to a small problem): the same results as a single
     call within 1e-13 relative, deterministic."""
     A = ops.random_matrix((1 << 21) + 77, 32, torch.float64, seed=SEED, stream_id=1)  # 1 GiB + A
@@ -111,6 +111,40 @@ def test_batched_large_problem_with_capped_items():
     plan.run()
     torch.cuda.synchronize()
     assert all(torch.equal(C, F) for C, F in zip(plan_outs, first))
+    plan.close()
+
+
+def test_queue_deals_items_beyond_the_grid():
+    """Items past the first grid-full come from the kernel's queue, whose
+    tickets the kernel zeroes at its end: many more items than workgroups, run
+    eagerly and by graph replay several times, give the same bits each time
+    and match single calls."""
+    rng = np.random.default_rng(7)
+    As, Bs = [], []
+    for i in range(48):
+        n = int(rng.integers(1 << 19, 1 << 20))
+        k, m = (int(x) for x in rng.choice([2, 8, 16, 32], 2))
+        As.append(ops.random_matrix(n, k, torch.float64, seed=SEED, stream_id=5000 + 2 * i))
+        Bs.append(ops.random_matrix(n, m, torch.float64, seed=SEED, stream_id=5001 + 2 * i))
+    outs = [torch.empty((A.shape[1], B.shape[1]), dtype=torch.float64, device="cuda")
+            for A, B in zip(As, Bs)]
+    plan = ops.BatchedPlan(As, Bs, outs)
+    assert plan.items > 4 * plan.grid, (plan.items, plan.grid)
+    plan.run()
+    torch.cuda.synchronize()
+    first = [C.clone() for C in outs]
+    for A, B, C in zip(As, Bs, first):
+        single = ops.atb(A, B)
+        assert ((C - single).norm() / single.norm()).item() <= 1e-13
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.run()
+    for i in range(3):
+        for C in outs:
+            C.zero_()
+        plan.run() if i % 2 else g.replay()
+        torch.cuda.synchronize()
+        assert all(torch.equal(C, F) for C, F in zip(outs, first)), f"call {i}"
     plan.close()
 
 

@@ -567,13 +567,15 @@ def emit_batched_file(dtype: str, classes: list[KernelConfig]) -> str:
            f"extern \"C\" __global__ void __launch_bounds__(256) {name}(tsm::BatchArgs ba) {{",
            f"  __shared__ __attribute__((aligned(16))) unsigned char smem[{lds}];",
            "  __shared__ int s_flag;",
-           "  for (int64_t i = blockIdx.x; i < ba.n_items; i += gridDim.x) {",
+           "  __shared__ unsigned s_next;",
+           "  for (int64_t i = blockIdx.x; i < ba.n_items; i = (int64_t)gridDim.x + s_next) {",
            "    const tsm::BatchItem it = ba.items[i];",
            "    switch (ba.probs[it.problem].cls) {"]
     for i, c in enumerate(classes):
         out.append(f"      case {i}: tsm::batched_item_call<{t}, {c.kc}, {c.mc}, {c.tm}, {c.tn}, {c.rows}, "
-                   f"{c.pf}, {c.nt}>(ba, it, smem, &s_flag); break;  // {c.name}")
-    out += ["      default: break;", "    }", "  }", "}", "",
+                   f"{c.pf}, {c.nt}>(ba, it, smem, &s_flag, &s_next); break;  // {c.name}")
+    out += ["      default: tsm::batched_skip(ba, &s_next); break;", "    }", "  }",
+            "  tsm::batched_exit(ba);", "}", "",
             f"static const tsm::BatchClass kClasses[] = {{"]
     out += [f"  {{{c.kc}, {c.mc}, {c.rows}}}," for c in classes]
     out += ["};", "",

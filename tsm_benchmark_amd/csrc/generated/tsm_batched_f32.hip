@@ -7,48 +7,50 @@
 extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f32(tsm::BatchArgs ba) {
   __shared__ __attribute__((aligned(16))) unsigned char smem[81920];
   __shared__ int s_flag;
-  for (int64_t i = blockIdx.x; i < ba.n_items; i += gridDim.x) {
+  __shared__ unsigned s_next;
+  for (int64_t i = blockIdx.x; i < ba.n_items; i = (int64_t)gridDim.x + s_next) {
     const tsm::BatchItem it = ba.items[i];
     switch (ba.probs[it.problem].cls) {
-      case 0: tsm::batched_item_call<float, 1, 1, 1, 1, 4096, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
-      case 1: tsm::batched_item_call<float, 1, 2, 1, 2, 2048, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
-      case 2: tsm::batched_item_call<float, 1, 4, 1, 4, 2048, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x4_t1x4_r2048_p2_nt
-      case 3: tsm::batched_item_call<float, 1, 8, 1, 4, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x8_t1x4_r1024_p2_nt
-      case 4: tsm::batched_item_call<float, 1, 16, 1, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x16_t1x4_r512_p2_nt
-      case 5: tsm::batched_item_call<float, 1, 32, 1, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_1x32_t1x4_r256_p2_nt
-      case 6: tsm::batched_item_call<float, 2, 1, 2, 1, 2048, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
-      case 7: tsm::batched_item_call<float, 2, 2, 2, 2, 2048, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
-      case 8: tsm::batched_item_call<float, 2, 4, 2, 4, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
-      case 9: tsm::batched_item_call<float, 2, 8, 2, 4, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x8_t2x4_r1024_p2_nt
-      case 10: tsm::batched_item_call<float, 2, 16, 2, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x16_t2x4_r512_p2_nt
-      case 11: tsm::batched_item_call<float, 2, 32, 2, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_2x32_t2x4_r256_p2_nt
-      case 12: tsm::batched_item_call<float, 4, 1, 4, 1, 2048, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x1_t4x1_r2048_p2_nt
-      case 13: tsm::batched_item_call<float, 4, 2, 4, 2, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
-      case 14: tsm::batched_item_call<float, 4, 4, 4, 4, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
-      case 15: tsm::batched_item_call<float, 4, 8, 4, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
-      case 16: tsm::batched_item_call<float, 4, 16, 4, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x16_t4x4_r512_p2_nt
-      case 17: tsm::batched_item_call<float, 4, 32, 4, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_4x32_t4x4_r256_p2_nt
-      case 18: tsm::batched_item_call<float, 8, 1, 8, 1, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x1_t8x1_r1024_p2_nt
-      case 19: tsm::batched_item_call<float, 8, 2, 8, 2, 1024, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x2_t8x2_r1024_p2_nt
-      case 20: tsm::batched_item_call<float, 8, 4, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
-      case 21: tsm::batched_item_call<float, 8, 8, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x8_t8x4_r512_p2_nt
-      case 22: tsm::batched_item_call<float, 8, 16, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
-      case 23: tsm::batched_item_call<float, 8, 32, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_8x32_t8x4_r256_p2_nt
-      case 24: tsm::batched_item_call<float, 16, 1, 8, 1, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x1_t8x1_r512_p2_nt
-      case 25: tsm::batched_item_call<float, 16, 2, 8, 2, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x2_t8x2_r512_p2_nt
-      case 26: tsm::batched_item_call<float, 16, 4, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x4_t8x4_r512_p2_nt
-      case 27: tsm::batched_item_call<float, 16, 8, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
-      case 28: tsm::batched_item_call<float, 16, 16, 0, 0, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x16_mfma_r256_p2_nt
-      case 29: tsm::batched_item_call<float, 16, 32, 8, 4, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_16x32_t8x4_r128_p2_nt
-      case 30: tsm::batched_item_call<float, 32, 1, 8, 1, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x1_t8x1_r256_p2_nt
-      case 31: tsm::batched_item_call<float, 32, 2, 8, 2, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x2_t8x2_r256_p2_nt
-      case 32: tsm::batched_item_call<float, 32, 4, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x4_t8x4_r256_p2_nt
-      case 33: tsm::batched_item_call<float, 32, 8, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x8_t8x4_r256_p2_nt
-      case 34: tsm::batched_item_call<float, 32, 16, 8, 4, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x16_t8x4_r128_p2_nt
-      case 35: tsm::batched_item_call<float, 32, 32, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag); break;  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-      default: break;
+      case 0: tsm::batched_item_call<float, 1, 1, 1, 1, 4096, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
+      case 1: tsm::batched_item_call<float, 1, 2, 1, 2, 2048, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
+      case 2: tsm::batched_item_call<float, 1, 4, 1, 4, 2048, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x4_t1x4_r2048_p2_nt
+      case 3: tsm::batched_item_call<float, 1, 8, 1, 4, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x8_t1x4_r1024_p2_nt
+      case 4: tsm::batched_item_call<float, 1, 16, 1, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x16_t1x4_r512_p2_nt
+      case 5: tsm::batched_item_call<float, 1, 32, 1, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_1x32_t1x4_r256_p2_nt
+      case 6: tsm::batched_item_call<float, 2, 1, 2, 1, 2048, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
+      case 7: tsm::batched_item_call<float, 2, 2, 2, 2, 2048, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
+      case 8: tsm::batched_item_call<float, 2, 4, 2, 4, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
+      case 9: tsm::batched_item_call<float, 2, 8, 2, 4, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x8_t2x4_r1024_p2_nt
+      case 10: tsm::batched_item_call<float, 2, 16, 2, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x16_t2x4_r512_p2_nt
+      case 11: tsm::batched_item_call<float, 2, 32, 2, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_2x32_t2x4_r256_p2_nt
+      case 12: tsm::batched_item_call<float, 4, 1, 4, 1, 2048, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x1_t4x1_r2048_p2_nt
+      case 13: tsm::batched_item_call<float, 4, 2, 4, 2, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
+      case 14: tsm::batched_item_call<float, 4, 4, 4, 4, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
+      case 15: tsm::batched_item_call<float, 4, 8, 4, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
+      case 16: tsm::batched_item_call<float, 4, 16, 4, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x16_t4x4_r512_p2_nt
+      case 17: tsm::batched_item_call<float, 4, 32, 4, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_4x32_t4x4_r256_p2_nt
+      case 18: tsm::batched_item_call<float, 8, 1, 8, 1, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x1_t8x1_r1024_p2_nt
+      case 19: tsm::batched_item_call<float, 8, 2, 8, 2, 1024, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x2_t8x2_r1024_p2_nt
+      case 20: tsm::batched_item_call<float, 8, 4, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
+      case 21: tsm::batched_item_call<float, 8, 8, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x8_t8x4_r512_p2_nt
+      case 22: tsm::batched_item_call<float, 8, 16, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
+      case 23: tsm::batched_item_call<float, 8, 32, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_8x32_t8x4_r256_p2_nt
+      case 24: tsm::batched_item_call<float, 16, 1, 8, 1, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x1_t8x1_r512_p2_nt
+      case 25: tsm::batched_item_call<float, 16, 2, 8, 2, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x2_t8x2_r512_p2_nt
+      case 26: tsm::batched_item_call<float, 16, 4, 8, 4, 512, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x4_t8x4_r512_p2_nt
+      case 27: tsm::batched_item_call<float, 16, 8, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
+      case 28: tsm::batched_item_call<float, 16, 16, 0, 0, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x16_mfma_r256_p2_nt
+      case 29: tsm::batched_item_call<float, 16, 32, 8, 4, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_16x32_t8x4_r128_p2_nt
+      case 30: tsm::batched_item_call<float, 32, 1, 8, 1, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x1_t8x1_r256_p2_nt
+      case 31: tsm::batched_item_call<float, 32, 2, 8, 2, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x2_t8x2_r256_p2_nt
+      case 32: tsm::batched_item_call<float, 32, 4, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x4_t8x4_r256_p2_nt
+      case 33: tsm::batched_item_call<float, 32, 8, 8, 4, 256, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x8_t8x4_r256_p2_nt
+      case 34: tsm::batched_item_call<float, 32, 16, 8, 4, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x16_t8x4_r128_p2_nt
+      case 35: tsm::batched_item_call<float, 32, 32, 0, 0, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+      default: tsm::batched_skip(ba, &s_next); break;
     }
   }
+  tsm::batched_exit(ba);
 }
 
 static const tsm::BatchClass kClasses[] = {

@@ -1,12 +1,15 @@
 // Batched C_i (+)= A_i^T B_i for many independent problems in one persistent
 // launch.  The host cuts every problem into work items (runs of row tiles of
-// about 2 MiB of A + B each), sorts them by size and deals item i to
-// workgroup i % G.  A workgroup streams its item with the same body as the
-// single-problem kernel (atb_partial, specialised for the problem's width
-// class), stores the item's FP64 partial in the problem's partial slots, and
-// arrives at the problem's counter; the last item of a problem to arrive adds
-// that problem's partials in slot order and writes C_i.  The summation tree
-// depends only on the plan, so results are deterministic.
+// up to 4 MiB of A + B each) and sorts them by size.  Workgroup g starts with
+// item g; every later item is taken from a queue (an atomic ticket, drawn one
+// item ahead, so its latency hides behind the item being streamed), which
+// evens out compute units that stream slower than others.  A workgroup streams
+// its item with the same body as the single-problem kernel (atb_partial,
+// specialised for the problem's width class), stores the item's FP64 partial
+// in the item's fixed slot, and arrives at the problem's counter; the last
+// item of a problem to arrive adds that problem's partials in slot order and
+// writes C_i.  The summation tree depends only on the plan, not on which
+// workgroup took which item, so results are deterministic.
 #pragma once
 
 #include "tsm_device.hpp"
@@ -39,6 +42,7 @@ struct BatchArgs {
   int64_t n_items;
   double* partials;
   unsigned int* counters;  // one per problem, zero between calls
+  unsigned int* queue;     // [next ticket, workgroups done], zero between calls
   int accumulate;
   int c_f32;
 };
@@ -73,9 +77,19 @@ __device__ __forceinline__ void finish_outputs_joint(const AtbArgs& a, int G, in
   for (int r = 0; r < R; ++r) store_output(a, MC, tid + r * kThreads, acc[r]);
 }
 
+// Ticket of the item after the current one: item gridDim.x + ticket.  Thread
+// 0 draws it before the current item's loads and stores it to LDS at the end
+// of the item; every thread reads it after the item's final barrier.
+__device__ __forceinline__ unsigned draw_ticket(const BatchArgs& ba) {
+  return __hip_atomic_fetch_add(ba.queue, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
 __device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchItem& it,
-                                             unsigned char* smem, int* s_flag) {
+                                             unsigned char* smem, int* s_flag,
+                                             unsigned* s_next) {
+  unsigned ticket = 0;
+  if (threadIdx.x == 0) ticket = draw_ticket(ba);
   const BatchProblem& pr = ba.probs[it.problem];
   constexpr int KCMC = KC * MC;
   AtbArgs a{};
@@ -109,7 +123,26 @@ __device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchIte
     if (threadIdx.x == 0)
       __hip_atomic_store(a.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();  // the next item reuses the LDS this one read last
+  if (threadIdx.x == 0) *s_next = ticket;
+  __syncthreads();  // the next item reuses the LDS this one read last; s_next is visible
+}
+
+// An item of no compiled class (the host never plans one): only the ticket.
+__device__ __forceinline__ void batched_skip(const BatchArgs& ba, unsigned* s_next) {
+  if (threadIdx.x == 0) *s_next = draw_ticket(ba);
+  __syncthreads();
+}
+
+// After a workgroup's last item: the last workgroup to finish (every other
+// one has drawn its final ticket, and its draw has returned) zeroes the queue
+// for the next call.
+__device__ __forceinline__ void batched_exit(const BatchArgs& ba) {
+  if (threadIdx.x != 0) return;
+  if (__hip_atomic_fetch_add(ba.queue + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+      gridDim.x - 1) {
+    __hip_atomic_store(ba.queue, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(ba.queue + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // One class body behind a call boundary: the batched kernel switches over many
@@ -117,8 +150,9 @@ __device__ __forceinline__ void batched_item(const BatchArgs& ba, const BatchIte
 // (inlined together, the bodies share one allocation and spill).
 template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT>
 __device__ __noinline__ void batched_item_call(const BatchArgs& ba, const BatchItem& it,
-                                               unsigned char* smem, int* s_flag) {
-  batched_item<S, KC, MC, TM, TN, T, PF, NT>(ba, it, smem, s_flag);
+                                               unsigned char* smem, int* s_flag,
+                                               unsigned* s_next) {
+  batched_item<S, KC, MC, TM, TN, T, PF, NT>(ba, it, smem, s_flag, s_next);
 }
 
 }  // namespace tsm

@@ -1,5 +1,5 @@
 // Host side of the batched C_i (+)= A_i^T B_i: validation, the work plan
-// (per-problem chunks of about 2 MiB of A + B, sorted largest first), the
+// (per-problem chunks of up to 4 MiB of A + B, sorted largest first), the
 // workspace layout, and the launches (tsm_atb_batched and the plan API).
 #include <hip/hip_runtime.h>
 
@@ -17,7 +17,16 @@
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr int64_t kItemBytes = 2 << 20;  // target bytes of A + B per work item
+// Bytes of A + B per work item.  Each item costs about a microsecond to start
+// and to reduce; the mixed 512-pair FP64 batch (28 GB) ran at 6.45, 6.62,
+// 6.72 and 6.73 TB/s with items of 1, 2, 4 and 8 MiB (FP32: 6.24, 6.43, 6.59,
+// 6.58).  Smaller batches take smaller items, down to kMinItemBytes, so that
+// there are about kItemsPerWorkgroup items for each workgroup of a grid of
+// kNominalGrid (MI355X's compute units) for the queue to balance.
+constexpr int64_t kMaxItemBytes = 4 << 20;
+constexpr int64_t kMinItemBytes = 256 << 10;
+constexpr int64_t kItemsPerWorkgroup = 8;
+constexpr int64_t kNominalGrid = 256;
 // At most this many items per problem (larger items beyond; the persistent
 // grid has 256 workgroups on MI355X): the last item of a problem adds all of
 // the problem's item partials in one workgroup, so a problem cut into
@@ -94,6 +103,24 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
   if (parts.size() > (size_t)0x7fffffff) return TSM_ERR_INVALID_ARGUMENT;
   count = (int)parts.size();
   problems = parts.data();
+  auto class_of = [bk](const tsm_problem& q) {
+    const int kc = width_class(q.k), mc = width_class(q.m);
+    int cls = -1;
+    for (int c = 0; c < bk->n_classes; ++c)
+      if (bk->classes[c].kc == kc && bk->classes[c].mc == mc) cls = c;
+    return cls;
+  };
+  int64_t total_bytes = 0;  // tiles x tile bytes over the batch
+  for (int p = 0; p < count; ++p) {
+    const int cls = class_of(problems[p]);
+    if (cls < 0) return TSM_ERR_UNSUPPORTED;
+    const int64_t T = bk->classes[cls].rows;
+    total_bytes += (problems[p].n + T - 1) / T * T *
+                   (bk->classes[cls].kc + bk->classes[cls].mc) * es;
+  }
+  const int64_t item_bytes = std::min(
+      kMaxItemBytes,
+      std::max(kMinItemBytes, total_bytes / (kItemsPerWorkgroup * kNominalGrid)));
   int64_t partial_words = 0;
   L->probs.resize(count);
   for (int p = 0; p < count; ++p) {
@@ -102,15 +129,12 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
                                            q.ldb, q.C, q.ldc);
     if (st != TSM_SUCCESS) return st;
     const int kc = width_class(q.k), mc = width_class(q.m);
-    int cls = -1;
-    for (int c = 0; c < bk->n_classes; ++c)
-      if (bk->classes[c].kc == kc && bk->classes[c].mc == mc) cls = c;
-    if (cls < 0) return TSM_ERR_UNSUPPORTED;
+    const int cls = class_of(q);
     const int64_t T = bk->classes[cls].rows;
     const int64_t tiles = (q.n + T - 1) / T;
     const int64_t tile_bytes = T * (kc + mc) * es;
     const int64_t chunk = std::max<int64_t>(
-        std::max<int64_t>(1, kItemBytes / tile_bytes),
+        std::max<int64_t>(1, item_bytes / tile_bytes),
         (tiles + kMaxItemsPerProblem - 1) / kMaxItemsPerProblem);
     const int64_t nitems = std::max<int64_t>(1, (tiles + chunk - 1) / chunk);
     if (nitems > 0x7fffffff) return TSM_ERR_INVALID_ARGUMENT;
@@ -142,7 +166,9 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
       L->items.push_back(it);
     }
   }
-  // Largest items first, dealt round-robin: a static, deterministic balance.
+  if (L->items.size() > 0x7fffffffu) return TSM_ERR_INVALID_ARGUMENT;  // 32-bit queue tickets
+  // Largest items first (the first grid-full by workgroup index, the rest from
+  // the queue in this order; by bytes instead of tiles measured the same).
   // Among items of one size, slot-major order: the workgroups running at the
   // same time read items of different problems.  Problem-major order had all
   // workgroups streaming neighbouring 2 MiB items of one problem at once,
@@ -155,7 +181,8 @@ tsm_status build_layout(tsm_dtype in_type, tsm_dtype out_type, int count,
                      if (x.slot != y.slot) return x.slot < y.slot;
                      return x.problem < y.problem;
                    });
-  const size_t counters = align_up(sizeof(unsigned) * (size_t)std::max(count, 1));
+  // one arrival counter per problem, then the item queue's two words
+  const size_t counters = align_up(sizeof(unsigned) * ((size_t)count + 2));
   L->probs_off = counters;
   L->items_off = L->probs_off + align_up(sizeof(tsm::BatchProblem) * (size_t)count);
   L->partials_off = L->items_off + align_up(sizeof(tsm::BatchItem) * L->items.size());
@@ -174,6 +201,7 @@ tsm::BatchArgs make_args(const Layout& L, void* base, int accumulate, tsm_dtype 
   unsigned char* b = static_cast<unsigned char*>(base);
   tsm::BatchArgs a{};
   a.counters = reinterpret_cast<unsigned*>(b);
+  a.queue = a.counters + L.probs.size();
   a.probs = reinterpret_cast<const tsm::BatchProblem*>(b + L.probs_off);
   a.items = reinterpret_cast<const tsm::BatchItem*>(b + L.items_off);
   a.partials = reinterpret_cast<double*>(b + L.partials_off);
