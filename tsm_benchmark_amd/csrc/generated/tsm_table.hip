@@ -1916,14 +1916,14 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 32, 8, 682, 1, 1, 682, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
   {0, 32, 16, 683, 1, 0, 684, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
   {0, 32, 32, 699, 1, 0, 699, 0},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
-  {0, 32, 64, 743, 2, 0, 743, 0},  // tsm_atb_f64_32x64_mfma_r32_p2_nt
+  {0, 32, 64, 742, 1, 0, 743, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
   {0, 64, 1, 744, 2, 1, 744, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
   {0, 64, 2, 745, 2, 1, 745, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
   {0, 64, 4, 746, 2, 1, 746, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
   {0, 64, 8, 747, 2, 1, 747, 1},  // tsm_atb_f64_64x8_t8x8_r32_p2_nt
   {0, 64, 16, 749, 1, 1, 749, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
-  {0, 64, 32, 751, 1, 0, 751, 0},  // tsm_atb_f64_64x32_mfma_r32_p2_nt
-  {0, 64, 64, 754, 2, 0, 762, 0},  // tsm_atb_f64_64x64_direct_d16_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
+  {0, 64, 32, 750, 1, 0, 751, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2_nt)
+  {0, 64, 64, 753, 1, 0, 762, 0},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);

@@ -1053,27 +1053,49 @@ __device__ __forceinline__ void atb_body(const AtbArgs& a) {
 // ---- direct matrix-core engine ------------------------------------------------------
 // The FP64 and FP32 matrix instructions (16x16x4) take, per lane l, one
 // element of row l / 16 of a 4-row K-step; the 16 lanes of a row group supply
-// the 16 columns of a block.  With A row-major those 16 lanes can read the
-// whole row with one coalesced load each: lane j of the group loads
-// LB = KC * sizeof(S) / 16 bytes, the IB = KC / 16 consecutive columns
-// j * IB ... j * IB + IB - 1, and element b of its load feeds block b.  Block
-// b therefore holds the columns {j * IB + b : j = 0..15}; its output row r is
-// C row r * IB + b (likewise for B and the C columns).  Each wave streams its
-// own contiguous run of 4-row steps straight into registers, D steps in
-// flight: no LDS staging and no workgroup barrier until the final sum.
+// the 16 columns of a block.  With A row-major those 16 lanes read the whole
+// row: lane j of the group loads LB = KC * sizeof(S) / 16 bytes, which hold
+// IB = KC / 16 columns, and element b of its load feeds block b.  Up to 16
+// bytes a lane's columns are consecutive (j * IB ... j * IB + IB - 1).  Wider
+// rows (FP64, KC = 64: 32 bytes) are read as 16-byte chunks 256 bytes apart,
+// so that each load instruction covers whole 256-byte row segments: chunk x of
+// lane j holds the E = 16 / sizeof(S) columns 16 E x + E j ... 16 E x + E j +
+// E - 1.  (Two adjacent 16-byte chunks per lane made every instruction touch
+// each cache line half: 6.1 TB/s for the loads alone at 64 x 64, against 7.0
+// with whole segments.)  lane_col gives the column of block b in lane j; block
+// b's output row r is C row lane_col(b, r) (likewise for B and the C
+// columns).  Each wave streams its own contiguous run of 4-row steps straight
+// into registers, D steps in flight: no LDS staging and no workgroup barrier
+// until the final sum.
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) const unsigned char gu8;
 typedef __attribute__((address_space(1))) const v2u gv2u;
 typedef __attribute__((address_space(1))) const uint32_t gu32;
 
-// LB bytes (4, 8, 16 or 32) at p into w[LB / 4].
+// Byte offset of lane j's first chunk within a row, and the column that
+// element b of lane j's load holds (see above).
+template <class S, int LB>
+__device__ __forceinline__ constexpr uint32_t lane_offset(int j) {
+  return (uint32_t)(j * (LB < 16 ? LB : 16));
+}
+template <class S, int LB>
+__device__ __forceinline__ constexpr int lane_col(int b, int j) {
+  constexpr int E = 16 / (int)sizeof(S);
+  if constexpr (LB < 16)
+    return j * (LB / (int)sizeof(S)) + b;
+  else
+    return (b / E) * 16 * E + j * E + b % E;
+}
+
+// LB bytes (4, 8, 16 or 32) into w[LB / 4]: one load at p below 16 bytes,
+// else 16-byte chunks at p, p + 256, ...
 template <int LB, int NT>
 __device__ __forceinline__ void lane_load(uint32_t (&w)[LB / 4], gu8* p) {
   static_assert(LB == 4 || LB == 8 || LB % 16 == 0, "lane load size");
   if constexpr (LB >= 16) {
 #pragma unroll
     for (int x = 0; x < LB / 16; ++x) {
-      const v4u v = stream_load<NT>((gv4u*)(p + 16 * x));
+      const v4u v = stream_load<NT>((gv4u*)(p + 256 * x));
       w[4 * x] = v.x;
       w[4 * x + 1] = v.y;
       w[4 * x + 2] = v.z;
@@ -1135,7 +1157,8 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   const int64_t s0 = wid * q;
   gu8* A8 = (gu8*)a.A;
   gu8* B8 = (gu8*)a.B;
-  const uint32_t la = (uint32_t)(g * RA + j * LBA), lb = (uint32_t)(g * RB + j * LBB);
+  const uint32_t la = (uint32_t)(g * RA) + lane_offset<S, LBA>(j);
+  const uint32_t lb = (uint32_t)(g * RB) + lane_offset<S, LBB>(j);
 
   AccV acc[IB][JB];
   AccV hi[kF32 ? IB : 1][kF32 ? JB : 1], lo[kF32 ? IB : 1][kF32 ? JB : 1];
@@ -1223,8 +1246,8 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
   if (wid == W - 1 && a.n > 4 * F) {  // the last n % 4 rows
     const int64_t r = 4 * F + g;
     if (r < a.n) {
-      lane_load<LBA, NT>(wa[0], A8 + r * RA + (uint32_t)(j * LBA));
-      if constexpr (!SYM) lane_load<LBB, NT>(wb[0], B8 + r * RB + (uint32_t)(j * LBB));
+      lane_load<LBA, NT>(wa[0], A8 + r * RA + lane_offset<S, LBA>(j));
+      if constexpr (!SYM) lane_load<LBB, NT>(wb[0], B8 + r * RB + lane_offset<S, LBB>(j));
     } else {
 #pragma unroll
       for (int x = 0; x < LBA / 4; ++x) wa[0][x] = 0u;
@@ -1253,9 +1276,9 @@ __device__ __forceinline__ void direct_partial(const AtbArgs& a, double* red,
         for (int v = 0; v < 4; ++v) {
           if (!used(i, jj)) continue;
           const int r = MfmaTraits<S>::row_major_out ? 4 * g + v : g + 4 * v;
-          const int o = (r * IB + i) * MC + j * JB + jj;
-          f(o, value(i, jj, v));
-          if (SYM && i < jj) f((j * JB + jj) * MC + r * IB + i, value(i, jj, v));
+          const int ca = lane_col<S, LBA>(i, r), cb = lane_col<S, LBB>(jj, j);
+          f(ca * MC + cb, value(i, jj, v));
+          if (SYM && i < jj) f(cb * MC + ca, value(i, jj, v));
         }
   };
   if (wave >= 2) for_each([&](int o, double x) { red[(wave - 2) * KCMC + o] = x; });
