@@ -13,6 +13,12 @@ Prints ONE JSON line with GB/s (algorithmic bytes s*n*(k+m) + 8*k*m per
 call), the roofline of the kernel (achieved / 8 TB/s, timed with HIP events on
 the launch stream) and a CPU baseline: the reference's kernels restated for
 host cores (oracle/ref_kernels.cc) timed on the same n = 2^22 workload.
+
+Timing window: under back-to-back load from idle the chip lowers its shader
+clock about 2-4 ms in and restores it over some 30 ms (steps in that trough
+run 3-5% slower).  The default window, 600 untimed steps (about 47 ms) and
+then 1000 timed ones (79 ms), gives the sustained rate after that transient;
+--warmup 5 --steps 20 times the first 2 ms, before it.
 """
 from __future__ import annotations
 
@@ -188,8 +194,10 @@ def pmc_traffic(workload: str):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=600,
+                    help="untimed steps; the default runs past the clock trough of a load "
+                         "that starts from idle")
     ap.add_argument("--n", type=int, default=1 << 22)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--m", type=int, default=8)
