@@ -111,6 +111,9 @@ def test_backward_argument_errors():
     assert bwd(dB=FAKE + 2) == _lib.ERR_MISALIGNED
     assert bwd(g=_lib.BF16) == _lib.ERR_UNSUPPORTED
     assert bwd(B=None) == _lib.ERR_INVALID_ARGUMENT
+    # (n - 1) * ld would leave int64: refused before any launch
+    assert bwd(n=1 << 60, lda=64, k=4) == _lib.ERR_INVALID_ARGUMENT
+    assert bwd(n=1 << 59, lddb=64) == _lib.ERR_INVALID_ARGUMENT
     # nothing to do: succeeds without touching the device
     assert bwd(dA=None, dB=None) == _lib.SUCCESS
     assert bwd(n=0) == _lib.SUCCESS
@@ -155,6 +158,7 @@ def test_tsmm_argument_errors():
     assert tsmm(Y=None) == _lib.ERR_INVALID_ARGUMENT
     assert tsmm(Y=FAKE + 4) == _lib.ERR_MISALIGNED
     assert tsmm(x_type=_lib.BF16) == _lib.ERR_UNSUPPORTED
+    assert tsmm(n=1 << 60, ldy=64) == _lib.ERR_INVALID_ARGUMENT
     assert tsmm(n=0) == _lib.SUCCESS
 
 
@@ -164,6 +168,7 @@ def test_fill_argument_errors():
     assert L.tsm_fill_random(_lib.F64, 0, 1, 1, FAKE, 4, 4, 3, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_fill_random(_lib.F32, 0, 1, 1, FAKE + 2, 4, 4, 4, None) == _lib.ERR_MISALIGNED
     assert L.tsm_fill_random(9, 0, 1, 1, FAKE, 4, 4, 4, None) == _lib.ERR_UNSUPPORTED
+    assert L.tsm_fill_random(_lib.F64, 0, 1, 1, FAKE, 1 << 60, 8, 8, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.tsm_fill_random(_lib.F64, 0, 1, 1, None, 0, 4, 4, None) == _lib.SUCCESS
 
 

@@ -1101,6 +1101,12 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
   if (dA && ldda < k) return TSM_ERR_INVALID_ARGUMENT;
   if (dB && lddb < m) return TSM_ERR_INVALID_ARGUMENT;
   if (n > 0 && ((dA && !B) || (dB && !A))) return TSM_ERR_INVALID_ARGUMENT;
+  {  // row offsets (n - 1) * ld stay far from int64 overflow, as in tsm_atb
+    int64_t ld = lda > ldb ? lda : ldb;
+    if (dA && ldda > ld) ld = ldda;
+    if (dB && lddb > ld) ld = lddb;
+    if (n > ((int64_t)1 << 62) / ld) return TSM_ERR_INVALID_ARGUMENT;
+  }
   const uintptr_t es = (uintptr_t)elem(in_type);
   if ((uintptr_t)A % es || (uintptr_t)B % es || (uintptr_t)dA % es || (uintptr_t)dB % es)
     return TSM_ERR_MISALIGNED;
@@ -1135,9 +1141,9 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
                (!dB || al16(dB)))
                   ? 1
                   : 0;
-  int dev = 0, cus = 256;
-  if (hipGetDevice(&dev) == hipSuccess)
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int cus = 0;
+  const tsm_status cst = tsm_internal_device_cus(&cus);
+  if (cst != TSM_SUCCESS) return cst;
   int64_t blocks = (n + 255) / 256;
   // Staged when every row that moves is dense: both dense flags cover the
   // inputs read and the outputs written (rows of k * s and m * s bytes, each

@@ -574,6 +574,7 @@ tsm_status tsm_fill_random(tsm_dtype type, tsm_dist dist, uint64_t seed, uint64_
   if (!valid_in_type(type)) return TSM_ERR_UNSUPPORTED;
   if (dist != TSM_DIST_UNIFORM && dist != TSM_DIST_POW2) return TSM_ERR_INVALID_ARGUMENT;
   if (rows < 0 || cols < 1 || ld < cols) return TSM_ERR_INVALID_ARGUMENT;
+  if (rows > ((int64_t)1 << 62) / ld) return TSM_ERR_INVALID_ARGUMENT;
   if (rows > 0 && !X) return TSM_ERR_INVALID_ARGUMENT;
   if ((uintptr_t)X % (uintptr_t)elem_bytes(type)) return TSM_ERR_MISALIGNED;
   if (rows == 0) return TSM_SUCCESS;
