@@ -388,8 +388,9 @@ __device__ __forceinline__ void finish_small(const AtbArgs& a, int G, int MC, do
   }
 }
 
-// Finisher kernel body for classes of more than 512 outputs (launched after
-// the streaming kernel, kFinishOutputs outputs per workgroup).  Thread t sums
+// Finisher kernel body (launched after the streaming kernel for classes of
+// more than 512 outputs and for large grids, kFinishOutputs outputs per
+// workgroup).  Thread t sums
 // output o0 + t % 16 over slice t / 16 of the G workgroup partials, in
 // workgroup order, with up to 16 independent loads in flight (each wave reads
 // four partial rows of 16 consecutive outputs); the 16 slice sums are then
