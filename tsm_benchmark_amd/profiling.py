@@ -2,11 +2,12 @@
 
   python -m tsm_benchmark_amd.profiling atb  --n 33554432 --k 8 --m 8 --dtype f64 --reps 5
   python -m tsm_benchmark_amd.profiling copy --bytes 1073741824 --reps 5
-  python -m tsm_benchmark_amd.profiling bwd  --n 8388608 --k 16 --m 16 --dtype f32 --reps 5
+  python -m tsm_benchmark_amd.profiling bwd  --n 8388608 --k 16 --m 16 --dtype f32 --reps 5 [--dA-only]
   python -m tsm_benchmark_amd.profiling batched --dtype f64 --reps 5
 
 `atb --sets S` uses S input sets in rotation (as bench.py does) and
-`--back-to-back` launches all reps without a synchronisation between them.
+`--back-to-back` (atb, bwd, batched) launches all reps without a
+synchronisation between them.
 `batched` is the batched workload of the measurement suite (512 pairs, n
 log-uniform in [2^10, 2^21], k, m in {2, 4, 8, 16, 32}) as one plan launch per
 rep.
@@ -43,6 +44,8 @@ def main(argv=None) -> int:
     ap.add_argument("--bytes", type=int, default=1 << 30)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--dist", default="uniform", choices=["uniform", "pow2"])
+    ap.add_argument("--dA-only", dest="da_only", action="store_true",
+                    help="bwd: the gradient of A only (dB not computed)")
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
     if args.what == "copy":
@@ -95,10 +98,14 @@ def main(argv=None) -> int:
         print(f"atb: {args.reps} launches done", flush=True)
     else:
         G = torch.randn((args.k, args.m), dtype=torch.float32, device="cuda")
+        G = G.to(torch.float64 if dt == torch.float64 else torch.float32)
         for i in range(args.reps):
-            ops.atb_backward(A, B, G.to(torch.float64 if dt == torch.float64 else torch.float32))
-            torch.cuda.synchronize()
-            print(f"bwd {i}", flush=True)
+            ops.atb_backward(A, B, G, need_dB=not args.da_only)
+            if not args.back_to_back:
+                torch.cuda.synchronize()
+                print(f"bwd {i}", flush=True)
+        torch.cuda.synchronize()
+        print(f"bwd: {args.reps} launches done", flush=True)
     return 0
 
 
