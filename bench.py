@@ -4,14 +4,16 @@
 Headline workload (plain FP64): A is n x 8 and B is n x 8, FP64, row-major,
 n = 2^22 (A + B = 512 MiB), entries uniform on [-1, 1) from the counter-based
 generator (seed 20261203, generated on the device).  A step is one tsm_atb
-call (one kernel launch) over resident inputs.  Three such input sets (1.5 GiB,
+call over resident inputs: the streaming kernel, whose 256 workgroups each
+store a partial 8 x 8 sum, then the finisher kernel that adds the partials in
+a fixed order and writes C.  Three such input sets (1.5 GiB,
 six times the 256 MB of last-level cache) are used in rotation, so no step
 reads data that the previous one left in the cache; the same measurement on
 one set is reported beside it.
 
 Prints ONE JSON line with GB/s (algorithmic bytes s*n*(k+m) + 8*k*m per
-call), the roofline of the kernel (achieved / 8 TB/s, timed with HIP events on
-the launch stream) and a CPU baseline: the reference's kernels restated for
+call), the roofline of the step (achieved / 8 TB/s, both kernels timed with
+HIP events on the launch stream) and a CPU baseline: the reference's kernels restated for
 host cores (oracle/ref_kernels.cc) timed on the same n = 2^22 workload.
 
 Timing window: under back-to-back load from idle the chip lowers its shader
@@ -81,7 +83,7 @@ def cpu_baseline(n: int, k: int, m: int, budget_s: float = 24.0) -> dict:
 def fp64_rate(n: int, k: int, m: int, sets: int, steps: int, warmup: int) -> dict:
     """GB/s of tsm_atb over `sets` FP64 input sets used in rotation: `warmup`
     untimed steps, then `steps` steps between two HIP events on the launch
-    stream (one kernel launch per step)."""
+    stream (one tsm_atb call per step)."""
     import torch
 
     from tsm_benchmark_amd import _lib, ops
