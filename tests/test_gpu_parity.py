@@ -128,6 +128,28 @@ def test_tile_orders_and_grids(il, grid):
         check_close(C, host(S), host(B), torch.float64)
 
 
+@pytest.mark.parametrize("dt,k,m", [("f64", 8, 8), ("f64", 2, 2), ("f32", 16, 16), ("bf16", 8, 4)])
+def test_final_sum_protocols_at_their_boundary(dt, k, m):
+    """Grids below 256 workgroups (and below 16384 partial words) are summed by
+    the streaming kernel's last workgroup; from 256 on (or 16384 words) by the
+    finisher kernel.  Both sides of each boundary against the oracle, plain and
+    accumulating; a grid forced through the config reaches the same paths."""
+    kcmc = (1 << (k - 1).bit_length()) * (1 << (m - 1).bit_length())
+    idx, _ = _lib.plan(_lib.F64 if dt == "f64" else _lib.F32 if dt == "f32" else _lib.BF16, 1, k, m)
+    rows = _lib.kernels()[idx]["rows"]
+    small = min(255, 16383 // kcmc)
+    for grid in (small, small + 1, 256, 1000):
+        n = grid * rows - 7  # the last tile is partial
+        A, B = rand(n, k, DT[dt], 11), rand(n, m, DT[dt], 12)
+        cfg = _lib.make_config(grid=grid)
+        C = ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy()
+        check_close(C, host(A), host(B), DT[dt])
+        C0 = torch.full((k, m), 0.25, dtype=torch.float64, device="cuda")
+        ops.atb(A, B, out=C0, accumulate=True, config=cfg)
+        ref = C + 0.25
+        assert np.all(np.abs(C0.cpu().numpy() - ref) <= np.spacing(np.abs(ref)) * 2)
+
+
 @pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
 def test_strided_views_and_misaligned_bases(dt):
     n = 50021
