@@ -112,7 +112,11 @@ typedef struct {
  *             first 256 bytes are zero before its first use (the library
  *             leaves them zero after each call).  Calls sharing a workspace
  *             must be ordered on one stream.
- * Asynchronous on `stream`; no allocation when a workspace is given. */
+ * Asynchronous on `stream`; no allocation when a workspace is given.  A call
+ * is one kernel launch (short sums: the last workgroup adds the workgroup
+ * partials) or two (grids of 256 workgroups or more, 16384 partial words or
+ * more, or more than 512 outputs: a finisher kernel adds them); either form
+ * can be captured in a HIP graph when a workspace is given. */
 tsm_status tsm_atb(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, int m,
                    const void* A, int64_t lda, const void* B, int64_t ldb,
                    void* C, int64_t ldc, int accumulate,
