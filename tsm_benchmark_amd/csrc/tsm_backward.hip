@@ -1196,7 +1196,8 @@ extern "C" tsm_status tsm_atb_backward(tsm_dtype in_type, tsm_dtype grad_type, i
       // of inputs, FP32 W = 16 both gradients 372.1 -> 359.5 us and dA only
       // 179.9 -> 177.8 us, FP64 and FP32 W = 32 383.1 / 381.6 -> 369.7 /
       // 364.7 us, against four per CU.
-      if (wi >= 4) nb = nb < (in_type == TSM_BF16 ? 3 : 2) ? nb : (in_type == TSM_BF16 ? 3 : 2);
+      const int most = in_type == TSM_BF16 ? 3 : 2;
+      if (wi >= 4 && nb > most) nb = most;
       per_cu = nb;
       resident[(int)in_type][wi].store(per_cu, std::memory_order_relaxed);
     }
