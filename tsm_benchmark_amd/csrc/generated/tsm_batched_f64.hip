@@ -48,10 +48,10 @@ extern "C" __global__ void __launch_bounds__(256) tsm_atb_batched_f64(tsm::Batch
       case 34: tsm::batched_item_call<double, 16, 64, 0, 0, 64, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_16x64_mfma_r64_p2_nt
       case 35: tsm::batched_item_call<double, 32, 1, 8, 1, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x1_t8x1_r128_p2_nt
       case 36: tsm::batched_item_call<double, 32, 2, 8, 2, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x2_t8x2_r128_p2_nt
-      case 37: tsm::batched_item_call<double, 32, 4, 8, 4, 128, 1, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x4_t8x4_r128_p1_nt
+      case 37: tsm::batched_item_call<double, 32, 4, 4, 4, 128, 4, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x4_t4x4_r128_p4_nt
       case 38: tsm::batched_item_call<double, 32, 8, 8, 8, 128, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x8_t8x8_r128_p2_nt
       case 39: tsm::batched_item_call<double, 32, 16, 8, 8, 64, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x16_t8x8_r64_p2_nt
-      case 40: tsm::batched_item_call<double, 32, 32, 0, 0, 64, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x32_mfma_r64_p2_nt
+      case 40: tsm::batched_item_call<double, 32, 32, 0, 0, 64, 1, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x32_mfma_r64_p1_nt
       case 41: tsm::batched_item_call<double, 32, 64, 0, 0, 32, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_32x64_mfma_r32_p2_nt
       case 42: tsm::batched_item_call<double, 64, 1, 8, 1, 64, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_64x1_t8x1_r64_p2_nt
       case 43: tsm::batched_item_call<double, 64, 2, 8, 2, 64, 2, 1>(ba, it, smem, &s_flag, &s_next); break;  // tsm_atb_f64_64x2_t8x2_r64_p2_nt

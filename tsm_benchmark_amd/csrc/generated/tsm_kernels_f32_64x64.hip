@@ -243,12 +243,12 @@ hipError_t launch_tsm_atb_f32_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs& a, unsign
   return hipGetLastError();
 }
 
-// tsm_ata_f32_64x64_mfma_r64_p2_nt: KC=64 MC=64 TM=0 TN=0 rows=64 pf=2 nt=1
+// tsm_ata_f32_64x64_mfma_r64_p2: KC=64 MC=64 TM=0 TN=0 rows=64 pf=2 nt=0
 //   matrix cores: 16 K-steps of 4 rows per tile, LDS 32768 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_64x64_mfma_r64_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<float, 64, 64, 0, 0, 64, 2, 1, 1>(a);
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_64x64_mfma_r64_p2(tsm::AtbArgs a) {
+  tsm::atb_body<float, 64, 64, 0, 0, 64, 2, 0, 1>(a);
 }
-hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f32_64x64_mfma_r64_p2_nt, dim3(grid), dim3(256), 0, s, a);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_64x64_mfma_r64_p2, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

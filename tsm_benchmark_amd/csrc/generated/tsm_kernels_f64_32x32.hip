@@ -543,6 +543,46 @@ hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs& a, unsig
   return hipGetLastError();
 }
 
+// tsm_ata_f64_32x32_mfma_r16_p1_nt: KC=32 MC=32 TM=0 TN=0 rows=16 pf=1 nt=1
+//   matrix cores: 4 K-steps of 4 rows per tile, LDS 8192 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r16_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 32, 32, 0, 0, 16, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r16_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_32x32_mfma_r16_p2_nt: KC=32 MC=32 TM=0 TN=0 rows=16 pf=2 nt=1
+//   matrix cores: 4 K-steps of 4 rows per tile, LDS 8192 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r16_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 32, 32, 0, 0, 16, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r16_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_32x32_mfma_r32_p1_nt: KC=32 MC=32 TM=0 TN=0 rows=32 pf=1 nt=1
+//   matrix cores: 8 K-steps of 4 rows per tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r32_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 32, 32, 0, 0, 32, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r32_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f64_32x32_mfma_r32_p2_nt: KC=32 MC=32 TM=0 TN=0 rows=32 pf=2 nt=1
+//   matrix cores: 8 K-steps of 4 rows per tile, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r32_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<double, 32, 32, 0, 0, 32, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r32_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_ata_f64_32x32_mfma_r64_p1_nt: KC=32 MC=32 TM=0 TN=0 rows=64 pf=1 nt=1
 //   matrix cores: 16 K-steps of 4 rows per tile, LDS 32768 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r64_p1_nt(tsm::AtbArgs a) {
@@ -580,25 +620,5 @@ extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r128_p2
 }
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
   hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r128_p2_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_32x32_mfma_r256_p1_nt: KC=32 MC=32 TM=0 TN=0 rows=256 pf=1 nt=1
-//   matrix cores: 64 K-steps of 4 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r256_p1_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 32, 32, 0, 0, 256, 1, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r256_p1_nt, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// tsm_ata_f64_32x32_mfma_r256_p2_nt: KC=32 MC=32 TM=0 TN=0 rows=256 pf=2 nt=1
-//   matrix cores: 64 K-steps of 4 rows per tile, LDS 131072 B
-extern "C" __global__ void __launch_bounds__(256) tsm_ata_f64_32x32_mfma_r256_p2_nt(tsm::AtbArgs a) {
-  tsm::atb_body<double, 32, 32, 0, 0, 256, 2, 1, 1>(a);
-}
-hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
-  hipLaunchKernelGGL(tsm_ata_f64_32x32_mfma_r256_p2_nt, dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }

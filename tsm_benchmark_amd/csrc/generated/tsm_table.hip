@@ -830,7 +830,7 @@ hipError_t launch_tsm_ata_f32_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f32_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_64x64_mfma_r64_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -864,12 +864,14 @@ hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
-hipError_t launch_tsm_ata_f64_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_64x64_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1706,7 +1708,7 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f32_64x64_direct_d4_nt", 1, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_ata_f32_64x64_direct_d4_nt, 3},
   {"tsm_ata_f32_64x64_direct_d8_nt", 1, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_ata_f32_64x64_direct_d8_nt, 3},
   {"tsm_ata_f32_64x64_direct_d16_nt", 1, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_ata_f32_64x64_direct_d16_nt, 3},
-  {"tsm_ata_f32_64x64_mfma_r64_p2_nt", 1, 64, 64, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2_nt, 2},
+  {"tsm_ata_f32_64x64_mfma_r64_p2", 1, 64, 64, 0, 0, 64, 2, 0, 32768, launch_tsm_ata_f32_64x64_mfma_r64_p2, 2},
   {"tsm_ata_f64_1x1_t1x1_r2048_p1_nt", 0, 1, 1, 1, 1, 2048, 1, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p1_nt, 1},
   {"tsm_ata_f64_1x1_t1x1_r2048_p2_nt", 0, 1, 1, 1, 1, 2048, 2, 1, 32768, launch_tsm_ata_f64_1x1_t1x1_r2048_p2_nt, 1},
   {"tsm_ata_f64_1x1_t1x1_r4096_p1_nt", 0, 1, 1, 1, 1, 4096, 1, 1, 65536, launch_tsm_ata_f64_1x1_t1x1_r4096_p1_nt, 1},
@@ -1740,12 +1742,14 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt, 2},
+  {"tsm_ata_f64_32x32_mfma_r32_p2_nt", 0, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r64_p1_nt", 0, 32, 32, 0, 0, 64, 1, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r64_p2_nt", 0, 32, 32, 0, 0, 64, 2, 1, 32768, launch_tsm_ata_f64_32x32_mfma_r64_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p1_nt", 0, 32, 32, 0, 0, 128, 1, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r128_p2_nt", 0, 32, 32, 0, 0, 128, 2, 1, 65536, launch_tsm_ata_f64_32x32_mfma_r128_p2_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r256_p1_nt", 0, 32, 32, 0, 0, 256, 1, 1, 131072, launch_tsm_ata_f64_32x32_mfma_r256_p1_nt, 2},
-  {"tsm_ata_f64_32x32_mfma_r256_p2_nt", 0, 32, 32, 0, 0, 256, 2, 1, 131072, launch_tsm_ata_f64_32x32_mfma_r256_p2_nt, 2},
   {"tsm_ata_f64_64x64_direct_d4_nt", 0, 64, 64, 0, 0, 4, 4, 1, 65536, launch_tsm_ata_f64_64x64_direct_d4_nt, 3},
   {"tsm_ata_f64_64x64_direct_d8_nt", 0, 64, 64, 0, 0, 4, 8, 1, 65536, launch_tsm_ata_f64_64x64_direct_d8_nt, 3},
   {"tsm_ata_f64_64x64_direct_d16_nt", 0, 64, 64, 0, 0, 4, 16, 1, 65536, launch_tsm_ata_f64_64x64_direct_d16_nt, 3},
@@ -1756,7 +1760,7 @@ static const tsm::TunedEntry kSymTuned[] = {
   {2, 1, 1, 0, 1, 1, 0, 1},  // tsm_ata_bf16_1x1_t1x1_r4096_p2_nt
   {2, 2, 2, 1, 1, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2_nt
   {2, 4, 4, 2, 1, 1, 2, 1},  // tsm_ata_bf16_4x4_t4x4_r4096_p2_nt
-  {2, 8, 8, 6, 6, 1, 6, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p2_nt
+  {2, 8, 8, 5, 6, 1, 5, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
   {2, 16, 16, 10, 1, 1, 10, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
   {2, 32, 32, 16, 1, 1, 16, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
   {2, 64, 64, 22, 1, 0, 23, 1},  // tsm_ata_bf16_64x64_direct_d4_nt (else tsm_ata_bf16_64x64_mfma_r256_p2_nt)
@@ -1764,16 +1768,16 @@ static const tsm::TunedEntry kSymTuned[] = {
   {1, 2, 2, 25, 1, 1, 25, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
   {1, 4, 4, 26, 1, 1, 26, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
   {1, 8, 8, 29, 2, 0, 29, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
-  {1, 16, 16, 38, 1, 0, 38, 0},  // tsm_ata_f32_16x16_mfma_r256_p2_nt
-  {1, 32, 32, 42, 3, 0, 42, 0},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
-  {1, 64, 64, 47, 2, 0, 48, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2_nt)
+  {1, 16, 16, 37, 2, 0, 37, 0},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
+  {1, 32, 32, 42, 4, 0, 42, 0},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
+  {1, 64, 64, 47, 2, 0, 48, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2)
   {0, 1, 1, 49, 2, 1, 49, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
   {0, 2, 2, 54, 1, 1, 54, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
   {0, 4, 4, 62, 1, 1, 62, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
   {0, 8, 8, 73, 1, 0, 73, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
   {0, 16, 16, 79, 1, 1, 79, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 84, 2, 0, 84, 0},  // tsm_ata_f64_32x32_mfma_r128_p1_nt
-  {0, 64, 64, 90, 2, 0, 91, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
+  {0, 32, 32, 86, 2, 0, 86, 0},  // tsm_ata_f64_32x32_mfma_r64_p1_nt
+  {0, 64, 64, 92, 2, 0, 93, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
 };
 
 static const tsm::TunedEntry kTuned[] = {
@@ -1782,8 +1786,8 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 1, 4, 2, 1, 1, 2, 1},  // tsm_atb_bf16_1x4_t1x4_r2048_p2_nt
   {2, 1, 8, 3, 2, 1, 3, 1},  // tsm_atb_bf16_1x8_t1x4_r1024_p2_nt
   {2, 1, 16, 4, 1, 1, 4, 1},  // tsm_atb_bf16_1x16_t1x4_r512_p2_nt
-  {2, 1, 32, 5, 1, 1, 5, 1},  // tsm_atb_bf16_1x32_t1x4_r256_p2_nt
-  {2, 1, 64, 6, 1, 1, 6, 1},  // tsm_atb_bf16_1x64_t1x4_r128_p2_nt
+  {2, 1, 32, 5, 2, 1, 5, 1},  // tsm_atb_bf16_1x32_t1x4_r256_p2_nt
+  {2, 1, 64, 6, 2, 1, 6, 1},  // tsm_atb_bf16_1x64_t1x4_r128_p2_nt
   {2, 2, 1, 7, 1, 0, 7, 0},  // tsm_atb_bf16_2x1_t2x1_r4096_p2_nt
   {2, 2, 2, 8, 1, 0, 8, 0},  // tsm_atb_bf16_2x2_t2x2_r4096_p2_nt
   {2, 2, 4, 9, 1, 1, 9, 1},  // tsm_atb_bf16_2x4_t2x4_r2048_p2_nt
@@ -1824,7 +1828,7 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 64, 4, 199, 2, 1, 199, 1},  // tsm_atb_bf16_64x4_t8x4_r128_p2_nt
   {2, 64, 8, 200, 2, 1, 200, 1},  // tsm_atb_bf16_64x8_t8x4_r128_p2_nt
   {2, 64, 16, 202, 1, 1, 202, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
-  {2, 64, 32, 204, 2, 1, 204, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
+  {2, 64, 32, 204, 1, 1, 204, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
   {2, 64, 64, 206, 1, 0, 214, 1},  // tsm_atb_bf16_64x64_direct_d4_nt (else tsm_atb_bf16_64x64_mfma_r256_p2_nt)
   {1, 1, 1, 227, 1, 1, 227, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
   {1, 1, 2, 228, 1, 1, 228, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
@@ -1867,14 +1871,14 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 32, 8, 363, 1, 0, 363, 0},  // tsm_atb_f32_32x8_t8x4_r128_p2_nt
   {1, 32, 16, 364, 1, 0, 365, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
   {1, 32, 32, 376, 1, 1, 376, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 424, 2, 1, 424, 1},  // tsm_atb_f32_32x64_mfma_r64_p2_nt
+  {1, 32, 64, 423, 1, 0, 424, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2_nt)
   {1, 64, 1, 425, 2, 1, 425, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
   {1, 64, 2, 426, 2, 1, 426, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
   {1, 64, 4, 427, 2, 1, 427, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
   {1, 64, 8, 428, 2, 1, 428, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
   {1, 64, 16, 430, 1, 1, 430, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
   {1, 64, 32, 431, 1, 0, 432, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
-  {1, 64, 64, 435, 1, 0, 443, 1},  // tsm_atb_f32_64x64_direct_d16_nt (else tsm_atb_f32_64x64_mfma_r64_p2_nt)
+  {1, 64, 64, 434, 1, 0, 442, 1},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2)
   {0, 1, 1, 471, 1, 1, 471, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
   {0, 1, 2, 474, 1, 1, 474, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
   {0, 1, 4, 475, 1, 1, 475, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
@@ -1912,10 +1916,10 @@ static const tsm::TunedEntry kTuned[] = {
   {0, 16, 64, 659, 1, 1, 659, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
   {0, 32, 1, 660, 2, 1, 660, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
   {0, 32, 2, 661, 2, 1, 661, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
-  {0, 32, 4, 674, 2, 1, 674, 1},  // tsm_atb_f64_32x4_t8x4_r64_p1_nt
+  {0, 32, 4, 665, 1, 1, 665, 1},  // tsm_atb_f64_32x4_t4x4_r32_p4_nt
   {0, 32, 8, 682, 1, 1, 682, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
   {0, 32, 16, 683, 1, 0, 684, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
-  {0, 32, 32, 699, 1, 0, 699, 0},  // tsm_atb_f64_32x32_mfma_r128_p2_nt
+  {0, 32, 32, 689, 2, 1, 689, 1},  // tsm_atb_f64_32x32_mfma_r32_p1_nt
   {0, 32, 64, 742, 1, 0, 743, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
   {0, 64, 1, 744, 2, 1, 744, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
   {0, 64, 2, 745, 2, 1, 745, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
