@@ -1,11 +1,14 @@
-"""Every width pair on the GPU (run on an MI355X: -m gpu).
+"""Every width pair and every short row count on the GPU (run on an MI355X:
+-m gpu).
 
 The library maps each (k, m) with k, m <= 64 onto a generated class (exact,
 padded or split into column blocks) and each backward width onto one of its
 kernels, so every pair takes a mapping path of its own.  These tests run all
 4096 pairs of C = A^T B per storage type, dense and as strided views, in
 single calls and in one batched launch, every A^T A width, and both
-gradients of every pair, each against the oracle.
+gradients of every pair, each against the oracle.  Then every n up to 520
+and every n within two rows of one to three tiles, for forward classes of
+each engine and backward classes of each kernel.
 
 One oracle call serves every pair: C_ij, dA_ij and dB_ij depend only on the
 columns they name, so the 64 x 64 result (or, for the gradients, the result
@@ -186,3 +189,86 @@ def test_every_width_pair_backward(dt):
     excess = torch.stack(excess).cpu().numpy()
     bad = [PAIRS[i] for i in np.flatnonzero(excess > 0)]
     assert not bad, f"{len(bad)} failing (k, m), first: {bad[:8]}, worst excess {excess.max():.3g}"
+
+
+# ---- every short row count -----------------------------------------------------------------
+# (storage, k, m, tile rows of the tuned forward kernel): VALU register tiles,
+# LDS-staged and direct matrix-core kernels, tiles of 4 to 4096 rows
+FWD_CLASSES = [("f64", 8, 8, 128), ("f64", 1, 1, 4096), ("f64", 32, 32, 32), ("f64", 64, 64, 4),
+               ("f32", 4, 4, 1024), ("f32", 16, 16, 128), ("f32", 64, 64, 4),
+               ("bf16", 1, 1, 4096), ("bf16", 32, 32, 512), ("bf16", 64, 64, 32)]
+
+
+def row_counts(tile):
+    """Every n up to 520, and n within 2 of one, two and three tiles."""
+    ns = set(range(521))
+    for j in (1, 2, 3):
+        ns.update(j * tile + d for d in range(-2, 3))
+    return sorted(ns)
+
+
+@pytest.mark.parametrize("cls", FWD_CLASSES, ids=lambda c: f"{c[0]}-{c[1]}x{c[2]}")
+def test_every_short_row_count_forward(cls):
+    """Each tail length of the dense kernels: n rows are a prefix of one
+    matrix pair, one launch per n, each against the oracle on that prefix."""
+    dt, k, m, tile = cls
+    ns = row_counts(tile)
+    A = ops.random_matrix(ns[-1], k, DT[dt], seed=SEED, stream_id=1)
+    B = ops.random_matrix(ns[-1], m, DT[dt], seed=SEED, stream_id=2)
+    out = [ops.atb(A[:n], B[:n], out_dtype=torch.float64) for n in ns]
+    torch.cuda.synchronize()
+    Ah, Bh = host(A), host(B)
+    bad = []
+    for n, C in zip(ns, out):
+        C = C.cpu().numpy()
+        if n == 0:
+            ok = np.array_equal(C, np.zeros((k, m)))
+        else:
+            r, absC = oracle.atb_dot2(Ah[:n], Bh[:n], with_abs=True)
+            ok = np.all(np.abs(C - r) <= 4.0 * np.sqrt(n) * EPS[dt] * absC)
+            if ok and dt != "f64" and np.linalg.norm(r) > 0:
+                ok = np.linalg.norm(C - r) / np.linalg.norm(r) <= 1e-6
+        if not ok:
+            bad.append(n)
+    assert not bad, f"{len(bad)} failing row counts, first: {bad[:10]}"
+
+
+# packed (rows of at most 16 bytes), LDS-staged and matrix-core backward
+# kernels; their tiles are 128 or 256 rows
+BWD_CLASSES = [("f64", 2, 2), ("f64", 4, 4), ("f64", 8, 8), ("f64", 16, 16), ("f64", 64, 64),
+               ("f32", 4, 4), ("f32", 16, 16), ("f32", 32, 32), ("bf16", 8, 8), ("bf16", 32, 32),
+               ("bf16", 64, 16)]
+
+
+@pytest.mark.parametrize("cls", BWD_CLASSES, ids=lambda c: f"{c[0]}-{c[1]}x{c[2]}")
+def test_every_short_row_count_backward(cls):
+    """Each tail length of the backward kernels.  Row r of dA (dB) depends on
+    row r of B (A) only, so the oracle at the largest n serves every prefix."""
+    dt, k, m = cls
+    ns = row_counts(256)
+    N = ns[-1]
+    gdt = torch.float64 if dt == "f64" else torch.float32
+    A = ops.random_matrix(N, k, DT[dt], seed=SEED, stream_id=1)
+    B = ops.random_matrix(N, m, DT[dt], seed=SEED, stream_id=2)
+    G = torch.from_numpy(oracle.gen_matrix(k, m, dist=oracle.DIST_NORMAL, seed=SEED, stream=3)).to(gdt)
+    dA_ref, dB_ref = oracle.atb_backward(host(A), host(B), G.double().numpy())
+    dA_ref, dB_ref = torch.from_numpy(dA_ref).to("cuda"), torch.from_numpy(dB_ref).to("cuda")
+    if dt == "f64":
+        rtol = atol = 1e-13
+    elif dt == "f32":
+        rtol = atol = 1e-5
+    else:
+        rtol, atol = 2.0**-7, 2.0**-7 * float(np.abs(G.double().numpy()).sum())
+    Gd = G.to("cuda")
+    excess = []
+    for n in ns:
+        dA, dB = ops.atb_backward(A[:n], B[:n], Gd)
+        if n == 0:
+            continue
+        ra, rb = dA_ref[:n], dB_ref[:n]
+        ea = ((dA.double() - ra).abs() - rtol * ra.abs() - atol).max()
+        eb = ((dB.double() - rb).abs() - rtol * rb.abs() - atol).max()
+        excess.append(torch.maximum(ea, eb))
+    excess = torch.stack(excess).cpu().numpy()
+    bad = [n for n, e in zip(ns[1:], excess) if e > 0]
+    assert not bad, f"{len(bad)} failing row counts, first: {bad[:10]}"
