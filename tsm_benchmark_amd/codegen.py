@@ -347,11 +347,15 @@ def load_tuning(path: str = TUNING_JSON) -> dict:
 def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
     """A^T A variants the autotuner chooses from for a square tuned class: the
     tuned A^T B consumer of the class with half, one, two and four times its
-    rows per tile and one or two tiles in flight."""
+    rows per tile and one or two tiles in flight, and the direct engine's
+    kernels of the class (matrix-core widths).  A^T A reads half the bytes of
+    A^T B per row, so the direct engine, which keeps every row's operands in
+    registers, can win where A^T B prefers the staged kernel."""
     c = defaults[(dtype, kc, kc)][0]
+    direct = {dataclasses.replace(v, sym=1) for v in direct_variants(dtype, kc, kc)}
     if c.direct:
-        return sorted(dataclasses.replace(v, sym=1) for v in direct_variants(dtype, kc, kc))
-    out = set()
+        return sorted(direct)
+    out = set(direct)
     for rows in (c.rows // 2, c.rows, 2 * c.rows, 4 * c.rows):
         for pf in (1, 2):
             v = dataclasses.replace(c, sym=1, rows=rows, pf=pf)

@@ -138,6 +138,60 @@ hipError_t launch_tsm_atb_f32_64x64_direct_d16_nt(const tsm::AtbArgs& a, unsigne
   return hipGetLastError();
 }
 
+// tsm_ata_f32_16x16_direct_d4_nt: KC=16 MC=16, matrix cores fed from global memory, 4 4-row steps in flight per wave, nt=1, LDS 4096 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_16x16_direct_d4_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 16, 16, 4, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_16x16_direct_d4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_16x16_direct_d4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f32_16x16_direct_d8_nt: KC=16 MC=16, matrix cores fed from global memory, 8 4-row steps in flight per wave, nt=1, LDS 4096 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_16x16_direct_d8_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 16, 16, 8, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_16x16_direct_d8_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_16x16_direct_d8_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f32_16x16_direct_d16_nt: KC=16 MC=16, matrix cores fed from global memory, 16 4-row steps in flight per wave, nt=1, LDS 4096 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_16x16_direct_d16_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 16, 16, 16, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_16x16_direct_d16_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_16x16_direct_d16_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f32_32x32_direct_d4_nt: KC=32 MC=32, matrix cores fed from global memory, 4 4-row steps in flight per wave, nt=1, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_32x32_direct_d4_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 32, 32, 4, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_32x32_direct_d4_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_32x32_direct_d4_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f32_32x32_direct_d8_nt: KC=32 MC=32, matrix cores fed from global memory, 8 4-row steps in flight per wave, nt=1, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_32x32_direct_d8_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 32, 32, 8, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_32x32_direct_d8_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_32x32_direct_d8_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_f32_32x32_direct_d16_nt: KC=32 MC=32, matrix cores fed from global memory, 16 4-row steps in flight per wave, nt=1, LDS 16384 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_32x32_direct_d16_nt(tsm::AtbArgs a) {
+  tsm::direct_body<float, 32, 32, 16, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_f32_32x32_direct_d16_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_f32_32x32_direct_d16_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_ata_f32_64x64_direct_d4_nt: KC=64 MC=64, matrix cores fed from global memory, 4 4-row steps in flight per wave, nt=1, LDS 65536 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_f32_64x64_direct_d4_nt(tsm::AtbArgs a) {
   tsm::direct_body<float, 64, 64, 4, 1, 1>(a);

@@ -791,12 +791,16 @@ hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_direct_d2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_direct_d2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -815,12 +819,18 @@ hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_32x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -856,6 +866,9 @@ hipError_t launch_tsm_ata_f64_8x8_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_ata_f64_8x8_t4x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_16x16_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -864,6 +877,9 @@ hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_ata_f64_16x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_16x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_direct_d4_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_32x32_direct_d16_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_32x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1669,12 +1685,16 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_bf16_8x8_t8x4_r1024_p2_nt", 2, 8, 8, 8, 4, 1024, 2, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p2_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r2048_p1_nt", 2, 8, 8, 8, 4, 2048, 1, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p1_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r2048_p2_nt", 2, 8, 8, 8, 4, 2048, 2, 1, 65536, launch_tsm_ata_bf16_8x8_t8x4_r2048_p2_nt, 1},
+  {"tsm_ata_bf16_16x16_direct_d2_nt", 2, 16, 16, 0, 0, 32, 2, 1, 8192, launch_tsm_ata_bf16_16x16_direct_d2_nt, 3},
+  {"tsm_ata_bf16_16x16_direct_d4_nt", 2, 16, 16, 0, 0, 32, 4, 1, 8192, launch_tsm_ata_bf16_16x16_direct_d4_nt, 3},
   {"tsm_ata_bf16_16x16_mfma_r512_p1_nt", 2, 16, 16, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p1_nt, 2},
   {"tsm_ata_bf16_16x16_mfma_r512_p2_nt", 2, 16, 16, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_16x16_mfma_r512_p2_nt, 2},
   {"tsm_ata_bf16_16x16_mfma_r1024_p1_nt", 2, 16, 16, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p1_nt, 2},
   {"tsm_ata_bf16_16x16_mfma_r1024_p2_nt", 2, 16, 16, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_16x16_mfma_r1024_p2_nt, 2},
   {"tsm_ata_bf16_16x16_mfma_r2048_p1_nt", 2, 16, 16, 0, 0, 2048, 1, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p1_nt, 2},
   {"tsm_ata_bf16_16x16_mfma_r2048_p2_nt", 2, 16, 16, 0, 0, 2048, 2, 1, 131072, launch_tsm_ata_bf16_16x16_mfma_r2048_p2_nt, 2},
+  {"tsm_ata_bf16_32x32_direct_d2_nt", 2, 32, 32, 0, 0, 32, 2, 1, 16384, launch_tsm_ata_bf16_32x32_direct_d2_nt, 3},
+  {"tsm_ata_bf16_32x32_direct_d4_nt", 2, 32, 32, 0, 0, 32, 4, 1, 16384, launch_tsm_ata_bf16_32x32_direct_d4_nt, 3},
   {"tsm_ata_bf16_32x32_mfma_r256_p1_nt", 2, 32, 32, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p1_nt, 2},
   {"tsm_ata_bf16_32x32_mfma_r256_p2_nt", 2, 32, 32, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_bf16_32x32_mfma_r256_p2_nt, 2},
   {"tsm_ata_bf16_32x32_mfma_r512_p1_nt", 2, 32, 32, 0, 0, 512, 1, 1, 65536, launch_tsm_ata_bf16_32x32_mfma_r512_p1_nt, 2},
@@ -1693,12 +1713,18 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f32_8x8_t8x4_r1024_p2_nt", 1, 8, 8, 8, 4, 1024, 2, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p2_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r2048_p1_nt", 1, 8, 8, 8, 4, 2048, 1, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p1_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r2048_p2_nt", 1, 8, 8, 8, 4, 2048, 2, 1, 131072, launch_tsm_ata_f32_8x8_t8x4_r2048_p2_nt, 1},
+  {"tsm_ata_f32_16x16_direct_d4_nt", 1, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_ata_f32_16x16_direct_d4_nt, 3},
+  {"tsm_ata_f32_16x16_direct_d8_nt", 1, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_ata_f32_16x16_direct_d8_nt, 3},
+  {"tsm_ata_f32_16x16_direct_d16_nt", 1, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_ata_f32_16x16_direct_d16_nt, 3},
   {"tsm_ata_f32_16x16_mfma_r64_p1_nt", 1, 16, 16, 0, 0, 64, 1, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p1_nt, 2},
   {"tsm_ata_f32_16x16_mfma_r64_p2_nt", 1, 16, 16, 0, 0, 64, 2, 1, 8192, launch_tsm_ata_f32_16x16_mfma_r64_p2_nt, 2},
   {"tsm_ata_f32_16x16_mfma_r128_p1_nt", 1, 16, 16, 0, 0, 128, 1, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p1_nt, 2},
   {"tsm_ata_f32_16x16_mfma_r128_p2_nt", 1, 16, 16, 0, 0, 128, 2, 1, 16384, launch_tsm_ata_f32_16x16_mfma_r128_p2_nt, 2},
   {"tsm_ata_f32_16x16_mfma_r256_p1_nt", 1, 16, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p1_nt, 2},
   {"tsm_ata_f32_16x16_mfma_r256_p2_nt", 1, 16, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_f32_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f32_32x32_direct_d4_nt", 1, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_ata_f32_32x32_direct_d4_nt, 3},
+  {"tsm_ata_f32_32x32_direct_d8_nt", 1, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_ata_f32_32x32_direct_d8_nt, 3},
+  {"tsm_ata_f32_32x32_direct_d16_nt", 1, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_ata_f32_32x32_direct_d16_nt, 3},
   {"tsm_ata_f32_32x32_mfma_r64_p1_nt", 1, 32, 32, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p1_nt, 2},
   {"tsm_ata_f32_32x32_mfma_r64_p2_nt", 1, 32, 32, 0, 0, 64, 2, 1, 16384, launch_tsm_ata_f32_32x32_mfma_r64_p2_nt, 2},
   {"tsm_ata_f32_32x32_mfma_r128_p1_nt", 1, 32, 32, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f32_32x32_mfma_r128_p1_nt, 2},
@@ -1734,6 +1760,9 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_8x8_t4x4_r512_p1_nt", 0, 8, 8, 4, 4, 512, 1, 1, 65536, launch_tsm_ata_f64_8x8_t4x4_r512_p1_nt, 1},
   {"tsm_ata_f64_8x8_t4x4_r512_p2_nt", 0, 8, 8, 4, 4, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t4x4_r512_p2_nt, 1},
   {"tsm_ata_f64_8x8_t8x8_r512_p2_nt", 0, 8, 8, 8, 8, 512, 2, 1, 65536, launch_tsm_ata_f64_8x8_t8x8_r512_p2_nt, 1},
+  {"tsm_ata_f64_16x16_direct_d4_nt", 0, 16, 16, 0, 0, 4, 4, 1, 4096, launch_tsm_ata_f64_16x16_direct_d4_nt, 3},
+  {"tsm_ata_f64_16x16_direct_d8_nt", 0, 16, 16, 0, 0, 4, 8, 1, 4096, launch_tsm_ata_f64_16x16_direct_d8_nt, 3},
+  {"tsm_ata_f64_16x16_direct_d16_nt", 0, 16, 16, 0, 0, 4, 16, 1, 4096, launch_tsm_ata_f64_16x16_direct_d16_nt, 3},
   {"tsm_ata_f64_16x16_mfma_r32_p1_nt", 0, 16, 16, 0, 0, 32, 1, 1, 8192, launch_tsm_ata_f64_16x16_mfma_r32_p1_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r32_p2_nt", 0, 16, 16, 0, 0, 32, 2, 1, 8192, launch_tsm_ata_f64_16x16_mfma_r32_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r64_p1_nt", 0, 16, 16, 0, 0, 64, 1, 1, 16384, launch_tsm_ata_f64_16x16_mfma_r64_p1_nt, 2},
@@ -1742,6 +1771,9 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_16x16_mfma_r128_p2_nt", 0, 16, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_16x16_mfma_r128_p2_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p1_nt", 0, 16, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p1_nt, 2},
   {"tsm_ata_f64_16x16_mfma_r256_p2_nt", 0, 16, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_16x16_mfma_r256_p2_nt, 2},
+  {"tsm_ata_f64_32x32_direct_d4_nt", 0, 32, 32, 0, 0, 4, 4, 1, 16384, launch_tsm_ata_f64_32x32_direct_d4_nt, 3},
+  {"tsm_ata_f64_32x32_direct_d8_nt", 0, 32, 32, 0, 0, 4, 8, 1, 16384, launch_tsm_ata_f64_32x32_direct_d8_nt, 3},
+  {"tsm_ata_f64_32x32_direct_d16_nt", 0, 32, 32, 0, 0, 4, 16, 1, 16384, launch_tsm_ata_f64_32x32_direct_d16_nt, 3},
   {"tsm_ata_f64_32x32_mfma_r16_p1_nt", 0, 32, 32, 0, 0, 16, 1, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p1_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r16_p2_nt", 0, 32, 32, 0, 0, 16, 2, 1, 8192, launch_tsm_ata_f64_32x32_mfma_r16_p2_nt, 2},
   {"tsm_ata_f64_32x32_mfma_r32_p1_nt", 0, 32, 32, 0, 0, 32, 1, 1, 16384, launch_tsm_ata_f64_32x32_mfma_r32_p1_nt, 2},
@@ -1761,23 +1793,23 @@ static const tsm::TunedEntry kSymTuned[] = {
   {2, 2, 2, 1, 1, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2_nt
   {2, 4, 4, 2, 1, 1, 2, 1},  // tsm_ata_bf16_4x4_t4x4_r4096_p2_nt
   {2, 8, 8, 5, 6, 1, 5, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 16, 16, 10, 1, 1, 10, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
-  {2, 32, 32, 16, 1, 1, 16, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
-  {2, 64, 64, 22, 1, 0, 23, 1},  // tsm_ata_bf16_64x64_direct_d4_nt (else tsm_ata_bf16_64x64_mfma_r256_p2_nt)
-  {1, 1, 1, 24, 1, 1, 24, 1},  // tsm_ata_f32_1x1_t1x1_r4096_p2_nt
-  {1, 2, 2, 25, 1, 1, 25, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
-  {1, 4, 4, 26, 1, 1, 26, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
-  {1, 8, 8, 29, 2, 0, 29, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
-  {1, 16, 16, 37, 2, 0, 37, 0},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
-  {1, 32, 32, 42, 4, 0, 42, 0},  // tsm_ata_f32_32x32_mfma_r128_p2_nt
-  {1, 64, 64, 47, 2, 0, 48, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2)
-  {0, 1, 1, 49, 2, 1, 49, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
-  {0, 2, 2, 54, 1, 1, 54, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
-  {0, 4, 4, 62, 1, 1, 62, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 73, 1, 0, 73, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
-  {0, 16, 16, 79, 1, 1, 79, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 86, 2, 0, 86, 0},  // tsm_ata_f64_32x32_mfma_r64_p1_nt
-  {0, 64, 64, 92, 2, 0, 93, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
+  {2, 16, 16, 12, 1, 1, 12, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
+  {2, 32, 32, 20, 1, 1, 20, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
+  {2, 64, 64, 26, 1, 0, 27, 1},  // tsm_ata_bf16_64x64_direct_d4_nt (else tsm_ata_bf16_64x64_mfma_r256_p2_nt)
+  {1, 1, 1, 28, 1, 1, 28, 1},  // tsm_ata_f32_1x1_t1x1_r4096_p2_nt
+  {1, 2, 2, 29, 1, 1, 29, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
+  {1, 4, 4, 30, 1, 1, 30, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
+  {1, 8, 8, 33, 2, 0, 33, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
+  {1, 16, 16, 44, 2, 0, 44, 0},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
+  {1, 32, 32, 48, 3, 0, 52, 1},  // tsm_ata_f32_32x32_direct_d16_nt (else tsm_ata_f32_32x32_mfma_r128_p2_nt)
+  {1, 64, 64, 57, 2, 0, 58, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2)
+  {0, 1, 1, 59, 2, 1, 59, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
+  {0, 2, 2, 64, 1, 1, 64, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
+  {0, 4, 4, 72, 1, 1, 72, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 83, 1, 0, 83, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
+  {0, 16, 16, 92, 1, 1, 92, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
+  {0, 32, 32, 96, 1, 0, 100, 1},  // tsm_ata_f64_32x32_direct_d8_nt (else tsm_ata_f64_32x32_mfma_r32_p1_nt)
+  {0, 64, 64, 108, 2, 0, 109, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
 };
 
 static const tsm::TunedEntry kTuned[] = {
