@@ -165,7 +165,8 @@ tsm_status tsm_ata(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, cons
                    int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
                    size_t workspace_bytes, tsm_stream_t stream);
 /* tsm_ata with a kernel request, as tsm_atb_ex (the A^T A kernels of the
- * tuned square classes come in several rows-per-tile and pipeline depths). */
+ * tuned square classes come in several rows-per-tile and pipeline depths,
+ * and at widths 16, 32 and 64 also on the direct engine, engine = 3). */
 tsm_status tsm_ata_ex(tsm_dtype in_type, tsm_dtype out_type, int64_t n, int k, const void* A,
                       int64_t lda, void* C, int64_t ldc, int accumulate, void* workspace,
                       size_t workspace_bytes, tsm_stream_t stream,

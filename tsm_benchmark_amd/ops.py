@@ -105,7 +105,9 @@ def ata(A: torch.Tensor, *, out_dtype: torch.dtype | None = None,
         config: _lib.KernelConfig | None = None,
         workspace: torch.Tensor | None = None) -> torch.Tensor:
     """C = A^T A (or C += A^T A with ``accumulate`` and ``out``): the Gram matrix
-    of a tall-skinny A, reading A once.  Same bits as ``atb(A, A)``.
+    of a tall-skinny A, reading A once.  C is exactly symmetric and within the
+    error bound of ``atb(A, A)`` (its kernels tile the rows differently, so
+    the bits may differ).
     ``workspace`` as for ``atb``."""
     _check_tall(A, "A")
     n, k = A.shape
