@@ -272,3 +272,43 @@ def test_every_short_row_count_backward(cls):
     excess = torch.stack(excess).cpu().numpy()
     bad = [n for n, e in zip(ns[1:], excess) if e > 0]
     assert not bad, f"{len(bad)} failing row counts, first: {bad[:10]}"
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("op", ["atb", "ata"])
+def test_every_width_accumulates_into_a_strided_fp32_output(dt, op):
+    """C += A^T B (or A^T A) in FP32 through a k x m view of a 64 x 64 buffer
+    (ldc = 64) for every width: each view ends as fl32(C0 + sum), and no
+    element of the buffer outside the view changes."""
+    n = 4099
+    A, B = rand(n, dt, 1), rand(n, dt, 2)
+    Ah = host(A)
+    Bh = Ah if op == "ata" else host(B)
+    ref, absC = oracle.atb_dot2(Ah, Bh, with_abs=True)
+    bound = 4.0 * np.sqrt(n) * EPS[dt] * absC
+    A_k = [None] + [A[:, :k].contiguous() for k in range(1, W + 1)]
+    B_m = [None] + [B[:, :m].contiguous() for m in range(1, W + 1)]
+    pairs = [(k, k) for k in range(1, W + 1)] if op == "ata" else PAIRS
+    g = torch.Generator(device="cuda").manual_seed(7)
+    bufs = []
+    for k, m in pairs:
+        buf = torch.rand((W, W), dtype=torch.float32, device="cuda", generator=g) * 4 - 2
+        before = buf.clone()
+        C = buf[:k, :m]
+        if op == "ata":
+            ops.ata(A_k[k], out=C, accumulate=True)
+        else:
+            ops.atb(A_k[k], B_m[m], out=C, accumulate=True)
+        bufs.append((buf, before))
+    torch.cuda.synchronize()
+    bad = []
+    for (k, m), (buf, before) in zip(pairs, bufs):
+        got, b0 = buf.cpu().double().numpy(), before.cpu().double().numpy()
+        want = b0[:k, :m] + ref[:k, :m]
+        tol = bound[:k, :m] + 2.0**-23 * np.abs(want) + 2.0**-24 * np.abs(b0[:k, :m])
+        inside = np.all(np.abs(got[:k, :m] - want) <= tol)
+        rest = got.copy()
+        rest[:k, :m] = b0[:k, :m]
+        if not (inside and np.array_equal(rest, b0)):
+            bad.append((k, m, bool(inside)))
+    assert not bad, f"{len(bad)} failing (k, m, inside ok), first: {bad[:8]}"
