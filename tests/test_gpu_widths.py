@@ -27,12 +27,14 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
-from tsm_benchmark_amd import ops  # noqa: E402
+from tsm_benchmark_amd import _lib, ops  # noqa: E402
 
 SEED = oracle.SEED
 W = 64
 DT = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}
 EPS = {"f64": 2.0**-52, "f32": 2.0**-23, "bf16": 2.0**-23}
+IN = {"f64": _lib.F64, "f32": _lib.F32, "bf16": _lib.BF16}
+GUARD = 7.0  # fill of the guard rows around written outputs (exact in every type)
 # ragged row counts: a partial tile only, a few tiles, a few hundred tiles
 NS = (77, 4099, 70001)
 PAIRS = [(k, m) for k in range(1, W + 1) for m in range(1, W + 1)]
@@ -152,6 +154,7 @@ def test_every_width_ata(dt):
 @pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
 def test_every_width_pair_backward(dt):
     n = 4099
+    L = _lib.lib()
     gdt = torch.float64 if dt == "f64" else torch.float32
     A, B = rand(n, dt, 1), rand(n, dt, 2)
     Ah, Bh = host(A), host(B)
@@ -185,7 +188,23 @@ def test_every_width_pair_backward(dt):
             eb = ((dB.double() - rb).abs() - rtol * rb.abs() - atol).max()
             e = torch.maximum(ea, eb)
             worst = e if worst is None else torch.maximum(worst, e)
-        excess.append(worst)
+        # the dense kernels again, through the C ABI, writing dA and dB into
+        # the middle of buffers with 16 guard rows on either side
+        gA = torch.full((n + 32, k), GUARD, dtype=DT[dt], device="cuda")
+        gB = torch.full((n + 32, m), GUARD, dtype=DT[dt], device="cuda")
+        G_km = Gd[:k, :m].contiguous()
+        st = L.tsm_atb_backward(IN[dt], IN[dt if dt == "f64" else "f32"], n, k, m,
+                                A_k[k].data_ptr(), k, B_m[m].data_ptr(), m, G_km.data_ptr(), m,
+                                gA[16:].data_ptr(), k, gB[16:].data_ptr(), m,
+                                torch.cuda.current_stream().cuda_stream)
+        assert st == _lib.SUCCESS, (k, m, st)
+        dA, dB = gA[16:16 + n], gB[16:16 + n]
+        ea = ((dA.double() - ra).abs() - rtol * ra.abs() - atol).max()
+        eb = ((dB.double() - rb).abs() - rtol * rb.abs() - atol).max()
+        spill = ((gA[:16] != GUARD).any() | (gA[16 + n:] != GUARD).any() |
+                 (gB[:16] != GUARD).any() | (gB[16 + n:] != GUARD).any())
+        e = torch.where(spill, torch.full_like(ea, float("inf")), torch.maximum(ea, eb))
+        excess.append(torch.maximum(worst, e))
     excess = torch.stack(excess).cpu().numpy()
     bad = [PAIRS[i] for i in np.flatnonzero(excess > 0)]
     assert not bad, f"{len(bad)} failing (k, m), first: {bad[:8]}, worst excess {excess.max():.3g}"
