@@ -331,3 +331,33 @@ def test_every_width_accumulates_into_a_strided_fp32_output(dt, op):
         if not (inside and np.array_equal(rest, b0)):
             bad.append((k, m, bool(inside)))
     assert not bad, f"{len(bad)} failing (k, m, inside ok), first: {bad[:8]}"
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+def test_every_width_pair_in_one_batch_accumulates_into_strided_fp32_outputs(dt):
+    """The 4096 pairs of one batched launch, each adding into an FP32 k x m
+    view of its own 64 x 64 buffer (ldc = 64): each view ends as
+    fl32(C0 + sum), and no element outside the views changes."""
+    n = 4099
+    A, B = rand(n, dt, 1), rand(n, dt, 2)
+    ref, absC = oracle.atb_dot2(host(A), host(B), with_abs=True)
+    bound = 4.0 * np.sqrt(n) * EPS[dt] * absC
+    A_k = [None] + [A[:, :k].contiguous() for k in range(1, W + 1)]
+    B_m = [None] + [B[:, :m].contiguous() for m in range(1, W + 1)]
+    g = torch.Generator(device="cuda").manual_seed(11)
+    bufs = torch.rand((len(PAIRS), W, W), dtype=torch.float32, device="cuda", generator=g) * 4 - 2
+    before = bufs.clone()
+    outs = [bufs[p, :k, :m] for p, (k, m) in enumerate(PAIRS)]
+    ops.atb_batched([A_k[k] for k, _ in PAIRS], [B_m[m] for _, m in PAIRS], outs=outs,
+                    accumulate=True)
+    got, b0 = bufs.cpu().double().numpy(), before.cpu().double().numpy()
+    bad = []
+    for p, (k, m) in enumerate(PAIRS):
+        want = b0[p, :k, :m] + ref[:k, :m]
+        tol = bound[:k, :m] + 2.0**-23 * np.abs(want) + 2.0**-24 * np.abs(b0[p, :k, :m])
+        inside = np.all(np.abs(got[p, :k, :m] - want) <= tol)
+        rest = got[p].copy()
+        rest[:k, :m] = b0[p, :k, :m]
+        if not (inside and np.array_equal(rest, b0[p])):
+            bad.append((k, m, bool(inside)))
+    assert not bad, f"{len(bad)} failing (k, m, inside ok), first: {bad[:8]}"
