@@ -224,6 +224,54 @@ def test_determinism_bitwise():
         assert torch.equal(C, first)
 
 
+@pytest.mark.parametrize("dt,k,m,n", [
+    ("f64", 8, 8, 1 << 22),    # register tiles, finisher kernel (the headline shape)
+    ("f64", 1, 1, 1 << 20),    # register tiles, finisher kernel
+    ("f64", 8, 8, 20000),      # register tiles, final sum inside the streaming kernel
+    ("f64", 64, 64, 1 << 18),  # direct matrix-core engine
+    ("f32", 32, 32, 1 << 20),  # LDS-staged matrix-core engine, (hi, lo) folds
+    ("bf16", 64, 64, 1 << 20), # BF16 direct engine (wave-private LDS tiles)
+])
+def test_determinism_bitwise_every_engine(dt, k, m, n):
+    """100 calls give the same bits, on alternating streams while a copy
+    kernel on a third stream competes for the compute units (so workgroups
+    start, finish and arrive in different orders from call to call)."""
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    first = ops.atb(A, B, out_dtype=torch.float64)
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    noise = torch.cuda.Stream()
+    src = torch.empty(1 << 24, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    outs = []
+    for i in range(100):
+        if i % 10 == 0:
+            with torch.cuda.stream(noise):
+                dst.copy_(src)
+        with torch.cuda.stream(streams[i % 2]):
+            outs.append(ops.atb(A, B, out_dtype=torch.float64))
+    torch.cuda.synchronize()
+    assert all(torch.equal(C, first) for C in outs)
+
+
+def test_determinism_bitwise_backward_and_batched():
+    """The gradients and a batched launch give the same bits over 50 calls."""
+    n = 300_001
+    G = torch.from_numpy(oracle.gen_matrix(16, 16, dist=oracle.DIST_NORMAL, seed=SEED, stream=3))
+    for dt in ("f64", "f32", "bf16"):
+        A, B = rand(n, 16, DT[dt], 1), rand(n, 16, DT[dt], 2)
+        g = G.to("cuda", torch.float64 if dt == "f64" else torch.float32)
+        dA0, dB0 = ops.atb_backward(A, B, g)
+        for _ in range(50):
+            dA, dB = ops.atb_backward(A, B, g)
+            assert torch.equal(dA, dA0) and torch.equal(dB, dB0)
+    As = [rand(int(n * f), w, torch.float64, 1) for f, w in ((1, 8), (0.3, 32), (2.1, 4), (0.01, 64))]
+    Bs = [rand(X.shape[0], w, torch.float64, 2) for X, w in zip(As, (8, 16, 32, 2))]
+    first = ops.atb_batched(As, Bs)
+    for _ in range(50):
+        assert all(torch.equal(C, C0) for C, C0 in zip(ops.atb_batched(As, Bs), first))
+
+
 def test_size_independent_properties():
     n = 3_000_017
     A, B = rand(n, 8, torch.float64, 1), rand(n, 8, torch.float64, 2)
