@@ -4,6 +4,8 @@
   python -m tsm_benchmark_amd.profiling copy --bytes 1073741824 --reps 5
   python -m tsm_benchmark_amd.profiling bwd  --n 8388608 --k 16 --m 16 --dtype f32 --reps 5 [--dA-only]
   python -m tsm_benchmark_amd.profiling batched --dtype f64 --reps 5
+  python -m tsm_benchmark_amd.profiling ata  --n 33554432 --k 8 --dtype f64 --reps 5
+  python -m tsm_benchmark_amd.profiling tsmm --n 33554432 --k 8 --m 8 --dtype f64 --reps 5
 
 `atb --sets S` uses S input sets in rotation (as bench.py does) and
 `--back-to-back` (atb, bwd, batched) launches all reps without a
@@ -34,7 +36,7 @@ SEED = 20261203
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["atb", "copy", "bwd", "batched"])
+    ap.add_argument("what", choices=["atb", "copy", "bwd", "batched", "ata", "tsmm"])
     ap.add_argument("--sets", type=int, default=1)
     ap.add_argument("--back-to-back", action="store_true")
     ap.add_argument("--n", type=int, default=1 << 25)
@@ -96,6 +98,26 @@ def main(argv=None) -> int:
                 print(f"atb {i}: {A.element_size() * args.n * (args.k + args.m)} bytes", flush=True)
         torch.cuda.synchronize()
         print(f"atb: {args.reps} launches done", flush=True)
+    elif args.what in ("ata", "tsmm"):
+        # A^T A reads A only; Y = A X reads A and writes Y (X is k x m)
+        if args.what == "ata":
+            C = torch.empty((args.k, args.k), dtype=torch.float64, device="cuda")
+            fn = lambda: ops.ata(A, out=C)  # noqa: E731
+            moved = A.element_size() * args.n * args.k
+        else:
+            X = torch.randn((args.k, args.m), dtype=torch.float64, device="cuda",
+                            generator=torch.Generator(device="cuda").manual_seed(5))
+            if dt != torch.float64:
+                X = X.float()
+            fn = lambda: ops.tsmm(A, X)  # noqa: E731
+            moved = A.element_size() * args.n * (args.k + args.m)
+        for i in range(args.reps):
+            fn()
+            if not args.back_to_back:
+                torch.cuda.synchronize()
+                print(f"{args.what} {i}: {moved} bytes", flush=True)
+        torch.cuda.synchronize()
+        print(f"{args.what}: {args.reps} launches done", flush=True)
     else:
         G = torch.randn((args.k, args.m), dtype=torch.float32, device="cuda")
         G = G.to(torch.float64 if dt == torch.float64 else torch.float32)
