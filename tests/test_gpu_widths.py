@@ -361,3 +361,57 @@ def test_every_width_pair_in_one_batch_accumulates_into_strided_fp32_outputs(dt)
         if not (inside and np.array_equal(rest, b0[p])):
             bad.append((k, m, bool(inside)))
     assert not bad, f"{len(bad)} failing (k, m, inside ok), first: {bad[:8]}"
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "bf16"])
+def test_every_width_on_misaligned_views(dt):
+    """Views that start one element into a 69-column buffer: every base
+    misaligned by one element, ld = 69.  A^T B for all 4096 pairs, A^T A for
+    every width and both gradients for every pair, against the oracle."""
+    n, ld, off = 4099, W + 5, 1
+    A = ops.random_matrix(n, ld, DT[dt], seed=SEED, stream_id=5)
+    B = ops.random_matrix(n, ld, DT[dt], seed=SEED, stream_id=6)
+    Ah, Bh = host(A)[:, off:off + W], host(B)[:, off:off + W]
+    ref, absC = oracle.atb_dot2(Ah, Bh, with_abs=True)
+    bound = 4.0 * np.sqrt(n) * EPS[dt] * absC
+    sref, sabs = oracle.atb_dot2(Ah, Ah, with_abs=True)
+    sbound = 4.0 * np.sqrt(n) * EPS[dt] * sabs
+    Av = [None] + [A[:, off:off + k] for k in range(1, W + 1)]
+    Bv = [None] + [B[:, off:off + m] for m in range(1, W + 1)]
+    fwd = [ops.atb(Av[k], Bv[m], out_dtype=torch.float64) for k, m in PAIRS]
+    sym = [ops.ata(Av[k], out_dtype=torch.float64) for k in range(1, W + 1)]
+    # gradients: one oracle call per width at the full other width, sliced
+    gdt = torch.float64 if dt == "f64" else torch.float32
+    G = torch.from_numpy(oracle.gen_matrix(W, W, dist=oracle.DIST_NORMAL, seed=SEED, stream=3)).to(gdt)
+    Gh, Gd = G.double().numpy(), G.to("cuda")
+    dA_ref = [None] + [torch.from_numpy(oracle.atb_backward(Ah, Bh[:, :m], Gh[:, :m])[0]).to("cuda")
+                       for m in range(1, W + 1)]
+    dB_ref = [None] + [torch.from_numpy(oracle.atb_backward(Ah[:, :k], Bh, Gh[:k, :])[1]).to("cuda")
+                       for k in range(1, W + 1)]
+    absG = np.abs(Gh)
+    excess = []
+    for k, m in PAIRS:
+        if dt == "f64":
+            rtol = atol = 1e-13
+        elif dt == "f32":
+            rtol = atol = 1e-5
+        else:
+            rtol, atol = 2.0**-7, 2.0**-7 * float(absG[:k, :m].sum())
+        dA, dB = ops.atb_backward(Av[k], Bv[m], Gd[:k, :m])
+        ra, rb = dA_ref[m][:, :k], dB_ref[k][:, :m]
+        excess.append(torch.maximum(((dA.double() - ra).abs() - rtol * ra.abs() - atol).max(),
+                                    ((dB.double() - rb).abs() - rtol * rb.abs() - atol).max()))
+    excess = torch.stack(excess).cpu().numpy()
+    bad = [("bwd",) + PAIRS[i] for i in np.flatnonzero(excess > 0)]
+    for (k, m), C in zip(PAIRS, fwd):
+        C = C.cpu().numpy()
+        ok = np.all(np.abs(C - ref[:k, :m]) <= bound[:k, :m])
+        if ok and dt != "f64" and np.linalg.norm(ref[:k, :m]) > 0:
+            ok = np.linalg.norm(C - ref[:k, :m]) / np.linalg.norm(ref[:k, :m]) <= 1e-6
+        if not ok:
+            bad.append(("atb", k, m))
+    for k, C in zip(range(1, W + 1), sym):
+        C = C.cpu().numpy()
+        if not (np.all(np.abs(C - sref[:k, :k]) <= sbound[:k, :k]) and np.array_equal(C, C.T)):
+            bad.append(("ata", k, k))
+    assert not bad, f"{len(bad)} failing (op, k, m), first: {bad[:8]}"
