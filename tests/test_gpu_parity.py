@@ -206,6 +206,41 @@ def test_every_generated_variant(dt, k, m):
     assert seen >= 1
 
 
+PADDED_CLASSES = [(dt, k, m) for dt in ("f64", "f32", "bf16")
+                  for k, m in ((8, 16), (8, 32), (8, 64), (16, 8), (32, 8), (64, 8))]
+
+
+@pytest.mark.parametrize("dt,k,m", PADDED_CLASSES, ids=lambda x: str(x))
+def test_padded_matrix_core_kernels(dt, k, m):
+    """The matrix-core kernels of the classes with one 8-wide side (that
+    side's rows staged into LDS rows of 16, the upper half zero), forced one
+    by one: dense inputs on several grids and tile orders, strided views
+    with fewer columns than the class (the element path), and sign * 2^e
+    inputs against the exact int128 oracle."""
+    dtype_id = {"f64": 0, "f32": 1, "bf16": 2}[dt]
+    kds = [kd for kd in _lib.kernels() if kd["in_type"] == dtype_id and kd["kc"] == k
+           and kd["mc"] == m and kd["engine"] == _lib.ENGINE_MFMA_LDS]
+    assert kds, "no padded matrix-core kernel compiled for this class"
+    n = 70001
+    A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
+    Ah, Bh = host(A), host(B)
+    Aw = rand(5003, k + 3, DT[dt], 3)[:, 1:k]  # k - 1 columns, misaligned base
+    Bw = rand(5003, m + 5, DT[dt], 4)[:, 2:m]
+    P, Q = rand(20011, k, DT[dt], 5, dist=1), rand(20011, m, DT[dt], 6, dist=1)
+    Ph, Qh = host(P), host(Q)
+    exact = oracle.atb_exact(Ph, Qh, 20, 20)
+    for kd in kds:
+        for grid, il in ((0, 0), (0, 1), (7, 1), (300, 0)):
+            cfg = _lib.config_of(kd, grid=grid, interleave=il)
+            check_close(ops.atb(A, B, out_dtype=torch.float64, config=cfg).cpu().numpy(),
+                        Ah, Bh, DT[dt])
+        cfg = _lib.config_of(kd)
+        check_close(ops.atb(Aw, Bw, out_dtype=torch.float64, config=cfg).cpu().numpy(),
+                    host(Aw), host(Bw), DT[dt])
+        check_close(ops.atb(P, Q, out_dtype=torch.float64, config=cfg).cpu().numpy(),
+                    Ph, Qh, DT[dt], exact=exact)
+
+
 def test_unknown_variant_is_reported():
     A, B = rand(100, 8, torch.float64, 1), rand(100, 8, torch.float64, 2)
     with pytest.raises(_lib.TsmError) as e:

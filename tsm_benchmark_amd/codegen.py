@@ -122,8 +122,20 @@ class KernelConfig:
         return THREADS // self.ntiles
 
     @property
+    def kp(self) -> int:
+        """Width of A the consumer works on: the staged matrix-core consumer
+        takes 16-wide blocks, so an 8-wide side is staged into LDS rows of 16
+        (the upper half zero)."""
+        return 16 if self.mfma and not self.direct and self.kc < 16 else self.kc
+
+    @property
+    def mp(self) -> int:
+        """Width of B the consumer works on (as kp)."""
+        return 16 if self.mfma and not self.direct and self.mc < 16 else self.mc
+
+    @property
     def stage_bytes(self) -> int:
-        return self.rows * (self.kc + (0 if self.sym else self.mc)) * ELEM_BYTES[self.dtype]
+        return self.rows * (self.kp + (0 if self.sym else self.mp)) * ELEM_BYTES[self.dtype]
 
     @property
     def direct_depths(self) -> tuple:
@@ -136,7 +148,7 @@ class KernelConfig:
             return max(staging, self.kc * self.mc * 8, THREADS * 16)
         if self.direct:  # two KC x MC FP64 halves of the cross-wave sum
             return max(2 * self.kc * self.mc * 8, THREADS * 16)
-        return max(2 * self.stage_bytes, self.kc * self.mc * 8, THREADS * 8)
+        return max(2 * self.stage_bytes, self.kp * self.mp * 8, THREADS * 8)
 
     def problems(self) -> list[str]:
         """Reasons this configuration cannot be emitted (empty when valid)."""
@@ -166,8 +178,10 @@ class KernelConfig:
                 p.append("LDS budget")
             return p
         if self.mfma:
-            if self.kc % 16 or self.mc % 16:
-                p.append("MFMA needs width classes that are multiples of 16")
+            # an 8-wide side is padded to 16 in LDS, only beside a side of 16
+            # or more (so never for A^T A, whose sides are equal)
+            if any(w % 16 and w != 8 for w in (self.kc, self.mc)) or max(self.kc, self.mc) < 16:
+                p.append("MFMA needs width classes that are multiples of 16 (one side may be 8)")
             if self.rows % (4 * self.kstep):
                 p.append(f"MFMA rows must be a multiple of {4 * self.kstep}")
             if self.rows // (4 * self.kstep) > MAX_ROWS_PER_THREAD:
@@ -290,6 +304,24 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
                     out.add(c)
     out.update(direct_variants(dtype, kc, mc))
     return sorted(out)
+
+
+def padded_mfma_variants(dtype: str) -> list[KernelConfig]:
+    """Matrix-core kernels of the classes with one side of 8 and the other of
+    16 or more, the 8-wide side padded to 16 in LDS.  Their register-tile
+    kernels need 8 to 64 FMAs per element read (plus the BF16 unpacking),
+    more than the vector units sustain at the HBM rate for FP32 and BF16.
+    Three tile sizes around 32 KiB of staged rows, one or two tiles in
+    flight, streaming loads."""
+    out = []
+    for kc, mc in [(8, w) for w in (16, 32, 64)] + [(w, 8) for w in (16, 32, 64)]:
+        mrows = _pow2_floor(32768 // ((max(kc, 16) + max(mc, 16)) * ELEM_BYTES[dtype]))
+        for rows in (mrows // 2, mrows, 2 * mrows):
+            for pf in (1, 2):
+                c = KernelConfig(dtype, kc, mc, 0, 0, rows, pf, 1)
+                if not c.problems():
+                    out.append(c)
+    return out
 
 
 def direct_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
@@ -456,6 +488,7 @@ def library_kernels(tuning: dict) -> tuple[list[KernelConfig], dict]:
                     defaults[(dt, kc, mc)] = (direct[0], 1, 0)
         for kc, mc in TUNE_SHAPES[dt]:
             cfgs.update(tuning_variants(dt, kc, mc))
+        cfgs.update(padded_mfma_variants(dt))
     cfgs.update(reference_variants([4, 8, 16], [2, 4]))
     for e in tuning.get("entries", []):
         c = config_of_entry(e)
@@ -550,7 +583,10 @@ def batched_classes(defaults: dict, dtype: str, tuning: dict | None = None) -> l
     for kc in widths:
         for mc in widths:
             c = fbs[(dtype, kc, mc)][0]
-            if c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32):
+            # (the padded matrix-core kernels of the 8-wide sides stay out:
+            # their zeroed LDS padding would be refilled for every item)
+            if (c.lds_bytes > BATCH_LDS or (dtype != "f64" and c.tm * c.tn > 32)
+                    or (c.mfma and min(kc, mc) < 16)):
                 c = default_config(dtype, kc, mc)
             while dtype != "bf16":  # BF16's wider class bodies would spill
                 wider = dataclasses.replace(c, rows=2 * c.rows)

@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace tsm {
 
 constexpr int kThreads = 256;  // 4 waves of 64
@@ -676,6 +678,15 @@ constexpr int atb_smem_bytes() {
   return m1 > kThreads * 16 ? m1 : kThreads * 16;
 }
 
+// Width the consumer works on for a side of class width W: the matrix-core
+// consumer (TM = TN = 0) takes 16-wide blocks, so an 8-wide side is staged
+// into LDS rows of 16 whose upper half stays zero.  The register tiles work
+// on the class width itself.
+template <int W, int TM, int TN>
+constexpr int consumer_width() {
+  return (TM == 0 && TN == 0 && W < 16) ? 16 : W;
+}
+
 // Register-tile consumer (VALU): the k x m result (padded to KC x MC) is cut
 // into NTILES = (KC/TM) * (MC/TN) register tiles; thread t owns tile
 // p = t % NTILES (A columns [mb*TM, mb*TM+TM), B columns [nb*TN, nb*TN+TN))
@@ -875,18 +886,45 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
                                             int64_t full, int tail_rows, double* partial_out) {
   static_assert(!SYM || KC == MC, "A^T A is square");
   constexpr int ES = (int)sizeof(S);
-  constexpr int A_BYTES = T * KC * ES, B_BYTES = SYM ? 0 : T * MC * ES;
+  // Consumer widths: a matrix-core kernel of an 8-wide side stages that
+  // side's rows into LDS rows of KP (MP) = 16 elements, the upper half zero.
+  constexpr int KP = consumer_width<KC, TM, TN>(), MP = consumer_width<MC, TM, TN>();
+  constexpr bool PAD_A = KP != KC, PAD_B = MP != MC;
+  static_assert(!(PAD_A || PAD_B) || (!SYM && KC * ES % 16 == 0 && MC * ES % 16 == 0),
+                "padded rows are whole 16-byte chunks");
+  constexpr int A_BYTES = T * KC * ES, B_BYTES = SYM ? 0 : T * MC * ES;  // a tile in memory
+  constexpr int LA_BYTES = T * KP * ES, LB_BYTES = SYM ? 0 : T * MP * ES;  // ... in LDS
   static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
   constexpr int A_VEC = A_BYTES / 16, B_VEC = B_BYTES / 16;
   constexpr int A_PER = (A_VEC + kThreads - 1) / kThreads;
   constexpr int B_PER = (B_VEC + kThreads - 1) / kThreads;
-  constexpr int STAGE = A_BYTES + B_BYTES;
-  constexpr int B_OFF = SYM ? 0 : A_BYTES;  // B's tile within a stage
+  constexpr int STAGE = LA_BYTES + LB_BYTES;
+  constexpr int B_OFF = SYM ? 0 : LA_BYTES;  // B's tile within a stage
   constexpr int KCMC = KC * MC;
   static_assert(PF >= 1 && PF <= 4, "one to four tiles in flight");
   const int tid = threadIdx.x;
-  typename ConsumerOf<S, KC, MC, TM, TN, T, SYM>::type cons;
+  typename ConsumerOf<S, KP, MP, TM, TN, T, SYM>::type cons;
   cons.zero(tid);
+  if constexpr (PAD_A || PAD_B) {
+    // The padding columns are never written below: zero both stages once
+    // (the previous call's reduction may have left values in them).
+    v4u* z = reinterpret_cast<v4u*>(smem);
+    for (int x = tid; x < 2 * STAGE / 16; x += kThreads) z[x] = v4u{0u, 0u, 0u, 0u};
+    __syncthreads();
+  }
+  // 16-byte chunk c of a tile row of W elements -> its chunk index in an LDS
+  // row of WP elements
+  auto lds_chunk = [](int c, auto w, auto wp) __attribute__((always_inline)) {
+    constexpr int CW = decltype(w)::value * ES / 16, CP = decltype(wp)::value * ES / 16;
+    if constexpr (CW == CP)
+      return c;
+    else
+      return (c / CW) * CP + c % CW;
+  };
+  using KW = std::integral_constant<int, KC>;
+  using KWP = std::integral_constant<int, KP>;
+  using MW = std::integral_constant<int, MC>;
+  using MWP = std::integral_constant<int, MP>;
 
   const int64_t nfull = a.dense ? count : 0;
 
@@ -915,16 +953,16 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
     };
     auto store = [&](const Regs& r, unsigned char* stage) __attribute__((always_inline)) {
       v4u* la = reinterpret_cast<v4u*>(stage);
-      v4u* lb = reinterpret_cast<v4u*>(stage + A_BYTES);
+      v4u* lb = reinterpret_cast<v4u*>(stage + LA_BYTES);
 #pragma unroll
       for (int v = 0; v < A_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (A_VEC % kThreads == 0 || idx < A_VEC) la[idx] = r.a[v];
+        if (A_VEC % kThreads == 0 || idx < A_VEC) la[lds_chunk(idx, KW{}, KWP{})] = r.a[v];
       }
 #pragma unroll
       for (int v = 0; v < B_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (B_VEC % kThreads == 0 || idx < B_VEC) lb[idx] = r.b[v];
+        if (B_VEC % kThreads == 0 || idx < B_VEC) lb[lds_chunk(idx, MW{}, MWP{})] = r.b[v];
       }
     };
     // One pipeline step on tile tt held in r: publish it to LDS, refill r with
@@ -968,19 +1006,19 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
     const S* gB = static_cast<const S*>(a.B);
     __syncthreads();
     S* la = reinterpret_cast<S*>(smem);
-    S* lb = reinterpret_cast<S*>(smem + A_BYTES);
+    S* lb = reinterpret_cast<S*>(smem + LA_BYTES);
     for (int e = tid; e < T * KC; e += kThreads) {
       const int r = e / KC, c = e % KC;
       S v{};
       if (r < nr && c < a.k) v = gA[(r0 + r) * a.lda + c];
-      la[e] = v;
+      la[r * KP + c] = v;
     }
     if constexpr (!SYM) {
       for (int e = tid; e < T * MC; e += kThreads) {
         const int r = e / MC, c = e % MC;
         S v{};
         if (r < nr && c < a.m) v = gB[(r0 + r) * a.ldb + c];
-        lb[e] = v;
+        lb[r * MP + c] = v;
       }
     }
     __syncthreads();
@@ -994,7 +1032,9 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   // ---- reduction -------------------------------------------------------------------
   double* red = reinterpret_cast<double*>(smem);
   cons.reduce(red, tid);
-  for (int o = tid; o < KCMC; o += kThreads) publish(partial_out + o, cons.total(red, o));
+  // the consumer's KP x MP result holds the class's KC x MC block at its top left
+  for (int o = tid; o < KCMC; o += kThreads)
+    publish(partial_out + o, cons.total(red, (PAD_A || PAD_B) ? (o / MC) * MP + o % MC : o));
 }
 
 // Arrival at a problem's counter after its partial is stored: returns true
@@ -1021,7 +1061,8 @@ __device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int 
 // workgroup to arrive adds the G partials in workgroup order and writes C.
 template <class S, int KC, int MC, int TM, int TN, int T, int PF, int NT, int SYM = 0>
 __device__ __forceinline__ void atb_body(const AtbArgs& a) {
-  constexpr int SMEM = atb_smem_bytes<S, KC, MC, T, SYM>();
+  constexpr int SMEM = atb_smem_bytes<S, consumer_width<KC, TM, TN>(),
+                                      consumer_width<MC, TM, TN>(), T, SYM>();
   __shared__ __attribute__((aligned(16))) unsigned char smem[SMEM];
   __shared__ int s_last;
   const int64_t full = a.n / T;
