@@ -91,7 +91,9 @@ typedef struct {
                           global memory (BF16 through a wave-private LDS tile; pf = steps
                           in flight per wave, of 4 rows (FP64/FP32) or 32 rows (BF16);
                           inputs that are not dense run the class's staged kernel).
-                          2 and 3 need width classes that are multiples of 16. */
+                          3 needs width classes that are multiples of 16; 2 also takes
+                          one 8-wide side beside a side of 16 or more (padded to 16 in
+                          LDS). */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------
