@@ -109,6 +109,33 @@ def test_ata_64bit_addressing():
     del A
 
 
+@pytest.mark.parametrize("dt,rows", [("f64", 128), ("f64", 256), ("f32", 256), ("bf16", 512),
+                                     ("bf16", 1024), ("bf16", 2048)])
+@pytest.mark.parametrize("pf", [1, 2])
+def test_ata_padded_matrix_core_kernels(dt, rows, pf):
+    """A^T A at k = 8 on the staged matrix-core consumer with A's rows padded
+    to 16 in LDS (one diagonal block), forced: dense and ragged inputs on
+    several grids, a strided view of fewer columns (the element path) and
+    sign * 2^e inputs against the exact oracle; C exactly symmetric."""
+    k = 8
+    cfg = dict(rows=rows, pf=pf, nt=True, engine=_lib.ENGINE_MFMA_LDS)
+    for n in (70001, 1000):
+        A = rand(n, k, DT[dt], 1)
+        for grid, il in ((0, 0), (0, 1), (7, 1)):
+            C = ops.ata(A, out_dtype=torch.float64,
+                        config=_lib.make_config(grid=grid, interleave=il, **cfg))
+            check_ata(C, A, n, dt)
+    W = rand(5003, k + 3, DT[dt], 3)[:, 1:k]  # 7 columns, misaligned base
+    check_ata(ops.ata(W, out_dtype=torch.float64, config=_lib.make_config(**cfg)), W, 5003, dt)
+    P = rand(20011, k, DT[dt], 5, dist=1)
+    Ph = host(P)
+    exact = oracle.atb_exact(Ph, Ph, 20, 20)
+    absC = oracle.atb_exact(np.abs(Ph), np.abs(Ph), 20, 20)
+    C = ops.ata(P, out_dtype=torch.float64, config=_lib.make_config(**cfg)).cpu().numpy()
+    assert np.array_equal(C, C.T)
+    assert np.all(np.abs(C - exact) <= 4.0 * np.sqrt(20011) * EPS[DT[dt]] * absC)
+
+
 def test_ata_deterministic():
     A = rand(1 << 22, 8, torch.float64, 9)
     first = ops.ata(A)

@@ -178,9 +178,10 @@ class KernelConfig:
                 p.append("LDS budget")
             return p
         if self.mfma:
-            # an 8-wide side is padded to 16 in LDS, only beside a side of 16
-            # or more (so never for A^T A, whose sides are equal)
-            if any(w % 16 and w != 8 for w in (self.kc, self.mc)) or max(self.kc, self.mc) < 16:
+            # an 8-wide side is padded to 16 in LDS: beside a side of 16 or
+            # more, or for A^T A at 8 (one diagonal block from one padded tile)
+            if (any(w % 16 and w != 8 for w in (self.kc, self.mc))
+                    or (max(self.kc, self.mc) < 16 and not self.sym)):
                 p.append("MFMA needs width classes that are multiples of 16 (one side may be 8)")
             if self.rows % (4 * self.kstep):
                 p.append(f"MFMA rows must be a multiple of {4 * self.kstep}")
@@ -382,7 +383,9 @@ def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
     rows per tile and one or two tiles in flight, and the direct engine's
     kernels of the class (matrix-core widths).  A^T A reads half the bytes of
     A^T B per row, so the direct engine, which keeps every row's operands in
-    registers, can win where A^T B prefers the staged kernel."""
+    registers, can win where A^T B prefers the staged kernel.  At 8 the
+    staged matrix-core consumer on A's rows padded to 16 is offered too: the
+    register tiles there do 8 FMAs per element read (plus BF16 unpacking)."""
     c = defaults[(dtype, kc, kc)][0]
     direct = {dataclasses.replace(v, sym=1) for v in direct_variants(dtype, kc, kc)}
     if c.direct:
@@ -393,6 +396,13 @@ def sym_variants(defaults: dict, dtype: str, kc: int) -> list[KernelConfig]:
             v = dataclasses.replace(c, sym=1, rows=rows, pf=pf)
             if rows >= 1 and not v.problems():
                 out.add(v)
+    if kc == 8:
+        mrows = _pow2_floor(32768 // (16 * ELEM_BYTES[dtype]))
+        for rows in (mrows // 2, mrows, 2 * mrows):
+            for pf in (1, 2):
+                v = KernelConfig(dtype, 8, 8, 0, 0, rows, pf, 1, sym=1)
+                if not v.problems():
+                    out.add(v)
     return sorted(out)
 
 

@@ -423,6 +423,66 @@ hipError_t launch_tsm_atb_bf16_8x8_t8x8_r2048_p2_nt(const tsm::AtbArgs& a, unsig
   return hipGetLastError();
 }
 
+// tsm_ata_bf16_8x8_mfma_r512_p1_nt: KC=8 MC=8 TM=0 TN=0 rows=512 pf=1 nt=1
+//   matrix cores: 16 K-steps of 32 rows per tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r512_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 512, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r512_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r512_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_mfma_r512_p2_nt: KC=8 MC=8 TM=0 TN=0 rows=512 pf=2 nt=1
+//   matrix cores: 16 K-steps of 32 rows per tile, LDS 32768 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r512_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 512, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r512_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r512_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_mfma_r1024_p1_nt: KC=8 MC=8 TM=0 TN=0 rows=1024 pf=1 nt=1
+//   matrix cores: 32 K-steps of 32 rows per tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r1024_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 1024, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r1024_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r1024_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_mfma_r1024_p2_nt: KC=8 MC=8 TM=0 TN=0 rows=1024 pf=2 nt=1
+//   matrix cores: 32 K-steps of 32 rows per tile, LDS 65536 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r1024_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 1024, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r1024_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r1024_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_mfma_r2048_p1_nt: KC=8 MC=8 TM=0 TN=0 rows=2048 pf=1 nt=1
+//   matrix cores: 64 K-steps of 32 rows per tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r2048_p1_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 2048, 1, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r2048_p1_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r2048_p1_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+// tsm_ata_bf16_8x8_mfma_r2048_p2_nt: KC=8 MC=8 TM=0 TN=0 rows=2048 pf=2 nt=1
+//   matrix cores: 64 K-steps of 32 rows per tile, LDS 131072 B
+extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_mfma_r2048_p2_nt(tsm::AtbArgs a) {
+  tsm::atb_body<tsm::bf16, 8, 8, 0, 0, 2048, 2, 1, 1>(a);
+}
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r2048_p2_nt(const tsm::AtbArgs& a, unsigned grid, hipStream_t s) {
+  hipLaunchKernelGGL(tsm_ata_bf16_8x8_mfma_r2048_p2_nt, dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 // tsm_ata_bf16_8x8_t8x4_r512_p1_nt: KC=8 MC=8 TM=8 TN=4 rows=512 pf=1 nt=1
 //   2 register tiles x 128 row lanes, 4 rows/thread/tile, LDS 16384 B
 extern "C" __global__ void __launch_bounds__(256) tsm_ata_bf16_8x8_t8x4_r512_p1_nt(tsm::AtbArgs a) {

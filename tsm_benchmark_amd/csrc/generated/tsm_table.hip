@@ -885,6 +885,12 @@ hipError_t launch_tsm_atb_f64_64x64_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_ata_bf16_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_4x4_t4x4_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r2048_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_bf16_8x8_mfma_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -913,6 +919,8 @@ hipError_t launch_tsm_ata_bf16_64x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsign
 hipError_t launch_tsm_ata_f32_1x1_t1x1_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_2x2_t2x2_r4096_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_4x4_t4x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f32_8x8_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -957,6 +965,10 @@ hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p1_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_ata_f64_8x8_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t4x4_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_ata_f64_8x8_t4x4_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1879,6 +1891,12 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_bf16_1x1_t1x1_r4096_p2_nt", 2, 1, 1, 1, 1, 4096, 2, 1, 16384, launch_tsm_ata_bf16_1x1_t1x1_r4096_p2_nt, 1},
   {"tsm_ata_bf16_2x2_t2x2_r4096_p2_nt", 2, 2, 2, 2, 2, 4096, 2, 1, 32768, launch_tsm_ata_bf16_2x2_t2x2_r4096_p2_nt, 1},
   {"tsm_ata_bf16_4x4_t4x4_r4096_p2_nt", 2, 4, 4, 4, 4, 4096, 2, 1, 65536, launch_tsm_ata_bf16_4x4_t4x4_r4096_p2_nt, 1},
+  {"tsm_ata_bf16_8x8_mfma_r512_p1_nt", 2, 8, 8, 0, 0, 512, 1, 1, 32768, launch_tsm_ata_bf16_8x8_mfma_r512_p1_nt, 2},
+  {"tsm_ata_bf16_8x8_mfma_r512_p2_nt", 2, 8, 8, 0, 0, 512, 2, 1, 32768, launch_tsm_ata_bf16_8x8_mfma_r512_p2_nt, 2},
+  {"tsm_ata_bf16_8x8_mfma_r1024_p1_nt", 2, 8, 8, 0, 0, 1024, 1, 1, 65536, launch_tsm_ata_bf16_8x8_mfma_r1024_p1_nt, 2},
+  {"tsm_ata_bf16_8x8_mfma_r1024_p2_nt", 2, 8, 8, 0, 0, 1024, 2, 1, 65536, launch_tsm_ata_bf16_8x8_mfma_r1024_p2_nt, 2},
+  {"tsm_ata_bf16_8x8_mfma_r2048_p1_nt", 2, 8, 8, 0, 0, 2048, 1, 1, 131072, launch_tsm_ata_bf16_8x8_mfma_r2048_p1_nt, 2},
+  {"tsm_ata_bf16_8x8_mfma_r2048_p2_nt", 2, 8, 8, 0, 0, 2048, 2, 1, 131072, launch_tsm_ata_bf16_8x8_mfma_r2048_p2_nt, 2},
   {"tsm_ata_bf16_8x8_t8x4_r512_p1_nt", 2, 8, 8, 8, 4, 512, 1, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r512_p2_nt", 2, 8, 8, 8, 4, 512, 2, 1, 16384, launch_tsm_ata_bf16_8x8_t8x4_r512_p2_nt, 1},
   {"tsm_ata_bf16_8x8_t8x4_r1024_p1_nt", 2, 8, 8, 8, 4, 1024, 1, 1, 32768, launch_tsm_ata_bf16_8x8_t8x4_r1024_p1_nt, 1},
@@ -1907,6 +1925,8 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f32_1x1_t1x1_r4096_p2_nt", 1, 1, 1, 1, 1, 4096, 2, 1, 32768, launch_tsm_ata_f32_1x1_t1x1_r4096_p2_nt, 1},
   {"tsm_ata_f32_2x2_t2x2_r4096_p2_nt", 1, 2, 2, 2, 2, 4096, 2, 1, 65536, launch_tsm_ata_f32_2x2_t2x2_r4096_p2_nt, 1},
   {"tsm_ata_f32_4x4_t4x4_r2048_p2_nt", 1, 4, 4, 4, 4, 2048, 2, 1, 65536, launch_tsm_ata_f32_4x4_t4x4_r2048_p2_nt, 1},
+  {"tsm_ata_f32_8x8_mfma_r256_p1_nt", 1, 8, 8, 0, 0, 256, 1, 1, 32768, launch_tsm_ata_f32_8x8_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f32_8x8_mfma_r256_p2_nt", 1, 8, 8, 0, 0, 256, 2, 1, 32768, launch_tsm_ata_f32_8x8_mfma_r256_p2_nt, 2},
   {"tsm_ata_f32_8x8_t8x4_r512_p1_nt", 1, 8, 8, 8, 4, 512, 1, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p1_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r512_p2_nt", 1, 8, 8, 8, 4, 512, 2, 1, 32768, launch_tsm_ata_f32_8x8_t8x4_r512_p2_nt, 1},
   {"tsm_ata_f32_8x8_t8x4_r1024_p1_nt", 1, 8, 8, 8, 4, 1024, 1, 1, 65536, launch_tsm_ata_f32_8x8_t8x4_r1024_p1_nt, 1},
@@ -1951,6 +1971,10 @@ static const tsm::KernelEntry kSym[] = {
   {"tsm_ata_f64_4x4_t4x4_r512_p2_nt", 0, 4, 4, 4, 4, 512, 2, 1, 32768, launch_tsm_ata_f64_4x4_t4x4_r512_p2_nt, 1},
   {"tsm_ata_f64_4x4_t4x4_r1024_p1_nt", 0, 4, 4, 4, 4, 1024, 1, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p1_nt, 1},
   {"tsm_ata_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_ata_f64_4x4_t4x4_r1024_p2_nt, 1},
+  {"tsm_ata_f64_8x8_mfma_r128_p1_nt", 0, 8, 8, 0, 0, 128, 1, 1, 32768, launch_tsm_ata_f64_8x8_mfma_r128_p1_nt, 2},
+  {"tsm_ata_f64_8x8_mfma_r128_p2_nt", 0, 8, 8, 0, 0, 128, 2, 1, 32768, launch_tsm_ata_f64_8x8_mfma_r128_p2_nt, 2},
+  {"tsm_ata_f64_8x8_mfma_r256_p1_nt", 0, 8, 8, 0, 0, 256, 1, 1, 65536, launch_tsm_ata_f64_8x8_mfma_r256_p1_nt, 2},
+  {"tsm_ata_f64_8x8_mfma_r256_p2_nt", 0, 8, 8, 0, 0, 256, 2, 1, 65536, launch_tsm_ata_f64_8x8_mfma_r256_p2_nt, 2},
   {"tsm_ata_f64_8x8_t4x4_r64_p1_nt", 0, 8, 8, 4, 4, 64, 1, 1, 8192, launch_tsm_ata_f64_8x8_t4x4_r64_p1_nt, 1},
   {"tsm_ata_f64_8x8_t4x4_r64_p2_nt", 0, 8, 8, 4, 4, 64, 2, 1, 8192, launch_tsm_ata_f64_8x8_t4x4_r64_p2_nt, 1},
   {"tsm_ata_f64_8x8_t4x4_r128_p1_nt", 0, 8, 8, 4, 4, 128, 1, 1, 16384, launch_tsm_ata_f64_8x8_t4x4_r128_p1_nt, 1},
@@ -1992,24 +2016,24 @@ static const tsm::TunedEntry kSymTuned[] = {
   {2, 1, 1, 0, 1, 1, 0, 1},  // tsm_ata_bf16_1x1_t1x1_r4096_p2_nt
   {2, 2, 2, 1, 1, 0, 1, 0},  // tsm_ata_bf16_2x2_t2x2_r4096_p2_nt
   {2, 4, 4, 2, 1, 1, 2, 1},  // tsm_ata_bf16_4x4_t4x4_r4096_p2_nt
-  {2, 8, 8, 5, 6, 1, 5, 1},  // tsm_ata_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 16, 16, 12, 1, 1, 12, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
-  {2, 32, 32, 20, 1, 1, 20, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
-  {2, 64, 64, 26, 1, 0, 27, 1},  // tsm_ata_bf16_64x64_direct_d4_nt (else tsm_ata_bf16_64x64_mfma_r256_p2_nt)
-  {1, 1, 1, 28, 1, 1, 28, 1},  // tsm_ata_f32_1x1_t1x1_r4096_p2_nt
-  {1, 2, 2, 29, 1, 1, 29, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
-  {1, 4, 4, 30, 1, 1, 30, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
-  {1, 8, 8, 33, 2, 0, 33, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
-  {1, 16, 16, 44, 2, 0, 44, 0},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
-  {1, 32, 32, 48, 3, 0, 52, 1},  // tsm_ata_f32_32x32_direct_d16_nt (else tsm_ata_f32_32x32_mfma_r128_p2_nt)
-  {1, 64, 64, 57, 2, 0, 58, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2)
-  {0, 1, 1, 59, 2, 1, 59, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
-  {0, 2, 2, 64, 1, 1, 64, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
-  {0, 4, 4, 72, 1, 1, 72, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
-  {0, 8, 8, 83, 1, 0, 83, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
-  {0, 16, 16, 92, 1, 1, 92, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
-  {0, 32, 32, 96, 1, 0, 100, 1},  // tsm_ata_f64_32x32_direct_d8_nt (else tsm_ata_f64_32x32_mfma_r32_p1_nt)
-  {0, 64, 64, 108, 2, 0, 109, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
+  {2, 8, 8, 6, 1, 1, 6, 1},  // tsm_ata_bf16_8x8_mfma_r1024_p2_nt
+  {2, 16, 16, 18, 1, 1, 18, 1},  // tsm_ata_bf16_16x16_mfma_r512_p2_nt
+  {2, 32, 32, 26, 1, 1, 26, 1},  // tsm_ata_bf16_32x32_mfma_r256_p2_nt
+  {2, 64, 64, 32, 1, 0, 33, 1},  // tsm_ata_bf16_64x64_direct_d4_nt (else tsm_ata_bf16_64x64_mfma_r256_p2_nt)
+  {1, 1, 1, 34, 1, 1, 34, 1},  // tsm_ata_f32_1x1_t1x1_r4096_p2_nt
+  {1, 2, 2, 35, 1, 1, 35, 1},  // tsm_ata_f32_2x2_t2x2_r4096_p2_nt
+  {1, 4, 4, 36, 1, 1, 36, 1},  // tsm_ata_f32_4x4_t4x4_r2048_p2_nt
+  {1, 8, 8, 41, 2, 0, 41, 0},  // tsm_ata_f32_8x8_t8x4_r1024_p1_nt
+  {1, 16, 16, 52, 2, 0, 52, 0},  // tsm_ata_f32_16x16_mfma_r256_p1_nt
+  {1, 32, 32, 56, 3, 0, 60, 1},  // tsm_ata_f32_32x32_direct_d16_nt (else tsm_ata_f32_32x32_mfma_r128_p2_nt)
+  {1, 64, 64, 65, 2, 0, 66, 1},  // tsm_ata_f32_64x64_direct_d16_nt (else tsm_ata_f32_64x64_mfma_r64_p2)
+  {0, 1, 1, 67, 2, 1, 67, 1},  // tsm_ata_f64_1x1_t1x1_r2048_p1_nt
+  {0, 2, 2, 72, 1, 1, 72, 1},  // tsm_ata_f64_2x2_t2x2_r1024_p2_nt
+  {0, 4, 4, 80, 1, 1, 80, 1},  // tsm_ata_f64_4x4_t4x4_r512_p2_nt
+  {0, 8, 8, 95, 1, 0, 95, 0},  // tsm_ata_f64_8x8_t8x8_r512_p2_nt
+  {0, 16, 16, 104, 1, 1, 104, 1},  // tsm_ata_f64_16x16_mfma_r128_p2_nt
+  {0, 32, 32, 108, 1, 0, 112, 1},  // tsm_ata_f64_32x32_direct_d8_nt (else tsm_ata_f64_32x32_mfma_r32_p1_nt)
+  {0, 64, 64, 120, 2, 0, 121, 0},  // tsm_ata_f64_64x64_direct_d16_nt (else tsm_ata_f64_64x64_mfma_r32_p2_nt)
 };
 
 static const tsm::TunedEntry kTuned[] = {

@@ -890,7 +890,7 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   // side's rows into LDS rows of KP (MP) = 16 elements, the upper half zero.
   constexpr int KP = consumer_width<KC, TM, TN>(), MP = consumer_width<MC, TM, TN>();
   constexpr bool PAD_A = KP != KC, PAD_B = MP != MC;
-  static_assert(!(PAD_A || PAD_B) || (!SYM && KC * ES % 16 == 0 && MC * ES % 16 == 0),
+  static_assert(!(PAD_A || PAD_B) || (KC * ES % 16 == 0 && MC * ES % 16 == 0),
                 "padded rows are whole 16-byte chunks");
   constexpr int A_BYTES = T * KC * ES, B_BYTES = SYM ? 0 : T * MC * ES;  // a tile in memory
   constexpr int LA_BYTES = T * KP * ES, LB_BYTES = SYM ? 0 : T * MP * ES;  // ... in LDS
