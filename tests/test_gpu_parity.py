@@ -206,14 +206,16 @@ def test_every_generated_variant(dt, k, m):
     assert seen >= 1
 
 
-PADDED_CLASSES = [(dt, k, m) for dt in ("f64", "f32", "bf16")
-                  for k, m in ((8, 16), (8, 32), (8, 64), (16, 8), (32, 8), (64, 8))]
+# classes with a side narrower than 16 whose rows are whole 16-byte chunks
+# beside a side of 16 or more
+PADDED_CLASSES = [(dt, k, m) for dt, narrow in (("f64", (2, 4, 8)), ("f32", (4, 8)), ("bf16", (8,)))
+                  for w in narrow for x in (16, 32, 64) for k, m in ((w, x), (x, w))]
 
 
 @pytest.mark.parametrize("dt,k,m", PADDED_CLASSES, ids=lambda x: str(x))
 def test_padded_matrix_core_kernels(dt, k, m):
-    """The matrix-core kernels of the classes with one 8-wide side (that
-    side's rows staged into LDS rows of 16, the upper half zero), forced one
+    """The matrix-core kernels of the classes with one narrow side (that
+    side's rows staged into LDS rows of 16, the rest zero), forced one
     by one: dense inputs on several grids and tile orders, strided views
     with fewer columns than the class (the element path), and sign * 2^e
     inputs against the exact int128 oracle."""
@@ -224,8 +226,11 @@ def test_padded_matrix_core_kernels(dt, k, m):
     n = 70001
     A, B = rand(n, k, DT[dt], 1), rand(n, m, DT[dt], 2)
     Ah, Bh = host(A), host(B)
-    Aw = rand(5003, k + 3, DT[dt], 3)[:, 1:k]  # k - 1 columns, misaligned base
-    Bw = rand(5003, m + 5, DT[dt], 4)[:, 2:m]
+    # views on misaligned bases with one column fewer than the class where
+    # that stays in the class (else all of it, ld > width): the element path
+    kw, mw = (k - 1 if k > 2 else k), (m - 1 if m > 2 else m)
+    Aw = rand(5003, k + 3, DT[dt], 3)[:, 1:1 + kw]
+    Bw = rand(5003, m + 5, DT[dt], 4)[:, 2:2 + mw]
     P, Q = rand(20011, k, DT[dt], 5, dist=1), rand(20011, m, DT[dt], 6, dist=1)
     Ph, Qh = host(P), host(Q)
     exact = oracle.atb_exact(Ph, Qh, 20, 20)

@@ -178,11 +178,13 @@ class KernelConfig:
                 p.append("LDS budget")
             return p
         if self.mfma:
-            # an 8-wide side is padded to 16 in LDS: beside a side of 16 or
-            # more, or for A^T A at 8 (one diagonal block from one padded tile)
-            if (any(w % 16 and w != 8 for w in (self.kc, self.mc))
-                    or (max(self.kc, self.mc) < 16 and not self.sym)):
-                p.append("MFMA needs width classes that are multiples of 16 (one side may be 8)")
+            # a narrower side whose rows are whole 16-byte chunks is padded to
+            # 16 in LDS: beside a side of 16 or more, or for A^T A at 8 (one
+            # diagonal block from one padded tile)
+            if (any(w % 16 and (w * es) % 16 for w in (self.kc, self.mc))
+                    or (max(self.kc, self.mc) < 16 and not (self.sym and self.kc == 8))):
+                p.append("MFMA needs width classes that are multiples of 16 (or a narrower "
+                         "side of whole 16-byte rows beside one)")
             if self.rows % (4 * self.kstep):
                 p.append(f"MFMA rows must be a multiple of {4 * self.kstep}")
             if self.rows // (4 * self.kstep) > MAX_ROWS_PER_THREAD:
@@ -308,14 +310,17 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
 
 
 def padded_mfma_variants(dtype: str) -> list[KernelConfig]:
-    """Matrix-core kernels of the classes with one side of 8 and the other of
-    16 or more, the 8-wide side padded to 16 in LDS.  Their register-tile
-    kernels need 8 to 64 FMAs per element read (plus the BF16 unpacking),
-    more than the vector units sustain at the HBM rate for FP32 and BF16.
-    Three tile sizes around 32 KiB of staged rows, one or two tiles in
-    flight, streaming loads."""
+    """Matrix-core kernels of the classes with one side narrower than 16
+    (rows of whole 16-byte chunks: 8 for BF16, 4 and 8 for FP32, 2, 4 and 8
+    for FP64) and the other of 16 or more, the narrow side padded to 16 in
+    LDS.  Their register-tile kernels need up to 64 FMAs per element read
+    (plus the BF16 unpacking), more than the vector units sustain at the HBM
+    rate for FP32 and BF16.  Three tile sizes around 32 KiB of staged rows,
+    one or two tiles in flight, streaming loads."""
     out = []
-    for kc, mc in [(8, w) for w in (16, 32, 64)] + [(w, 8) for w in (16, 32, 64)]:
+    narrow = [w for w in (2, 4, 8) if (w * ELEM_BYTES[dtype]) % 16 == 0]
+    for kc, mc in ([(n, w) for n in narrow for w in (16, 32, 64)]
+                   + [(w, n) for n in narrow for w in (16, 32, 64)]):
         mrows = _pow2_floor(32768 // ((max(kc, 16) + max(mc, 16)) * ELEM_BYTES[dtype]))
         for rows in (mrows // 2, mrows, 2 * mrows):
             for pf in (1, 2):

@@ -280,8 +280,24 @@ hipError_t launch_tsm_atb_f32_4x1_t4x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_f32_4x2_t4x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_4x8_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_4x16_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_4x32_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_4x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_4x64_t4x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x1_t8x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_8x2_t8x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -352,6 +368,10 @@ hipError_t launch_tsm_atb_f32_8x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_16x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x4_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x8_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_16x8_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -417,6 +437,12 @@ hipError_t launch_tsm_atb_f32_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_16x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_32x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x8_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_32x8_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -488,6 +514,12 @@ hipError_t launch_tsm_atb_f32_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, 
 hipError_t launch_tsm_atb_f32_32x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x2_t8x2_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f32_64x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x4_t8x4_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x8_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f32_64x8_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -564,8 +596,26 @@ hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x4_t2x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x8_t2x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x16_t2x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x32_t2x8_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_2x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_2x64_t2x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x1_t4x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x2_t4x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -586,7 +636,19 @@ hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p1_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x8_t4x8_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -607,6 +669,12 @@ hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1(const tsm::AtbArgs&, unsigned, h
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_4x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_4x64_t4x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -676,7 +744,19 @@ hipError_t launch_tsm_atb_f64_8x64_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_f64_8x64_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_8x64_t8x8_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x2_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_16x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x8_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x8_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -752,7 +832,19 @@ hipError_t launch_tsm_atb_f64_16x32_t8x8_r64_p2_nt(const tsm::AtbArgs&, unsigned
 hipError_t launch_tsm_atb_f64_16x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_16x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x1_t8x1_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x2_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_32x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p1(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p2(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x4_t4x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -842,7 +934,19 @@ hipError_t launch_tsm_atb_f64_32x32_t8x8_r128_p2_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_f64_32x64_direct_d8_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_32x64_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x1_t8x1_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x2_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x2_t8x2_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r32_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r64_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_f64_64x4_mfma_r64_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x4_t8x4_r32_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x8_mfma_r16_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_f64_64x8_mfma_r16_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1283,8 +1387,24 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_4x2_t4x2_r1024_p2_nt", 1, 4, 2, 4, 2, 1024, 2, 1, 49152, launch_tsm_atb_f32_4x2_t4x2_r1024_p2_nt, 1},
   {"tsm_atb_f32_4x4_t4x4_r1024_p2_nt", 1, 4, 4, 4, 4, 1024, 2, 1, 65536, launch_tsm_atb_f32_4x4_t4x4_r1024_p2_nt, 1},
   {"tsm_atb_f32_4x8_t4x4_r512_p2_nt", 1, 4, 8, 4, 4, 512, 2, 1, 49152, launch_tsm_atb_f32_4x8_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_f32_4x16_mfma_r128_p1_nt", 1, 4, 16, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_f32_4x16_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_4x16_mfma_r128_p2_nt", 1, 4, 16, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_f32_4x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_4x16_mfma_r256_p1_nt", 1, 4, 16, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_f32_4x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_4x16_mfma_r256_p2_nt", 1, 4, 16, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_f32_4x16_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_4x16_t4x4_r256_p2_nt", 1, 4, 16, 4, 4, 256, 2, 1, 40960, launch_tsm_atb_f32_4x16_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_f32_4x32_mfma_r64_p1_nt", 1, 4, 32, 0, 0, 64, 1, 1, 24576, launch_tsm_atb_f32_4x32_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_4x32_mfma_r64_p2_nt", 1, 4, 32, 0, 0, 64, 2, 1, 24576, launch_tsm_atb_f32_4x32_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_4x32_mfma_r128_p1_nt", 1, 4, 32, 0, 0, 128, 1, 1, 49152, launch_tsm_atb_f32_4x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_4x32_mfma_r128_p2_nt", 1, 4, 32, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_f32_4x32_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_4x32_mfma_r256_p1_nt", 1, 4, 32, 0, 0, 256, 1, 1, 98304, launch_tsm_atb_f32_4x32_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_4x32_mfma_r256_p2_nt", 1, 4, 32, 0, 0, 256, 2, 1, 98304, launch_tsm_atb_f32_4x32_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_4x32_t4x4_r128_p2_nt", 1, 4, 32, 4, 4, 128, 2, 1, 36864, launch_tsm_atb_f32_4x32_t4x4_r128_p2_nt, 1},
+  {"tsm_atb_f32_4x64_mfma_r32_p1_nt", 1, 4, 64, 0, 0, 32, 1, 1, 20480, launch_tsm_atb_f32_4x64_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f32_4x64_mfma_r32_p2_nt", 1, 4, 64, 0, 0, 32, 2, 1, 20480, launch_tsm_atb_f32_4x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f32_4x64_mfma_r64_p1_nt", 1, 4, 64, 0, 0, 64, 1, 1, 40960, launch_tsm_atb_f32_4x64_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_4x64_mfma_r64_p2_nt", 1, 4, 64, 0, 0, 64, 2, 1, 40960, launch_tsm_atb_f32_4x64_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_4x64_mfma_r128_p1_nt", 1, 4, 64, 0, 0, 128, 1, 1, 81920, launch_tsm_atb_f32_4x64_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_4x64_mfma_r128_p2_nt", 1, 4, 64, 0, 0, 128, 2, 1, 81920, launch_tsm_atb_f32_4x64_mfma_r128_p2_nt, 2},
   {"tsm_atb_f32_4x64_t4x4_r64_p2_nt", 1, 4, 64, 4, 4, 64, 2, 1, 34816, launch_tsm_atb_f32_4x64_t4x4_r64_p2_nt, 1},
   {"tsm_atb_f32_8x1_t8x1_r512_p2_nt", 1, 8, 1, 8, 1, 512, 2, 1, 36864, launch_tsm_atb_f32_8x1_t8x1_r512_p2_nt, 1},
   {"tsm_atb_f32_8x2_t8x2_r512_p2_nt", 1, 8, 2, 8, 2, 512, 2, 1, 40960, launch_tsm_atb_f32_8x2_t8x2_r512_p2_nt, 1},
@@ -1355,6 +1475,10 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_8x64_t8x4_r64_p2_nt", 1, 8, 64, 8, 4, 64, 2, 1, 36864, launch_tsm_atb_f32_8x64_t8x4_r64_p2_nt, 1},
   {"tsm_atb_f32_16x1_t8x1_r256_p2_nt", 1, 16, 1, 8, 1, 256, 2, 1, 34816, launch_tsm_atb_f32_16x1_t8x1_r256_p2_nt, 1},
   {"tsm_atb_f32_16x2_t8x2_r256_p2_nt", 1, 16, 2, 8, 2, 256, 2, 1, 36864, launch_tsm_atb_f32_16x2_t8x2_r256_p2_nt, 1},
+  {"tsm_atb_f32_16x4_mfma_r128_p1_nt", 1, 16, 4, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_f32_16x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_16x4_mfma_r128_p2_nt", 1, 16, 4, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_f32_16x4_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_16x4_mfma_r256_p1_nt", 1, 16, 4, 0, 0, 256, 1, 1, 65536, launch_tsm_atb_f32_16x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_16x4_mfma_r256_p2_nt", 1, 16, 4, 0, 0, 256, 2, 1, 65536, launch_tsm_atb_f32_16x4_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_16x4_t8x4_r256_p2_nt", 1, 16, 4, 8, 4, 256, 2, 1, 40960, launch_tsm_atb_f32_16x4_t8x4_r256_p2_nt, 1},
   {"tsm_atb_f32_16x8_mfma_r128_p1_nt", 1, 16, 8, 0, 0, 128, 1, 1, 32768, launch_tsm_atb_f32_16x8_mfma_r128_p1_nt, 2},
   {"tsm_atb_f32_16x8_mfma_r128_p2_nt", 1, 16, 8, 0, 0, 128, 2, 1, 32768, launch_tsm_atb_f32_16x8_mfma_r128_p2_nt, 2},
@@ -1420,6 +1544,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_16x64_mfma_r64_p2_nt", 1, 16, 64, 0, 0, 64, 2, 1, 40960, launch_tsm_atb_f32_16x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f32_32x1_t8x1_r128_p2_nt", 1, 32, 1, 8, 1, 128, 2, 1, 33792, launch_tsm_atb_f32_32x1_t8x1_r128_p2_nt, 1},
   {"tsm_atb_f32_32x2_t8x2_r128_p2_nt", 1, 32, 2, 8, 2, 128, 2, 1, 34816, launch_tsm_atb_f32_32x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_f32_32x4_mfma_r64_p1_nt", 1, 32, 4, 0, 0, 64, 1, 1, 24576, launch_tsm_atb_f32_32x4_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_32x4_mfma_r64_p2_nt", 1, 32, 4, 0, 0, 64, 2, 1, 24576, launch_tsm_atb_f32_32x4_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_32x4_mfma_r128_p1_nt", 1, 32, 4, 0, 0, 128, 1, 1, 49152, launch_tsm_atb_f32_32x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_32x4_mfma_r128_p2_nt", 1, 32, 4, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_f32_32x4_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f32_32x4_mfma_r256_p1_nt", 1, 32, 4, 0, 0, 256, 1, 1, 98304, launch_tsm_atb_f32_32x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f32_32x4_mfma_r256_p2_nt", 1, 32, 4, 0, 0, 256, 2, 1, 98304, launch_tsm_atb_f32_32x4_mfma_r256_p2_nt, 2},
   {"tsm_atb_f32_32x4_t8x4_r128_p2_nt", 1, 32, 4, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_f32_32x4_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f32_32x8_mfma_r64_p1_nt", 1, 32, 8, 0, 0, 64, 1, 1, 24576, launch_tsm_atb_f32_32x8_mfma_r64_p1_nt, 2},
   {"tsm_atb_f32_32x8_mfma_r64_p2_nt", 1, 32, 8, 0, 0, 64, 2, 1, 24576, launch_tsm_atb_f32_32x8_mfma_r64_p2_nt, 2},
@@ -1491,6 +1621,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f32_32x64_mfma_r64_p2_nt", 1, 32, 64, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f32_32x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f32_64x1_t8x1_r64_p2_nt", 1, 64, 1, 8, 1, 64, 2, 1, 33280, launch_tsm_atb_f32_64x1_t8x1_r64_p2_nt, 1},
   {"tsm_atb_f32_64x2_t8x2_r64_p2_nt", 1, 64, 2, 8, 2, 64, 2, 1, 33792, launch_tsm_atb_f32_64x2_t8x2_r64_p2_nt, 1},
+  {"tsm_atb_f32_64x4_mfma_r32_p1_nt", 1, 64, 4, 0, 0, 32, 1, 1, 20480, launch_tsm_atb_f32_64x4_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f32_64x4_mfma_r32_p2_nt", 1, 64, 4, 0, 0, 32, 2, 1, 20480, launch_tsm_atb_f32_64x4_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f32_64x4_mfma_r64_p1_nt", 1, 64, 4, 0, 0, 64, 1, 1, 40960, launch_tsm_atb_f32_64x4_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f32_64x4_mfma_r64_p2_nt", 1, 64, 4, 0, 0, 64, 2, 1, 40960, launch_tsm_atb_f32_64x4_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f32_64x4_mfma_r128_p1_nt", 1, 64, 4, 0, 0, 128, 1, 1, 81920, launch_tsm_atb_f32_64x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f32_64x4_mfma_r128_p2_nt", 1, 64, 4, 0, 0, 128, 2, 1, 81920, launch_tsm_atb_f32_64x4_mfma_r128_p2_nt, 2},
   {"tsm_atb_f32_64x4_t8x4_r64_p2_nt", 1, 64, 4, 8, 4, 64, 2, 1, 34816, launch_tsm_atb_f32_64x4_t8x4_r64_p2_nt, 1},
   {"tsm_atb_f32_64x8_mfma_r32_p1_nt", 1, 64, 8, 0, 0, 32, 1, 1, 20480, launch_tsm_atb_f32_64x8_mfma_r32_p1_nt, 2},
   {"tsm_atb_f32_64x8_mfma_r32_p2_nt", 1, 64, 8, 0, 0, 32, 2, 1, 20480, launch_tsm_atb_f32_64x8_mfma_r32_p2_nt, 2},
@@ -1567,8 +1703,26 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_2x2_t2x2_r2048_p2_nt", 0, 2, 2, 2, 2, 2048, 2, 1, 131072, launch_tsm_atb_f64_2x2_t2x2_r2048_p2_nt, 1},
   {"tsm_atb_f64_2x4_t2x4_r512_p2_nt", 0, 2, 4, 2, 4, 512, 2, 1, 49152, launch_tsm_atb_f64_2x4_t2x4_r512_p2_nt, 1},
   {"tsm_atb_f64_2x8_t2x8_r256_p2_nt", 0, 2, 8, 2, 8, 256, 2, 1, 40960, launch_tsm_atb_f64_2x8_t2x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_2x16_mfma_r64_p1_nt", 0, 2, 16, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_2x16_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_2x16_mfma_r64_p2_nt", 0, 2, 16, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_2x16_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_2x16_mfma_r128_p1_nt", 0, 2, 16, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_2x16_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_2x16_mfma_r128_p2_nt", 0, 2, 16, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_2x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_2x16_mfma_r256_p1_nt", 0, 2, 16, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_2x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f64_2x16_mfma_r256_p2_nt", 0, 2, 16, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_2x16_mfma_r256_p2_nt, 2},
   {"tsm_atb_f64_2x16_t2x8_r128_p2_nt", 0, 2, 16, 2, 8, 128, 2, 1, 36864, launch_tsm_atb_f64_2x16_t2x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_2x32_mfma_r32_p1_nt", 0, 2, 32, 0, 0, 32, 1, 1, 24576, launch_tsm_atb_f64_2x32_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_2x32_mfma_r32_p2_nt", 0, 2, 32, 0, 0, 32, 2, 1, 24576, launch_tsm_atb_f64_2x32_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_2x32_mfma_r64_p1_nt", 0, 2, 32, 0, 0, 64, 1, 1, 49152, launch_tsm_atb_f64_2x32_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_2x32_mfma_r64_p2_nt", 0, 2, 32, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f64_2x32_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_2x32_mfma_r128_p1_nt", 0, 2, 32, 0, 0, 128, 1, 1, 98304, launch_tsm_atb_f64_2x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_2x32_mfma_r128_p2_nt", 0, 2, 32, 0, 0, 128, 2, 1, 98304, launch_tsm_atb_f64_2x32_mfma_r128_p2_nt, 2},
   {"tsm_atb_f64_2x32_t2x8_r64_p2_nt", 0, 2, 32, 2, 8, 64, 2, 1, 34816, launch_tsm_atb_f64_2x32_t2x8_r64_p2_nt, 1},
+  {"tsm_atb_f64_2x64_mfma_r16_p1_nt", 0, 2, 64, 0, 0, 16, 1, 1, 20480, launch_tsm_atb_f64_2x64_mfma_r16_p1_nt, 2},
+  {"tsm_atb_f64_2x64_mfma_r16_p2_nt", 0, 2, 64, 0, 0, 16, 2, 1, 20480, launch_tsm_atb_f64_2x64_mfma_r16_p2_nt, 2},
+  {"tsm_atb_f64_2x64_mfma_r32_p1_nt", 0, 2, 64, 0, 0, 32, 1, 1, 40960, launch_tsm_atb_f64_2x64_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_2x64_mfma_r32_p2_nt", 0, 2, 64, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_2x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_2x64_mfma_r64_p1_nt", 0, 2, 64, 0, 0, 64, 1, 1, 81920, launch_tsm_atb_f64_2x64_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_2x64_mfma_r64_p2_nt", 0, 2, 64, 0, 0, 64, 2, 1, 81920, launch_tsm_atb_f64_2x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f64_2x64_t2x8_r32_p2_nt", 0, 2, 64, 2, 8, 32, 2, 1, 33792, launch_tsm_atb_f64_2x64_t2x8_r32_p2_nt, 1},
   {"tsm_atb_f64_4x1_t4x1_r512_p2_nt", 0, 4, 1, 4, 1, 512, 2, 1, 40960, launch_tsm_atb_f64_4x1_t4x1_r512_p2_nt, 1},
   {"tsm_atb_f64_4x2_t4x2_r512_p2_nt", 0, 4, 2, 4, 2, 512, 2, 1, 49152, launch_tsm_atb_f64_4x2_t4x2_r512_p2_nt, 1},
@@ -1589,7 +1743,19 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x4_t4x4_r1024_p2", 0, 4, 4, 4, 4, 1024, 2, 0, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2, 1},
   {"tsm_atb_f64_4x4_t4x4_r1024_p2_nt", 0, 4, 4, 4, 4, 1024, 2, 1, 131072, launch_tsm_atb_f64_4x4_t4x4_r1024_p2_nt, 1},
   {"tsm_atb_f64_4x8_t4x8_r256_p2_nt", 0, 4, 8, 4, 8, 256, 2, 1, 49152, launch_tsm_atb_f64_4x8_t4x8_r256_p2_nt, 1},
+  {"tsm_atb_f64_4x16_mfma_r64_p1_nt", 0, 4, 16, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_4x16_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_4x16_mfma_r64_p2_nt", 0, 4, 16, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_4x16_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_4x16_mfma_r128_p1_nt", 0, 4, 16, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_4x16_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_4x16_mfma_r128_p2_nt", 0, 4, 16, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_4x16_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_4x16_mfma_r256_p1_nt", 0, 4, 16, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_4x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f64_4x16_mfma_r256_p2_nt", 0, 4, 16, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_4x16_mfma_r256_p2_nt, 2},
   {"tsm_atb_f64_4x16_t4x8_r128_p2_nt", 0, 4, 16, 4, 8, 128, 2, 1, 40960, launch_tsm_atb_f64_4x16_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_4x32_mfma_r32_p1_nt", 0, 4, 32, 0, 0, 32, 1, 1, 24576, launch_tsm_atb_f64_4x32_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_4x32_mfma_r32_p2_nt", 0, 4, 32, 0, 0, 32, 2, 1, 24576, launch_tsm_atb_f64_4x32_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_4x32_mfma_r64_p1_nt", 0, 4, 32, 0, 0, 64, 1, 1, 49152, launch_tsm_atb_f64_4x32_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_4x32_mfma_r64_p2_nt", 0, 4, 32, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f64_4x32_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_4x32_mfma_r128_p1_nt", 0, 4, 32, 0, 0, 128, 1, 1, 98304, launch_tsm_atb_f64_4x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_4x32_mfma_r128_p2_nt", 0, 4, 32, 0, 0, 128, 2, 1, 98304, launch_tsm_atb_f64_4x32_mfma_r128_p2_nt, 2},
   {"tsm_atb_f64_4x32_t4x4_r32_p1", 0, 4, 32, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p1, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p2", 0, 4, 32, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2, 1},
   {"tsm_atb_f64_4x32_t4x4_r32_p2_nt", 0, 4, 32, 4, 4, 32, 2, 1, 18432, launch_tsm_atb_f64_4x32_t4x4_r32_p2_nt, 1},
@@ -1610,6 +1776,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_4x32_t4x8_r128_p1_nt", 0, 4, 32, 4, 8, 128, 1, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p1_nt, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p2", 0, 4, 32, 4, 8, 128, 2, 0, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2, 1},
   {"tsm_atb_f64_4x32_t4x8_r128_p2_nt", 0, 4, 32, 4, 8, 128, 2, 1, 73728, launch_tsm_atb_f64_4x32_t4x8_r128_p2_nt, 1},
+  {"tsm_atb_f64_4x64_mfma_r16_p1_nt", 0, 4, 64, 0, 0, 16, 1, 1, 20480, launch_tsm_atb_f64_4x64_mfma_r16_p1_nt, 2},
+  {"tsm_atb_f64_4x64_mfma_r16_p2_nt", 0, 4, 64, 0, 0, 16, 2, 1, 20480, launch_tsm_atb_f64_4x64_mfma_r16_p2_nt, 2},
+  {"tsm_atb_f64_4x64_mfma_r32_p1_nt", 0, 4, 64, 0, 0, 32, 1, 1, 40960, launch_tsm_atb_f64_4x64_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_4x64_mfma_r32_p2_nt", 0, 4, 64, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_4x64_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_4x64_mfma_r64_p1_nt", 0, 4, 64, 0, 0, 64, 1, 1, 81920, launch_tsm_atb_f64_4x64_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_4x64_mfma_r64_p2_nt", 0, 4, 64, 0, 0, 64, 2, 1, 81920, launch_tsm_atb_f64_4x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f64_4x64_t4x8_r32_p2_nt", 0, 4, 64, 4, 8, 32, 2, 1, 34816, launch_tsm_atb_f64_4x64_t4x8_r32_p2_nt, 1},
   {"tsm_atb_f64_8x1_t8x1_r256_p2_nt", 0, 8, 1, 8, 1, 256, 2, 1, 36864, launch_tsm_atb_f64_8x1_t8x1_r256_p2_nt, 1},
   {"tsm_atb_f64_8x2_t8x2_r256_p2_nt", 0, 8, 2, 8, 2, 256, 2, 1, 40960, launch_tsm_atb_f64_8x2_t8x2_r256_p2_nt, 1},
@@ -1679,7 +1851,19 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_8x64_mfma_r64_p2_nt", 0, 8, 64, 0, 0, 64, 2, 1, 81920, launch_tsm_atb_f64_8x64_mfma_r64_p2_nt, 2},
   {"tsm_atb_f64_8x64_t8x8_r32_p2_nt", 0, 8, 64, 8, 8, 32, 2, 1, 36864, launch_tsm_atb_f64_8x64_t8x8_r32_p2_nt, 1},
   {"tsm_atb_f64_16x1_t8x1_r128_p2_nt", 0, 16, 1, 8, 1, 128, 2, 1, 34816, launch_tsm_atb_f64_16x1_t8x1_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x2_mfma_r64_p1_nt", 0, 16, 2, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_16x2_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_16x2_mfma_r64_p2_nt", 0, 16, 2, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_16x2_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_16x2_mfma_r128_p1_nt", 0, 16, 2, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_16x2_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_16x2_mfma_r128_p2_nt", 0, 16, 2, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_16x2_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_16x2_mfma_r256_p1_nt", 0, 16, 2, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_16x2_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f64_16x2_mfma_r256_p2_nt", 0, 16, 2, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_16x2_mfma_r256_p2_nt, 2},
   {"tsm_atb_f64_16x2_t8x2_r128_p2_nt", 0, 16, 2, 8, 2, 128, 2, 1, 36864, launch_tsm_atb_f64_16x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_f64_16x4_mfma_r64_p1_nt", 0, 16, 4, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_16x4_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_16x4_mfma_r64_p2_nt", 0, 16, 4, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_16x4_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_16x4_mfma_r128_p1_nt", 0, 16, 4, 0, 0, 128, 1, 1, 65536, launch_tsm_atb_f64_16x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_16x4_mfma_r128_p2_nt", 0, 16, 4, 0, 0, 128, 2, 1, 65536, launch_tsm_atb_f64_16x4_mfma_r128_p2_nt, 2},
+  {"tsm_atb_f64_16x4_mfma_r256_p1_nt", 0, 16, 4, 0, 0, 256, 1, 1, 131072, launch_tsm_atb_f64_16x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_f64_16x4_mfma_r256_p2_nt", 0, 16, 4, 0, 0, 256, 2, 1, 131072, launch_tsm_atb_f64_16x4_mfma_r256_p2_nt, 2},
   {"tsm_atb_f64_16x4_t8x4_r128_p2_nt", 0, 16, 4, 8, 4, 128, 2, 1, 40960, launch_tsm_atb_f64_16x4_t8x4_r128_p2_nt, 1},
   {"tsm_atb_f64_16x8_mfma_r64_p1_nt", 0, 16, 8, 0, 0, 64, 1, 1, 32768, launch_tsm_atb_f64_16x8_mfma_r64_p1_nt, 2},
   {"tsm_atb_f64_16x8_mfma_r64_p2_nt", 0, 16, 8, 0, 0, 64, 2, 1, 32768, launch_tsm_atb_f64_16x8_mfma_r64_p2_nt, 2},
@@ -1755,7 +1939,19 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_16x64_direct_d8_nt", 0, 16, 64, 0, 0, 4, 8, 1, 16384, launch_tsm_atb_f64_16x64_direct_d8_nt, 3},
   {"tsm_atb_f64_16x64_mfma_r32_p2_nt", 0, 16, 64, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_16x64_mfma_r32_p2_nt, 2},
   {"tsm_atb_f64_32x1_t8x1_r64_p2_nt", 0, 32, 1, 8, 1, 64, 2, 1, 33792, launch_tsm_atb_f64_32x1_t8x1_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x2_mfma_r32_p1_nt", 0, 32, 2, 0, 0, 32, 1, 1, 24576, launch_tsm_atb_f64_32x2_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_32x2_mfma_r32_p2_nt", 0, 32, 2, 0, 0, 32, 2, 1, 24576, launch_tsm_atb_f64_32x2_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_32x2_mfma_r64_p1_nt", 0, 32, 2, 0, 0, 64, 1, 1, 49152, launch_tsm_atb_f64_32x2_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_32x2_mfma_r64_p2_nt", 0, 32, 2, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f64_32x2_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_32x2_mfma_r128_p1_nt", 0, 32, 2, 0, 0, 128, 1, 1, 98304, launch_tsm_atb_f64_32x2_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_32x2_mfma_r128_p2_nt", 0, 32, 2, 0, 0, 128, 2, 1, 98304, launch_tsm_atb_f64_32x2_mfma_r128_p2_nt, 2},
   {"tsm_atb_f64_32x2_t8x2_r64_p2_nt", 0, 32, 2, 8, 2, 64, 2, 1, 34816, launch_tsm_atb_f64_32x2_t8x2_r64_p2_nt, 1},
+  {"tsm_atb_f64_32x4_mfma_r32_p1_nt", 0, 32, 4, 0, 0, 32, 1, 1, 24576, launch_tsm_atb_f64_32x4_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_32x4_mfma_r32_p2_nt", 0, 32, 4, 0, 0, 32, 2, 1, 24576, launch_tsm_atb_f64_32x4_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_32x4_mfma_r64_p1_nt", 0, 32, 4, 0, 0, 64, 1, 1, 49152, launch_tsm_atb_f64_32x4_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_32x4_mfma_r64_p2_nt", 0, 32, 4, 0, 0, 64, 2, 1, 49152, launch_tsm_atb_f64_32x4_mfma_r64_p2_nt, 2},
+  {"tsm_atb_f64_32x4_mfma_r128_p1_nt", 0, 32, 4, 0, 0, 128, 1, 1, 98304, launch_tsm_atb_f64_32x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_f64_32x4_mfma_r128_p2_nt", 0, 32, 4, 0, 0, 128, 2, 1, 98304, launch_tsm_atb_f64_32x4_mfma_r128_p2_nt, 2},
   {"tsm_atb_f64_32x4_t4x4_r32_p1", 0, 32, 4, 4, 4, 32, 1, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p1, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p2", 0, 32, 4, 4, 4, 32, 2, 0, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2, 1},
   {"tsm_atb_f64_32x4_t4x4_r32_p2_nt", 0, 32, 4, 4, 4, 32, 2, 1, 18432, launch_tsm_atb_f64_32x4_t4x4_r32_p2_nt, 1},
@@ -1845,7 +2041,19 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_f64_32x64_direct_d8_nt", 0, 32, 64, 0, 0, 4, 8, 1, 32768, launch_tsm_atb_f64_32x64_direct_d8_nt, 3},
   {"tsm_atb_f64_32x64_mfma_r32_p2_nt", 0, 32, 64, 0, 0, 32, 2, 1, 49152, launch_tsm_atb_f64_32x64_mfma_r32_p2_nt, 2},
   {"tsm_atb_f64_64x1_t8x1_r32_p2_nt", 0, 64, 1, 8, 1, 32, 2, 1, 33280, launch_tsm_atb_f64_64x1_t8x1_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x2_mfma_r16_p1_nt", 0, 64, 2, 0, 0, 16, 1, 1, 20480, launch_tsm_atb_f64_64x2_mfma_r16_p1_nt, 2},
+  {"tsm_atb_f64_64x2_mfma_r16_p2_nt", 0, 64, 2, 0, 0, 16, 2, 1, 20480, launch_tsm_atb_f64_64x2_mfma_r16_p2_nt, 2},
+  {"tsm_atb_f64_64x2_mfma_r32_p1_nt", 0, 64, 2, 0, 0, 32, 1, 1, 40960, launch_tsm_atb_f64_64x2_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_64x2_mfma_r32_p2_nt", 0, 64, 2, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_64x2_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_64x2_mfma_r64_p1_nt", 0, 64, 2, 0, 0, 64, 1, 1, 81920, launch_tsm_atb_f64_64x2_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_64x2_mfma_r64_p2_nt", 0, 64, 2, 0, 0, 64, 2, 1, 81920, launch_tsm_atb_f64_64x2_mfma_r64_p2_nt, 2},
   {"tsm_atb_f64_64x2_t8x2_r32_p2_nt", 0, 64, 2, 8, 2, 32, 2, 1, 33792, launch_tsm_atb_f64_64x2_t8x2_r32_p2_nt, 1},
+  {"tsm_atb_f64_64x4_mfma_r16_p1_nt", 0, 64, 4, 0, 0, 16, 1, 1, 20480, launch_tsm_atb_f64_64x4_mfma_r16_p1_nt, 2},
+  {"tsm_atb_f64_64x4_mfma_r16_p2_nt", 0, 64, 4, 0, 0, 16, 2, 1, 20480, launch_tsm_atb_f64_64x4_mfma_r16_p2_nt, 2},
+  {"tsm_atb_f64_64x4_mfma_r32_p1_nt", 0, 64, 4, 0, 0, 32, 1, 1, 40960, launch_tsm_atb_f64_64x4_mfma_r32_p1_nt, 2},
+  {"tsm_atb_f64_64x4_mfma_r32_p2_nt", 0, 64, 4, 0, 0, 32, 2, 1, 40960, launch_tsm_atb_f64_64x4_mfma_r32_p2_nt, 2},
+  {"tsm_atb_f64_64x4_mfma_r64_p1_nt", 0, 64, 4, 0, 0, 64, 1, 1, 81920, launch_tsm_atb_f64_64x4_mfma_r64_p1_nt, 2},
+  {"tsm_atb_f64_64x4_mfma_r64_p2_nt", 0, 64, 4, 0, 0, 64, 2, 1, 81920, launch_tsm_atb_f64_64x4_mfma_r64_p2_nt, 2},
   {"tsm_atb_f64_64x4_t8x4_r32_p2_nt", 0, 64, 4, 8, 4, 32, 2, 1, 34816, launch_tsm_atb_f64_64x4_t8x4_r32_p2_nt, 1},
   {"tsm_atb_f64_64x8_mfma_r16_p1_nt", 0, 64, 8, 0, 0, 16, 1, 1, 20480, launch_tsm_atb_f64_64x8_mfma_r16_p1_nt, 2},
   {"tsm_atb_f64_64x8_mfma_r16_p2_nt", 0, 64, 8, 0, 0, 16, 2, 1, 20480, launch_tsm_atb_f64_64x8_mfma_r16_p2_nt, 2},
@@ -2104,86 +2312,86 @@ static const tsm::TunedEntry kTuned[] = {
   {1, 4, 2, 274, 1, 1, 274, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
   {1, 4, 4, 275, 1, 1, 275, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
   {1, 4, 8, 276, 1, 1, 276, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
-  {1, 4, 16, 277, 1, 1, 277, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
-  {1, 4, 32, 278, 1, 1, 278, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
-  {1, 4, 64, 279, 1, 1, 279, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
-  {1, 8, 1, 280, 2, 1, 280, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
-  {1, 8, 2, 281, 1, 1, 281, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
-  {1, 8, 4, 282, 1, 1, 282, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
-  {1, 8, 8, 318, 1, 1, 318, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
-  {1, 8, 16, 332, 1, 1, 332, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
-  {1, 8, 32, 334, 2, 1, 334, 1},  // tsm_atb_f32_8x32_mfma_r64_p2_nt
-  {1, 8, 64, 346, 2, 1, 346, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
-  {1, 16, 1, 347, 2, 1, 347, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
-  {1, 16, 2, 348, 2, 1, 348, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
-  {1, 16, 4, 349, 1, 0, 349, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
-  {1, 16, 8, 354, 1, 1, 354, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
-  {1, 16, 16, 361, 1, 1, 361, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-  {1, 16, 32, 408, 1, 0, 409, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
-  {1, 16, 64, 411, 1, 1, 411, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
-  {1, 32, 1, 412, 2, 1, 412, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
-  {1, 32, 2, 413, 2, 1, 413, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
-  {1, 32, 4, 414, 2, 1, 414, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
-  {1, 32, 8, 415, 3, 1, 415, 1},  // tsm_atb_f32_32x8_mfma_r64_p1_nt
-  {1, 32, 16, 422, 1, 0, 423, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
-  {1, 32, 32, 434, 1, 1, 434, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 481, 1, 0, 482, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2_nt)
-  {1, 64, 1, 483, 2, 1, 483, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
-  {1, 64, 2, 484, 2, 1, 484, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
-  {1, 64, 4, 485, 2, 1, 485, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
-  {1, 64, 8, 492, 2, 1, 492, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
-  {1, 64, 16, 494, 1, 1, 494, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
-  {1, 64, 32, 495, 1, 0, 496, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
-  {1, 64, 64, 498, 1, 0, 506, 1},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2)
-  {0, 1, 1, 535, 1, 1, 535, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
-  {0, 1, 2, 538, 1, 1, 538, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
-  {0, 1, 4, 539, 1, 1, 539, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
-  {0, 1, 8, 540, 1, 1, 540, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
-  {0, 1, 16, 541, 1, 1, 541, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
-  {0, 1, 32, 542, 1, 1, 542, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
-  {0, 1, 64, 543, 1, 1, 543, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
-  {0, 2, 1, 544, 1, 1, 544, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
-  {0, 2, 2, 556, 1, 1, 556, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
-  {0, 2, 4, 559, 1, 1, 559, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
-  {0, 2, 8, 560, 1, 1, 560, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
-  {0, 2, 16, 561, 1, 1, 561, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
-  {0, 2, 32, 562, 1, 1, 562, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
-  {0, 2, 64, 563, 1, 1, 563, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
-  {0, 4, 1, 564, 1, 1, 564, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
-  {0, 4, 2, 565, 1, 1, 565, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
-  {0, 4, 4, 569, 2, 1, 569, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-  {0, 4, 8, 582, 1, 1, 582, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
-  {0, 4, 16, 583, 1, 1, 583, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
-  {0, 4, 32, 596, 2, 1, 596, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-  {0, 4, 64, 604, 1, 1, 604, 1},  // tsm_atb_f64_4x64_t4x8_r32_p2_nt
-  {0, 8, 1, 605, 2, 1, 605, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
-  {0, 8, 2, 606, 1, 1, 606, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
-  {0, 8, 4, 607, 1, 1, 607, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
-  {0, 8, 8, 612, 1, 1, 612, 1},  // tsm_atb_f64_8x8_t4x4_r128_p2_nt
-  {0, 8, 16, 657, 1, 1, 657, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
-  {0, 8, 32, 664, 1, 1, 664, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
-  {0, 8, 64, 667, 2, 1, 667, 1},  // tsm_atb_f64_8x64_mfma_r32_p1_nt
-  {0, 16, 1, 672, 2, 1, 672, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
-  {0, 16, 2, 673, 2, 1, 673, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
-  {0, 16, 4, 674, 1, 1, 674, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
-  {0, 16, 8, 681, 1, 1, 681, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
-  {0, 16, 16, 688, 1, 1, 688, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
-  {0, 16, 32, 744, 1, 0, 745, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
-  {0, 16, 64, 747, 1, 1, 747, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
-  {0, 32, 1, 748, 2, 1, 748, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
-  {0, 32, 2, 749, 2, 1, 749, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
-  {0, 32, 4, 753, 1, 1, 753, 1},  // tsm_atb_f64_32x4_t4x4_r32_p4_nt
-  {0, 32, 8, 776, 1, 1, 776, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
-  {0, 32, 16, 777, 1, 0, 778, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
-  {0, 32, 32, 783, 2, 1, 783, 1},  // tsm_atb_f64_32x32_mfma_r32_p1_nt
-  {0, 32, 64, 836, 1, 0, 837, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
-  {0, 64, 1, 838, 2, 1, 838, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
-  {0, 64, 2, 839, 2, 1, 839, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
-  {0, 64, 4, 840, 2, 1, 840, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
-  {0, 64, 8, 843, 2, 1, 843, 1},  // tsm_atb_f64_64x8_mfma_r32_p1_nt
-  {0, 64, 16, 849, 1, 1, 849, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
-  {0, 64, 32, 850, 1, 0, 851, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2_nt)
-  {0, 64, 64, 853, 1, 0, 862, 0},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
+  {1, 4, 16, 281, 1, 1, 281, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
+  {1, 4, 32, 288, 1, 1, 288, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
+  {1, 4, 64, 295, 1, 1, 295, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
+  {1, 8, 1, 296, 2, 1, 296, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
+  {1, 8, 2, 297, 1, 1, 297, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
+  {1, 8, 4, 298, 1, 1, 298, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
+  {1, 8, 8, 334, 1, 1, 334, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
+  {1, 8, 16, 348, 1, 1, 348, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
+  {1, 8, 32, 350, 2, 1, 350, 1},  // tsm_atb_f32_8x32_mfma_r64_p2_nt
+  {1, 8, 64, 362, 2, 1, 362, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
+  {1, 16, 1, 363, 2, 1, 363, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
+  {1, 16, 2, 364, 2, 1, 364, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
+  {1, 16, 4, 369, 1, 0, 369, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
+  {1, 16, 8, 374, 1, 1, 374, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
+  {1, 16, 16, 381, 1, 1, 381, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
+  {1, 16, 32, 428, 1, 0, 429, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
+  {1, 16, 64, 431, 1, 1, 431, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
+  {1, 32, 1, 432, 2, 1, 432, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
+  {1, 32, 2, 433, 2, 1, 433, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
+  {1, 32, 4, 440, 2, 1, 440, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
+  {1, 32, 8, 441, 3, 1, 441, 1},  // tsm_atb_f32_32x8_mfma_r64_p1_nt
+  {1, 32, 16, 448, 1, 0, 449, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
+  {1, 32, 32, 460, 1, 1, 460, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+  {1, 32, 64, 507, 1, 0, 508, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2_nt)
+  {1, 64, 1, 509, 2, 1, 509, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
+  {1, 64, 2, 510, 2, 1, 510, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
+  {1, 64, 4, 517, 2, 1, 517, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
+  {1, 64, 8, 524, 2, 1, 524, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
+  {1, 64, 16, 526, 1, 1, 526, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
+  {1, 64, 32, 527, 1, 0, 528, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
+  {1, 64, 64, 530, 1, 0, 538, 1},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2)
+  {0, 1, 1, 567, 1, 1, 567, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
+  {0, 1, 2, 570, 1, 1, 570, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
+  {0, 1, 4, 571, 1, 1, 571, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
+  {0, 1, 8, 572, 1, 1, 572, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
+  {0, 1, 16, 573, 1, 1, 573, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
+  {0, 1, 32, 574, 1, 1, 574, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
+  {0, 1, 64, 575, 1, 1, 575, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
+  {0, 2, 1, 576, 1, 1, 576, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
+  {0, 2, 2, 588, 1, 1, 588, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
+  {0, 2, 4, 591, 1, 1, 591, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
+  {0, 2, 8, 592, 1, 1, 592, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
+  {0, 2, 16, 599, 1, 1, 599, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
+  {0, 2, 32, 606, 1, 1, 606, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
+  {0, 2, 64, 613, 1, 1, 613, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
+  {0, 4, 1, 614, 1, 1, 614, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
+  {0, 4, 2, 615, 1, 1, 615, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
+  {0, 4, 4, 619, 2, 1, 619, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
+  {0, 4, 8, 632, 1, 1, 632, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
+  {0, 4, 16, 639, 1, 1, 639, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
+  {0, 4, 32, 658, 2, 1, 658, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
+  {0, 4, 64, 668, 2, 1, 668, 1},  // tsm_atb_f64_4x64_mfma_r32_p1_nt
+  {0, 8, 1, 673, 2, 1, 673, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
+  {0, 8, 2, 674, 1, 1, 674, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
+  {0, 8, 4, 675, 1, 1, 675, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
+  {0, 8, 8, 680, 1, 1, 680, 1},  // tsm_atb_f64_8x8_t4x4_r128_p2_nt
+  {0, 8, 16, 725, 1, 1, 725, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
+  {0, 8, 32, 732, 1, 1, 732, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
+  {0, 8, 64, 735, 2, 1, 735, 1},  // tsm_atb_f64_8x64_mfma_r32_p1_nt
+  {0, 16, 1, 740, 2, 1, 740, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
+  {0, 16, 2, 747, 2, 1, 747, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
+  {0, 16, 4, 754, 1, 1, 754, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
+  {0, 16, 8, 761, 1, 1, 761, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
+  {0, 16, 16, 768, 1, 1, 768, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
+  {0, 16, 32, 824, 1, 0, 825, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
+  {0, 16, 64, 827, 1, 1, 827, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
+  {0, 32, 1, 828, 2, 1, 828, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
+  {0, 32, 2, 835, 2, 1, 835, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
+  {0, 32, 4, 845, 1, 1, 845, 1},  // tsm_atb_f64_32x4_t4x4_r32_p4_nt
+  {0, 32, 8, 868, 1, 1, 868, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
+  {0, 32, 16, 869, 1, 0, 870, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
+  {0, 32, 32, 875, 2, 1, 875, 1},  // tsm_atb_f64_32x32_mfma_r32_p1_nt
+  {0, 32, 64, 928, 1, 0, 929, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
+  {0, 64, 1, 930, 2, 1, 930, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
+  {0, 64, 2, 937, 2, 1, 937, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
+  {0, 64, 4, 944, 2, 1, 944, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
+  {0, 64, 8, 947, 2, 1, 947, 1},  // tsm_atb_f64_64x8_mfma_r32_p1_nt
+  {0, 64, 16, 953, 1, 1, 953, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
+  {0, 64, 32, 954, 1, 0, 955, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2_nt)
+  {0, 64, 64, 957, 1, 0, 966, 0},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);
