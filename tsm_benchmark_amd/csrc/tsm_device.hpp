@@ -679,9 +679,9 @@ constexpr int atb_smem_bytes() {
 }
 
 // Width the consumer works on for a side of class width W: the matrix-core
-// consumer (TM = TN = 0) takes 16-wide blocks, so an 8-wide side is staged
-// into LDS rows of 16 whose upper half stays zero.  The register tiles work
-// on the class width itself.
+// consumer (TM = TN = 0) takes 16-wide blocks, so a narrower side is staged
+// into LDS rows of 16 whose remaining columns stay zero.  The register tiles
+// work on the class width itself.
 template <int W, int TM, int TN>
 constexpr int consumer_width() {
   return (TM == 0 && TN == 0 && W < 16) ? 16 : W;
@@ -886,8 +886,10 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
                                             int64_t full, int tail_rows, double* partial_out) {
   static_assert(!SYM || KC == MC, "A^T A is square");
   constexpr int ES = (int)sizeof(S);
-  // Consumer widths: a matrix-core kernel of an 8-wide side stages that
-  // side's rows into LDS rows of KP (MP) = 16 elements, the upper half zero.
+  // Consumer widths: a matrix-core kernel of a side narrower than 16 (rows of
+  // whole 16-byte chunks) stages that side's rows into LDS rows of KP (MP) =
+  // 16 elements, the rest zero.  The padding columns feed only outputs
+  // outside the class's KC x MC block, which are dropped.
   constexpr int KP = consumer_width<KC, TM, TN>(), MP = consumer_width<MC, TM, TN>();
   constexpr bool PAD_A = KP != KC, PAD_B = MP != MC;
   static_assert(!(PAD_A || PAD_B) || (KC * ES % 16 == 0 && MC * ES % 16 == 0),
@@ -906,8 +908,9 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   typename ConsumerOf<S, KP, MP, TM, TN, T, SYM>::type cons;
   cons.zero(tid);
   if constexpr (PAD_A || PAD_B) {
-    // The padding columns are never written below: zero both stages once
-    // (the previous call's reduction may have left values in them).
+    // The padding columns are never written below: zero both stages once,
+    // so that the dropped outputs are sums of zeros rather than of whatever
+    // an earlier use of this LDS left there.
     v4u* z = reinterpret_cast<v4u*>(smem);
     for (int x = tid; x < 2 * STAGE / 16; x += kThreads) z[x] = v4u{0u, 0u, 0u, 0u};
     __syncthreads();
