@@ -207,8 +207,8 @@ def test_every_generated_variant(dt, k, m):
 
 
 # classes with a side narrower than 16 whose rows are whole 16-byte chunks
-# beside a side of 16 or more
-PADDED_CLASSES = [(dt, k, m) for dt, narrow in (("f64", (2, 4, 8)), ("f32", (4, 8)), ("bf16", (8,)))
+# (or, BF16 at 4, 8 bytes) beside a side of 16 or more
+PADDED_CLASSES = [(dt, k, m) for dt, narrow in (("f64", (2, 4, 8)), ("f32", (4, 8)), ("bf16", (4, 8)))
                   for w in narrow for x in (16, 32, 64) for k, m in ((w, x), (x, w))]
 
 

@@ -178,13 +178,14 @@ class KernelConfig:
                 p.append("LDS budget")
             return p
         if self.mfma:
-            # a narrower side whose rows are whole 16-byte chunks is padded to
-            # 16 in LDS: beside a side of 16 or more, or for A^T A at 8 (one
-            # diagonal block from one padded tile)
-            if (any(w % 16 and (w * es) % 16 for w in (self.kc, self.mc))
+            # a narrower side whose rows are whole 16-byte chunks (or 8 bytes,
+            # two rows to a chunk) is padded to 16 in LDS: beside a side of 16
+            # or more, or for A^T A at 8 (one diagonal block from one padded
+            # tile)
+            if (any(w % 16 and (w * es) % 16 and w * es != 8 for w in (self.kc, self.mc))
                     or (max(self.kc, self.mc) < 16 and not (self.sym and self.kc == 8))):
                 p.append("MFMA needs width classes that are multiples of 16 (or a narrower "
-                         "side of whole 16-byte rows beside one)")
+                         "side of whole 16-byte or 8-byte rows beside one)")
             if self.rows % (4 * self.kstep):
                 p.append(f"MFMA rows must be a multiple of {4 * self.kstep}")
             if self.rows // (4 * self.kstep) > MAX_ROWS_PER_THREAD:
@@ -311,14 +312,16 @@ def tuning_variants(dtype: str, kc: int, mc: int) -> list[KernelConfig]:
 
 def padded_mfma_variants(dtype: str) -> list[KernelConfig]:
     """Matrix-core kernels of the classes with one side narrower than 16
-    (rows of whole 16-byte chunks: 8 for BF16, 4 and 8 for FP32, 2, 4 and 8
-    for FP64) and the other of 16 or more, the narrow side padded to 16 in
-    LDS.  Their register-tile kernels need up to 64 FMAs per element read
+    (4 and 8 for BF16, 4 and 8 for FP32, 2, 4 and 8 for FP64) and the other
+    of 16 or more, the narrow side padded to 16 in LDS.  Their register-tile kernels need up to 64 FMAs per element read
     (plus the BF16 unpacking), more than the vector units sustain at the HBM
     rate for FP32 and BF16.  Three tile sizes around 32 KiB of staged rows,
     one or two tiles in flight, streaming loads."""
     out = []
-    narrow = [w for w in (2, 4, 8) if (w * ELEM_BYTES[dtype]) % 16 == 0]
+    # (8-byte rows, two to a 16-byte chunk, are padded too; of those only
+    # BF16 at 4 has register tiles that the vector units limit)
+    narrow = [w for w in (2, 4, 8)
+              if (w * ELEM_BYTES[dtype]) % 16 == 0 or (dtype == "bf16" and w == 4)]
     for kc, mc in ([(n, w) for n in narrow for w in (16, 32, 64)]
                    + [(w, n) for n in narrow for w in (16, 32, 64)]):
         mrows = _pow2_floor(32768 // ((max(kc, 16) + max(mc, 16)) * ELEM_BYTES[dtype]))

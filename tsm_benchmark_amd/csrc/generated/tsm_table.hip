@@ -21,8 +21,24 @@ hipError_t launch_tsm_atb_bf16_4x1_t4x1_r2048_p2_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_bf16_4x2_t4x2_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_4x4_t4x4_r2048_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_4x8_t4x4_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x16_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_4x16_t4x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x32_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_4x32_t4x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x64_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x64_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_4x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_4x64_t4x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x1_t8x1_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_8x2_t8x2_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -90,6 +106,12 @@ hipError_t launch_tsm_atb_bf16_8x64_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigne
 hipError_t launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x1_t8x1_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x2_t8x2_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r1024_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_16x4_mfma_r1024_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x4_t8x4_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x8_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_16x8_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -160,6 +182,12 @@ hipError_t launch_tsm_atb_bf16_16x64_direct_d4_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_bf16_16x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x1_t8x1_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x2_t8x2_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r512_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_32x4_mfma_r512_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x4_t8x4_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x8_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_32x8_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -230,6 +258,10 @@ hipError_t launch_tsm_atb_bf16_32x64_direct_d4_nt(const tsm::AtbArgs&, unsigned,
 hipError_t launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x2_t8x2_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x4_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x4_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x4_mfma_r256_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
+hipError_t launch_tsm_atb_bf16_64x4_mfma_r256_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x4_t8x4_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x8_mfma_r128_p1_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
 hipError_t launch_tsm_atb_bf16_64x8_mfma_r128_p2_nt(const tsm::AtbArgs&, unsigned, hipStream_t);
@@ -1128,8 +1160,24 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_4x2_t4x2_r2048_p2_nt", 2, 4, 2, 4, 2, 2048, 2, 1, 49152, launch_tsm_atb_bf16_4x2_t4x2_r2048_p2_nt, 1},
   {"tsm_atb_bf16_4x4_t4x4_r2048_p2_nt", 2, 4, 4, 4, 4, 2048, 2, 1, 65536, launch_tsm_atb_bf16_4x4_t4x4_r2048_p2_nt, 1},
   {"tsm_atb_bf16_4x8_t4x4_r1024_p2_nt", 2, 4, 8, 4, 4, 1024, 2, 1, 49152, launch_tsm_atb_bf16_4x8_t4x4_r1024_p2_nt, 1},
+  {"tsm_atb_bf16_4x16_mfma_r256_p1_nt", 2, 4, 16, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_4x16_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_4x16_mfma_r256_p2_nt", 2, 4, 16, 0, 0, 256, 2, 1, 32768, launch_tsm_atb_bf16_4x16_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_4x16_mfma_r512_p1_nt", 2, 4, 16, 0, 0, 512, 1, 1, 65536, launch_tsm_atb_bf16_4x16_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_4x16_mfma_r512_p2_nt", 2, 4, 16, 0, 0, 512, 2, 1, 65536, launch_tsm_atb_bf16_4x16_mfma_r512_p2_nt, 2},
+  {"tsm_atb_bf16_4x16_mfma_r1024_p1_nt", 2, 4, 16, 0, 0, 1024, 1, 1, 131072, launch_tsm_atb_bf16_4x16_mfma_r1024_p1_nt, 2},
+  {"tsm_atb_bf16_4x16_mfma_r1024_p2_nt", 2, 4, 16, 0, 0, 1024, 2, 1, 131072, launch_tsm_atb_bf16_4x16_mfma_r1024_p2_nt, 2},
   {"tsm_atb_bf16_4x16_t4x4_r512_p2_nt", 2, 4, 16, 4, 4, 512, 2, 1, 40960, launch_tsm_atb_bf16_4x16_t4x4_r512_p2_nt, 1},
+  {"tsm_atb_bf16_4x32_mfma_r128_p1_nt", 2, 4, 32, 0, 0, 128, 1, 1, 24576, launch_tsm_atb_bf16_4x32_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_4x32_mfma_r128_p2_nt", 2, 4, 32, 0, 0, 128, 2, 1, 24576, launch_tsm_atb_bf16_4x32_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_4x32_mfma_r256_p1_nt", 2, 4, 32, 0, 0, 256, 1, 1, 49152, launch_tsm_atb_bf16_4x32_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_4x32_mfma_r256_p2_nt", 2, 4, 32, 0, 0, 256, 2, 1, 49152, launch_tsm_atb_bf16_4x32_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_4x32_mfma_r512_p1_nt", 2, 4, 32, 0, 0, 512, 1, 1, 98304, launch_tsm_atb_bf16_4x32_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_4x32_mfma_r512_p2_nt", 2, 4, 32, 0, 0, 512, 2, 1, 98304, launch_tsm_atb_bf16_4x32_mfma_r512_p2_nt, 2},
   {"tsm_atb_bf16_4x32_t4x4_r256_p2_nt", 2, 4, 32, 4, 4, 256, 2, 1, 36864, launch_tsm_atb_bf16_4x32_t4x4_r256_p2_nt, 1},
+  {"tsm_atb_bf16_4x64_mfma_r128_p1_nt", 2, 4, 64, 0, 0, 128, 1, 1, 40960, launch_tsm_atb_bf16_4x64_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_4x64_mfma_r128_p2_nt", 2, 4, 64, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_4x64_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_4x64_mfma_r256_p1_nt", 2, 4, 64, 0, 0, 256, 1, 1, 81920, launch_tsm_atb_bf16_4x64_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_4x64_mfma_r256_p2_nt", 2, 4, 64, 0, 0, 256, 2, 1, 81920, launch_tsm_atb_bf16_4x64_mfma_r256_p2_nt, 2},
   {"tsm_atb_bf16_4x64_t4x4_r128_p2_nt", 2, 4, 64, 4, 4, 128, 2, 1, 34816, launch_tsm_atb_bf16_4x64_t4x4_r128_p2_nt, 1},
   {"tsm_atb_bf16_8x1_t8x1_r1024_p2_nt", 2, 8, 1, 8, 1, 1024, 2, 1, 36864, launch_tsm_atb_bf16_8x1_t8x1_r1024_p2_nt, 1},
   {"tsm_atb_bf16_8x2_t8x2_r1024_p2_nt", 2, 8, 2, 8, 2, 1024, 2, 1, 40960, launch_tsm_atb_bf16_8x2_t8x2_r1024_p2_nt, 1},
@@ -1197,6 +1245,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_8x64_t8x4_r128_p2_nt", 2, 8, 64, 8, 4, 128, 2, 1, 36864, launch_tsm_atb_bf16_8x64_t8x4_r128_p2_nt, 1},
   {"tsm_atb_bf16_16x1_t8x1_r512_p2_nt", 2, 16, 1, 8, 1, 512, 2, 1, 34816, launch_tsm_atb_bf16_16x1_t8x1_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x2_t8x2_r512_p2_nt", 2, 16, 2, 8, 2, 512, 2, 1, 36864, launch_tsm_atb_bf16_16x2_t8x2_r512_p2_nt, 1},
+  {"tsm_atb_bf16_16x4_mfma_r256_p1_nt", 2, 16, 4, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_16x4_mfma_r256_p2_nt", 2, 16, 4, 0, 0, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x4_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_16x4_mfma_r512_p1_nt", 2, 16, 4, 0, 0, 512, 1, 1, 65536, launch_tsm_atb_bf16_16x4_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_16x4_mfma_r512_p2_nt", 2, 16, 4, 0, 0, 512, 2, 1, 65536, launch_tsm_atb_bf16_16x4_mfma_r512_p2_nt, 2},
+  {"tsm_atb_bf16_16x4_mfma_r1024_p1_nt", 2, 16, 4, 0, 0, 1024, 1, 1, 131072, launch_tsm_atb_bf16_16x4_mfma_r1024_p1_nt, 2},
+  {"tsm_atb_bf16_16x4_mfma_r1024_p2_nt", 2, 16, 4, 0, 0, 1024, 2, 1, 131072, launch_tsm_atb_bf16_16x4_mfma_r1024_p2_nt, 2},
   {"tsm_atb_bf16_16x4_t8x4_r512_p2_nt", 2, 16, 4, 8, 4, 512, 2, 1, 40960, launch_tsm_atb_bf16_16x4_t8x4_r512_p2_nt, 1},
   {"tsm_atb_bf16_16x8_mfma_r256_p1_nt", 2, 16, 8, 0, 0, 256, 1, 1, 32768, launch_tsm_atb_bf16_16x8_mfma_r256_p1_nt, 2},
   {"tsm_atb_bf16_16x8_mfma_r256_p2_nt", 2, 16, 8, 0, 0, 256, 2, 1, 32768, launch_tsm_atb_bf16_16x8_mfma_r256_p2_nt, 2},
@@ -1267,6 +1321,12 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_16x64_mfma_r128_p2_nt", 2, 16, 64, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_16x64_mfma_r128_p2_nt, 2},
   {"tsm_atb_bf16_32x1_t8x1_r256_p2_nt", 2, 32, 1, 8, 1, 256, 2, 1, 33792, launch_tsm_atb_bf16_32x1_t8x1_r256_p2_nt, 1},
   {"tsm_atb_bf16_32x2_t8x2_r256_p2_nt", 2, 32, 2, 8, 2, 256, 2, 1, 34816, launch_tsm_atb_bf16_32x2_t8x2_r256_p2_nt, 1},
+  {"tsm_atb_bf16_32x4_mfma_r128_p1_nt", 2, 32, 4, 0, 0, 128, 1, 1, 24576, launch_tsm_atb_bf16_32x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_32x4_mfma_r128_p2_nt", 2, 32, 4, 0, 0, 128, 2, 1, 24576, launch_tsm_atb_bf16_32x4_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_32x4_mfma_r256_p1_nt", 2, 32, 4, 0, 0, 256, 1, 1, 49152, launch_tsm_atb_bf16_32x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_32x4_mfma_r256_p2_nt", 2, 32, 4, 0, 0, 256, 2, 1, 49152, launch_tsm_atb_bf16_32x4_mfma_r256_p2_nt, 2},
+  {"tsm_atb_bf16_32x4_mfma_r512_p1_nt", 2, 32, 4, 0, 0, 512, 1, 1, 98304, launch_tsm_atb_bf16_32x4_mfma_r512_p1_nt, 2},
+  {"tsm_atb_bf16_32x4_mfma_r512_p2_nt", 2, 32, 4, 0, 0, 512, 2, 1, 98304, launch_tsm_atb_bf16_32x4_mfma_r512_p2_nt, 2},
   {"tsm_atb_bf16_32x4_t8x4_r256_p2_nt", 2, 32, 4, 8, 4, 256, 2, 1, 36864, launch_tsm_atb_bf16_32x4_t8x4_r256_p2_nt, 1},
   {"tsm_atb_bf16_32x8_mfma_r128_p1_nt", 2, 32, 8, 0, 0, 128, 1, 1, 24576, launch_tsm_atb_bf16_32x8_mfma_r128_p1_nt, 2},
   {"tsm_atb_bf16_32x8_mfma_r128_p2_nt", 2, 32, 8, 0, 0, 128, 2, 1, 24576, launch_tsm_atb_bf16_32x8_mfma_r128_p2_nt, 2},
@@ -1337,6 +1397,10 @@ static const tsm::KernelEntry kTable[] = {
   {"tsm_atb_bf16_32x64_mfma_r128_p2_nt", 2, 32, 64, 0, 0, 128, 2, 1, 49152, launch_tsm_atb_bf16_32x64_mfma_r128_p2_nt, 2},
   {"tsm_atb_bf16_64x1_t8x1_r128_p2_nt", 2, 64, 1, 8, 1, 128, 2, 1, 33280, launch_tsm_atb_bf16_64x1_t8x1_r128_p2_nt, 1},
   {"tsm_atb_bf16_64x2_t8x2_r128_p2_nt", 2, 64, 2, 8, 2, 128, 2, 1, 33792, launch_tsm_atb_bf16_64x2_t8x2_r128_p2_nt, 1},
+  {"tsm_atb_bf16_64x4_mfma_r128_p1_nt", 2, 64, 4, 0, 0, 128, 1, 1, 40960, launch_tsm_atb_bf16_64x4_mfma_r128_p1_nt, 2},
+  {"tsm_atb_bf16_64x4_mfma_r128_p2_nt", 2, 64, 4, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_64x4_mfma_r128_p2_nt, 2},
+  {"tsm_atb_bf16_64x4_mfma_r256_p1_nt", 2, 64, 4, 0, 0, 256, 1, 1, 81920, launch_tsm_atb_bf16_64x4_mfma_r256_p1_nt, 2},
+  {"tsm_atb_bf16_64x4_mfma_r256_p2_nt", 2, 64, 4, 0, 0, 256, 2, 1, 81920, launch_tsm_atb_bf16_64x4_mfma_r256_p2_nt, 2},
   {"tsm_atb_bf16_64x4_t8x4_r128_p2_nt", 2, 64, 4, 8, 4, 128, 2, 1, 34816, launch_tsm_atb_bf16_64x4_t8x4_r128_p2_nt, 1},
   {"tsm_atb_bf16_64x8_mfma_r128_p1_nt", 2, 64, 8, 0, 0, 128, 1, 1, 40960, launch_tsm_atb_bf16_64x8_mfma_r128_p1_nt, 2},
   {"tsm_atb_bf16_64x8_mfma_r128_p2_nt", 2, 64, 8, 0, 0, 128, 2, 1, 40960, launch_tsm_atb_bf16_64x8_mfma_r128_p2_nt, 2},
@@ -2263,135 +2327,135 @@ static const tsm::TunedEntry kTuned[] = {
   {2, 4, 2, 15, 1, 1, 15, 1},  // tsm_atb_bf16_4x2_t4x2_r2048_p2_nt
   {2, 4, 4, 16, 1, 1, 16, 1},  // tsm_atb_bf16_4x4_t4x4_r2048_p2_nt
   {2, 4, 8, 17, 1, 1, 17, 1},  // tsm_atb_bf16_4x8_t4x4_r1024_p2_nt
-  {2, 4, 16, 18, 2, 0, 18, 0},  // tsm_atb_bf16_4x16_t4x4_r512_p2_nt
-  {2, 4, 32, 19, 2, 1, 19, 1},  // tsm_atb_bf16_4x32_t4x4_r256_p2_nt
-  {2, 4, 64, 20, 2, 1, 20, 1},  // tsm_atb_bf16_4x64_t4x4_r128_p2_nt
-  {2, 8, 1, 21, 2, 1, 21, 1},  // tsm_atb_bf16_8x1_t8x1_r1024_p2_nt
-  {2, 8, 2, 22, 1, 1, 22, 1},  // tsm_atb_bf16_8x2_t8x2_r1024_p2_nt
-  {2, 8, 4, 23, 1, 1, 23, 1},  // tsm_atb_bf16_8x4_t8x4_r1024_p2_nt
-  {2, 8, 8, 49, 2, 1, 49, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
-  {2, 8, 16, 67, 1, 1, 67, 1},  // tsm_atb_bf16_8x16_mfma_r256_p2_nt
-  {2, 8, 32, 74, 2, 1, 74, 1},  // tsm_atb_bf16_8x32_mfma_r128_p2_nt
-  {2, 8, 64, 80, 2, 1, 80, 1},  // tsm_atb_bf16_8x64_mfma_r128_p1_nt
-  {2, 16, 1, 85, 2, 1, 85, 1},  // tsm_atb_bf16_16x1_t8x1_r512_p2_nt
-  {2, 16, 2, 86, 2, 1, 86, 1},  // tsm_atb_bf16_16x2_t8x2_r512_p2_nt
-  {2, 16, 4, 87, 1, 0, 87, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2_nt
-  {2, 16, 8, 89, 1, 1, 89, 1},  // tsm_atb_bf16_16x8_mfma_r256_p2_nt
-  {2, 16, 16, 106, 1, 1, 106, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
-  {2, 16, 32, 152, 1, 1, 152, 1},  // tsm_atb_bf16_16x32_mfma_r256_p2_nt
-  {2, 16, 64, 154, 1, 1, 154, 1},  // tsm_atb_bf16_16x64_mfma_r128_p2_nt
-  {2, 32, 1, 155, 2, 1, 155, 1},  // tsm_atb_bf16_32x1_t8x1_r256_p2_nt
-  {2, 32, 2, 156, 2, 1, 156, 1},  // tsm_atb_bf16_32x2_t8x2_r256_p2_nt
-  {2, 32, 4, 157, 2, 1, 157, 1},  // tsm_atb_bf16_32x4_t8x4_r256_p2_nt
-  {2, 32, 8, 161, 1, 1, 161, 1},  // tsm_atb_bf16_32x8_mfma_r256_p2_nt
-  {2, 32, 16, 166, 1, 1, 166, 1},  // tsm_atb_bf16_32x16_mfma_r256_p2_nt
-  {2, 32, 32, 178, 1, 1, 178, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
-  {2, 32, 64, 224, 1, 1, 224, 1},  // tsm_atb_bf16_32x64_mfma_r128_p2_nt
-  {2, 64, 1, 225, 2, 1, 225, 1},  // tsm_atb_bf16_64x1_t8x1_r128_p2_nt
-  {2, 64, 2, 226, 2, 1, 226, 1},  // tsm_atb_bf16_64x2_t8x2_r128_p2_nt
-  {2, 64, 4, 227, 2, 1, 227, 1},  // tsm_atb_bf16_64x4_t8x4_r128_p2_nt
-  {2, 64, 8, 228, 2, 1, 228, 1},  // tsm_atb_bf16_64x8_mfma_r128_p1_nt
-  {2, 64, 16, 234, 1, 1, 234, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
-  {2, 64, 32, 236, 1, 1, 236, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
-  {2, 64, 64, 238, 1, 0, 246, 1},  // tsm_atb_bf16_64x64_direct_d4_nt (else tsm_atb_bf16_64x64_mfma_r256_p2_nt)
-  {1, 1, 1, 259, 1, 1, 259, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
-  {1, 1, 2, 260, 1, 1, 260, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
-  {1, 1, 4, 261, 1, 1, 261, 1},  // tsm_atb_f32_1x4_t1x4_r1024_p2_nt
-  {1, 1, 8, 262, 1, 1, 262, 1},  // tsm_atb_f32_1x8_t1x4_r512_p2_nt
-  {1, 1, 16, 263, 1, 1, 263, 1},  // tsm_atb_f32_1x16_t1x4_r256_p2_nt
-  {1, 1, 32, 264, 1, 1, 264, 1},  // tsm_atb_f32_1x32_t1x4_r128_p2_nt
-  {1, 1, 64, 265, 1, 1, 265, 1},  // tsm_atb_f32_1x64_t1x4_r64_p2_nt
-  {1, 2, 1, 266, 1, 1, 266, 1},  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
-  {1, 2, 2, 267, 1, 1, 267, 1},  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
-  {1, 2, 4, 268, 1, 1, 268, 1},  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
-  {1, 2, 8, 269, 1, 1, 269, 1},  // tsm_atb_f32_2x8_t2x4_r512_p2_nt
-  {1, 2, 16, 270, 1, 1, 270, 1},  // tsm_atb_f32_2x16_t2x4_r256_p2_nt
-  {1, 2, 32, 271, 1, 1, 271, 1},  // tsm_atb_f32_2x32_t2x4_r128_p2_nt
-  {1, 2, 64, 272, 1, 1, 272, 1},  // tsm_atb_f32_2x64_t2x4_r64_p2_nt
-  {1, 4, 1, 273, 1, 1, 273, 1},  // tsm_atb_f32_4x1_t4x1_r1024_p2_nt
-  {1, 4, 2, 274, 1, 1, 274, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
-  {1, 4, 4, 275, 1, 1, 275, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
-  {1, 4, 8, 276, 1, 1, 276, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
-  {1, 4, 16, 281, 1, 1, 281, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
-  {1, 4, 32, 288, 1, 1, 288, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
-  {1, 4, 64, 295, 1, 1, 295, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
-  {1, 8, 1, 296, 2, 1, 296, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
-  {1, 8, 2, 297, 1, 1, 297, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
-  {1, 8, 4, 298, 1, 1, 298, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
-  {1, 8, 8, 334, 1, 1, 334, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
-  {1, 8, 16, 348, 1, 1, 348, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
-  {1, 8, 32, 350, 2, 1, 350, 1},  // tsm_atb_f32_8x32_mfma_r64_p2_nt
-  {1, 8, 64, 362, 2, 1, 362, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
-  {1, 16, 1, 363, 2, 1, 363, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
-  {1, 16, 2, 364, 2, 1, 364, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
-  {1, 16, 4, 369, 1, 0, 369, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
-  {1, 16, 8, 374, 1, 1, 374, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
-  {1, 16, 16, 381, 1, 1, 381, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
-  {1, 16, 32, 428, 1, 0, 429, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
-  {1, 16, 64, 431, 1, 1, 431, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
-  {1, 32, 1, 432, 2, 1, 432, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
-  {1, 32, 2, 433, 2, 1, 433, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
-  {1, 32, 4, 440, 2, 1, 440, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
-  {1, 32, 8, 441, 3, 1, 441, 1},  // tsm_atb_f32_32x8_mfma_r64_p1_nt
-  {1, 32, 16, 448, 1, 0, 449, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
-  {1, 32, 32, 460, 1, 1, 460, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
-  {1, 32, 64, 507, 1, 0, 508, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2_nt)
-  {1, 64, 1, 509, 2, 1, 509, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
-  {1, 64, 2, 510, 2, 1, 510, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
-  {1, 64, 4, 517, 2, 1, 517, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
-  {1, 64, 8, 524, 2, 1, 524, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
-  {1, 64, 16, 526, 1, 1, 526, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
-  {1, 64, 32, 527, 1, 0, 528, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
-  {1, 64, 64, 530, 1, 0, 538, 1},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2)
-  {0, 1, 1, 567, 1, 1, 567, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
-  {0, 1, 2, 570, 1, 1, 570, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
-  {0, 1, 4, 571, 1, 1, 571, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
-  {0, 1, 8, 572, 1, 1, 572, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
-  {0, 1, 16, 573, 1, 1, 573, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
-  {0, 1, 32, 574, 1, 1, 574, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
-  {0, 1, 64, 575, 1, 1, 575, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
-  {0, 2, 1, 576, 1, 1, 576, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
-  {0, 2, 2, 588, 1, 1, 588, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
-  {0, 2, 4, 591, 1, 1, 591, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
-  {0, 2, 8, 592, 1, 1, 592, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
-  {0, 2, 16, 599, 1, 1, 599, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
-  {0, 2, 32, 606, 1, 1, 606, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
-  {0, 2, 64, 613, 1, 1, 613, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
-  {0, 4, 1, 614, 1, 1, 614, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
-  {0, 4, 2, 615, 1, 1, 615, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
-  {0, 4, 4, 619, 2, 1, 619, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
-  {0, 4, 8, 632, 1, 1, 632, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
-  {0, 4, 16, 639, 1, 1, 639, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
-  {0, 4, 32, 658, 2, 1, 658, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
-  {0, 4, 64, 668, 2, 1, 668, 1},  // tsm_atb_f64_4x64_mfma_r32_p1_nt
-  {0, 8, 1, 673, 2, 1, 673, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
-  {0, 8, 2, 674, 1, 1, 674, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
-  {0, 8, 4, 675, 1, 1, 675, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
-  {0, 8, 8, 680, 1, 1, 680, 1},  // tsm_atb_f64_8x8_t4x4_r128_p2_nt
-  {0, 8, 16, 725, 1, 1, 725, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
-  {0, 8, 32, 732, 1, 1, 732, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
-  {0, 8, 64, 735, 2, 1, 735, 1},  // tsm_atb_f64_8x64_mfma_r32_p1_nt
-  {0, 16, 1, 740, 2, 1, 740, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
-  {0, 16, 2, 747, 2, 1, 747, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
-  {0, 16, 4, 754, 1, 1, 754, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
-  {0, 16, 8, 761, 1, 1, 761, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
-  {0, 16, 16, 768, 1, 1, 768, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
-  {0, 16, 32, 824, 1, 0, 825, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
-  {0, 16, 64, 827, 1, 1, 827, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
-  {0, 32, 1, 828, 2, 1, 828, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
-  {0, 32, 2, 835, 2, 1, 835, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
-  {0, 32, 4, 845, 1, 1, 845, 1},  // tsm_atb_f64_32x4_t4x4_r32_p4_nt
-  {0, 32, 8, 868, 1, 1, 868, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
-  {0, 32, 16, 869, 1, 0, 870, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
-  {0, 32, 32, 875, 2, 1, 875, 1},  // tsm_atb_f64_32x32_mfma_r32_p1_nt
-  {0, 32, 64, 928, 1, 0, 929, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
-  {0, 64, 1, 930, 2, 1, 930, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
-  {0, 64, 2, 937, 2, 1, 937, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
-  {0, 64, 4, 944, 2, 1, 944, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
-  {0, 64, 8, 947, 2, 1, 947, 1},  // tsm_atb_f64_64x8_mfma_r32_p1_nt
-  {0, 64, 16, 953, 1, 1, 953, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
-  {0, 64, 32, 954, 1, 0, 955, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2_nt)
-  {0, 64, 64, 957, 1, 0, 966, 0},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
+  {2, 4, 16, 19, 2, 1, 19, 1},  // tsm_atb_bf16_4x16_mfma_r256_p2_nt
+  {2, 4, 32, 28, 1, 1, 28, 1},  // tsm_atb_bf16_4x32_mfma_r256_p2_nt
+  {2, 4, 64, 32, 2, 1, 32, 1},  // tsm_atb_bf16_4x64_mfma_r128_p1_nt
+  {2, 8, 1, 37, 2, 1, 37, 1},  // tsm_atb_bf16_8x1_t8x1_r1024_p2_nt
+  {2, 8, 2, 38, 1, 1, 38, 1},  // tsm_atb_bf16_8x2_t8x2_r1024_p2_nt
+  {2, 8, 4, 39, 1, 1, 39, 1},  // tsm_atb_bf16_8x4_t8x4_r1024_p2_nt
+  {2, 8, 8, 65, 2, 1, 65, 1},  // tsm_atb_bf16_8x8_t8x4_r1024_p1_nt
+  {2, 8, 16, 83, 1, 1, 83, 1},  // tsm_atb_bf16_8x16_mfma_r256_p2_nt
+  {2, 8, 32, 90, 2, 1, 90, 1},  // tsm_atb_bf16_8x32_mfma_r128_p2_nt
+  {2, 8, 64, 96, 2, 1, 96, 1},  // tsm_atb_bf16_8x64_mfma_r128_p1_nt
+  {2, 16, 1, 101, 2, 1, 101, 1},  // tsm_atb_bf16_16x1_t8x1_r512_p2_nt
+  {2, 16, 2, 102, 2, 1, 102, 1},  // tsm_atb_bf16_16x2_t8x2_r512_p2_nt
+  {2, 16, 4, 109, 1, 0, 109, 0},  // tsm_atb_bf16_16x4_t8x4_r512_p2_nt
+  {2, 16, 8, 111, 1, 1, 111, 1},  // tsm_atb_bf16_16x8_mfma_r256_p2_nt
+  {2, 16, 16, 128, 1, 1, 128, 1},  // tsm_atb_bf16_16x16_mfma_r1024_p1_nt
+  {2, 16, 32, 174, 1, 1, 174, 1},  // tsm_atb_bf16_16x32_mfma_r256_p2_nt
+  {2, 16, 64, 176, 1, 1, 176, 1},  // tsm_atb_bf16_16x64_mfma_r128_p2_nt
+  {2, 32, 1, 177, 2, 1, 177, 1},  // tsm_atb_bf16_32x1_t8x1_r256_p2_nt
+  {2, 32, 2, 178, 2, 1, 178, 1},  // tsm_atb_bf16_32x2_t8x2_r256_p2_nt
+  {2, 32, 4, 185, 2, 1, 185, 1},  // tsm_atb_bf16_32x4_t8x4_r256_p2_nt
+  {2, 32, 8, 189, 1, 1, 189, 1},  // tsm_atb_bf16_32x8_mfma_r256_p2_nt
+  {2, 32, 16, 194, 1, 1, 194, 1},  // tsm_atb_bf16_32x16_mfma_r256_p2_nt
+  {2, 32, 32, 206, 1, 1, 206, 1},  // tsm_atb_bf16_32x32_mfma_r512_p1_nt
+  {2, 32, 64, 252, 1, 1, 252, 1},  // tsm_atb_bf16_32x64_mfma_r128_p2_nt
+  {2, 64, 1, 253, 2, 1, 253, 1},  // tsm_atb_bf16_64x1_t8x1_r128_p2_nt
+  {2, 64, 2, 254, 2, 1, 254, 1},  // tsm_atb_bf16_64x2_t8x2_r128_p2_nt
+  {2, 64, 4, 255, 2, 1, 255, 1},  // tsm_atb_bf16_64x4_mfma_r128_p1_nt
+  {2, 64, 8, 260, 2, 1, 260, 1},  // tsm_atb_bf16_64x8_mfma_r128_p1_nt
+  {2, 64, 16, 266, 1, 1, 266, 1},  // tsm_atb_bf16_64x16_mfma_r128_p2_nt
+  {2, 64, 32, 268, 1, 1, 268, 1},  // tsm_atb_bf16_64x32_mfma_r128_p2_nt
+  {2, 64, 64, 270, 1, 0, 278, 1},  // tsm_atb_bf16_64x64_direct_d4_nt (else tsm_atb_bf16_64x64_mfma_r256_p2_nt)
+  {1, 1, 1, 291, 1, 1, 291, 1},  // tsm_atb_f32_1x1_t1x1_r4096_p2_nt
+  {1, 1, 2, 292, 1, 1, 292, 1},  // tsm_atb_f32_1x2_t1x2_r2048_p2_nt
+  {1, 1, 4, 293, 1, 1, 293, 1},  // tsm_atb_f32_1x4_t1x4_r1024_p2_nt
+  {1, 1, 8, 294, 1, 1, 294, 1},  // tsm_atb_f32_1x8_t1x4_r512_p2_nt
+  {1, 1, 16, 295, 1, 1, 295, 1},  // tsm_atb_f32_1x16_t1x4_r256_p2_nt
+  {1, 1, 32, 296, 1, 1, 296, 1},  // tsm_atb_f32_1x32_t1x4_r128_p2_nt
+  {1, 1, 64, 297, 1, 1, 297, 1},  // tsm_atb_f32_1x64_t1x4_r64_p2_nt
+  {1, 2, 1, 298, 1, 1, 298, 1},  // tsm_atb_f32_2x1_t2x1_r2048_p2_nt
+  {1, 2, 2, 299, 1, 1, 299, 1},  // tsm_atb_f32_2x2_t2x2_r2048_p2_nt
+  {1, 2, 4, 300, 1, 1, 300, 1},  // tsm_atb_f32_2x4_t2x4_r1024_p2_nt
+  {1, 2, 8, 301, 1, 1, 301, 1},  // tsm_atb_f32_2x8_t2x4_r512_p2_nt
+  {1, 2, 16, 302, 1, 1, 302, 1},  // tsm_atb_f32_2x16_t2x4_r256_p2_nt
+  {1, 2, 32, 303, 1, 1, 303, 1},  // tsm_atb_f32_2x32_t2x4_r128_p2_nt
+  {1, 2, 64, 304, 1, 1, 304, 1},  // tsm_atb_f32_2x64_t2x4_r64_p2_nt
+  {1, 4, 1, 305, 1, 1, 305, 1},  // tsm_atb_f32_4x1_t4x1_r1024_p2_nt
+  {1, 4, 2, 306, 1, 1, 306, 1},  // tsm_atb_f32_4x2_t4x2_r1024_p2_nt
+  {1, 4, 4, 307, 1, 1, 307, 1},  // tsm_atb_f32_4x4_t4x4_r1024_p2_nt
+  {1, 4, 8, 308, 1, 1, 308, 1},  // tsm_atb_f32_4x8_t4x4_r512_p2_nt
+  {1, 4, 16, 313, 1, 1, 313, 1},  // tsm_atb_f32_4x16_t4x4_r256_p2_nt
+  {1, 4, 32, 320, 1, 1, 320, 1},  // tsm_atb_f32_4x32_t4x4_r128_p2_nt
+  {1, 4, 64, 327, 1, 1, 327, 1},  // tsm_atb_f32_4x64_t4x4_r64_p2_nt
+  {1, 8, 1, 328, 2, 1, 328, 1},  // tsm_atb_f32_8x1_t8x1_r512_p2_nt
+  {1, 8, 2, 329, 1, 1, 329, 1},  // tsm_atb_f32_8x2_t8x2_r512_p2_nt
+  {1, 8, 4, 330, 1, 1, 330, 1},  // tsm_atb_f32_8x4_t8x4_r512_p2_nt
+  {1, 8, 8, 366, 1, 1, 366, 1},  // tsm_atb_f32_8x8_t8x4_r1024_p2_nt
+  {1, 8, 16, 380, 1, 1, 380, 1},  // tsm_atb_f32_8x16_t8x4_r256_p2_nt
+  {1, 8, 32, 382, 2, 1, 382, 1},  // tsm_atb_f32_8x32_mfma_r64_p2_nt
+  {1, 8, 64, 394, 2, 1, 394, 1},  // tsm_atb_f32_8x64_t8x4_r64_p2_nt
+  {1, 16, 1, 395, 2, 1, 395, 1},  // tsm_atb_f32_16x1_t8x1_r256_p2_nt
+  {1, 16, 2, 396, 2, 1, 396, 1},  // tsm_atb_f32_16x2_t8x2_r256_p2_nt
+  {1, 16, 4, 401, 1, 0, 401, 0},  // tsm_atb_f32_16x4_t8x4_r256_p2_nt
+  {1, 16, 8, 406, 1, 1, 406, 1},  // tsm_atb_f32_16x8_t8x4_r256_p2_nt
+  {1, 16, 16, 413, 1, 1, 413, 1},  // tsm_atb_f32_16x16_mfma_r128_p2_nt
+  {1, 16, 32, 460, 1, 0, 461, 0},  // tsm_atb_f32_16x32_direct_d8_nt (else tsm_atb_f32_16x32_t8x4_r128_p2_nt)
+  {1, 16, 64, 463, 1, 1, 463, 1},  // tsm_atb_f32_16x64_mfma_r64_p2_nt
+  {1, 32, 1, 464, 2, 1, 464, 1},  // tsm_atb_f32_32x1_t8x1_r128_p2_nt
+  {1, 32, 2, 465, 2, 1, 465, 1},  // tsm_atb_f32_32x2_t8x2_r128_p2_nt
+  {1, 32, 4, 472, 2, 1, 472, 1},  // tsm_atb_f32_32x4_t8x4_r128_p2_nt
+  {1, 32, 8, 473, 3, 1, 473, 1},  // tsm_atb_f32_32x8_mfma_r64_p1_nt
+  {1, 32, 16, 480, 1, 0, 481, 0},  // tsm_atb_f32_32x16_direct_d8_nt (else tsm_atb_f32_32x16_t8x4_r128_p2_nt)
+  {1, 32, 32, 492, 1, 1, 492, 1},  // tsm_atb_f32_32x32_mfma_r128_p2_nt
+  {1, 32, 64, 539, 1, 0, 540, 0},  // tsm_atb_f32_32x64_direct_d8_nt (else tsm_atb_f32_32x64_mfma_r64_p2_nt)
+  {1, 64, 1, 541, 2, 1, 541, 1},  // tsm_atb_f32_64x1_t8x1_r64_p2_nt
+  {1, 64, 2, 542, 2, 1, 542, 1},  // tsm_atb_f32_64x2_t8x2_r64_p2_nt
+  {1, 64, 4, 549, 2, 1, 549, 1},  // tsm_atb_f32_64x4_t8x4_r64_p2_nt
+  {1, 64, 8, 556, 2, 1, 556, 1},  // tsm_atb_f32_64x8_t8x4_r64_p2_nt
+  {1, 64, 16, 558, 1, 1, 558, 1},  // tsm_atb_f32_64x16_mfma_r64_p2_nt
+  {1, 64, 32, 559, 1, 0, 560, 1},  // tsm_atb_f32_64x32_direct_d8_nt (else tsm_atb_f32_64x32_mfma_r64_p2_nt)
+  {1, 64, 64, 562, 1, 0, 570, 1},  // tsm_atb_f32_64x64_direct_d8_nt (else tsm_atb_f32_64x64_mfma_r64_p2)
+  {0, 1, 1, 599, 1, 1, 599, 1},  // tsm_atb_f64_1x1_t1x1_r4096_p1_nt
+  {0, 1, 2, 602, 1, 1, 602, 1},  // tsm_atb_f64_1x2_t1x2_r1024_p2_nt
+  {0, 1, 4, 603, 1, 1, 603, 1},  // tsm_atb_f64_1x4_t1x4_r512_p2_nt
+  {0, 1, 8, 604, 1, 1, 604, 1},  // tsm_atb_f64_1x8_t1x8_r256_p2_nt
+  {0, 1, 16, 605, 1, 1, 605, 1},  // tsm_atb_f64_1x16_t1x8_r128_p2_nt
+  {0, 1, 32, 606, 1, 1, 606, 1},  // tsm_atb_f64_1x32_t1x8_r64_p2_nt
+  {0, 1, 64, 607, 1, 1, 607, 1},  // tsm_atb_f64_1x64_t1x8_r32_p2_nt
+  {0, 2, 1, 608, 1, 1, 608, 1},  // tsm_atb_f64_2x1_t2x1_r1024_p2_nt
+  {0, 2, 2, 620, 1, 1, 620, 1},  // tsm_atb_f64_2x2_t2x2_r2048_p1_nt
+  {0, 2, 4, 623, 1, 1, 623, 1},  // tsm_atb_f64_2x4_t2x4_r512_p2_nt
+  {0, 2, 8, 624, 1, 1, 624, 1},  // tsm_atb_f64_2x8_t2x8_r256_p2_nt
+  {0, 2, 16, 631, 1, 1, 631, 1},  // tsm_atb_f64_2x16_t2x8_r128_p2_nt
+  {0, 2, 32, 638, 1, 1, 638, 1},  // tsm_atb_f64_2x32_t2x8_r64_p2_nt
+  {0, 2, 64, 645, 1, 1, 645, 1},  // tsm_atb_f64_2x64_t2x8_r32_p2_nt
+  {0, 4, 1, 646, 1, 1, 646, 1},  // tsm_atb_f64_4x1_t4x1_r512_p2_nt
+  {0, 4, 2, 647, 1, 1, 647, 1},  // tsm_atb_f64_4x2_t4x2_r512_p2_nt
+  {0, 4, 4, 651, 2, 1, 651, 1},  // tsm_atb_f64_4x4_t4x4_r256_p1_nt
+  {0, 4, 8, 664, 1, 1, 664, 1},  // tsm_atb_f64_4x8_t4x8_r256_p2_nt
+  {0, 4, 16, 671, 1, 1, 671, 1},  // tsm_atb_f64_4x16_t4x8_r128_p2_nt
+  {0, 4, 32, 690, 2, 1, 690, 1},  // tsm_atb_f64_4x32_t4x8_r64_p1_nt
+  {0, 4, 64, 700, 2, 1, 700, 1},  // tsm_atb_f64_4x64_mfma_r32_p1_nt
+  {0, 8, 1, 705, 2, 1, 705, 1},  // tsm_atb_f64_8x1_t8x1_r256_p2_nt
+  {0, 8, 2, 706, 1, 1, 706, 1},  // tsm_atb_f64_8x2_t8x2_r256_p2_nt
+  {0, 8, 4, 707, 1, 1, 707, 1},  // tsm_atb_f64_8x4_t8x4_r256_p2_nt
+  {0, 8, 8, 712, 1, 1, 712, 1},  // tsm_atb_f64_8x8_t4x4_r128_p2_nt
+  {0, 8, 16, 757, 1, 1, 757, 1},  // tsm_atb_f64_8x16_t8x8_r128_p2_nt
+  {0, 8, 32, 764, 1, 1, 764, 1},  // tsm_atb_f64_8x32_t8x8_r64_p2_nt
+  {0, 8, 64, 767, 2, 1, 767, 1},  // tsm_atb_f64_8x64_mfma_r32_p1_nt
+  {0, 16, 1, 772, 2, 1, 772, 1},  // tsm_atb_f64_16x1_t8x1_r128_p2_nt
+  {0, 16, 2, 779, 2, 1, 779, 1},  // tsm_atb_f64_16x2_t8x2_r128_p2_nt
+  {0, 16, 4, 786, 1, 1, 786, 1},  // tsm_atb_f64_16x4_t8x4_r128_p2_nt
+  {0, 16, 8, 793, 1, 1, 793, 1},  // tsm_atb_f64_16x8_t8x8_r128_p2_nt
+  {0, 16, 16, 800, 1, 1, 800, 1},  // tsm_atb_f64_16x16_mfma_r64_p2_nt
+  {0, 16, 32, 856, 1, 0, 857, 0},  // tsm_atb_f64_16x32_direct_d8_nt (else tsm_atb_f64_16x32_t8x8_r64_p2_nt)
+  {0, 16, 64, 859, 1, 1, 859, 1},  // tsm_atb_f64_16x64_mfma_r32_p2_nt
+  {0, 32, 1, 860, 2, 1, 860, 1},  // tsm_atb_f64_32x1_t8x1_r64_p2_nt
+  {0, 32, 2, 867, 2, 1, 867, 1},  // tsm_atb_f64_32x2_t8x2_r64_p2_nt
+  {0, 32, 4, 877, 1, 1, 877, 1},  // tsm_atb_f64_32x4_t4x4_r32_p4_nt
+  {0, 32, 8, 900, 1, 1, 900, 1},  // tsm_atb_f64_32x8_t8x8_r64_p2_nt
+  {0, 32, 16, 901, 1, 0, 902, 0},  // tsm_atb_f64_32x16_direct_d8_nt (else tsm_atb_f64_32x16_t8x8_r64_p2_nt)
+  {0, 32, 32, 907, 2, 1, 907, 1},  // tsm_atb_f64_32x32_mfma_r32_p1_nt
+  {0, 32, 64, 960, 1, 0, 961, 0},  // tsm_atb_f64_32x64_direct_d8_nt (else tsm_atb_f64_32x64_mfma_r32_p2_nt)
+  {0, 64, 1, 962, 2, 1, 962, 1},  // tsm_atb_f64_64x1_t8x1_r32_p2_nt
+  {0, 64, 2, 969, 2, 1, 969, 1},  // tsm_atb_f64_64x2_t8x2_r32_p2_nt
+  {0, 64, 4, 976, 2, 1, 976, 1},  // tsm_atb_f64_64x4_t8x4_r32_p2_nt
+  {0, 64, 8, 979, 2, 1, 979, 1},  // tsm_atb_f64_64x8_mfma_r32_p1_nt
+  {0, 64, 16, 985, 1, 1, 985, 1},  // tsm_atb_f64_64x16_mfma_r32_p2_nt
+  {0, 64, 32, 986, 1, 0, 987, 0},  // tsm_atb_f64_64x32_direct_d8_nt (else tsm_atb_f64_64x32_mfma_r32_p2_nt)
+  {0, 64, 64, 989, 1, 0, 998, 0},  // tsm_atb_f64_64x64_direct_d8_nt (else tsm_atb_f64_64x64_mfma_r32_p2_nt)
 };
 
 hipError_t launch_tsm_atb_batched_f64(const tsm::BatchArgs&, unsigned, hipStream_t);

@@ -887,13 +887,14 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
   static_assert(!SYM || KC == MC, "A^T A is square");
   constexpr int ES = (int)sizeof(S);
   // Consumer widths: a matrix-core kernel of a side narrower than 16 (rows of
-  // whole 16-byte chunks) stages that side's rows into LDS rows of KP (MP) =
-  // 16 elements, the rest zero.  The padding columns feed only outputs
-  // outside the class's KC x MC block, which are dropped.
+  // whole 16-byte chunks, or of 8 bytes) stages that side's rows into LDS
+  // rows of KP (MP) = 16 elements, the rest zero.  The padding columns feed
+  // only outputs outside the class's KC x MC block, which are dropped.
   constexpr int KP = consumer_width<KC, TM, TN>(), MP = consumer_width<MC, TM, TN>();
   constexpr bool PAD_A = KP != KC, PAD_B = MP != MC;
-  static_assert(!(PAD_A || PAD_B) || (KC * ES % 16 == 0 && MC * ES % 16 == 0),
-                "padded rows are whole 16-byte chunks");
+  static_assert(!(PAD_A || PAD_B) || ((KC * ES % 16 == 0 || KC * ES == 8) &&
+                                      (MC * ES % 16 == 0 || MC * ES == 8)),
+                "padded rows are whole 16-byte chunks or 8 bytes");
   constexpr int A_BYTES = T * KC * ES, B_BYTES = SYM ? 0 : T * MC * ES;  // a tile in memory
   constexpr int LA_BYTES = T * KP * ES, LB_BYTES = SYM ? 0 : T * MP * ES;  // ... in LDS
   static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "tiles are whole 16-byte chunks");
@@ -915,14 +916,22 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
     for (int x = tid; x < 2 * STAGE / 16; x += kThreads) z[x] = v4u{0u, 0u, 0u, 0u};
     __syncthreads();
   }
-  // 16-byte chunk c of a tile row of W elements -> its chunk index in an LDS
-  // row of WP elements
-  auto lds_chunk = [](int c, auto w, auto wp) __attribute__((always_inline)) {
-    constexpr int CW = decltype(w)::value * ES / 16, CP = decltype(wp)::value * ES / 16;
-    if constexpr (CW == CP)
-      return c;
-    else
-      return (c / CW) * CP + c % CW;
+  // Stores 16-byte chunk c of a tile of rows of W elements into an LDS tile
+  // of rows of WP elements.  Rows of 8 bytes come two to a chunk: its halves
+  // go to consecutive LDS rows.
+  auto put = [](unsigned char* base, int c, const v4u& u, auto w, auto wp)
+      __attribute__((always_inline)) {
+    constexpr int RW = decltype(w)::value * ES, RP = decltype(wp)::value * ES;  // row bytes
+    if constexpr (RW == RP) {
+      reinterpret_cast<v4u*>(base)[c] = u;
+    } else if constexpr (RW == 8) {
+      typedef uint32_t half_chunk __attribute__((ext_vector_type(2)));
+      half_chunk* p = reinterpret_cast<half_chunk*>(base);
+      p[(2 * c) * (RP / 8)] = half_chunk{u.x, u.y};
+      p[(2 * c + 1) * (RP / 8)] = half_chunk{u.z, u.w};
+    } else {
+      reinterpret_cast<v4u*>(base)[(c / (RW / 16)) * (RP / 16) + c % (RW / 16)] = u;
+    }
   };
   using KW = std::integral_constant<int, KC>;
   using KWP = std::integral_constant<int, KP>;
@@ -955,17 +964,16 @@ __device__ __forceinline__ void atb_partial(const AtbArgs& a, unsigned char* sme
       }
     };
     auto store = [&](const Regs& r, unsigned char* stage) __attribute__((always_inline)) {
-      v4u* la = reinterpret_cast<v4u*>(stage);
-      v4u* lb = reinterpret_cast<v4u*>(stage + LA_BYTES);
 #pragma unroll
       for (int v = 0; v < A_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (A_VEC % kThreads == 0 || idx < A_VEC) la[lds_chunk(idx, KW{}, KWP{})] = r.a[v];
+        if (A_VEC % kThreads == 0 || idx < A_VEC) put(stage, idx, r.a[v], KW{}, KWP{});
       }
 #pragma unroll
       for (int v = 0; v < B_PER; ++v) {
         const int idx = v * kThreads + tid;
-        if (B_VEC % kThreads == 0 || idx < B_VEC) lb[lds_chunk(idx, MW{}, MWP{})] = r.b[v];
+        if (B_VEC % kThreads == 0 || idx < B_VEC)
+          put(stage + LA_BYTES, idx, r.b[v], MW{}, MWP{});
       }
     };
     // One pipeline step on tile tt held in r: publish it to LDS, refill r with
