@@ -92,9 +92,9 @@ typedef struct {
                           in flight per wave, of 4 rows (FP64/FP32) or 32 rows (BF16);
                           inputs that are not dense run the class's staged kernel).
                           3 needs width classes that are multiples of 16; 2 also takes
-                          one narrower side whose rows are whole 16-byte chunks (BF16 8,
-                          FP32 4 and 8, FP64 2, 4 and 8) beside a side of 16 or more,
-                          padded to 16 in LDS, and A^T A at 8. */
+                          one narrower side (BF16 4 and 8, FP32 4 and 8, FP64 2, 4 and
+                          8) beside a side of 16 or more, padded to 16 in LDS, and
+                          A^T A at 8. */
 } tsm_kernel_config;
 
 /* ---- the hot path ------------------------------------------------------
